@@ -1,0 +1,207 @@
+/*
+ * legendre_gpu — MI355X (gfx950) execution backend for legendre's per-block hot path.
+ *
+ * C ABI: plain pointers, sizes and status codes; no C++ or HIP types cross it.
+ * Every entry point replaces one reference interface (cited file:line, relative to
+ * the legendre crate root) and keeps its argument meaning and error behaviour: the
+ * reference's invariant panics come back as LGD_ERR_* codes with a message from
+ * lgd_ctx_last_error().
+ *
+ * Storage contract (src/core/state.rs:219-264, src/geometry/cartesian.rs:9-11):
+ * a *field buffer* (lgd_buf) holds the ghost-inclusive slab of one field for every
+ * block, block-major (block b at element offset b * lgd_grid_block_len(grid, ghost)),
+ * each slab dimension-0 fastest.  Buffers are device-resident and zero-initialised;
+ * the host sees them only through lgd_buf_upload / lgd_buf_download.
+ *
+ * Threading: one context is driven by one host thread.  Calls are stream-ordered
+ * and asynchronous until lgd_ctx_sync() or a download.
+ */
+#ifndef LEGENDRE_GPU_H
+#define LEGENDRE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGD_ABI_VERSION 1u
+
+typedef enum {
+  LGD_OK = 0,
+  LGD_ERR_INVALID_ARGUMENT = 1,
+  /* ghost width exceeds a block extent (src/geometry/cartesian.rs:456-462) */
+  LGD_ERR_GHOST_TOO_WIDE = 2,
+  /* buffers not congruent with the grid / each other (src/core/state.rs:550, 560) */
+  LGD_ERR_SIZE_MISMATCH = 3,
+  /* stochastic driver declared but no field driven (src/core/simulation.rs:149-160) */
+  LGD_ERR_UNDRIVEN_DRIVER = 4,
+  /* configuration the backend does not implement (message names it) */
+  LGD_ERR_UNSUPPORTED = 5,
+  LGD_ERR_OUT_OF_MEMORY = 6,
+  LGD_ERR_HIP = 7
+} lgd_status;
+
+typedef struct lgd_ctx lgd_ctx;
+typedef struct lgd_buf lgd_buf;
+typedef struct lgd_grid lgd_grid;
+typedef struct lgd_sim lgd_sim;
+
+/* CartesianGrid<D>::new(cells, block_cells, origin, spacing).with_periodic(periodic)
+ * (src/geometry/cartesian.rs:48-97); 1 <= dim <= 3, entries >= dim ignored. */
+typedef struct {
+  int32_t dim;
+  int32_t periodic[3];
+  int64_t cells[3];
+  int64_t block_cells[3];
+  double origin[3];
+  double spacing[3];
+} lgd_cartesian_desc;
+
+/* FaceBc (src/geometry/boundary.rs:36-54). */
+typedef enum {
+  LGD_BC_MIRROR = 0,
+  LGD_BC_DIRICHLET = 1,
+  LGD_BC_FLUX = 2,
+  LGD_BC_EXTRAPOLATE = 3
+} lgd_bc_kind;
+
+typedef struct {
+  int32_t kind; /* lgd_bc_kind */
+  double value; /* Dirichlet value / outward normal flux; unused otherwise */
+} lgd_face_bc;
+
+/* ModelC public fields (src/physics/phasefield.rs:55-67) plus the finite-volume
+ * policy's regularisation threshold FiniteVolume::tol (finite_volume.rs:41-44). */
+typedef struct {
+  double eps4;
+  double lambda;
+  double d_thermal;
+  double noise_amplitude;
+  double anisotropy_tol;
+  double fv_tol;
+} lgd_modelc_params;
+
+uint32_t lgd_abi_version(void);
+
+/* ---------------------------------------------------------------- context */
+/* stream: an existing hipStream_t to launch on, or NULL for a library-owned one. */
+int lgd_ctx_create(int32_t device, void *stream, lgd_ctx **out);
+int lgd_ctx_destroy(lgd_ctx *ctx);
+int lgd_ctx_sync(lgd_ctx *ctx);
+void *lgd_ctx_stream(lgd_ctx *ctx);
+const char *lgd_ctx_last_error(const lgd_ctx *ctx);
+
+/* ------------------------------------- Allocator / StorageBackend (storage.rs) */
+/* Allocator::allocate(len): zero-initialised storage of len f64
+ * (src/core/storage.rs:148-154, DenseStorage 158-189). */
+int lgd_buf_alloc(lgd_ctx *ctx, size_t len, lgd_buf **out);
+int lgd_buf_free(lgd_buf *buf);
+size_t lgd_buf_len(const lgd_buf *buf);
+/* Device address of element 0 (for callers that launch their own work). */
+void *lgd_buf_device_ptr(lgd_buf *buf);
+/* StorageBackend::as_slice / as_mut_slice served by explicit copies
+ * (src/core/storage.rs:126-138). */
+int lgd_buf_upload(lgd_ctx *ctx, lgd_buf *buf, size_t offset, const double *host, size_t n);
+int lgd_buf_download(lgd_ctx *ctx, const lgd_buf *buf, size_t offset, double *host, size_t n);
+/* State::fill_zero / copy_from over one field buffer (src/core/state.rs:571-597). */
+int lgd_buf_fill_zero(lgd_ctx *ctx, lgd_buf *buf);
+int lgd_buf_copy(lgd_ctx *ctx, lgd_buf *dst, const lgd_buf *src);
+
+/* ------------------------------------------------ Grid → device block table */
+int lgd_grid_create_cartesian(lgd_ctx *ctx, const lgd_cartesian_desc *desc, lgd_grid **out);
+int lgd_grid_destroy(lgd_grid *grid);
+/* Grid::num_blocks / block_len (src/geometry/cartesian.rs:568-574). */
+int64_t lgd_grid_num_blocks(const lgd_grid *grid);
+int64_t lgd_grid_block_len(const lgd_grid *grid, uint32_t ghost);
+/* CartesianGrid::face_neighbor (cartesian.rs:162-181); returns -1 for None. */
+int64_t lgd_grid_face_neighbor(const lgd_grid *grid, int64_t block, int32_t dim, int32_t dir);
+/* Grid::cell_key (cartesian.rs:599-619); returns -1 for a ghost entry. */
+int64_t lgd_grid_cell_key(const lgd_grid *grid, uint32_t ghost, int64_t offset);
+
+/* ------------------------------------------------------ per-intent kernels */
+/* fill_ghosts_bc (cartesian.rs:377-387, sweep 439-526): halo exchange, periodic
+ * wrap, then the per-face rule at physical faces.  bcs holds 2*dim entries
+ * indexed [d*2 + (side > 0)]; NULL means Mirror on every face
+ * (fill_ghosts_mirror, cartesian.rs:392-401). */
+int lgd_fill_ghosts(lgd_ctx *ctx, const lgd_grid *grid, lgd_buf *field, uint32_t ghost,
+                    const lgd_face_bc *bcs);
+/* CentralLaplacian::apply (src/discretization/finite_difference.rs:49-83):
+ * interior of out = ∇²in; both buffers ghost width `ghost`. */
+int lgd_central_laplacian(lgd_ctx *ctx, const lgd_grid *grid, const lgd_buf *in, lgd_buf *out,
+                          uint32_t ghost);
+/* KarmaRappelFlux::apply (src/discretization/finite_volume.rs:248-301), ghost 1, 2-D. */
+int lgd_karma_rappel_flux(lgd_ctx *ctx, const lgd_grid *grid, const lgd_buf *phi, lgd_buf *out,
+                          double eps4, double tol);
+/* ModelC::vector_field_block for Driver::Time (src/physics/phasefield.rs:367-421):
+ * kphi, ku interiors from ghost-filled phi, u. */
+int lgd_modelc_rhs(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                   const lgd_buf *phi, const lgd_buf *u, lgd_buf *kphi, lgd_buf *ku);
+/* eval_tendency for Driver::Wiener(0) (src/integrators/mod.rs:242-279 with
+ * phasefield.rs:361-366): zero the slab, interior = noise_amplitude. */
+int lgd_modelc_amplitude(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                         lgd_buf *amp_phi);
+/* Heat3 / Heat<D>::vector_field_block (examples/heat3d.rs:63-79): k = κ·∇²u, ghost 1. */
+int lgd_heat_rhs(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_buf *u, lgd_buf *k);
+/* block_axpy over every slab entry: x += alpha*y (src/core/state.rs:485-498, 549-568);
+ * also Driver::Time's kernel x += dt*k (src/core/driver.rs:131-138). */
+int lgd_axpy(lgd_ctx *ctx, lgd_buf *x, const lgd_buf *y, double alpha);
+/* Driver::Wiener arm of apply_slab (src/core/driver.rs:139-158):
+ * x += (√dt·amp)·N(splitmix64(mix_key(seed,[salt,stream,block]) ^ cell)) on interior
+ * entries with amp != 0.  stream = Driver::stream() (driver.rs:55-61). */
+int lgd_apply_wiener(lgd_ctx *ctx, const lgd_grid *grid, lgd_buf *x, const lgd_buf *amp,
+                     uint32_t ghost, double dt, uint64_t seed, uint64_t salt, uint64_t stream);
+
+/* ------------------------------------------------------ fused hot path */
+/* One Euler–Maruyama step of Model C (src/integrators/euler_maruyama.rs:34-75 on
+ * ModelC<NoNoise | Wiener<1>>, mirror ghosts, src/physics/phasefield.rs:431-439) in one
+ * kernel: halos are read from neighbour interiors (no ghost pass), the RHS and the
+ * stage update are fused, and the new interiors are written to phi_next/u_next
+ * (ghost entries of *_next are left stale; fill_ghosts restores them if needed).
+ * salt = t.to_bits() of the step's start time; wiener = 0 for NoNoise.
+ * Interiors equal the reference step's bit for bit (noise: up to the device log/cos). */
+int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                    const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
+                    double dt, uint64_t seed, uint64_t salt, int32_t wiener);
+/* One forward-Euler step of Heat<D> (src/integrators/euler.rs:104-130,
+ * examples/heat3d.rs:54-79) with mirror/periodic ghosts, fused the same way. */
+int lgd_heat_step(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_buf *u,
+                  lgd_buf *u_next, double dt);
+
+/* ---------------------------------------- Simulation (src/core/simulation.rs) */
+/* mode: how a step is executed. */
+typedef enum {
+  /* the reference's call sequence, one kernel per intent: fill_ghosts → RHS →
+   * amplitude → apply_step (src/integrators/euler_maruyama.rs:57-74) */
+  LGD_MODE_PER_INTENT = 0,
+  /* one fused kernel per step over ping-pong state buffers */
+  LGD_MODE_FUSED = 1
+} lgd_mode;
+
+/* Simulation::new(grid, FiniteVolume{fv_tol}, ModelC, EulerMaruyama{seed}, GpuScheduler,
+ * DeviceAllocator) (simulation.rs:108-183): registers phi (field 0, Time + Wiener(0)) and
+ * u (field 1, Time), ghost 1, allocates state and stage buffers.  wiener = 0 selects
+ * ModelC<NoNoise>, 1 ModelC<Wiener<1>>.  Requires a 2-D grid. */
+int lgd_sim_create_modelc(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                          int32_t wiener, uint64_t seed, int32_t mode, lgd_sim **out);
+/* Simulation::new(grid, FiniteDifference, Heat<D>{kappa}, ForwardEuler, ...): field 0 = u,
+ * ghost 1; bcs as in lgd_fill_ghosts (fused mode needs Mirror/periodic). */
+int lgd_sim_create_heat(lgd_ctx *ctx, const lgd_grid *grid, double kappa,
+                        const lgd_face_bc *bcs, int32_t mode, lgd_sim **out);
+int lgd_sim_destroy(lgd_sim *sim);
+/* The current state buffer of field `index` (valid until the next lgd_sim_step). */
+int lgd_sim_field(lgd_sim *sim, int32_t index, lgd_buf **out);
+/* nsteps × Simulation::step(dt) (simulation.rs:255-290): t advances by t += dt on
+ * the host in f64, exactly as the reference accumulates it (the noise salt). */
+int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps);
+double lgd_sim_time(const lgd_sim *sim);
+int64_t lgd_sim_step_index(const lgd_sim *sim);
+/* ModelC::initialize_grains with orientation 0 (phasefield.rs:227-286), computed on the
+ * host (glibc tanh/hypot, as the reference) and uploaded: grains = [cx, cy, r] × n. */
+int lgd_sim_initialize_grains(lgd_sim *sim, const double *grains, size_t ngrains, double u0);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LEGENDRE_GPU_H */

@@ -1,0 +1,403 @@
+"""legendre_amd — Python handle on the MI355X backend's C ABI (include/legendre_gpu.h).
+
+Thin plumbing over ctypes: every call goes to liblegendre_gpu.so, whose kernels
+run on the GPU.  Names follow the reference crate (CartesianGrid, FaceBc,
+ModelC, Simulation, ...) so that tests read like the reference's own tests.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import struct
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import (
+    LGD_BC_DIRICHLET,
+    LGD_BC_EXTRAPOLATE,
+    LGD_BC_FLUX,
+    LGD_BC_MIRROR,
+    LGD_MODE_FUSED,
+    LGD_MODE_PER_INTENT,
+    LegendreError,
+)
+
+__all__ = [
+    "A2",
+    "CartesianGrid",
+    "Context",
+    "DeviceBuffer",
+    "FaceBc",
+    "LegendreError",
+    "ModelCParams",
+    "Simulation",
+    "time_bits",
+    "LGD_MODE_FUSED",
+    "LGD_MODE_PER_INTENT",
+]
+
+# Karma–Rappel thin-interface constant a₂ (src/physics/phasefield.rs:44).
+A2 = 0.6267
+
+
+def time_bits(t: float) -> int:
+    """f64::to_bits — the noise salt of a step starting at time t."""
+    return struct.unpack("<Q", struct.pack("<d", t))[0]
+
+
+class Context:
+    """A device + stream (lgd_ctx).  `stream` may be an existing hipStream_t
+    handle (an int, e.g. torch.cuda.Stream().cuda_stream)."""
+
+    def __init__(self, device: int = 0, stream: Optional[int] = None):
+        self._lib = _abi.lib()
+        h = C.c_void_p()
+        st = self._lib.lgd_ctx_create(device, C.c_void_p(stream) if stream else None, C.byref(h))
+        if st != 0:
+            raise LegendreError(st, f"could not create a context on device {device}")
+        self.handle = h
+
+    def sync(self) -> None:
+        _abi.check(self._lib.lgd_ctx_sync(self.handle), self.handle)
+
+    @property
+    def stream(self) -> int:
+        return self._lib.lgd_ctx_stream(self.handle) or 0
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.lgd_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class DeviceBuffer:
+    """Device storage from the DeviceAllocator (zero-initialised f64)."""
+
+    def __init__(self, ctx: Context, length: int):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_buf_alloc(ctx.handle, int(length), C.byref(h)), ctx.handle)
+        self.handle = h
+        self.length = int(length)
+        self.owned = True
+
+    @classmethod
+    def _borrow(cls, ctx: Context, handle: C.c_void_p) -> "DeviceBuffer":
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.handle = handle
+        self.length = int(ctx._lib.lgd_buf_len(handle))
+        self.owned = False
+        return self
+
+    def __len__(self) -> int:
+        return self.length
+
+    def upload(self, host: np.ndarray, offset: int = 0) -> None:
+        a = np.ascontiguousarray(host, dtype=np.float64).ravel()
+        _abi.check(
+            self._lib.lgd_buf_upload(
+                self.ctx.handle, self.handle, offset, a.ctypes.data_as(_abi.DP), a.size
+            ),
+            self.ctx.handle,
+        )
+
+    def download(self, offset: int = 0, n: Optional[int] = None) -> np.ndarray:
+        n = self.length - offset if n is None else n
+        out = np.empty(n, dtype=np.float64)
+        _abi.check(
+            self._lib.lgd_buf_download(
+                self.ctx.handle, self.handle, offset, out.ctypes.data_as(_abi.DP), n
+            ),
+            self.ctx.handle,
+        )
+        return out
+
+    def fill_zero(self) -> None:
+        _abi.check(self._lib.lgd_buf_fill_zero(self.ctx.handle, self.handle), self.ctx.handle)
+
+    def copy_from(self, other: "DeviceBuffer") -> None:
+        _abi.check(self._lib.lgd_buf_copy(self.ctx.handle, self.handle, other.handle), self.ctx.handle)
+
+    def free(self) -> None:
+        if self.owned and self.handle:
+            self._lib.lgd_buf_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+@dataclasses.dataclass(frozen=True)
+class FaceBc:
+    """Per-face physical boundary condition (src/geometry/boundary.rs:36-54)."""
+
+    kind: int = LGD_BC_MIRROR
+    value: float = 0.0
+
+    @staticmethod
+    def mirror() -> "FaceBc":
+        return FaceBc(LGD_BC_MIRROR, 0.0)
+
+    @staticmethod
+    def dirichlet(v: float) -> "FaceBc":
+        return FaceBc(LGD_BC_DIRICHLET, v)
+
+    @staticmethod
+    def flux(g: float) -> "FaceBc":
+        return FaceBc(LGD_BC_FLUX, g)
+
+    @staticmethod
+    def extrapolate() -> "FaceBc":
+        return FaceBc(LGD_BC_EXTRAPOLATE, 0.0)
+
+
+def _bcs_array(bcs: Optional[Sequence[FaceBc]]):
+    if bcs is None:
+        return None
+    arr = (_abi.FaceBcC * len(bcs))()
+    for i, b in enumerate(bcs):
+        arr[i].kind = b.kind
+        arr[i].value = b.value
+    return arr
+
+
+class CartesianGrid:
+    """CartesianGrid<D> (src/geometry/cartesian.rs:28-97) as a device block table."""
+
+    def __init__(self, ctx: Context, cells, block_cells, origin, spacing, periodic=None):
+        dim = len(cells)
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.dim = dim
+        self.cells = tuple(int(c) for c in cells)
+        self.block_cells = tuple(int(c) for c in block_cells)
+        self.origin = tuple(float(o) for o in origin)
+        self.spacing = tuple(float(s) for s in spacing)
+        self.periodic = tuple(bool(p) for p in (periodic or [False] * dim))
+        d = _abi.CartesianDesc()
+        d.dim = dim
+        for i in range(dim):
+            d.cells[i] = self.cells[i]
+            d.block_cells[i] = self.block_cells[i]
+            d.origin[i] = self.origin[i]
+            d.spacing[i] = self.spacing[i]
+            d.periodic[i] = int(self.periodic[i])
+        self.desc = d
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_grid_create_cartesian(ctx.handle, C.byref(d), C.byref(h)), ctx.handle)
+        self.handle = h
+
+    @property
+    def num_blocks(self) -> int:
+        return int(self._lib.lgd_grid_num_blocks(self.handle))
+
+    def block_len(self, ghost: int) -> int:
+        return int(self._lib.lgd_grid_block_len(self.handle, ghost))
+
+    def field_len(self, ghost: int) -> int:
+        return self.num_blocks * self.block_len(ghost)
+
+    def face_neighbor(self, block: int, dim: int, direction: int) -> Optional[int]:
+        r = int(self._lib.lgd_grid_face_neighbor(self.handle, block, dim, direction))
+        return None if r < 0 else r
+
+    def cell_key(self, ghost: int, offset: int) -> Optional[int]:
+        r = int(self._lib.lgd_grid_cell_key(self.handle, ghost, offset))
+        return None if r < 0 else r
+
+    def alloc_field(self, ghost: int = 1) -> DeviceBuffer:
+        return DeviceBuffer(self.ctx, self.field_len(ghost))
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.lgd_grid_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    # ---- per-intent kernels -------------------------------------------------
+    def fill_ghosts(self, field: DeviceBuffer, ghost: int = 1, bcs=None) -> None:
+        arr = _bcs_array(bcs)
+        _abi.check(
+            self._lib.lgd_fill_ghosts(self.ctx.handle, self.handle, field.handle, ghost, arr),
+            self.ctx.handle,
+        )
+
+    def central_laplacian(self, inp: DeviceBuffer, out: DeviceBuffer, ghost: int = 1) -> None:
+        _abi.check(
+            self._lib.lgd_central_laplacian(self.ctx.handle, self.handle, inp.handle, out.handle, ghost),
+            self.ctx.handle,
+        )
+
+    def karma_rappel_flux(self, phi: DeviceBuffer, out: DeviceBuffer, eps4: float, tol: float = 1e-8):
+        _abi.check(
+            self._lib.lgd_karma_rappel_flux(self.ctx.handle, self.handle, phi.handle, out.handle, eps4, tol),
+            self.ctx.handle,
+        )
+
+    def heat_rhs(self, kappa: float, u: DeviceBuffer, k: DeviceBuffer) -> None:
+        _abi.check(
+            self._lib.lgd_heat_rhs(self.ctx.handle, self.handle, kappa, u.handle, k.handle),
+            self.ctx.handle,
+        )
+
+    def modelc_rhs(self, params: "ModelCParams", phi, u, kphi, ku) -> None:
+        p = params.c()
+        _abi.check(
+            self._lib.lgd_modelc_rhs(
+                self.ctx.handle, self.handle, C.byref(p), phi.handle, u.handle, kphi.handle, ku.handle
+            ),
+            self.ctx.handle,
+        )
+
+    def modelc_amplitude(self, params: "ModelCParams", amp: DeviceBuffer) -> None:
+        p = params.c()
+        _abi.check(
+            self._lib.lgd_modelc_amplitude(self.ctx.handle, self.handle, C.byref(p), amp.handle),
+            self.ctx.handle,
+        )
+
+    def axpy(self, x: DeviceBuffer, y: DeviceBuffer, alpha: float) -> None:
+        _abi.check(self._lib.lgd_axpy(self.ctx.handle, x.handle, y.handle, alpha), self.ctx.handle)
+
+    def apply_wiener(self, x, amp, ghost, dt, seed, salt, stream) -> None:
+        _abi.check(
+            self._lib.lgd_apply_wiener(
+                self.ctx.handle, self.handle, x.handle, amp.handle, ghost, dt, seed, salt, stream
+            ),
+            self.ctx.handle,
+        )
+
+    def modelc_step(self, params, phi, u, phi_next, u_next, dt, seed, salt, wiener) -> None:
+        p = params.c()
+        _abi.check(
+            self._lib.lgd_modelc_step(
+                self.ctx.handle, self.handle, C.byref(p), phi.handle, u.handle, phi_next.handle,
+                u_next.handle, dt, seed, salt, int(wiener)
+            ),
+            self.ctx.handle,
+        )
+
+    def heat_step(self, kappa, u, u_next, dt) -> None:
+        _abi.check(
+            self._lib.lgd_heat_step(self.ctx.handle, self.handle, kappa, u.handle, u_next.handle, dt),
+            self.ctx.handle,
+        )
+
+
+@dataclasses.dataclass
+class ModelCParams:
+    """ModelC public fields + FiniteVolume::tol (src/physics/phasefield.rs:55-116)."""
+
+    eps4: float
+    lambda_: float
+    d_thermal: float
+    noise_amplitude: float = 0.0
+    anisotropy_tol: float = 1e-8
+    fv_tol: float = 1e-8
+
+    @staticmethod
+    def new(eps4: float, lambda_: float, noise_amplitude: float) -> "ModelCParams":
+        """ModelC::new: D = a₂λ (phasefield.rs:100-116)."""
+        return ModelCParams(eps4, lambda_, A2 * lambda_, noise_amplitude)
+
+    @staticmethod
+    def classic() -> "ModelCParams":
+        """ModelC::classic: ε₄ = 0.06, λ = 3.19, no noise (phasefield.rs:93-95)."""
+        return ModelCParams.new(0.06, 3.19, 0.0)
+
+    def c(self) -> _abi.ModelCParamsC:
+        p = _abi.ModelCParamsC()
+        p.eps4 = self.eps4
+        p.lambda_ = self.lambda_
+        p.d_thermal = self.d_thermal
+        p.noise_amplitude = self.noise_amplitude
+        p.anisotropy_tol = self.anisotropy_tol
+        p.fv_tol = self.fv_tol
+        return p
+
+    def stable_dt(self, h: float) -> float:
+        """ModelC::stable_dt (phasefield.rs:446-449)."""
+        return 0.2 * h**2 / self.d_thermal
+
+
+class Simulation:
+    """Simulation (src/core/simulation.rs) driven by the GPU backend."""
+
+    def __init__(self, grid: CartesianGrid, handle: C.c_void_p, nfields: int):
+        self.grid = grid
+        self.ctx = grid.ctx
+        self._lib = grid._lib
+        self.handle = handle
+        self.nfields = nfields
+
+    @classmethod
+    def model_c(cls, grid: CartesianGrid, params: ModelCParams, wiener: bool = False,
+                seed: int = 0, mode: int = LGD_MODE_FUSED) -> "Simulation":
+        p = params.c()
+        h = C.c_void_p()
+        _abi.check(
+            grid._lib.lgd_sim_create_modelc(grid.ctx.handle, grid.handle, C.byref(p), int(wiener),
+                                            seed, mode, C.byref(h)),
+            grid.ctx.handle,
+        )
+        return cls(grid, h, 2)
+
+    @classmethod
+    def heat(cls, grid: CartesianGrid, kappa: float, bcs=None, mode: int = LGD_MODE_FUSED):
+        arr = _bcs_array(bcs)
+        h = C.c_void_p()
+        _abi.check(
+            grid._lib.lgd_sim_create_heat(grid.ctx.handle, grid.handle, kappa, arr, mode, C.byref(h)),
+            grid.ctx.handle,
+        )
+        return cls(grid, h, 1)
+
+    def field(self, index: int) -> DeviceBuffer:
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_sim_field(self.handle, index, C.byref(h)), self.ctx.handle)
+        return DeviceBuffer._borrow(self.ctx, h)
+
+    def step(self, dt: float, nsteps: int = 1) -> None:
+        _abi.check(self._lib.lgd_sim_step(self.handle, dt, nsteps), self.ctx.handle)
+
+    @property
+    def time(self) -> float:
+        return float(self._lib.lgd_sim_time(self.handle))
+
+    @property
+    def step_index(self) -> int:
+        return int(self._lib.lgd_sim_step_index(self.handle))
+
+    def initialize_grains(self, grains: np.ndarray, u0: float) -> None:
+        g = np.ascontiguousarray(grains, dtype=np.float64).reshape(-1, 3)
+        _abi.check(
+            self._lib.lgd_sim_initialize_grains(self.handle, g.ctypes.data_as(_abi.DP), g.shape[0], u0),
+            self.ctx.handle,
+        )
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.lgd_sim_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

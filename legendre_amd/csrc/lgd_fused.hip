@@ -1,0 +1,180 @@
+// Fused hot-path kernels: one launch per time step.
+//
+// Halos are not materialised: a cell outside the block is read straight from the
+// neighbour's interior (exchange / periodic wrap) or from this block's own
+// interior (mirror), which is the value fill_ghosts_mirror would have written
+// there (src/geometry/cartesian.rs:439-526) — for copy-type rules the dimension
+// sweep resolves every ghost, corners included, to the cell at the per-dimension
+// reflected/wrapped global coordinate.  The RHS and the stage update are fused, and
+// the result goes to a second buffer pair (neighbours still read the old values).
+#include <cmath>
+
+#include "lgd_internal.h"
+
+namespace {
+
+// Global-coordinate resolution of one dimension: wrap (periodic) or reflect
+// (mirror, reflect_coord in src/geometry/boundary.rs:82-90) into [0, N).
+__device__ __forceinline__ int64_t resolve_coord(int64_t gi, int64_t N, int periodic) {
+  if (gi < 0) return periodic ? gi + N : -1 - gi;
+  if (gi >= N) return periodic ? gi - N : 2 * N - 1 - gi;
+  return gi;
+}
+
+// Element offset of global cell (gi, gj, gk) (already in range) in a ghost-1 field buffer.
+__device__ __forceinline__ int64_t cell_offset(const lgd_cart &c, int64_t gi, int64_t gj,
+                                               int64_t gk) {
+  int64_t bx = gi / c.n[0], by = gj / c.n[1], bz = gk / c.n[2];
+  int64_t i = gi - bx * c.n[0], j = gj - by * c.n[1], k = gk - bz * c.n[2];
+  int64_t ex = c.dim > 0 ? c.n[0] + 2 : 1;
+  int64_t ey = c.dim > 1 ? c.n[1] + 2 : 1;
+  int64_t ez = c.dim > 2 ? c.n[2] + 2 : 1;
+  int64_t b = bx + c.nb[0] * (by + c.nb[1] * bz);
+  int64_t o = (i + 1);
+  if (c.dim > 1) o += (j + 1) * ex;
+  if (c.dim > 2) o += (k + 1) * ex * ey;
+  return b * ex * ey * ez + o;
+}
+
+constexpr int kTX = 64;
+constexpr int kTY = 4;
+
+// Model C Euler–Maruyama step, one thread per interior cell (2-D).
+// Workgroups tile one block at a time: blockIdx.x = block * tiles + tile.
+template <bool kNoise>
+__global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
+    lgd_cart c, lgd_kr_consts k, int64_t tiles_x, int64_t tiles_per_block, const double *phi,
+    const double *u, double *phi_n, double *u_n, double dt, double noise_scale, uint64_t seed,
+    uint64_t salt, uint64_t stream) {
+  int64_t b = blockIdx.x / tiles_per_block;
+  int64_t tile = blockIdx.x - b * tiles_per_block;
+  int64_t ty = tile / tiles_x, tx = tile - ty * tiles_x;
+  int64_t i = tx * kTX + threadIdx.x;
+  int64_t j = ty * kTY + threadIdx.y;
+  if (i >= c.n[0] || j >= c.n[1]) return;
+  int64_t bx = b % c.nb[0], by = b / c.nb[0];
+  int64_t NX = c.nb[0] * c.n[0], NY = c.nb[1] * c.n[1];
+  int64_t gi = bx * c.n[0] + i, gj = by * c.n[1] + j;
+  double p[3][3];
+  for (int dj = -1; dj <= 1; ++dj) {
+    int64_t rj = resolve_coord(gj + dj, NY, c.periodic[1]);
+    for (int di = -1; di <= 1; ++di) {
+      int64_t ri = resolve_coord(gi + di, NX, c.periodic[0]);
+      p[dj + 1][di + 1] = phi[cell_offset(c, ri, rj, 0)];
+    }
+  }
+  int64_t oc = cell_offset(c, gi, gj, 0);
+  double uc = u[oc];
+  double ue = u[cell_offset(c, resolve_coord(gi + 1, NX, c.periodic[0]), gj, 0)];
+  double uw = u[cell_offset(c, resolve_coord(gi - 1, NX, c.periodic[0]), gj, 0)];
+  double un = u[cell_offset(c, gi, resolve_coord(gj + 1, NY, c.periodic[1]), 0)];
+  double us = u[cell_offset(c, gi, resolve_coord(gj - 1, NY, c.periodic[1]), 0)];
+  double dphi, du;
+  lgd_modelc_cell(k, p, uc, ue, uw, un, us, dphi, du);
+  // apply_step_with: Time first (x += dt*k), then Wiener(0) on phi
+  // (src/core/state.rs:656-699, src/core/driver.rs:131-158).
+  double pn = p[1][1] + dt * dphi;
+  if (kNoise) {
+    uint64_t block_key = lgd_mix_key3(seed, salt, stream, (uint64_t)b);
+    uint64_t key = lgd_splitmix64(block_key ^ (uint64_t)(i + j * c.n[0]));
+    pn = pn + noise_scale * lgd_standard_normal(key);
+  }
+  phi_n[oc] = pn;
+  u_n[oc] = uc + dt * du;
+}
+
+// Heat<D> forward-Euler step (examples/heat3d.rs:63-79, src/integrators/euler.rs:128-129):
+// u' = u + dt*(κ·∇²u), one thread per interior cell, any dimension 1..3.
+__global__ void heat_step_kernel(lgd_cart c, double ih0, double ih1, double ih2, double kappa,
+                                 double dt, const double *u, double *u_n) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t ncell = c.n[0] * c.n[1] * c.n[2];
+  if (t >= c.nblocks * ncell) return;
+  int64_t b = t / ncell;
+  int64_t r = t - b * ncell;
+  int64_t i = r % c.n[0];
+  r /= c.n[0];
+  int64_t j = r % c.n[1];
+  int64_t kk = r / c.n[1];
+  int64_t bc[3] = {b % c.nb[0], (b / c.nb[0]) % c.nb[1], b / (c.nb[0] * c.nb[1])};
+  int64_t g[3] = {bc[0] * c.n[0] + i, bc[1] * c.n[1] + j, bc[2] * c.n[2] + kk};
+  int64_t N[3] = {c.nb[0] * c.n[0], c.nb[1] * c.n[1], c.nb[2] * c.n[2]};
+  int64_t oc = cell_offset(c, g[0], g[1], g[2]);
+  double center = u[oc];
+  double inv[3] = {ih0, ih1, ih2};
+  double acc = 0.0;
+  for (int d = 0; d < c.dim; ++d) {
+    int64_t gp[3] = {g[0], g[1], g[2]};
+    int64_t gm[3] = {g[0], g[1], g[2]};
+    gp[d] = resolve_coord(g[d] + 1, N[d], c.periodic[d]);
+    gm[d] = resolve_coord(g[d] - 1, N[d], c.periodic[d]);
+    double up = u[cell_offset(c, gp[0], gp[1], gp[2])];
+    double um = u[cell_offset(c, gm[0], gm[1], gm[2])];
+    acc = acc + (((up - (center + center)) + um) * inv[d]);
+  }
+  u_n[oc] = center + dt * (acc * kappa);
+}
+
+}  // namespace
+
+using namespace lgd;
+
+extern "C" {
+
+int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                    const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
+                    double dt, uint64_t seed, uint64_t salt, int32_t wiener) {
+  if (!ctx || !grid || !params) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  const lgd_cart &c = grid->cart;
+  if (c.dim != 2) return fail(ctx, LGD_ERR_UNSUPPORTED, "Model C is 2-D");
+  int rc;
+  if ((rc = check_field(ctx, grid, phi, 1, "phi"))) return rc;
+  if ((rc = check_field(ctx, grid, u, 1, "u"))) return rc;
+  if ((rc = check_field(ctx, grid, phi_next, 1, "phi_next"))) return rc;
+  if ((rc = check_field(ctx, grid, u_next, 1, "u_next"))) return rc;
+  if (phi_next == phi || u_next == u || phi_next == u || u_next == phi || phi_next == u_next)
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT,
+                "the fused step reads neighbours' old values: output buffers must be distinct "
+                "from the inputs");
+  lgd_kr_consts k = lgd_make_kr_consts(*params, c);
+  int64_t tiles_x = (c.n[0] + kTX - 1) / kTX;
+  int64_t tiles_y = (c.n[1] + kTY - 1) / kTY;
+  int64_t tiles = tiles_x * tiles_y;
+  int64_t nwg = c.nblocks * tiles;
+  if (nwg > (int64_t)INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "grid too large for one launch");
+  dim3 wg(kTX, kTY);
+  // Wiener amplitude b on interior cells; a zero amplitude draws nothing
+  // (src/core/driver.rs:144-146).
+  bool noise = wiener && params->noise_amplitude != 0.0;
+  double scale = std::sqrt(dt) * params->noise_amplitude;
+  const uint64_t stream = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
+  if (noise)
+    modelc_step_kernel<true><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
+        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, scale, seed,
+        salt, stream);
+  else
+    modelc_step_kernel<false><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
+        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, scale, seed,
+        salt, stream);
+  return launch_check(ctx, "modelc_step");
+}
+
+int lgd_heat_step(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_buf *u,
+                  lgd_buf *u_next, double dt) {
+  if (!ctx || !grid) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  const lgd_cart &c = grid->cart;
+  int rc;
+  if ((rc = check_field(ctx, grid, u, 1, "u"))) return rc;
+  if ((rc = check_field(ctx, grid, u_next, 1, "u_next"))) return rc;
+  if ((rc = check_ghost(ctx, grid, 1))) return rc;
+  if (u == u_next)
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "the fused step needs a distinct output buffer");
+  double ih[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < c.dim; ++d) ih[d] = 1.0 / (c.h[d] * c.h[d]);
+  int64_t total = c.nblocks * c.n[0] * c.n[1] * c.n[2];
+  heat_step_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(c, ih[0], ih[1], ih[2], kappa,
+                                                                    dt, u->data, u_next->data);
+  return launch_check(ctx, "heat_step");
+}
+
+}  // extern "C"

@@ -1,0 +1,183 @@
+// Internal declarations shared by the legendre_gpu translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "../../include/legendre_gpu.h"
+
+struct lgd_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool owns_stream = false;
+  std::string last_error;
+};
+
+struct lgd_buf {
+  lgd_ctx *ctx = nullptr;
+  double *data = nullptr;
+  size_t len = 0;
+};
+
+// Host and device view of a CartesianGrid<D> (src/geometry/cartesian.rs:28-35).
+struct lgd_cart {
+  int32_t dim;
+  int32_t periodic[3];
+  int64_t n[3];    // block_cells, 1 for unused dims
+  int64_t nb[3];   // blocks_per_dim, 1 for unused dims
+  double h[3];     // spacing
+  double origin[3];
+  int64_t nblocks;
+};
+
+struct lgd_grid {
+  lgd_ctx *ctx = nullptr;
+  lgd_cart cart{};
+};
+
+namespace lgd {
+
+// Record an error message on the context and return the status.
+int fail(lgd_ctx *ctx, int status, const std::string &msg);
+int hip_fail(lgd_ctx *ctx, hipError_t e, const char *what);
+
+inline int64_t slab_len(const lgd_cart &c, uint32_t ghost) {
+  int64_t len = 1;
+  for (int d = 0; d < c.dim; ++d) len *= c.n[d] + 2 * (int64_t)ghost;
+  return len;
+}
+
+// Validate that `buf` holds exactly one slab of ghost width `ghost` per block.
+int check_field(lgd_ctx *ctx, const lgd_grid *grid, const lgd_buf *buf, uint32_t ghost,
+                const char *name);
+int check_ghost(lgd_ctx *ctx, const lgd_grid *grid, uint32_t ghost);
+
+// Number of workgroups for `n` items at `wg` items each.
+inline unsigned blocks_for(int64_t n, int wg) { return (unsigned)((n + wg - 1) / wg); }
+
+inline int launch_check(lgd_ctx *ctx, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(ctx, e, what);
+  return LGD_OK;
+}
+
+}  // namespace lgd
+
+// ---------------------------------------------------------------- device RNG
+// SplitMix64 finaliser, key chain, 53-bit open uniform and Box–Muller normal
+// (src/util/rng.rs:13-49).  Integer arithmetic wraps mod 2^64 as in the reference.
+__host__ __device__ inline uint64_t lgd_splitmix64(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__host__ __device__ inline uint64_t lgd_mix_key3(uint64_t seed, uint64_t a, uint64_t b,
+                                                 uint64_t c) {
+  uint64_t s = lgd_splitmix64(seed);
+  s = lgd_splitmix64(s ^ a);
+  s = lgd_splitmix64(s ^ b);
+  return lgd_splitmix64(s ^ c);
+}
+
+__host__ __device__ inline double lgd_unit_open(uint64_t bits) {
+  return (double)((bits >> 11) + 1) * (1.0 / 9007199254740992.0);
+}
+
+__device__ inline double lgd_standard_normal(uint64_t key) {
+  uint64_t a = lgd_splitmix64(key);
+  uint64_t b = lgd_splitmix64(a ^ 0xD1B54A32D192ED03ull);
+  double u = lgd_unit_open(a);
+  double v = lgd_unit_open(b);
+  return sqrt(-2.0 * log(u)) * cos(6.283185307179586 * v);
+}
+
+// --------------------------------------------------- Karma–Rappel constants
+// KarmaRappelFlux::new / ModelC::new, computed once on the host
+// (src/discretization/finite_volume.rs:113-122, src/physics/phasefield.rs:100-116).
+struct lgd_kr_consts {
+  double a_bar;      // fma(3, -eps4, 1)
+  double eps_prime;  // 3*eps4 / a_bar
+  double a12;        // 4*a_bar*eps_prime
+  double fv_tol;     // FiniteVolume::tol
+  double model_tol;  // ModelC::anisotropy_tol
+  double inv_h2;     // 1/(hx*hy)
+  double inv_hx2;    // 1/(hx*hx)  (CentralLaplacian on u)
+  double inv_hy2;    // 1/(hy*hy)
+  double lambda;
+  double d_thermal;
+};
+
+lgd_kr_consts lgd_make_kr_consts(const lgd_modelc_params &p, const lgd_cart &c);
+
+// face(dx, dy) (src/discretization/finite_volume.rs:129-142): anisotropy A and its
+// angular derivative with the |∇φ|² normalisation folded in.
+__device__ __forceinline__ void lgd_kr_face(const lgd_kr_consts &k, double dx, double dy,
+                                            double &a, double &da) {
+  double dx2 = dx * dx;
+  double dy2 = dy * dy;
+  double sum = dx2 + dy2;
+  double mag2 = sum * sum;
+  if (mag2 <= k.fv_tol) {
+    a = k.a_bar;
+    da = 0.0;
+  } else {
+    a = k.a_bar * (1.0 + (k.eps_prime * (dx2 * dx2 + dy2 * dy2)) / mag2);
+    da = ((((-k.a12) * dx) * dy) * (dx2 - dy2)) / mag2;
+  }
+}
+
+// ModelC::center_anisotropy (src/physics/phasefield.rs:159-175).
+__device__ __forceinline__ double lgd_center_anisotropy(const lgd_kr_consts &k, double pe,
+                                                        double pw, double pn, double ps) {
+  double dx = 0.5 * (pe - pw);
+  double dy = 0.5 * (pn - ps);
+  double dx2 = dx * dx;
+  double dy2 = dy * dy;
+  double sum = dx2 + dy2;
+  double mag2 = sum * sum;
+  if (mag2 <= k.model_tol) return k.a_bar;
+  return k.a_bar * (1.0 + (k.eps_prime * (dx2 * dx2 + dy2 * dy2)) / mag2);
+}
+
+// Model C right-hand side at one interior cell from its 3x3 phi neighbourhood
+// (p[j][i], j = row offset +1, i = column offset +1) and 5-point u neighbourhood:
+// the composition KarmaRappelFlux::kernel → pointwise dφ → CentralLaplacian(u) → du
+// (finite_volume.rs:258-282, phasefield.rs:394-421), in the reference's order.
+__device__ __forceinline__ void lgd_modelc_cell(const lgd_kr_consts &k, const double p[3][3],
+                                                double uc, double ue, double uw, double un,
+                                                double us, double &dphi, double &du) {
+  const double q = 0.25;
+  double derx_r = p[1][2] - p[1][1];
+  double derx_l = p[1][1] - p[1][0];
+  double derx_t = q * (((p[2][2] - p[2][0]) + p[1][2]) - p[1][0]);
+  double derx_b = q * (((p[1][2] - p[1][0]) + p[0][2]) - p[0][0]);
+  double dery_t = p[2][1] - p[1][1];
+  double dery_b = p[1][1] - p[0][1];
+  double dery_r = q * (((p[2][2] - p[0][2]) + p[2][1]) - p[0][1]);
+  double dery_l = q * (((p[2][1] - p[0][1]) + p[2][0]) - p[0][0]);
+  double a_r, da_r, a_l, da_l, a_t, da_t, a_b, da_b;
+  lgd_kr_face(k, derx_r, dery_r, a_r, da_r);
+  lgd_kr_face(k, derx_l, dery_l, a_l, da_l);
+  lgd_kr_face(k, derx_t, dery_t, a_t, da_t);
+  lgd_kr_face(k, derx_b, dery_b, a_b, da_b);
+  double j_r = a_r * ((a_r * derx_r) - (da_r * dery_r));
+  double j_l = a_l * ((a_l * derx_l) - (da_l * dery_l));
+  double j_t = a_t * ((a_t * dery_t) + (da_t * derx_t));
+  double j_b = a_b * ((a_b * dery_b) + (da_b * derx_b));
+  double flux = (((j_r - j_l) + j_t) - j_b) * k.inv_h2;
+  double pc = p[1][1];
+  double g_prime = fma(pc * pc, pc, -pc);
+  double one_m_p2 = fma(pc, -pc, 1.0);
+  double p_prime = one_m_p2 * one_m_p2;
+  double a = lgd_center_anisotropy(k, p[1][2], p[1][0], p[2][1], p[0][1]);
+  dphi = fma(k.lambda * uc, -p_prime, flux - g_prime) / (a * a);
+  double acc = 0.0;
+  acc = acc + (((ue - (uc + uc)) + uw) * k.inv_hx2);
+  acc = acc + (((un - (uc + uc)) + us) * k.inv_hy2);
+  du = fma(acc, k.d_thermal, 0.5 * dphi);
+}

@@ -1,0 +1,131 @@
+/*
+ * legendre CPU oracle — TEST INFRASTRUCTURE ONLY.
+ *
+ * A C++ restatement of the reference crate's SerialScheduler semantics for the
+ * block-structured hot path (counter RNG, Cartesian block topology, ghost fill,
+ * 2nd-order stencils, Karma–Rappel Model C, forward-Euler / Euler–Maruyama
+ * stage updates, time accumulation, initial conditions).  It is the checker the
+ * GPU backend is compared against; nothing in the product path may link it.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Layout contract (identical to the device backend): a *field buffer* holds the
+ * ghost-inclusive slabs of every block of one field, block-major
+ * (block b at offset b * block_len), each slab dimension-0 fastest
+ * (src/geometry/cartesian.rs:9-11, 189-204).
+ *
+ * Arithmetic contract: built with -ffp-contract=off; std::fma exactly where the
+ * reference calls mul_add; operation order as the reference writes it.
+ */
+#ifndef LEGENDRE_ORACLE_H
+#define LEGENDRE_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CartesianGrid<D> (src/geometry/cartesian.rs:28-35); dims >= dim are unused. */
+typedef struct {
+  int32_t dim;
+  int32_t periodic[3];
+  int64_t cells[3];
+  int64_t block_cells[3];
+  double origin[3];
+  double spacing[3];
+} orc_grid;
+
+/* FaceBc (src/geometry/boundary.rs:36-54). */
+enum { ORC_BC_MIRROR = 0, ORC_BC_DIRICHLET = 1, ORC_BC_FLUX = 2, ORC_BC_EXTRAPOLATE = 3 };
+typedef struct {
+  int32_t kind;
+  double value;
+} orc_face_bc;
+
+/* ModelC fields (src/physics/phasefield.rs:55-76) + FiniteVolume::tol
+ * (src/discretization/finite_volume.rs:41-50). */
+typedef struct {
+  double eps4;
+  double lambda;
+  double d_thermal;
+  double noise_amplitude;
+  double anisotropy_tol;
+  double fv_tol;
+} orc_modelc_params;
+
+/* ---- src/util/rng.rs ---- */
+uint64_t orc_splitmix64(uint64_t z);
+uint64_t orc_mix_key(uint64_t seed, const uint64_t *parts, size_t n);
+double orc_unit_open(uint64_t bits);
+double orc_standard_normal(uint64_t key);
+
+/* ---- src/geometry/cartesian.rs ---- */
+int orc_grid_check(const orc_grid *g);
+int64_t orc_num_blocks(const orc_grid *g);
+int64_t orc_block_len(const orc_grid *g, uint32_t ghost);
+int64_t orc_face_neighbor(const orc_grid *g, int64_t block, int32_t dim, int32_t dir);
+int64_t orc_cell_key(const orc_grid *g, uint32_t ghost, int64_t offset);
+void orc_cell_center(const orc_grid *g, int64_t block, const int64_t *idx, double *out);
+
+/* fill_ghosts_bc / fill_ghosts_mirror (cartesian.rs:377-401, 439-526).
+ * bcs: 2*dim entries [d*2 + (side>0)], NULL = Mirror everywhere.
+ * Returns 0, or -1 when the ghost width exceeds a block extent. */
+int orc_fill_ghosts(const orc_grid *g, double *field, uint32_t ghost, const orc_face_bc *bcs);
+
+/* central_laplacian_kernel (src/discretization/finite_difference.rs:49-67),
+ * interior of out; both buffers ghost-width `ghost`. */
+void orc_central_laplacian(const orc_grid *g, const double *in, double *out, uint32_t ghost);
+
+/* KarmaRappelFlux::kernel (src/discretization/finite_volume.rs:248-284), ghost 1. */
+void orc_karma_rappel_flux(const orc_grid *g, const double *phi, double *out, double eps4,
+                           double tol);
+
+/* ModelC::vector_field_block, Time (phasefield.rs:367-421) and Wiener (361-366).
+ * Ghost width 1 for phi and u. The Wiener call first zeroes amp (fill_zero,
+ * src/integrators/mod.rs:263). */
+void orc_modelc_rhs(const orc_grid *g, const orc_modelc_params *p, const double *phi,
+                    const double *u, double *kphi, double *ku);
+void orc_modelc_amplitude(const orc_grid *g, const orc_modelc_params *p, double *amp_phi);
+
+/* Heat3 / Heat<D>::vector_field_block (examples/heat3d.rs:63-79): out = lap(u)*kappa. */
+void orc_heat_rhs(const orc_grid *g, double kappa, const double *u, double *k);
+
+/* block_axpy over whole slabs (src/core/state.rs:485-498). */
+void orc_axpy(double *x, const double *y, double alpha, size_t n);
+
+/* Driver::apply_slab (src/core/driver.rs:115-158) for one field buffer.
+ * stream: Driver::stream() of the driver (Time = 0, Wiener(j) = (1<<32)|j). */
+void orc_apply_time(double *x, const double *k, double dt, size_t n);
+void orc_apply_wiener(const orc_grid *g, double *x, const double *amp, uint32_t ghost, double dt,
+                      uint64_t seed, uint64_t salt, uint64_t stream);
+
+/* Simulation::step loops (src/core/simulation.rs:255-290) with
+ * EulerMaruyama::step (src/integrators/euler_maruyama.rs:34-75), Model C,
+ * on a CartesianGrid<2> with mirror ghosts. wiener = 0 runs ModelC<NoNoise>,
+ * 1 runs ModelC<Wiener<1>>.  *t is read as the start time and advanced by
+ * t += dt per step.  nthreads > 1 dispatches blocks over OpenMP threads
+ * (RayonScheduler equivalent; bitwise identical to serial). */
+int orc_modelc_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                   int64_t nsteps, double dt, double *t, uint64_t seed, int32_t wiener,
+                   int32_t nthreads);
+
+/* ForwardEuler::step (src/integrators/euler.rs:104-130) on Heat<D>; bcs as in
+ * orc_fill_ghosts. */
+int orc_heat_run(const orc_grid *g, double kappa, double *u, int64_t nsteps, double dt, double *t,
+                 const orc_face_bc *bcs, int32_t nthreads);
+
+/* ModelC::initialize_grains with orientation 0 (phasefield.rs:227-286):
+ * grains as [cx, cy, radius] triples. phi, u ghost width 1. */
+void orc_initialize_grains(const orc_grid *g, double *phi, double *u, const double *grains,
+                           size_t ngrains, double u0);
+
+/* seed_positions recipe (examples/model_c.rs:175-206), multi-seed branch:
+ * writes ngrains [cx, cy, radius] triples. */
+void orc_seed_positions(uint64_t seed, size_t nseeds, double extent, double radius,
+                        double *grains_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

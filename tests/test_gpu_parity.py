@@ -1,0 +1,332 @@
+"""GPU parity: the HIP backend (through the C ABI) against the CPU oracle on the
+same seeded inputs.  Integer/copy work and the deterministic FP64 path are
+compared bit for bit; the noise path (device log/cos vs glibc) within the
+tolerance the benchmark specification states (1e-12 relative per step, 1e-9
+after 1000 steps)."""
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+lg = pytest.importorskip("legendre_amd")
+
+
+def say(msg: str) -> None:
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def grids(ctx, orc, cells, block, h, periodic=None):
+    dim = len(cells)
+    per = periodic or [False] * dim
+    og = orc.grid(cells, block, [0.0] * dim, h, periodic=per)
+    gg = lg.CartesianGrid(ctx, cells, block, [0.0] * dim, h, periodic=per)
+    return og, gg
+
+
+def upload(gg, arr):
+    b = lg.DeviceBuffer(gg.ctx, arr.size)
+    b.upload(arr)
+    return b
+
+
+# ------------------------------------------------------------ per-intent kernels
+FILL_CASES = [
+    ([12], [3], [False], 2, None),
+    ([12, 8], [4, 4], [False, False], 1, None),
+    ([12, 8], [6, 2], [True, False], 2, None),
+    ([6, 6, 4], [3, 2, 2], [False, True, True], 1, None),
+    ([8, 8], [4, 4], [False, False], 2, [(1, 0.37), (2, -1.3), (3, 0.0), (1, -0.25)]),
+    ([8, 6, 4], [4, 3, 2], [False, False, False], 1,
+     [(2, 0.5), (3, 0.0), (1, 1.0), (0, 0.0), (3, 0.0), (2, -2.0)]),
+]
+
+
+@pytest.mark.parametrize("cells,block,periodic,ghost,bcs", FILL_CASES)
+def test_fill_ghosts_matches_oracle(ctx, orc, cells, block, periodic, ghost, bcs):
+    say(f"fill_ghosts cells={cells} block={block} periodic={periodic} ghost={ghost} bcs={bcs}")
+    og, gg = grids(ctx, orc, cells, block, [0.5] * len(cells), periodic)
+    rng = np.random.default_rng(len(cells) * 10 + ghost)
+    f = rng.standard_normal(orc.field_len(og, ghost))
+    d = upload(gg, f)
+    orc.fill_ghosts(og, f, ghost, bcs)
+    gg.fill_ghosts(d, ghost, [lg.FaceBc(k, v) for k, v in bcs] if bcs else None)
+    assert np.array_equal(d.download(), f)
+
+
+def test_fill_ghosts_rejects_ghost_wider_than_block(ctx, orc):
+    say("fill_ghosts ghost too wide")
+    _, gg = grids(ctx, orc, [4, 4], [2, 2], [1.0, 1.0])
+    d = gg.alloc_field(3)
+    with pytest.raises(lg.LegendreError) as e:
+        gg.fill_ghosts(d, 3)
+    assert e.value.status == 2
+
+
+@pytest.mark.parametrize("cells,block,periodic", [([16], [4], [True]), ([12, 8], [4, 4], [False, True]),
+                                                  ([8, 6, 4], [4, 3, 2], [True, False, False])])
+def test_laplacian_and_heat_rhs_match_oracle(ctx, orc, cells, block, periodic):
+    say(f"laplacian cells={cells}")
+    h = [0.1, 0.25, 0.3][: len(cells)]
+    og, gg = grids(ctx, orc, cells, block, h, periodic)
+    rng = np.random.default_rng(7)
+    f = rng.standard_normal(orc.field_len(og, 1))
+    orc.fill_ghosts(og, f, 1)
+    want = np.zeros_like(f)
+    orc.central_laplacian(og, f, want, 1)
+    d, out = upload(gg, f), gg.alloc_field(1)
+    gg.central_laplacian(d, out, 1)
+    assert np.array_equal(out.download(), want)
+    orc.heat_rhs(og, 0.7, f, want)
+    gg.heat_rhs(0.7, d, out)
+    assert np.array_equal(out.download(), want)
+
+
+def _phase_field(cells, seed=3):
+    ny, nx = cells[1], cells[0]
+    y, x = np.mgrid[0:ny, 0:nx] + 0.5
+    rng = np.random.default_rng(seed)
+    r = np.hypot(x - 0.37 * nx, y - 0.55 * ny)
+    phi = -np.tanh((r - 0.2 * min(nx, ny)) / math.sqrt(2.0)) + 0.02 * rng.standard_normal((ny, nx))
+    u = -0.55 + 0.05 * rng.standard_normal((ny, nx))
+    return phi, u
+
+
+@pytest.mark.parametrize("cells,block,periodic", [([24, 16], [8, 4], [False, False]),
+                                                  ([30, 20], [10, 20], [True, False]),
+                                                  ([16, 16], [16, 16], [True, True])])
+def test_karma_rappel_and_modelc_rhs_match_oracle(ctx, orc, cells, block, periodic):
+    say(f"modelc_rhs cells={cells} block={block} periodic={periodic}")
+    og, gg = grids(ctx, orc, cells, block, [0.4, 0.4], periodic)
+    pg, ug = _phase_field(cells)
+    phi = H.global_to_field(pg, cells, block, 1)
+    u = H.global_to_field(ug, cells, block, 1)
+    orc.fill_ghosts(og, phi, 1)
+    orc.fill_ghosts(og, u, 1)
+    dphi, du = upload(gg, phi), upload(gg, u)
+    want = np.zeros_like(phi)
+    orc.karma_rappel_flux(og, phi, want, 0.05)
+    out = gg.alloc_field(1)
+    gg.karma_rappel_flux(dphi, out, 0.05)
+    assert np.array_equal(out.download(), want)
+    p = lg.ModelCParams(0.05, 3.19, 2.0, 0.01)
+    op = orc.params(0.05, 3.19, 2.0, 0.01)
+    kp, ku = np.zeros_like(phi), np.zeros_like(phi)
+    orc.modelc_rhs(og, op, phi, u, kp, ku)
+    gkp, gku = gg.alloc_field(1), gg.alloc_field(1)
+    gg.modelc_rhs(p, dphi, du, gkp, gku)
+    assert np.array_equal(gkp.download(), kp)
+    assert np.array_equal(gku.download(), ku)
+    amp = np.full_like(phi, 9.0)
+    orc.modelc_amplitude(og, op, amp)
+    gamp = upload(gg, np.full_like(phi, 9.0))
+    gg.modelc_amplitude(p, gamp)
+    assert np.array_equal(gamp.download(), amp)
+
+
+def test_axpy_matches_oracle(ctx, orc):
+    say("axpy")
+    rng = np.random.default_rng(1)
+    x, y = rng.standard_normal(1000), rng.standard_normal(1000)
+    gx, gy = lg.DeviceBuffer(ctx, 1000), lg.DeviceBuffer(ctx, 1000)
+    gx.upload(x)
+    gy.upload(y)
+    _, gg = grids(ctx, orc, [4], [4], [1.0])
+    gg.axpy(gx, gy, 0.37)
+    orc.axpy(x, y, 0.37)
+    assert np.array_equal(gx.download(), x)
+
+
+def test_wiener_increments_match_oracle(ctx, orc):
+    """The integer key stream is exact; the deviate goes through the device
+    log/cos, so values agree to a few ulp (most bit for bit)."""
+    say("apply_wiener")
+    cells, block = [40, 30], [20, 10]
+    og, gg = grids(ctx, orc, cells, block, [0.4, 0.4])
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(orc.field_len(og, 1))
+    amp = np.zeros_like(x)
+    orc.modelc_amplitude(og, orc.params(0.05, 3.19, 2.0, 0.5), amp)
+    amp[::7] = 0.0  # some interior zeros must be skipped too
+    gx, ga = upload(gg, x), upload(gg, amp)
+    salt = lg.time_bits(0.008 * 17)
+    orc.apply_wiener(og, x, amp, 1, 0.008, 42, salt, (1 << 32) | 0)
+    gg.apply_wiener(gx, ga, 1, 0.008, 42, salt, (1 << 32) | 0)
+    got = gx.download()
+    assert np.array_equal(got[amp == 0.0], x[amp == 0.0])
+    assert np.allclose(got, x, rtol=2e-15, atol=1e-15)
+    assert np.mean(got == x) > 0.9
+
+
+# ------------------------------------------------------------------- heat
+@pytest.mark.parametrize("mode", [lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT])
+def test_heat3d_config_matches_oracle_bitwise(ctx, orc, mode):
+    """128³ as 4×4×4 blocks of 32³, periodic, 7-point Laplacian, FE, 100 steps."""
+    from legendre_amd.workloads import HEAT3D as C
+
+    say(f"heat3d 128^3 mode={mode}")
+    cells, block = [C.cells] * 3, [C.block] * 3
+    og, gg = grids(ctx, orc, cells, block, [C.dx] * 3, [True] * 3)
+    u0 = H.global_to_field(C.initial_global(), cells, block, 1)
+    sim = lg.Simulation.heat(gg, C.kappa, mode=mode)
+    sim.field(0).upload(u0)
+    sim.step(C.dt, C.steps)
+    want = u0.copy()
+    orc.heat_run(og, C.kappa, want, C.steps, C.dt, nthreads=8)
+    got = sim.field(0).download()
+    assert np.array_equal(H.field_to_global(got, cells, block, 1), H.field_to_global(want, cells, block, 1))
+    assert sim.step_index == C.steps
+
+
+def test_heat_per_intent_with_rule_faces_matches_oracle(ctx, orc):
+    say("heat 2-D Dirichlet/Flux faces per-intent")
+    cells, block, h, kappa = [16, 16], [8, 8], [0.5, 0.5], 0.7
+    og, gg = grids(ctx, orc, cells, block, h)
+    bcs = [(1, 0.3), (1, 1.7), (2, 0.1), (3, 0.0)]
+    rng = np.random.default_rng(4)
+    u0 = H.global_to_field(rng.random((16, 16)), cells, block, 1)
+    sim = lg.Simulation.heat(gg, kappa, [lg.FaceBc(k, v) for k, v in bcs], mode=lg.LGD_MODE_PER_INTENT)
+    sim.field(0).upload(u0)
+    dt = 0.25 * 0.25 / kappa
+    sim.step(dt, 50)
+    want = u0.copy()
+    orc.heat_run(og, kappa, want, 50, dt, bcs=bcs)
+    assert np.array_equal(H.field_to_global(sim.field(0).download(), cells, block, 1),
+                          H.field_to_global(want, cells, block, 1))
+
+
+def test_heat_fused_rejects_rule_faces(ctx, orc):
+    say("heat fused rejects Dirichlet")
+    _, gg = grids(ctx, orc, [8, 8], [4, 4], [0.5, 0.5])
+    with pytest.raises(lg.LegendreError) as e:
+        lg.Simulation.heat(gg, 0.7, [lg.FaceBc.dirichlet(1.0)] * 4, mode=lg.LGD_MODE_FUSED)
+    assert e.value.status == 5
+
+
+# ---------------------------------------------------------------- Model C
+def _modelc_pair(ctx, orc, cells, block, periodic, grains, noise, wiener, mode, u0=0.55,
+                 eps4=0.05, lam=3.19, D=2.0, seed=42):
+    og, gg = grids(ctx, orc, cells, block, [0.4, 0.4], periodic)
+    p = lg.ModelCParams(eps4, lam, D, noise)
+    sim = lg.Simulation.model_c(gg, p, wiener=wiener, seed=seed, mode=mode)
+    sim.initialize_grains(grains, u0)
+    phi = np.zeros(orc.field_len(og, 1))
+    u = np.zeros_like(phi)
+    orc.initialize_grains(og, phi, u, grains, u0)
+    return og, gg, sim, orc.params(eps4, lam, D, noise), phi, u
+
+
+def _interiors(f, cells, block):
+    return H.field_to_global(f, cells, block, 1)
+
+
+MODELC_CASES = [
+    # (cells, block, periodic, grains)
+    ([64, 64], [32, 32], [False, False], [[0.4, 0.4, 4.0]]),
+    ([90, 60], [30, 20], [True, False], [[10.0, 12.0, 3.0], [30.0, 5.0, 2.0]]),
+    ([50, 70], [50, 70], [False, True], [[10.0, 14.0, 4.0]]),
+    ([252, 126], [126, 126], [False, False], [[50.0, 25.0, 2.0], [51.0, 20.0, 2.0]]),
+    ([12, 8], [1, 2], [True, True], [[2.0, 1.6, 1.0]]),
+]
+
+
+@pytest.mark.parametrize("mode", [lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT])
+@pytest.mark.parametrize("case", range(len(MODELC_CASES)))
+def test_modelc_matches_oracle_bitwise(ctx, orc, case, mode):
+    cells, block, periodic, grains = MODELC_CASES[case]
+    steps = 200
+    say(f"modelc b=0 cells={cells} block={block} periodic={periodic} mode={mode} steps={steps}")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, periodic, np.array(grains), 0.0,
+                                           False, mode)
+    dt = 0.008
+    sim.step(dt, steps)
+    t = orc.modelc_run(og, op, phi, u, steps, dt, nthreads=8)
+    assert sim.time == t
+    assert np.array_equal(_interiors(sim.field(0).download(), cells, block), _interiors(phi, cells, block))
+    assert np.array_equal(_interiors(sim.field(1).download(), cells, block), _interiors(u, cells, block))
+
+
+@pytest.mark.parametrize("mode", [lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT])
+def test_modelc_noise_matches_oracle_within_tolerance(ctx, orc, mode):
+    cells, block = [252, 126], [126, 63]
+    grains = np.array([[50.0, 25.0, 3.0]])
+    say(f"modelc b=0.01 cells={cells} mode={mode}")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], grains, 0.01,
+                                           True, mode)
+    dt = 0.008
+    sim.step(dt, 1)
+    ph1, u1 = phi.copy(), u.copy()
+    orc.modelc_run(og, op, ph1, u1, 1, dt, seed=42, wiener=True, nthreads=8)
+    g1 = _interiors(sim.field(0).download(), cells, block)
+    w1 = _interiors(ph1, cells, block)
+    assert np.max(np.abs(g1 - w1) / np.maximum(np.abs(w1), 1e-300)) <= 1e-12
+    sim.step(dt, 199)
+    orc.modelc_run(og, op, ph1, u1, 199, dt, t0=dt, seed=42, wiener=True, nthreads=8)
+    g = _interiors(sim.field(0).download(), cells, block)
+    w = _interiors(ph1, cells, block)
+    assert np.max(np.abs(g - w)) <= 1e-9
+    assert np.max(np.abs(_interiors(sim.field(1).download(), cells, block) - _interiors(u1, cells, block))) <= 1e-9
+    # the noise is on: the trajectory differs from the quiet one
+    quiet = phi.copy()
+    orc.modelc_run(og, op, quiet, u.copy(), 200, dt, seed=42, wiener=False, nthreads=8)
+    assert not np.array_equal(_interiors(quiet, cells, block), w)
+
+
+def test_fused_and_per_intent_agree_and_runs_are_deterministic(ctx, orc):
+    cells, block = [252, 252], [126, 126]
+    grains = np.array([[20.0, 30.0, 2.0], [70.0, 60.0, 2.0]])
+    say("determinism fused twice + per-intent, b=0.01")
+    outs = []
+    for mode in (lg.LGD_MODE_FUSED, lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT):
+        _, gg, sim, *_ = _modelc_pair(ctx, orc, cells, block, [False, False], grains, 0.01, True, mode)
+        sim.step(0.008, 100)
+        outs.append((sim.field(0).download(), sim.field(1).download()))
+        sim.close()
+    for o in outs[1:]:
+        assert np.array_equal(_interiors(o[0], cells, block), _interiors(outs[0][0], cells, block))
+        assert np.array_equal(_interiors(o[1], cells, block), _interiors(outs[0][1], cells, block))
+
+
+@pytest.mark.slow
+def test_modelc_reduced_config_1000_steps_bitwise(ctx, orc):
+    """The benchmark physics at 1008² (8×8 blocks of 126²), 64 seeds, 1000 steps."""
+    from legendre_amd.workloads import ModelCConfig
+
+    C = ModelCConfig(cells=1008)
+    cells, block = [C.cells] * 2, [C.block] * 2
+    say("modelc 1008^2 x 1000 steps b=0 fused")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], C.grains(), 0.0,
+                                           False, lg.LGD_MODE_FUSED, u0=C.delta, eps4=C.eps4,
+                                           lam=C.lambda_, D=C.d_thermal)
+    sim.step(C.dt, C.steps)
+    orc.modelc_run(og, op, phi, u, C.steps, C.dt, nthreads=16)
+    assert np.array_equal(_interiors(sim.field(0).download(), cells, block), _interiors(phi, cells, block))
+    assert np.array_equal(_interiors(sim.field(1).download(), cells, block), _interiors(u, cells, block))
+
+
+@pytest.mark.slow
+def test_modelc_full_scale_first_steps_bitwise(ctx, orc):
+    """10080² (80×80 blocks of 126², 101,606,400 cells): 3 steps against the oracle."""
+    from legendre_amd.workloads import MODEL_C as C
+
+    cells, block = [C.cells] * 2, [C.block] * 2
+    say("modelc 10080^2 x 3 steps b=0 fused")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], C.grains(), 0.0,
+                                           False, lg.LGD_MODE_FUSED, u0=C.delta, eps4=C.eps4,
+                                           lam=C.lambda_, D=C.d_thermal)
+    sim.step(C.dt, 3)
+    orc.modelc_run(og, op, phi, u, 3, C.dt, nthreads=16)
+    got = sim.field(0).download()
+    fv, wv = H.field_view(got, cells, block, 1), H.field_view(phi, cells, block, 1)
+    assert np.array_equal(fv[:, 1:-1, 1:-1], wv[:, 1:-1, 1:-1])
+    del got, fv
+    gu = H.field_view(sim.field(1).download(), cells, block, 1)
+    uv = H.field_view(u, cells, block, 1)
+    assert np.array_equal(gu[:, 1:-1, 1:-1], uv[:, 1:-1, 1:-1])
