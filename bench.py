@@ -39,6 +39,8 @@ def run_gpu(args, C):
 
     stream = torch.cuda.Stream()  # default priority
     ctx = lg.Context(0, stream.cuda_stream)
+    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2}[args.kernel])
+    ctx.set_tuning("modelc_strips", args.strips)
     grid = lg.CartesianGrid(ctx, [C.cells] * 2, [C.block] * 2, [0.0, 0.0], [C.h, C.h])
     params = lg.ModelCParams(C.eps4, C.lambda_, C.d_thermal, C.noise)
     sim = lg.Simulation.model_c(grid, params, wiener=C.noise > 0.0, seed=C.seed,
@@ -119,6 +121,8 @@ def main() -> int:
     ap.add_argument("--noise", type=float, default=0.0)
     ap.add_argument("--mode", choices=["fused", "per-intent"], default="fused")
     ap.add_argument("--cells", type=int, default=10080)
+    ap.add_argument("--kernel", choices=["auto", "percell", "stream"], default="auto")
+    ap.add_argument("--strips", type=int, default=0, help="row strips per block (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
@@ -145,6 +149,8 @@ def main() -> int:
         cpu = run_cpu_baseline(C, args.cpu_budget)
         log(f"[bench] cpu baseline: {cpu}")
     workload = f"model_c_{C.cells}sq_{C.block}blocks_b{C.noise:g}_{args.mode}"
+    if args.kernel != "auto" or args.strips:
+        workload += f"_{args.kernel}_s{args.strips}"
     line = {
         "metric": METRIC,
         "value": round(value, 4),

@@ -92,6 +92,11 @@ int lgd_ctx_destroy(lgd_ctx *ctx);
 int lgd_ctx_sync(lgd_ctx *ctx);
 void *lgd_ctx_stream(lgd_ctx *ctx);
 const char *lgd_ctx_last_error(const lgd_ctx *ctx);
+/* Launch-shape knobs for this context (no reference counterpart: the reference has
+ * no launch shapes).  Keys: "modelc_kernel" (0 auto, 1 per-cell, 2 streaming) and
+ * "modelc_strips" (row strips per block for the streaming kernel, 0 = default).
+ * Results are bitwise independent of every knob. */
+int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value);
 
 /* ------------------------------------- Allocator / StorageBackend (storage.rs) */
 /* Allocator::allocate(len): zero-initialised storage of len f64

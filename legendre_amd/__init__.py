@@ -66,6 +66,10 @@ class Context:
     def stream(self) -> int:
         return self._lib.lgd_ctx_stream(self.handle) or 0
 
+    def set_tuning(self, key: str, value: int) -> None:
+        """Launch-shape knob (results are bitwise independent of it)."""
+        _abi.check(self._lib.lgd_ctx_set_tuning(self.handle, key.encode(), int(value)), self.handle)
+
     def close(self) -> None:
         if self.handle:
             self._lib.lgd_ctx_destroy(self.handle)

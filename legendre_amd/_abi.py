@@ -66,6 +66,7 @@ SIGNATURES = {
     "lgd_ctx_sync": (C.c_int, [P]),
     "lgd_ctx_stream": (P, [P]),
     "lgd_ctx_last_error": (C.c_char_p, [P]),
+    "lgd_ctx_set_tuning": (C.c_int, [P, C.c_char_p, I64]),
     "lgd_buf_alloc": (C.c_int, [P, SZ, PP]),
     "lgd_buf_free": (C.c_int, [P]),
     "lgd_buf_len": (SZ, [P]),

@@ -117,6 +117,21 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx) {
   return ctx ? ctx->last_error.c_str() : "null context";
 }
 
+int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
+  if (!ctx || !key) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  std::string k(key);
+  if (value < 0) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "tuning values are non-negative");
+  if (k == "modelc_kernel") {
+    if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "modelc_kernel is 0, 1 or 2");
+    ctx->modelc_kernel = value;
+  } else if (k == "modelc_strips") {
+    ctx->modelc_strips = value;
+  } else {
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "unknown tuning key " + k);
+  }
+  return LGD_OK;
+}
+
 int lgd_buf_alloc(lgd_ctx *ctx, size_t len, lgd_buf **out) {
   if (!ctx || !out) return LGD_ERR_INVALID_ARGUMENT;
   *out = nullptr;

@@ -137,25 +137,35 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
                 "the fused step reads neighbours' old values: output buffers must be distinct "
                 "from the inputs");
   lgd_kr_consts k = lgd_make_kr_consts(*params, c);
+  // Wiener amplitude b on interior cells; a zero amplitude draws nothing
+  // (src/core/driver.rs:144-146).
+  const bool noise_on = wiener && params->noise_amplitude != 0.0;
+  const double noise_scale = std::sqrt(dt) * params->noise_amplitude;
+  const uint64_t wiener0 = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
+  if (ctx->modelc_kernel != 1) {
+    // Streaming kernel: two strips per block by default (enough waves to fill
+    // the chip at the benchmark size with ~3% halo-row overhead).
+    int strips = ctx->modelc_strips > 0 ? (int)ctx->modelc_strips : 2;
+    rc = lgd_modelc_stream_launch(ctx, c, k, phi->data, u->data, phi_next->data, u_next->data,
+                                  dt, noise_on, noise_scale, seed, salt, wiener0, strips);
+    if (rc != LGD_ERR_UNSUPPORTED) return rc;
+    if (ctx->modelc_kernel == 2)
+      return fail(ctx, LGD_ERR_UNSUPPORTED, "streaming kernel needs blocks at most 126 cells wide");
+  }
   int64_t tiles_x = (c.n[0] + kTX - 1) / kTX;
   int64_t tiles_y = (c.n[1] + kTY - 1) / kTY;
   int64_t tiles = tiles_x * tiles_y;
   int64_t nwg = c.nblocks * tiles;
   if (nwg > (int64_t)INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "grid too large for one launch");
   dim3 wg(kTX, kTY);
-  // Wiener amplitude b on interior cells; a zero amplitude draws nothing
-  // (src/core/driver.rs:144-146).
-  bool noise = wiener && params->noise_amplitude != 0.0;
-  double scale = std::sqrt(dt) * params->noise_amplitude;
-  const uint64_t stream = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
-  if (noise)
+  if (noise_on)
     modelc_step_kernel<true><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
-        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, scale, seed,
-        salt, stream);
+        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, noise_scale,
+        seed, salt, wiener0);
   else
     modelc_step_kernel<false><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
-        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, scale, seed,
-        salt, stream);
+        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, noise_scale,
+        seed, salt, wiener0);
   return launch_check(ctx, "modelc_step");
 }
 

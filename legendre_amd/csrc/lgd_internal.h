@@ -14,6 +14,9 @@ struct lgd_ctx {
   hipStream_t stream = nullptr;
   bool owns_stream = false;
   std::string last_error;
+  // Launch-shape knobs (lgd_ctx_set_tuning); 0 = pick automatically.
+  int64_t modelc_kernel = 0;  // 1 = per-cell kernel, 2 = streaming kernel
+  int64_t modelc_strips = 0;  // strips (row ranges) per block for the streaming kernel
 };
 
 struct lgd_buf {
@@ -113,6 +116,13 @@ struct lgd_kr_consts {
 };
 
 lgd_kr_consts lgd_make_kr_consts(const lgd_modelc_params &p, const lgd_cart &c);
+
+// Streaming Model C step (lgd_modelc_stream.hip); LGD_ERR_UNSUPPORTED when the
+// block is wider than 126 interior cells.
+int lgd_modelc_stream_launch(lgd_ctx *ctx, const lgd_cart &c, const lgd_kr_consts &k,
+                             const double *phi, const double *u, double *phi_n, double *u_n,
+                             double dt, bool noise, double noise_scale, uint64_t seed,
+                             uint64_t salt, uint64_t stream, int strips);
 
 // face(dx, dy) (src/discretization/finite_volume.rs:129-142): anisotropy A and its
 // angular derivative with the |∇φ|² normalisation folded in.

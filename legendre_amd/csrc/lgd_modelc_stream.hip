@@ -1,0 +1,331 @@
+// Model C fused step, streaming form (the hot path for blocks up to 126 cells wide).
+//
+// Work decomposition: one wavefront advances one strip of rows of one block.  The
+// 64 lanes cover the ghost-inclusive slab row (up to 128 columns) with two columns
+// per lane, so a row of phi or u is one coalesced 16-byte load per lane.  The wave
+// marches up the strip keeping rows j-1, j, j+1 in registers and prefetching row
+// j+2 while row j is computed.
+//
+// Face sharing: the Karma–Rappel face flux at (i+1/2, j) is the same expression,
+// operand for operand, as the left-face flux of cell i+1, and the flux at
+// (i, j+1/2) the same as the bottom-face flux of cell (i, j+1)
+// (src/discretization/finite_volume.rs:261-280), so each face is evaluated once:
+// y-faces carry over in registers from one row to the next, x-faces come from the
+// neighbouring lane by a cross-lane shuffle.  Results are bitwise those of the
+// per-cell evaluation; divisions per cell drop from 10 to 6.
+//
+// Halos: the ghost ring of the slab is never read.  The two halo columns come
+// from the left/right neighbour block's interior (or this block's own edge column
+// at a mirror face, or the wrapped block at a periodic face) and the halo rows from
+// the block below/above, resolved per dimension in global coordinates — the values
+// fill_ghosts_mirror would have written (src/geometry/cartesian.rs:439-526).
+//
+// Scheduling: workgroups of four independent waves; workgroup ids are remapped so
+// that each of the eight XCDs sweeps one contiguous range of blocks in order, so
+// neighbouring blocks (whose edge columns a wave reads) are in flight on the same
+// XCD and share its L2.
+#include <cmath>
+
+#include "lgd_internal.h"
+
+namespace {
+
+constexpr int kWavesPerWG = 4;
+constexpr int kXcds = 8;
+
+struct StreamArgs {
+  lgd_kr_consts k;
+  const double *phi;
+  const double *u;
+  double *phi_n;
+  double *u_n;
+  double dt;
+  double noise_scale;  // sqrt(dt) * b
+  uint64_t seed;
+  uint64_t salt;
+  uint64_t stream;
+  int64_t nblocks;
+  int32_t nx, ny;      // block interior extent
+  int32_t nbx, nby;    // blocks per dimension
+  int32_t per_x, per_y;
+  int32_t strips;      // strips per block
+  int32_t rows;        // rows per strip
+  int32_t nwg;         // workgroups in the launch
+};
+
+__device__ __forceinline__ int64_t wrap_or_reflect(int64_t g, int64_t n, int periodic) {
+  if (g < 0) return periodic ? g + n : -1 - g;
+  if (g >= n) return periodic ? g - n : 2 * n - 1 - g;
+  return g;
+}
+
+struct RowSrc {
+  const double *center;  // slab row start (column 0) in this block column
+  const double *left;    // halo cell at column 0
+  const double *right;   // halo cell at column nx+1
+};
+
+// Source pointers of interior row r (-1 <= r <= ny) of block (bx, by).
+__device__ __forceinline__ RowSrc row_src(const StreamArgs &a, const double *f, int64_t bx,
+                                          int64_t by, int r, int64_t bxl, int il, int64_t bxr,
+                                          int ir) {
+  const int64_t sx = a.nx + 2;
+  const int64_t slab = sx * (a.ny + 2);
+  int64_t byr = by, jr = r;
+  if (r < 0 || r >= a.ny) {
+    // Halo row: the block below/above, or this block's own edge row (mirror).
+    int64_t gj = by * a.ny + r;
+    int64_t rj = wrap_or_reflect(gj, (int64_t)a.nby * a.ny, a.per_y);
+    byr = rj / a.ny;
+    jr = rj - byr * a.ny;
+  }
+  int64_t rowoff = (jr + 1) * sx;
+  RowSrc s;
+  s.center = f + (byr * a.nbx + bx) * slab + rowoff;
+  s.left = f + (byr * a.nbx + bxl) * slab + rowoff + (il + 1);
+  s.right = f + (byr * a.nbx + bxr) * slab + rowoff + (ir + 1);
+  return s;
+}
+
+// Load this lane's two columns of a row, with the halo columns patched in.
+template <bool kVec>
+__device__ __forceinline__ void load_row(const RowSrc &s, int lane, int sx, int hr_lane,
+                                         int hr_slot, double &v0, double &v1) {
+  const int c0 = 2 * lane;
+  if (kVec) {
+    if (c0 + 1 < sx) {
+      double2 v = *reinterpret_cast<const double2 *>(s.center + c0);
+      v0 = v.x;
+      v1 = v.y;
+    } else {
+      v0 = 0.0;
+      v1 = 0.0;
+    }
+  } else {
+    v0 = c0 < sx ? s.center[c0] : 0.0;
+    v1 = c0 + 1 < sx ? s.center[c0 + 1] : 0.0;
+  }
+  if (lane == 0) v0 = *s.left;
+  if (lane == hr_lane) {
+    if (hr_slot == 0)
+      v0 = *s.right;
+    else
+      v1 = *s.right;
+  }
+}
+
+__device__ __forceinline__ double from_left(double v) { return __shfl_up(v, 1, 64); }
+__device__ __forceinline__ double from_right(double v) { return __shfl_down(v, 1, 64); }
+
+// Karma–Rappel flux through an x-face: J = A·(A·derx − A′·dery).
+__device__ __forceinline__ double flux_x(const lgd_kr_consts &k, double derx, double dery) {
+  double a, da;
+  lgd_kr_face(k, derx, dery, a, da);
+  return a * ((a * derx) - (da * dery));
+}
+
+// Karma–Rappel flux through a y-face: J = A·(A·dery + A′·derx).
+__device__ __forceinline__ double flux_y(const lgd_kr_consts &k, double derx, double dery) {
+  double a, da;
+  lgd_kr_face(k, derx, dery, a, da);
+  return a * ((a * dery) + (da * derx));
+}
+
+// Pointwise Model C update of one cell from its flux divergence
+// (src/physics/phasefield.rs:395-421) and the stage update (src/core/driver.rs:131-138).
+__device__ __forceinline__ void cell_update(const lgd_kr_consts &k, double flux, double pc,
+                                            double pe, double pw, double pn, double ps, double uc,
+                                            double ue, double uw, double un, double us,
+                                            double dt, double &phi_new, double &u_new,
+                                            double &dphi_out) {
+  double g_prime = fma(pc * pc, pc, -pc);
+  double one_m_p2 = fma(pc, -pc, 1.0);
+  double p_prime = one_m_p2 * one_m_p2;
+  double a = lgd_center_anisotropy(k, pe, pw, pn, ps);
+  double dphi = fma(k.lambda * uc, -p_prime, flux - g_prime) / (a * a);
+  double acc = 0.0;
+  acc = acc + (((ue - (uc + uc)) + uw) * k.inv_hx2);
+  acc = acc + (((un - (uc + uc)) + us) * k.inv_hy2);
+  double du = fma(acc, k.d_thermal, 0.5 * dphi);
+  phi_new = pc + dt * dphi;
+  u_new = uc + dt * du;
+  dphi_out = dphi;
+}
+
+template <bool kNoise, bool kVec>
+__global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamArgs a) {
+  // XCD-aware workgroup order: XCD x (= blockIdx.x % 8) takes one contiguous
+  // range of logical workgroups.
+  const int w = blockIdx.x;
+  const int xcd = w % kXcds, idx = w / kXcds;
+  const int per = a.nwg / kXcds, rem = a.nwg % kXcds;
+  const int lw = xcd * per + min(xcd, rem) + idx;
+  // Wave-uniform by construction; readfirstlane lets the compiler keep the
+  // block/strip arithmetic on the scalar unit.
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t item = (int64_t)lw * kWavesPerWG + wave;
+  if (item >= a.nblocks * a.strips) return;  // whole wave exits together
+  const int64_t b = item / a.strips;
+  const int strip = (int)(item - b * a.strips);
+  const int j0 = strip * a.rows;
+  const int j1 = min(j0 + a.rows, a.ny);
+  const int64_t bx = b % a.nbx, by = b / a.nbx;
+  const int nx = a.nx, sx = nx + 2;
+  const int64_t slab = (int64_t)sx * (a.ny + 2);
+  const lgd_kr_consts &k = a.k;
+
+  // Halo columns: global column just left / right of this block, resolved.
+  const int64_t NX = (int64_t)a.nbx * nx;
+  int64_t gl = wrap_or_reflect(bx * nx - 1, NX, a.per_x);
+  int64_t gr = wrap_or_reflect(bx * nx + nx, NX, a.per_x);
+  const int64_t bxl = gl / nx, bxr = gr / nx;
+  const int il = (int)(gl - bxl * nx), ir = (int)(gr - bxr * nx);
+  const int hr_lane = (nx + 1) >> 1, hr_slot = (nx + 1) & 1;
+
+  const int c0 = 2 * lane, c1 = c0 + 1;
+  const bool in0 = c0 >= 1 && c0 <= nx;
+  const bool in1 = c1 <= nx;
+
+  uint64_t block_key = 0;
+  if (kNoise) block_key = lgd_mix_key3(a.seed, a.salt, a.stream, (uint64_t)b);
+
+  // Rows j0-1, j0, j0+1 of phi and u.
+  double pm0, pm1, pc0, pc1, pp0, pp1, um0, um1, uc0, uc1, up0, up1;
+  {
+    RowSrc s = row_src(a, a.phi, bx, by, j0 - 1, bxl, il, bxr, ir);
+    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pm0, pm1);
+    s = row_src(a, a.phi, bx, by, j0, bxl, il, bxr, ir);
+    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pc0, pc1);
+    s = row_src(a, a.phi, bx, by, j0 + 1, bxl, il, bxr, ir);
+    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pp0, pp1);
+    s = row_src(a, a.u, bx, by, j0 - 1, bxl, il, bxr, ir);
+    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, um0, um1);
+    s = row_src(a, a.u, bx, by, j0, bxl, il, bxr, ir);
+    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, uc0, uc1);
+    s = row_src(a, a.u, bx, by, j0 + 1, bxl, il, bxr, ir);
+    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, up0, up1);
+  }
+  const double q = 0.25;
+  double pmR = from_right(pm0);
+  double pcL = from_left(pc1), pcR = from_right(pc0);
+  double ppL = from_left(pp1), ppR = from_right(pp0);
+  double ucL = from_left(uc1), ucR = from_right(uc0);
+  // Flux through the bottom face of row j0 (the top face of row j0-1).
+  double jb0, jb1;
+  {
+    double pmL = from_left(pm1);
+    jb0 = flux_y(k, q * (((pc1 - pcL) + pm1) - pmL), pc0 - pm0);
+    jb1 = flux_y(k, q * (((pcR - pc0) + pmR) - pm0), pc1 - pm1);
+  }
+
+  for (int j = j0; j < j1; ++j) {
+    // Prefetch row j+2 (needed from the next iteration on).
+    double pn0 = 0.0, pn1 = 0.0, un0 = 0.0, un1 = 0.0;
+    const bool more = j + 1 < j1;
+    if (more) {
+      RowSrc s = row_src(a, a.phi, bx, by, j + 2, bxl, il, bxr, ir);
+      load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pn0, pn1);
+      s = row_src(a, a.u, bx, by, j + 2, bxl, il, bxr, ir);
+      load_row<kVec>(s, lane, sx, hr_lane, hr_slot, un0, un1);
+    }
+    // x-faces right of c0 and c1; the face left of c0 is the left lane's right-of-c1.
+    double jx0 = flux_x(k, pc1 - pc0, q * (((pp1 - pm1) + pp0) - pm0));
+    double jx1 = flux_x(k, pcR - pc1, q * (((ppR - pmR) + pp1) - pm1));
+    double jxl0 = from_left(jx1);
+    // y-faces above c0 and c1.
+    double jt0 = flux_y(k, q * (((pp1 - ppL) + pc1) - pcL), pp0 - pc0);
+    double jt1 = flux_y(k, q * (((ppR - pp0) + pcR) - pc0), pp1 - pc1);
+    double f0 = (((jx0 - jxl0) + jt0) - jb0) * k.inv_h2;
+    double f1 = (((jx1 - jx0) + jt1) - jb1) * k.inv_h2;
+    double n0, n1, v0, v1, d0, d1;
+    cell_update(k, f0, pc0, pc1, pcL, pp0, pm0, uc0, uc1, ucL, up0, um0, a.dt, n0, v0, d0);
+    cell_update(k, f1, pc1, pcR, pc0, pp1, pm1, uc1, ucR, uc0, up1, um1, a.dt, n1, v1, d1);
+    if (kNoise) {
+      // Wiener(0) on phi after the drift (src/core/driver.rs:139-158); cell key =
+      // interior linear index (src/geometry/cartesian.rs:599-619).
+      uint64_t rowkey = (uint64_t)j * (uint64_t)nx;
+      if (in0) n0 = n0 + a.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c0 - 1))));
+      if (in1) n1 = n1 + a.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c1 - 1))));
+    }
+    const int64_t o = b * slab + (int64_t)(j + 1) * sx + c0;
+    if (kVec && in0 && in1) {
+      *reinterpret_cast<double2 *>(a.phi_n + o) = make_double2(n0, n1);
+      *reinterpret_cast<double2 *>(a.u_n + o) = make_double2(v0, v1);
+    } else {
+      if (in0) {
+        a.phi_n[o] = n0;
+        a.u_n[o] = v0;
+      }
+      if (in1) {
+        a.phi_n[o + 1] = n1;
+        a.u_n[o + 1] = v1;
+      }
+    }
+    jb0 = jt0;
+    jb1 = jt1;
+    if (more) {
+      pm0 = pc0; pm1 = pc1; pmR = pcR;
+      pc0 = pp0; pc1 = pp1; pcL = ppL; pcR = ppR;
+      pp0 = pn0; pp1 = pn1;
+      ppL = from_left(pp1);
+      ppR = from_right(pp0);
+      um0 = uc0; um1 = uc1;
+      uc0 = up0; uc1 = up1;
+      up0 = un0; up1 = un1;
+      ucL = from_left(uc1);
+      ucR = from_right(uc0);
+    }
+  }
+}
+
+}  // namespace
+
+// Streaming launch; returns LGD_ERR_UNSUPPORTED when the block is wider than one
+// wavefront row (126 interior cells), in which case the caller uses the per-cell kernel.
+int lgd_modelc_stream_launch(lgd_ctx *ctx, const lgd_cart &c, const lgd_kr_consts &k,
+                             const double *phi, const double *u, double *phi_n, double *u_n,
+                             double dt, bool noise, double noise_scale, uint64_t seed,
+                             uint64_t salt, uint64_t stream, int strips) {
+  if (c.dim != 2 || c.n[0] + 2 > 128) return LGD_ERR_UNSUPPORTED;
+  StreamArgs a;
+  a.k = k;
+  a.phi = phi;
+  a.u = u;
+  a.phi_n = phi_n;
+  a.u_n = u_n;
+  a.dt = dt;
+  a.noise_scale = noise_scale;
+  a.seed = seed;
+  a.salt = salt;
+  a.stream = stream;
+  a.nblocks = c.nblocks;
+  a.nx = (int32_t)c.n[0];
+  a.ny = (int32_t)c.n[1];
+  a.nbx = (int32_t)c.nb[0];
+  a.nby = (int32_t)c.nb[1];
+  a.per_x = c.periodic[0];
+  a.per_y = c.periodic[1];
+  if (strips < 1) strips = 1;
+  if (strips > a.ny) strips = a.ny;
+  a.rows = (a.ny + strips - 1) / strips;
+  a.strips = (a.ny + a.rows - 1) / a.rows;
+  int64_t items = c.nblocks * a.strips;
+  int64_t nwg = (items + kWavesPerWG - 1) / kWavesPerWG;
+  if (nwg > INT32_MAX) return LGD_ERR_UNSUPPORTED;
+  a.nwg = (int32_t)nwg;
+  const bool vec = ((c.n[0] + 2) % 2) == 0;
+  dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
+  if (noise) {
+    if (vec)
+      modelc_stream_kernel<true, true><<<grid, block, 0, ctx->stream>>>(a);
+    else
+      modelc_stream_kernel<true, false><<<grid, block, 0, ctx->stream>>>(a);
+  } else {
+    if (vec)
+      modelc_stream_kernel<false, true><<<grid, block, 0, ctx->stream>>>(a);
+    else
+      modelc_stream_kernel<false, false><<<grid, block, 0, ctx->stream>>>(a);
+  }
+  return lgd::launch_check(ctx, "modelc_stream");
+}

@@ -253,3 +253,195 @@ def seed_positions(seed: int, nseeds: int, extent: float, radius: float) -> np.n
 
 def time_bits(t: float) -> int:
     return struct.unpack("<Q", struct.pack("<d", t))[0]
+
+
+# ---- AMR (2-D) -----------------------------------------------------------------
+class RegridPolicy(C.Structure):
+    _fields_ = [
+        ("every", C.c_int64),
+        ("buffer", C.c_int64),
+        ("efficiency", C.c_double),
+        ("min_width", C.c_int64),
+        ("threshold", C.c_double),
+        ("tag_field", C.c_int32),
+    ]
+
+
+_AMR_SIGS = {
+    "orc_amr_create": (C.c_int, [GP, C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_int64),
+                                 C.c_size_t, C.POINTER(C.c_void_p)]),
+    "orc_amr_destroy": (None, [C.c_void_p]),
+    "orc_amr_num_blocks": (C.c_int64, [C.c_void_p]),
+    "orc_amr_num_levels": (C.c_int32, [C.c_void_p]),
+    "orc_amr_patch": (None, [C.c_void_p, C.c_int64, C.POINTER(C.c_int64)]),
+    "orc_amr_block_offset": (C.c_int64, [C.c_void_p, C.c_int64, C.c_uint32]),
+    "orc_amr_field_len": (C.c_int64, [C.c_void_p, C.c_uint32]),
+    "orc_amr_find_patch": (C.c_int64, [C.c_void_p, C.c_int32, C.c_int64, C.c_int64]),
+    "orc_amr_cell_center": (None, [C.c_void_p, C.c_int64, C.c_int64, C.c_int64, DP]),
+    "orc_amr_restrict_level": (None, [C.c_void_p, DP, C.c_uint32, C.c_int32]),
+    "orc_amr_restrict": (None, [C.c_void_p, DP, C.c_uint32]),
+    "orc_amr_fill_level": (None, [C.c_void_p, DP, DP, C.c_double, C.c_uint32, C.c_int32]),
+    "orc_amr_fill_ghosts": (None, [C.c_void_p, DP, C.c_uint32]),
+    "orc_cluster": (C.c_size_t, [C.POINTER(C.c_int64), C.c_size_t, C.c_double, C.c_size_t,
+                                 C.POINTER(C.c_int64), C.c_size_t]),
+    "orc_amr_sim_create": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(ModelCParams), C.c_double,
+                                     C.c_int32, C.c_uint64, C.c_int32, C.POINTER(RegridPolicy),
+                                     C.POINTER(C.c_void_p)]),
+    "orc_amr_sim_destroy": (None, [C.c_void_p]),
+    "orc_amr_sim_grid": (C.c_void_p, [C.c_void_p]),
+    "orc_amr_sim_field": (DP, [C.c_void_p, C.c_int32]),
+    "orc_amr_sim_time": (C.c_double, [C.c_void_p]),
+    "orc_amr_sim_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int64]),
+}
+
+
+def _amr_lib():
+    h = lib()
+    if not getattr(h, "_amr_bound", False):
+        for name, (res, args) in _AMR_SIGS.items():
+            f = getattr(h, name)
+            f.restype = res
+            f.argtypes = args
+        h._amr_bound = True
+    return h
+
+
+class AmrGrid:
+    """AmrGrid<2> (src/geometry/amr/grid.rs) — a view over an oracle handle."""
+
+    def __init__(self, handle, owned=True):
+        self.h = handle
+        self.owned = owned
+
+    @classmethod
+    def from_patches(cls, base: Grid, ratios, levels):
+        """levels[l] = list of ((lo0, lo1), (hi0, hi1)) boxes of level l+1."""
+        L = _amr_lib()
+        boxes = []
+        for l, bl in enumerate(levels):
+            for (lo, hi) in bl:
+                boxes += [l + 1, lo[0], lo[1], hi[0], hi[1]]
+        r = (C.c_uint32 * max(1, len(ratios)))(*ratios)
+        b = (C.c_int64 * max(1, len(boxes)))(*boxes)
+        h = C.c_void_p()
+        rc = L.orc_amr_create(C.byref(base), r, len(ratios), b, len(boxes) // 5, C.byref(h))
+        if rc != 0:
+            raise ValueError(f"AmrGrid::from_patches error {rc}")
+        return cls(h)
+
+    def __del__(self):
+        if getattr(self, "owned", False) and self.h:
+            _amr_lib().orc_amr_destroy(self.h)
+            self.h = None
+
+    @property
+    def num_blocks(self):
+        return int(_amr_lib().orc_amr_num_blocks(self.h))
+
+    @property
+    def num_levels(self):
+        return int(_amr_lib().orc_amr_num_levels(self.h))
+
+    def patch(self, b):
+        out = (C.c_int64 * 5)()
+        _amr_lib().orc_amr_patch(self.h, b, out)
+        return int(out[0]), (int(out[1]), int(out[2])), (int(out[3]), int(out[4]))
+
+    def blocks_at(self, level):
+        return [b for b in range(self.num_blocks) if self.patch(b)[0] == level]
+
+    def offset(self, b, ghost=1):
+        return int(_amr_lib().orc_amr_block_offset(self.h, b, ghost))
+
+    def field_len(self, ghost=1):
+        return int(_amr_lib().orc_amr_field_len(self.h, ghost))
+
+    def find_patch(self, level, cell):
+        r = int(_amr_lib().orc_amr_find_patch(self.h, level, cell[0], cell[1]))
+        return None if r < 0 else r
+
+    def cell_center(self, b, i, j):
+        out = (C.c_double * 2)()
+        _amr_lib().orc_amr_cell_center(self.h, b, i, j, out)
+        return out[0], out[1]
+
+    def view(self, field: np.ndarray, b: int, ghost: int = 1) -> np.ndarray:
+        """(ny+2g, nx+2g) view of block b's slab."""
+        _, lo, hi = self.patch(b)
+        nx, ny = hi[0] - lo[0], hi[1] - lo[1]
+        o = self.offset(b, ghost)
+        return field[o : o + (nx + 2 * ghost) * (ny + 2 * ghost)].reshape(ny + 2 * ghost, nx + 2 * ghost)
+
+    def fill_by_center(self, field, f, ghost=1):
+        for b in range(self.num_blocks):
+            v = self.view(field, b, ghost)
+            ny, nx = v.shape[0] - 2 * ghost, v.shape[1] - 2 * ghost
+            for j in range(ny):
+                for i in range(nx):
+                    v[j + ghost, i + ghost] = f(self.cell_center(b, i, j))
+
+    def restrict(self, field, ghost=1):
+        _amr_lib().orc_amr_restrict(self.h, _dp(field), ghost)
+
+    def restrict_level(self, field, fine_level, ghost=1):
+        _amr_lib().orc_amr_restrict_level(self.h, _dp(field), ghost, fine_level)
+
+    def fill_ghosts(self, field, ghost=1):
+        _amr_lib().orc_amr_fill_ghosts(self.h, _dp(field), ghost)
+
+    def fill_level(self, field, level, old=None, alpha=0.0, ghost=1):
+        _amr_lib().orc_amr_fill_level(self.h, _dp(field), None if old is None else _dp(old), alpha,
+                                      ghost, level)
+
+
+def cluster(flags, efficiency, min_width):
+    """Berger–Rigoutsos (cluster.rs:144-155); returns boxes ((lo0, lo1), (hi0, hi1))."""
+    L = _amr_lib()
+    n = len(flags)
+    f = (C.c_int64 * max(1, 2 * n))(*[v for c in flags for v in c])
+    out = (C.c_int64 * max(4, 4 * n))()
+    k = int(L.orc_cluster(f, n, efficiency, min_width, out, max(1, n)))
+    return [((out[4 * i], out[4 * i + 1]), (out[4 * i + 2], out[4 * i + 3])) for i in range(k)]
+
+
+class AmrSim:
+    """Simulation on an AmrGrid (model 'heat' or 'modelc'; integ 'global' or 'subcycle')."""
+
+    def __init__(self, grid: AmrGrid, model="heat", params=None, kappa=0.0, integ="global",
+                 seed=0, wiener=False, policy=None):
+        L = _amr_lib()
+        grid.owned = False  # ownership moves to the simulation
+        h = C.c_void_p()
+        pol = None
+        if policy is not None:
+            pol = RegridPolicy(*policy) if not isinstance(policy, RegridPolicy) else policy
+        rc = L.orc_amr_sim_create(grid.h, {"heat": 0, "modelc": 1}[model],
+                                  C.byref(params) if params is not None else None, kappa,
+                                  {"global": 0, "subcycle": 1}[integ], seed, int(wiener),
+                                  C.byref(pol) if pol is not None else None, C.byref(h))
+        if rc != 0:
+            raise ValueError("orc_amr_sim_create failed")
+        self.h = h
+        self.nfields = 2 if model == "modelc" else 1
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            _amr_lib().orc_amr_sim_destroy(self.h)
+            self.h = None
+
+    @property
+    def grid(self) -> AmrGrid:
+        return AmrGrid(C.c_void_p(_amr_lib().orc_amr_sim_grid(self.h)), owned=False)
+
+    def field(self, f=0) -> np.ndarray:
+        """Live numpy view of the current field buffer (valid until the next step)."""
+        n = self.grid.field_len(1)
+        ptr = _amr_lib().orc_amr_sim_field(self.h, f)
+        return np.ctypeslib.as_array(ptr, shape=(n,))
+
+    @property
+    def time(self):
+        return float(_amr_lib().orc_amr_sim_time(self.h))
+
+    def step(self, dt, n=1):
+        _amr_lib().orc_amr_sim_step(self.h, dt, n)

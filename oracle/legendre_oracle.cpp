@@ -576,3 +576,815 @@ void orc_seed_positions(uint64_t seed, size_t nseeds, double extent, double radi
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// AMR (2-D): src/geometry/amr/{grid,intergrid,cluster,adapt}.rs and
+// src/integrators/subcycling.rs.
+// ===========================================================================
+#include <algorithm>
+#include <array>
+#include <memory>
+
+namespace {
+
+struct Box2 {
+  int64_t lo[2], hi[2];
+  int64_t extent(int d) const { return hi[d] - lo[d]; }
+  uint64_t cells() const {
+    int64_t a = extent(0), b = extent(1);
+    if (a <= 0 || b <= 0) return 0;
+    return (uint64_t)a * (uint64_t)b;
+  }
+  bool contains(int64_t x, int64_t y) const {
+    return lo[0] <= x && x < hi[0] && lo[1] <= y && y < hi[1];
+  }
+  bool intersects(const Box2 &o) const {
+    return lo[0] < o.hi[0] && o.lo[0] < hi[0] && lo[1] < o.hi[1] && o.lo[1] < hi[1];
+  }
+  Box2 grown(int64_t b) const { return Box2{{lo[0] - b, lo[1] - b}, {hi[0] + b, hi[1] + b}}; }
+  // CellBox::clipped (cluster.rs:102-109).
+  Box2 clipped(const Box2 &dom) const {
+    Box2 o = *this;
+    for (int d = 0; d < 2; ++d) {
+      o.lo[d] = std::max(o.lo[d], dom.lo[d]);
+      o.hi[d] = std::max(std::min(o.hi[d], dom.hi[d]), o.lo[d]);
+    }
+    return o;
+  }
+  bool operator==(const Box2 &o) const {
+    return lo[0] == o.lo[0] && lo[1] == o.lo[1] && hi[0] == o.hi[0] && hi[1] == o.hi[1];
+  }
+};
+
+inline int64_t div_euclid(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if (a % b < 0) q -= 1;
+  return q;
+}
+
+// reflect_coord (src/geometry/boundary.rs:82-90).
+inline int64_t reflect(int64_t c, int64_t lo, int64_t hi) {
+  if (c < lo) return 2 * lo - 1 - c;
+  if (c >= hi) return 2 * hi - 1 - c;
+  return c;
+}
+
+}  // namespace
+
+struct orc_amr {
+  orc_grid base;
+  std::vector<uint32_t> ratios;
+  std::vector<int32_t> level;        // per patch
+  std::vector<Box2> box;             // per patch
+  std::vector<uint32_t> level_start;  // populated levels + 1 entries
+  std::vector<std::array<double, 2>> spacing;  // per level (capacity)
+  std::vector<std::vector<uint32_t>> lookup;    // per populated level
+
+  int num_levels() const { return (int)level_start.size() - 1; }
+  int max_levels() const { return (int)ratios.size() + 1; }
+  Box2 domain(int l) const {
+    int64_t s = 1;
+    for (int i = 0; i < l; ++i) s *= (int64_t)ratios[i];
+    return Box2{{0, 0}, {base.cells[0] * s, base.cells[1] * s}};
+  }
+  int64_t nblocks() const { return (int64_t)box.size(); }
+  int64_t ext(int64_t b, int d) const { return box[b].extent(d); }
+  int64_t block_len(int64_t b, uint32_t g) const {
+    return (ext(b, 0) + 2 * (int64_t)g) * (ext(b, 1) + 2 * (int64_t)g);
+  }
+  std::vector<int64_t> offs[3];  // prefix slab offsets for ghost 0, 1, 2
+  void build_offsets() {
+    for (uint32_t g = 0; g < 3; ++g) {
+      offs[g].assign(box.size() + 1, 0);
+      for (size_t b = 0; b < box.size(); ++b) offs[g][b + 1] = offs[g][b] + block_len((int64_t)b, g);
+    }
+  }
+  int64_t offset(int64_t b, uint32_t g) const {
+    if (g < 3) return offs[g][(size_t)b];
+    int64_t o = 0;
+    for (int64_t i = 0; i < b; ++i) o += block_len(i, g);
+    return o;
+  }
+  int64_t field_len(uint32_t g) const { return offset(nblocks(), g); }
+  // AmrGrid::find_patch (src/geometry/amr/grid.rs:264-278); -1 for None.
+  int64_t find_patch(int l, int64_t x, int64_t y) const {
+    if (l < 0 || l >= (int)lookup.size()) return -1;
+    Box2 dom = domain(l);
+    if (!dom.contains(x, y)) return -1;
+    uint32_t pid = lookup[l][(size_t)(x + y * dom.hi[0])];
+    return pid == UINT32_MAX ? -1 : (int64_t)pid;
+  }
+  // A patch as a one-block Cartesian grid (extent, level spacing) so the
+  // per-block kernels above apply unchanged (src/discretization/*: the AMR
+  // Stencil impls call the same kernels with grid.spacing(block)).
+  orc_grid patch_grid(int64_t b) const {
+    orc_grid g{};
+    g.dim = 2;
+    for (int d = 0; d < 2; ++d) {
+      g.cells[d] = ext(b, d);
+      g.block_cells[d] = ext(b, d);
+      g.spacing[d] = spacing[level[b]][d];
+      g.origin[d] = 0.0;
+    }
+    return g;
+  }
+};
+
+namespace {
+
+// covered_by (grid.rs:329-356): every cell of `need` in some box of `cover`.
+bool covered_by(const Box2 &need, const std::vector<Box2> &cover) {
+  if (need.cells() == 0) return true;
+  int64_t c[2] = {need.lo[0], need.lo[1]};
+  for (;;) {
+    const Box2 *hit = nullptr;
+    for (const Box2 &b : cover)
+      if (b.contains(c[0], c[1])) {
+        hit = &b;
+        break;
+      }
+    if (!hit) return false;
+    c[0] = hit->hi[0] - 1;
+    int d = 0;
+    for (;;) {
+      c[d] += 1;
+      if (c[d] < need.hi[d]) break;
+      c[d] = need.lo[d];
+      d += 1;
+      if (d == 2) return true;
+    }
+  }
+}
+
+// AmrGrid::from_patches (grid.rs:90-196). Returns 0 or an error code:
+// 1 ratio, 2 empty patch, 3 not nested, 4 misaligned, 5 overlap.
+int amr_build(const orc_grid &base, const std::vector<uint32_t> &ratios,
+              const std::vector<std::vector<Box2>> &levels, orc_amr &out) {
+  if (levels.size() > ratios.size()) return 1;
+  for (uint32_t r : ratios)
+    if (r < 2) return 1;
+  out.base = base;
+  out.ratios = ratios;
+  out.level.clear();
+  out.box.clear();
+  int64_t nb = orc_num_blocks(&base);
+  for (int64_t b = 0; b < nb; ++b) {
+    int64_t c[3];
+    block_coords(&base, b, c);
+    Box2 bx{{c[0] * base.block_cells[0], c[1] * base.block_cells[1]},
+            {(c[0] + 1) * base.block_cells[0], (c[1] + 1) * base.block_cells[1]}};
+    out.level.push_back(0);
+    out.box.push_back(bx);
+  }
+  out.level_start = {0u, (uint32_t)out.box.size()};
+  std::vector<Box2> coarser(out.box);
+  for (size_t l = 0; l < levels.size(); ++l) {
+    int lev = (int)l + 1;
+    int64_t r = ratios[l];
+    Box2 dom = out.domain(lev);
+    const std::vector<Box2> &boxes = levels[l];
+    for (size_t i = 0; i < boxes.size(); ++i) {
+      const Box2 &bx = boxes[i];
+      if (bx.cells() == 0) return 2;
+      if (!(bx.clipped(dom) == bx)) return 3;
+      for (int d = 0; d < 2; ++d)
+        if (bx.lo[d] % r != 0 || bx.hi[d] % r != 0) return 4;
+      for (size_t k = 0; k < i; ++k)
+        if (bx.intersects(boxes[k])) return 5;
+      Box2 coarse{{div_euclid(bx.lo[0], r), div_euclid(bx.lo[1], r)},
+                  {div_euclid(bx.hi[0] + r - 1, r), div_euclid(bx.hi[1] + r - 1, r)}};
+      Box2 need = coarse.grown(1).clipped(out.domain(lev - 1));
+      if (!covered_by(need, coarser)) return 3;
+    }
+    for (const Box2 &bx : boxes) {
+      out.level.push_back(lev);
+      out.box.push_back(bx);
+    }
+    out.level_start.push_back((uint32_t)out.box.size());
+    coarser = boxes;
+  }
+  out.spacing.clear();
+  out.spacing.push_back({base.spacing[0], base.spacing[1]});
+  for (uint32_t r : ratios) {
+    auto prev = out.spacing.back();
+    out.spacing.push_back({prev[0] / (double)r, prev[1] / (double)r});
+  }
+  out.lookup.clear();
+  for (int l = 0; l < out.num_levels(); ++l) {
+    Box2 dom = out.domain(l);
+    std::vector<uint32_t> map((size_t)(dom.hi[0] * dom.hi[1]), UINT32_MAX);
+    for (uint32_t p = out.level_start[l]; p < out.level_start[l + 1]; ++p) {
+      const Box2 &b = out.box[p];
+      for (int64_t y = b.lo[1]; y < b.hi[1]; ++y)
+        for (int64_t x = b.lo[0]; x < b.hi[0]; ++x) map[(size_t)(x + y * dom.hi[0])] = p;
+    }
+    out.lookup.push_back(std::move(map));
+  }
+  out.build_offsets();
+  return 0;
+}
+
+inline double &at(const orc_amr &g, double *f, uint32_t ghost, int64_t b, int64_t i, int64_t j) {
+  int64_t sx = g.ext(b, 0) + 2 * (int64_t)ghost;
+  return f[g.offset(b, ghost) + (j + ghost) * sx + (i + ghost)];
+}
+
+// restrict_level (src/geometry/amr/intergrid.rs:110-150).
+void amr_restrict_level(const orc_amr &g, double *f, uint32_t ghost, int fine_level) {
+  int64_t r = g.ratios[fine_level - 1];
+  double inv = 1.0 / std::pow((double)r, 2);
+  for (uint32_t fb = g.level_start[fine_level]; fb < g.level_start[fine_level + 1]; ++fb) {
+    const Box2 &fp = g.box[fb];
+    Box2 covered{{fp.lo[0] / r, fp.lo[1] / r}, {fp.hi[0] / r, fp.hi[1] / r}};
+    for (uint32_t cb = g.level_start[fine_level - 1]; cb < g.level_start[fine_level]; ++cb) {
+      const Box2 &cp = g.box[cb];
+      Box2 ov = covered.clipped(cp);
+      if (ov.cells() == 0) continue;
+      for (int64_t cy = ov.lo[1]; cy < ov.hi[1]; ++cy)
+        for (int64_t cx = ov.lo[0]; cx < ov.hi[0]; ++cx) {
+          double sum = 0.0;
+          for (int64_t ky = cy * r - fp.lo[1]; ky < (cy + 1) * r - fp.lo[1]; ++ky)
+            for (int64_t kx = cx * r - fp.lo[0]; kx < (cx + 1) * r - fp.lo[0]; ++kx)
+              sum = sum + at(g, f, ghost, fb, kx, ky);
+          at(g, f, ghost, cb, cx - cp.lo[0], cy - cp.lo[1]) = sum * inv;
+        }
+    }
+  }
+}
+
+// prolonged (intergrid.rs:414-459): bilinear from level-1 interiors.
+double amr_prolonged(const orc_amr &g, const double *f, uint32_t ghost, int level, int64_t cx,
+                     int64_t cy) {
+  double r = (double)g.ratios[level - 1];
+  Box2 cd = g.domain(level - 1);
+  int64_t c0[2];
+  double w[2];
+  int64_t cell[2] = {cx, cy};
+  for (int d = 0; d < 2; ++d) {
+    double pc = ((double)cell[d] + 0.5) / r - 0.5;
+    double fl = std::floor(pc);
+    c0[d] = (int64_t)fl;
+    w[d] = pc - fl;
+  }
+  double acc = 0.0;
+  for (unsigned corner = 0; corner < 4; ++corner) {
+    int64_t cc[2] = {c0[0], c0[1]};
+    double weight = 1.0;
+    for (int d = 0; d < 2; ++d) {
+      if ((corner >> d) & 1u) {
+        cc[d] += 1;
+        weight *= w[d];
+      } else {
+        weight *= 1.0 - w[d];
+      }
+    }
+    if (weight == 0.0) continue;
+    for (int d = 0; d < 2; ++d) cc[d] = reflect(cc[d], cd.lo[d], cd.hi[d]);
+    int64_t qb = g.find_patch(level - 1, cc[0], cc[1]);
+    if (qb < 0) std::abort();  // proper nesting guarantees a donor
+    const Box2 &q = g.box[qb];
+    acc = acc + at(g, const_cast<double *>(f), ghost, qb, cc[0] - q.lo[0], cc[1] - q.lo[1]) * weight;
+  }
+  return acc;
+}
+
+// fill_one_level / fill_level (intergrid.rs:183-223, 358-408): gather every
+// ring value of a block, then write.
+void amr_fill_level(const orc_amr &g, double *f, const double *old, double alpha, uint32_t ghost,
+                    int level) {
+  if (ghost == 0) return;
+  Box2 dom = g.domain(level);
+  int64_t gw = ghost;
+  std::vector<std::pair<std::array<int64_t, 2>, double>> staged;
+  for (uint32_t pb = g.level_start[level]; pb < g.level_start[level + 1]; ++pb) {
+    const Box2 &p = g.box[pb];
+    staged.clear();
+    auto visit = [&](int64_t x, int64_t y) {
+      int64_t tx = reflect(x, dom.lo[0], dom.hi[0]), ty = reflect(y, dom.lo[1], dom.hi[1]);
+      int64_t src = g.find_patch(level, tx, ty);
+      double v;
+      if (src >= 0) {
+        const Box2 &sp = g.box[src];
+        v = at(g, f, ghost, src, tx - sp.lo[0], ty - sp.lo[1]);
+      } else {
+        double vn = amr_prolonged(g, f, ghost, level, tx, ty);
+        if (old) {
+          double vo = amr_prolonged(g, old, ghost, level, tx, ty);
+          v = vo * (1.0 - alpha) + vn * alpha;
+        } else {
+          v = vn;
+        }
+      }
+      staged.push_back({{x - p.lo[0], y - p.lo[1]}, v});
+    };
+    // for_each_ring_cell (intergrid.rs:73-86).
+    for (int d = 0; d < 2; ++d) {
+      for (int side = 0; side < 2; ++side) {
+        Box2 s = p;
+        for (int e = 0; e < d; ++e) {
+          s.lo[e] = p.lo[e] - gw;
+          s.hi[e] = p.hi[e] + gw;
+        }
+        s.lo[d] = side ? p.hi[d] : p.lo[d] - gw;
+        s.hi[d] = side ? p.hi[d] + gw : p.lo[d];
+        for (int64_t y = s.lo[1]; y < s.hi[1]; ++y)
+          for (int64_t x = s.lo[0]; x < s.hi[0]; ++x) visit(x, y);
+      }
+    }
+    for (auto &e : staged) at(g, f, ghost, pb, e.first[0], e.first[1]) = e.second;
+  }
+}
+
+void amr_fill_all(const orc_amr &g, double *f, uint32_t ghost) {
+  for (int l = 0; l < g.num_levels(); ++l) amr_fill_level(g, f, nullptr, 0.0, ghost, l);
+}
+
+void amr_restrict_all(const orc_amr &g, double *f, uint32_t ghost) {
+  for (int l = g.num_levels() - 1; l >= 1; --l) amr_restrict_level(g, f, ghost, l);
+}
+
+// ---- Berger–Rigoutsos (src/geometry/amr/cluster.rs:144-281) ----
+using Cell = std::array<int64_t, 2>;
+
+Box2 bounding(const Cell *c, size_t n) {
+  Box2 b{{c[0][0], c[0][1]}, {c[0][0], c[0][1]}};
+  for (size_t i = 0; i < n; ++i)
+    for (int d = 0; d < 2; ++d) {
+      b.lo[d] = std::min(b.lo[d], c[i][d]);
+      b.hi[d] = std::max(b.hi[d], c[i][d]);
+    }
+  b.hi[0] += 1;
+  b.hi[1] += 1;
+  return b;
+}
+
+bool find_hole(const Box2 &bx, const std::vector<uint32_t> sig[2], int &axis, int64_t &plane) {
+  bool found = false;
+  uint64_t bestc = 0;
+  for (int d = 0; d < 2; ++d) {
+    size_t n = sig[d].size();
+    for (size_t i = 1; i + 1 < n; ++i) {
+      if (sig[d][i] == 0) {
+        uint64_t c = std::min((uint64_t)i, (uint64_t)(n - 1 - i));
+        if (!found || c > bestc) {
+          found = true;
+          bestc = c;
+          axis = d;
+          plane = bx.lo[d] + (int64_t)i;
+        }
+      }
+    }
+  }
+  return found;
+}
+
+bool find_inflection(const Box2 &bx, const std::vector<uint32_t> sig[2], size_t min_width,
+                     int &axis, int64_t &plane) {
+  bool found = false;
+  int64_t bests = 0;
+  uint64_t bestc = 0;
+  for (int d = 0; d < 2; ++d) {
+    size_t n = sig[d].size();
+    if (n < 4) continue;
+    auto delta = [&](size_t i) {
+      return (int64_t)sig[d][i + 1] - 2 * (int64_t)sig[d][i] + (int64_t)sig[d][i - 1];
+    };
+    for (size_t i = 1; i < n - 2; ++i) {
+      int64_t a = delta(i), b = delta(i + 1);
+      if (a * b < 0) {
+        size_t pl = i + 1;
+        if (pl < min_width || n - pl < min_width) continue;
+        int64_t s = std::llabs(b - a);
+        uint64_t c = std::min((uint64_t)pl, (uint64_t)(n - pl));
+        if (!found || s > bests || (s == bests && c > bestc)) {
+          found = true;
+          bests = s;
+          bestc = c;
+          axis = d;
+          plane = bx.lo[d] + (int64_t)pl;
+        }
+      }
+    }
+  }
+  return found;
+}
+
+bool bisection(const Box2 &bx, size_t min_width, int &axis, int64_t &plane) {
+  // Iterator::max_by_key keeps the last maximum.
+  int d = 0;
+  for (int e = 1; e < 2; ++e)
+    if (bx.extent(e) >= bx.extent(d)) d = e;
+  if ((uint64_t)bx.extent(d) < 2 * (uint64_t)min_width) return false;
+  axis = d;
+  plane = bx.lo[d] + bx.extent(d) / 2;
+  return true;
+}
+
+void split_or_accept(Cell *flags, size_t n, double efficiency, size_t min_width,
+                     std::vector<Box2> &out) {
+  Box2 bx = bounding(flags, n);
+  double eff = (double)n / (double)bx.cells();
+  if (eff >= efficiency) {
+    out.push_back(bx);
+    return;
+  }
+  std::vector<uint32_t> sig[2];
+  for (int d = 0; d < 2; ++d) sig[d].assign((size_t)bx.extent(d), 0u);
+  for (size_t i = 0; i < n; ++i)
+    for (int d = 0; d < 2; ++d) sig[d][(size_t)(flags[i][d] - bx.lo[d])] += 1;
+  int axis = 0;
+  int64_t plane = 0;
+  if (!find_hole(bx, sig, axis, plane) && !find_inflection(bx, sig, min_width, axis, plane) &&
+      !bisection(bx, min_width, axis, plane)) {
+    out.push_back(bx);
+    return;
+  }
+  size_t lt = 0;
+  for (size_t j = 0; j < n; ++j)
+    if (flags[j][axis] < plane) {
+      std::swap(flags[lt], flags[j]);
+      lt += 1;
+    }
+  split_or_accept(flags, lt, efficiency, min_width, out);
+  split_or_accept(flags + lt, n - lt, efficiency, min_width, out);
+}
+
+// GradientTagger::tag_level (src/geometry/amr/adapt.rs:78-103).
+void tag_level(const orc_amr &g, const double *f, int level, double thr, std::vector<Cell> &flags) {
+  for (uint32_t pb = g.level_start[level]; pb < g.level_start[level + 1]; ++pb) {
+    const Box2 &p = g.box[pb];
+    for (int64_t j = 0; j < p.extent(1); ++j)
+      for (int64_t i = 0; i < p.extent(0); ++i) {
+        double gm = 0.0;
+        double *ff = const_cast<double *>(f);
+        double dx = std::fabs((at(g, ff, 1, pb, i + 1, j) - at(g, ff, 1, pb, i - 1, j)) * 0.5);
+        gm = std::fmax(gm, dx);
+        double dy = std::fabs((at(g, ff, 1, pb, i, j + 1) - at(g, ff, 1, pb, i, j - 1)) * 0.5);
+        gm = std::fmax(gm, dy);
+        if (gm > thr) flags.push_back({p.lo[0] + i, p.lo[1] + j});
+      }
+  }
+}
+
+void push_cells(const Box2 &b, std::vector<Cell> &out) {
+  if (b.lo[0] >= b.hi[0] || b.lo[1] >= b.hi[1]) return;
+  for (int64_t y = b.lo[1]; y < b.hi[1]; ++y)
+    for (int64_t x = b.lo[0]; x < b.hi[0]; ++x) out.push_back({x, y});
+}
+
+// hierarchy_acceptable (adapt.rs:269-290).
+bool hierarchy_acceptable(const orc_amr &g, const std::vector<std::vector<Box2>> &nl) {
+  if ((int)nl.size() != g.num_levels() - 1) return false;
+  for (size_t l = 0; l < nl.size(); ++l) {
+    std::vector<Box2> cur(g.box.begin() + g.level_start[l + 1], g.box.begin() + g.level_start[l + 2]);
+    uint64_t pc = 0, cc = 0;
+    for (auto &b : nl[l]) pc += b.cells();
+    for (auto &b : cur) cc += b.cells();
+    if (cc > pc + pc / 2) return false;
+    for (auto &b : nl[l])
+      if (!covered_by(b, cur)) return false;
+  }
+  return true;
+}
+
+// sample (adapt.rs:352-402).
+double amr_sample(const orc_amr &g, const double *f, uint32_t ghost, int level, int64_t x,
+                  int64_t y) {
+  if (level < g.num_levels()) {
+    int64_t b = g.find_patch(level, x, y);
+    if (b >= 0) return at(g, const_cast<double *>(f), ghost, b, x - g.box[b].lo[0], y - g.box[b].lo[1]);
+  }
+  double r = (double)g.ratios[level - 1];
+  Box2 cd = g.domain(level - 1);
+  int64_t c0[2];
+  double w[2];
+  int64_t cell[2] = {x, y};
+  for (int d = 0; d < 2; ++d) {
+    double pc = ((double)cell[d] + 0.5) / r - 0.5;
+    double fl = std::floor(pc);
+    c0[d] = (int64_t)fl;
+    w[d] = pc - fl;
+  }
+  double acc = 0.0;
+  for (unsigned corner = 0; corner < 4; ++corner) {
+    int64_t cc[2] = {c0[0], c0[1]};
+    double weight = 1.0;
+    for (int d = 0; d < 2; ++d) {
+      if ((corner >> d) & 1u) {
+        cc[d] += 1;
+        weight *= w[d];
+      } else {
+        weight *= 1.0 - w[d];
+      }
+    }
+    if (weight == 0.0) continue;
+    for (int d = 0; d < 2; ++d) cc[d] = reflect(cc[d], cd.lo[d], cd.hi[d]);
+    acc = acc + amr_sample(g, f, ghost, level - 1, cc[0], cc[1]) * weight;
+  }
+  return acc;
+}
+
+}  // namespace
+
+// ---- simulation on an AMR hierarchy ----
+struct orc_amr_sim {
+  std::unique_ptr<orc_amr> grid;
+  int32_t model = 0;  // 0 heat, 1 Model C
+  orc_modelc_params p{};
+  double kappa = 0.0;
+  int32_t integ = 0;  // 0 global dt (FE / EM), 1 Berger–Oliger subcycling
+  uint64_t seed = 0;
+  int32_t wiener = 0;
+  bool adaptive = false;
+  orc_regrid_policy policy{};
+  std::vector<std::vector<double>> state;  // one buffer per field (ghost 1)
+  std::vector<std::vector<double>> k, amp, snaps;
+  double t = 0.0;
+  int64_t step_index = 0;
+  int nfields() const { return model == 1 ? 2 : 1; }
+  double stable_dt(double h) const {
+    return model == 1 ? 0.2 * std::pow(h, 2) / p.d_thermal : 0.25 * std::pow(h, 2) / kappa;
+  }
+};
+
+namespace {
+
+void sim_alloc_stages(orc_amr_sim &s) {
+  size_t n = (size_t)s.grid->field_len(1);
+  s.k.assign((size_t)s.nfields(), std::vector<double>(n, 0.0));
+  s.amp.clear();
+  if (s.model == 1 && s.wiener) s.amp.assign(1, std::vector<double>(n, 0.0));
+  s.snaps.clear();
+  if (s.integ == 1) s.snaps.assign((size_t)s.grid->max_levels() * s.nfields(), std::vector<double>(n, 0.0));
+}
+
+// ModelC / Heat fill_ghosts on an AmrGrid (phasefield.rs:454-467):
+// restrict every field, then fill every field's ghosts.
+void sim_fill_global(orc_amr_sim &s) {
+  for (int f = 0; f < s.nfields(); ++f) amr_restrict_all(*s.grid, s.state[f].data(), 1);
+  for (int f = 0; f < s.nfields(); ++f) amr_fill_all(*s.grid, s.state[f].data(), 1);
+}
+
+// vector_field_block(Time) on one patch.
+void sim_rhs_block(orc_amr_sim &s, int64_t b) {
+  const orc_amr &g = *s.grid;
+  orc_grid pg = g.patch_grid(b);
+  int64_t off = g.offset(b, 1);
+  if (s.model == 1) {
+    modelc_rhs_block(&pg, &s.p, s.state[0].data() + off, s.state[1].data() + off,
+                     s.k[0].data() + off, s.k[1].data() + off);
+  } else {
+    double *kb = s.k[0].data() + off;
+    laplacian_block(&pg, s.state[0].data() + off, kb, 1);
+    for_each_interior(&pg, [&](const int64_t *idx) {
+      int64_t o = lin(&pg, idx, 1);
+      kb[o] = kb[o] * s.kappa;
+    });
+  }
+}
+
+// apply_step on one patch: Time then Wiener(0) (state.rs:656-753).
+void sim_apply_block(orc_amr_sim &s, int64_t b, double dt, uint64_t salt) {
+  const orc_amr &g = *s.grid;
+  int64_t off = g.offset(b, 1), len = g.block_len(b, 1);
+  for (int f = 0; f < s.nfields(); ++f)
+    orc_axpy(s.state[f].data() + off, s.k[f].data() + off, dt, (size_t)len);
+  if (s.model == 1 && s.wiener) {
+    orc_grid pg = g.patch_grid(b);
+    double *a = s.amp[0].data() + off;
+    // eval_tendency: zero the slab, interior = b.
+    for (int64_t i = 0; i < len; ++i) a[i] = 0.0;
+    for_each_interior(&pg, [&](const int64_t *idx) { a[lin(&pg, idx, 1)] = s.p.noise_amplitude; });
+    wiener_block(&pg, b, s.state[0].data() + off, a, 1, dt, s.seed, salt, (1ull << 32) | 0ull);
+  }
+}
+
+void sim_step_global(orc_amr_sim &s, double dt) {
+  uint64_t salt;
+  std::memcpy(&salt, &s.t, sizeof salt);
+  sim_fill_global(s);
+  for (int64_t b = 0; b < s.grid->nblocks(); ++b) sim_rhs_block(s, b);
+  // EM evaluates every amplitude before any update (Itô); the Model C
+  // amplitude does not depend on the state, so evaluation order is moot.
+  for (int64_t b = 0; b < s.grid->nblocks(); ++b) sim_apply_block(s, b, dt, salt);
+}
+
+// Subcycle::advance (src/integrators/subcycling.rs:195-267).
+void sim_advance(orc_amr_sim &s, int level, double t, double dt, double alpha) {
+  const orc_amr &g = *s.grid;
+  int nf = s.nfields();
+  for (int f = 0; f < nf; ++f) s.snaps[(size_t)level * nf + f] = s.state[f];
+  for (int f = 0; f < nf; ++f) {
+    const double *old = level > 0 ? s.snaps[(size_t)(level - 1) * nf + f].data() : nullptr;
+    amr_fill_level(g, s.state[f].data(), old, alpha, 1, level);
+  }
+  uint64_t salt;
+  std::memcpy(&salt, &t, sizeof salt);
+  for (uint32_t b = g.level_start[level]; b < g.level_start[level + 1]; ++b) sim_rhs_block(s, b);
+  for (uint32_t b = g.level_start[level]; b < g.level_start[level + 1]; ++b)
+    sim_apply_block(s, b, dt, salt);
+  int child = level + 1;
+  if (child < g.num_levels()) {
+    double dtc = s.stable_dt(std::fmin(g.spacing[level][0], g.spacing[level][1]));
+    double dtf = s.stable_dt(std::fmin(g.spacing[child][0], g.spacing[child][1]));
+    double nn = std::fmax(std::round(dtc / dtf), 1.0);
+    uint64_t n = (uint64_t)nn;
+    double dt_child = dt / (double)n;
+    for (uint64_t k = 0; k < n; ++k)
+      sim_advance(s, child, std::fma((double)k, dt_child, t), dt_child, (double)k / (double)n);
+    for (int f = 0; f < nf; ++f) amr_restrict_level(g, s.state[f].data(), 1, child);
+  }
+}
+
+// BergerOliger::regrid (adapt.rs:152-237) with GradientTagger; returns whether
+// the hierarchy was replaced.
+bool sim_regrid(orc_amr_sim &s) {
+  const orc_regrid_policy &pol = s.policy;
+  if (s.step_index % pol.every != 0) return false;
+  orc_amr &g = *s.grid;
+  double *tf = s.state[(size_t)pol.tag_field].data();
+  amr_restrict_all(g, tf, 1);
+  amr_fill_all(g, tf, 1);
+  int capacity = g.max_levels() - 1;
+  std::vector<std::vector<Box2>> nl((size_t)capacity);
+  for (int target = capacity; target >= 1; --target) {
+    int src = target - 1;
+    std::vector<Cell> flags;
+    if (src < g.num_levels()) tag_level(g, tf, src, pol.threshold, flags);
+    if (pol.buffer > 0) {
+      std::vector<Cell> seeds;
+      seeds.swap(flags);
+      for (const Cell &c : seeds)
+        push_cells(Box2{{c[0] - pol.buffer, c[1] - pol.buffer},
+                        {c[0] + pol.buffer + 1, c[1] + pol.buffer + 1}},
+                   flags);
+    }
+    if (target < capacity && !nl[(size_t)target].empty()) {
+      int64_t rf = g.ratios[(size_t)target], rs = g.ratios[(size_t)target - 1];
+      for (const Box2 &bx : nl[(size_t)target]) {
+        Box2 need{{bx.lo[0] / rf - 1, bx.lo[1] / rf - 1}, {bx.hi[0] / rf + 1, bx.hi[1] / rf + 1}};
+        Box2 par{{div_euclid(need.lo[0], rs), div_euclid(need.lo[1], rs)},
+                 {div_euclid(need.hi[0] + rs - 1, rs), div_euclid(need.hi[1] + rs - 1, rs)}};
+        push_cells(par, flags);
+      }
+    }
+    Box2 dom = g.domain(src);
+    std::vector<Cell> kept;
+    for (const Cell &c : flags)
+      if (dom.contains(c[0], c[1])) kept.push_back(c);
+    std::sort(kept.begin(), kept.end());
+    kept.erase(std::unique(kept.begin(), kept.end()), kept.end());
+    if (kept.empty()) continue;
+    int64_t r = g.ratios[(size_t)target - 1];
+    std::vector<Box2> boxes;
+    split_or_accept(kept.data(), kept.size(), pol.efficiency, (size_t)pol.min_width, boxes);
+    for (Box2 &b : boxes)
+      for (int d = 0; d < 2; ++d) {
+        b.lo[d] *= r;
+        b.hi[d] *= r;
+      }
+    nl[(size_t)target - 1] = boxes;
+  }
+  while (!nl.empty() && nl.back().empty()) nl.pop_back();
+  if (hierarchy_acceptable(g, nl)) return false;
+  auto ng = std::make_unique<orc_amr>();
+  if (amr_build(g.base, g.ratios, nl, *ng) != 0) std::abort();
+  // State::reshaped + migrate (adapt.rs:323-347).
+  std::vector<std::vector<double>> ns;
+  for (int f = 0; f < s.nfields(); ++f) {
+    std::vector<double> out((size_t)ng->field_len(1), 0.0);
+    for (uint32_t b = 0; b < ng->level_start[1]; ++b) {
+      int64_t len = ng->block_len(b, 1);
+      std::memcpy(out.data() + ng->offset(b, 1), s.state[f].data() + g.offset(b, 1),
+                  sizeof(double) * (size_t)len);
+    }
+    for (int l = 1; l < ng->num_levels(); ++l)
+      for (uint32_t b = ng->level_start[l]; b < ng->level_start[l + 1]; ++b) {
+        const Box2 &p = ng->box[b];
+        for (int64_t j = 0; j < p.extent(1); ++j)
+          for (int64_t i = 0; i < p.extent(0); ++i)
+            at(*ng, out.data(), 1, b, i, j) = amr_sample(g, s.state[f].data(), 1, l, p.lo[0] + i, p.lo[1] + j);
+      }
+    ns.push_back(std::move(out));
+  }
+  s.grid = std::move(ng);
+  s.state = std::move(ns);
+  sim_alloc_stages(s);
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_amr_create(const orc_grid *base, const uint32_t *ratios, size_t nratios,
+                   const int64_t *boxes, size_t nboxes, orc_amr **out) {
+  *out = nullptr;
+  if (base->dim != 2 || orc_grid_check(base) != 0) return -1;
+  std::vector<uint32_t> r(ratios, ratios + nratios);
+  std::vector<std::vector<Box2>> levels;
+  for (size_t i = 0; i < nboxes; ++i) {
+    int64_t lev = boxes[5 * i];
+    if (lev < 1) return -1;
+    while ((int64_t)levels.size() < lev) levels.emplace_back();
+    levels[(size_t)lev - 1].push_back(
+        Box2{{boxes[5 * i + 1], boxes[5 * i + 2]}, {boxes[5 * i + 3], boxes[5 * i + 4]}});
+  }
+  auto g = std::make_unique<orc_amr>();
+  int rc = amr_build(*base, r, levels, *g);
+  if (rc != 0) return rc;
+  *out = g.release();
+  return 0;
+}
+
+void orc_amr_destroy(orc_amr *g) { delete g; }
+int64_t orc_amr_num_blocks(const orc_amr *g) { return g->nblocks(); }
+int32_t orc_amr_num_levels(const orc_amr *g) { return g->num_levels(); }
+void orc_amr_patch(const orc_amr *g, int64_t b, int64_t *out) {
+  out[0] = g->level[b];
+  out[1] = g->box[b].lo[0];
+  out[2] = g->box[b].lo[1];
+  out[3] = g->box[b].hi[0];
+  out[4] = g->box[b].hi[1];
+}
+int64_t orc_amr_block_offset(const orc_amr *g, int64_t b, uint32_t ghost) { return g->offset(b, ghost); }
+int64_t orc_amr_field_len(const orc_amr *g, uint32_t ghost) { return g->field_len(ghost); }
+int64_t orc_amr_find_patch(const orc_amr *g, int32_t level, int64_t x, int64_t y) {
+  return g->find_patch(level, x, y);
+}
+void orc_amr_cell_center(const orc_amr *g, int64_t b, int64_t i, int64_t j, double *out) {
+  // AmrGrid::cell_center (grid.rs:397-405).
+  int l = g->level[b];
+  out[0] = std::fma((double)(g->box[b].lo[0] + i) + 0.5, g->spacing[l][0], g->base.origin[0]);
+  out[1] = std::fma((double)(g->box[b].lo[1] + j) + 0.5, g->spacing[l][1], g->base.origin[1]);
+}
+void orc_amr_restrict_level(const orc_amr *g, double *f, uint32_t ghost, int32_t fine_level) {
+  amr_restrict_level(*g, f, ghost, fine_level);
+}
+void orc_amr_restrict(const orc_amr *g, double *f, uint32_t ghost) { amr_restrict_all(*g, f, ghost); }
+void orc_amr_fill_level(const orc_amr *g, double *f, const double *old, double alpha,
+                        uint32_t ghost, int32_t level) {
+  amr_fill_level(*g, f, old, alpha, ghost, level);
+}
+void orc_amr_fill_ghosts(const orc_amr *g, double *f, uint32_t ghost) { amr_fill_all(*g, f, ghost); }
+
+int orc_amr_sim_create(orc_amr *grid, int32_t model, const orc_modelc_params *p, double kappa,
+                       int32_t integ, uint64_t seed, int32_t wiener,
+                       const orc_regrid_policy *policy, orc_amr_sim **out) {
+  auto s = new orc_amr_sim();
+  s->grid.reset(grid);
+  s->model = model;
+  if (p) s->p = *p;
+  s->kappa = kappa;
+  s->integ = integ;
+  s->seed = seed;
+  s->wiener = wiener;
+  if (policy) {
+    s->adaptive = true;
+    s->policy = *policy;
+  }
+  size_t n = (size_t)s->grid->field_len(1);
+  s->state.assign((size_t)s->nfields(), std::vector<double>(n, 0.0));
+  sim_alloc_stages(*s);
+  *out = s;
+  return 0;
+}
+
+void orc_amr_sim_destroy(orc_amr_sim *s) { delete s; }
+const orc_amr *orc_amr_sim_grid(const orc_amr_sim *s) { return s->grid.get(); }
+double *orc_amr_sim_field(orc_amr_sim *s, int32_t f) { return s->state[(size_t)f].data(); }
+double orc_amr_sim_time(const orc_amr_sim *s) { return s->t; }
+
+int orc_amr_sim_step(orc_amr_sim *s, double dt, int64_t nsteps) {
+  for (int64_t n = 0; n < nsteps; ++n) {
+    if (s->adaptive) sim_regrid(*s);
+    if (s->integ == 1)
+      sim_advance(*s, 0, s->t, dt, 0.0);
+    else
+      sim_step_global(*s, dt);
+    s->t = s->t + dt;
+    s->step_index += 1;
+  }
+  return 0;
+}
+
+}  // extern "C"
+
+extern "C" size_t orc_cluster(int64_t *flags, size_t n, double efficiency, size_t min_width,
+                              int64_t *out, size_t max_out) {
+  std::vector<Cell> cells(n);
+  for (size_t i = 0; i < n; ++i) cells[i] = {flags[2 * i], flags[2 * i + 1]};
+  std::vector<Box2> boxes;
+  if (n) split_or_accept(cells.data(), n, efficiency, min_width, boxes);
+  for (size_t i = 0; i < n; ++i) {
+    flags[2 * i] = cells[i][0];
+    flags[2 * i + 1] = cells[i][1];
+  }
+  for (size_t i = 0; i < boxes.size() && i < max_out; ++i) {
+    out[4 * i] = boxes[i].lo[0];
+    out[4 * i + 1] = boxes[i].lo[1];
+    out[4 * i + 2] = boxes[i].hi[0];
+    out[4 * i + 3] = boxes[i].hi[1];
+  }
+  return boxes.size();
+}

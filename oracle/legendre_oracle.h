@@ -125,6 +125,57 @@ void orc_initialize_grains(const orc_grid *g, double *phi, double *u, const doub
 void orc_seed_positions(uint64_t seed, size_t nseeds, double extent, double radius,
                         double *grains_out);
 
+/* ---- AMR, 2-D (src/geometry/amr, src/integrators/subcycling.rs) ---- */
+typedef struct orc_amr orc_amr;
+typedef struct orc_amr_sim orc_amr_sim;
+
+/* BergerOliger + GradientTagger knobs (src/geometry/amr/adapt.rs:55-129). */
+typedef struct {
+  int64_t every;
+  int64_t buffer;
+  double efficiency;
+  int64_t min_width;
+  double threshold;
+  int32_t tag_field;
+} orc_regrid_policy;
+
+/* AmrGrid::from_patches (grid.rs:90-196). boxes: nboxes × [level, lo0, lo1, hi0, hi1],
+ * levels >= 1 in level-(l) cell coordinates.  Returns 0, -1 (bad base) or the error
+ * kind: 1 ratio, 2 empty patch, 3 not nested, 4 misaligned, 5 overlap. */
+int orc_amr_create(const orc_grid *base, const uint32_t *ratios, size_t nratios,
+                   const int64_t *boxes, size_t nboxes, orc_amr **out);
+void orc_amr_destroy(orc_amr *g);
+int64_t orc_amr_num_blocks(const orc_amr *g);
+int32_t orc_amr_num_levels(const orc_amr *g);
+void orc_amr_patch(const orc_amr *g, int64_t block, int64_t *out5);
+int64_t orc_amr_block_offset(const orc_amr *g, int64_t block, uint32_t ghost);
+int64_t orc_amr_field_len(const orc_amr *g, uint32_t ghost);
+int64_t orc_amr_find_patch(const orc_amr *g, int32_t level, int64_t x, int64_t y);
+void orc_amr_cell_center(const orc_amr *g, int64_t block, int64_t i, int64_t j, double *out2);
+void orc_amr_restrict_level(const orc_amr *g, double *field, uint32_t ghost, int32_t fine_level);
+void orc_amr_restrict(const orc_amr *g, double *field, uint32_t ghost);
+/* fill_level (intergrid.rs:183-223) with optional time interpolation from `old`. */
+void orc_amr_fill_level(const orc_amr *g, double *field, const double *old, double alpha,
+                        uint32_t ghost, int32_t level);
+/* fill_ghosts_mirror (intergrid.rs:155-168): every level, global dt. */
+void orc_amr_fill_ghosts(const orc_amr *g, double *field, uint32_t ghost);
+/* cluster (cluster.rs:144-155): flags n × [x, y] (reordered in place); writes up to
+ * max_out boxes [lo0, lo1, hi0, hi1]; returns the number of boxes. */
+size_t orc_cluster(int64_t *flags, size_t n, double efficiency, size_t min_width, int64_t *out,
+                   size_t max_out);
+
+/* Simulation on an AmrGrid: model 0 = Heat (κ∇²u, FE), 1 = Model C (EM);
+ * integ 0 = global dt (fill = restrict + fill_ghosts_mirror), 1 = Subcycling;
+ * policy NULL = static hierarchy.  Takes ownership of `grid`. */
+int orc_amr_sim_create(orc_amr *grid, int32_t model, const orc_modelc_params *p, double kappa,
+                       int32_t integ, uint64_t seed, int32_t wiener,
+                       const orc_regrid_policy *policy, orc_amr_sim **out);
+void orc_amr_sim_destroy(orc_amr_sim *s);
+const orc_amr *orc_amr_sim_grid(const orc_amr_sim *s);
+double *orc_amr_sim_field(orc_amr_sim *s, int32_t field);
+double orc_amr_sim_time(const orc_amr_sim *s);
+int orc_amr_sim_step(orc_amr_sim *s, double dt, int64_t nsteps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -237,12 +237,33 @@ MODELC_CASES = [
 ]
 
 
-@pytest.mark.parametrize("mode", [lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT])
+VARIANTS = {
+    # name: (mode, modelc_kernel, modelc_strips)
+    "per_intent": (lg.LGD_MODE_PER_INTENT, 0, 0),
+    "fused_percell": (lg.LGD_MODE_FUSED, 1, 0),
+    "fused_stream_1strip": (lg.LGD_MODE_FUSED, 2, 1),
+    "fused_stream_3strips": (lg.LGD_MODE_FUSED, 2, 3),
+}
+
+
+@pytest.fixture
+def tuned(ctx):
+    def apply(kernel, strips):
+        ctx.set_tuning("modelc_kernel", kernel)
+        ctx.set_tuning("modelc_strips", strips)
+
+    yield apply
+    apply(0, 0)
+
+
+@pytest.mark.parametrize("variant", list(VARIANTS))
 @pytest.mark.parametrize("case", range(len(MODELC_CASES)))
-def test_modelc_matches_oracle_bitwise(ctx, orc, case, mode):
+def test_modelc_matches_oracle_bitwise(ctx, orc, tuned, case, variant):
     cells, block, periodic, grains = MODELC_CASES[case]
+    mode, kernel, strips = VARIANTS[variant]
     steps = 200
-    say(f"modelc b=0 cells={cells} block={block} periodic={periodic} mode={mode} steps={steps}")
+    say(f"modelc b=0 cells={cells} block={block} periodic={periodic} {variant} steps={steps}")
+    tuned(kernel, strips)
     og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, periodic, np.array(grains), 0.0,
                                            False, mode)
     dt = 0.008
@@ -253,11 +274,13 @@ def test_modelc_matches_oracle_bitwise(ctx, orc, case, mode):
     assert np.array_equal(_interiors(sim.field(1).download(), cells, block), _interiors(u, cells, block))
 
 
-@pytest.mark.parametrize("mode", [lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT])
-def test_modelc_noise_matches_oracle_within_tolerance(ctx, orc, mode):
+@pytest.mark.parametrize("variant", list(VARIANTS))
+def test_modelc_noise_matches_oracle_within_tolerance(ctx, orc, tuned, variant):
     cells, block = [252, 126], [126, 63]
     grains = np.array([[50.0, 25.0, 3.0]])
-    say(f"modelc b=0.01 cells={cells} mode={mode}")
+    mode, kernel, strips = VARIANTS[variant]
+    tuned(kernel, strips)
+    say(f"modelc b=0.01 cells={cells} {variant}")
     og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], grains, 0.01,
                                            True, mode)
     dt = 0.008
@@ -279,12 +302,16 @@ def test_modelc_noise_matches_oracle_within_tolerance(ctx, orc, mode):
     assert not np.array_equal(_interiors(quiet, cells, block), w)
 
 
-def test_fused_and_per_intent_agree_and_runs_are_deterministic(ctx, orc):
+def test_fused_and_per_intent_agree_and_runs_are_deterministic(ctx, orc, tuned):
+    """Every execution variant — and a repeat of the default — gives the same bits,
+    noise included (all variants share the device deviate)."""
     cells, block = [252, 252], [126, 126]
     grains = np.array([[20.0, 30.0, 2.0], [70.0, 60.0, 2.0]])
-    say("determinism fused twice + per-intent, b=0.01")
+    say("determinism across variants + repeat, b=0.01")
     outs = []
-    for mode in (lg.LGD_MODE_FUSED, lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT):
+    for variant in ["fused_stream_1strip"] + list(VARIANTS):
+        mode, kernel, strips = VARIANTS[variant]
+        tuned(kernel, strips)
         _, gg, sim, *_ = _modelc_pair(ctx, orc, cells, block, [False, False], grains, 0.01, True, mode)
         sim.step(0.008, 100)
         outs.append((sim.field(0).download(), sim.field(1).download()))
