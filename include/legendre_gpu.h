@@ -46,6 +46,7 @@ typedef enum {
 typedef struct lgd_ctx lgd_ctx;
 typedef struct lgd_buf lgd_buf;
 typedef struct lgd_grid lgd_grid;
+typedef struct lgd_amr lgd_amr;
 typedef struct lgd_sim lgd_sim;
 
 /* CartesianGrid<D>::new(cells, block_cells, origin, spacing).with_periodic(periodic)
@@ -174,6 +175,42 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
 int lgd_heat_step(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_buf *u,
                   lgd_buf *u_next, double dt);
 
+/* ------------------------------------- AMR hierarchy (src/geometry/amr), 2-D */
+/* AmrGrid::from_patches(base, ratios, levels) (src/geometry/amr/grid.rs:90-196).
+ * boxes: nboxes × [level, lo0, lo1, hi0, hi1] in level cell coordinates, level >= 1
+ * (level 0 is the base grid's block tiling).  Invalid hierarchies (ratio < 2, empty,
+ * overlapping, misaligned or improperly nested patches) return
+ * LGD_ERR_INVALID_ARGUMENT with the reason.  Field buffers on the hierarchy hold
+ * ghost-1 slabs of every patch, patch order (level-major), lgd_amr_field_len long. */
+int lgd_amr_create(lgd_ctx *ctx, const lgd_cartesian_desc *base, const uint32_t *ratios,
+                   size_t nratios, const int64_t *boxes, size_t nboxes, lgd_amr **out);
+int lgd_amr_destroy(lgd_amr *amr);
+int64_t lgd_amr_num_blocks(const lgd_amr *amr);
+int32_t lgd_amr_num_levels(const lgd_amr *amr);
+/* AmrGrid::patch: [level, lo0, lo1, hi0, hi1] of block `block` (grid.rs:237-240). */
+int lgd_amr_patch(const lgd_amr *amr, int64_t block, int64_t *out5);
+int64_t lgd_amr_field_len(const lgd_amr *amr);
+int64_t lgd_amr_block_offset(const lgd_amr *amr, int64_t block);
+/* AmrGrid::find_patch (grid.rs:264-278); -1 for None. */
+int64_t lgd_amr_find_patch(const lgd_amr *amr, int32_t level, int64_t x, int64_t y);
+/* restrict_level (src/geometry/amr/intergrid.rs:110-150). */
+int lgd_amr_restrict_level(lgd_ctx *ctx, const lgd_amr *amr, lgd_buf *field, int32_t fine_level);
+/* fill_level (intergrid.rs:183-223): same-level exchange, physical mirror and bilinear
+ * coarse→fine prolongation into every ghost of one level; old != NULL interpolates the
+ * coarse source in time: old·(1−alpha) + field·alpha. */
+int lgd_amr_fill_level(lgd_ctx *ctx, const lgd_amr *amr, lgd_buf *field, const lgd_buf *old,
+                       double alpha, int32_t level);
+
+/* BergerOliger policy with a GradientTagger (src/geometry/amr/adapt.rs:55-129). */
+typedef struct {
+  int64_t every;      /* regrid every this many steps (step 0 always) */
+  int64_t buffer;     /* flag dilation in cells */
+  double efficiency;  /* ClusterParams::efficiency */
+  int64_t min_width;  /* ClusterParams::min_width */
+  double threshold;   /* GradientTagger::threshold */
+  int32_t tag_field;  /* field index the tagger reads */
+} lgd_regrid_policy;
+
 /* ---------------------------------------- Simulation (src/core/simulation.rs) */
 /* mode: how a step is executed. */
 typedef enum {
@@ -194,6 +231,16 @@ int lgd_sim_create_modelc(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_p
  * ghost 1; bcs as in lgd_fill_ghosts (fused mode needs Mirror/periodic). */
 int lgd_sim_create_heat(lgd_ctx *ctx, const lgd_grid *grid, double kappa,
                         const lgd_face_bc *bcs, int32_t mode, lgd_sim **out);
+/* Simulation::adaptive on an AmrGrid (simulation.rs:137-183): model 0 = Heat (κ∇²u,
+ * stable dt 0.25·h²/κ), 1 = Model C; subcycle = 0 runs EulerMaruyama with a global dt
+ * (fill_ghosts = restrict + fill_ghosts_mirror every field, phasefield.rs:452-467),
+ * 1 runs Subcycling (src/integrators/subcycling.rs:75-267).  policy NULL keeps the
+ * hierarchy static.  The simulation takes ownership of `amr`; after a regrid the
+ * current hierarchy is lgd_sim_amr(). */
+int lgd_sim_create_amr(lgd_ctx *ctx, lgd_amr *amr, int32_t model, const lgd_modelc_params *params,
+                       double kappa, int32_t wiener, uint64_t seed, int32_t subcycle,
+                       const lgd_regrid_policy *policy, lgd_sim **out);
+const lgd_amr *lgd_sim_amr(const lgd_sim *sim);
 int lgd_sim_destroy(lgd_sim *sim);
 /* The current state buffer of field `index` (valid until the next lgd_sim_step). */
 int lgd_sim_field(lgd_sim *sim, int32_t index, lgd_buf **out);

@@ -33,6 +33,8 @@ __all__ = [
     "LegendreError",
     "ModelCParams",
     "Simulation",
+    "AmrGrid",
+    "RegridPolicy",
     "time_bits",
     "LGD_MODE_FUSED",
     "LGD_MODE_PER_INTENT",
@@ -343,10 +345,10 @@ class ModelCParams:
 class Simulation:
     """Simulation (src/core/simulation.rs) driven by the GPU backend."""
 
-    def __init__(self, grid: CartesianGrid, handle: C.c_void_p, nfields: int):
+    def __init__(self, grid: Optional[CartesianGrid], handle: C.c_void_p, nfields: int, ctx=None):
         self.grid = grid
-        self.ctx = grid.ctx
-        self._lib = grid._lib
+        self.ctx = grid.ctx if grid is not None else ctx
+        self._lib = self.ctx._lib
         self.handle = handle
         self.nfields = nfields
 
@@ -405,3 +407,114 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+@dataclasses.dataclass(frozen=True)
+class RegridPolicy:
+    """BergerOliger + GradientTagger knobs (src/geometry/amr/adapt.rs:55-129)."""
+
+    every: int = 4
+    buffer: int = 2
+    efficiency: float = 0.8
+    min_width: int = 4
+    threshold: float = 0.1
+    tag_field: int = 0
+
+    def c(self) -> _abi.RegridPolicyC:
+        return _abi.RegridPolicyC(self.every, self.buffer, self.efficiency, self.min_width,
+                                  self.threshold, self.tag_field)
+
+
+class AmrGrid:
+    """AmrGrid<2> (src/geometry/amr/grid.rs) on the device."""
+
+    def __init__(self, ctx: Context, handle, owned: bool = True):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.handle = handle
+        self.owned = owned
+
+    @classmethod
+    def from_patches(cls, ctx: Context, cells, block_cells, origin, spacing, ratios, levels):
+        """levels[l] = list of ((lo0, lo1), (hi0, hi1)) boxes of level l+1."""
+        d = _abi.CartesianDesc()
+        d.dim = 2
+        for i in range(2):
+            d.cells[i], d.block_cells[i] = int(cells[i]), int(block_cells[i])
+            d.origin[i], d.spacing[i] = float(origin[i]), float(spacing[i])
+        boxes = [v for l, bl in enumerate(levels) for (lo, hi) in bl for v in (l + 1, lo[0], lo[1], hi[0], hi[1])]
+        r = (C.c_uint32 * max(1, len(ratios)))(*ratios)
+        b = (C.c_int64 * max(1, len(boxes)))(*boxes)
+        h = C.c_void_p()
+        _abi.check(ctx._lib.lgd_amr_create(ctx.handle, C.byref(d), r, len(ratios), b, len(boxes) // 5,
+                                           C.byref(h)), ctx.handle)
+        return cls(ctx, h)
+
+    def __del__(self):
+        try:
+            if self.owned and self.handle:
+                self._lib.lgd_amr_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    @property
+    def num_blocks(self) -> int:
+        return int(self._lib.lgd_amr_num_blocks(self.handle))
+
+    @property
+    def num_levels(self) -> int:
+        return int(self._lib.lgd_amr_num_levels(self.handle))
+
+    @property
+    def field_len(self) -> int:
+        return int(self._lib.lgd_amr_field_len(self.handle))
+
+    def patch(self, b: int):
+        out = (C.c_int64 * 5)()
+        _abi.check(self._lib.lgd_amr_patch(self.handle, b, out))
+        return int(out[0]), (int(out[1]), int(out[2])), (int(out[3]), int(out[4]))
+
+    def offset(self, b: int) -> int:
+        return int(self._lib.lgd_amr_block_offset(self.handle, b))
+
+    def find_patch(self, level: int, cell) -> Optional[int]:
+        r = int(self._lib.lgd_amr_find_patch(self.handle, level, cell[0], cell[1]))
+        return None if r < 0 else r
+
+    def alloc_field(self) -> DeviceBuffer:
+        return DeviceBuffer(self.ctx, self.field_len)
+
+    def restrict_level(self, field: DeviceBuffer, fine_level: int) -> None:
+        _abi.check(self._lib.lgd_amr_restrict_level(self.ctx.handle, self.handle, field.handle, fine_level),
+                   self.ctx.handle)
+
+    def fill_level(self, field: DeviceBuffer, level: int, old: Optional[DeviceBuffer] = None,
+                   alpha: float = 0.0) -> None:
+        _abi.check(self._lib.lgd_amr_fill_level(self.ctx.handle, self.handle, field.handle,
+                                                old.handle if old is not None else None, alpha, level),
+                   self.ctx.handle)
+
+
+def amr_simulation(amr: AmrGrid, model: str = "heat", params: Optional[ModelCParams] = None,
+                   kappa: float = 0.0, wiener: bool = False, seed: int = 0, subcycle: bool = False,
+                   policy: Optional[RegridPolicy] = None) -> Simulation:
+    """Simulation::adaptive on an AmrGrid (takes ownership of `amr`)."""
+    h = C.c_void_p()
+    p = params.c() if params is not None else None
+    pol = policy.c() if policy is not None else None
+    _abi.check(amr._lib.lgd_sim_create_amr(amr.ctx.handle, amr.handle, {"heat": 0, "modelc": 1}[model],
+                                           C.byref(p) if p is not None else None, kappa, int(wiener), seed,
+                                           int(subcycle), C.byref(pol) if pol is not None else None,
+                                           C.byref(h)), amr.ctx.handle)
+    amr.owned = False
+    sim = Simulation(None, h, 2 if model == "modelc" else 1, ctx=amr.ctx)
+    return sim
+
+
+def _sim_amr(self) -> AmrGrid:
+    """The simulation's current hierarchy (replaced by a regrid)."""
+    return AmrGrid(self.ctx, C.c_void_p(self._lib.lgd_sim_amr(self.handle)), owned=False)
+
+
+Simulation.amr = property(_sim_amr)

@@ -53,6 +53,17 @@ class ModelCParamsC(C.Structure):
     ]
 
 
+class RegridPolicyC(C.Structure):
+    _fields_ = [
+        ("every", C.c_int64),
+        ("buffer", C.c_int64),
+        ("efficiency", C.c_double),
+        ("min_width", C.c_int64),
+        ("threshold", C.c_double),
+        ("tag_field", C.c_int32),
+    ]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 I32, I64, U32, U64, SZ, DBL = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_size_t, C.c_double
@@ -91,6 +102,19 @@ SIGNATURES = {
     "lgd_apply_wiener": (C.c_int, [P, P, P, P, U32, DBL, U64, U64, U64]),
     "lgd_modelc_step": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P, P, P, P, DBL, U64, U64, I32]),
     "lgd_heat_step": (C.c_int, [P, P, DBL, P, P, DBL]),
+    "lgd_amr_create": (C.c_int, [P, C.POINTER(CartesianDesc), C.POINTER(U32), SZ, C.POINTER(I64), SZ, PP]),
+    "lgd_amr_destroy": (C.c_int, [P]),
+    "lgd_amr_num_blocks": (I64, [P]),
+    "lgd_amr_num_levels": (I32, [P]),
+    "lgd_amr_patch": (C.c_int, [P, I64, C.POINTER(I64)]),
+    "lgd_amr_field_len": (I64, [P]),
+    "lgd_amr_block_offset": (I64, [P, I64]),
+    "lgd_amr_find_patch": (I64, [P, I32, I64, I64]),
+    "lgd_amr_restrict_level": (C.c_int, [P, P, P, I32]),
+    "lgd_amr_fill_level": (C.c_int, [P, P, P, P, DBL, I32]),
+    "lgd_sim_create_amr": (C.c_int, [P, P, I32, C.POINTER(ModelCParamsC), DBL, I32, U64, I32,
+                                     C.POINTER(RegridPolicyC), PP]),
+    "lgd_sim_amr": (P, [P]),
     "lgd_sim_create_modelc": (C.c_int, [P, P, C.POINTER(ModelCParamsC), I32, U64, I32, PP]),
     "lgd_sim_create_heat": (C.c_int, [P, P, DBL, C.POINTER(FaceBcC), I32, PP]),
     "lgd_sim_destroy": (C.c_int, [P]),

@@ -1,0 +1,579 @@
+// Block-structured AMR on the device (2-D): the patch hierarchy as a device
+// block table, and the Berger–Oliger grid transfers as gather kernels over
+// host-built plans.
+//
+// Host side (construction / regrid cadence, integer work): AmrGrid validation
+// (src/geometry/amr/grid.rs:90-196), the per-level ownership maps behind
+// find_patch, and the transfer plans:
+//   * fill plan per level — one op per ghost-ring cell of every patch: copy one
+//     interior cell (same-level exchange, mirror) or prolong bilinearly from the
+//     coarser level's interiors (corner cells + weights), the FillPlan of
+//     src/geometry/amr/intergrid.rs:225-351;
+//   * restriction items per fine level — one per covered coarse cell
+//     (intergrid.rs:110-150);
+//   * tiles per level — (block, 64×4 cell tile) work items for the level kernels.
+// Device side (every substep): the ops are applied by one launch per field and
+// level; sources are always interiors and targets always ghosts (fill) or
+// covered coarse interiors (restriction), so one launch has no write conflicts
+// and visit order cannot change results (intergrid.rs:23-26).
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <new>
+#include <vector>
+
+#include "lgd_amr.h"
+
+using namespace lgd;
+
+namespace {
+
+constexpr int kTX = 64;
+constexpr int kTY = 4;
+
+inline int64_t reflect_coord(int64_t c, int64_t lo, int64_t hi) {
+  if (c < lo) return 2 * lo - 1 - c;
+  if (c >= hi) return 2 * hi - 1 - c;
+  return c;
+}
+
+inline int64_t floor_div(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  return (a % b < 0) ? q - 1 : q;
+}
+
+template <class T>
+int upload_vec(lgd_ctx *ctx, const std::vector<T> &v, T **dptr) {
+  *dptr = nullptr;
+  if (v.empty()) return LGD_OK;
+  hipError_t e = hipMalloc((void **)dptr, v.size() * sizeof(T));
+  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(amr plan)");
+  e = hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
+  if (e != hipSuccess) return hip_fail(ctx, e, "upload(amr plan)");
+  return LGD_OK;
+}
+
+// Whether every cell of `need` lies in a box of `cover` (run-skipping walk
+// along axis 0).
+bool covered(const lgd_box &need, const std::vector<lgd_box> &cover) {
+  if (need.cells() == 0) return true;
+  int64_t x = need.lo[0], y = need.lo[1];
+  while (true) {
+    const lgd_box *hit = nullptr;
+    for (const lgd_box &b : cover)
+      if (b.contains(x, y)) {
+        hit = &b;
+        break;
+      }
+    if (!hit) return false;
+    x = hit->hi[0];
+    if (x >= need.hi[0]) {
+      x = need.lo[0];
+      if (++y >= need.hi[1]) return true;
+    }
+  }
+}
+
+}  // namespace
+
+lgd_box lgd_amr::domain(int l) const {
+  int64_t s = 1;
+  for (int i = 0; i < l; ++i) s *= (int64_t)ratios[i];
+  return lgd_box{{0, 0}, {cells[0] * s, cells[1] * s}};
+}
+
+int64_t lgd_amr::find_patch(int l, int64_t x, int64_t y) const {
+  if (l < 0 || l >= num_levels()) return -1;
+  lgd_box d = domain(l);
+  if (!d.contains(x, y)) return -1;
+  uint32_t p = lookup[l][(size_t)(x + y * d.hi[0])];
+  return p == UINT32_MAX ? -1 : (int64_t)p;
+}
+
+int64_t lgd_amr::index(int64_t b, int64_t i, int64_t j) const {
+  int64_t sx = box[b].extent(0) + 2;
+  return off[b] + (j + 1) * sx + (i + 1);
+}
+
+void lgd_amr::free_device() {
+  (void)hipFree(d_blocks);
+  d_blocks = nullptr;
+  for (auto &p : plans) {
+    (void)hipFree(p.tiles);
+    (void)hipFree(p.ops);
+    (void)hipFree(p.corners);
+    (void)hipFree(p.ritems);
+  }
+  plans.clear();
+}
+
+// AmrGrid::from_patches (src/geometry/amr/grid.rs:90-196): level 0 is the base
+// tiling; finer boxes are validated (non-empty, inside the domain, aligned to
+// the ratio, disjoint, properly nested) and appended level-major.
+int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t> &ratios,
+                  const std::vector<std::vector<lgd_box>> &levels, lgd_amr &g) {
+  if (base.dim != 2) return fail(ctx, LGD_ERR_UNSUPPORTED, "AMR hierarchies are 2-D");
+  if (levels.size() > ratios.size())
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "more refined levels than ratios");
+  for (uint32_t r : ratios)
+    if (r < 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "refinement ratio below 2");
+  g.ctx = ctx;
+  g.base = base;
+  g.cells[0] = base.n[0] * base.nb[0];
+  g.cells[1] = base.n[1] * base.nb[1];
+  g.ratios = ratios;
+  g.level.clear();
+  g.box.clear();
+  for (int64_t by = 0; by < base.nb[1]; ++by)
+    for (int64_t bx = 0; bx < base.nb[0]; ++bx) {
+      g.level.push_back(0);
+      g.box.push_back(lgd_box{{bx * base.n[0], by * base.n[1]},
+                              {(bx + 1) * base.n[0], (by + 1) * base.n[1]}});
+    }
+  g.level_start = {0, (uint32_t)g.box.size()};
+  std::vector<lgd_box> coarser(g.box);
+  for (size_t l = 0; l < levels.size(); ++l) {
+    const int lev = (int)l + 1;
+    const int64_t r = ratios[l];
+    const lgd_box dom = g.domain(lev), cdom = g.domain(lev - 1);
+    const std::vector<lgd_box> &boxes = levels[l];
+    for (size_t i = 0; i < boxes.size(); ++i) {
+      const lgd_box &b = boxes[i];
+      char m[160];
+      if (b.cells() == 0) {
+        snprintf(m, sizeof m, "empty patch at level %d", lev);
+        return fail(ctx, LGD_ERR_INVALID_ARGUMENT, m);
+      }
+      if (b.lo[0] < dom.lo[0] || b.lo[1] < dom.lo[1] || b.hi[0] > dom.hi[0] || b.hi[1] > dom.hi[1]) {
+        snprintf(m, sizeof m, "patch outside the level-%d domain", lev);
+        return fail(ctx, LGD_ERR_INVALID_ARGUMENT, m);
+      }
+      for (int d = 0; d < 2; ++d)
+        if (b.lo[d] % r != 0 || b.hi[d] % r != 0) {
+          snprintf(m, sizeof m, "patch not aligned to the refinement ratio at level %d", lev);
+          return fail(ctx, LGD_ERR_INVALID_ARGUMENT, m);
+        }
+      for (size_t k = 0; k < i; ++k)
+        if (b.intersects(boxes[k])) {
+          snprintf(m, sizeof m, "overlapping patches at level %d", lev);
+          return fail(ctx, LGD_ERR_INVALID_ARGUMENT, m);
+        }
+      lgd_box need{{floor_div(b.lo[0], r) - 1, floor_div(b.lo[1], r) - 1},
+                   {floor_div(b.hi[0] + r - 1, r) + 1, floor_div(b.hi[1] + r - 1, r) + 1}};
+      need = need.clipped(cdom);
+      if (!covered(need, coarser)) {
+        snprintf(m, sizeof m, "level-%d patch is not properly nested in level %d", lev, lev - 1);
+        return fail(ctx, LGD_ERR_INVALID_ARGUMENT, m);
+      }
+    }
+    for (const lgd_box &b : boxes) {
+      g.level.push_back(lev);
+      g.box.push_back(b);
+    }
+    g.level_start.push_back((uint32_t)g.box.size());
+    coarser = boxes;
+  }
+  g.spacing.assign(1, {base.h[0], base.h[1]});
+  for (uint32_t r : ratios) {
+    auto p = g.spacing.back();
+    g.spacing.push_back({p[0] / (double)r, p[1] / (double)r});
+  }
+  g.lookup.clear();
+  for (int l = 0; l < g.num_levels(); ++l) {
+    lgd_box d = g.domain(l);
+    std::vector<uint32_t> map((size_t)(d.hi[0] * d.hi[1]), UINT32_MAX);
+    for (uint32_t p = g.level_start[l]; p < g.level_start[l + 1]; ++p)
+      for (int64_t y = g.box[p].lo[1]; y < g.box[p].hi[1]; ++y)
+        std::fill(map.begin() + (size_t)(g.box[p].lo[0] + y * d.hi[0]),
+                  map.begin() + (size_t)(g.box[p].hi[0] + y * d.hi[0]), p);
+    g.lookup.push_back(std::move(map));
+  }
+  g.off.assign(g.box.size() + 1, 0);
+  for (size_t b = 0; b < g.box.size(); ++b)
+    g.off[b + 1] = g.off[b] + (g.box[b].extent(0) + 2) * (g.box[b].extent(1) + 2);
+  return lgd_amr_build_plans(g);
+}
+
+// Bilinear prolongation stencil of fine cell (x, y) at `level` from level-1
+// interiors (prolong_corners, intergrid.rs:307-351): corners in bitmask order,
+// zero weights skipped, corners outside the domain mirrored.
+static void prolong_stencil(const lgd_amr &g, int level, int64_t x, int64_t y,
+                            std::vector<lgd_corner> &out) {
+  const double r = (double)g.ratios[level - 1];
+  const lgd_box cd = g.domain(level - 1);
+  int64_t c0[2];
+  double w[2];
+  const int64_t cell[2] = {x, y};
+  for (int d = 0; d < 2; ++d) {
+    double pc = ((double)cell[d] + 0.5) / r - 0.5;
+    double f = std::floor(pc);
+    c0[d] = (int64_t)f;
+    w[d] = pc - f;
+  }
+  for (unsigned corner = 0; corner < 4; ++corner) {
+    int64_t cc[2] = {c0[0], c0[1]};
+    double weight = 1.0;
+    for (int d = 0; d < 2; ++d) {
+      if ((corner >> d) & 1u) {
+        cc[d] += 1;
+        weight *= w[d];
+      } else {
+        weight *= 1.0 - w[d];
+      }
+    }
+    if (weight == 0.0) continue;
+    for (int d = 0; d < 2; ++d) cc[d] = reflect_coord(cc[d], cd.lo[d], cd.hi[d]);
+    int64_t q = g.find_patch(level - 1, cc[0], cc[1]);
+    lgd_corner c;
+    c.src = q < 0 ? -1 : g.index(q, cc[0] - g.box[q].lo[0], cc[1] - g.box[q].lo[1]);
+    c.weight = weight;
+    out.push_back(c);
+  }
+}
+
+int lgd_amr_build_plans(lgd_amr &g) {
+  lgd_ctx *ctx = g.ctx;
+  g.free_device();
+  std::vector<lgd_block_desc> blocks(g.box.size());
+  for (size_t b = 0; b < g.box.size(); ++b) {
+    blocks[b].nx = (int32_t)g.box[b].extent(0);
+    blocks[b].ny = (int32_t)g.box[b].extent(1);
+    blocks[b].level = g.level[b];
+    blocks[b].off = g.off[b];
+  }
+  int rc = upload_vec(ctx, blocks, &g.d_blocks);
+  if (rc) return rc;
+  g.plans.assign((size_t)g.num_levels(), lgd_level_plan{});
+  for (int l = 0; l < g.num_levels(); ++l) {
+    lgd_level_plan &P = g.plans[(size_t)l];
+    // Level kernels: one 64×4 tile of cells per workgroup.
+    std::vector<lgd_tile> tiles;
+    for (uint32_t b = g.level_start[l]; b < g.level_start[l + 1]; ++b)
+      for (int64_t y0 = 0; y0 < g.box[b].extent(1); y0 += kTY)
+        for (int64_t x0 = 0; x0 < g.box[b].extent(0); x0 += kTX)
+          tiles.push_back(lgd_tile{(int32_t)b, (int32_t)x0, (int32_t)y0, 0});
+    // Fill plan: every ghost-ring cell (ghost width 1) of every patch.
+    std::vector<lgd_fill_op> ops;
+    std::vector<lgd_corner> corners;
+    const lgd_box dom = g.domain(l);
+    for (uint32_t b = g.level_start[l]; b < g.level_start[l + 1]; ++b) {
+      const lgd_box &p = g.box[b];
+      auto visit = [&](int64_t x, int64_t y) {
+        int64_t tx = reflect_coord(x, dom.lo[0], dom.hi[0]);
+        int64_t ty = reflect_coord(y, dom.lo[1], dom.hi[1]);
+        lgd_fill_op op;
+        op.target = g.index(b, x - p.lo[0], y - p.lo[1]);
+        int64_t src = g.find_patch(l, tx, ty);
+        if (src >= 0) {
+          op.src = g.index(src, tx - g.box[src].lo[0], ty - g.box[src].lo[1]);
+          op.cstart = 0;
+          op.clen = 0;
+        } else {
+          op.src = -1;
+          op.cstart = (int32_t)corners.size();
+          prolong_stencil(g, l, tx, ty, corners);
+          op.clen = (int32_t)(corners.size() - (size_t)op.cstart);
+        }
+        ops.push_back(op);
+      };
+      for (int64_t x = p.lo[0] - 1; x < p.hi[0] + 1; ++x) {
+        visit(x, p.lo[1] - 1);
+        visit(x, p.hi[1]);
+      }
+      for (int64_t y = p.lo[1]; y < p.hi[1]; ++y) {
+        visit(p.lo[0] - 1, y);
+        visit(p.hi[0], y);
+      }
+    }
+    for (const lgd_corner &c : corners)
+      if (c.src < 0) {
+        return fail(ctx, LGD_ERR_INVALID_ARGUMENT,
+                    "no coarse donor for a prolongation corner (hierarchy not properly nested)");
+      }
+    for (const lgd_fill_op &op : ops)
+      if (op.src < 0 && op.clen == 0)
+        return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "level-0 ghost without a same-level donor");
+    // Restriction of this level onto the coarser one.
+    std::vector<lgd_restrict_item> items;
+    if (l >= 1) {
+      const int64_t r = g.ratios[l - 1];
+      for (uint32_t fb = g.level_start[l]; fb < g.level_start[l + 1]; ++fb) {
+        const lgd_box &fp = g.box[fb];
+        lgd_box cov{{fp.lo[0] / r, fp.lo[1] / r}, {fp.hi[0] / r, fp.hi[1] / r}};
+        for (uint32_t cb = g.level_start[l - 1]; cb < g.level_start[l]; ++cb) {
+          const lgd_box &cp = g.box[cb];
+          lgd_box ov = cov.clipped(cp);
+          for (int64_t cy = ov.lo[1]; cy < ov.hi[1]; ++cy)
+            for (int64_t cx = ov.lo[0]; cx < ov.hi[0]; ++cx) {
+              lgd_restrict_item it;
+              it.target = g.index(cb, cx - cp.lo[0], cy - cp.lo[1]);
+              it.child = g.index(fb, cx * r - fp.lo[0], cy * r - fp.lo[1]);
+              it.stride = fp.extent(0) + 2;
+              items.push_back(it);
+            }
+        }
+      }
+    }
+    P.ntiles = (int64_t)tiles.size();
+    P.nops = (int64_t)ops.size();
+    P.ncorners = (int64_t)corners.size();
+    P.nritems = (int64_t)items.size();
+    if ((rc = upload_vec(ctx, tiles, &P.tiles))) return rc;
+    if ((rc = upload_vec(ctx, ops, &P.ops))) return rc;
+    if ((rc = upload_vec(ctx, corners, &P.corners))) return rc;
+    if ((rc = upload_vec(ctx, items, &P.ritems))) return rc;
+  }
+  return LGD_OK;
+}
+
+// ------------------------------------------------------------------ kernels
+namespace {
+
+// Apply one level's fill plan (fill_level, intergrid.rs:183-223): copy or
+// prolong (acc from 0 in corner order), optionally blended in time between the
+// parent's snapshot `old` and its current state: old·(1−α) + new·α.
+template <bool kTime>
+__global__ void amr_fill_kernel(const lgd_fill_op *__restrict__ ops, int64_t nops,
+                                const lgd_corner *__restrict__ corners, double *field,
+                                const double *old, double one_minus_alpha, double alpha) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nops) return;
+  const lgd_fill_op op = ops[t];
+  double v;
+  if (op.clen == 0) {
+    v = field[op.src];
+  } else {
+    double vn = 0.0, vo = 0.0;
+    for (int c = 0; c < op.clen; ++c) {
+      const lgd_corner k = corners[op.cstart + c];
+      vn = vn + field[k.src] * k.weight;
+      if (kTime) vo = vo + old[k.src] * k.weight;
+    }
+    v = kTime ? vo * one_minus_alpha + vn * alpha : vn;
+  }
+  field[op.target] = v;
+}
+
+// restrict_level (intergrid.rs:110-150): coarse = (Σ children, dimension 0
+// fastest, from 0) · 1/r².
+__global__ void amr_restrict_kernel(const lgd_restrict_item *__restrict__ items, int64_t n,
+                                    int r, double inv, double *field) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const lgd_restrict_item it = items[t];
+  double sum = 0.0;
+  for (int ky = 0; ky < r; ++ky)
+    for (int kx = 0; kx < r; ++kx) sum = sum + field[it.child + ky * it.stride + kx];
+  field[it.target] = sum * inv;
+}
+
+// Model C vector field (Time) on every cell of one level's patches, from
+// ghost-filled slabs (ModelC::vector_field_block on AmrGrid, phasefield.rs:452-475).
+__global__ __launch_bounds__(kTX *kTY) void amr_modelc_rhs_kernel(
+    const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks,
+    lgd_kr_consts k, const double *phi, const double *u, double *kphi, double *ku) {
+  const lgd_tile tl = tiles[blockIdx.x];
+  const lgd_block_desc bd = blocks[tl.block];
+  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
+  if (i >= bd.nx || j >= bd.ny) return;
+  const int64_t sx = bd.nx + 2;
+  const int64_t o = bd.off + (int64_t)(j + 1) * sx + (i + 1);
+  double p[3][3];
+  for (int dj = -1; dj <= 1; ++dj)
+    for (int di = -1; di <= 1; ++di) p[dj + 1][di + 1] = phi[o + dj * sx + di];
+  double dphi, du;
+  lgd_modelc_cell(k, p, u[o], u[o + 1], u[o - 1], u[o + sx], u[o - sx], dphi, du);
+  kphi[o] = dphi;
+  ku[o] = du;
+}
+
+// Heat vector field κ·∇²u on one level's patches (tests/amr.rs Heat model).
+__global__ __launch_bounds__(kTX *kTY) void amr_heat_rhs_kernel(
+    const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks, double ihx,
+    double ihy, double kappa, const double *u, double *k) {
+  const lgd_tile tl = tiles[blockIdx.x];
+  const lgd_block_desc bd = blocks[tl.block];
+  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
+  if (i >= bd.nx || j >= bd.ny) return;
+  const int64_t sx = bd.nx + 2;
+  const int64_t o = bd.off + (int64_t)(j + 1) * sx + (i + 1);
+  const double c = u[o];
+  double acc = 0.0;
+  acc = acc + (((u[o + 1] - (c + c)) + u[o - 1]) * ihx);
+  acc = acc + (((u[o + sx] - (c + c)) + u[o - sx]) * ihy);
+  k[o] = acc * kappa;
+}
+
+// Wiener(0) increment on the interior cells of one level's patches
+// (src/core/driver.rs:139-158; cell key per patch extent, grid.rs:407-425).
+__global__ __launch_bounds__(kTX *kTY) void amr_wiener_kernel(
+    const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks,
+    double noise_scale, uint64_t seed, uint64_t salt, uint64_t stream, double *phi) {
+  const lgd_tile tl = tiles[blockIdx.x];
+  const lgd_block_desc bd = blocks[tl.block];
+  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
+  if (i >= bd.nx || j >= bd.ny) return;
+  const int64_t o = bd.off + (int64_t)(j + 1) * (bd.nx + 2) + (i + 1);
+  uint64_t bk = lgd_mix_key3(seed, salt, stream, (uint64_t)tl.block);
+  uint64_t key = lgd_splitmix64(bk ^ (uint64_t)(i + (int64_t)j * bd.nx));
+  phi[o] = phi[o] + noise_scale * lgd_standard_normal(key);
+}
+
+__global__ void amr_axpy_kernel(int64_t n, double alpha, double *x, const double *y) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  x[t] = x[t] + alpha * y[t];
+}
+
+}  // namespace
+
+int lgd_amr_fill(lgd_ctx *ctx, const lgd_amr &g, double *field, const double *old, double alpha,
+                 int level) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  if (P.nops == 0) return LGD_OK;
+  unsigned nb = blocks_for(P.nops, 256);
+  if (old)
+    amr_fill_kernel<true><<<nb, 256, 0, ctx->stream>>>(P.ops, P.nops, P.corners, field, old,
+                                                      1.0 - alpha, alpha);
+  else
+    amr_fill_kernel<false><<<nb, 256, 0, ctx->stream>>>(P.ops, P.nops, P.corners, field, nullptr,
+                                                       0.0, 0.0);
+  return launch_check(ctx, "amr_fill");
+}
+
+int lgd_amr_restrict(lgd_ctx *ctx, const lgd_amr &g, double *field, int fine_level) {
+  const lgd_level_plan &P = g.plans[(size_t)fine_level];
+  if (P.nritems == 0) return LGD_OK;
+  int r = (int)g.ratios[fine_level - 1];
+  double inv = 1.0 / std::pow((double)r, 2);
+  amr_restrict_kernel<<<blocks_for(P.nritems, 256), 256, 0, ctx->stream>>>(P.ritems, P.nritems, r,
+                                                                         inv, field);
+  return launch_check(ctx, "amr_restrict");
+}
+
+int lgd_amr_modelc_rhs(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
+                       const double *phi, const double *u, double *kphi, double *ku) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  if (P.ntiles == 0) return LGD_OK;
+  lgd_cart c = g.base;
+  c.h[0] = g.spacing[(size_t)level][0];
+  c.h[1] = g.spacing[(size_t)level][1];
+  lgd_kr_consts k = lgd_make_kr_consts(p, c);
+  amr_modelc_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
+      P.tiles, g.d_blocks, k, phi, u, kphi, ku);
+  return launch_check(ctx, "amr_modelc_rhs");
+}
+
+int lgd_amr_heat_rhs(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, const double *u,
+                     double *k) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  if (P.ntiles == 0) return LGD_OK;
+  double hx = g.spacing[(size_t)level][0], hy = g.spacing[(size_t)level][1];
+  amr_heat_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
+      P.tiles, g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, u, k);
+  return launch_check(ctx, "amr_heat_rhs");
+}
+
+int lgd_amr_apply(lgd_ctx *ctx, const lgd_amr &g, int level, double *x, const double *k, double dt) {
+  int64_t lo = g.off[g.level_start[level]], hi = g.off[g.level_start[level + 1]];
+  if (hi <= lo) return LGD_OK;
+  amr_axpy_kernel<<<blocks_for(hi - lo, 256), 256, 0, ctx->stream>>>(hi - lo, dt, x + lo, k + lo);
+  return launch_check(ctx, "amr_apply");
+}
+
+int lgd_amr_wiener(lgd_ctx *ctx, const lgd_amr &g, int level, double *phi, double noise_scale,
+                   uint64_t seed, uint64_t salt) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  if (P.ntiles == 0) return LGD_OK;
+  amr_wiener_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
+      P.tiles, g.d_blocks, noise_scale, seed, salt, (1ull << 32) | 0ull, phi);
+  return launch_check(ctx, "amr_wiener");
+}
+
+// ------------------------------------------------------------------- C ABI
+extern "C" {
+
+int lgd_amr_create(lgd_ctx *ctx, const lgd_cartesian_desc *base, const uint32_t *ratios,
+                   size_t nratios, const int64_t *boxes, size_t nboxes, lgd_amr **out) {
+  if (!ctx || !base || !out || (!ratios && nratios) || (!boxes && nboxes))
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  lgd_grid *bg = nullptr;
+  int rc = lgd_grid_create_cartesian(ctx, base, &bg);
+  if (rc) return rc;
+  lgd_cart c = bg->cart;
+  lgd_grid_destroy(bg);
+  std::vector<uint32_t> r(ratios, ratios + nratios);
+  std::vector<std::vector<lgd_box>> levels;
+  for (size_t i = 0; i < nboxes; ++i) {
+    int64_t lev = boxes[5 * i];
+    if (lev < 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "patch levels start at 1");
+    while ((int64_t)levels.size() < lev) levels.emplace_back();
+    levels[(size_t)lev - 1].push_back(
+        lgd_box{{boxes[5 * i + 1], boxes[5 * i + 2]}, {boxes[5 * i + 3], boxes[5 * i + 4]}});
+  }
+  lgd_amr *g = new (std::nothrow) lgd_amr();
+  if (!g) return fail(ctx, LGD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  rc = lgd_amr_build(ctx, c, r, levels, *g);
+  if (rc) {
+    g->free_device();
+    delete g;
+    return rc;
+  }
+  *out = g;
+  return LGD_OK;
+}
+
+int lgd_amr_destroy(lgd_amr *g) {
+  if (!g) return LGD_OK;
+  if (g->ctx) (void)hipStreamSynchronize(g->ctx->stream);
+  g->free_device();
+  delete g;
+  return LGD_OK;
+}
+
+int64_t lgd_amr_num_blocks(const lgd_amr *g) { return g ? (int64_t)g->box.size() : -1; }
+int32_t lgd_amr_num_levels(const lgd_amr *g) { return g ? g->num_levels() : -1; }
+
+int lgd_amr_patch(const lgd_amr *g, int64_t block, int64_t *out5) {
+  if (!g || !out5 || block < 0 || block >= (int64_t)g->box.size()) return LGD_ERR_INVALID_ARGUMENT;
+  out5[0] = g->level[(size_t)block];
+  out5[1] = g->box[(size_t)block].lo[0];
+  out5[2] = g->box[(size_t)block].lo[1];
+  out5[3] = g->box[(size_t)block].hi[0];
+  out5[4] = g->box[(size_t)block].hi[1];
+  return LGD_OK;
+}
+
+int64_t lgd_amr_field_len(const lgd_amr *g) { return g ? g->off.back() : -1; }
+
+int64_t lgd_amr_block_offset(const lgd_amr *g, int64_t block) {
+  if (!g || block < 0 || block > (int64_t)g->box.size()) return -1;
+  return g->off[(size_t)block];
+}
+
+int64_t lgd_amr_find_patch(const lgd_amr *g, int32_t level, int64_t x, int64_t y) {
+  return g ? g->find_patch(level, x, y) : -1;
+}
+
+int lgd_amr_restrict_level(lgd_ctx *ctx, const lgd_amr *g, lgd_buf *field, int32_t fine_level) {
+  if (!ctx || !g || !field) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  if (fine_level < 1 || fine_level >= g->num_levels())
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "restriction needs an existing finer level >= 1");
+  if ((int64_t)field->len != g->off.back())
+    return fail(ctx, LGD_ERR_SIZE_MISMATCH, "field is not congruent with the hierarchy");
+  return lgd_amr_restrict(ctx, *g, field->data, fine_level);
+}
+
+int lgd_amr_fill_level(lgd_ctx *ctx, const lgd_amr *g, lgd_buf *field, const lgd_buf *old,
+                       double alpha, int32_t level) {
+  if (!ctx || !g || !field) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  if (level < 0 || level >= g->num_levels())
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "level out of range");
+  if ((int64_t)field->len != g->off.back() || (old && old->len != field->len))
+    return fail(ctx, LGD_ERR_SIZE_MISMATCH, "field is not congruent with the hierarchy");
+  return lgd_amr_fill(ctx, *g, field->data, old ? old->data : nullptr, alpha, level);
+}
+
+}  // extern "C"

@@ -11,7 +11,7 @@
 #include <new>
 #include <vector>
 
-#include "lgd_internal.h"
+#include "lgd_regrid.h"
 
 struct lgd_sim {
   lgd_ctx *ctx = nullptr;
@@ -28,6 +28,12 @@ struct lgd_sim {
   std::vector<lgd_buf *> next;   // fused mode: ping-pong partner
   std::vector<lgd_buf *> k;      // per-intent mode: Tendency(Time) buffer
   lgd_buf *amp = nullptr;        // per-intent mode: Tendency(Wiener(0)) buffer (phi only)
+  // AMR hierarchy (owned) — set for simulations on an AmrGrid.
+  lgd_amr *amr = nullptr;
+  int32_t subcycle = 0;
+  bool adaptive = false;
+  lgd_regrid_policy policy{};
+  std::vector<lgd_buf *> snaps;  // subcycling: per (level, field) time-interpolation snapshot
   double t = 0.0;
   int64_t step_index = 0;
 };
@@ -36,15 +42,21 @@ using namespace lgd;
 
 namespace {
 
-void free_all(lgd_sim *s) {
-  for (auto *b : s->state) lgd_buf_free(b);
+void free_stages(lgd_sim *s) {
   for (auto *b : s->next) lgd_buf_free(b);
   for (auto *b : s->k) lgd_buf_free(b);
+  for (auto *b : s->snaps) lgd_buf_free(b);
   lgd_buf_free(s->amp);
-  s->state.clear();
   s->next.clear();
   s->k.clear();
+  s->snaps.clear();
   s->amp = nullptr;
+}
+
+void free_all(lgd_sim *s) {
+  for (auto *b : s->state) lgd_buf_free(b);
+  s->state.clear();
+  free_stages(s);
 }
 
 int alloc_fields(lgd_sim *s, int nfields) {
@@ -124,6 +136,173 @@ inline void cell_center2(const lgd_cart &c, int64_t b, int64_t i, int64_t j, dou
   out[1] = std::fma((double)(by * c.n[1]) + (double)j + 0.5, c.h[1], c.origin[1]);
 }
 
+
+// ------------------------------------------------------------------ AMR path
+int nfields(const lgd_sim *s) { return s->kind == lgd_sim::kModelC ? 2 : 1; }
+
+// Stage buffers of a simulation on an AmrGrid: Tendency(Time) per field, the
+// Wiener(0) amplitude (Model C with noise), one snapshot per (level, field)
+// when subcycling (src/integrators/subcycling.rs:63-73).
+int alloc_amr_stages(lgd_sim *s) {
+  free_stages(s);
+  size_t len = (size_t)s->amr->off.back();
+  int nf = nfields(s), rc;
+  for (int f = 0; f < nf; ++f) {
+    lgd_buf *b = nullptr;
+    if ((rc = lgd_buf_alloc(s->ctx, len, &b))) return rc;
+    s->k.push_back(b);
+  }
+  if (s->kind == lgd_sim::kModelC && s->wiener && (rc = lgd_buf_alloc(s->ctx, len, &s->amp)))
+    return rc;
+  if (s->subcycle)
+    for (int i = 0; i < s->amr->max_levels() * nf; ++i) {
+      lgd_buf *b = nullptr;
+      if ((rc = lgd_buf_alloc(s->ctx, len, &b))) return rc;
+      s->snaps.push_back(b);
+    }
+  return LGD_OK;
+}
+
+double amr_stable_dt(const lgd_sim *s, int level) {
+  double h = std::fmin(s->amr->spacing[(size_t)level][0], s->amr->spacing[(size_t)level][1]);
+  return s->kind == lgd_sim::kModelC ? 0.2 * std::pow(h, 2) / s->params.d_thermal
+                                     : 0.25 * std::pow(h, 2) / s->kappa;
+}
+
+// Vector field + stage update on one level's patches (eval_drift_level,
+// eval_tendency_level, apply_step_level: src/integrators/mod.rs:205-240,
+// src/core/state.rs:706-753).
+int amr_level_update(lgd_sim *s, int level, double dt, uint64_t salt) {
+  lgd_ctx *ctx = s->ctx;
+  const lgd_amr &g = *s->amr;
+  int rc;
+  if (s->kind == lgd_sim::kModelC)
+    rc = lgd_amr_modelc_rhs(ctx, g, s->params, level, s->state[0]->data, s->state[1]->data,
+                            s->k[0]->data, s->k[1]->data);
+  else
+    rc = lgd_amr_heat_rhs(ctx, g, s->kappa, level, s->state[0]->data, s->k[0]->data);
+  return rc;
+}
+
+int amr_level_apply(lgd_sim *s, int level, double dt, uint64_t salt) {
+  int rc;
+  for (int f = 0; f < nfields(s); ++f)
+    if ((rc = lgd_amr_apply(s->ctx, *s->amr, level, s->state[f]->data, s->k[f]->data, dt))) return rc;
+  if (s->kind == lgd_sim::kModelC && s->wiener && s->params.noise_amplitude != 0.0) {
+    double scale = std::sqrt(dt) * s->params.noise_amplitude;
+    if ((rc = lgd_amr_wiener(s->ctx, *s->amr, level, s->state[0]->data, scale, s->seed, salt)))
+      return rc;
+  }
+  return LGD_OK;
+}
+
+// EulerMaruyama on the AmrGrid with one global dt: ModelC/Heat fill_ghosts
+// (restrict every field finest→coarsest, then fill every level), every level's
+// vector field, then every level's update.
+int amr_global_step(lgd_sim *s, double dt, uint64_t salt) {
+  const lgd_amr &g = *s->amr;
+  int rc, nf = nfields(s);
+  for (int f = 0; f < nf; ++f)
+    for (int l = g.num_levels() - 1; l >= 1; --l)
+      if ((rc = lgd_amr_restrict(s->ctx, g, s->state[f]->data, l))) return rc;
+  for (int f = 0; f < nf; ++f)
+    for (int l = 0; l < g.num_levels(); ++l)
+      if ((rc = lgd_amr_fill(s->ctx, g, s->state[f]->data, nullptr, 0.0, l))) return rc;
+  for (int l = 0; l < g.num_levels(); ++l)
+    if ((rc = amr_level_update(s, l, dt, salt))) return rc;
+  for (int l = 0; l < g.num_levels(); ++l)
+    if ((rc = amr_level_apply(s, l, dt, salt))) return rc;
+  return LGD_OK;
+}
+
+// Subcycle::advance (src/integrators/subcycling.rs:195-267), flattened into a
+// launch sequence: snapshot the level, fill its ghosts (coarse source blended in
+// time from the parent's snapshot), step it, recurse n = round(dt_l/dt_{l+1})
+// times into the finer level at t + s·dt_child (fma), then restrict it back.
+int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
+  const lgd_amr &g = *s->amr;
+  const int nf = nfields(s);
+  int rc;
+  // Only this level's cells of a snapshot are ever read (as prolongation
+  // donors of the finer level), and a level's patches are contiguous.
+  int64_t lo = g.off[g.level_start[level]], hi = g.off[g.level_start[level + 1]];
+  for (int f = 0; f < nf; ++f) {
+    hipError_t e = hipMemcpyAsync(s->snaps[(size_t)(level * nf + f)]->data + lo,
+                                  s->state[f]->data + lo, (size_t)(hi - lo) * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s->ctx->stream);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "snapshot");
+  }
+  for (int f = 0; f < nf; ++f) {
+    const double *old = level > 0 ? s->snaps[(size_t)((level - 1) * nf + f)]->data : nullptr;
+    if ((rc = lgd_amr_fill(s->ctx, g, s->state[f]->data, old, alpha, level))) return rc;
+  }
+  uint64_t salt;
+  std::memcpy(&salt, &t, sizeof salt);
+  if ((rc = amr_level_update(s, level, dt, salt))) return rc;
+  if ((rc = amr_level_apply(s, level, dt, salt))) return rc;
+  const int child = level + 1;
+  if (child < g.num_levels()) {
+    double n = std::fmax(std::round(amr_stable_dt(s, level) / amr_stable_dt(s, child)), 1.0);
+    uint64_t nn = (uint64_t)n;
+    double dt_child = dt / (double)nn;
+    for (uint64_t k = 0; k < nn; ++k)
+      if ((rc = amr_advance(s, child, std::fma((double)k, dt_child, t), dt_child,
+                            (double)k / (double)nn)))
+        return rc;
+    for (int f = 0; f < nf; ++f)
+      if ((rc = lgd_amr_restrict(s->ctx, g, s->state[f]->data, child))) return rc;
+  }
+  return LGD_OK;
+}
+
+// BergerOliger::regrid at the top of a step (src/core/simulation.rs:255-270).
+int amr_regrid(lgd_sim *s) {
+  lgd_amr &g = *s->amr;
+  const int tf = s->policy.tag_field;
+  int rc;
+  // GradientTagger::prepare: restrict + fill the tagged field (adapt.rs:72-76).
+  for (int l = g.num_levels() - 1; l >= 1; --l)
+    if ((rc = lgd_amr_restrict(s->ctx, g, s->state[(size_t)tf]->data, l))) return rc;
+  for (int l = 0; l < g.num_levels(); ++l)
+    if ((rc = lgd_amr_fill(s->ctx, g, s->state[(size_t)tf]->data, nullptr, 0.0, l))) return rc;
+  std::vector<double> tag((size_t)g.off.back());
+  if ((rc = lgd_buf_download(s->ctx, s->state[(size_t)tf], 0, tag.data(), tag.size()))) return rc;
+  std::vector<std::vector<lgd_box>> levels;
+  if (lgd_regrid_propose(g, tag, s->policy, levels) == 0) return LGD_OK;
+  lgd_amr *ng = new (std::nothrow) lgd_amr();
+  if (!ng) return fail(s->ctx, LGD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  if ((rc = lgd_amr_build(s->ctx, g.base, g.ratios, levels, *ng))) {
+    ng->free_device();
+    delete ng;
+    return rc;
+  }
+  std::vector<lgd_buf *> ns;
+  std::vector<double> old_f, new_f;
+  for (size_t f = 0; f < s->state.size(); ++f) {
+    old_f.resize((size_t)g.off.back());
+    if ((rc = lgd_buf_download(s->ctx, s->state[f], 0, old_f.data(), old_f.size()))) return rc;
+    lgd_regrid_migrate(g, old_f, *ng, new_f);
+    lgd_buf *b = nullptr;
+    if ((rc = lgd_buf_alloc(s->ctx, new_f.size(), &b))) return rc;
+    if ((rc = lgd_buf_upload(s->ctx, b, 0, new_f.data(), new_f.size()))) return rc;
+    ns.push_back(b);
+  }
+  for (auto *b : s->state) lgd_buf_free(b);
+  s->state = ns;
+  (void)hipStreamSynchronize(s->ctx->stream);
+  s->amr->free_device();
+  delete s->amr;
+  s->amr = ng;
+  return alloc_amr_stages(s);
+}
+
+int step_amr(lgd_sim *s, double dt, uint64_t salt) {
+  int rc;
+  if (s->adaptive && s->step_index % s->policy.every == 0 && (rc = amr_regrid(s))) return rc;
+  if (s->subcycle) return amr_advance(s, 0, s->t, dt, 0.0);
+  return amr_global_step(s, dt, salt);
+}
+
 }  // namespace
 
 extern "C" {
@@ -189,9 +368,56 @@ int lgd_sim_create_heat(lgd_ctx *ctx, const lgd_grid *grid, double kappa,
   return LGD_OK;
 }
 
+int lgd_sim_create_amr(lgd_ctx *ctx, lgd_amr *amr, int32_t model, const lgd_modelc_params *params,
+                       double kappa, int32_t wiener, uint64_t seed, int32_t subcycle,
+                       const lgd_regrid_policy *policy, lgd_sim **out) {
+  if (!ctx || !amr || !out) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (model != 0 && model != 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "model is 0 (heat) or 1 (Model C)");
+  if (model == 1 && !params) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "Model C needs parameters");
+  if (policy && (policy->every < 1 || policy->tag_field < 0 || policy->tag_field >= (model == 1 ? 2 : 1) ||
+                 policy->min_width < 1 || !(policy->efficiency > 0.0 && policy->efficiency <= 1.0)))
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "invalid regrid policy");
+  lgd_sim *s = new (std::nothrow) lgd_sim();
+  if (!s) return fail(ctx, LGD_ERR_OUT_OF_MEMORY, "host allocation failed");
+  s->ctx = ctx;
+  s->grid = nullptr;
+  s->kind = model == 1 ? lgd_sim::kModelC : lgd_sim::kHeat;
+  s->mode = LGD_MODE_PER_INTENT;
+  if (params) s->params = *params;
+  s->kappa = kappa;
+  s->wiener = wiener ? 1 : 0;
+  s->seed = seed;
+  s->amr = amr;
+  s->subcycle = subcycle ? 1 : 0;
+  if (policy) {
+    s->adaptive = true;
+    s->policy = *policy;
+  }
+  int rc = LGD_OK;
+  size_t len = (size_t)amr->off.back();
+  for (int f = 0; f < nfields(s) && !rc; ++f) {
+    lgd_buf *b = nullptr;
+    rc = lgd_buf_alloc(ctx, len, &b);
+    if (!rc) s->state.push_back(b);
+  }
+  if (!rc) rc = alloc_amr_stages(s);
+  if (rc) {
+    free_all(s);
+    s->amr = nullptr;  // the caller keeps the hierarchy on failure
+    delete s;
+    return rc;
+  }
+  *out = s;
+  return LGD_OK;
+}
+
+const lgd_amr *lgd_sim_amr(const lgd_sim *sim) { return sim ? sim->amr : nullptr; }
+
 int lgd_sim_destroy(lgd_sim *sim) {
   if (!sim) return LGD_OK;
   free_all(sim);
+  if (sim->amr) lgd_amr_destroy(sim->amr);
   delete sim;
   return LGD_OK;
 }
@@ -209,7 +435,9 @@ int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps) {
   for (int64_t n = 0; n < nsteps; ++n) {
     uint64_t salt;
     std::memcpy(&salt, &sim->t, sizeof salt);  // t.to_bits()
-    int rc = sim->kind == lgd_sim::kModelC ? step_modelc(sim, dt, salt) : step_heat(sim, dt);
+    int rc = sim->amr ? step_amr(sim, dt, salt)
+             : sim->kind == lgd_sim::kModelC ? step_modelc(sim, dt, salt)
+                                             : step_heat(sim, dt);
     if (rc) return rc;
     sim->t = sim->t + dt;
     sim->step_index += 1;

@@ -1,0 +1,191 @@
+"""GPU parity of the AMR path (2-D): device transfers, level kernels, global-dt and
+subcycled stepping, and adaptive regrid, against the oracle's restatement of
+src/geometry/amr and src/integrators/subcycling.rs on the same inputs."""
+from __future__ import annotations
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+lg = pytest.importorskip("legendre_amd")
+
+
+def say(msg: str) -> None:
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+HIERARCHIES = {
+    # name: (cells, block, h, ratios, levels)
+    "two_patches": ([8, 8], [4, 4], 0.5, [2], [[((4, 4), (8, 12)), ((8, 4), (12, 12))]]),
+    "three_levels": ([16, 16], [8, 8], 0.5, [2, 2],
+                     [[((4, 4), (20, 24)), ((20, 8), (28, 20))], [((12, 12), (36, 40)), ((40, 20), (52, 36))]]),
+    "edge_patch": ([12, 12], [6, 6], 0.4, [2], [[((0, 0), (10, 8)), ((14, 16), (24, 24))]]),
+}
+
+
+def pair(ctx, orc, name):
+    cells, block, h, ratios, levels = HIERARCHIES[name]
+    og = orc.AmrGrid.from_patches(orc.grid(cells, block, [0, 0], [h, h]), ratios, levels)
+    gg = lg.AmrGrid.from_patches(ctx, cells, block, [0, 0], [h, h], ratios, levels)
+    assert og.field_len(1) == gg.field_len
+    for b in range(og.num_blocks):
+        assert og.patch(b) == gg.patch(b)
+        assert og.offset(b, 1) == gg.offset(b)
+    return og, gg
+
+
+def smooth(c):
+    return math.cos(0.7 * c[0]) + math.sin(1.3 * c[1]) + 0.1 * c[0] * c[1]
+
+
+@pytest.mark.parametrize("name", list(HIERARCHIES))
+def test_fill_and_restrict_match_oracle(ctx, orc, name):
+    say(f"amr fill/restrict {name}")
+    og, gg = pair(ctx, orc, name)
+    rng = np.random.default_rng(5)
+    f = rng.standard_normal(og.field_len(1))
+    d = lg.DeviceBuffer(ctx, f.size)
+    d.upload(f)
+    for l in range(og.num_levels - 1, 0, -1):
+        og.restrict_level(f, l)
+        gg.restrict_level(d, l)
+    assert np.array_equal(d.download(), f)
+    for l in range(og.num_levels):
+        og.fill_level(f, l)
+        gg.fill_level(d, l)
+    assert np.array_equal(d.download(), f)
+    # time interpolation of the coarse source
+    old = rng.standard_normal(f.size)
+    dold = lg.DeviceBuffer(ctx, f.size)
+    dold.upload(old)
+    for l in range(1, og.num_levels):
+        og.fill_level(f, l, old=old, alpha=0.25)
+        gg.fill_level(d, l, old=dold, alpha=0.25)
+    assert np.array_equal(d.download(), f)
+
+
+def test_invalid_hierarchies_are_rejected(ctx):
+    say("amr invalid hierarchy")
+    for levels in ([[((4, 4), (8, 8)), ((6, 6), (10, 10))]], [[((3, 4), (7, 8))]], [[((4, 4), (4, 8))]]):
+        with pytest.raises(lg.LegendreError) as e:
+            lg.AmrGrid.from_patches(ctx, [8, 8], [4, 4], [0, 0], [0.5, 0.5], [2], levels)
+        assert e.value.status == 1
+
+
+@pytest.mark.parametrize("name", list(HIERARCHIES))
+@pytest.mark.parametrize("subcycle", [False, True])
+def test_heat_on_static_hierarchy_matches_oracle_bitwise(ctx, orc, name, subcycle):
+    say(f"amr heat {name} subcycle={subcycle}")
+    og, gg = pair(ctx, orc, name)
+    kappa = 0.7
+    sim_o = orc.AmrSim(og, "heat", kappa=kappa, integ="subcycle" if subcycle else "global")
+    sim_o.grid.fill_by_center(sim_o.field(0), smooth)
+    sim_g = lg.amr_simulation(gg, "heat", kappa=kappa, subcycle=subcycle)
+    sim_g.field(0).upload(sim_o.field(0))
+    h0 = HIERARCHIES[name][2]
+    nlev = len(HIERARCHIES[name][3]) + 1
+    dt = 0.25 * (h0 if subcycle else h0 / 2 ** (nlev - 1)) ** 2 / kappa
+    sim_o.step(dt, 12)
+    sim_g.step(dt, 12)
+    assert sim_g.time == sim_o.time
+    assert np.array_equal(sim_g.field(0).download(), sim_o.field(0))
+
+
+@pytest.mark.parametrize("subcycle", [False, True])
+@pytest.mark.parametrize("noise", [0.0, 0.02])
+def test_modelc_on_static_hierarchy_matches_oracle(ctx, orc, subcycle, noise):
+    say(f"amr modelc subcycle={subcycle} noise={noise}")
+    cells, block, h = [48, 48], [24, 24], 0.4
+    levels = [[((8, 8), (40, 32)), ((40, 8), (56, 24))]]
+    og = orc.AmrGrid.from_patches(orc.grid(cells, block, [0, 0], [h, h]), [2], levels)
+    gg = lg.AmrGrid.from_patches(ctx, cells, block, [0, 0], [h, h], [2], levels)
+    p = orc.params(0.05, 3.19, 2.0, noise)
+    sim_o = orc.AmrSim(og, "modelc", params=p, seed=11, wiener=noise > 0,
+                       integ="subcycle" if subcycle else "global")
+    gr = sim_o.grid
+    phi, u = sim_o.field(0), sim_o.field(1)
+    for b in range(gr.num_blocks):
+        vp, vu = gr.view(phi, b), gr.view(u, b)
+        ny, nx = vp.shape[0] - 2, vp.shape[1] - 2
+        for j in range(ny):
+            for i in range(nx):
+                x, y = gr.cell_center(b, i, j)
+                vp[j + 1, i + 1] = -math.tanh((math.hypot(x - 6.0, y - 5.0) - 3.0) / math.sqrt(2.0))
+                vu[j + 1, i + 1] = -0.55
+    sim_g = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.05, 3.19, 2.0, noise),
+                              wiener=noise > 0, seed=11, subcycle=subcycle)
+    sim_g.field(0).upload(phi)
+    sim_g.field(1).upload(u)
+    dt = 0.2 * (h if subcycle else h / 2) ** 2 / 2.0
+    sim_o.step(dt, 20)
+    sim_g.step(dt, 20)
+    gp, gu = sim_g.field(0).download(), sim_g.field(1).download()
+    if noise == 0.0:
+        assert np.array_equal(gp, sim_o.field(0)) and np.array_equal(gu, sim_o.field(1))
+    else:
+        assert np.max(np.abs(gp - sim_o.field(0))) <= 1e-10
+        assert np.max(np.abs(gu - sim_o.field(1))) <= 1e-10
+
+
+def gaussian(c):
+    dx, dy = c[0] - 4.0, c[1] - 4.0
+    return math.exp(-H._fma(dx, dx, dy * dy) / 0.4)
+
+
+@pytest.mark.parametrize("subcycle", [False, True])
+def test_adaptive_heat_matches_oracle_and_golden_numbers(ctx, orc, subcycle):
+    """The reference's adaptive test (tests/amr.rs:682-788) on the device: the
+    same regrid decisions (hierarchy after every step) and the same bits."""
+    say(f"amr adaptive heat subcycle={subcycle}")
+    kappa = 0.7
+    dt = 0.2 * 0.25 * 0.25 / kappa
+    pol = (4, 2, 0.7, 2, 0.05, 0)
+    base = orc.grid([16, 16], [8, 8], [0, 0], [0.5, 0.5])
+    sim_o = orc.AmrSim(orc.AmrGrid.from_patches(base, [2], []), "heat", kappa=kappa,
+                       integ="subcycle" if subcycle else "global", policy=orc.RegridPolicy(*pol))
+    sim_o.grid.fill_by_center(sim_o.field(0), gaussian)
+    gg = lg.AmrGrid.from_patches(ctx, [16, 16], [8, 8], [0, 0], [0.5, 0.5], [2], [])
+    sim_g = lg.amr_simulation(gg, "heat", kappa=kappa, subcycle=subcycle, policy=lg.RegridPolicy(*pol))
+    sim_g.field(0).upload(sim_o.field(0))
+    for s in range(60):
+        sim_o.step(dt)
+        sim_g.step(dt)
+        og, ga = sim_o.grid, sim_g.amr
+        assert og.num_blocks == ga.num_blocks, f"step {s}"
+        for b in range(og.num_blocks):
+            assert og.patch(b) == ga.patch(b), f"step {s} block {b}"
+    assert np.array_equal(sim_g.field(0).download(), sim_o.field(0))
+
+
+def test_adaptive_subcycled_model_c_matches_oracle_bitwise(ctx, orc):
+    """tests/amr.rs:1181-1228 on the device: adaptive, subcycled Model C."""
+    say("amr adaptive subcycled model C")
+    base = orc.grid([48, 48], [24, 24], [0, 0], [0.4, 0.4])
+    p = orc.params(0.06, 3.19, 0.6267 * 3.19, 0.0)
+    pol = (4, 2, 0.8, 4, 0.15, 0)
+    sim_o = orc.AmrSim(orc.AmrGrid.from_patches(base, [2], []), "modelc", params=p, seed=0,
+                       integ="subcycle", policy=orc.RegridPolicy(*pol))
+    phi = np.zeros(orc.field_len(base, 1))
+    u = np.zeros_like(phi)
+    orc.initialize_grains(base, phi, u, np.array([[0.4, 0.4, 4.0]]), 0.7)
+    sim_o.field(0)[:] = phi
+    sim_o.field(1)[:] = u
+    gg = lg.AmrGrid.from_patches(ctx, [48, 48], [24, 24], [0, 0], [0.4, 0.4], [2], [])
+    sim_g = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.06, 3.19, 0.6267 * 3.19, 0.0),
+                              subcycle=True, policy=lg.RegridPolicy(*pol))
+    sim_g.field(0).upload(phi)
+    sim_g.field(1).upload(u)
+    dt = 0.2 * 0.4**2 / p.d_thermal
+    sim_o.step(dt, 40)
+    sim_g.step(dt, 40)
+    assert sim_g.amr.num_levels == 2
+    assert sim_g.amr.num_blocks == sim_o.grid.num_blocks
+    assert np.array_equal(sim_g.field(0).download(), sim_o.field(0))
+    assert np.array_equal(sim_g.field(1).download(), sim_o.field(1))
