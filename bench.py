@@ -66,6 +66,16 @@ def run_gpu(args, C):
     if not np.all(np.isfinite(phi)):
         raise RuntimeError("non-finite state after the timed steps")
     sim.close()
+    if args.calibrate:
+        # A kernel of known traffic for scaling profiler byte counters: one
+        # axpy over 2^27 f64 reads 2 GiB and writes 1 GiB.
+        n = 1 << 27
+        x, y = lg.DeviceBuffer(ctx, n), lg.DeviceBuffer(ctx, n)
+        grid.axpy(x, y, 0.5)
+        ctx.sync()
+        log(f"[bench] calibration axpy: n={n}, algorithmic fetch {16 * n} B, write {8 * n} B")
+        x.free()
+        y.free()
     ctx.close()
     return total_ms, per, wall
 
@@ -124,6 +134,8 @@ def main() -> int:
     ap.add_argument("--kernel", choices=["auto", "percell", "stream"], default="auto")
     ap.add_argument("--strips", type=int, default=0, help="row strips per block (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run one axpy of known traffic (for profiler byte-counter scaling)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:

@@ -201,6 +201,43 @@ int lgd_amr_restrict_level(lgd_ctx *ctx, const lgd_amr *amr, lgd_buf *field, int
 int lgd_amr_fill_level(lgd_ctx *ctx, const lgd_amr *amr, lgd_buf *field, const lgd_buf *old,
                        double alpha, int32_t level);
 
+/* ------------------------------------------- batched ensemble of Model C runs */
+/* An ensemble of independent 2-D Model C realizations of uneven sizes, each its own
+ * Simulation (own BlockIds from 0, own EulerMaruyama seed, mirror faces), sharing
+ * one block shape and spacing and advanced together by ONE launch per step over a
+ * device table of every block of every realization.  No reference counterpart as a
+ * batch: each realization is exactly Simulation::new(CartesianGrid(cells,
+ * block_cells), FiniteVolume, ModelC{eps4, lambda, d_thermal, b}, EulerMaruyama{seed}). */
+typedef struct {
+  int64_t cells[2];
+  double eps4;
+  double lambda;
+  double d_thermal;
+  double noise_amplitude; /* b; ModelC<Wiener<1>> when > 0 */
+  uint64_t seed;
+} lgd_realization;
+
+typedef struct lgd_ensemble lgd_ensemble;
+
+int lgd_ensemble_create(lgd_ctx *ctx, const lgd_realization *r, size_t n,
+                        const int64_t *block_cells2, const double *spacing2, double anisotropy_tol,
+                        double fv_tol, lgd_ensemble **out);
+int lgd_ensemble_destroy(lgd_ensemble *e);
+int64_t lgd_ensemble_num_blocks(const lgd_ensemble *e);
+/* First block of realization r in the ensemble's field buffers (block-major). */
+int64_t lgd_ensemble_block_start(const lgd_ensemble *e, int64_t r);
+/* Current state buffer of field 0 (phi) or 1 (u) (valid until the next step). */
+int lgd_ensemble_field(lgd_ensemble *e, int32_t field, lgd_buf **out);
+/* ModelC::initialize_grains on realization r (host, glibc tanh/hypot). */
+int lgd_ensemble_initialize_grains(lgd_ensemble *e, int64_t r, const double *grains, size_t n,
+                                   double u0);
+/* nsteps × Simulation::step(dt) of every realization; per_block = 1 issues one launch
+ * per block instead of one per step (the unbatched dispatch, for comparison). */
+int lgd_ensemble_step(lgd_ensemble *e, double dt, int64_t nsteps, int32_t per_block);
+double lgd_ensemble_time(const lgd_ensemble *e);
+/* Fraction of interior cells with phi > 0, per realization (n doubles). */
+int lgd_ensemble_solid_fraction(lgd_ensemble *e, double *out);
+
 /* BergerOliger policy with a GradientTagger (src/geometry/amr/adapt.rs:55-129). */
 typedef struct {
   int64_t every;      /* regrid every this many steps (step 0 always) */

@@ -518,3 +518,83 @@ def _sim_amr(self) -> AmrGrid:
 
 
 Simulation.amr = property(_sim_amr)
+
+
+@dataclasses.dataclass(frozen=True)
+class Realization:
+    """One member of an ensemble: its own Simulation of ModelC on a CartesianGrid."""
+
+    cells: tuple
+    eps4: float
+    lambda_: float
+    d_thermal: float
+    noise_amplitude: float = 0.0
+    seed: int = 0
+
+
+class Ensemble:
+    """Independent Model C realizations of uneven sizes advanced together
+    (one launch per step over every block of every realization)."""
+
+    def __init__(self, ctx: Context, realizations, block_cells=(64, 64), spacing=(0.4, 0.4),
+                 anisotropy_tol: float = 1e-8, fv_tol: float = 1e-8):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.realizations = list(realizations)
+        arr = (_abi.RealizationC * len(self.realizations))()
+        for i, r in enumerate(self.realizations):
+            arr[i].cells[0], arr[i].cells[1] = int(r.cells[0]), int(r.cells[1])
+            arr[i].eps4, arr[i].lambda_, arr[i].d_thermal = r.eps4, r.lambda_, r.d_thermal
+            arr[i].noise_amplitude, arr[i].seed = r.noise_amplitude, r.seed
+        bc = (C.c_int64 * 2)(*block_cells)
+        sp = (C.c_double * 2)(*spacing)
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_ensemble_create(ctx.handle, arr, len(self.realizations), bc, sp,
+                                                 anisotropy_tol, fv_tol, C.byref(h)), ctx.handle)
+        self.handle = h
+        self.block_cells = tuple(block_cells)
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.lgd_ensemble_destroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass
+
+    @property
+    def num_blocks(self) -> int:
+        return int(self._lib.lgd_ensemble_num_blocks(self.handle))
+
+    def block_start(self, r: int) -> int:
+        return int(self._lib.lgd_ensemble_block_start(self.handle, r))
+
+    def field(self, index: int) -> DeviceBuffer:
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_ensemble_field(self.handle, index, C.byref(h)), self.ctx.handle)
+        return DeviceBuffer._borrow(self.ctx, h)
+
+    def initialize_grains(self, r: int, grains: np.ndarray, u0: float) -> None:
+        g = np.ascontiguousarray(grains, dtype=np.float64).reshape(-1, 3)
+        _abi.check(self._lib.lgd_ensemble_initialize_grains(self.handle, r, g.ctypes.data_as(_abi.DP),
+                                                            g.shape[0], u0), self.ctx.handle)
+
+    def step(self, dt: float, nsteps: int = 1, per_block: bool = False) -> None:
+        _abi.check(self._lib.lgd_ensemble_step(self.handle, dt, nsteps, int(per_block)), self.ctx.handle)
+
+    @property
+    def time(self) -> float:
+        return float(self._lib.lgd_ensemble_time(self.handle))
+
+    def solid_fraction(self) -> np.ndarray:
+        out = np.empty(len(self.realizations))
+        _abi.check(self._lib.lgd_ensemble_solid_fraction(self.handle, out.ctypes.data_as(_abi.DP)),
+                   self.ctx.handle)
+        return out
+
+    def realization_field(self, r: int, index: int) -> np.ndarray:
+        """Download realization r's slabs of one field (block-major)."""
+        nx, ny = self.block_cells
+        slab = (nx + 2) * (ny + 2)
+        b0, b1 = self.block_start(r), self.block_start(r + 1)
+        return self.field(index).download(b0 * slab, (b1 - b0) * slab)

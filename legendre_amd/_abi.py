@@ -64,6 +64,17 @@ class RegridPolicyC(C.Structure):
     ]
 
 
+class RealizationC(C.Structure):
+    _fields_ = [
+        ("cells", C.c_int64 * 2),
+        ("eps4", C.c_double),
+        ("lambda_", C.c_double),
+        ("d_thermal", C.c_double),
+        ("noise_amplitude", C.c_double),
+        ("seed", C.c_uint64),
+    ]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 I32, I64, U32, U64, SZ, DBL = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_size_t, C.c_double
@@ -115,6 +126,15 @@ SIGNATURES = {
     "lgd_sim_create_amr": (C.c_int, [P, P, I32, C.POINTER(ModelCParamsC), DBL, I32, U64, I32,
                                      C.POINTER(RegridPolicyC), PP]),
     "lgd_sim_amr": (P, [P]),
+    "lgd_ensemble_create": (C.c_int, [P, C.POINTER(RealizationC), SZ, C.POINTER(I64), DP, DBL, DBL, PP]),
+    "lgd_ensemble_destroy": (C.c_int, [P]),
+    "lgd_ensemble_num_blocks": (I64, [P]),
+    "lgd_ensemble_block_start": (I64, [P, I64]),
+    "lgd_ensemble_field": (C.c_int, [P, I32, PP]),
+    "lgd_ensemble_initialize_grains": (C.c_int, [P, I64, DP, SZ, DBL]),
+    "lgd_ensemble_step": (C.c_int, [P, DBL, I64, I32]),
+    "lgd_ensemble_time": (DBL, [P]),
+    "lgd_ensemble_solid_fraction": (C.c_int, [P, DP]),
     "lgd_sim_create_modelc": (C.c_int, [P, P, C.POINTER(ModelCParamsC), I32, U64, I32, PP]),
     "lgd_sim_create_heat": (C.c_int, [P, P, DBL, C.POINTER(FaceBcC), I32, PP]),
     "lgd_sim_destroy": (C.c_int, [P]),

@@ -117,11 +117,24 @@ struct lgd_kr_consts {
 
 lgd_kr_consts lgd_make_kr_consts(const lgd_modelc_params &p, const lgd_cart &c);
 
-// Streaming Model C step (lgd_modelc_stream.hip); LGD_ERR_UNSUPPORTED when the
-// block is wider than 126 interior cells.
-int lgd_modelc_stream_launch(lgd_ctx *ctx, const lgd_cart &c, const lgd_kr_consts &k,
-                             const double *phi, const double *u, double *phi_n, double *u_n,
-                             double dt, bool noise, double noise_scale, uint64_t seed,
+// One independent Cartesian domain of the streaming Model C kernel: its
+// constants, Wiener amplitude scale (sqrt(dt)*b, 0 = none), noise seed, first
+// block and block topology (mirror or periodic faces).
+struct lgd_domain {
+  lgd_kr_consts k;
+  double noise_scale;
+  uint64_t seed;
+  int64_t block0;
+  int32_t nbx, nby, per_x, per_y;
+};
+
+// Streaming Model C step (lgd_modelc_stream.hip) over blocks
+// [block_begin, block_begin + nblocks); LGD_ERR_UNSUPPORTED when the block is
+// wider than 126 interior cells.
+int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
+                             const lgd_domain *domains, const int32_t *block_domain,
+                             int64_t block_begin, int64_t nblocks, const double *phi,
+                             const double *u, double *phi_n, double *u_n, double dt, bool noise,
                              uint64_t salt, uint64_t stream, int strips);
 
 // face(dx, dy) (src/discretization/finite_volume.rs:129-142): anisotropy A and its

@@ -34,20 +34,22 @@ constexpr int kWavesPerWG = 4;
 constexpr int kXcds = 8;
 
 struct StreamArgs {
-  lgd_kr_consts k;
+  // The domains (independent Cartesian grids sharing the block shape) and,
+  // per block, which domain it belongs to; a single grid passes its domain in
+  // d0 with domains == nullptr.
+  lgd_domain d0;
+  const lgd_domain *domains;
+  const int32_t *block_domain;
   const double *phi;
   const double *u;
   double *phi_n;
   double *u_n;
   double dt;
-  double noise_scale;  // sqrt(dt) * b
-  uint64_t seed;
   uint64_t salt;
   uint64_t stream;
-  int64_t nblocks;
+  int64_t nblocks;      // blocks in this launch
+  int64_t block_begin;  // first block of this launch
   int32_t nx, ny;      // block interior extent
-  int32_t nbx, nby;    // blocks per dimension
-  int32_t per_x, per_y;
   int32_t strips;      // strips per block
   int32_t rows;        // rows per strip
   int32_t nwg;         // workgroups in the launch
@@ -65,25 +67,26 @@ struct RowSrc {
   const double *right;   // halo cell at column nx+1
 };
 
-// Source pointers of interior row r (-1 <= r <= ny) of block (bx, by).
-__device__ __forceinline__ RowSrc row_src(const StreamArgs &a, const double *f, int64_t bx,
-                                          int64_t by, int r, int64_t bxl, int il, int64_t bxr,
-                                          int ir) {
+// Source pointers of interior row r (-1 <= r <= ny) of block (bx, by) of domain D.
+__device__ __forceinline__ RowSrc row_src(const StreamArgs &a, const lgd_domain &D,
+                                          const double *f, int64_t bx, int64_t by, int r,
+                                          int64_t bxl, int il, int64_t bxr, int ir) {
   const int64_t sx = a.nx + 2;
   const int64_t slab = sx * (a.ny + 2);
   int64_t byr = by, jr = r;
   if (r < 0 || r >= a.ny) {
     // Halo row: the block below/above, or this block's own edge row (mirror).
     int64_t gj = by * a.ny + r;
-    int64_t rj = wrap_or_reflect(gj, (int64_t)a.nby * a.ny, a.per_y);
+    int64_t rj = wrap_or_reflect(gj, (int64_t)D.nby * a.ny, D.per_y);
     byr = rj / a.ny;
     jr = rj - byr * a.ny;
   }
   int64_t rowoff = (jr + 1) * sx;
+  const double *base = f + D.block0 * slab + rowoff;
   RowSrc s;
-  s.center = f + (byr * a.nbx + bx) * slab + rowoff;
-  s.left = f + (byr * a.nbx + bxl) * slab + rowoff + (il + 1);
-  s.right = f + (byr * a.nbx + bxr) * slab + rowoff + (ir + 1);
+  s.center = base + (byr * D.nbx + bx) * slab;
+  s.left = base + (byr * D.nbx + bxl) * slab + (il + 1);
+  s.right = base + (byr * D.nbx + bxr) * slab + (ir + 1);
   return s;
 }
 
@@ -152,7 +155,21 @@ __device__ __forceinline__ void cell_update(const lgd_kr_consts &k, double flux,
   dphi_out = dphi;
 }
 
-template <bool kNoise, bool kVec>
+// Read a table entry at a wave-uniform address into scalar registers (the
+// compiler cannot prove the table is not written by the kernel, so a plain load
+// would land in vector registers).
+template <class T>
+__device__ __forceinline__ T load_uniform(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "word-sized table entries");
+  T out;
+  const int *src = reinterpret_cast<const int *>(p);
+  int *dst = reinterpret_cast<int *>(&out);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+template <bool kNoise, bool kVec, bool kTable>
 __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamArgs a) {
   // XCD-aware workgroup order: XCD x (= blockIdx.x % 8) takes one contiguous
   // range of logical workgroups.
@@ -166,19 +183,23 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   const int lane = threadIdx.x & 63;
   const int64_t item = (int64_t)lw * kWavesPerWG + wave;
   if (item >= a.nblocks * a.strips) return;  // whole wave exits together
-  const int64_t b = item / a.strips;
-  const int strip = (int)(item - b * a.strips);
+  const int64_t b = a.block_begin + item / a.strips;
+  const int strip = (int)(item % a.strips);
   const int j0 = strip * a.rows;
   const int j1 = min(j0 + a.rows, a.ny);
-  const int64_t bx = b % a.nbx, by = b / a.nbx;
+  // The block's domain (wave-uniform): its topology, constants and noise key.
+  const lgd_domain D =
+      kTable ? load_uniform(a.domains + __builtin_amdgcn_readfirstlane(a.block_domain[b])) : a.d0;
+  const int64_t rb = b - D.block0;  // BlockId within the domain
+  const int64_t bx = rb % D.nbx, by = rb / D.nbx;
   const int nx = a.nx, sx = nx + 2;
   const int64_t slab = (int64_t)sx * (a.ny + 2);
-  const lgd_kr_consts &k = a.k;
+  const lgd_kr_consts &k = D.k;
 
   // Halo columns: global column just left / right of this block, resolved.
-  const int64_t NX = (int64_t)a.nbx * nx;
-  int64_t gl = wrap_or_reflect(bx * nx - 1, NX, a.per_x);
-  int64_t gr = wrap_or_reflect(bx * nx + nx, NX, a.per_x);
+  const int64_t NX = (int64_t)D.nbx * nx;
+  int64_t gl = wrap_or_reflect(bx * nx - 1, NX, D.per_x);
+  int64_t gr = wrap_or_reflect(bx * nx + nx, NX, D.per_x);
   const int64_t bxl = gl / nx, bxr = gr / nx;
   const int il = (int)(gl - bxl * nx), ir = (int)(gr - bxr * nx);
   const int hr_lane = (nx + 1) >> 1, hr_slot = (nx + 1) & 1;
@@ -187,23 +208,26 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   const bool in0 = c0 >= 1 && c0 <= nx;
   const bool in1 = c1 <= nx;
 
+  // Wiener(0) amplitude b on interior cells; zero amplitude draws nothing
+  // (src/core/driver.rs:144-146).
+  const bool noisy = kNoise && D.noise_scale != 0.0;
   uint64_t block_key = 0;
-  if (kNoise) block_key = lgd_mix_key3(a.seed, a.salt, a.stream, (uint64_t)b);
+  if (noisy) block_key = lgd_mix_key3(D.seed, a.salt, a.stream, (uint64_t)rb);
 
   // Rows j0-1, j0, j0+1 of phi and u.
   double pm0, pm1, pc0, pc1, pp0, pp1, um0, um1, uc0, uc1, up0, up1;
   {
-    RowSrc s = row_src(a, a.phi, bx, by, j0 - 1, bxl, il, bxr, ir);
+    RowSrc s = row_src(a, D, a.phi, bx, by, j0 - 1, bxl, il, bxr, ir);
     load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pm0, pm1);
-    s = row_src(a, a.phi, bx, by, j0, bxl, il, bxr, ir);
+    s = row_src(a, D, a.phi, bx, by, j0, bxl, il, bxr, ir);
     load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pc0, pc1);
-    s = row_src(a, a.phi, bx, by, j0 + 1, bxl, il, bxr, ir);
+    s = row_src(a, D, a.phi, bx, by, j0 + 1, bxl, il, bxr, ir);
     load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pp0, pp1);
-    s = row_src(a, a.u, bx, by, j0 - 1, bxl, il, bxr, ir);
+    s = row_src(a, D, a.u, bx, by, j0 - 1, bxl, il, bxr, ir);
     load_row<kVec>(s, lane, sx, hr_lane, hr_slot, um0, um1);
-    s = row_src(a, a.u, bx, by, j0, bxl, il, bxr, ir);
+    s = row_src(a, D, a.u, bx, by, j0, bxl, il, bxr, ir);
     load_row<kVec>(s, lane, sx, hr_lane, hr_slot, uc0, uc1);
-    s = row_src(a, a.u, bx, by, j0 + 1, bxl, il, bxr, ir);
+    s = row_src(a, D, a.u, bx, by, j0 + 1, bxl, il, bxr, ir);
     load_row<kVec>(s, lane, sx, hr_lane, hr_slot, up0, up1);
   }
   const double q = 0.25;
@@ -224,9 +248,9 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     double pn0 = 0.0, pn1 = 0.0, un0 = 0.0, un1 = 0.0;
     const bool more = j + 1 < j1;
     if (more) {
-      RowSrc s = row_src(a, a.phi, bx, by, j + 2, bxl, il, bxr, ir);
+      RowSrc s = row_src(a, D, a.phi, bx, by, j + 2, bxl, il, bxr, ir);
       load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pn0, pn1);
-      s = row_src(a, a.u, bx, by, j + 2, bxl, il, bxr, ir);
+      s = row_src(a, D, a.u, bx, by, j + 2, bxl, il, bxr, ir);
       load_row<kVec>(s, lane, sx, hr_lane, hr_slot, un0, un1);
     }
     // x-faces right of c0 and c1; the face left of c0 is the left lane's right-of-c1.
@@ -241,12 +265,12 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     double n0, n1, v0, v1, d0, d1;
     cell_update(k, f0, pc0, pc1, pcL, pp0, pm0, uc0, uc1, ucL, up0, um0, a.dt, n0, v0, d0);
     cell_update(k, f1, pc1, pcR, pc0, pp1, pm1, uc1, ucR, uc0, up1, um1, a.dt, n1, v1, d1);
-    if (kNoise) {
+    if (noisy) {
       // Wiener(0) on phi after the drift (src/core/driver.rs:139-158); cell key =
       // interior linear index (src/geometry/cartesian.rs:599-619).
       uint64_t rowkey = (uint64_t)j * (uint64_t)nx;
-      if (in0) n0 = n0 + a.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c0 - 1))));
-      if (in1) n1 = n1 + a.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c1 - 1))));
+      if (in0) n0 = n0 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c0 - 1))));
+      if (in1) n1 = n1 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c1 - 1))));
     }
     const int64_t o = b * slab + (int64_t)(j + 1) * sx + c0;
     if (kVec && in0 && in1) {
@@ -281,51 +305,57 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
 
 }  // namespace
 
-// Streaming launch; returns LGD_ERR_UNSUPPORTED when the block is wider than one
-// wavefront row (126 interior cells), in which case the caller uses the per-cell kernel.
-int lgd_modelc_stream_launch(lgd_ctx *ctx, const lgd_cart &c, const lgd_kr_consts &k,
-                             const double *phi, const double *u, double *phi_n, double *u_n,
-                             double dt, bool noise, double noise_scale, uint64_t seed,
+// Streaming launch over blocks [block_begin, block_begin + nblocks) of a set of
+// domains sharing the block shape nx × ny.  Returns LGD_ERR_UNSUPPORTED when the
+// block is wider than one wavefront row (126 interior cells); the caller then
+// uses the per-cell kernel.
+int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
+                             const lgd_domain *domains, const int32_t *block_domain,
+                             int64_t block_begin, int64_t nblocks, const double *phi,
+                             const double *u, double *phi_n, double *u_n, double dt, bool noise,
                              uint64_t salt, uint64_t stream, int strips) {
-  if (c.dim != 2 || c.n[0] + 2 > 128) return LGD_ERR_UNSUPPORTED;
+  if (nx + 2 > 128 || nx < 1 || ny < 1) return LGD_ERR_UNSUPPORTED;
+  if (nblocks <= 0) return LGD_OK;
   StreamArgs a;
-  a.k = k;
+  a.d0 = d0;
+  a.domains = domains;
+  a.block_domain = block_domain;
   a.phi = phi;
   a.u = u;
   a.phi_n = phi_n;
   a.u_n = u_n;
   a.dt = dt;
-  a.noise_scale = noise_scale;
-  a.seed = seed;
   a.salt = salt;
   a.stream = stream;
-  a.nblocks = c.nblocks;
-  a.nx = (int32_t)c.n[0];
-  a.ny = (int32_t)c.n[1];
-  a.nbx = (int32_t)c.nb[0];
-  a.nby = (int32_t)c.nb[1];
-  a.per_x = c.periodic[0];
-  a.per_y = c.periodic[1];
+  a.nblocks = nblocks;
+  a.block_begin = block_begin;
+  a.nx = (int32_t)nx;
+  a.ny = (int32_t)ny;
   if (strips < 1) strips = 1;
   if (strips > a.ny) strips = a.ny;
   a.rows = (a.ny + strips - 1) / strips;
   a.strips = (a.ny + a.rows - 1) / a.rows;
-  int64_t items = c.nblocks * a.strips;
+  int64_t items = nblocks * a.strips;
   int64_t nwg = (items + kWavesPerWG - 1) / kWavesPerWG;
   if (nwg > INT32_MAX) return LGD_ERR_UNSUPPORTED;
   a.nwg = (int32_t)nwg;
-  const bool vec = ((c.n[0] + 2) % 2) == 0;
+  const bool vec = ((nx + 2) % 2) == 0;
   dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
+#define LGD_STREAM_LAUNCH(N, V, T) modelc_stream_kernel<N, V, T><<<grid, block, 0, ctx->stream>>>(a)
+  const bool table = domains != nullptr;
   if (noise) {
-    if (vec)
-      modelc_stream_kernel<true, true><<<grid, block, 0, ctx->stream>>>(a);
-    else
-      modelc_stream_kernel<true, false><<<grid, block, 0, ctx->stream>>>(a);
+    if (vec) {
+      if (table) LGD_STREAM_LAUNCH(true, true, true); else LGD_STREAM_LAUNCH(true, true, false);
+    } else {
+      if (table) LGD_STREAM_LAUNCH(true, false, true); else LGD_STREAM_LAUNCH(true, false, false);
+    }
   } else {
-    if (vec)
-      modelc_stream_kernel<false, true><<<grid, block, 0, ctx->stream>>>(a);
-    else
-      modelc_stream_kernel<false, false><<<grid, block, 0, ctx->stream>>>(a);
+    if (vec) {
+      if (table) LGD_STREAM_LAUNCH(false, true, true); else LGD_STREAM_LAUNCH(false, true, false);
+    } else {
+      if (table) LGD_STREAM_LAUNCH(false, false, true); else LGD_STREAM_LAUNCH(false, false, false);
+    }
   }
+#undef LGD_STREAM_LAUNCH
   return lgd::launch_check(ctx, "modelc_stream");
 }
