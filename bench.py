@@ -80,13 +80,16 @@ def run_gpu(args, C):
     return total_ms, per, wall
 
 
-def read_traffic(profile_dir: str):
-    path = os.path.join(profile_dir, "modelc_step_traffic.json")
-    if not os.path.exists(path):
+def read_traffic(profile: str):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary (each scaled by a copy kernel's reading)."""
+    if not os.path.exists(profile):
         return None
-    with open(path) as f:
-        d = json.load(f)
-    return d.get("bytes_per_launch")
+    with open(profile) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    return None
 
 
 def run_cpu_baseline(C, budget_s: float):
@@ -155,7 +158,9 @@ def main() -> int:
     achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9
     log(f"[bench] {args.steps} steps: {total_ms:.3f} ms on-stream, {wall * 1e3:.3f} ms wall; "
         f"per launch min {min(per):.4f} max {max(per):.4f} ms")
-    traffic = read_traffic(os.path.join(ROOT, "profiles"))
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "modelc_stream_kernel_10080sq.md"))
+    if args.cells != 10080 or args.noise != 0.0 or args.kernel == "percell" or args.mode != "fused":
+        traffic = None  # the committed profile covers the default workload only
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(C, args.cpu_budget)
@@ -199,7 +204,7 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "modelc_step_kernel",
+            "kernel": "modelc_step_kernel" if args.kernel == "percell" else "modelc_stream_kernel",
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_launch_ms, 5),
         },
