@@ -41,6 +41,7 @@ def run_gpu(args, C):
     ctx = lg.Context(0, stream.cuda_stream)
     ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2}[args.kernel])
     ctx.set_tuning("modelc_strips", args.strips)
+    ctx.set_tuning("modelc_prefetch", args.prefetch)
     grid = lg.CartesianGrid(ctx, [C.cells] * 2, [C.block] * 2, [0.0, 0.0], [C.h, C.h])
     params = lg.ModelCParams(C.eps4, C.lambda_, C.d_thermal, C.noise)
     sim = lg.Simulation.model_c(grid, params, wiener=C.noise > 0.0, seed=C.seed,
@@ -136,6 +137,8 @@ def main() -> int:
     ap.add_argument("--cells", type=int, default=10080)
     ap.add_argument("--kernel", choices=["auto", "percell", "stream"], default="auto")
     ap.add_argument("--strips", type=int, default=0, help="row strips per block (0 = default)")
+    ap.add_argument("--prefetch", type=int, default=0, choices=[0, 1, 2],
+                    help="rows the streaming kernel loads ahead (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one axpy of known traffic (for profiler byte-counter scaling)")
@@ -166,8 +169,8 @@ def main() -> int:
         cpu = run_cpu_baseline(C, args.cpu_budget)
         log(f"[bench] cpu baseline: {cpu}")
     workload = f"model_c_{C.cells}sq_{C.block}blocks_b{C.noise:g}_{args.mode}"
-    if args.kernel != "auto" or args.strips:
-        workload += f"_{args.kernel}_s{args.strips}"
+    if args.kernel != "auto" or args.strips or args.prefetch:
+        workload += f"_{args.kernel}_s{args.strips}_p{args.prefetch}"
     line = {
         "metric": METRIC,
         "value": round(value, 4),

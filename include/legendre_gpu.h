@@ -94,8 +94,9 @@ int lgd_ctx_sync(lgd_ctx *ctx);
 void *lgd_ctx_stream(lgd_ctx *ctx);
 const char *lgd_ctx_last_error(const lgd_ctx *ctx);
 /* Launch-shape knobs for this context (no reference counterpart: the reference has
- * no launch shapes).  Keys: "modelc_kernel" (0 auto, 1 per-cell, 2 streaming) and
- * "modelc_strips" (row strips per block for the streaming kernel, 0 = default).
+ * no launch shapes).  Keys: "modelc_kernel" (0 auto, 1 per-cell, 2 streaming),
+ * "modelc_strips" (row strips per block for the streaming kernel, 0 = default) and
+ * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default).
  * Results are bitwise independent of every knob. */
 int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value);
 
@@ -164,7 +165,8 @@ int lgd_apply_wiener(lgd_ctx *ctx, const lgd_grid *grid, lgd_buf *x, const lgd_b
  * ModelC<NoNoise | Wiener<1>>, mirror ghosts, src/physics/phasefield.rs:431-439) in one
  * kernel: halos are read from neighbour interiors (no ghost pass), the RHS and the
  * stage update are fused, and the new interiors are written to phi_next/u_next
- * (ghost entries of *_next are left stale; fill_ghosts restores them if needed).
+ * (ghost entries of *_next are unspecified afterwards, as the reference's are until the
+ * next ghost fill, src/core/driver.rs:128-130; fill_ghosts restores them if needed).
  * salt = t.to_bits() of the step's start time; wiener = 0 for NoNoise.
  * Interiors equal the reference step's bit for bit (noise: up to the device log/cos). */
 int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,

@@ -126,6 +126,9 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
     ctx->modelc_kernel = value;
   } else if (k == "modelc_strips") {
     ctx->modelc_strips = value;
+  } else if (k == "modelc_prefetch") {
+    if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "modelc_prefetch is 0, 1 or 2");
+    ctx->modelc_prefetch = value;
   } else {
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "unknown tuning key " + k);
   }

@@ -17,6 +17,7 @@ struct lgd_ctx {
   // Launch-shape knobs (lgd_ctx_set_tuning); 0 = pick automatically.
   int64_t modelc_kernel = 0;  // 1 = per-cell kernel, 2 = streaming kernel
   int64_t modelc_strips = 0;  // strips (row ranges) per block for the streaming kernel
+  int64_t modelc_prefetch = 0;  // rows the streaming kernel loads ahead (1 or 2; 0 = default)
 };
 
 struct lgd_buf {

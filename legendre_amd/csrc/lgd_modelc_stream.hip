@@ -2,16 +2,17 @@
 //
 // Work decomposition: one wavefront advances one strip of rows of one block.  The
 // 64 lanes cover the ghost-inclusive slab row (up to 128 columns) with two columns
-// per lane, so a row of phi or u is one coalesced 16-byte load per lane.  The wave
-// marches up the strip keeping rows j-1, j, j+1 in registers and prefetching row
-// j+2 while row j is computed.
+// per lane (two coalesced 8-byte loads per lane per row and field, the halo
+// cells folded into lane 0 and lane (nx+1)/2 by their addresses).  The wave
+// marches up the strip keeping rows j-1, j, j+1 in registers while the loads of
+// the next one or two rows are in flight.
 //
 // Face sharing: the Karma–Rappel face flux at (i+1/2, j) is the same expression,
 // operand for operand, as the left-face flux of cell i+1, and the flux at
 // (i, j+1/2) the same as the bottom-face flux of cell (i, j+1)
 // (src/discretization/finite_volume.rs:261-280), so each face is evaluated once:
 // y-faces carry over in registers from one row to the next, x-faces come from the
-// neighbouring lane by a cross-lane shuffle.  Results are bitwise those of the
+// neighbouring lane by a DPP wavefront shift.  Results are bitwise those of the
 // per-cell evaluation; divisions per cell drop from 10 to 6.
 //
 // Halos: the ghost ring of the slab is never read.  The two halo columns come
@@ -61,64 +62,46 @@ __device__ __forceinline__ int64_t wrap_or_reflect(int64_t g, int64_t n, int per
   return g;
 }
 
-struct RowSrc {
-  const double *center;  // slab row start (column 0) in this block column
-  const double *left;    // halo cell at column 0
-  const double *right;   // halo cell at column nx+1
-};
-
-// Source pointers of interior row r (-1 <= r <= ny) of block (bx, by) of domain D.
-__device__ __forceinline__ RowSrc row_src(const StreamArgs &a, const lgd_domain &D,
-                                          const double *f, int64_t bx, int64_t by, int r,
-                                          int64_t bxl, int il, int64_t bxr, int ir) {
-  const int64_t sx = a.nx + 2;
-  const int64_t slab = sx * (a.ny + 2);
+// Element offset, from a field's base, of the first column of the slab row that
+// holds row r (-1 <= r <= ny) of block row by, in block column 0 of that block row.
+// Rows -1 and ny are halo rows: the block below/above (exchange or periodic wrap)
+// or this block's own edge row (mirror, reflect_coord in src/geometry/boundary.rs:82-90).
+__device__ __forceinline__ int64_t row_base(const StreamArgs &a, const lgd_domain &D, int64_t by,
+                                            int r, int64_t slab, int sx) {
   int64_t byr = by, jr = r;
   if (r < 0 || r >= a.ny) {
-    // Halo row: the block below/above, or this block's own edge row (mirror).
     int64_t gj = by * a.ny + r;
     int64_t rj = wrap_or_reflect(gj, (int64_t)D.nby * a.ny, D.per_y);
     byr = rj / a.ny;
     jr = rj - byr * a.ny;
   }
-  int64_t rowoff = (jr + 1) * sx;
-  const double *base = f + D.block0 * slab + rowoff;
-  RowSrc s;
-  s.center = base + (byr * D.nbx + bx) * slab;
-  s.left = base + (byr * D.nbx + bxl) * slab + (il + 1);
-  s.right = base + (byr * D.nbx + bxr) * slab + (ir + 1);
-  return s;
+  return (D.block0 + byr * D.nbx) * slab + (jr + 1) * (int64_t)sx;
 }
 
-// Load this lane's two columns of a row, with the halo columns patched in.
-template <bool kVec>
-__device__ __forceinline__ void load_row(const RowSrc &s, int lane, int sx, int hr_lane,
-                                         int hr_slot, double &v0, double &v1) {
-  const int c0 = 2 * lane;
-  if (kVec) {
-    if (c0 + 1 < sx) {
-      double2 v = *reinterpret_cast<const double2 *>(s.center + c0);
-      v0 = v.x;
-      v1 = v.y;
-    } else {
-      v0 = 0.0;
-      v1 = 0.0;
-    }
+// Two consecutive f64 columns as one 16-byte vector (for nontemporal stores).
+typedef double F64x2 __attribute__((ext_vector_type(2)));
+
+// One row of a field as this lane holds it: two consecutive columns.
+struct RowVal {
+  double v0, v1;
+};
+
+// Cross-lane moves by one lane (DPP wavefront shifts: no LDS round trip).  The
+// lane with no source reads 0; it is never an interior column's neighbour.
+__device__ __forceinline__ double dpp_shift(double v, int ctrl_is_right) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  if (ctrl_is_right) {
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, true);  // wave_shr:1
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, true);
   } else {
-    v0 = c0 < sx ? s.center[c0] : 0.0;
-    v1 = c0 + 1 < sx ? s.center[c0 + 1] : 0.0;
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, true);  // wave_shl:1
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, true);
   }
-  if (lane == 0) v0 = *s.left;
-  if (lane == hr_lane) {
-    if (hr_slot == 0)
-      v0 = *s.right;
-    else
-      v1 = *s.right;
-  }
+  return __hiloint2double(hi, lo);
 }
-
-__device__ __forceinline__ double from_left(double v) { return __shfl_up(v, 1, 64); }
-__device__ __forceinline__ double from_right(double v) { return __shfl_down(v, 1, 64); }
+// Value of the lane to the left (lane - 1) / right (lane + 1).
+__device__ __forceinline__ double from_left(double v) { return dpp_shift(v, 1); }
+__device__ __forceinline__ double from_right(double v) { return dpp_shift(v, 0); }
 
 // Karma–Rappel flux through an x-face: J = A·(A·derx − A′·dery).
 __device__ __forceinline__ double flux_x(const lgd_kr_consts &k, double derx, double dery) {
@@ -169,7 +152,7 @@ __device__ __forceinline__ T load_uniform(const T *p) {
   return out;
 }
 
-template <bool kNoise, bool kVec, bool kTable>
+template <bool kNoise, bool kVec, bool kTable, int kDepth>
 __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamArgs a) {
   // XCD-aware workgroup order: XCD x (= blockIdx.x % 8) takes one contiguous
   // range of logical workgroups.
@@ -214,21 +197,32 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   uint64_t block_key = 0;
   if (noisy) block_key = lgd_mix_key3(D.seed, a.salt, a.stream, (uint64_t)rb);
 
+  // Per-lane column offsets (from the start of a slab row in block column 0),
+  // valid for every row: lane 0 reads the left halo cell in its first column,
+  // lane (nx+1)/2 the right halo cell in its first (nx odd) or second (nx even)
+  // column, lanes past the slab row re-read its last column (their values are
+  // never stored and never an interior column's neighbour).
+  const int64_t lcol = bxl * slab + (il + 1), rcol = bxr * slab + (ir + 1);
+  const int64_t col0 = lane == 0 ? lcol
+                       : (lane == hr_lane && hr_slot == 0) ? rcol
+                                                           : bx * slab + min(c0, sx - 1);
+  const int64_t col1 = (lane == hr_lane && hr_slot == 1) ? rcol : bx * slab + min(c1, sx - 1);
+  auto fetch = [&](const double *f, int r) {
+    const double *row = f + row_base(a, D, by, r, slab, sx);
+    return RowVal{row[col0], row[col1]};
+  };
+
   // Rows j0-1, j0, j0+1 of phi and u.
   double pm0, pm1, pc0, pc1, pp0, pp1, um0, um1, uc0, uc1, up0, up1;
   {
-    RowSrc s = row_src(a, D, a.phi, bx, by, j0 - 1, bxl, il, bxr, ir);
-    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pm0, pm1);
-    s = row_src(a, D, a.phi, bx, by, j0, bxl, il, bxr, ir);
-    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pc0, pc1);
-    s = row_src(a, D, a.phi, bx, by, j0 + 1, bxl, il, bxr, ir);
-    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pp0, pp1);
-    s = row_src(a, D, a.u, bx, by, j0 - 1, bxl, il, bxr, ir);
-    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, um0, um1);
-    s = row_src(a, D, a.u, bx, by, j0, bxl, il, bxr, ir);
-    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, uc0, uc1);
-    s = row_src(a, D, a.u, bx, by, j0 + 1, bxl, il, bxr, ir);
-    load_row<kVec>(s, lane, sx, hr_lane, hr_slot, up0, up1);
+    RowVal r0 = fetch(a.phi, j0 - 1), r1 = fetch(a.phi, j0), r2 = fetch(a.phi, j0 + 1);
+    RowVal r3 = fetch(a.u, j0 - 1), r4 = fetch(a.u, j0), r5 = fetch(a.u, j0 + 1);
+    pm0 = r0.v0; pm1 = r0.v1;
+    pc0 = r1.v0; pc1 = r1.v1;
+    pp0 = r2.v0; pp1 = r2.v1;
+    um0 = r3.v0; um1 = r3.v1;
+    uc0 = r4.v0; uc1 = r4.v1;
+    up0 = r5.v0; up1 = r5.v1;
   }
   const double q = 0.25;
   double pmR = from_right(pm0);
@@ -243,16 +237,18 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     jb1 = flux_y(k, q * (((pcR - pc0) + pmR) - pm0), pc1 - pm1);
   }
 
-  for (int j = j0; j < j1; ++j) {
-    // Prefetch row j+2 (needed from the next iteration on).
-    double pn0 = 0.0, pn1 = 0.0, un0 = 0.0, un1 = 0.0;
-    const bool more = j + 1 < j1;
-    if (more) {
-      RowSrc s = row_src(a, D, a.phi, bx, by, j + 2, bxl, il, bxr, ir);
-      load_row<kVec>(s, lane, sx, hr_lane, hr_slot, pn0, pn1);
-      s = row_src(a, D, a.u, bx, by, j + 2, bxl, il, bxr, ir);
-      load_row<kVec>(s, lane, sx, hr_lane, hr_slot, un0, un1);
-    }
+  // Row pipeline: at row j the loads of row j+1+kDepth are issued; the row that
+  // becomes the top neighbour of row j+1 (issued kDepth-1 rows earlier) is moved
+  // into place only after row j is computed, and row j is stored last, so the
+  // wave waits on loads issued one (kDepth 1) or two (kDepth 2) rows of compute
+  // ago.  Two buffers alternate (the loop is unrolled by two) so no register
+  // with a load in flight is copied.
+  auto body = [&](int j, RowVal &ip, RowVal &iu, RowVal &cp, RowVal &cu) {
+    // Issued on every row (past the strip's last needed row it re-reads row j1,
+    // unused) so the wait for the consumed row never depends on a branch.
+    const int rn = min(j + 1 + kDepth, j1);
+    ip = fetch(a.phi, rn);
+    iu = fetch(a.u, rn);
     // x-faces right of c0 and c1; the face left of c0 is the left lane's right-of-c1.
     double jx0 = flux_x(k, pc1 - pc0, q * (((pp1 - pm1) + pp0) - pm0));
     double jx1 = flux_x(k, pcR - pc1, q * (((ppR - pmR) + pp1) - pm1));
@@ -272,35 +268,85 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
       if (in0) n0 = n0 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c0 - 1))));
       if (in1) n1 = n1 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c1 - 1))));
     }
-    const int64_t o = b * slab + (int64_t)(j + 1) * sx + c0;
-    if (kVec && in0 && in1) {
-      *reinterpret_cast<double2 *>(a.phi_n + o) = make_double2(n0, n1);
-      *reinterpret_cast<double2 *>(a.u_n + o) = make_double2(v0, v1);
-    } else {
-      if (in0) {
-        a.phi_n[o] = n0;
-        a.u_n[o] = v0;
-      }
-      if (in1) {
-        a.phi_n[o + 1] = n1;
-        a.u_n[o + 1] = v1;
-      }
-    }
     jb0 = jt0;
     jb1 = jt1;
-    if (more) {
+    // The output's ghost columns receive the halo values read for this step, so
+    // every store covers whole cache lines (no read-modify-write of the line
+    // shared with the ghost cell).  Ghost cells carry no state between steps:
+    // they are refilled before any stencil reads them (src/core/driver.rs:128-130).
+    if (!in0) {
+      n0 = pc0;
+      v0 = uc0;
+    }
+    if (!in1) {
+      n1 = pc1;
+      v1 = uc1;
+    }
+    if (j + 1 < j1) {
       pm0 = pc0; pm1 = pc1; pmR = pcR;
       pc0 = pp0; pc1 = pp1; pcL = ppL; pcR = ppR;
-      pp0 = pn0; pp1 = pn1;
+      pp0 = cp.v0; pp1 = cp.v1;
       ppL = from_left(pp1);
       ppR = from_right(pp0);
       um0 = uc0; um1 = uc1;
       uc0 = up0; uc1 = up1;
-      up0 = un0; up1 = un1;
+      up0 = cu.v0; up1 = cu.v1;
       ucL = from_left(uc1);
       ucR = from_right(uc0);
     }
+    const int64_t o = b * slab + (int64_t)(j + 1) * sx + c0;
+    // Streaming (nontemporal) stores: the new state is not read again in this
+    // launch, so it need not displace the halo lines neighbouring waves re-read.
+    if (kVec) {
+      if (c1 < sx) {
+        __builtin_nontemporal_store(F64x2{n0, n1}, reinterpret_cast<F64x2 *>(a.phi_n + o));
+        __builtin_nontemporal_store(F64x2{v0, v1}, reinterpret_cast<F64x2 *>(a.u_n + o));
+      }
+    } else {
+      if (c0 < sx) {
+        __builtin_nontemporal_store(n0, a.phi_n + o);
+        __builtin_nontemporal_store(v0, a.u_n + o);
+      }
+      if (c1 < sx) {
+        __builtin_nontemporal_store(n1, a.phi_n + o + 1);
+        __builtin_nontemporal_store(v1, a.u_n + o + 1);
+      }
+    }
+  };
+  RowVal ap{}, au{}, bp{}, bu{};
+  if (kDepth == 2) {
+    ap = fetch(a.phi, min(j0 + 2, j1));
+    au = fetch(a.u, min(j0 + 2, j1));
   }
+  for (int j = j0; j < j1; j += 2) {
+    if (kDepth == 2) {
+      body(j, bp, bu, ap, au);
+      if (j + 1 < j1) body(j + 1, ap, au, bp, bu);
+    } else {
+      body(j, ap, au, ap, au);
+      if (j + 1 < j1) body(j + 1, ap, au, ap, au);
+    }
+  }
+}
+
+template <int kDepth>
+void launch_variant(dim3 grid, dim3 block, hipStream_t st, const StreamArgs &a, bool noise,
+                    bool vec, bool table) {
+#define LGD_STREAM_LAUNCH(N, V, T) modelc_stream_kernel<N, V, T, kDepth><<<grid, block, 0, st>>>(a)
+  if (noise) {
+    if (vec) {
+      if (table) LGD_STREAM_LAUNCH(true, true, true); else LGD_STREAM_LAUNCH(true, true, false);
+    } else {
+      if (table) LGD_STREAM_LAUNCH(true, false, true); else LGD_STREAM_LAUNCH(true, false, false);
+    }
+  } else {
+    if (vec) {
+      if (table) LGD_STREAM_LAUNCH(false, true, true); else LGD_STREAM_LAUNCH(false, true, false);
+    } else {
+      if (table) LGD_STREAM_LAUNCH(false, false, true); else LGD_STREAM_LAUNCH(false, false, false);
+    }
+  }
+#undef LGD_STREAM_LAUNCH
 }
 
 }  // namespace
@@ -340,22 +386,12 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   if (nwg > INT32_MAX) return LGD_ERR_UNSUPPORTED;
   a.nwg = (int32_t)nwg;
   const bool vec = ((nx + 2) % 2) == 0;
-  dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
-#define LGD_STREAM_LAUNCH(N, V, T) modelc_stream_kernel<N, V, T><<<grid, block, 0, ctx->stream>>>(a)
   const bool table = domains != nullptr;
-  if (noise) {
-    if (vec) {
-      if (table) LGD_STREAM_LAUNCH(true, true, true); else LGD_STREAM_LAUNCH(true, true, false);
-    } else {
-      if (table) LGD_STREAM_LAUNCH(true, false, true); else LGD_STREAM_LAUNCH(true, false, false);
-    }
-  } else {
-    if (vec) {
-      if (table) LGD_STREAM_LAUNCH(false, true, true); else LGD_STREAM_LAUNCH(false, true, false);
-    } else {
-      if (table) LGD_STREAM_LAUNCH(false, false, true); else LGD_STREAM_LAUNCH(false, false, false);
-    }
-  }
-#undef LGD_STREAM_LAUNCH
+  const int depth = ctx->modelc_prefetch == 1 ? 1 : 2;
+  dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
+  if (depth == 2)
+    launch_variant<2>(grid, block, ctx->stream, a, noise, vec, table);
+  else
+    launch_variant<1>(grid, block, ctx->stream, a, noise, vec, table);
   return lgd::launch_check(ctx, "modelc_stream");
 }

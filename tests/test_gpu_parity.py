@@ -238,32 +238,35 @@ MODELC_CASES = [
 
 
 VARIANTS = {
-    # name: (mode, modelc_kernel, modelc_strips)
-    "per_intent": (lg.LGD_MODE_PER_INTENT, 0, 0),
-    "fused_percell": (lg.LGD_MODE_FUSED, 1, 0),
-    "fused_stream_1strip": (lg.LGD_MODE_FUSED, 2, 1),
-    "fused_stream_3strips": (lg.LGD_MODE_FUSED, 2, 3),
+    # name: (mode, modelc_kernel, modelc_strips, modelc_prefetch)
+    "per_intent": (lg.LGD_MODE_PER_INTENT, 0, 0, 0),
+    "fused_percell": (lg.LGD_MODE_FUSED, 1, 0, 0),
+    "fused_stream_1strip": (lg.LGD_MODE_FUSED, 2, 1, 0),
+    "fused_stream_3strips": (lg.LGD_MODE_FUSED, 2, 3, 0),
+    "fused_stream_3strips_prefetch1": (lg.LGD_MODE_FUSED, 2, 3, 1),
+    "fused_stream_7strips_prefetch2": (lg.LGD_MODE_FUSED, 2, 7, 2),
 }
 
 
 @pytest.fixture
 def tuned(ctx):
-    def apply(kernel, strips):
+    def apply(kernel, strips, prefetch=0):
         ctx.set_tuning("modelc_kernel", kernel)
         ctx.set_tuning("modelc_strips", strips)
+        ctx.set_tuning("modelc_prefetch", prefetch)
 
     yield apply
-    apply(0, 0)
+    apply(0, 0, 0)
 
 
 @pytest.mark.parametrize("variant", list(VARIANTS))
 @pytest.mark.parametrize("case", range(len(MODELC_CASES)))
 def test_modelc_matches_oracle_bitwise(ctx, orc, tuned, case, variant):
     cells, block, periodic, grains = MODELC_CASES[case]
-    mode, kernel, strips = VARIANTS[variant]
+    mode, kernel, strips, prefetch = VARIANTS[variant]
     steps = 200
     say(f"modelc b=0 cells={cells} block={block} periodic={periodic} {variant} steps={steps}")
-    tuned(kernel, strips)
+    tuned(kernel, strips, prefetch)
     og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, periodic, np.array(grains), 0.0,
                                            False, mode)
     dt = 0.008
@@ -278,8 +281,8 @@ def test_modelc_matches_oracle_bitwise(ctx, orc, tuned, case, variant):
 def test_modelc_noise_matches_oracle_within_tolerance(ctx, orc, tuned, variant):
     cells, block = [252, 126], [126, 63]
     grains = np.array([[50.0, 25.0, 3.0]])
-    mode, kernel, strips = VARIANTS[variant]
-    tuned(kernel, strips)
+    mode, kernel, strips, prefetch = VARIANTS[variant]
+    tuned(kernel, strips, prefetch)
     say(f"modelc b=0.01 cells={cells} {variant}")
     og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], grains, 0.01,
                                            True, mode)
@@ -310,8 +313,8 @@ def test_fused_and_per_intent_agree_and_runs_are_deterministic(ctx, orc, tuned):
     say("determinism across variants + repeat, b=0.01")
     outs = []
     for variant in ["fused_stream_1strip"] + list(VARIANTS):
-        mode, kernel, strips = VARIANTS[variant]
-        tuned(kernel, strips)
+        mode, kernel, strips, prefetch = VARIANTS[variant]
+        tuned(kernel, strips, prefetch)
         _, gg, sim, *_ = _modelc_pair(ctx, orc, cells, block, [False, False], grains, 0.01, True, mode)
         sim.step(0.008, 100)
         outs.append((sim.field(0).download(), sim.field(1).download()))
