@@ -79,6 +79,12 @@ _SIGS = {
     "orc_heat_run": (C.c_int, [GP, C.c_double, DP, C.c_int64, C.c_double, DP, C.POINTER(FaceBc), C.c_int32]),
     "orc_initialize_grains": (None, [GP, DP, DP, DP, C.c_size_t, C.c_double]),
     "orc_seed_positions": (None, [C.c_uint64, C.c_size_t, C.c_double, C.c_double, DP]),
+    "orc_modelc_tlm_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, DP, DP, DP, C.c_int64,
+                                     C.c_double]),
+    "orc_modelc_adjoint_step": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_double, DP, DP,
+                                          DP]),
+    "orc_modelc_gradient": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, DP, C.c_int64, C.c_double,
+                                      DP, DP, DP, DP]),
 }
 
 _lib = None
@@ -249,6 +255,38 @@ def seed_positions(seed: int, nseeds: int, extent: float, radius: float) -> np.n
     out = np.empty((nseeds, 3), dtype=np.float64)
     lib().orc_seed_positions(seed, nseeds, extent, radius, _dp(out))
     return out
+
+
+# ---- Model C derivatives (checkers for the device adjoint) ------------------------
+def modelc_tlm_run(g: Grid, p: ModelCParams, phi, u, dphi, du, dtheta, nsteps, dt) -> None:
+    """Forward-Euler Model C (b = 0) with tangents along dtheta = (dλ, dD, dε₄):
+    values bitwise those of modelc_run, tangents the exact derivative."""
+    th = np.ascontiguousarray(dtheta, dtype=np.float64)
+    rc = lib().orc_modelc_tlm_run(C.byref(g), C.byref(p), _dp(phi), _dp(u), _dp(dphi), _dp(du),
+                                  _dp(th), nsteps, dt)
+    if rc != 0:
+        raise ValueError("orc_modelc_tlm_run rejected the configuration")
+
+
+def modelc_adjoint_step(g: Grid, p: ModelCParams, phi, u, dt, pphi, pu, grad) -> None:
+    """One reverse step: (pphi, pu) <- (dS/dx)^T (pphi, pu); grad (λ, D, ε₄) += (dS/dθ)^T."""
+    rc = lib().orc_modelc_adjoint_step(C.byref(g), C.byref(p), _dp(phi), _dp(u), dt, _dp(pphi),
+                                       _dp(pu), _dp(grad))
+    if rc != 0:
+        raise ValueError("orc_modelc_adjoint_step rejected the configuration")
+
+
+def modelc_gradient(g: Grid, p: ModelCParams, phi0, u0, target, nsteps, dt):
+    """L = ½‖φ_T − target‖² and its gradient w.r.t. (λ, D, ε₄), φ₀ and u₀ (all states kept)."""
+    loss = C.c_double(0.0)
+    grad = np.zeros(3)
+    gphi = np.zeros_like(phi0)
+    gu = np.zeros_like(u0)
+    rc = lib().orc_modelc_gradient(C.byref(g), C.byref(p), _dp(phi0), _dp(u0), _dp(target), nsteps,
+                                   dt, C.byref(loss), _dp(grad), _dp(gphi), _dp(gu))
+    if rc != 0:
+        raise ValueError("orc_modelc_gradient rejected the configuration")
+    return loss.value, grad, gphi, gu
 
 
 def time_bits(t: float) -> int:

@@ -176,6 +176,22 @@ double *orc_amr_sim_field(orc_amr_sim *s, int32_t field);
 double orc_amr_sim_time(const orc_amr_sim *s);
 int orc_amr_sim_step(orc_amr_sim *s, double dt, int64_t nsteps);
 
+/* ---- Model C derivatives (legendre_oracle_adjoint.cpp; no reference counterpart:
+ *      the checkers for the device adjoint).  θ = (λ, D, ε₄); b = 0 only. ---- */
+/* Tangent-linear run: advances (phi, u) exactly as orc_modelc_run (values bitwise) and
+ * their tangents (dphi, du) along dtheta.  Interiors only. */
+int orc_modelc_tlm_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                       double *dphi, double *du, const double *dtheta, int64_t nsteps,
+                       double dt);
+/* Hand-written adjoint of one step at state (phi, u): (pphi, pu) = ∂L/∂(φ', u') on
+ * entry, ∂L/∂(φ, u) on return; grad[3] += ∂L/∂θ of the step. */
+int orc_modelc_adjoint_step(const orc_grid *g, const orc_modelc_params *p, const double *phi,
+                            const double *u, double dt, double *pphi, double *pu, double *grad);
+/* Reverse-mode gradient of L = ½‖φ_T − target‖² over nsteps (all states kept). */
+int orc_modelc_gradient(const orc_grid *g, const orc_modelc_params *p, const double *phi0,
+                        const double *u0, const double *target, int64_t nsteps, double dt,
+                        double *loss, double *grad, double *gphi0, double *gu0);
+
 #ifdef __cplusplus
 }
 #endif
