@@ -89,6 +89,8 @@ uint32_t lgd_abi_version(void);
 /* ---------------------------------------------------------------- context */
 /* stream: an existing hipStream_t to launch on, or NULL for a library-owned one. */
 int lgd_ctx_create(int32_t device, void *stream, lgd_ctx **out);
+/* Every buffer, grid, simulation, ensemble, adjoint and calibration created on the
+ * context must be destroyed before it (they synchronise on its stream). */
 int lgd_ctx_destroy(lgd_ctx *ctx);
 int lgd_ctx_sync(lgd_ctx *ctx);
 void *lgd_ctx_stream(lgd_ctx *ctx);
@@ -291,6 +293,50 @@ int64_t lgd_sim_step_index(const lgd_sim *sim);
 /* ModelC::initialize_grains with orientation 0 (phasefield.rs:227-286), computed on the
  * host (glibc tanh/hypot, as the reference) and uploaded: grains = [cx, cy, r] × n. */
 int lgd_sim_initialize_grains(lgd_sim *sim, const double *grains, size_t ngrains, double u0);
+
+/* -------------------------------------------------------------- Model C adjoint */
+/* The reverse-mode derivative of the explicit Model C step (Euler–Maruyama with
+ * b = 0, which is forward Euler: src/integrators/euler_maruyama.rs:34-75,
+ * tests/convergence.rs:100-107) for calibrating θ = (λ, D, ε₄) and the initial
+ * fields.  No counterpart in the reference, which has no derivative code; the
+ * callers are a forward/backward trait in the autograd-Function role on the host
+ * side and the Python ModelCCalibration. */
+typedef struct lgd_adjoint lgd_adjoint;
+/* Workspace for reverse steps on `grid` (2-D; mirror or periodic faces). */
+int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out);
+int lgd_adjoint_destroy(lgd_adjoint *adj);
+/* One reverse step at the state (phi, u) the forward step started from: on entry
+ * (pphi, pu) hold dL/d(phi', u') on the interiors, on return dL/d(phi, u); the step's
+ * dL/dθ is added to the workspace's accumulator. */
+int lgd_modelc_adjoint_step(lgd_adjoint *adj, const lgd_modelc_params *params,
+                            const lgd_buf *phi, const lgd_buf *u, double dt, lgd_buf *pphi,
+                            lgd_buf *pu);
+/* Zero the accumulator / read it: grad = (dL/dλ, dL/dD, dL/dε₄), reduced in a fixed
+ * order (bitwise reproducible). */
+int lgd_adjoint_reset(lgd_adjoint *adj);
+int lgd_adjoint_grad(lgd_adjoint *adj, double *grad);
+
+/* Calibration: L(θ, φ₀, u₀) = ½‖φ_T − target‖² over the interiors after nsteps steps,
+ * as a forward and a backward call, reversed by binomial checkpointing in `snapshots`
+ * state slots (the initial state included; >= 1).  Results are bitwise independent of
+ * `snapshots`. */
+typedef struct lgd_calibration lgd_calibration;
+int lgd_calibration_create(lgd_ctx *ctx, const lgd_grid *grid, int64_t nsteps, double dt,
+                           int32_t snapshots, lgd_calibration **out);
+int lgd_calibration_destroy(lgd_calibration *cal);
+/* The forward sweep (keeping the schedule's first checkpoints); returns the loss. */
+int lgd_calibration_forward(lgd_calibration *cal, const lgd_modelc_params *params,
+                            const lgd_buf *phi0, const lgd_buf *u0, const lgd_buf *target,
+                            double *loss);
+/* The reverse sweep after lgd_calibration_forward: grad = dL/dθ (3 values);
+ * gphi0 / gu0 = dL/d(φ₀, u₀) on the interiors (either may be NULL). */
+int lgd_calibration_backward(lgd_calibration *cal, double *grad, lgd_buf *gphi0, lgd_buf *gu0);
+/* Forward steps executed so far (sweep and recomputation) and reverse steps. */
+int lgd_calibration_counts(const lgd_calibration *cal, int64_t *forward_steps,
+                           int64_t *reverse_steps);
+/* Forward steps of one forward + backward pass under the checkpointing schedule
+ * (host arithmetic only, no device work). */
+int64_t lgd_checkpoint_forward_steps(int64_t nsteps, int32_t snapshots);
 
 #ifdef __cplusplus
 }

@@ -9,6 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import dataclasses
 import struct
+import weakref
 from typing import Optional, Sequence
 
 import numpy as np
@@ -60,6 +61,12 @@ class Context:
         if st != 0:
             raise LegendreError(st, f"could not create a context on device {device}")
         self.handle = h
+        self._children = []  # weak references, creation order
+
+    def _adopt(self, obj) -> None:
+        """Register an object that holds device resources of this context; closing
+        the context closes it first (no library object may outlive its context)."""
+        self._children.append(weakref.ref(obj))
 
     def sync(self) -> None:
         _abi.check(self._lib.lgd_ctx_sync(self.handle), self.handle)
@@ -74,6 +81,14 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            for ref in reversed(self._children):  # dependents before what they use
+                obj = ref()
+                if obj is not None:
+                    try:
+                        obj.close()
+                    except Exception:
+                        pass
+            self._children.clear()
             self._lib.lgd_ctx_destroy(self.handle)
             self.handle = None
 
@@ -95,6 +110,7 @@ class DeviceBuffer:
         self.handle = h
         self.length = int(length)
         self.owned = True
+        ctx._adopt(self)
 
     @classmethod
     def _borrow(cls, ctx: Context, handle: C.c_void_p) -> "DeviceBuffer":
@@ -139,6 +155,8 @@ class DeviceBuffer:
         if self.owned and self.handle:
             self._lib.lgd_buf_free(self.handle)
         self.handle = None
+
+    close = free
 
     def __del__(self):
         try:
@@ -206,6 +224,7 @@ class CartesianGrid:
         h = C.c_void_p()
         _abi.check(self._lib.lgd_grid_create_cartesian(ctx.handle, C.byref(d), C.byref(h)), ctx.handle)
         self.handle = h
+        ctx._adopt(self)
 
     @property
     def num_blocks(self) -> int:
@@ -228,11 +247,14 @@ class CartesianGrid:
     def alloc_field(self, ghost: int = 1) -> DeviceBuffer:
         return DeviceBuffer(self.ctx, self.field_len(ghost))
 
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._lib.lgd_grid_destroy(self.handle)
+            self.handle = None
+
     def __del__(self):
         try:
-            if self.handle:
-                self._lib.lgd_grid_destroy(self.handle)
-                self.handle = None
+            self.close()
         except Exception:
             pass
 
@@ -351,6 +373,7 @@ class Simulation:
         self._lib = self.ctx._lib
         self.handle = handle
         self.nfields = nfields
+        self.ctx._adopt(self)
 
     @classmethod
     def model_c(cls, grid: CartesianGrid, params: ModelCParams, wiener: bool = False,
@@ -433,6 +456,8 @@ class AmrGrid:
         self._lib = ctx._lib
         self.handle = handle
         self.owned = owned
+        if owned:
+            ctx._adopt(self)
 
     @classmethod
     def from_patches(cls, ctx: Context, cells, block_cells, origin, spacing, ratios, levels):
@@ -450,11 +475,14 @@ class AmrGrid:
                                            C.byref(h)), ctx.handle)
         return cls(ctx, h)
 
+    def close(self) -> None:
+        if self.owned and getattr(self, "handle", None):
+            self._lib.lgd_amr_destroy(self.handle)
+        self.handle = None
+
     def __del__(self):
         try:
-            if self.owned and self.handle:
-                self._lib.lgd_amr_destroy(self.handle)
-                self.handle = None
+            self.close()
         except Exception:
             pass
 
@@ -553,12 +581,16 @@ class Ensemble:
                                                  anisotropy_tol, fv_tol, C.byref(h)), ctx.handle)
         self.handle = h
         self.block_cells = tuple(block_cells)
+        ctx._adopt(self)
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._lib.lgd_ensemble_destroy(self.handle)
+            self.handle = None
 
     def __del__(self):
         try:
-            if self.handle:
-                self._lib.lgd_ensemble_destroy(self.handle)
-                self.handle = None
+            self.close()
         except Exception:
             pass
 
@@ -598,3 +630,102 @@ class Ensemble:
         slab = (nx + 2) * (ny + 2)
         b0, b1 = self.block_start(r), self.block_start(r + 1)
         return self.field(index).download(b0 * slab, (b1 - b0) * slab)
+
+
+# ---------------------------------------------------------------- Model C adjoint
+class Adjoint:
+    """Reverse steps of the explicit Model C step (b = 0) on one grid: no
+    reference counterpart (the reference has no derivative code)."""
+
+    def __init__(self, grid: CartesianGrid):
+        self.grid = grid
+        self.ctx = grid.ctx
+        self._lib = grid._lib
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_adjoint_create(self.ctx.handle, grid.handle, C.byref(h)), self.ctx.handle)
+        self.handle = h
+        self.ctx._adopt(self)
+
+    def step(self, params: ModelCParams, phi: DeviceBuffer, u: DeviceBuffer, dt: float,
+             pphi: DeviceBuffer, pu: DeviceBuffer) -> None:
+        """(pphi, pu) <- transpose of d(step)/d(phi, u) applied to (pphi, pu); the
+        step's dL/d(λ, D, ε₄) is accumulated."""
+        pc = params.c()
+        _abi.check(self._lib.lgd_modelc_adjoint_step(self.handle, C.byref(pc), phi.handle, u.handle,
+                                                     dt, pphi.handle, pu.handle), self.ctx.handle)
+
+    def reset(self) -> None:
+        _abi.check(self._lib.lgd_adjoint_reset(self.handle), self.ctx.handle)
+
+    def grad(self) -> np.ndarray:
+        out = np.zeros(3)
+        _abi.check(self._lib.lgd_adjoint_grad(self.handle, out.ctypes.data_as(_abi.DP)), self.ctx.handle)
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._lib.lgd_adjoint_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ModelCCalibration:
+    """L(θ, φ₀, u₀) = ½‖φ_T − target‖² after `nsteps` Model C steps, with
+    forward() and backward() in the autograd-Function role: forward runs the
+    sweep and keeps the checkpoints, backward returns dL/d(λ, D, ε₄) and
+    dL/d(φ₀, u₀) by binomial checkpointing in `snapshots` state slots."""
+
+    def __init__(self, grid: CartesianGrid, nsteps: int, dt: float, snapshots: int = 45):
+        self.grid = grid
+        self.ctx = grid.ctx
+        self._lib = grid._lib
+        self.nsteps = int(nsteps)
+        self.dt = float(dt)
+        self.snapshots = int(snapshots)
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_calibration_create(self.ctx.handle, grid.handle, self.nsteps, self.dt,
+                                                    self.snapshots, C.byref(h)), self.ctx.handle)
+        self.handle = h
+        self.ctx._adopt(self)
+
+    def forward(self, params: ModelCParams, phi0: DeviceBuffer, u0: DeviceBuffer,
+                target: DeviceBuffer) -> float:
+        loss = C.c_double(0.0)
+        pc = params.c()
+        _abi.check(self._lib.lgd_calibration_forward(self.handle, C.byref(pc), phi0.handle, u0.handle,
+                                                     target.handle, C.byref(loss)), self.ctx.handle)
+        return loss.value
+
+    def backward(self, gphi0: Optional[DeviceBuffer] = None, gu0: Optional[DeviceBuffer] = None):
+        grad = np.zeros(3)
+        _abi.check(self._lib.lgd_calibration_backward(
+            self.handle, grad.ctypes.data_as(_abi.DP), gphi0.handle if gphi0 is not None else None,
+            gu0.handle if gu0 is not None else None), self.ctx.handle)
+        return grad
+
+    def counts(self):
+        f, r = C.c_int64(0), C.c_int64(0)
+        _abi.check(self._lib.lgd_calibration_counts(self.handle, C.byref(f), C.byref(r)), self.ctx.handle)
+        return f.value, r.value
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._lib.lgd_calibration_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def checkpoint_forward_steps(nsteps: int, snapshots: int) -> int:
+    """Forward steps of one forward + backward pass of ModelCCalibration."""
+    return int(_abi.lib().lgd_checkpoint_forward_steps(int(nsteps), int(snapshots)))
+

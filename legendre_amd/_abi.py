@@ -143,6 +143,17 @@ SIGNATURES = {
     "lgd_sim_time": (DBL, [P]),
     "lgd_sim_step_index": (I64, [P]),
     "lgd_sim_initialize_grains": (C.c_int, [P, DP, SZ, DBL]),
+    "lgd_adjoint_create": (C.c_int, [P, P, PP]),
+    "lgd_adjoint_destroy": (C.c_int, [P]),
+    "lgd_modelc_adjoint_step": (C.c_int, [P, C.POINTER(ModelCParamsC), P, P, DBL, P, P]),
+    "lgd_adjoint_reset": (C.c_int, [P]),
+    "lgd_adjoint_grad": (C.c_int, [P, DP]),
+    "lgd_calibration_create": (C.c_int, [P, P, I64, DBL, I32, PP]),
+    "lgd_calibration_destroy": (C.c_int, [P]),
+    "lgd_calibration_forward": (C.c_int, [P, C.POINTER(ModelCParamsC), P, P, P, DP]),
+    "lgd_calibration_backward": (C.c_int, [P, DP, P, P]),
+    "lgd_calibration_counts": (C.c_int, [P, C.POINTER(I64), C.POINTER(I64)]),
+    "lgd_checkpoint_forward_steps": (I64, [I64, I32]),
 }
 
 _lib = None
