@@ -127,14 +127,176 @@ def run_cpu_baseline(C, budget_s: float):
     }
 
 
+def run_adjoint(args) -> int:
+    """Calibration gradient (the adjoint workload): L = ½‖φ_T − φ_target‖² after
+    2000 steps on 1024² (16×16 blocks of 64²), dL/d(λ, D, ε₄, φ₀, u₀) by the device
+    adjoint with binomial checkpointing, timed against one plain forward run of the
+    same steps; checked against central differences of the device loss and the
+    oracle's tangent-linear run (four directions carried at once)."""
+    import numpy as np
+    import torch
+
+    import legendre_amd as lg
+    from legendre_amd.workloads import AdjointConfig
+
+    A = AdjointConfig(steps=args.adj_steps or AdjointConfig.steps,
+                      snapshots=args.snapshots or AdjointConfig.snapshots)
+    ctx = lg.Context(0)
+    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2}[args.kernel])
+    ctx.set_tuning("modelc_strips", args.strips)
+    grid = lg.CartesianGrid(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])
+    p = lg.ModelCParams(A.eps4, A.lambda_, A.d_thermal)
+    pt = lg.ModelCParams(A.target_eps4, A.target_lambda, A.target_d)
+    grains = A.grains()
+
+    def initial(params):
+        sim = lg.Simulation.model_c(grid, params)
+        sim.initialize_grains(grains, A.delta)
+        return sim
+
+    sim0 = initial(p)
+    phi0, u0 = grid.alloc_field(), grid.alloc_field()
+    phi0.copy_from(sim0.field(0))
+    u0.copy_from(sim0.field(1))
+    simt = initial(pt)
+    simt.step(A.dt, A.steps)
+    target = grid.alloc_field()
+    target.copy_from(simt.field(0))
+    simt.close()
+    ctx.sync()
+    # One plain forward run (warm-up run first).
+    sim0.step(A.dt, 20)
+    ctx.sync()
+    fwd = []
+    for _ in range(2):
+        sim = initial(p)
+        ctx.sync()
+        t = time.perf_counter()
+        sim.step(A.dt, A.steps)
+        ctx.sync()
+        fwd.append(time.perf_counter() - t)
+        sim.close()
+    t_fwd = min(fwd)
+    log(f"[adjoint] forward {A.steps} steps: {t_fwd * 1e3:.2f} ms")
+    cal = lg.ModelCCalibration(grid, A.steps, A.dt, snapshots=A.snapshots)
+    gphi, gu = grid.alloc_field(), grid.alloc_field()
+    runs = []
+    for _ in range(2):
+        ctx.sync()
+        t = time.perf_counter()
+        loss = cal.forward(p, phi0, u0, target)
+        grad = cal.backward(gphi, gu)
+        ctx.sync()
+        runs.append(time.perf_counter() - t)
+    t_grad = min(runs)
+    nfwd, nrev = cal.counts()
+    log(f"[adjoint] gradient: {t_grad * 1e3:.2f} ms ({nfwd} forward + {nrev} reverse steps), "
+        f"loss {loss:.10e}, grad {grad}")
+    # Central differences of the device loss.
+    base = dict(eps4=A.eps4, lambda_=A.lambda_, d_thermal=A.d_thermal)
+    fd = {}
+    for k, name in enumerate(["lambda_", "d_thermal", "eps4"]):
+        errs = {}
+        for h in (1e-6, 1e-7):
+            vals = []
+            for sgn in (1.0, -1.0):
+                q = dict(base)
+                q[name] += sgn * h
+                vals.append(cal.forward(lg.ModelCParams(q["eps4"], q["lambda_"], q["d_thermal"]),
+                                        phi0, u0, target))
+            d = (vals[0] - vals[1]) / (2 * h)
+            errs[f"h={h:g}"] = abs(d - grad[k]) / abs(grad[k])
+        fd[name.rstrip("_")] = errs
+    log(f"[adjoint] finite-difference relative errors: {fd}")
+    tlm = None
+    if not args.no_cpu:
+        import oracle
+
+        g = oracle.grid([A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])
+        op = oracle.params(A.eps4, A.lambda_, A.d_thermal)
+        ph, uu, tg = phi0.download(), u0.download(), target.download()
+        n = ph.size
+        rng = np.random.default_rng(0)
+        vu = rng.standard_normal(n)
+        dth = np.zeros((4, 3))
+        dth[0, 0] = dth[1, 1] = dth[2, 2] = 1.0
+        dp, du = np.zeros((4, n)), np.zeros((4, n))
+        du[3] = vu
+        t = time.perf_counter()
+        done, chunk = 0, 250
+        while done < A.steps:
+            m = min(chunk, A.steps - done)
+            oracle.modelc_tlm_run(g, op, ph, uu, dp, du, dth, m, A.dt)
+            done += m
+            log(f"[adjoint] oracle tangent-linear run: {done}/{A.steps} steps, "
+                f"{time.perf_counter() - t:.0f}s")
+        mask = np.zeros(n, dtype=bool)
+        b = A.block + 2
+        mask.reshape(-1, b, b)[:, 1:-1, 1:-1] = True
+        r = (ph - tg)[mask]
+        dl = np.array([float(np.sum(r * dp[k][mask])) for k in range(4)])
+        gu_h = gu.download()
+        adj = np.array([grad[0], grad[1], grad[2], float(np.sum(gu_h[mask] * vu[mask]))])
+        tlm = {
+            "rel_err": [float(abs(adj[k] - dl[k]) / abs(dl[k])) for k in range(4)],
+            "directions": ["lambda", "D", "eps4", "u0 (random field)"],
+            "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
+            "seconds": round(time.perf_counter() - t, 1),
+        }
+        log(f"[adjoint] tangent-linear check: {tlm}")
+    cal.close()
+    sim0.close()
+    ctx.close()
+    value = A.interior_cells * A.steps / t_grad / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": A.steps,
+        "warmup": 1,
+        "ms_per_step": round(t_grad / A.steps * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"modelc_adjoint_{A.cells}sq_{A.block}blocks_{A.steps}steps_{A.snapshots}snapshots",
+            "cells": A.interior_cells,
+            "theta": [A.lambda_, A.d_thermal, A.eps4],
+            "target_theta": [A.target_lambda, A.target_d, A.target_eps4],
+            "seeds": A.nseeds,
+        },
+        "adjoint": {
+            "forward_ms": round(t_fwd * 1e3, 3),
+            "gradient_ms": round(t_grad * 1e3, 3),
+            "gradient_over_forward": round(t_grad / t_fwd, 3),
+            "forward_steps": nfwd,
+            "reverse_steps": nrev,
+            "loss": loss,
+            "grad": [float(x) for x in grad],
+            "fd_rel_err": fd,
+            "tlm": tlm,
+        },
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["modelc", "adjoint"], default="modelc",
+                    help="modelc: the headline plain step; adjoint: the calibration gradient")
+    ap.add_argument("--adj-steps", type=int, default=0, help="adjoint workload steps (0 = 2000)")
+    ap.add_argument("--snapshots", type=int, default=0, help="adjoint checkpoint slots (0 = 45)")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--noise", type=float, default=0.0)
     ap.add_argument("--mode", choices=["fused", "per-intent"], default="fused")
     ap.add_argument("--cells", type=int, default=10080)
+    ap.add_argument("--block", type=int, default=126)
     ap.add_argument("--kernel", choices=["auto", "percell", "stream"], default="auto")
     ap.add_argument("--strips", type=int, default=0, help="row strips per block (0 = default)")
     ap.add_argument("--prefetch", type=int, default=0, choices=[0, 1, 2],
@@ -150,9 +312,12 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    if args.workload == "adjoint":
+        return run_adjoint(args)
+
     from legendre_amd.workloads import ModelCConfig
 
-    C = ModelCConfig(cells=args.cells, noise=args.noise)
+    C = ModelCConfig(cells=args.cells, block=args.block, noise=args.noise)
     total_ms, per, wall = run_gpu(args, C)
     ms_per_step = total_ms / args.steps
     value = C.interior_cells * args.steps / (total_ms / 1e3) / 1e9
@@ -162,7 +327,8 @@ def main() -> int:
     log(f"[bench] {args.steps} steps: {total_ms:.3f} ms on-stream, {wall * 1e3:.3f} ms wall; "
         f"per launch min {min(per):.4f} max {max(per):.4f} ms")
     traffic = read_traffic(os.path.join(ROOT, "profiles", "modelc_stream_kernel_10080sq.md"))
-    if args.cells != 10080 or args.noise != 0.0 or args.kernel == "percell" or args.mode != "fused":
+    if (args.cells != 10080 or args.block != 126 or args.noise != 0.0 or args.kernel == "percell"
+            or args.mode != "fused"):
         traffic = None  # the committed profile covers the default workload only
     cpu = None
     if not args.no_cpu:

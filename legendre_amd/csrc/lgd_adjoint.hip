@@ -34,29 +34,26 @@
 namespace {
 
 struct AdjGrid {
-  int64_t N0, N1;   // global interior cells
-  int64_t n0, n1;   // block interior cells
-  int64_t nb0;      // blocks per row
-  int64_t sx, slab; // slab row length and slab length (ghost 1)
+  int N0, N1;    // global interior cells
+  int n0, n1;    // block interior cells
+  int nb0;       // blocks per row
+  int sx;        // slab row length (ghost 1)
+  int64_t slab;  // slab length
   int per0, per1;
+  int P0;        // padded row length N0 + 2
 };
 
-__device__ __forceinline__ int64_t resolve(int64_t i, int64_t n, int per) {
+__device__ __forceinline__ int resolve(int i, int n, int per) {
   if (i < 0) return per ? i + n : -1 - i;
   if (i >= n) return per ? i - n : 2 * n - 1 - i;
   return i;
 }
 
 // Slab offset of global interior cell (gi, gj) (both in range).
-__device__ __forceinline__ int64_t slab_off(const AdjGrid &G, int64_t gi, int64_t gj) {
-  int64_t bx = gi / G.n0, by = gj / G.n1;
-  int64_t i = gi - bx * G.n0, j = gj - by * G.n1;
-  return (bx + G.nb0 * by) * G.slab + (j + 1) * G.sx + (i + 1);
-}
-
-// Field value at any position within one cell of the domain, resolved per dimension.
-__device__ __forceinline__ double at(const double *f, const AdjGrid &G, int64_t gi, int64_t gj) {
-  return f[slab_off(G, resolve(gi, G.N0, G.per0), resolve(gj, G.N1, G.per1))];
+__device__ __forceinline__ int64_t slab_off(const AdjGrid &G, int gi, int gj) {
+  const int bx = gi / G.n0, by = gj / G.n1;
+  const int i = gi - bx * G.n0, j = gj - by * G.n1;
+  return (int64_t)(bx + G.nb0 * by) * G.slab + (int64_t)((j + 1) * G.sx + (i + 1));
 }
 
 // face(dx, dy) (finite_volume.rs:129-142) with its partial derivatives w.r.t. dx, dy
@@ -96,137 +93,186 @@ __device__ __forceinline__ FaceD face_deriv(const lgd_kr_consts &k, double tol, 
   return f;
 }
 
-// Pass 1: per interior cell.
-__global__ void adj_cell_kernel(AdjGrid G, lgd_kr_consts k, double dt, const double *phi,
-                                const double *u, const double *pphi, const double *pu, double *R,
-                                double *SX, double *SY, double *PW, double *PU, double *V,
-                                double *gl, double *gd, double *ge) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= G.N0 * G.N1) return;
-  const int64_t i = t % G.N0, j = t / G.N0;
-  double p[3][3];
-  for (int dj = -1; dj <= 1; ++dj)
-    for (int di = -1; di <= 1; ++di) p[dj + 1][di + 1] = at(phi, G, i + di, j + dj);
-  const int64_t o = slab_off(G, i, j);
-  const double uc = u[o];
-  const double ue = at(u, G, i + 1, j), uw = at(u, G, i - 1, j);
-  const double un = at(u, G, i, j + 1), us = at(u, G, i, j - 1);
-  // The forward quantities, in the forward's operation order.
-  double dphi, du_unused;
-  lgd_modelc_cell(k, p, uc, ue, uw, un, us, dphi, du_unused);
-  (void)du_unused;
-  const double pv = p[1][1];
-  const double one_m_p2 = fma(pv, -pv, 1.0);
-  const double p_prime = one_m_p2 * one_m_p2;
-  const FaceD ca = face_deriv(k, k.model_tol, 0.5 * (p[1][2] - p[1][0]), 0.5 * (p[2][1] - p[0][1]));
-  const double A = ca.a;
-  double acc = 0.0;
-  acc = acc + (((ue - (uc + uc)) + uw) * k.inv_hx2);
-  acc = acc + (((un - (uc + uc)) + us) * k.inv_hy2);
-  // Reverse: ∂L/∂F = dt·(P_φ' + ½P_u'), ∂L/∂acc = dt·D·P_u'.
-  const double ap = pphi[o], au = pu[o];
-  const double W = dt * (ap + 0.5 * au);
-  const double Rr = W / (A * A);
-  const double S = -2.0 * W * dphi / A;
-  R[t] = Rr;
-  SX[t] = S * ca.a_x * 0.5;
-  SY[t] = S * ca.a_y * 0.5;
-  PW[t] = ap + Rr * (-(3.0 * pv * pv - 1.0) - k.lambda * uc * (2.0 * one_m_p2 * (-2.0 * pv)));
-  PU[t] = au + Rr * (-k.lambda * p_prime);
-  V[t] = dt * k.d_thermal * au;
-  gl[t] += Rr * (-uc * p_prime);
-  gd[t] += dt * au * acc;
-  ge[t] += S * ca.a_e;
+constexpr int kWG = 256;
+
+// Fixed-order workgroup sum of kN values per thread; thread 0 adds the totals to
+// this workgroup's accumulator slots (acc[kN*blockIdx.x + n]), so across launches
+// each slot sums in step order: bitwise reproducible.
+template <int kN>
+__device__ __forceinline__ void wg_accumulate(const double (&v)[kN], double *acc) {
+  __shared__ double part[kN][kWG];
+  for (int n = 0; n < kN; ++n) part[n][threadIdx.x] = v[n];
+  __syncthreads();
+  for (int w = kWG / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int n = 0; n < kN; ++n) part[n][threadIdx.x] += part[n][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int n = 0; n < kN; ++n) acc[kN * blockIdx.x + n] += part[n][0];
+}
+
+// Stage: φ and u on the padded global grid ((N0+2) × (N1+2); the ring holds the
+// resolved neighbour values, i.e. the ghost fill), and the incoming adjoint on the
+// unpadded global grid.
+__global__ __launch_bounds__(kWG) void adj_stage_kernel(AdjGrid G, const double *phi,
+                                                        const double *u, const double *pphi,
+                                                        const double *pu, double *Pp, double *Up,
+                                                        double *AP, double *AU) {
+  const int t = blockIdx.x * kWG + threadIdx.x;
+  if (t >= G.P0 * (G.N1 + 2)) return;
+  const int x = t % G.P0 - 1, y = t / G.P0 - 1;
+  const int rx = resolve(x, G.N0, G.per0), ry = resolve(y, G.N1, G.per1);
+  const int64_t o = slab_off(G, rx, ry);
+  Pp[t] = phi[o];
+  Up[t] = u[o];
+  if (x == rx && y == ry) {
+    AP[x + G.N0 * y] = pphi[o];
+    AU[x + G.N0 * y] = pu[o];
+  }
+}
+
+// Pass 1: per interior cell.  acc gets (∂L/∂λ, ∂L/∂D, ∂L/∂ε₄ centre part) per workgroup.
+__global__ __launch_bounds__(kWG) void adj_cell_kernel(AdjGrid G, lgd_kr_consts k, double dt,
+                                                       const double *Pp, const double *Up,
+                                                       const double *AP, const double *AU,
+                                                       double *R, double *SX, double *SY,
+                                                       double *PW, double *PU, double *V,
+                                                       double *acc) {
+  const int t = blockIdx.x * kWG + threadIdx.x;
+  double part[3] = {0.0, 0.0, 0.0};
+  if (t < G.N0 * G.N1) {
+    const int i = t % G.N0, j = t / G.N0;
+    const int c = (i + 1) + G.P0 * (j + 1);
+    double p[3][3];
+    for (int dj = -1; dj <= 1; ++dj)
+      for (int di = -1; di <= 1; ++di) p[dj + 1][di + 1] = Pp[c + di + G.P0 * dj];
+    const double uc = Up[c], ue = Up[c + 1], uw = Up[c - 1];
+    const double un = Up[c + G.P0], us = Up[c - G.P0];
+    // The forward quantities, in the forward's operation order.
+    double F, du_unused;
+    lgd_modelc_cell(k, p, uc, ue, uw, un, us, F, du_unused);
+    (void)du_unused;
+    const double pv = p[1][1];
+    const double one_m_p2 = fma(pv, -pv, 1.0);
+    const double p_prime = one_m_p2 * one_m_p2;
+    const FaceD ca =
+        face_deriv(k, k.model_tol, 0.5 * (p[1][2] - p[1][0]), 0.5 * (p[2][1] - p[0][1]));
+    const double A = ca.a;
+    double lap = 0.0;
+    lap = lap + (((ue - (uc + uc)) + uw) * k.inv_hx2);
+    lap = lap + (((un - (uc + uc)) + us) * k.inv_hy2);
+    // Reverse: ∂L/∂F = dt·(P_φ' + ½P_u'), ∂L/∂lap = dt·D·P_u'.
+    const double ap = AP[t], au = AU[t];
+    const double W = dt * (ap + 0.5 * au);
+    const double Rr = W / (A * A);
+    const double S = -2.0 * W * F / A;
+    R[t] = Rr;
+    SX[t] = S * ca.a_x * 0.5;
+    SY[t] = S * ca.a_y * 0.5;
+    PW[t] = ap + Rr * (-(3.0 * pv * pv - 1.0) - k.lambda * uc * (2.0 * one_m_p2 * (-2.0 * pv)));
+    PU[t] = au + Rr * (-k.lambda * p_prime);
+    V[t] = dt * k.d_thermal * au;
+    part[0] = Rr * (-uc * p_prime);
+    part[1] = dt * au * lap;
+    part[2] = S * ca.a_e;
+  }
+  wg_accumulate<3>(part, acc);
 }
 
 // Pass 2: faces.  Thread (xp, yp) over [0, N0] × [0, N1] owns the x-face between
 // global columns xp−1 and xp at row yp (yp < N1) and the y-face between rows yp−1
-// and yp at column xp (xp < N0).
-__global__ void adj_face_kernel(AdjGrid G, lgd_kr_consts k, const double *phi, const double *R,
-                                double *GXx, double *GYx, double *GXy, double *GYy, double *gex,
-                                double *gey) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t W0 = G.N0 + 1;
-  if (t >= W0 * (G.N1 + 1)) return;
-  const int64_t xp = t % W0, yp = t / W0;
-  const double q = 0.25;
-  auto Rin = [&](int64_t i, int64_t j) {
-    return (i >= 0 && i < G.N0 && j >= 0 && j < G.N1) ? R[i + G.N0 * j] : 0.0;
-  };
-  if (yp < G.N1) {
-    const int64_t i = xp - 1, j = yp;
-    const double Gf = (Rin(i, j) - Rin(i + 1, j)) * k.inv_h2;
-    double gx = 0.0, gy = 0.0, g_e = 0.0;
-    if (Gf != 0.0) {
-      const double derx = at(phi, G, i + 1, j) - at(phi, G, i, j);
-      const double dery = q * (((at(phi, G, i + 1, j + 1) - at(phi, G, i + 1, j - 1)) +
-                                at(phi, G, i, j + 1)) - at(phi, G, i, j - 1));
-      const FaceD h = face_deriv(k, k.fv_tol, derx, dery);
-      const double inner = h.a * derx - h.da * dery;  // J = a·inner
-      const double j_x = h.a_x * inner + h.a * (h.a_x * derx + h.a - h.da_x * dery);
-      const double j_y = h.a_y * inner + h.a * (h.a_y * derx - h.da_y * dery - h.da);
-      const double j_e = h.a_e * inner + h.a * (h.a_e * derx - h.da_e * dery);
-      gx = Gf * j_x;
-      gy = Gf * j_y;
-      g_e = Gf * j_e;
+// and yp at column xp (xp < N0).  acc gets the faces' ∂L/∂ε₄ per workgroup.
+__global__ __launch_bounds__(kWG) void adj_face_kernel(AdjGrid G, lgd_kr_consts k,
+                                                       const double *Pp, const double *R,
+                                                       double *GXx, double *GYx, double *GXy,
+                                                       double *GYy, double *acc) {
+  const int t = blockIdx.x * kWG + threadIdx.x;
+  const int W0 = G.N0 + 1;
+  double part[1] = {0.0};
+  if (t < W0 * (G.N1 + 1)) {
+    const int xp = t % W0, yp = t / W0;
+    const double q = 0.25;
+    auto Rin = [&](int i, int j) {
+      return (i >= 0 && i < G.N0 && j >= 0 && j < G.N1) ? R[i + G.N0 * j] : 0.0;
+    };
+    // padded value at extended position (x, y) ∈ [−1, N0] × [−1, N1]
+    auto P = [&](int x, int y) { return Pp[(x + 1) + G.P0 * (y + 1)]; };
+    if (yp < G.N1) {
+      const int i = xp - 1, j = yp;
+      const double Gf = (Rin(i, j) - Rin(i + 1, j)) * k.inv_h2;
+      double gx = 0.0, gy = 0.0;
+      if (Gf != 0.0) {
+        const double derx = P(i + 1, j) - P(i, j);
+        const double dery = q * (((P(i + 1, j + 1) - P(i + 1, j - 1)) + P(i, j + 1)) - P(i, j - 1));
+        const FaceD h = face_deriv(k, k.fv_tol, derx, dery);
+        const double inner = h.a * derx - h.da * dery;  // J = a·inner
+        const double j_x = h.a_x * inner + h.a * (h.a_x * derx + h.a - h.da_x * dery);
+        const double j_y = h.a_y * inner + h.a * (h.a_y * derx - h.da_y * dery - h.da);
+        const double j_e = h.a_e * inner + h.a * (h.a_e * derx - h.da_e * dery);
+        gx = Gf * j_x;
+        gy = Gf * j_y;
+        part[0] += Gf * j_e;
+      }
+      const int fx = xp + W0 * yp;
+      GXx[fx] = gx;
+      GYx[fx] = gy;
     }
-    const int64_t fx = xp + W0 * yp;
-    GXx[fx] = gx;
-    GYx[fx] = gy;
-    gex[fx] += g_e;
-  }
-  if (xp < G.N0) {
-    const int64_t i = xp, j = yp - 1;
-    const double Gf = (Rin(i, j) - Rin(i, j + 1)) * k.inv_h2;
-    double gx = 0.0, gy = 0.0, g_e = 0.0;
-    if (Gf != 0.0) {
-      const double derx = q * (((at(phi, G, i + 1, j + 1) - at(phi, G, i - 1, j + 1)) +
-                                at(phi, G, i + 1, j)) - at(phi, G, i - 1, j));
-      const double dery = at(phi, G, i, j + 1) - at(phi, G, i, j);
-      const FaceD h = face_deriv(k, k.fv_tol, derx, dery);
-      const double inner = h.a * dery + h.da * derx;  // J = a·inner
-      const double j_x = h.a_x * inner + h.a * (h.a_x * dery + h.da_x * derx + h.da);
-      const double j_y = h.a_y * inner + h.a * (h.a_y * dery + h.a + h.da_y * derx);
-      const double j_e = h.a_e * inner + h.a * (h.a_e * dery + h.da_e * derx);
-      gx = Gf * j_x;
-      gy = Gf * j_y;
-      g_e = Gf * j_e;
+    if (xp < G.N0) {
+      const int i = xp, j = yp - 1;
+      const double Gf = (Rin(i, j) - Rin(i, j + 1)) * k.inv_h2;
+      double gx = 0.0, gy = 0.0;
+      if (Gf != 0.0) {
+        const double derx = q * (((P(i + 1, j + 1) - P(i - 1, j + 1)) + P(i + 1, j)) - P(i - 1, j));
+        const double dery = P(i, j + 1) - P(i, j);
+        const FaceD h = face_deriv(k, k.fv_tol, derx, dery);
+        const double inner = h.a * dery + h.da * derx;  // J = a·inner
+        const double j_x = h.a_x * inner + h.a * (h.a_x * dery + h.da_x * derx + h.da);
+        const double j_y = h.a_y * inner + h.a * (h.a_y * dery + h.a + h.da_y * derx);
+        const double j_e = h.a_e * inner + h.a * (h.a_e * dery + h.da_e * derx);
+        gx = Gf * j_x;
+        gy = Gf * j_y;
+        part[0] += Gf * j_e;
+      }
+      const int fy = xp + G.N0 * yp;
+      GXy[fy] = gx;
+      GYy[fy] = gy;
     }
-    const int64_t fy = xp + G.N0 * yp;
-    GXy[fy] = gx;
-    GYy[fy] = gy;
-    gey[fy] += g_e;
   }
+  wg_accumulate<1>(part, acc);
 }
 
 // Pass 3: gather at each interior cell.
-__global__ void adj_gather_kernel(AdjGrid G, lgd_kr_consts k, const double *GXx,
-                                  const double *GYx, const double *GXy, const double *GYy,
-                                  const double *SX, const double *SY, const double *PW,
-                                  const double *PU, const double *V, double *pphi, double *pu) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(kWG) void adj_gather_kernel(AdjGrid G, lgd_kr_consts k,
+                                                         const double *GXx, const double *GYx,
+                                                         const double *GXy, const double *GYy,
+                                                         const double *SX, const double *SY,
+                                                         const double *PW, const double *PU,
+                                                         const double *V, double *pphi,
+                                                         double *pu) {
+  const int t = blockIdx.x * kWG + threadIdx.x;
   if (t >= G.N0 * G.N1) return;
-  const int64_t a = t % G.N0, b = t / G.N0;
-  const int64_t N0 = G.N0, N1 = G.N1, W0 = N0 + 1;
+  const int a = t % G.N0, b = t / G.N0;
+  const int N0 = G.N0, N1 = G.N1, W0 = N0 + 1;
   const double q = 0.25;
   // x-face (x|x+1) at row y, x ∈ [−1, N0−1]; y-face (y|y+1) at column x, y ∈ [−1, N1−1].
-  auto gxx = [&](int64_t x, int64_t y) {
+  auto gxx = [&](int x, int y) {
     return (x >= -1 && x < N0 && y >= 0 && y < N1) ? GXx[(x + 1) + W0 * y] : 0.0;
   };
-  auto gyx = [&](int64_t x, int64_t y) {
+  auto gyx = [&](int x, int y) {
     return (x >= -1 && x < N0 && y >= 0 && y < N1) ? GYx[(x + 1) + W0 * y] : 0.0;
   };
-  auto gxy = [&](int64_t x, int64_t y) {
+  auto gxy = [&](int x, int y) {
     return (x >= 0 && x < N0 && y >= -1 && y < N1) ? GXy[x + N0 * (y + 1)] : 0.0;
   };
-  auto gyy = [&](int64_t x, int64_t y) {
+  auto gyy = [&](int x, int y) {
     return (x >= 0 && x < N0 && y >= -1 && y < N1) ? GYy[x + N0 * (y + 1)] : 0.0;
   };
-  auto inner = [&](const double *f, int64_t x, int64_t y) {
+  auto inner = [&](const double *f, int x, int y) {
     return (x >= 0 && x < N0 && y >= 0 && y < N1) ? f[x + N0 * y] : 0.0;
   };
   // Everything that reads the value at extended position (x, y).
-  auto contrib = [&](int64_t x, int64_t y) {
+  auto contrib = [&](int x, int y) {
     double xf = (gxx(x - 1, y) - gxx(x, y)) +
                 q * ((gyx(x, y - 1) + gyx(x - 1, y - 1)) - (gyx(x, y + 1) + gyx(x - 1, y + 1)));
     double yf = (gyy(x, y - 1) - gyy(x, y)) +
@@ -236,7 +282,7 @@ __global__ void adj_gather_kernel(AdjGrid G, lgd_kr_consts k, const double *GXx,
     return (xf + yf) + cf;
   };
   // Positions along each dimension that resolve to this cell.
-  int64_t xs[3], ys[3];
+  int xs[3], ys[3];
   int nx = 0, ny = 0;
   xs[nx++] = a;
   if (a == 0) xs[nx++] = G.per0 ? N0 : -1;
@@ -266,7 +312,7 @@ __global__ void adj_zero_kernel(int64_t n, double *x) {
 // Terminal adjoint of L = ½‖φ − target‖²: P_φ = φ − target, P_u = 0, r² per cell.
 __global__ void adj_loss_kernel(AdjGrid G, const double *phi, const double *target, double *pphi,
                                 double *pu, double *r2) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G.N0 * G.N1) return;
   const int64_t o = slab_off(G, t % G.N0, t / G.N0);
   const double r = phi[o] - target[o];
@@ -277,22 +323,25 @@ __global__ void adj_loss_kernel(AdjGrid G, const double *phi, const double *targ
 
 // Fixed-order sum of n values into out[0]: one workgroup, strided partial sums,
 // then a fixed tree.
+// Fixed-order sums of n records of `stride` values (record i at x[stride*i]):
+// out[c] = Σ_i x[stride*i + c] for c < stride.
 constexpr int kSumThreads = 1024;
 __global__ __launch_bounds__(kSumThreads) void adj_sum_kernel(int64_t n, const double *x,
-                                                              double *out) {
+                                                              int stride, double *out) {
   __shared__ double part[kSumThreads];
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kSumThreads) s += x[i];
-  part[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = kSumThreads / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+  for (int c = 0; c < stride; ++c) {
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += kSumThreads) s += x[stride * i + c];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kSumThreads / 2; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = part[0];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[0] = part[0];
 }
-
-constexpr int kWG = 256;
 
 // Binomial checkpointing cost: forward steps to reverse n steps from a stored
 // state with s further free slots (T(n,0) = n(n−1)/2; otherwise r·n − β(s+2, r−1)
@@ -339,13 +388,15 @@ struct lgd_adjoint {
   lgd_ctx *ctx = nullptr;
   const lgd_grid *grid = nullptr;
   AdjGrid G{};
-  int64_t ncell = 0, nfx = 0, nfy = 0;
-  // Scratch (global layout).
+  int64_t ncell = 0, nfx = 0, nfy = 0, npad = 0;
+  int64_t wg_cell = 0, wg_face = 0;  // workgroups of the cell and face passes
+  // Scratch (global layout): padded state, incoming adjoint, per-cell and per-face terms.
+  double *Pp = nullptr, *Up = nullptr, *AP = nullptr, *AU = nullptr;
   double *R = nullptr, *SX = nullptr, *SY = nullptr, *PW = nullptr, *PU = nullptr, *V = nullptr;
   double *GXx = nullptr, *GYx = nullptr, *GXy = nullptr, *GYy = nullptr;
-  // θ accumulators: per cell (λ, D, ε₄ centre part) and per face (ε₄ face parts).
-  double *gl = nullptr, *gd = nullptr, *ge = nullptr, *gex = nullptr, *gey = nullptr;
-  double *sums = nullptr;  // 6 reduction outputs
+  // θ accumulators, per workgroup: cell pass (λ, D, ε₄ centre) and face pass (ε₄ faces).
+  double *acc_cell = nullptr, *acc_face = nullptr;
+  double *sums = nullptr;  // reduction outputs
   std::vector<double *> allocs;
 };
 
@@ -367,9 +418,13 @@ int zero(lgd_ctx *ctx, double *x, int64_t n) {
   return launch_check(ctx, "adjoint zero");
 }
 
-int sum_into(lgd_ctx *ctx, int64_t n, const double *x, double *out) {
-  adj_sum_kernel<<<1, kSumThreads, 0, ctx->stream>>>(n, x, out);
+int sum_strided(lgd_ctx *ctx, int64_t n, const double *x, int stride, double *out) {
+  adj_sum_kernel<<<1, kSumThreads, 0, ctx->stream>>>(n, x, stride, out);
   return launch_check(ctx, "adjoint sum");
+}
+
+int sum_into(lgd_ctx *ctx, int64_t n, const double *x, double *out) {
+  return sum_strided(ctx, n, x, 1, out);
 }
 
 }  // namespace
@@ -386,28 +441,40 @@ int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out) {
   a->ctx = ctx;
   a->grid = grid;
   AdjGrid &G = a->G;
-  G.n0 = c.n[0];
-  G.n1 = c.n[1];
-  G.nb0 = c.nb[0];
-  G.N0 = c.n[0] * c.nb[0];
-  G.N1 = c.n[1] * c.nb[1];
-  G.sx = c.n[0] + 2;
-  G.slab = G.sx * (c.n[1] + 2);
+  // Flat indices of the global scratch arrays are 32-bit.
+  const int64_t N0 = c.n[0] * c.nb[0], N1 = c.n[1] * c.nb[1];
+  if ((N0 + 2) * (N1 + 2) >= ((int64_t)1 << 31)) {
+    delete a;
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "adjoint domain above 2^31 cells");
+  }
+  G.n0 = (int)c.n[0];
+  G.n1 = (int)c.n[1];
+  G.nb0 = (int)c.nb[0];
+  G.N0 = (int)N0;
+  G.N1 = (int)N1;
+  G.sx = (int)(c.n[0] + 2);
+  G.slab = (int64_t)G.sx * (c.n[1] + 2);
   G.per0 = c.periodic[0];
   G.per1 = c.periodic[1];
-  a->ncell = G.N0 * G.N1;
-  a->nfx = (G.N0 + 1) * G.N1;
-  a->nfy = G.N0 * (G.N1 + 1);
+  G.P0 = G.N0 + 2;
+  a->ncell = (int64_t)G.N0 * G.N1;
+  a->nfx = (int64_t)(G.N0 + 1) * G.N1;
+  a->nfy = (int64_t)G.N0 * (G.N1 + 1);
+  a->npad = (int64_t)G.P0 * (G.N1 + 2);
+  a->wg_cell = (a->ncell + kWG - 1) / kWG;
+  a->wg_face = ((int64_t)(G.N0 + 1) * (G.N1 + 1) + kWG - 1) / kWG;
   int rc = LGD_OK;
-  double **cells[] = {&a->R, &a->SX, &a->SY, &a->PW, &a->PU, &a->V, &a->gl, &a->gd, &a->ge};
+  double **cells[] = {&a->AP, &a->AU, &a->R, &a->SX, &a->SY, &a->PW, &a->PU, &a->V};
   for (double **p : cells)
     if (!rc) rc = adj_alloc(a, a->ncell, p);
+  if (!rc) rc = adj_alloc(a, a->npad, &a->Pp);
+  if (!rc) rc = adj_alloc(a, a->npad, &a->Up);
   if (!rc) rc = adj_alloc(a, a->nfx, &a->GXx);
   if (!rc) rc = adj_alloc(a, a->nfx, &a->GYx);
-  if (!rc) rc = adj_alloc(a, a->nfx, &a->gex);
   if (!rc) rc = adj_alloc(a, a->nfy, &a->GXy);
   if (!rc) rc = adj_alloc(a, a->nfy, &a->GYy);
-  if (!rc) rc = adj_alloc(a, a->nfy, &a->gey);
+  if (!rc) rc = adj_alloc(a, 3 * a->wg_cell, &a->acc_cell);
+  if (!rc) rc = adj_alloc(a, a->wg_face, &a->acc_face);
   if (!rc) rc = adj_alloc(a, 8, &a->sums);
   if (!rc) rc = lgd_adjoint_reset(a);
   if (rc) {
@@ -428,11 +495,8 @@ int lgd_adjoint_destroy(lgd_adjoint *a) {
 int lgd_adjoint_reset(lgd_adjoint *a) {
   if (!a) return LGD_ERR_INVALID_ARGUMENT;
   int rc;
-  if ((rc = zero(a->ctx, a->gl, a->ncell))) return rc;
-  if ((rc = zero(a->ctx, a->gd, a->ncell))) return rc;
-  if ((rc = zero(a->ctx, a->ge, a->ncell))) return rc;
-  if ((rc = zero(a->ctx, a->gex, a->nfx))) return rc;
-  return zero(a->ctx, a->gey, a->nfy);
+  if ((rc = zero(a->ctx, a->acc_cell, 3 * a->wg_cell))) return rc;
+  return zero(a->ctx, a->acc_face, a->wg_face);
 }
 
 int lgd_modelc_adjoint_step(lgd_adjoint *a, const lgd_modelc_params *params, const lgd_buf *phi,
@@ -448,14 +512,14 @@ int lgd_modelc_adjoint_step(lgd_adjoint *a, const lgd_modelc_params *params, con
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "adjoint buffers must be distinct from the state");
   const lgd_kr_consts k = lgd_make_kr_consts(*params, a->grid->cart);
   const AdjGrid &G = a->G;
-  adj_cell_kernel<<<blocks_for(a->ncell, kWG), kWG, 0, ctx->stream>>>(
-      G, k, dt, phi->data, u->data, pphi->data, pu->data, a->R, a->SX, a->SY, a->PW, a->PU, a->V,
-      a->gl, a->gd, a->ge);
+  adj_stage_kernel<<<blocks_for(a->npad, kWG), kWG, 0, ctx->stream>>>(
+      G, phi->data, u->data, pphi->data, pu->data, a->Pp, a->Up, a->AP, a->AU);
+  if ((rc = launch_check(ctx, "adjoint stage"))) return rc;
+  adj_cell_kernel<<<(unsigned)a->wg_cell, kWG, 0, ctx->stream>>>(
+      G, k, dt, a->Pp, a->Up, a->AP, a->AU, a->R, a->SX, a->SY, a->PW, a->PU, a->V, a->acc_cell);
   if ((rc = launch_check(ctx, "adjoint cells"))) return rc;
-  const int64_t nf = (G.N0 + 1) * (G.N1 + 1);
-  adj_face_kernel<<<blocks_for(nf, kWG), kWG, 0, ctx->stream>>>(G, k, phi->data, a->R, a->GXx,
-                                                                a->GYx, a->GXy, a->GYy, a->gex,
-                                                                a->gey);
+  adj_face_kernel<<<(unsigned)a->wg_face, kWG, 0, ctx->stream>>>(G, k, a->Pp, a->R, a->GXx, a->GYx,
+                                                                 a->GXy, a->GYy, a->acc_face);
   if ((rc = launch_check(ctx, "adjoint faces"))) return rc;
   adj_gather_kernel<<<blocks_for(a->ncell, kWG), kWG, 0, ctx->stream>>>(
       G, k, a->GXx, a->GYx, a->GXy, a->GYy, a->SX, a->SY, a->PW, a->PU, a->V, pphi->data, pu->data);
@@ -466,18 +530,15 @@ int lgd_adjoint_grad(lgd_adjoint *a, double *grad) {
   if (!a || !grad) return LGD_ERR_INVALID_ARGUMENT;
   lgd_ctx *ctx = a->ctx;
   int rc;
-  if ((rc = sum_into(ctx, a->ncell, a->gl, a->sums + 0))) return rc;
-  if ((rc = sum_into(ctx, a->ncell, a->gd, a->sums + 1))) return rc;
-  if ((rc = sum_into(ctx, a->ncell, a->ge, a->sums + 2))) return rc;
-  if ((rc = sum_into(ctx, a->nfx, a->gex, a->sums + 3))) return rc;
-  if ((rc = sum_into(ctx, a->nfy, a->gey, a->sums + 4))) return rc;
-  double h[5];
+  if ((rc = sum_strided(ctx, a->wg_cell, a->acc_cell, 3, a->sums))) return rc;
+  if ((rc = sum_strided(ctx, a->wg_face, a->acc_face, 1, a->sums + 3))) return rc;
+  double h[4];
   hipError_t e = hipMemcpyAsync(h, a->sums, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "adjoint gradient download");
   grad[0] = h[0];
   grad[1] = h[1];
-  grad[2] = (h[2] + h[3]) + h[4];
+  grad[2] = h[2] + h[3];
   return LGD_OK;
 }
 

@@ -5,6 +5,10 @@ heat3d   128³ cells as 4×4×4 blocks of 32³, periodic, D = 1, dx = 1/128,
 model_c  10080² cells as 80×80 blocks of 126², h = 0.4, dt = 0.008, λ = 3.19,
          D = 2.0, ε₄ = 0.05, u = −0.55, 64 seeds of radius 5h placed by the
          seed_positions recipe (examples/model_c.rs:175-206), mirror faces.
+adjoint  1024² cells as 16×16 blocks of 64², b = 0, 2000 steps, L = ½‖φ_T − φ_target‖²
+         with φ_target from a forward run at (λ, D, ε₄) = (3.4, 2.1, 0.06); gradient at
+         (3.19, 2.0, 0.05) w.r.t. (λ, D, ε₄) and the initial fields; 45 snapshots;
+         16 seeds of radius 5h by the same seed_positions recipe, u = −0.55.
 """
 from __future__ import annotations
 
@@ -106,3 +110,31 @@ def _fma(a: float, b: float, c: float) -> float:
 
 HEAT3D = Heat3dConfig()
 MODEL_C = ModelCConfig()
+
+
+@dataclasses.dataclass(frozen=True)
+class AdjointConfig:
+    cells: int = 1024
+    block: int = 64
+    h: float = 0.4
+    dt: float = 0.008
+    steps: int = 2000
+    snapshots: int = 45
+    lambda_: float = 3.19
+    d_thermal: float = 2.0
+    eps4: float = 0.05
+    target_lambda: float = 3.4
+    target_d: float = 2.1
+    target_eps4: float = 0.06
+    delta: float = 0.55
+    nseeds: int = 16
+    seed: int = 42
+
+    @property
+    def interior_cells(self) -> int:
+        return self.cells * self.cells
+
+    def grains(self) -> np.ndarray:
+        return ModelCConfig(cells=self.cells, block=self.block, h=self.h, nseeds=self.nseeds,
+                            seed=self.seed).grains()
+

@@ -79,8 +79,8 @@ _SIGS = {
     "orc_heat_run": (C.c_int, [GP, C.c_double, DP, C.c_int64, C.c_double, DP, C.POINTER(FaceBc), C.c_int32]),
     "orc_initialize_grains": (None, [GP, DP, DP, DP, C.c_size_t, C.c_double]),
     "orc_seed_positions": (None, [C.c_uint64, C.c_size_t, C.c_double, C.c_double, DP]),
-    "orc_modelc_tlm_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, DP, DP, DP, C.c_int64,
-                                     C.c_double]),
+    "orc_modelc_tlm_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_int32, DP, DP, DP,
+                                     C.c_int64, C.c_double]),
     "orc_modelc_adjoint_step": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_double, DP, DP,
                                           DP]),
     "orc_modelc_gradient": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, DP, C.c_int64, C.c_double,
@@ -259,11 +259,16 @@ def seed_positions(seed: int, nseeds: int, extent: float, radius: float) -> np.n
 
 # ---- Model C derivatives (checkers for the device adjoint) ------------------------
 def modelc_tlm_run(g: Grid, p: ModelCParams, phi, u, dphi, du, dtheta, nsteps, dt) -> None:
-    """Forward-Euler Model C (b = 0) with tangents along dtheta = (dλ, dD, dε₄):
-    values bitwise those of modelc_run, tangents the exact derivative."""
+    """Forward-Euler Model C (b = 0) with tangents: values bitwise those of
+    modelc_run, tangents the exact derivative.  One direction: dphi, du are field
+    buffers and dtheta = (dλ, dD, dε₄); up to four at once: dphi, du of shape
+    (ndir, len) and dtheta of shape (ndir, 3)."""
     th = np.ascontiguousarray(dtheta, dtype=np.float64)
-    rc = lib().orc_modelc_tlm_run(C.byref(g), C.byref(p), _dp(phi), _dp(u), _dp(dphi), _dp(du),
-                                  _dp(th), nsteps, dt)
+    ndir = 1 if th.ndim == 1 else th.shape[0]
+    assert dphi.flags["C_CONTIGUOUS"] and du.flags["C_CONTIGUOUS"]
+    assert dphi.size == ndir * phi.size and du.size == ndir * u.size
+    rc = lib().orc_modelc_tlm_run(C.byref(g), C.byref(p), _dp(phi), _dp(u), ndir,
+                                  dphi.ctypes.data_as(DP), du.ctypes.data_as(DP), _dp(th), nsteps, dt)
     if rc != 0:
         raise ValueError("orc_modelc_tlm_run rejected the configuration")
 

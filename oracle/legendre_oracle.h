@@ -179,10 +179,12 @@ int orc_amr_sim_step(orc_amr_sim *s, double dt, int64_t nsteps);
 /* ---- Model C derivatives (legendre_oracle_adjoint.cpp; no reference counterpart:
  *      the checkers for the device adjoint).  θ = (λ, D, ε₄); b = 0 only. ---- */
 /* Tangent-linear run: advances (phi, u) exactly as orc_modelc_run (values bitwise) and
- * their tangents (dphi, du) along dtheta.  Interiors only. */
+ * ndir (1..4) tangents at once: direction k has initial field tangents
+ * dphi + k*len, du + k*len (len = field length) and parameter tangent
+ * dtheta[3k .. 3k+2]; the field tangents are advanced in place.  Interiors only. */
 int orc_modelc_tlm_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
-                       double *dphi, double *du, const double *dtheta, int64_t nsteps,
-                       double dt);
+                       int32_t ndir, double *dphi, double *du, const double *dtheta,
+                       int64_t nsteps, double dt);
 /* Hand-written adjoint of one step at state (phi, u): (pphi, pu) = ∂L/∂(φ', u') on
  * entry, ∂L/∂(φ, u) on return; grad[3] += ∂L/∂θ of the step. */
 int orc_modelc_adjoint_step(const orc_grid *g, const orc_modelc_params *p, const double *phi,

@@ -31,25 +31,66 @@
 
 namespace {
 
-struct Dual {
-  double v, d;
-  Dual() = default;
-  Dual(double x) : v(x), d(0.0) {}  // NOLINT: constants enter as exact duals
-  Dual(double x, double y) : v(x), d(y) {}
+// Dual number with K tangent directions (forward-mode derivative).
+template <int K>
+struct DualN {
+  double v;
+  double d[K];
+  DualN() = default;
+  DualN(double x) : v(x) {  // NOLINT: constants enter as exact duals
+    for (int k = 0; k < K; ++k) d[k] = 0.0;
+  }
 };
-inline Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
-inline Dual operator-(Dual a, Dual b) { return {a.v - b.v, a.d - b.d}; }
-inline Dual operator-(Dual a) { return {-a.v, -a.d}; }
-inline Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
-inline Dual operator/(Dual a, Dual b) {
-  double q = a.v / b.v;
-  return {q, (a.d - q * b.d) / b.v};
+template <int K>
+inline DualN<K> operator+(const DualN<K> &a, const DualN<K> &b) {
+  DualN<K> r;
+  r.v = a.v + b.v;
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] + b.d[k];
+  return r;
 }
+template <int K>
+inline DualN<K> operator-(const DualN<K> &a, const DualN<K> &b) {
+  DualN<K> r;
+  r.v = a.v - b.v;
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] - b.d[k];
+  return r;
+}
+template <int K>
+inline DualN<K> operator-(const DualN<K> &a) {
+  DualN<K> r;
+  r.v = -a.v;
+  for (int k = 0; k < K; ++k) r.d[k] = -a.d[k];
+  return r;
+}
+template <int K>
+inline DualN<K> operator*(const DualN<K> &a, const DualN<K> &b) {
+  DualN<K> r;
+  r.v = a.v * b.v;
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+  return r;
+}
+template <int K>
+inline DualN<K> operator/(const DualN<K> &a, const DualN<K> &b) {
+  DualN<K> r;
+  r.v = a.v / b.v;
+  for (int k = 0; k < K; ++k) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+  return r;
+}
+// Mixed operands: a plain double is a constant.
+template <int K> inline DualN<K> operator+(double a, const DualN<K> &b) { return DualN<K>(a) + b; }
+template <int K> inline DualN<K> operator*(double a, const DualN<K> &b) { return DualN<K>(a) * b; }
 inline double val(double x) { return x; }
-inline double val(Dual x) { return x.v; }
+template <int K>
+inline double val(const DualN<K> &x) {
+  return x.v;
+}
 inline double mfma(double a, double b, double c) { return std::fma(a, b, c); }
-inline Dual mfma(Dual a, Dual b, Dual c) {
-  return {std::fma(a.v, b.v, c.v), a.d * b.v + a.v * b.d + c.d};
+template <int K>
+inline DualN<K> mfma(const DualN<K> &a, const DualN<K> &b, const DualN<K> &c) {
+  DualN<K> r;
+  r.v = std::fma(a.v, b.v, c.v);
+  for (int k = 0; k < K; ++k) r.d[k] = a.d[k] * b.v + a.v * b.d[k] + c.d[k];
+  return r;
 }
 
 // 2-D global grid with per-dimension mirror (reflect_coord, src/geometry/boundary.rs:82-90)
@@ -236,39 +277,69 @@ inline FaceD face_deriv(const Consts<double> &k, double tol, double dx, double d
 
 extern "C" {
 
-int orc_modelc_tlm_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
-                       double *dphi, double *du, const double *dtheta, int64_t nsteps,
-                       double dt) {
-  if (g->dim != 2 || orc_grid_check(g) != 0) return -1;
+}  // extern "C"
+
+namespace {
+
+template <int K>
+void tlm_run_k(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+               int32_t ndir, double *dphi, double *du, const double *dtheta, int64_t nsteps,
+               double dt) {
+  using T = DualN<K>;
   Grid2 G = grid2(g);
-  size_t n = (size_t)(G.n0 * G.n1);
-  std::vector<double> a(n), b(n), c(n), e(n);
+  const size_t n = (size_t)(G.n0 * G.n1);
+  const size_t len = (size_t)(orc_num_blocks(g) * orc_block_len(g, 1));
+  std::vector<double> a(n), b(n), c(n);
   to_global(g, phi, a.data());
   to_global(g, u, b.data());
-  to_global(g, dphi, c.data());
-  to_global(g, du, e.data());
-  std::vector<Dual> P(n), U(n), Pn(n), Un(n);
+  std::vector<T> P(n), U(n), Pn(n), Un(n);
   for (size_t i = 0; i < n; ++i) {
-    P[i] = Dual(a[i], c[i]);
-    U[i] = Dual(b[i], e[i]);
+    P[i] = T(a[i]);
+    U[i] = T(b[i]);
   }
-  Consts<Dual> k = make_consts<Dual>(p, G, Dual(p->lambda, dtheta[0]), Dual(p->d_thermal, dtheta[1]),
-                                     Dual(p->eps4, dtheta[2]));
+  for (int k = 0; k < ndir; ++k) {
+    to_global(g, dphi + k * len, c.data());
+    for (size_t i = 0; i < n; ++i) P[i].d[k] = c[i];
+    to_global(g, du + k * len, c.data());
+    for (size_t i = 0; i < n; ++i) U[i].d[k] = c[i];
+  }
+  T th[3] = {T(p->lambda), T(p->d_thermal), T(p->eps4)};
+  for (int k = 0; k < ndir; ++k)
+    for (int q = 0; q < 3; ++q) th[q].d[k] = dtheta[3 * k + q];
+  Consts<T> kc = make_consts<T>(p, G, th[0], th[1], th[2]);
   for (int64_t s = 0; s < nsteps; ++s) {
-    step_global(G, k, dt, P.data(), U.data(), Pn.data(), Un.data());
+    step_global(G, kc, dt, P.data(), U.data(), Pn.data(), Un.data());
     P.swap(Pn);
     U.swap(Un);
   }
   for (size_t i = 0; i < n; ++i) {
     a[i] = P[i].v;
-    c[i] = P[i].d;
     b[i] = U[i].v;
-    e[i] = U[i].d;
   }
   from_global(g, a.data(), phi);
   from_global(g, b.data(), u);
-  from_global(g, c.data(), dphi);
-  from_global(g, e.data(), du);
+  for (int k = 0; k < ndir; ++k) {
+    for (size_t i = 0; i < n; ++i) c[i] = P[i].d[k];
+    from_global(g, c.data(), dphi + k * len);
+    for (size_t i = 0; i < n; ++i) c[i] = U[i].d[k];
+    from_global(g, c.data(), du + k * len);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_modelc_tlm_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                       int32_t ndir, double *dphi, double *du, const double *dtheta,
+                       int64_t nsteps, double dt) {
+  if (g->dim != 2 || orc_grid_check(g) != 0 || ndir < 1 || ndir > 4) return -1;
+  if (ndir == 1)
+    tlm_run_k<1>(g, p, phi, u, ndir, dphi, du, dtheta, nsteps, dt);
+  else if (ndir == 2)
+    tlm_run_k<2>(g, p, phi, u, ndir, dphi, du, dtheta, nsteps, dt);
+  else
+    tlm_run_k<4>(g, p, phi, u, ndir, dphi, du, dtheta, nsteps, dt);
   return 0;
 }
 

@@ -188,3 +188,24 @@ def test_full_gradient_matches_tlm_and_finite_differences(case):
         fd = (L(orc.params(up["eps4"], up["lambda_"], up["d_thermal"]))
               - L(orc.params(dn["eps4"], dn["lambda_"], dn["d_thermal"]))) / (2 * h)
         assert abs(fd - grad[k]) <= 1e-6 * abs(grad[k]), (name, fd, grad[k])
+
+
+def test_tlm_several_directions_equal_single_runs():
+    """Four tangents carried at once equal four single-direction runs bitwise."""
+    g, p, phi, u, cells, block = setup(0)
+    rng = np.random.default_rng(7)
+    n = phi.size
+    dth = rng.standard_normal((4, 3))
+    dp0 = np.stack([H.global_to_field(rng.standard_normal(cells[::-1]), cells, block, 1) for _ in range(4)])
+    du0 = np.stack([H.global_to_field(rng.standard_normal(cells[::-1]), cells, block, 1) for _ in range(4)])
+    c, d = phi.copy(), u.copy()
+    dp, du = dp0.copy(), du0.copy()
+    orc.modelc_tlm_run(g, p, c, d, dp, du, dth, 5, DT)
+    for k in range(4):
+        c1, d1 = phi.copy(), u.copy()
+        a, b = dp0[k].copy(), du0[k].copy()
+        orc.modelc_tlm_run(g, p, c1, d1, a, b, dth[k], 5, DT)
+        assert np.array_equal(interior(a, cells, block), interior(dp[k], cells, block))
+        assert np.array_equal(interior(b, cells, block), interior(du[k], cells, block))
+        assert np.array_equal(interior(c1, cells, block), interior(c, cells, block))
+    assert n == dp.shape[1]
