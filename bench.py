@@ -284,11 +284,255 @@ def run_adjoint(args) -> int:
     return 0
 
 
+def run_ensemble(args) -> int:
+    """Batched stochastic ensemble: 512 Model C realizations of uneven sizes
+    (blocks of 64²) advanced by one launch per step over every block of every
+    realization, against one launch per block; solid-fraction statistics checked
+    against the oracle on an evenly spaced subset of the same realizations."""
+    import numpy as np
+
+    import legendre_amd as lg
+    from legendre_amd.workloads import EnsembleConfig
+
+    E = EnsembleConfig(steps=args.ens_steps or EnsembleConfig.steps)
+    members = E.members()
+    ctx = lg.Context(0)
+    if args.strips:
+        ctx.set_tuning("modelc_strips", args.strips)
+    reals = [lg.Realization((m["side"], m["side"]), m["eps4"], E.lambda_, E.d_thermal, m["b"],
+                            seed=m["seed"]) for m in members]
+    cells = sum(m["side"] ** 2 for m in members)
+
+    def build():
+        ens = lg.Ensemble(ctx, reals, block_cells=(E.block, E.block), spacing=(E.h, E.h))
+        for r, m in enumerate(members):
+            ens.initialize_grains(r, E.grains(m["side"]), m["delta"])
+        ctx.sync()
+        return ens
+
+    t = time.perf_counter()
+    ens = build()
+    log(f"[ensemble] {len(reals)} realizations, {cells / 1e6:.1f} M cells, {ens.num_blocks} blocks, "
+        f"set up in {time.perf_counter() - t:.1f}s")
+    ctx.sync()
+    t = time.perf_counter()
+    done = 0
+    while done < E.steps:
+        m = min(250, E.steps - done)
+        ens.step(E.dt, m)
+        ctx.sync()
+        done += m
+        log(f"[ensemble] batched: {done}/{E.steps} steps, {time.perf_counter() - t:.1f}s")
+    t_batched = (time.perf_counter() - t) / E.steps
+    frac = ens.solid_fraction()
+    ens.close()
+    # One launch per block (the per-block dispatch of the reference's scheduler).
+    ens = build()
+    ens.step(E.dt, 1, per_block=True)
+    ctx.sync()
+    pb = max(1, args.per_block_steps)
+    t = time.perf_counter()
+    ens.step(E.dt, pb, per_block=True)
+    ctx.sync()
+    t_per_block = (time.perf_counter() - t) / pb
+    ens.close()
+    log(f"[ensemble] per step: batched {t_batched * 1e3:.3f} ms, per-block launches "
+        f"{t_per_block * 1e3:.3f} ms ({t_per_block / t_batched:.1f}x)")
+    mean, var = float(np.mean(frac)), float(np.var(frac, ddof=1))
+    se = (var / len(frac)) ** 0.5
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        idx = list(range(0, len(members), max(1, len(members) // args.cpu_members)))[: args.cpu_members]
+        t = time.perf_counter()
+        ref = []
+        for n_done, r in enumerate(idx):
+            m = members[r]
+            g = oracle.grid([m["side"]] * 2, [E.block] * 2, [0.0, 0.0], [E.h, E.h])
+            p = oracle.params(m["eps4"], E.lambda_, E.d_thermal, m["b"])
+            n = oracle.field_len(g, 1)
+            phi, u = np.zeros(n), np.zeros(n)
+            oracle.initialize_grains(g, phi, u, E.grains(m["side"]), m["delta"])
+            oracle.modelc_run(g, p, phi, u, E.steps, E.dt, seed=m["seed"], wiener=True,
+                              nthreads=min(16, len(os.sched_getaffinity(0))))
+            b = E.block + 2
+            inner = phi.reshape(-1, b, b)[:, 1:-1, 1:-1]
+            ref.append(float(np.count_nonzero(inner > 0.0)) / m["side"] ** 2)
+            log(f"[ensemble] oracle realization {r} ({m['side']}^2): {n_done + 1}/{len(idx)}, "
+                f"{time.perf_counter() - t:.0f}s")
+        ref = np.array(ref)
+        sub = frac[idx]
+        sub_se = (float(np.var(sub, ddof=1)) / len(sub)) ** 0.5 if len(sub) > 1 else 0.0
+        cpu = {
+            "members": idx,
+            "max_abs_diff": float(np.max(np.abs(sub - ref))),
+            "mean_gpu": float(np.mean(sub)),
+            "mean_oracle": float(np.mean(ref)),
+            "var_gpu": float(np.var(sub, ddof=1)) if len(sub) > 1 else 0.0,
+            "var_oracle": float(np.var(ref, ddof=1)) if len(ref) > 1 else 0.0,
+            "within_3se": bool(abs(float(np.mean(sub)) - float(np.mean(ref))) <= 3 * sub_se),
+            "seconds": round(time.perf_counter() - t, 1),
+        }
+        log(f"[ensemble] oracle check: {cpu}")
+    ctx.close()
+    value = cells / t_batched / 1e9
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": E.steps,
+        "warmup": 0,
+        "ms_per_step": round(t_batched * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"modelc_ensemble_{len(reals)}x_{E.block}blocks_{E.steps}steps",
+            "cells": cells,
+            "blocks": int(sum((m["side"] // E.block) ** 2 for m in members)),
+            "noise_b": "U[0.005,0.02]",
+        },
+        "ensemble": {
+            "batched_ms_per_step": round(t_batched * 1e3, 4),
+            "per_block_ms_per_step": round(t_per_block * 1e3, 4),
+            "speedup_vs_per_block": round(t_per_block / t_batched, 2),
+            "solid_fraction_mean": mean,
+            "solid_fraction_var": var,
+            "solid_fraction_se": se,
+            "oracle": cpu,
+        },
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def run_amr(args) -> int:
+    """Batched AMR: subcycled Model C on a 3-level Berger–Oliger hierarchy with
+    regrids, every patch of a level advanced by one launch per stage, against one
+    launch per patch on the same schedule.  Throughput counts cell updates:
+    Σ_levels cells_l × substeps_l per coarse step (substeps 4^l: the stable dt
+    shrinks with h²)."""
+    import legendre_amd as lg
+    from legendre_amd.workloads import AmrConfig
+
+    A = AmrConfig(coarse_steps=args.amr_steps or AmrConfig.coarse_steps)
+    ctx = lg.Context(0)
+    params = lg.ModelCParams(A.eps4, A.lambda_, A.d_thermal)
+    pol = lg.RegridPolicy(A.every, A.buffer, A.efficiency, A.min_width, A.threshold, 0)
+    grid = lg.CartesianGrid(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])
+    ic = lg.Simulation.model_c(grid, params)
+    ic.initialize_grains(A.grains(), A.delta)
+
+    def build():
+        gg = lg.AmrGrid.from_patches(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h],
+                                     list(A.ratios), [])
+        sim = lg.amr_simulation(gg, "modelc", params=params, subcycle=True, policy=pol)
+        sim.field(0).copy_from(ic.field(0))
+        sim.field(1).copy_from(ic.field(1))
+        ctx.sync()
+        return sim
+
+    def level_cells(sim):
+        g = sim.amr
+        cells = [0] * g.num_levels
+        for b in range(g.num_blocks):
+            lvl, lo, hi = g.patch(b)
+            cells[lvl] += (hi[0] - lo[0]) * (hi[1] - lo[1])
+        return cells
+
+    def run(sim, chunks, tag):
+        times, updates, shapes, first = [], [], [], []
+        for c in range(chunks):
+            t = time.perf_counter()
+            sim.step(A.dt, 1)  # the regrid runs at the top of this step
+            ctx.sync()
+            t1 = time.perf_counter()
+            sim.step(A.dt, A.every - 1)
+            ctx.sync()
+            times.append(time.perf_counter() - t)
+            first.append(t1 - t)
+            cells = level_cells(sim)
+            updates.append(A.every * sum(n * 4 ** l for l, n in enumerate(cells)))
+            shapes.append((sim.amr.num_blocks, cells))
+            log(f"[amr] {tag}: coarse steps {(c + 1) * A.every}, {times[-1] * 1e3:.1f} ms "
+                f"(regrid + first step {first[-1] * 1e3:.1f} ms), patches {sim.amr.num_blocks}, "
+                f"cells per level {cells}")
+        return times, updates, shapes, first
+
+    nchunks = max(1, A.coarse_steps // A.every)
+    sim = build()
+    times, updates, shapes, first = run(sim, nchunks, "batched")
+    sim.close()
+    pb_chunks = min(nchunks, max(1, args.per_block_chunks))
+    ctx.set_tuning("amr_per_block", 1)
+    sim = build()
+    pb_times, pb_updates, pb_shapes, pb_first = run(sim, pb_chunks, "per-patch launches")
+    sim.close()
+    ctx.set_tuning("amr_per_block", 0)
+    same = all(pb_shapes[i] == shapes[i] for i in range(pb_chunks))
+    total = sum(times)
+    value = sum(updates) / total / 1e9
+    speedup = sum(pb_times) / sum(times[:pb_chunks])
+    ic.close()
+    ctx.close()
+    line = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": nchunks * A.every,
+        "warmup": 0,
+        "ms_per_step": round(total / (nchunks * A.every) * 1e3, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"modelc_amr_{A.cells}sq_3levels_subcycled_{nchunks * A.every}coarse_steps",
+            "base_cells": A.cells * A.cells,
+            "ratios": list(A.ratios),
+            "regrid_every": A.every,
+            "seeds": A.nseeds,
+            "noise_b": 0.0,
+        },
+        "amr": {
+            "cell_updates": int(sum(updates)),
+            "seconds": round(total, 3),
+            "final_patches": shapes[-1][0],
+            "final_cells_per_level": shapes[-1][1],
+            "per_patch_chunks_compared": pb_chunks,
+            "batched_ms_first_chunks": round(sum(times[:pb_chunks]) * 1e3, 3),
+            "per_patch_ms_first_chunks": round(sum(pb_times) * 1e3, 3),
+            "speedup_vs_per_patch_launches": round(speedup, 2),
+            "regrid_and_first_step_seconds": round(sum(first), 3),
+            "speedup_excluding_regrid_steps": round(
+                (sum(pb_times) - sum(pb_first)) / (sum(times[:pb_chunks]) - sum(first[:pb_chunks])), 2),
+            "same_hierarchy": same,
+        },
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["modelc", "adjoint"], default="modelc",
-                    help="modelc: the headline plain step; adjoint: the calibration gradient")
+    ap.add_argument("--workload", choices=["modelc", "adjoint", "ensemble", "amr"], default="modelc",
+                    help="modelc: the headline plain step; adjoint: the calibration gradient; "
+                         "ensemble: 512 batched realizations; amr: 3-level subcycled hierarchy")
+    ap.add_argument("--amr-steps", type=int, default=0, help="AMR coarse steps (0 = 500)")
+    ap.add_argument("--per-block-chunks", type=int, default=2,
+                    help="regrid intervals timed with one launch per patch (amr)")
+    ap.add_argument("--ens-steps", type=int, default=0, help="ensemble steps (0 = 2000)")
+    ap.add_argument("--per-block-steps", type=int, default=3,
+                    help="steps timed with one launch per block (ensemble)")
+    ap.add_argument("--cpu-members", type=int, default=16,
+                    help="realizations re-run on the oracle (ensemble)")
     ap.add_argument("--adj-steps", type=int, default=0, help="adjoint workload steps (0 = 2000)")
     ap.add_argument("--snapshots", type=int, default=0, help="adjoint checkpoint slots (0 = 45)")
     ap.add_argument("--steps", type=int, default=50)
@@ -314,6 +558,10 @@ def main() -> int:
 
     if args.workload == "adjoint":
         return run_adjoint(args)
+    if args.workload == "ensemble":
+        return run_ensemble(args)
+    if args.workload == "amr":
+        return run_amr(args)
 
     from legendre_amd.workloads import ModelCConfig
 

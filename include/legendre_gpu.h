@@ -98,7 +98,9 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
 /* Launch-shape knobs for this context (no reference counterpart: the reference has
  * no launch shapes).  Keys: "modelc_kernel" (0 auto, 1 per-cell, 2 streaming),
  * "modelc_strips" (row strips per block for the streaming kernel, 0 = default) and
- * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default).
+ * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default) and
+ * "amr_per_block" (1: the AMR level kernels launch once per patch instead of once per
+ * level — the per-block dispatch baseline).
  * Results are bitwise independent of every knob. */
 int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value);
 

@@ -71,6 +71,7 @@ struct lgd_restrict_item {
 struct lgd_level_plan {
   lgd_tile *tiles = nullptr;
   int64_t ntiles = 0;
+  std::vector<int64_t> tile0;  // first tile of each of the level's patches, + end
   lgd_fill_op *ops = nullptr;
   int64_t nops = 0;
   lgd_corner *corners = nullptr;
@@ -88,7 +89,14 @@ struct lgd_amr {
   std::vector<lgd_box> box;           // per patch, level cell coordinates
   std::vector<uint32_t> level_start;  // populated levels + 1
   std::vector<std::array<double, 2>> spacing;  // per level, ratio capacity
-  std::vector<std::vector<uint32_t>> lookup;    // per populated level: cell → patch
+  // Per populated level: patches bucketed by 16×16-cell squares of the level domain
+  // (CSR: bucket k holds ids[start[k] .. start[k+1])), for find_patch.
+  struct bucket_index {
+    int64_t nbx = 0, nby = 0;
+    std::vector<uint32_t> start, ids;
+  };
+  static constexpr int64_t kBucket = 16;
+  std::vector<bucket_index> lookup;
   std::vector<int64_t> off;           // ghost-1 slab offsets, nblocks + 1
   lgd_block_desc *d_blocks = nullptr;
   std::vector<lgd_level_plan> plans;  // per populated level

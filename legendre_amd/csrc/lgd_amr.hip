@@ -87,8 +87,11 @@ int64_t lgd_amr::find_patch(int l, int64_t x, int64_t y) const {
   if (l < 0 || l >= num_levels()) return -1;
   lgd_box d = domain(l);
   if (!d.contains(x, y)) return -1;
-  uint32_t p = lookup[l][(size_t)(x + y * d.hi[0])];
-  return p == UINT32_MAX ? -1 : (int64_t)p;
+  const bucket_index &ix = lookup[(size_t)l];
+  const int64_t k = (x / kBucket) + ix.nbx * (y / kBucket);
+  for (uint32_t i = ix.start[(size_t)k]; i < ix.start[(size_t)k + 1]; ++i)
+    if (box[ix.ids[i]].contains(x, y)) return (int64_t)ix.ids[i];
+  return -1;
 }
 
 int64_t lgd_amr::index(int64_t b, int64_t i, int64_t j) const {
@@ -179,15 +182,28 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
     auto p = g.spacing.back();
     g.spacing.push_back({p[0] / (double)r, p[1] / (double)r});
   }
+  // Patches of one level do not overlap; each bucket lists the few that touch it.
+  const int64_t B = lgd_amr::kBucket;
   g.lookup.clear();
   for (int l = 0; l < g.num_levels(); ++l) {
     lgd_box d = g.domain(l);
-    std::vector<uint32_t> map((size_t)(d.hi[0] * d.hi[1]), UINT32_MAX);
+    lgd_amr::bucket_index ix;
+    ix.nbx = (d.hi[0] + B - 1) / B;
+    ix.nby = (d.hi[1] + B - 1) / B;
+    ix.start.assign((size_t)(ix.nbx * ix.nby + 1), 0);
+    auto each_bucket = [&](uint32_t p, auto &&f) {
+      const lgd_box &bx = g.box[p];
+      for (int64_t by = bx.lo[1] / B; by <= (bx.hi[1] - 1) / B; ++by)
+        for (int64_t bx0 = bx.lo[0] / B; bx0 <= (bx.hi[0] - 1) / B; ++bx0) f(bx0 + ix.nbx * by);
+    };
     for (uint32_t p = g.level_start[l]; p < g.level_start[l + 1]; ++p)
-      for (int64_t y = g.box[p].lo[1]; y < g.box[p].hi[1]; ++y)
-        std::fill(map.begin() + (size_t)(g.box[p].lo[0] + y * d.hi[0]),
-                  map.begin() + (size_t)(g.box[p].hi[0] + y * d.hi[0]), p);
-    g.lookup.push_back(std::move(map));
+      each_bucket(p, [&](int64_t k) { ++ix.start[(size_t)k + 1]; });
+    for (size_t k = 1; k < ix.start.size(); ++k) ix.start[k] += ix.start[k - 1];
+    ix.ids.resize(ix.start.back());
+    std::vector<uint32_t> fillpos(ix.start.begin(), ix.start.end() - 1);
+    for (uint32_t p = g.level_start[l]; p < g.level_start[l + 1]; ++p)
+      each_bucket(p, [&](int64_t k) { ix.ids[fillpos[(size_t)k]++] = p; });
+    g.lookup.push_back(std::move(ix));
   }
   g.off.assign(g.box.size() + 1, 0);
   for (size_t b = 0; b < g.box.size(); ++b)
@@ -249,10 +265,14 @@ int lgd_amr_build_plans(lgd_amr &g) {
     lgd_level_plan &P = g.plans[(size_t)l];
     // Level kernels: one 64×4 tile of cells per workgroup.
     std::vector<lgd_tile> tiles;
-    for (uint32_t b = g.level_start[l]; b < g.level_start[l + 1]; ++b)
+    P.tile0.clear();
+    for (uint32_t b = g.level_start[l]; b < g.level_start[l + 1]; ++b) {
+      P.tile0.push_back((int64_t)tiles.size());
       for (int64_t y0 = 0; y0 < g.box[b].extent(1); y0 += kTY)
         for (int64_t x0 = 0; x0 < g.box[b].extent(0); x0 += kTX)
           tiles.push_back(lgd_tile{(int32_t)b, (int32_t)x0, (int32_t)y0, 0});
+    }
+    P.tile0.push_back((int64_t)tiles.size());
     // Fill plan: every ghost-ring cell (ghost width 1) of every patch.
     std::vector<lgd_fill_op> ops;
     std::vector<lgd_corner> corners;
@@ -460,8 +480,16 @@ int lgd_amr_modelc_rhs(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &
   c.h[0] = g.spacing[(size_t)level][0];
   c.h[1] = g.spacing[(size_t)level][1];
   lgd_kr_consts k = lgd_make_kr_consts(p, c);
-  amr_modelc_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
-      P.tiles, g.d_blocks, k, phi, u, kphi, ku);
+  if (ctx->amr_per_block) {
+    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) {
+      const int64_t n = P.tile0[b + 1] - P.tile0[b];
+      amr_modelc_rhs_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+          P.tiles + P.tile0[b], g.d_blocks, k, phi, u, kphi, ku);
+    }
+  } else {
+    amr_modelc_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
+        P.tiles, g.d_blocks, k, phi, u, kphi, ku);
+  }
   return launch_check(ctx, "amr_modelc_rhs");
 }
 
@@ -470,15 +498,30 @@ int lgd_amr_heat_rhs(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, co
   const lgd_level_plan &P = g.plans[(size_t)level];
   if (P.ntiles == 0) return LGD_OK;
   double hx = g.spacing[(size_t)level][0], hy = g.spacing[(size_t)level][1];
-  amr_heat_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
-      P.tiles, g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, u, k);
+  if (ctx->amr_per_block) {
+    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) {
+      const int64_t n = P.tile0[b + 1] - P.tile0[b];
+      amr_heat_rhs_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+          P.tiles + P.tile0[b], g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, u, k);
+    }
+  } else {
+    amr_heat_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
+        P.tiles, g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, u, k);
+  }
   return launch_check(ctx, "amr_heat_rhs");
 }
 
 int lgd_amr_apply(lgd_ctx *ctx, const lgd_amr &g, int level, double *x, const double *k, double dt) {
   int64_t lo = g.off[g.level_start[level]], hi = g.off[g.level_start[level + 1]];
   if (hi <= lo) return LGD_OK;
-  amr_axpy_kernel<<<blocks_for(hi - lo, 256), 256, 0, ctx->stream>>>(hi - lo, dt, x + lo, k + lo);
+  if (ctx->amr_per_block) {
+    for (uint32_t b = g.level_start[level]; b < g.level_start[level + 1]; ++b) {
+      const int64_t b0 = g.off[b], n = g.off[b + 1] - g.off[b];
+      amr_axpy_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, dt, x + b0, k + b0);
+    }
+  } else {
+    amr_axpy_kernel<<<blocks_for(hi - lo, 256), 256, 0, ctx->stream>>>(hi - lo, dt, x + lo, k + lo);
+  }
   return launch_check(ctx, "amr_apply");
 }
 
@@ -486,8 +529,16 @@ int lgd_amr_wiener(lgd_ctx *ctx, const lgd_amr &g, int level, double *phi, doubl
                    uint64_t seed, uint64_t salt) {
   const lgd_level_plan &P = g.plans[(size_t)level];
   if (P.ntiles == 0) return LGD_OK;
-  amr_wiener_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
-      P.tiles, g.d_blocks, noise_scale, seed, salt, (1ull << 32) | 0ull, phi);
+  if (ctx->amr_per_block) {
+    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) {
+      const int64_t n = P.tile0[b + 1] - P.tile0[b];
+      amr_wiener_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+          P.tiles + P.tile0[b], g.d_blocks, noise_scale, seed, salt, (1ull << 32) | 0ull, phi);
+    }
+  } else {
+    amr_wiener_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
+        P.tiles, g.d_blocks, noise_scale, seed, salt, (1ull << 32) | 0ull, phi);
+  }
   return launch_check(ctx, "amr_wiener");
 }
 

@@ -129,6 +129,9 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
   } else if (k == "modelc_prefetch") {
     if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "modelc_prefetch is 0, 1 or 2");
     ctx->modelc_prefetch = value;
+  } else if (k == "amr_per_block") {
+    if (value > 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_per_block is 0 or 1");
+    ctx->amr_per_block = value;
   } else {
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "unknown tuning key " + k);
   }

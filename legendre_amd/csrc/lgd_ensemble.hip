@@ -236,11 +236,13 @@ int lgd_ensemble_step(lgd_ensemble *e, double dt, int64_t nsteps, int32_t per_bl
                                     e->state[1]->data, e->next[0]->data, e->next[1]->data, dt,
                                     noise, salt, wiener0, strips);
     } else {
+      // The per-block dispatch baseline: one launch per block, each in its
+      // fastest single-block shape (one wave per row).
       for (int64_t b = 0; b < nblocks && !rc; ++b)
         rc = lgd_modelc_stream_launch(e->ctx, e->nx, e->ny, e->domains[0], e->d_domains,
                                       e->d_block_domain, b, 1, e->state[0]->data,
                                       e->state[1]->data, e->next[0]->data, e->next[1]->data, dt,
-                                      noise, salt, wiener0, strips);
+                                      noise, salt, wiener0, (int)e->ny);
     }
     if (rc) return rc;
     std::swap(e->state[0], e->next[0]);

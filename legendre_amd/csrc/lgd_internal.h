@@ -18,6 +18,7 @@ struct lgd_ctx {
   int64_t modelc_kernel = 0;  // 1 = per-cell kernel, 2 = streaming kernel
   int64_t modelc_strips = 0;  // strips (row ranges) per block for the streaming kernel
   int64_t modelc_prefetch = 0;  // rows the streaming kernel loads ahead (1 or 2; 0 = default)
+  int64_t amr_per_block = 0;    // 1 = AMR level kernels launched once per patch (baseline)
 };
 
 struct lgd_buf {

@@ -254,16 +254,17 @@ int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
 
 void lgd_regrid_migrate(const lgd_amr &old_g, const std::vector<double> &old_f, const lgd_amr &new_g,
                         std::vector<double> &new_f) {
-  new_f.assign((size_t)new_g.off.back(), 0.0);
-  // Level 0 verbatim (the base tiling never changes), ghosts included.
-  for (uint32_t b = 0; b < new_g.level_start[1]; ++b)
-    std::copy(old_f.begin() + old_g.off[b], old_f.begin() + old_g.off[b + 1],
-              new_f.begin() + new_g.off[b]);
+  // Level 0 is identical in both hierarchies (the base tiling never changes) and
+  // is copied on the device by the caller; new_f holds levels >= 1 only, from
+  // new_g.off[level_start[1]] on.
+  const int64_t fine0 = new_g.off[new_g.level_start[1]];
+  new_f.assign((size_t)(new_g.off.back() - fine0), 0.0);
   for (int l = 1; l < new_g.num_levels(); ++l)
     for (uint32_t b = new_g.level_start[l]; b < new_g.level_start[l + 1]; ++b) {
       const lgd_box &p = new_g.box[b];
       for (int64_t j = 0; j < p.extent(1); ++j)
         for (int64_t i = 0; i < p.extent(0); ++i)
-          new_f[(size_t)new_g.index(b, i, j)] = sample_old(old_g, old_f.data(), l, p.lo[0] + i, p.lo[1] + j);
+          new_f[(size_t)(new_g.index(b, i, j) - fine0)] =
+              sample_old(old_g, old_f.data(), l, p.lo[0] + i, p.lo[1] + j);
     }
 }

@@ -142,24 +142,41 @@ int nfields(const lgd_sim *s) { return s->kind == lgd_sim::kModelC ? 2 : 1; }
 
 // Stage buffers of a simulation on an AmrGrid: Tendency(Time) per field, the
 // Wiener(0) amplitude (Model C with noise), one snapshot per (level, field)
-// when subcycling (src/integrators/subcycling.rs:63-73).
+// when subcycling (src/integrators/subcycling.rs:63-73).  After a regrid the
+// buffers are kept when they still hold the new field length (kernels address
+// them by block offsets, so a longer buffer is fine) and grown by a quarter
+// otherwise; tendencies are cleared (their ghost entries stay zero, as the
+// reference's zero-initialised stages), snapshots need no clearing (a level's
+// range is copied in before it is read).
 int alloc_amr_stages(lgd_sim *s) {
+  const size_t len = (size_t)s->amr->off.back();
+  const int nf = nfields(s);
+  const bool noise = s->kind == lgd_sim::kModelC && s->wiener;
+  const size_t nsnap = s->subcycle ? (size_t)(s->amr->max_levels() * nf) : 0;
+  bool fits = s->k.size() == (size_t)nf && s->snaps.size() == nsnap && (!noise || s->amp);
+  for (auto *b : s->k) fits = fits && b->len >= len;
+  for (auto *b : s->snaps) fits = fits && b->len >= len;
+  if (noise && s->amp) fits = fits && s->amp->len >= len;
+  int rc;
+  if (fits) {
+    for (auto *b : s->k)
+      if ((rc = lgd_buf_fill_zero(s->ctx, b))) return rc;
+    if (noise && (rc = lgd_buf_fill_zero(s->ctx, s->amp))) return rc;
+    return LGD_OK;
+  }
+  const size_t cap = s->k.empty() ? len : len + len / 4;  // grow with headroom after a regrid
   free_stages(s);
-  size_t len = (size_t)s->amr->off.back();
-  int nf = nfields(s), rc;
   for (int f = 0; f < nf; ++f) {
     lgd_buf *b = nullptr;
-    if ((rc = lgd_buf_alloc(s->ctx, len, &b))) return rc;
+    if ((rc = lgd_buf_alloc(s->ctx, cap, &b))) return rc;
     s->k.push_back(b);
   }
-  if (s->kind == lgd_sim::kModelC && s->wiener && (rc = lgd_buf_alloc(s->ctx, len, &s->amp)))
-    return rc;
-  if (s->subcycle)
-    for (int i = 0; i < s->amr->max_levels() * nf; ++i) {
-      lgd_buf *b = nullptr;
-      if ((rc = lgd_buf_alloc(s->ctx, len, &b))) return rc;
-      s->snaps.push_back(b);
-    }
+  if (noise && (rc = lgd_buf_alloc(s->ctx, cap, &s->amp))) return rc;
+  for (size_t i = 0; i < nsnap; ++i) {
+    lgd_buf *b = nullptr;
+    if ((rc = lgd_buf_alloc(s->ctx, cap, &b))) return rc;
+    s->snaps.push_back(b);
+  }
   return LGD_OK;
 }
 
@@ -278,13 +295,22 @@ int amr_regrid(lgd_sim *s) {
   }
   std::vector<lgd_buf *> ns;
   std::vector<double> old_f, new_f;
+  const int64_t fine0 = ng->off[ng->level_start[1]];  // level-0 elements (same in both)
   for (size_t f = 0; f < s->state.size(); ++f) {
-    old_f.resize((size_t)g.off.back());
-    if ((rc = lgd_buf_download(s->ctx, s->state[f], 0, old_f.data(), old_f.size()))) return rc;
+    if ((int)f == tf) {
+      old_f.swap(tag);  // already downloaded
+    } else {
+      old_f.resize((size_t)g.off.back());
+      if ((rc = lgd_buf_download(s->ctx, s->state[f], 0, old_f.data(), old_f.size()))) return rc;
+    }
     lgd_regrid_migrate(g, old_f, *ng, new_f);
     lgd_buf *b = nullptr;
-    if ((rc = lgd_buf_alloc(s->ctx, new_f.size(), &b))) return rc;
-    if ((rc = lgd_buf_upload(s->ctx, b, 0, new_f.data(), new_f.size()))) return rc;
+    if ((rc = lgd_buf_alloc(s->ctx, (size_t)ng->off.back(), &b))) return rc;
+    hipError_t e = hipMemcpyAsync(b->data, s->state[f]->data, (size_t)fine0 * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s->ctx->stream);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "migrate level 0");
+    if (!new_f.empty() && (rc = lgd_buf_upload(s->ctx, b, (size_t)fine0, new_f.data(), new_f.size())))
+      return rc;
     ns.push_back(b);
   }
   for (auto *b : s->state) lgd_buf_free(b);

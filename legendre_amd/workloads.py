@@ -9,6 +9,16 @@ adjoint  1024² cells as 16×16 blocks of 64², b = 0, 2000 steps, L = ½‖φ_T
          with φ_target from a forward run at (λ, D, ε₄) = (3.4, 2.1, 0.06); gradient at
          (3.19, 2.0, 0.05) w.r.t. (λ, D, ε₄) and the initial fields; 45 snapshots;
          16 seeds of radius 5h by the same seed_positions recipe, u = −0.55.
+ensemble 512 independent Model C realizations, side ∈ {256, 384, 512, 768, 1024}
+         (square), b ∈ U[0.005, 0.02], Δ ∈ U[0.45, 0.65], ε₄ ∈ U[0.02, 0.06], all drawn
+         as unit_open(mix_key(seed, [r, k])), blocks of 64², h = 0.4, dt = 0.008,
+         λ = 3.19, D = 2.0, one seed of radius 5h at the centre, noise seed 1000 + r,
+         2000 steps.
+amr      Model C on a Berger–Oliger hierarchy: 2048² base in 64² blocks (h = 0.4), ratios
+         [2, 2] (3 levels), subcycled, coarse dt = 0.2·h²/D, gradient tagging
+         max_d |½(φ[+]−φ[−])| > 0.1, regrid every 20 coarse steps (buffer 2, efficiency
+         0.8, minimum width 16), 16 seeds of radius 5h (seed_positions recipe), b = 0,
+         500 coarse steps.
 """
 from __future__ import annotations
 
@@ -133,6 +143,61 @@ class AdjointConfig:
     @property
     def interior_cells(self) -> int:
         return self.cells * self.cells
+
+    def grains(self) -> np.ndarray:
+        return ModelCConfig(cells=self.cells, block=self.block, h=self.h, nseeds=self.nseeds,
+                            seed=self.seed).grains()
+
+
+@dataclasses.dataclass(frozen=True)
+class EnsembleConfig:
+    n: int = 512
+    sides: tuple = (256, 384, 512, 768, 1024)
+    block: int = 64
+    h: float = 0.4
+    dt: float = 0.008
+    steps: int = 2000
+    lambda_: float = 3.19
+    d_thermal: float = 2.0
+    seed: int = 42
+
+    def members(self) -> list:
+        """Per realization: dict(side, b, delta, eps4, seed)."""
+        out = []
+        for r in range(self.n):
+            u = [float(unit_open_np(mix_key_np(self.seed, [r, k]))[0]) for k in range(4)]
+            side = self.sides[min(int(u[0] * len(self.sides)), len(self.sides) - 1)]
+            out.append(dict(side=side, b=0.005 + 0.015 * u[1], delta=0.45 + 0.2 * u[2],
+                            eps4=0.02 + 0.04 * u[3], seed=1000 + r))
+        return out
+
+    def grains(self, side: int) -> np.ndarray:
+        c = side * self.h / 2.0
+        return np.array([[c, c, 5.0 * self.h]])
+
+
+@dataclasses.dataclass(frozen=True)
+class AmrConfig:
+    cells: int = 2048
+    block: int = 64
+    h: float = 0.4
+    ratios: tuple = (2, 2)
+    eps4: float = 0.05
+    lambda_: float = 3.19
+    d_thermal: float = 2.0
+    delta: float = 0.55
+    nseeds: int = 16
+    seed: int = 42
+    coarse_steps: int = 500
+    every: int = 20
+    buffer: int = 2
+    efficiency: float = 0.8
+    min_width: int = 16
+    threshold: float = 0.1
+
+    @property
+    def dt(self) -> float:
+        return 0.2 * self.h * self.h / self.d_thermal
 
     def grains(self) -> np.ndarray:
         return ModelCConfig(cells=self.cells, block=self.block, h=self.h, nseeds=self.nseeds,

@@ -189,3 +189,30 @@ def test_adaptive_subcycled_model_c_matches_oracle_bitwise(ctx, orc):
     assert sim_g.amr.num_blocks == sim_o.grid.num_blocks
     assert np.array_equal(sim_g.field(0).download(), sim_o.field(0))
     assert np.array_equal(sim_g.field(1).download(), sim_o.field(1))
+
+
+def test_per_block_dispatch_matches_batched_bitwise(ctx, orc):
+    """One launch per patch (the per-block dispatch baseline) computes the same bits
+    as one launch per level, noise included."""
+    say("amr per-block vs batched dispatch")
+    base = orc.grid([48, 48], [24, 24], [0, 0], [0.4, 0.4])
+    phi = np.zeros(orc.field_len(base, 1))
+    u = np.zeros_like(phi)
+    orc.initialize_grains(base, phi, u, np.array([[0.4, 0.4, 4.0], [12.0, 9.0, 2.0]]), 0.7)
+    pol = lg.RegridPolicy(4, 2, 0.8, 4, 0.15, 0)
+    outs = []
+    try:
+        for per_block in (0, 1):
+            ctx.set_tuning("amr_per_block", per_block)
+            gg = lg.AmrGrid.from_patches(ctx, [48, 48], [24, 24], [0, 0], [0.4, 0.4], [2, 2], [])
+            sim = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.06, 3.19, 2.0, 0.01),
+                                    wiener=True, seed=5, subcycle=True, policy=pol)
+            sim.field(0).upload(phi)
+            sim.field(1).upload(u)
+            sim.step(0.2 * 0.4**2 / 2.0, 24)
+            outs.append((sim.amr.num_blocks, sim.field(0).download(), sim.field(1).download()))
+            sim.close()
+    finally:
+        ctx.set_tuning("amr_per_block", 0)
+    assert outs[0][0] == outs[1][0] and outs[0][0] > 4
+    assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
