@@ -113,8 +113,9 @@ __device__ __forceinline__ void wg_accumulate(const double (&v)[kN], double *acc
 }
 
 // Stage: φ and u on the padded global grid ((N0+2) × (N1+2); the ring holds the
-// resolved neighbour values, i.e. the ghost fill), and the incoming adjoint on the
-// unpadded global grid.
+// resolved neighbour values, i.e. the ghost fill), and (kAdj) the incoming adjoint
+// from slabs onto the unpadded global grid.
+template <bool kAdj>
 __global__ __launch_bounds__(kWG) void adj_stage_kernel(AdjGrid G, const double *phi,
                                                         const double *u, const double *pphi,
                                                         const double *pu, double *Pp, double *Up,
@@ -126,7 +127,7 @@ __global__ __launch_bounds__(kWG) void adj_stage_kernel(AdjGrid G, const double 
   const int64_t o = slab_off(G, rx, ry);
   Pp[t] = phi[o];
   Up[t] = u[o];
-  if (x == rx && y == ry) {
+  if (kAdj && x == rx && y == ry) {
     AP[x + G.N0 * y] = pphi[o];
     AU[x + G.N0 * y] = pu[o];
   }
@@ -242,7 +243,9 @@ __global__ __launch_bounds__(kWG) void adj_face_kernel(AdjGrid G, lgd_kr_consts 
   wg_accumulate<1>(part, acc);
 }
 
-// Pass 3: gather at each interior cell.
+// Pass 3: gather at each interior cell; the outgoing adjoint goes to slabs (kSlab)
+// or, in place, to the global arrays the next reverse step reads.
+template <bool kSlab>
 __global__ __launch_bounds__(kWG) void adj_gather_kernel(AdjGrid G, lgd_kr_consts k,
                                                          const double *GXx, const double *GYx,
                                                          const double *GXy, const double *GYy,
@@ -299,9 +302,25 @@ __global__ __launch_bounds__(kWG) void adj_gather_kernel(AdjGrid G, lgd_kr_const
   for (int iy = 0; iy < ny; ++iy) ly += inner(V, a, ys[iy] - 1) + inner(V, a, ys[iy] + 1);
   const double v = V[t];
   const double lap = (lx - 2.0 * v) * k.inv_hx2 + (ly - 2.0 * v) * k.inv_hy2;
-  const int64_t o = slab_off(G, a, b);
-  pphi[o] = acc;
-  pu[o] = PU[t] + lap;
+  if (kSlab) {
+    const int64_t o = slab_off(G, a, b);
+    pphi[o] = acc;
+    pu[o] = PU[t] + lap;
+  } else {
+    pphi[t] = acc;
+    pu[t] = PU[t] + lap;
+  }
+}
+
+// Global adjoint arrays → slab interiors.
+__global__ __launch_bounds__(kWG) void adj_unflatten_kernel(AdjGrid G, const double *AP,
+                                                            const double *AU, double *pphi,
+                                                            double *pu) {
+  const int t = blockIdx.x * kWG + threadIdx.x;
+  if (t >= G.N0 * G.N1) return;
+  const int64_t o = slab_off(G, t % G.N0, t / G.N0);
+  if (pphi) pphi[o] = AP[t];
+  if (pu) pu[o] = AU[t];
 }
 
 __global__ void adj_zero_kernel(int64_t n, double *x) {
@@ -310,14 +329,14 @@ __global__ void adj_zero_kernel(int64_t n, double *x) {
 }
 
 // Terminal adjoint of L = ½‖φ − target‖²: P_φ = φ − target, P_u = 0, r² per cell.
-__global__ void adj_loss_kernel(AdjGrid G, const double *phi, const double *target, double *pphi,
-                                double *pu, double *r2) {
+__global__ void adj_loss_kernel(AdjGrid G, const double *phi, const double *target, double *AP,
+                                double *AU, double *r2) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= G.N0 * G.N1) return;
   const int64_t o = slab_off(G, t % G.N0, t / G.N0);
   const double r = phi[o] - target[o];
-  pphi[o] = r;
-  pu[o] = 0.0;
+  AP[t] = r;
+  AU[t] = 0.0;
   r2[t] = r * r;
 }
 
@@ -429,6 +448,41 @@ int sum_into(lgd_ctx *ctx, int64_t n, const double *x, double *out) {
 
 }  // namespace
 
+namespace {
+
+// One reverse step.  slab = true: (pphi, pu) are slab fields, read and written in
+// place; slab = false: the adjoint lives in the workspace's global arrays AP, AU
+// (the calibration's reverse sweep) and pphi, pu are unused.
+int adjoint_passes(lgd_adjoint *a, const lgd_modelc_params &params, const double *phi,
+                   const double *u, double dt, double *pphi, double *pu, bool slab) {
+  lgd_ctx *ctx = a->ctx;
+  int rc;
+  const lgd_kr_consts k = lgd_make_kr_consts(params, a->grid->cart);
+  const AdjGrid &G = a->G;
+  if (slab)
+    adj_stage_kernel<true><<<blocks_for(a->npad, kWG), kWG, 0, ctx->stream>>>(
+        G, phi, u, pphi, pu, a->Pp, a->Up, a->AP, a->AU);
+  else
+    adj_stage_kernel<false><<<blocks_for(a->npad, kWG), kWG, 0, ctx->stream>>>(
+        G, phi, u, nullptr, nullptr, a->Pp, a->Up, a->AP, a->AU);
+  if ((rc = launch_check(ctx, "adjoint stage"))) return rc;
+  adj_cell_kernel<<<(unsigned)a->wg_cell, kWG, 0, ctx->stream>>>(
+      G, k, dt, a->Pp, a->Up, a->AP, a->AU, a->R, a->SX, a->SY, a->PW, a->PU, a->V, a->acc_cell);
+  if ((rc = launch_check(ctx, "adjoint cells"))) return rc;
+  adj_face_kernel<<<(unsigned)a->wg_face, kWG, 0, ctx->stream>>>(G, k, a->Pp, a->R, a->GXx, a->GYx,
+                                                                 a->GXy, a->GYy, a->acc_face);
+  if ((rc = launch_check(ctx, "adjoint faces"))) return rc;
+  if (slab)
+    adj_gather_kernel<true><<<blocks_for(a->ncell, kWG), kWG, 0, ctx->stream>>>(
+        G, k, a->GXx, a->GYx, a->GXy, a->GYy, a->SX, a->SY, a->PW, a->PU, a->V, pphi, pu);
+  else
+    adj_gather_kernel<false><<<blocks_for(a->ncell, kWG), kWG, 0, ctx->stream>>>(
+        G, k, a->GXx, a->GYx, a->GXy, a->GYy, a->SX, a->SY, a->PW, a->PU, a->V, a->AP, a->AU);
+  return launch_check(ctx, "adjoint gather");
+}
+
+}  // namespace
+
 extern "C" {
 
 int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out) {
@@ -510,20 +564,7 @@ int lgd_modelc_adjoint_step(lgd_adjoint *a, const lgd_modelc_params *params, con
   if ((rc = check_field(ctx, a->grid, pu, 1, "pu"))) return rc;
   if (pphi == pu || pphi == phi || pphi == u || pu == phi || pu == u)
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "adjoint buffers must be distinct from the state");
-  const lgd_kr_consts k = lgd_make_kr_consts(*params, a->grid->cart);
-  const AdjGrid &G = a->G;
-  adj_stage_kernel<<<blocks_for(a->npad, kWG), kWG, 0, ctx->stream>>>(
-      G, phi->data, u->data, pphi->data, pu->data, a->Pp, a->Up, a->AP, a->AU);
-  if ((rc = launch_check(ctx, "adjoint stage"))) return rc;
-  adj_cell_kernel<<<(unsigned)a->wg_cell, kWG, 0, ctx->stream>>>(
-      G, k, dt, a->Pp, a->Up, a->AP, a->AU, a->R, a->SX, a->SY, a->PW, a->PU, a->V, a->acc_cell);
-  if ((rc = launch_check(ctx, "adjoint cells"))) return rc;
-  adj_face_kernel<<<(unsigned)a->wg_face, kWG, 0, ctx->stream>>>(G, k, a->Pp, a->R, a->GXx, a->GYx,
-                                                                 a->GXy, a->GYy, a->acc_face);
-  if ((rc = launch_check(ctx, "adjoint faces"))) return rc;
-  adj_gather_kernel<<<blocks_for(a->ncell, kWG), kWG, 0, ctx->stream>>>(
-      G, k, a->GXx, a->GYx, a->GXy, a->GYy, a->SX, a->SY, a->PW, a->PU, a->V, pphi->data, pu->data);
-  return launch_check(ctx, "adjoint gather");
+  return adjoint_passes(a, *params, phi->data, u->data, dt, pphi->data, pu->data, true);
 }
 
 int lgd_adjoint_grad(lgd_adjoint *a, double *grad) {
@@ -556,7 +597,6 @@ struct lgd_calibration {
   std::vector<lgd_buf *> slot_phi, slot_u;
   std::vector<int64_t> slot_pos;  // state index held by each slot (-1 = none)
   lgd_buf *w_phi[2] = {nullptr, nullptr}, *w_u[2] = {nullptr, nullptr};
-  lgd_buf *pphi = nullptr, *pu = nullptr;
   double *r2 = nullptr;
   bool forward_done = false;
   int64_t fwd_steps = 0, rev_steps = 0;
@@ -571,66 +611,70 @@ int cal_free(lgd_calibration *c) {
     lgd_buf_free(c->w_phi[i]);
     lgd_buf_free(c->w_u[i]);
   }
-  lgd_buf_free(c->pphi);
-  lgd_buf_free(c->pu);
   if (c->r2) (void)hipFree(c->r2);
   lgd_adjoint_destroy(c->adj);
   delete c;
   return LGD_OK;
 }
 
-// Copy slot s into work pair 0.
-int load_slot(lgd_calibration *c, int s) {
-  int rc = lgd_buf_copy(c->ctx, c->w_phi[0], c->slot_phi[s]);
-  if (!rc) rc = lgd_buf_copy(c->ctx, c->w_u[0], c->slot_u[s]);
-  return rc;
-}
-
-int store_slot(lgd_calibration *c, int s, int64_t pos) {
-  int rc = lgd_buf_copy(c->ctx, c->slot_phi[s], c->w_phi[0]);
-  if (!rc) rc = lgd_buf_copy(c->ctx, c->slot_u[s], c->w_u[0]);
-  if (!rc) c->slot_pos[s] = pos;
-  return rc;
-}
-
-// Advance work pair 0 by n forward steps (the result ends in pair 0).
-int advance(lgd_calibration *c, int64_t n) {
-  for (int64_t s = 0; s < n; ++s) {
-    int rc = lgd_modelc_step(c->ctx, c->grid, &c->params, c->w_phi[0], c->w_u[0], c->w_phi[1],
-                             c->w_u[1], c->dt, 0, 0, 0);
+// Advance the state (sp, su) by m >= 1 forward steps.  Intermediate states
+// alternate between the two work pairs (never the source); the last lands in
+// (dp, du) when given (a checkpoint slot), else in a work pair.  *op, *ou receive
+// where the result is.  No state is ever copied.
+int advance_from(lgd_calibration *c, const lgd_buf *sp, const lgd_buf *su, int64_t m, lgd_buf *dp,
+                 lgd_buf *du, const lgd_buf **op, const lgd_buf **ou) {
+  const lgd_buf *cp = sp, *cu = su;
+  int wi = 0;
+  for (int64_t s = 0; s < m; ++s) {
+    lgd_buf *np, *nu;
+    if (s == m - 1 && dp) {
+      np = dp;
+      nu = du;
+    } else {
+      if (c->w_phi[wi] == cp) wi ^= 1;
+      np = c->w_phi[wi];
+      nu = c->w_u[wi];
+      wi ^= 1;
+    }
+    int rc = lgd_modelc_step(c->ctx, c->grid, &c->params, cp, cu, np, nu, c->dt, 0, 0, 0);
     if (rc) return rc;
-    std::swap(c->w_phi[0], c->w_phi[1]);
-    std::swap(c->w_u[0], c->w_u[1]);
     ++c->fwd_steps;
+    cp = np;
+    cu = nu;
   }
+  *op = cp;
+  *ou = cu;
   return LGD_OK;
 }
 
 int reverse_step(lgd_calibration *c, const lgd_buf *phi, const lgd_buf *u) {
   ++c->rev_steps;
-  return lgd_modelc_adjoint_step(c->adj, &c->params, phi, u, c->dt, c->pphi, c->pu);
+  return adjoint_passes(c->adj, c->params, phi->data, u->data, c->dt, nullptr, nullptr, false);
 }
 
 // Reverse steps [i0, i0 + n) given the state at i0 in slot `s0`, with slots
 // s0+1 .. s0+free available.
 int reverse(lgd_calibration *c, int64_t i0, int64_t n, int s0, int free) {
   int rc;
+  const lgd_buf *p, *u;
   if (n <= 0) return LGD_OK;
   if (n == 1) return reverse_step(c, c->slot_phi[s0], c->slot_u[s0]);
   if (free == 0) {
     for (int64_t m = n - 1; m >= 0; --m) {
-      if ((rc = load_slot(c, s0))) return rc;
-      if ((rc = advance(c, m))) return rc;
-      if ((rc = reverse_step(c, c->w_phi[0], c->w_u[0]))) return rc;
+      p = c->slot_phi[s0];
+      u = c->slot_u[s0];
+      if (m > 0 && (rc = advance_from(c, p, u, m, nullptr, nullptr, &p, &u))) return rc;
+      if ((rc = reverse_step(c, p, u))) return rc;
     }
     return LGD_OK;
   }
   const int64_t m = best_split(n, free);
   const int s1 = s0 + 1;
   if (c->slot_pos[s1] != i0 + m) {
-    if ((rc = load_slot(c, s0))) return rc;
-    if ((rc = advance(c, m))) return rc;
-    if ((rc = store_slot(c, s1, i0 + m))) return rc;
+    if ((rc = advance_from(c, c->slot_phi[s0], c->slot_u[s0], m, c->slot_phi[s1], c->slot_u[s1],
+                           &p, &u)))
+      return rc;
+    c->slot_pos[s1] = i0 + m;
   }
   if ((rc = reverse(c, i0 + m, n - m, s1, free - 1))) return rc;
   return reverse(c, i0, m, s0, free);
@@ -683,8 +727,6 @@ int lgd_calibration_create(lgd_ctx *ctx, const lgd_grid *grid, int64_t nsteps, d
     rc = lgd_buf_alloc(ctx, len, &c->w_phi[i]);
     if (!rc) rc = lgd_buf_alloc(ctx, len, &c->w_u[i]);
   }
-  if (!rc) rc = lgd_buf_alloc(ctx, len, &c->pphi);
-  if (!rc) rc = lgd_buf_alloc(ctx, len, &c->pu);
   if (!rc) {
     hipError_t e = hipMalloc((void **)&c->r2, (size_t)(c->adj->ncell + 1) * sizeof(double));
     if (e != hipSuccess) {
@@ -720,25 +762,28 @@ int lgd_calibration_forward(lgd_calibration *c, const lgd_modelc_params *params,
   c->fwd_steps = c->rev_steps = 0;
   c->forward_done = false;
   for (auto &p : c->slot_pos) p = -1;
-  // Slot 0 holds the initial state; the sweep lays down the schedule's first chain.
-  if ((rc = lgd_buf_copy(ctx, c->w_phi[0], phi0))) return rc;
-  if ((rc = lgd_buf_copy(ctx, c->w_u[0], u0))) return rc;
-  if ((rc = store_slot(c, 0, 0))) return rc;
+  // Slot 0 holds the initial state; the sweep lays down the schedule's first chain,
+  // each checkpoint written by the step that produces it.
+  if ((rc = lgd_buf_copy(ctx, c->slot_phi[0], phi0))) return rc;
+  if ((rc = lgd_buf_copy(ctx, c->slot_u[0], u0))) return rc;
+  c->slot_pos[0] = 0;
+  const lgd_buf *p = c->slot_phi[0], *u = c->slot_u[0];
   int64_t i0 = 0, n = c->nsteps;
   int free = c->snapshots - 1, s = 0;
   while (n >= 2 && free >= 1) {
     const int64_t m = best_split(n, free);
-    if ((rc = advance(c, m))) return rc;
-    if ((rc = store_slot(c, ++s, i0 + m))) return rc;
+    ++s;
+    if ((rc = advance_from(c, p, u, m, c->slot_phi[s], c->slot_u[s], &p, &u))) return rc;
+    c->slot_pos[s] = i0 + m;
     i0 += m;
     n -= m;
     --free;
   }
-  if ((rc = advance(c, n))) return rc;
-  // Terminal adjoint and loss.
+  if ((rc = advance_from(c, p, u, n, nullptr, nullptr, &p, &u))) return rc;
+  // Terminal adjoint (in the workspace's global arrays) and loss.
   const AdjGrid &G = c->adj->G;
   adj_loss_kernel<<<blocks_for(c->adj->ncell, kWG), kWG, 0, ctx->stream>>>(
-      G, c->w_phi[0]->data, target->data, c->pphi->data, c->pu->data, c->r2);
+      G, p->data, target->data, c->adj->AP, c->adj->AU, c->r2);
   if ((rc = launch_check(ctx, "calibration loss"))) return rc;
   if ((rc = sum_into(ctx, c->adj->ncell, c->r2, c->r2 + c->adj->ncell))) return rc;
   double s2 = 0.0;
@@ -762,8 +807,11 @@ int lgd_calibration_backward(lgd_calibration *c, double *grad, lgd_buf *gphi0, l
   if ((rc = lgd_adjoint_reset(c->adj))) return rc;
   if ((rc = reverse(c, 0, c->nsteps, 0, c->snapshots - 1))) return rc;
   if ((rc = lgd_adjoint_grad(c->adj, grad))) return rc;
-  if (gphi0 && (rc = lgd_buf_copy(ctx, gphi0, c->pphi))) return rc;
-  if (gu0 && (rc = lgd_buf_copy(ctx, gu0, c->pu))) return rc;
+  if (gphi0 || gu0) {
+    adj_unflatten_kernel<<<blocks_for(c->adj->ncell, kWG), kWG, 0, ctx->stream>>>(
+        c->adj->G, c->adj->AP, c->adj->AU, gphi0 ? gphi0->data : nullptr, gu0 ? gu0->data : nullptr);
+    if ((rc = launch_check(ctx, "calibration gradient fields"))) return rc;
+  }
   c->forward_done = false;
   return LGD_OK;
 }
