@@ -49,6 +49,10 @@ def run_gpu(args, C):
     t0 = time.time()
     sim.initialize_grains(C.grains(), C.delta)
     log(f"[bench] initial condition on {C.cells}^2 in {time.time() - t0:.1f}s")
+    if args.autotune:
+        r = ctx.autotune_modelc_step(grid, params, sim.field(0), sim.field(1), C.dt,
+                                     wiener=C.noise > 0.0, reps=5)
+        log(f"[bench] autotuned launch shape: {r}")
     sim.step(C.dt, args.warmup)
     ctx.sync()
     torch.cuda.synchronize()
@@ -164,6 +168,9 @@ def run_adjoint(args) -> int:
     target.copy_from(simt.field(0))
     simt.close()
     ctx.sync()
+    if args.autotune:
+        r = ctx.autotune_modelc_step(grid, p, phi0, u0, A.dt, reps=20)
+        log(f"[adjoint] autotuned launch shape: {r}")
     # One plain forward run (warm-up run first).
     sim0.step(A.dt, 20)
     ctx.sync()
@@ -546,6 +553,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=0, choices=[0, 1, 2],
                     help="rows the streaming kernel loads ahead (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every launch shape of the Model C step first and use the fastest")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run one axpy of known traffic (for profiler byte-counter scaling)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)

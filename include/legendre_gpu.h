@@ -103,6 +103,23 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
  * level — the per-block dispatch baseline).
  * Results are bitwise independent of every knob. */
 int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value);
+/* Autotuner (no reference counterpart: the reference has no launch shapes).  Times
+ * every launch shape of lgd_modelc_step — one thread per cell, or streaming with
+ * 1..32 strips per block and one or two rows loaded ahead — on (phi, u) over this
+ * grid (outputs to scratch), `reps` launches each, and keeps the fastest for this
+ * (block shape, block count, noise) in the context; lgd_modelc_step and the ensemble
+ * step use it whenever no knob is set.  kernel: 1 per-cell, 2 streaming. */
+typedef struct {
+  int32_t kernel;
+  int32_t strips;
+  int32_t prefetch;
+  int32_t candidates;
+  double ms;          /* the chosen shape's mean launch time */
+  double default_ms;  /* the untuned default's (streaming, 4 strips, 2 rows ahead) */
+} lgd_tuning_result;
+int lgd_autotune_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                             const lgd_buf *phi, const lgd_buf *u, double dt, int32_t wiener,
+                             int32_t reps, lgd_tuning_result *out);
 
 /* ------------------------------------- Allocator / StorageBackend (storage.rs) */
 /* Allocator::allocate(len): zero-initialised storage of len f64

@@ -79,6 +79,19 @@ class Context:
         """Launch-shape knob (results are bitwise independent of it)."""
         _abi.check(self._lib.lgd_ctx_set_tuning(self.handle, key.encode(), int(value)), self.handle)
 
+    def autotune_modelc_step(self, grid: "CartesianGrid", params: "ModelCParams", phi: "DeviceBuffer",
+                             u: "DeviceBuffer", dt: float, wiener: bool = False, reps: int = 5) -> dict:
+        """Time every launch shape of the fused Model C step on (phi, u) and keep
+        the fastest for this grid shape (used while no knob is set)."""
+        r = _abi.TuningResultC()
+        pc = params.c()
+        _abi.check(self._lib.lgd_autotune_modelc_step(self.handle, grid.handle, C.byref(pc), phi.handle,
+                                                      u.handle, dt, int(wiener), int(reps), C.byref(r)),
+                   self.handle)
+        return {"kernel": "percell" if r.kernel == 1 else "stream", "strips": r.strips,
+                "prefetch": r.prefetch, "ms": r.ms, "default_ms": r.default_ms,
+                "candidates": r.candidates}
+
     def close(self) -> None:
         if self.handle:
             for ref in reversed(self._children):  # dependents before what they use

@@ -75,6 +75,17 @@ class RealizationC(C.Structure):
     ]
 
 
+class TuningResultC(C.Structure):
+    _fields_ = [
+        ("kernel", C.c_int32),
+        ("strips", C.c_int32),
+        ("prefetch", C.c_int32),
+        ("candidates", C.c_int32),
+        ("ms", C.c_double),
+        ("default_ms", C.c_double),
+    ]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 I32, I64, U32, U64, SZ, DBL = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_size_t, C.c_double
@@ -89,6 +100,8 @@ SIGNATURES = {
     "lgd_ctx_stream": (P, [P]),
     "lgd_ctx_last_error": (C.c_char_p, [P]),
     "lgd_ctx_set_tuning": (C.c_int, [P, C.c_char_p, I64]),
+    "lgd_autotune_modelc_step": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P, P, DBL, I32, I32,
+                                           C.POINTER(TuningResultC)]),
     "lgd_buf_alloc": (C.c_int, [P, SZ, PP]),
     "lgd_buf_free": (C.c_int, [P]),
     "lgd_buf_len": (SZ, [P]),

@@ -225,7 +225,7 @@ int lgd_ensemble_step(lgd_ensemble *e, double dt, int64_t nsteps, int32_t per_bl
   bool noise = false;
   for (const lgd_realization &R : e->real) noise = noise || R.noise_amplitude != 0.0;
   const int64_t nblocks = e->block0.back();
-  const int strips = e->ctx->modelc_strips > 0 ? (int)e->ctx->modelc_strips : 1;
+  const lgd_launch_shape shape = lgd_modelc_shape(e->ctx, e->nx, e->ny, nblocks, noise, 1);
   const uint64_t wiener0 = (1ull << 32) | 0ull;
   for (int64_t s = 0; s < nsteps; ++s) {
     uint64_t salt;
@@ -234,7 +234,7 @@ int lgd_ensemble_step(lgd_ensemble *e, double dt, int64_t nsteps, int32_t per_bl
       rc = lgd_modelc_stream_launch(e->ctx, e->nx, e->ny, e->domains[0], e->d_domains,
                                     e->d_block_domain, 0, nblocks, e->state[0]->data,
                                     e->state[1]->data, e->next[0]->data, e->next[1]->data, dt,
-                                    noise, salt, wiener0, strips);
+                                    noise, salt, wiener0, shape.strips, shape.prefetch);
     } else {
       // The per-block dispatch baseline: one launch per block, each in its
       // fastest single-block shape (one wave per row).
@@ -242,7 +242,7 @@ int lgd_ensemble_step(lgd_ensemble *e, double dt, int64_t nsteps, int32_t per_bl
         rc = lgd_modelc_stream_launch(e->ctx, e->nx, e->ny, e->domains[0], e->d_domains,
                                       e->d_block_domain, b, 1, e->state[0]->data,
                                       e->state[1]->data, e->next[0]->data, e->next[1]->data, dt,
-                                      noise, salt, wiener0, (int)e->ny);
+                                      noise, salt, wiener0, (int)e->ny, shape.prefetch);
     }
     if (rc) return rc;
     std::swap(e->state[0], e->next[0]);

@@ -142,11 +142,8 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
   const bool noise_on = wiener && params->noise_amplitude != 0.0;
   const double noise_scale = std::sqrt(dt) * params->noise_amplitude;
   const uint64_t wiener0 = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
-  if (ctx->modelc_kernel != 1) {
-    // Streaming kernel: four strips per block by default (measured best at
-    // 10080² with 126² blocks: enough waves to fill the chip and balance the
-    // tail, for 2 extra halo rows per strip).
-    int strips = ctx->modelc_strips > 0 ? (int)ctx->modelc_strips : 4;
+  const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, noise_on, 4);
+  if (shape.kernel != 1) {
     lgd_domain d0;
     d0.k = k;
     d0.noise_scale = noise_on ? noise_scale : 0.0;
@@ -158,7 +155,7 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
     d0.per_y = c.periodic[1];
     rc = lgd_modelc_stream_launch(ctx, c.n[0], c.n[1], d0, nullptr, nullptr, 0, c.nblocks,
                                   phi->data, u->data, phi_next->data, u_next->data, dt, noise_on,
-                                  salt, wiener0, strips);
+                                  salt, wiener0, shape.strips, shape.prefetch);
     if (rc != LGD_ERR_UNSUPPORTED) return rc;
     if (ctx->modelc_kernel == 2)
       return fail(ctx, LGD_ERR_UNSUPPORTED, "streaming kernel needs blocks at most 126 cells wide");

@@ -5,9 +5,19 @@
 
 #include <cstdint>
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 
 #include "../../include/legendre_gpu.h"
+
+// A launch shape of the fused Model C step: kernel 1 = one thread per cell,
+// 2 = streaming (strips per block, rows loaded ahead).
+struct lgd_launch_shape {
+  int32_t kernel = 2;
+  int32_t strips = 4;
+  int32_t prefetch = 2;
+};
 
 struct lgd_ctx {
   int device = 0;
@@ -19,7 +29,15 @@ struct lgd_ctx {
   int64_t modelc_strips = 0;  // strips (row ranges) per block for the streaming kernel
   int64_t modelc_prefetch = 0;  // rows the streaming kernel loads ahead (1 or 2; 0 = default)
   int64_t amr_per_block = 0;    // 1 = AMR level kernels launched once per patch (baseline)
+  // Shapes chosen by lgd_autotune_modelc_step, keyed by (nx, ny, blocks, noise).
+  std::map<std::tuple<int64_t, int64_t, int64_t, int>, lgd_launch_shape> tuned;
 };
+
+// The launch shape for a Model C step over `nblocks` blocks of nx × ny: explicit
+// knobs first, then the autotuned shape for this key, then the defaults
+// (streaming, `default_strips` strips, two rows ahead).
+lgd_launch_shape lgd_modelc_shape(const lgd_ctx *ctx, int64_t nx, int64_t ny, int64_t nblocks,
+                                  bool noise, int32_t default_strips);
 
 struct lgd_buf {
   lgd_ctx *ctx = nullptr;
@@ -137,7 +155,7 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
                              const lgd_domain *domains, const int32_t *block_domain,
                              int64_t block_begin, int64_t nblocks, const double *phi,
                              const double *u, double *phi_n, double *u_n, double dt, bool noise,
-                             uint64_t salt, uint64_t stream, int strips);
+                             uint64_t salt, uint64_t stream, int strips, int prefetch);
 
 // face(dx, dy) (src/discretization/finite_volume.rs:129-142): anisotropy A and its
 // angular derivative with the |∇φ|² normalisation folded in.

@@ -359,7 +359,7 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
                              const lgd_domain *domains, const int32_t *block_domain,
                              int64_t block_begin, int64_t nblocks, const double *phi,
                              const double *u, double *phi_n, double *u_n, double dt, bool noise,
-                             uint64_t salt, uint64_t stream, int strips) {
+                             uint64_t salt, uint64_t stream, int strips, int prefetch) {
   if (nx + 2 > 128 || nx < 1 || ny < 1) return LGD_ERR_UNSUPPORTED;
   if (nblocks <= 0) return LGD_OK;
   StreamArgs a;
@@ -387,7 +387,7 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   a.nwg = (int32_t)nwg;
   const bool vec = ((nx + 2) % 2) == 0;
   const bool table = domains != nullptr;
-  const int depth = ctx->modelc_prefetch == 1 ? 1 : 2;
+  const int depth = prefetch == 1 ? 1 : 2;
   dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
   if (depth == 2)
     launch_variant<2>(grid, block, ctx->stream, a, noise, vec, table);

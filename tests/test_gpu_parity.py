@@ -360,3 +360,26 @@ def test_modelc_full_scale_first_steps_bitwise(ctx, orc):
     gu = H.field_view(sim.field(1).download(), cells, block, 1)
     uv = H.field_view(u, cells, block, 1)
     assert np.array_equal(gu[:, 1:-1, 1:-1], uv[:, 1:-1, 1:-1])
+
+
+def test_autotuner_picks_a_shape_and_results_do_not_change(ctx, orc, tuned):
+    """The autotuner times every launch shape and keeps one; the step it then
+    uses computes the same bits as the default shape."""
+    cells, block = [256, 192], [64, 48]
+    say(f"autotune cells={cells} block={block}")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False],
+                                           np.array([[30.0, 25.0, 3.0]]), 0.0, False, lg.LGD_MODE_FUSED)
+    params = lg.ModelCParams(0.05, 3.19, 2.0)
+    r = ctx.autotune_modelc_step(gg, params, sim.field(0), sim.field(1), 0.008, reps=2)
+    assert r["candidates"] >= 10 and r["ms"] > 0.0 and r["ms"] <= r["default_ms"] * 1.5
+    assert r["kernel"] in ("percell", "stream")
+    sim.step(0.008, 30)
+    tuned_out = (sim.field(0).download(), sim.field(1).download())
+    tuned(2, 4, 2)  # the explicit default shape
+    og2, gg2, sim2, *_ = _modelc_pair(ctx, orc, cells, block, [False, False],
+                                      np.array([[30.0, 25.0, 3.0]]), 0.0, False, lg.LGD_MODE_FUSED)
+    sim2.step(0.008, 30)
+    assert np.array_equal(_interiors(sim2.field(0).download(), cells, block),
+                          _interiors(tuned_out[0], cells, block))
+    assert np.array_equal(_interiors(sim2.field(1).download(), cells, block),
+                          _interiors(tuned_out[1], cells, block))
