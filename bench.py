@@ -148,6 +148,8 @@ def run_adjoint(args) -> int:
     ctx = lg.Context(0)
     ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2}[args.kernel])
     ctx.set_tuning("modelc_strips", args.strips)
+    ctx.set_tuning("adjoint_kernel", {"auto": 0, "passes": 1, "stream": 2}[args.adj_kernel])
+    ctx.set_tuning("adjoint_rows", args.adj_rows)
     grid = lg.CartesianGrid(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])
     p = lg.ModelCParams(A.eps4, A.lambda_, A.d_thermal)
     pt = lg.ModelCParams(A.target_eps4, A.target_lambda, A.target_d)
@@ -187,18 +189,21 @@ def run_adjoint(args) -> int:
     log(f"[adjoint] forward {A.steps} steps: {t_fwd * 1e3:.2f} ms")
     cal = lg.ModelCCalibration(grid, A.steps, A.dt, snapshots=A.snapshots)
     gphi, gu = grid.alloc_field(), grid.alloc_field()
-    runs = []
+    runs, bwd = [], []
     for _ in range(2):
         ctx.sync()
         t = time.perf_counter()
         loss = cal.forward(p, phi0, u0, target)
+        tb = time.perf_counter()
         grad = cal.backward(gphi, gu)
         ctx.sync()
         runs.append(time.perf_counter() - t)
+        bwd.append(time.perf_counter() - tb)
     t_grad = min(runs)
+    t_bwd = min(bwd)
     nfwd, nrev = cal.counts()
-    log(f"[adjoint] gradient: {t_grad * 1e3:.2f} ms ({nfwd} forward + {nrev} reverse steps), "
-        f"loss {loss:.10e}, grad {grad}")
+    log(f"[adjoint] gradient: {t_grad * 1e3:.2f} ms ({nfwd} forward + {nrev} reverse steps; "
+        f"backward sweep {t_bwd * 1e3:.2f} ms), loss {loss:.10e}, grad {grad}")
     # Central differences of the device loss.
     base = dict(eps4=A.eps4, lambda_=A.lambda_, d_thermal=A.d_thermal)
     fd = {}
@@ -279,6 +284,8 @@ def run_adjoint(args) -> int:
             "forward_ms": round(t_fwd * 1e3, 3),
             "gradient_ms": round(t_grad * 1e3, 3),
             "gradient_over_forward": round(t_grad / t_fwd, 3),
+            "backward_ms": round(t_bwd * 1e3, 3),
+            "reverse_kernel": args.adj_kernel,
             "forward_steps": nfwd,
             "reverse_steps": nrev,
             "loss": loss,
@@ -542,6 +549,11 @@ def main() -> int:
                     help="realizations re-run on the oracle (ensemble)")
     ap.add_argument("--adj-steps", type=int, default=0, help="adjoint workload steps (0 = 2000)")
     ap.add_argument("--snapshots", type=int, default=0, help="adjoint checkpoint slots (0 = 45)")
+    ap.add_argument("--adj-kernel", choices=["auto", "passes", "stream"], default="auto",
+                    help="reverse step: streaming kernel where the block shape allows (auto), "
+                         "the three passes, or streaming required")
+    ap.add_argument("--adj-rows", type=int, default=0,
+                    help="rows per strip of the streaming reverse step (0 = chosen for the device)")
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--noise", type=float, default=0.0)

@@ -20,8 +20,12 @@
 //              stencil reads it — at its own position and at each ghost position
 //              that resolves to it (mirror reflection or periodic wrap), so the
 //              transpose of the ghost fill needs no scatter and no atomics.
-// θ partials accumulate per cell / per face across steps and are reduced once, in a
-// fixed order, so the gradient is bitwise reproducible.
+// Where the block shape allows, one streaming kernel (lgd_adjoint_stream.hip)
+// replaces the three passes: it reads the state slabs and the incoming global
+// adjoint and writes the outgoing one into a second pair of global arrays.
+// θ partials accumulate per cell / per face (per wave in the streaming kernel)
+// across steps and are reduced once, in a fixed order, so the gradient is bitwise
+// reproducible.
 //
 // Checkpointing: binomial (Griewank–Walther) reversal with a fixed number of state
 // slots; the schedule's first chain of checkpoints is laid down by the forward sweep.
@@ -312,6 +316,17 @@ __global__ __launch_bounds__(kWG) void adj_gather_kernel(AdjGrid G, lgd_kr_const
   }
 }
 
+// Slab interiors → global adjoint arrays.
+__global__ __launch_bounds__(kWG) void adj_flatten_kernel(AdjGrid G, const double *pphi,
+                                                          const double *pu, double *AP,
+                                                          double *AU) {
+  const int t = blockIdx.x * kWG + threadIdx.x;
+  if (t >= G.N0 * G.N1) return;
+  const int64_t o = slab_off(G, t % G.N0, t / G.N0);
+  AP[t] = pphi[o];
+  AU[t] = pu[o];
+}
+
 // Global adjoint arrays → slab interiors.
 __global__ __launch_bounds__(kWG) void adj_unflatten_kernel(AdjGrid G, const double *AP,
                                                             const double *AU, double *pphi,
@@ -415,6 +430,12 @@ struct lgd_adjoint {
   double *GXx = nullptr, *GYx = nullptr, *GXy = nullptr, *GYy = nullptr;
   // θ accumulators, per workgroup: cell pass (λ, D, ε₄ centre) and face pass (ε₄ faces).
   double *acc_cell = nullptr, *acc_face = nullptr;
+  // Streaming reverse step: the outgoing adjoint arrays (swapped with AP, AU after
+  // each step) and 4 θ partials per wave, sized for the most waves any strip
+  // height gives (two rows per strip).
+  double *AP2 = nullptr, *AU2 = nullptr, *acc_stream = nullptr;
+  int64_t nacc_stream = 0;
+  bool stream_ok = false;
   double *sums = nullptr;  // reduction outputs
   std::vector<double *> allocs;
 };
@@ -459,6 +480,27 @@ int adjoint_passes(lgd_adjoint *a, const lgd_modelc_params &params, const double
   int rc;
   const lgd_kr_consts k = lgd_make_kr_consts(params, a->grid->cart);
   const AdjGrid &G = a->G;
+  if (ctx->adjoint_kernel == 2 && !a->stream_ok)
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "streaming adjoint step: unsupported block shape");
+  if (a->stream_ok && ctx->adjoint_kernel != 1) {
+    const int rows = (int)std::min<int64_t>(ctx->adjoint_rows, 1 << 20);  // 0: by occupancy
+    if (slab) {
+      adj_flatten_kernel<<<blocks_for(a->ncell, kWG), kWG, 0, ctx->stream>>>(G, pphi, pu, a->AP,
+                                                                            a->AU);
+      if ((rc = launch_check(ctx, "adjoint flatten"))) return rc;
+    }
+    rc = lgd_adjoint_stream_launch(ctx, a->grid->cart, k, dt, phi, u, a->AP, a->AU, a->AP2, a->AU2,
+                                   a->acc_stream, rows, nullptr);
+    if (rc) return rc;
+    std::swap(a->AP, a->AP2);
+    std::swap(a->AU, a->AU2);
+    if (slab) {
+      adj_unflatten_kernel<<<blocks_for(a->ncell, kWG), kWG, 0, ctx->stream>>>(G, a->AP, a->AU,
+                                                                              pphi, pu);
+      return launch_check(ctx, "adjoint unflatten");
+    }
+    return LGD_OK;
+  }
   if (slab)
     adj_stage_kernel<true><<<blocks_for(a->npad, kWG), kWG, 0, ctx->stream>>>(
         G, phi, u, pphi, pu, a->Pp, a->Up, a->AP, a->AU);
@@ -530,6 +572,19 @@ int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out) {
   if (!rc) rc = adj_alloc(a, 3 * a->wg_cell, &a->acc_cell);
   if (!rc) rc = adj_alloc(a, a->wg_face, &a->acc_face);
   if (!rc) rc = adj_alloc(a, 8, &a->sums);
+  {
+    // The streaming step's wave count is largest at two rows per strip.
+    int64_t nw = 0;
+    const lgd_kr_consts k0{};
+    a->stream_ok = lgd_adjoint_stream_launch(ctx, c, k0, 0.0, nullptr, nullptr, nullptr, nullptr,
+                                             nullptr, nullptr, nullptr, 2, &nw) == LGD_OK;
+    if (a->stream_ok) {
+      a->nacc_stream = 4 * nw;
+      if (!rc) rc = adj_alloc(a, a->ncell, &a->AP2);
+      if (!rc) rc = adj_alloc(a, a->ncell, &a->AU2);
+      if (!rc) rc = adj_alloc(a, a->nacc_stream, &a->acc_stream);
+    }
+  }
   if (!rc) rc = lgd_adjoint_reset(a);
   if (rc) {
     lgd_adjoint_destroy(a);
@@ -550,6 +605,7 @@ int lgd_adjoint_reset(lgd_adjoint *a) {
   if (!a) return LGD_ERR_INVALID_ARGUMENT;
   int rc;
   if ((rc = zero(a->ctx, a->acc_cell, 3 * a->wg_cell))) return rc;
+  if (a->acc_stream && (rc = zero(a->ctx, a->acc_stream, a->nacc_stream))) return rc;
   return zero(a->ctx, a->acc_face, a->wg_face);
 }
 
@@ -573,13 +629,16 @@ int lgd_adjoint_grad(lgd_adjoint *a, double *grad) {
   int rc;
   if ((rc = sum_strided(ctx, a->wg_cell, a->acc_cell, 3, a->sums))) return rc;
   if ((rc = sum_strided(ctx, a->wg_face, a->acc_face, 1, a->sums + 3))) return rc;
-  double h[4];
-  hipError_t e = hipMemcpyAsync(h, a->sums, sizeof h, hipMemcpyDeviceToHost, ctx->stream);
+  double h[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const size_t nh = a->acc_stream ? 8 : 4;
+  if (a->acc_stream && (rc = sum_strided(ctx, a->nacc_stream / 4, a->acc_stream, 4, a->sums + 4)))
+    return rc;
+  hipError_t e = hipMemcpyAsync(h, a->sums, nh * sizeof(double), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "adjoint gradient download");
-  grad[0] = h[0];
-  grad[1] = h[1];
-  grad[2] = h[2] + h[3];
+  grad[0] = h[0] + h[4];
+  grad[1] = h[1] + h[5];
+  grad[2] = (h[2] + h[3]) + (h[6] + h[7]);
   return LGD_OK;
 }
 

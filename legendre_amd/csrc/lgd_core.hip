@@ -92,6 +92,9 @@ int lgd_ctx_create(int32_t device, void *stream, lgd_ctx **out) {
     }
     ctx->owns_stream = true;
   }
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
+    ctx->num_cus = cus;
   *out = ctx;
   return LGD_OK;
 }
@@ -132,6 +135,11 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
   } else if (k == "amr_per_block") {
     if (value > 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_per_block is 0 or 1");
     ctx->amr_per_block = value;
+  } else if (k == "adjoint_kernel") {
+    if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "adjoint_kernel is 0, 1 or 2");
+    ctx->adjoint_kernel = value;
+  } else if (k == "adjoint_rows") {
+    ctx->adjoint_rows = value;
   } else {
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "unknown tuning key " + k);
   }

@@ -23,12 +23,15 @@ struct lgd_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   bool owns_stream = false;
+  int32_t num_cus = 0;  // compute units of the device (0 = unknown)
   std::string last_error;
   // Launch-shape knobs (lgd_ctx_set_tuning); 0 = pick automatically.
   int64_t modelc_kernel = 0;  // 1 = per-cell kernel, 2 = streaming kernel
   int64_t modelc_strips = 0;  // strips (row ranges) per block for the streaming kernel
   int64_t modelc_prefetch = 0;  // rows the streaming kernel loads ahead (1 or 2; 0 = default)
   int64_t amr_per_block = 0;    // 1 = AMR level kernels launched once per patch (baseline)
+  int64_t adjoint_kernel = 0;   // 1 = three-pass reverse step, 2 = streaming required
+  int64_t adjoint_rows = 0;     // rows per strip of the streaming reverse step (0 = by device)
   // Shapes chosen by lgd_autotune_modelc_step, keyed by (nx, ny, blocks, noise).
   std::map<std::tuple<int64_t, int64_t, int64_t, int>, lgd_launch_shape> tuned;
 };
@@ -156,6 +159,16 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
                              int64_t block_begin, int64_t nblocks, const double *phi,
                              const double *u, double *phi_n, double *u_n, double dt, bool noise,
                              uint64_t salt, uint64_t stream, int strips, int prefetch);
+
+// Streaming reverse Model C step (lgd_adjoint_stream.hip): adjoint (APi, AUi) of the
+// state after the step → (APo, AUo) before it, global N0 × N1 arrays; θ partials
+// (λ, D, ε₄ centre, ε₄ faces) added to acc[4·wave + n]; `rows` rows per strip (at
+// least 2; 0 = the height that balances the waves over the device's SIMDs).  With acc == nullptr only *nwaves is set.  LGD_ERR_UNSUPPORTED for a
+// grid that is not 2-D or has fewer than 2 cells along a dimension.
+int lgd_adjoint_stream_launch(lgd_ctx *ctx, const lgd_cart &c, const lgd_kr_consts &k, double dt,
+                              const double *phi, const double *u, const double *APi,
+                              const double *AUi, double *APo, double *AUo, double *acc,
+                              int rows, int64_t *nwaves);
 
 // face(dx, dy) (src/discretization/finite_volume.rs:129-142): anisotropy A and its
 // angular derivative with the |∇φ|² normalisation folded in.

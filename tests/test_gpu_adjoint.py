@@ -58,10 +58,26 @@ def upload(ctx, host):
     return b
 
 
+# Reverse-step paths: the three passes, and the streaming kernel with the default
+# strip height and with strip heights that split blocks raggedly.
+PATHS = {"passes": (1, 0), "stream": (2, 0), "stream_rows3": (2, 3), "stream_rows5": (2, 5)}
+
+
+@pytest.fixture
+def adjoint_path(ctx, request):
+    kernel, rows = PATHS[request.param]
+    ctx.set_tuning("adjoint_kernel", kernel)
+    ctx.set_tuning("adjoint_rows", rows)
+    yield request.param
+    ctx.set_tuning("adjoint_kernel", 0)
+    ctx.set_tuning("adjoint_rows", 0)
+
+
+@pytest.mark.parametrize("adjoint_path", list(PATHS), indirect=True)
 @pytest.mark.parametrize("case", range(len(CASES)))
-def test_adjoint_step_matches_oracle(ctx, orc, case):
+def test_adjoint_step_matches_oracle(ctx, orc, case, adjoint_path):
     cells, block = CASES[case][0], CASES[case][1]
-    say(f"adjoint step cells={cells} block={block}")
+    say(f"adjoint step cells={cells} block={block} path={adjoint_path}")
     og, op, phi, u = state(orc, case)
     rng = np.random.default_rng(case)
     pphi = H.global_to_field(rng.standard_normal(cells[::-1]), cells, block, 1)
@@ -126,6 +142,35 @@ def test_calibration_gradient_matches_oracle_for_every_snapshot_count(ctx, orc, 
             assert np.array_equal(out[1], ref[1])
             assert np.array_equal(out[2], ref[2]) and np.array_equal(out[3], ref[3])
         cal.close()
+
+
+@pytest.mark.parametrize("case", [0, 2])
+def test_calibration_streaming_and_three_pass_agree(ctx, orc, case):
+    """The streaming reverse step and the three passes are the same derivative; they
+    sum in different orders, so they agree to rounding, not bitwise."""
+    cells, block = CASES[case][0], CASES[case][1]
+    steps = 12
+    say(f"calibration paths cells={cells} steps={steps}")
+    og, op, phi, u, target = _calibration_inputs(orc, case, steps)
+    grid = gpu_grid(ctx, case)
+    params = lg.ModelCParams(0.05, 3.19, 2.0)
+    dphi, du, dt_ = upload(ctx, phi), upload(ctx, u), upload(ctx, target)
+    outs = []
+    try:
+        for kernel in (1, 2):
+            ctx.set_tuning("adjoint_kernel", kernel)
+            cal = lg.ModelCCalibration(grid, steps, DT, snapshots=3)
+            gp, gu = grid.alloc_field(), grid.alloc_field()
+            cal.forward(params, dphi, du, dt_)
+            grad = cal.backward(gp, gu)
+            outs.append((grad, H.field_to_global(gp.download(), cells, block, 1),
+                         H.field_to_global(gu.download(), cells, block, 1)))
+            cal.close()
+    finally:
+        ctx.set_tuning("adjoint_kernel", 0)
+    (g1, p1, u1), (g2, p2, u2) = outs
+    assert np.allclose(g2, g1, rtol=1e-11, atol=0.0), (g1, g2)
+    assert rel_close(p2, p1, 1e-11) and rel_close(u2, u1, 1e-11)
 
 
 def test_calibration_gradient_matches_central_differences(ctx, orc):
