@@ -101,7 +101,13 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
  * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default) and
  * "amr_per_block" (1: the AMR level kernels launch once per patch instead of once per
  * level — the per-block dispatch baseline).
- * Results are bitwise independent of every knob. */
+ * Results are bitwise independent of these knobs.  The adjoint has two more:
+ * "adjoint_kernel" (0: the streaming reverse step wherever the grid allows, else the
+ * three-pass form; 1: always the three passes; 2: streaming, error if the grid does
+ * not allow it) and "adjoint_rows" (rows per strip of the streaming reverse step,
+ * 0 = balanced over the device's SIMDs).  The two reverse-step forms sum in
+ * different orders and agree to rounding; for a fixed setting every result is
+ * bitwise reproducible. */
 int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value);
 /* Autotuner (no reference counterpart: the reference has no launch shapes).  Times
  * every launch shape of lgd_modelc_step — one thread per cell, or streaming with
