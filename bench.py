@@ -46,6 +46,8 @@ def run_gpu(args, C):
     params = lg.ModelCParams(C.eps4, C.lambda_, C.d_thermal, C.noise)
     sim = lg.Simulation.model_c(grid, params, wiener=C.noise > 0.0, seed=C.seed,
                                 mode=lg.LGD_MODE_FUSED if args.mode == "fused" else lg.LGD_MODE_PER_INTENT)
+    if args.graph:
+        sim.set_graph_steps(args.graph)
     t0 = time.time()
     sim.initialize_grains(C.grains(), C.delta)
     log(f"[bench] initial condition on {C.cells}^2 in {time.time() - t0:.1f}s")
@@ -445,6 +447,8 @@ def run_amr(args) -> int:
         gg = lg.AmrGrid.from_patches(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h],
                                      list(A.ratios), [])
         sim = lg.amr_simulation(gg, "modelc", params=params, subcycle=True, policy=pol)
+        if args.graph and not ctx_per_block[0]:
+            sim.set_graph_steps(args.graph)
         sim.field(0).copy_from(ic.field(0))
         sim.field(1).copy_from(ic.field(1))
         ctx.sync()
@@ -478,11 +482,13 @@ def run_amr(args) -> int:
         return times, updates, shapes, first
 
     nchunks = max(1, A.coarse_steps // A.every)
+    ctx_per_block = [False]
     sim = build()
     times, updates, shapes, first = run(sim, nchunks, "batched")
     sim.close()
     pb_chunks = min(nchunks, max(1, args.per_block_chunks))
     ctx.set_tuning("amr_per_block", 1)
+    ctx_per_block[0] = True  # the baseline launches eagerly, one launch per patch
     sim = build()
     pb_times, pb_updates, pb_shapes, pb_first = run(sim, pb_chunks, "per-patch launches")
     sim.close()
@@ -511,6 +517,7 @@ def run_amr(args) -> int:
             "base_cells": A.cells * A.cells,
             "ratios": list(A.ratios),
             "regrid_every": A.every,
+            "graph_steps": args.graph,
             "seeds": A.nseeds,
             "noise_b": 0.0,
         },
@@ -549,6 +556,9 @@ def main() -> int:
                     help="realizations re-run on the oracle (ensemble)")
     ap.add_argument("--adj-steps", type=int, default=0, help="adjoint workload steps (0 = 2000)")
     ap.add_argument("--snapshots", type=int, default=0, help="adjoint checkpoint slots (0 = 45)")
+    ap.add_argument("--graph", type=int, default=0,
+                    help="run the simulation's steps as replayed HIP graphs of this many steps "
+                         "(modelc and amr workloads; 0 = eager launches)")
     ap.add_argument("--adj-kernel", choices=["auto", "passes", "stream"], default="auto",
                     help="reverse step: streaming kernel where the block shape allows (auto), "
                          "the three passes, or streaming required")

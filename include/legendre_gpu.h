@@ -313,6 +313,16 @@ int lgd_sim_field(lgd_sim *sim, int32_t index, lgd_buf **out);
 /* nsteps × Simulation::step(dt) (simulation.rs:255-290): t advances by t += dt on
  * the host in f64, exactly as the reference accumulates it (the noise salt). */
 int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps);
+/* Whole-step HIP graphs (no reference counterpart: the reference launches no
+ * kernels; the caller is any driver of lgd_sim_step).  steps > 0: lgd_sim_step runs
+ * `steps` whole steps at a time as one captured graph — every launch of each step,
+ * the subcycled level recursion and the time accumulation — captured on first use
+ * and replayed while the buffers, dt, AMR hierarchy and tuning stay the same.  The
+ * noise salt comes from a device copy of t advanced inside the graph with the same
+ * f64 arithmetic, so results are bitwise those of steps = 0.  AMR regrids stay
+ * host-driven at the top of their step and no graph spans one.  The context's
+ * stream must not be the null stream.  0 turns graphs off (and frees them). */
+int lgd_sim_set_graph_steps(lgd_sim *sim, int64_t steps);
 double lgd_sim_time(const lgd_sim *sim);
 int64_t lgd_sim_step_index(const lgd_sim *sim);
 /* ModelC::initialize_grains with orientation 0 (phasefield.rs:227-286), computed on the

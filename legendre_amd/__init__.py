@@ -418,6 +418,11 @@ class Simulation:
     def step(self, dt: float, nsteps: int = 1) -> None:
         _abi.check(self._lib.lgd_sim_step(self.handle, dt, nsteps), self.ctx.handle)
 
+    def set_graph_steps(self, steps: int) -> None:
+        """Run `steps` whole steps at a time as one replayed HIP graph (0: eager
+        launches).  Results are bitwise those of eager steps."""
+        _abi.check(self._lib.lgd_sim_set_graph_steps(self.handle, int(steps)), self.ctx.handle)
+
     @property
     def time(self) -> float:
         return float(self._lib.lgd_sim_time(self.handle))

@@ -153,6 +153,7 @@ SIGNATURES = {
     "lgd_sim_destroy": (C.c_int, [P]),
     "lgd_sim_field": (C.c_int, [P, I32, PP]),
     "lgd_sim_step": (C.c_int, [P, DBL, I64]),
+    "lgd_sim_set_graph_steps": (C.c_int, [P, I64]),
     "lgd_sim_time": (DBL, [P]),
     "lgd_sim_step_index": (I64, [P]),
     "lgd_sim_initialize_grains": (C.c_int, [P, DP, SZ, DBL]),

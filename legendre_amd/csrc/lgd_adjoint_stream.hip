@@ -194,7 +194,6 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
   double Vmm = 0.0, Vm = 0.0, PUm = 0.0;
   double part_l = 0.0, part_d = 0.0, part_ec = 0.0, part_ef = 0.0;
 
-#pragma unroll 2
   for (int r = y0 - 1; r <= y1; ++r) {
     // Row r+2 of the state and the adjoint at row r+1 (rows y1+1 and y1 again at
     // the last row, where they go unused), then the offsets of the next ones.

@@ -429,13 +429,14 @@ __global__ __launch_bounds__(kTX *kTY) void amr_heat_rhs_kernel(
 // (src/core/driver.rs:139-158; cell key per patch extent, grid.rs:407-425).
 __global__ __launch_bounds__(kTX *kTY) void amr_wiener_kernel(
     const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks,
-    double noise_scale, uint64_t seed, uint64_t salt, uint64_t stream, double *phi) {
+    double noise_scale, uint64_t seed, uint64_t salt, const double *salt_src, uint64_t stream,
+    double *phi) {
   const lgd_tile tl = tiles[blockIdx.x];
   const lgd_block_desc bd = blocks[tl.block];
   const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
   if (i >= bd.nx || j >= bd.ny) return;
   const int64_t o = bd.off + (int64_t)(j + 1) * (bd.nx + 2) + (i + 1);
-  uint64_t bk = lgd_mix_key3(seed, salt, stream, (uint64_t)tl.block);
+  uint64_t bk = lgd_mix_key3(seed, lgd_salt(salt, salt_src), stream, (uint64_t)tl.block);
   uint64_t key = lgd_splitmix64(bk ^ (uint64_t)(i + (int64_t)j * bd.nx));
   phi[o] = phi[o] + noise_scale * lgd_standard_normal(key);
 }
@@ -533,11 +534,12 @@ int lgd_amr_wiener(lgd_ctx *ctx, const lgd_amr &g, int level, double *phi, doubl
     for (size_t b = 0; b + 1 < P.tile0.size(); ++b) {
       const int64_t n = P.tile0[b + 1] - P.tile0[b];
       amr_wiener_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
-          P.tiles + P.tile0[b], g.d_blocks, noise_scale, seed, salt, (1ull << 32) | 0ull, phi);
+          P.tiles + P.tile0[b], g.d_blocks, noise_scale, seed, salt, ctx->salt_src,
+          (1ull << 32) | 0ull, phi);
     }
   } else {
     amr_wiener_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
-        P.tiles, g.d_blocks, noise_scale, seed, salt, (1ull << 32) | 0ull, phi);
+        P.tiles, g.d_blocks, noise_scale, seed, salt, ctx->salt_src, (1ull << 32) | 0ull, phi);
   }
   return launch_check(ctx, "amr_wiener");
 }

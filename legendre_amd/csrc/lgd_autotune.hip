@@ -115,6 +115,7 @@ int lgd_autotune_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_model
   lgd_buf_free(un);
   if (rc) return rc;
   ctx->tuned[std::make_tuple(c.n[0], c.n[1], c.nblocks, noise ? 1 : 0)] = pick;
+  ++ctx->tuning_epoch;
   out->kernel = pick.kernel;
   out->strips = pick.strips;
   out->prefetch = pick.prefetch;

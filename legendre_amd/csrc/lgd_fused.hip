@@ -45,7 +45,7 @@ template <bool kNoise>
 __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
     lgd_cart c, lgd_kr_consts k, int64_t tiles_x, int64_t tiles_per_block, const double *phi,
     const double *u, double *phi_n, double *u_n, double dt, double noise_scale, uint64_t seed,
-    uint64_t salt, uint64_t stream) {
+    uint64_t salt, const double *salt_src, uint64_t stream) {
   int64_t b = blockIdx.x / tiles_per_block;
   int64_t tile = blockIdx.x - b * tiles_per_block;
   int64_t ty = tile / tiles_x, tx = tile - ty * tiles_x;
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
   // (src/core/state.rs:656-699, src/core/driver.rs:131-158).
   double pn = p[1][1] + dt * dphi;
   if (kNoise) {
-    uint64_t block_key = lgd_mix_key3(seed, salt, stream, (uint64_t)b);
+    uint64_t block_key = lgd_mix_key3(seed, lgd_salt(salt, salt_src), stream, (uint64_t)b);
     uint64_t key = lgd_splitmix64(block_key ^ (uint64_t)(i + j * c.n[0]));
     pn = pn + noise_scale * lgd_standard_normal(key);
   }
@@ -169,11 +169,11 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
   if (noise_on)
     modelc_step_kernel<true><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
         c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, noise_scale,
-        seed, salt, wiener0);
+        seed, salt, ctx->salt_src, wiener0);
   else
     modelc_step_kernel<false><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
         c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, noise_scale,
-        seed, salt, wiener0);
+        seed, salt, nullptr, wiener0);
   return launch_check(ctx, "modelc_step");
 }
 

@@ -24,6 +24,10 @@ struct lgd_ctx {
   hipStream_t stream = nullptr;
   bool owns_stream = false;
   int32_t num_cus = 0;  // compute units of the device (0 = unknown)
+  // Set while a simulation captures a graph: every noise launch reads its salt from
+  // this device time instead of the value argument, so replays draw fresh noise.
+  const double *salt_src = nullptr;
+  int64_t tuning_epoch = 0;  // bumped by every tuning change (captured graphs go stale)
   std::string last_error;
   // Launch-shape knobs (lgd_ctx_set_tuning); 0 = pick automatically.
   int64_t modelc_kernel = 0;  // 1 = per-cell kernel, 2 = streaming kernel
@@ -108,6 +112,13 @@ __host__ __device__ inline uint64_t lgd_mix_key3(uint64_t seed, uint64_t a, uint
   s = lgd_splitmix64(s ^ a);
   s = lgd_splitmix64(s ^ b);
   return lgd_splitmix64(s ^ c);
+}
+
+// The noise salt, t.to_bits() (src/core/simulation.rs:271-281): the value passed,
+// or the bits of the time at `src` when a graph-captured step keeps its time in
+// device memory.
+__device__ inline uint64_t lgd_salt(uint64_t salt, const double *src) {
+  return src ? (uint64_t)__double_as_longlong(*src) : salt;
 }
 
 __host__ __device__ inline double lgd_unit_open(uint64_t bits) {

@@ -233,7 +233,8 @@ __global__ void fill_zero_kernel(int64_t n, double *x) {
 // Driver::Wiener arm of apply_slab (src/core/driver.rs:139-158) with the ghost
 // gate of Grid::cell_key (src/geometry/cartesian.rs:599-619).
 __global__ void wiener_kernel(lgd_cart c, Ext x, double scale, uint64_t seed, uint64_t salt,
-                              uint64_t stream, double *state, const double *amp) {
+                              const double *salt_src, uint64_t stream, double *state,
+                              const double *amp) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t slab = x.e[0] * x.e[1] * x.e[2];
   if (t >= c.nblocks * slab) return;
@@ -249,7 +250,7 @@ __global__ void wiener_kernel(lgd_cart c, Ext x, double scale, uint64_t seed, ui
     key += (cc - x.g) * stride;
     stride *= c.n[d];
   }
-  uint64_t block_key = lgd_mix_key3(seed, salt, stream, (uint64_t)b);
+  uint64_t block_key = lgd_mix_key3(seed, lgd_salt(salt, salt_src), stream, (uint64_t)b);
   uint64_t k = lgd_splitmix64(block_key ^ (uint64_t)key);
   state[t] = state[t] + ((scale * v) * lgd_standard_normal(k));
 }
@@ -401,7 +402,7 @@ int lgd_apply_wiener(lgd_ctx *ctx, const lgd_grid *grid, lgd_buf *x, const lgd_b
   Ext e = make_ext(grid->cart, ghost);
   double scale = std::sqrt(dt);
   wiener_kernel<<<blocks_for((int64_t)x->len, kWG), kWG, 0, ctx->stream>>>(
-      grid->cart, e, scale, seed, salt, stream, x->data, amp->data);
+      grid->cart, e, scale, seed, salt, ctx->salt_src, stream, x->data, amp->data);
   return launch_check(ctx, "apply_wiener");
 }
 

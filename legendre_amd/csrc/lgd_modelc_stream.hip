@@ -47,6 +47,7 @@ struct StreamArgs {
   double *u_n;
   double dt;
   uint64_t salt;
+  const double *salt_src;  // time to take the salt from (graph capture), or null
   uint64_t stream;
   int64_t nblocks;      // blocks in this launch
   int64_t block_begin;  // first block of this launch
@@ -195,7 +196,7 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   // (src/core/driver.rs:144-146).
   const bool noisy = kNoise && D.noise_scale != 0.0;
   uint64_t block_key = 0;
-  if (noisy) block_key = lgd_mix_key3(D.seed, a.salt, a.stream, (uint64_t)rb);
+  if (noisy) block_key = lgd_mix_key3(D.seed, lgd_salt(a.salt, a.salt_src), a.stream, (uint64_t)rb);
 
   // Per-lane column offsets (from the start of a slab row in block column 0),
   // valid for every row: lane 0 reads the left halo cell in its first column,
@@ -372,6 +373,7 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   a.u_n = u_n;
   a.dt = dt;
   a.salt = salt;
+  a.salt_src = ctx->salt_src;
   a.stream = stream;
   a.nblocks = nblocks;
   a.block_begin = block_begin;

@@ -6,6 +6,16 @@
 // control flow and the f64 time accumulation (t += dt, the noise salt), the
 // device owns every per-block sweep.  Initial conditions are host work, as in
 // the reference (ModelC::initialize_grains, src/physics/phasefield.rs:227-314).
+//
+// Whole-step graphs (lgd_sim_set_graph_steps; no reference counterpart): a run of
+// steps — every launch of each step, the subcycled level recursion and the time
+// advance — is captured once into a HIP graph and replayed.  The noise salt, t's
+// bits, is the one value that changes from step to step; during capture the noise
+// kernels read it from a device copy of the times (d_t[0] = t, d_t[l] = the current
+// substep time of AMR level l), which small kernels in the graph advance with the
+// host's own arithmetic (t + dt, fma(k, dt_l, t)), so replays are bitwise equal to
+// eager steps.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -36,11 +46,52 @@ struct lgd_sim {
   std::vector<lgd_buf *> snaps;  // subcycling: per (level, field) time-interpolation snapshot
   double t = 0.0;
   int64_t step_index = 0;
+  // Graphs.
+  struct graph {
+    lgd_buf *state0;   // state[0] at capture (the fused step's ping-pong parity)
+    int64_t epoch;     // AMR hierarchy epoch
+    int64_t tuning;    // context tuning epoch
+    uint64_t dt_bits;
+    int64_t nsteps;
+    hipGraphExec_t exec;
+  };
+  int64_t graph_steps = 0;    // 0: eager steps
+  std::vector<graph> graphs;  // at most the two parities of one key
+  int64_t amr_epoch = 0;      // bumped by every regrid
+  double *d_t = nullptr;      // device times: t, then one substep time per level
+  bool capturing = false;     // noise launches read their salt from d_t
+  bool d_t_known = false;     // d_t[0] holds d_t_value
+  double d_t_value = 0.0;
 };
 
 using namespace lgd;
 
 namespace {
+
+// Simulation::step's time accumulation t += dt, and Subcycle's substep times
+// fma(k, dt_child, t) (src/integrators/subcycling.rs:245-250), on the device.
+__global__ void time_advance_kernel(double *t, double dt) { t[0] = t[0] + dt; }
+__global__ void time_set_kernel(double *t, double v) { t[0] = v; }
+__global__ void time_substep_kernel(double *t, int dst, int src, double k, double dtc) {
+  t[dst] = fma(k, dtc, t[src]);
+}
+__global__ void copy_kernel(int64_t n, const double *src, double *dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// The device time the noise of a (sub)step at `level` reads while capturing.
+const double *time_slot(const lgd_sim *s, int level) {
+  return s->capturing ? s->d_t + level : nullptr;
+}
+
+// Graphs are destroyed only once the stream has drained (a launch may be in flight).
+void drop_graphs(lgd_sim *s) {
+  if (s->graphs.empty()) return;
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
 
 void free_stages(lgd_sim *s) {
   for (auto *b : s->next) lgd_buf_free(b);
@@ -93,11 +144,11 @@ int step_modelc(lgd_sim *s, double dt, uint64_t salt) {
   lgd_ctx *ctx = s->ctx;
   int rc;
   if (s->mode == LGD_MODE_FUSED) {
+    ctx->salt_src = time_slot(s, 0);
     rc = lgd_modelc_step(ctx, s->grid, &s->params, s->state[0], s->state[1], s->next[0],
                          s->next[1], dt, s->seed, salt, s->wiener);
-    if (rc) return rc;
-    std::swap(s->state, s->next);
-    return LGD_OK;
+    ctx->salt_src = nullptr;
+    return rc;  // step_done swaps the ping-pong pair
   }
   // eval_drift: ModelC::fill_ghosts then the per-block Time vector field.
   if ((rc = lgd_fill_ghosts(ctx, s->grid, s->state[0], 1, nullptr))) return rc;
@@ -110,8 +161,10 @@ int step_modelc(lgd_sim *s, double dt, uint64_t salt) {
   if ((rc = lgd_axpy(ctx, s->state[1], s->k[1], dt))) return rc;
   if (s->wiener) {
     const uint64_t stream = (1ull << 32) | 0ull;
-    if ((rc = lgd_apply_wiener(ctx, s->grid, s->state[0], s->amp, 1, dt, s->seed, salt, stream)))
-      return rc;
+    ctx->salt_src = time_slot(s, 0);
+    rc = lgd_apply_wiener(ctx, s->grid, s->state[0], s->amp, 1, dt, s->seed, salt, stream);
+    ctx->salt_src = nullptr;
+    if (rc) return rc;
   }
   return LGD_OK;
 }
@@ -119,9 +172,7 @@ int step_modelc(lgd_sim *s, double dt, uint64_t salt) {
 int step_heat(lgd_sim *s, double dt) {
   int rc;
   if (s->mode == LGD_MODE_FUSED) {
-    if ((rc = lgd_heat_step(s->ctx, s->grid, s->kappa, s->state[0], s->next[0], dt))) return rc;
-    std::swap(s->state, s->next);
-    return LGD_OK;
+    return lgd_heat_step(s->ctx, s->grid, s->kappa, s->state[0], s->next[0], dt);
   }
   if ((rc = lgd_fill_ghosts(s->ctx, s->grid, s->state[0], 1, s->has_bcs ? s->bcs : nullptr)))
     return rc;
@@ -201,14 +252,17 @@ int amr_level_update(lgd_sim *s, int level, double dt, uint64_t salt) {
   return rc;
 }
 
-int amr_level_apply(lgd_sim *s, int level, double dt, uint64_t salt) {
+// `tslot`: the device time slot holding this (sub)step's time while capturing.
+int amr_level_apply(lgd_sim *s, int level, double dt, uint64_t salt, int tslot) {
   int rc;
   for (int f = 0; f < nfields(s); ++f)
     if ((rc = lgd_amr_apply(s->ctx, *s->amr, level, s->state[f]->data, s->k[f]->data, dt))) return rc;
   if (s->kind == lgd_sim::kModelC && s->wiener && s->params.noise_amplitude != 0.0) {
     double scale = std::sqrt(dt) * s->params.noise_amplitude;
-    if ((rc = lgd_amr_wiener(s->ctx, *s->amr, level, s->state[0]->data, scale, s->seed, salt)))
-      return rc;
+    s->ctx->salt_src = time_slot(s, tslot);
+    rc = lgd_amr_wiener(s->ctx, *s->amr, level, s->state[0]->data, scale, s->seed, salt);
+    s->ctx->salt_src = nullptr;
+    if (rc) return rc;
   }
   return LGD_OK;
 }
@@ -228,7 +282,7 @@ int amr_global_step(lgd_sim *s, double dt, uint64_t salt) {
   for (int l = 0; l < g.num_levels(); ++l)
     if ((rc = amr_level_update(s, l, dt, salt))) return rc;
   for (int l = 0; l < g.num_levels(); ++l)
-    if ((rc = amr_level_apply(s, l, dt, salt))) return rc;
+    if ((rc = amr_level_apply(s, l, dt, salt, 0))) return rc;
   return LGD_OK;
 }
 
@@ -243,11 +297,10 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
   // Only this level's cells of a snapshot are ever read (as prolongation
   // donors of the finer level), and a level's patches are contiguous.
   int64_t lo = g.off[g.level_start[level]], hi = g.off[g.level_start[level + 1]];
-  for (int f = 0; f < nf; ++f) {
-    hipError_t e = hipMemcpyAsync(s->snaps[(size_t)(level * nf + f)]->data + lo,
-                                  s->state[f]->data + lo, (size_t)(hi - lo) * sizeof(double),
-                                  hipMemcpyDeviceToDevice, s->ctx->stream);
-    if (e != hipSuccess) return hip_fail(s->ctx, e, "snapshot");
+  for (int f = 0; f < nf && hi > lo; ++f) {
+    copy_kernel<<<blocks_for(hi - lo, 256), 256, 0, s->ctx->stream>>>(
+        hi - lo, s->state[f]->data + lo, s->snaps[(size_t)(level * nf + f)]->data + lo);
+    if ((rc = launch_check(s->ctx, "snapshot"))) return rc;
   }
   for (int f = 0; f < nf; ++f) {
     const double *old = level > 0 ? s->snaps[(size_t)((level - 1) * nf + f)]->data : nullptr;
@@ -256,16 +309,21 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
   uint64_t salt;
   std::memcpy(&salt, &t, sizeof salt);
   if ((rc = amr_level_update(s, level, dt, salt))) return rc;
-  if ((rc = amr_level_apply(s, level, dt, salt))) return rc;
+  if ((rc = amr_level_apply(s, level, dt, salt, level))) return rc;
   const int child = level + 1;
   if (child < g.num_levels()) {
     double n = std::fmax(std::round(amr_stable_dt(s, level) / amr_stable_dt(s, child)), 1.0);
     uint64_t nn = (uint64_t)n;
     double dt_child = dt / (double)nn;
-    for (uint64_t k = 0; k < nn; ++k)
+    for (uint64_t k = 0; k < nn; ++k) {
+      if (s->capturing) {
+        time_substep_kernel<<<1, 1, 0, s->ctx->stream>>>(s->d_t, child, level, (double)k, dt_child);
+        if ((rc = launch_check(s->ctx, "substep time"))) return rc;
+      }
       if ((rc = amr_advance(s, child, std::fma((double)k, dt_child, t), dt_child,
                             (double)k / (double)nn)))
         return rc;
+    }
     for (int f = 0; f < nf; ++f)
       if ((rc = lgd_amr_restrict(s->ctx, g, s->state[f]->data, child))) return rc;
   }
@@ -313,6 +371,8 @@ int amr_regrid(lgd_sim *s) {
       return rc;
     ns.push_back(b);
   }
+  drop_graphs(s);  // they hold the old hierarchy's plans and stage buffers
+  ++s->amr_epoch;
   for (auto *b : s->state) lgd_buf_free(b);
   s->state = ns;
   (void)hipStreamSynchronize(s->ctx->stream);
@@ -322,11 +382,108 @@ int amr_regrid(lgd_sim *s) {
   return alloc_amr_stages(s);
 }
 
-int step_amr(lgd_sim *s, double dt, uint64_t salt) {
+// One step after any regrid due at its top.
+int step_body(lgd_sim *s, double dt, uint64_t salt) {
+  if (s->amr) {
+    if (s->subcycle) return amr_advance(s, 0, s->t, dt, 0.0);
+    return amr_global_step(s, dt, salt);
+  }
+  return s->kind == lgd_sim::kModelC ? step_modelc(s, dt, salt) : step_heat(s, dt);
+}
+
+bool regrid_due(const lgd_sim *s) {
+  return s->amr && s->adaptive && s->step_index % s->policy.every == 0;
+}
+
+// Host-side effects of one step: the fused ping-pong swap and the time accumulation
+// (Simulation::step, src/core/simulation.rs:281-289).
+void step_done(lgd_sim *s, double dt) {
+  if (!s->amr && s->mode == LGD_MODE_FUSED) std::swap(s->state, s->next);
+  s->t = s->t + dt;
+  s->step_index += 1;
+}
+
+// Capture `n` steps from the current buffers into *out.  The host bookkeeping the
+// capture pass performs (buffer swaps) is undone: the graph has not run yet.
+int capture(lgd_sim *s, double dt, int64_t n, hipGraphExec_t *out) {
+  lgd_ctx *ctx = s->ctx;
+  const auto state = s->state, next = s->next;
+  const double t = s->t;
+  const int64_t step_index = s->step_index;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_fail(ctx, e, "graph capture (needs a non-default stream)");
+  s->capturing = true;
+  int rc = LGD_OK;
+  for (int64_t k = 0; k < n && !rc; ++k) {
+    uint64_t salt;
+    std::memcpy(&salt, &s->t, sizeof salt);
+    rc = step_body(s, dt, salt);
+    if (!rc) {
+      time_advance_kernel<<<1, 1, 0, ctx->stream>>>(s->d_t, dt);
+      rc = launch_check(ctx, "time advance");
+    }
+    if (!rc) step_done(s, dt);
+  }
+  s->capturing = false;
+  ctx->salt_src = nullptr;
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(ctx->stream, &graph);
+  s->state = state;
+  s->next = next;
+  s->t = t;
+  s->step_index = step_index;
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(ctx, e, "graph capture");
+  e = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return hip_fail(ctx, e, "graph instantiate");
+  return LGD_OK;
+}
+
+// Run `n` steps (no regrid due among steps 2..n) as one graph launch.
+int graph_steps(lgd_sim *s, double dt, int64_t n) {
+  lgd_ctx *ctx = s->ctx;
+  uint64_t dt_bits;
+  std::memcpy(&dt_bits, &dt, sizeof dt_bits);
+  lgd_sim::graph *hit = nullptr;
+  for (auto &g : s->graphs)
+    if (g.state0 == s->state[0] && g.epoch == s->amr_epoch && g.tuning == ctx->tuning_epoch &&
+        g.dt_bits == dt_bits && g.nsteps == n)
+      hit = &g;
   int rc;
-  if (s->adaptive && s->step_index % s->policy.every == 0 && (rc = amr_regrid(s))) return rc;
-  if (s->subcycle) return amr_advance(s, 0, s->t, dt, 0.0);
-  return amr_global_step(s, dt, salt);
+  if (!hit) {
+    // Keep only graphs of this key (the other ping-pong parity).
+    std::vector<lgd_sim::graph> keep;
+    if (!s->graphs.empty()) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &g : s->graphs) {
+      if (g.epoch == s->amr_epoch && g.tuning == ctx->tuning_epoch && g.dt_bits == dt_bits &&
+          g.nsteps == n)
+        keep.push_back(g);
+      else
+        (void)hipGraphExecDestroy(g.exec);
+    }
+    s->graphs = keep;
+    lgd_sim::graph g{s->state[0], s->amr_epoch, ctx->tuning_epoch, dt_bits, n, nullptr};
+    if ((rc = capture(s, dt, n, &g.exec))) return rc;
+    s->graphs.push_back(g);
+    hit = &s->graphs.back();
+  }
+  uint64_t tb, kb;
+  std::memcpy(&tb, &s->t, sizeof tb);
+  std::memcpy(&kb, &s->d_t_value, sizeof kb);
+  if (!s->d_t_known || tb != kb) {
+    time_set_kernel<<<1, 1, 0, ctx->stream>>>(s->d_t, s->t);
+    if ((rc = launch_check(ctx, "time set"))) return rc;
+  }
+  hipError_t e = hipGraphLaunch(hit->exec, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "graph launch");
+  for (int64_t k = 0; k < n; ++k) step_done(s, dt);
+  s->d_t_known = true;
+  s->d_t_value = s->t;
+  return LGD_OK;
 }
 
 }  // namespace
@@ -442,6 +599,9 @@ const lgd_amr *lgd_sim_amr(const lgd_sim *sim) { return sim ? sim->amr : nullptr
 
 int lgd_sim_destroy(lgd_sim *sim) {
   if (!sim) return LGD_OK;
+  if (sim->ctx && sim->ctx->stream) (void)hipStreamSynchronize(sim->ctx->stream);
+  drop_graphs(sim);
+  if (sim->d_t) (void)hipFree(sim->d_t);
   free_all(sim);
   if (sim->amr) lgd_amr_destroy(sim->amr);
   delete sim;
@@ -458,15 +618,41 @@ int lgd_sim_field(lgd_sim *sim, int32_t index, lgd_buf **out) {
 
 int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps) {
   if (!sim) return LGD_ERR_INVALID_ARGUMENT;
-  for (int64_t n = 0; n < nsteps; ++n) {
+  int rc;
+  for (int64_t n = 0; n < nsteps;) {
+    // BergerOliger::regrid at the top of a step (src/core/simulation.rs:255-270).
+    if (regrid_due(sim) && (rc = amr_regrid(sim))) return rc;
+    if (sim->graph_steps > 0) {
+      int64_t k = std::min(sim->graph_steps, nsteps - n);
+      if (sim->amr && sim->adaptive)  // no regrid inside a graph
+        k = std::min(k, sim->policy.every - sim->step_index % sim->policy.every);
+      if ((rc = graph_steps(sim, dt, k))) return rc;
+      n += k;
+      continue;
+    }
     uint64_t salt;
     std::memcpy(&salt, &sim->t, sizeof salt);  // t.to_bits()
-    int rc = sim->amr ? step_amr(sim, dt, salt)
-             : sim->kind == lgd_sim::kModelC ? step_modelc(sim, dt, salt)
-                                             : step_heat(sim, dt);
-    if (rc) return rc;
-    sim->t = sim->t + dt;
-    sim->step_index += 1;
+    if ((rc = step_body(sim, dt, salt))) return rc;
+    step_done(sim, dt);
+    ++n;
+  }
+  return LGD_OK;
+}
+
+int lgd_sim_set_graph_steps(lgd_sim *sim, int64_t steps) {
+  if (!sim) return LGD_ERR_INVALID_ARGUMENT;
+  if (steps < 0) return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT, "graph steps are non-negative");
+  drop_graphs(sim);
+  sim->graph_steps = steps;
+  if (steps > 0 && !sim->d_t) {
+    const size_t n = (size_t)(sim->amr ? sim->amr->max_levels() + 1 : 1);
+    hipError_t e = hipMalloc((void **)&sim->d_t, n * sizeof(double));
+    if (e != hipSuccess) {
+      sim->d_t = nullptr;
+      sim->graph_steps = 0;
+      return hip_fail(sim->ctx, e, "graph time slots");
+    }
+    sim->d_t_known = false;
   }
   return LGD_OK;
 }
