@@ -485,6 +485,9 @@ def run_amr(args) -> int:
     ctx_per_block = [False]
     sim = build()
     times, updates, shapes, first = run(sim, nchunks, "batched")
+    nregrid, phases = sim.regrid_stats()
+    log(f"[amr] {nregrid} regrids, seconds by phase: tag field to host {phases[0]:.3f}, "
+        f"proposal {phases[1]:.3f}, new hierarchy {phases[2]:.3f}, migration {phases[3]:.3f}")
     sim.close()
     pb_chunks = min(nchunks, max(1, args.per_block_chunks))
     ctx.set_tuning("amr_per_block", 1)
@@ -531,6 +534,11 @@ def run_amr(args) -> int:
             "per_patch_ms_first_chunks": round(sum(pb_times) * 1e3, 3),
             "speedup_vs_per_patch_launches": round(speedup, 2),
             "regrid_and_first_step_seconds": round(sum(first), 3),
+            "regrids": nregrid,
+            "regrid_phase_seconds": {"tag_field_to_host": round(float(phases[0]), 4),
+                                     "proposal": round(float(phases[1]), 4),
+                                     "new_hierarchy": round(float(phases[2]), 4),
+                                     "migration": round(float(phases[3]), 4)},
             "speedup_excluding_regrid_steps": round(
                 (sum(pb_times) - sum(pb_first)) / (sum(times[:pb_chunks]) - sum(first[:pb_chunks])), 2),
             "same_hierarchy": same,

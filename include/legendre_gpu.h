@@ -324,6 +324,11 @@ int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps);
  * stream must not be the null stream.  0 turns graphs off (and frees them). */
 int lgd_sim_set_graph_steps(lgd_sim *sim, int64_t steps);
 double lgd_sim_time(const lgd_sim *sim);
+/* Regrid accounting (no reference counterpart; for benchmarks): regrids run so far
+ * and their wall seconds by phase — [0] the tagged field prepared and brought to the
+ * host, [1] the proposal (tagging, buffer, nesting, clustering, hysteresis), [2] the
+ * new hierarchy (patches, plans, uploads), [3] the state migrated. */
+int lgd_sim_regrid_stats(const lgd_sim *sim, int64_t *regrids, double *seconds);
 int64_t lgd_sim_step_index(const lgd_sim *sim);
 /* ModelC::initialize_grains with orientation 0 (phasefield.rs:227-286), computed on the
  * host (glibc tanh/hypot, as the reference) and uploaded: grains = [cx, cy, r] × n. */

@@ -418,6 +418,15 @@ class Simulation:
     def step(self, dt: float, nsteps: int = 1) -> None:
         _abi.check(self._lib.lgd_sim_step(self.handle, dt, nsteps), self.ctx.handle)
 
+    def regrid_stats(self) -> tuple:
+        """(regrids, seconds by phase: tag field to host, proposal, new hierarchy,
+        migration)."""
+        n = C.c_int64()
+        sec = np.zeros(4)
+        _abi.check(self._lib.lgd_sim_regrid_stats(self.handle, C.byref(n), sec.ctypes.data_as(_abi.DP)),
+                   self.ctx.handle)
+        return int(n.value), sec
+
     def set_graph_steps(self, steps: int) -> None:
         """Run `steps` whole steps at a time as one replayed HIP graph (0: eager
         launches).  Results are bitwise those of eager steps."""

@@ -154,6 +154,7 @@ SIGNATURES = {
     "lgd_sim_field": (C.c_int, [P, I32, PP]),
     "lgd_sim_step": (C.c_int, [P, DBL, I64]),
     "lgd_sim_set_graph_steps": (C.c_int, [P, I64]),
+    "lgd_sim_regrid_stats": (C.c_int, [P, C.POINTER(C.c_int64), DP]),
     "lgd_sim_time": (DBL, [P]),
     "lgd_sim_step_index": (I64, [P]),
     "lgd_sim_initialize_grains": (C.c_int, [P, DP, SZ, DBL]),

@@ -68,6 +68,14 @@ struct lgd_restrict_item {
   int64_t stride;
 };
 
+// Device copy of one level's find_patch index: domain extent, bucket grid and the
+// level's first entry in the concatenated bucket starts.
+struct lgd_dev_level {
+  int64_t w, h;
+  int64_t nbx, nby;
+  int64_t start0;
+};
+
 struct lgd_level_plan {
   lgd_tile *tiles = nullptr;
   int64_t ntiles = 0;
@@ -100,6 +108,12 @@ struct lgd_amr {
   std::vector<int64_t> off;           // ghost-1 slab offsets, nblocks + 1
   lgd_block_desc *d_blocks = nullptr;
   std::vector<lgd_level_plan> plans;  // per populated level
+  // find_patch on the device: per-level headers, concatenated bucket starts (into
+  // the concatenated ids), patch ids, and each patch's lower corner (x, y).
+  lgd_dev_level *d_levels = nullptr;
+  uint32_t *d_bstart = nullptr;
+  uint32_t *d_bids = nullptr;
+  int64_t *d_lo = nullptr;
 
   int num_levels() const { return (int)level_start.size() - 1; }
   int max_levels() const { return (int)ratios.size() + 1; }

@@ -102,6 +102,14 @@ int64_t lgd_amr::index(int64_t b, int64_t i, int64_t j) const {
 void lgd_amr::free_device() {
   (void)hipFree(d_blocks);
   d_blocks = nullptr;
+  (void)hipFree(d_levels);
+  (void)hipFree(d_bstart);
+  (void)hipFree(d_bids);
+  (void)hipFree(d_lo);
+  d_levels = nullptr;
+  d_bstart = nullptr;
+  d_bids = nullptr;
+  d_lo = nullptr;
   for (auto &p : plans) {
     (void)hipFree(p.tiles);
     (void)hipFree(p.ops);
@@ -260,6 +268,28 @@ int lgd_amr_build_plans(lgd_amr &g) {
   }
   int rc = upload_vec(ctx, blocks, &g.d_blocks);
   if (rc) return rc;
+  {
+    // find_patch's index, concatenated over the levels.
+    std::vector<lgd_dev_level> lv;
+    std::vector<uint32_t> bstart, bids;
+    std::vector<int64_t> lo;
+    for (int l = 0; l < g.num_levels(); ++l) {
+      const lgd_amr::bucket_index &ix = g.lookup[(size_t)l];
+      const lgd_box d = g.domain(l);
+      lv.push_back(lgd_dev_level{d.hi[0], d.hi[1], ix.nbx, ix.nby, (int64_t)bstart.size()});
+      const uint32_t base = (uint32_t)bids.size();
+      for (uint32_t s : ix.start) bstart.push_back(base + s);
+      bids.insert(bids.end(), ix.ids.begin(), ix.ids.end());
+    }
+    for (const lgd_box &b : g.box) {
+      lo.push_back(b.lo[0]);
+      lo.push_back(b.lo[1]);
+    }
+    if ((rc = upload_vec(ctx, lv, &g.d_levels))) return rc;
+    if ((rc = upload_vec(ctx, bstart, &g.d_bstart))) return rc;
+    if ((rc = upload_vec(ctx, bids, &g.d_bids))) return rc;
+    if ((rc = upload_vec(ctx, lo, &g.d_lo))) return rc;
+  }
   g.plans.assign((size_t)g.num_levels(), lgd_level_plan{});
   for (int l = 0; l < g.num_levels(); ++l) {
     lgd_level_plan &P = g.plans[(size_t)l];

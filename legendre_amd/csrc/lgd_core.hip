@@ -141,6 +141,9 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
     ctx->adjoint_kernel = value;
   } else if (k == "adjoint_rows") {
     ctx->adjoint_rows = value;
+  } else if (k == "regrid_host") {
+    if (value > 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "regrid_host is 0 or 1");
+    ctx->regrid_host = value;
   } else {
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "unknown tuning key " + k);
   }

@@ -1,4 +1,4 @@
-// Host-side regrid of the device AMR backend (lgd_regrid.hip).
+// Regrid of the device AMR backend (lgd_regrid.hip).
 #pragma once
 
 #include <vector>
@@ -9,8 +9,32 @@
 // current hierarchy still serves (no regrid), 1 when `out` should replace it.
 int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
                        const lgd_regrid_policy &policy, std::vector<std::vector<lgd_box>> &out);
+// The same from per-level bitmasks of tagged, buffer-dilated cells (row-major over
+// each level's domain, 64 cells per word; consumed).
+int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>> &bits,
+                            const lgd_regrid_policy &policy, std::vector<std::vector<lgd_box>> &out);
 // Migrate one field from the old hierarchy onto the new one: new_f receives the
 // levels >= 1 of the new field (elements from new_g.off[level_start[1]] on); level 0
 // is the same in both hierarchies and is copied on the device.
 void lgd_regrid_migrate(const lgd_amr &old_g, const std::vector<double> &old_f,
                         const lgd_amr &new_g, std::vector<double> &new_f);
+
+// Device scratch of the regrid's tagging pass (kept across regrids).
+struct lgd_regrid_scratch {
+  uint8_t *T = nullptr;
+  size_t tcap = 0;
+  uint64_t *bits = nullptr;
+  size_t wcap = 0;
+};
+void lgd_regrid_scratch_free(lgd_regrid_scratch &sc);
+// Whether the device path covers this hierarchy's depth.
+bool lgd_regrid_device_supported(const lgd_amr &g);
+// Tag and dilate every level that can be refined on the device (the tag field is
+// the ghost-filled device field); bits[level] receives its bitmask.
+int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field,
+                         const lgd_regrid_policy &policy, lgd_regrid_scratch &sc,
+                         std::vector<std::vector<uint64_t>> &bits);
+// Device migration: the interiors of new_g's levels >= 1 in new_f (a buffer of the
+// new layout) sampled from old_f on old_g; level 0 and ghosts are not written.
+int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *old_f,
+                              const lgd_amr &new_g, double *new_f);

@@ -181,51 +181,73 @@ double sample_old(const lgd_amr &g, const double *f, int level, int64_t x, int64
   return acc;
 }
 
+// The tagged and buffer-dilated cells of level `src` (GradientTagger::tag_level,
+// adapt.rs:78-103, then the dilation of adapt.rs:181-193) as a bitmask over the
+// level domain: row-major, 64 cells per word, bit x % 64 of word (y, x / 64).
+void host_flag_bits(const lgd_amr &g, const std::vector<double> &tag_field,
+                    const lgd_regrid_policy &pol, int src, std::vector<uint64_t> &bits) {
+  const lgd_box dom = g.domain(src);
+  const int64_t W = dom.hi[0], H = dom.hi[1], wpr = (W + 63) / 64;
+  bits.assign((size_t)(H * wpr), 0ull);
+  const double *v = tag_field.data();
+  for (uint32_t b = g.level_start[src]; b < g.level_start[src + 1]; ++b) {
+    const lgd_box &p = g.box[b];
+    for (int64_t j = 0; j < p.extent(1); ++j)
+      for (int64_t i = 0; i < p.extent(0); ++i) {
+        double gx = std::fabs((v[g.index(b, i + 1, j)] - v[g.index(b, i - 1, j)]) * 0.5);
+        double gy = std::fabs((v[g.index(b, i, j + 1)] - v[g.index(b, i, j - 1)]) * 0.5);
+        double gm = std::fmax(std::fmax(0.0, gx), gy);
+        if (!(gm > pol.threshold)) continue;
+        const int64_t cx = p.lo[0] + i, cy = p.lo[1] + j, r = pol.buffer > 0 ? pol.buffer : 0;
+        for (int64_t y = std::max<int64_t>(cy - r, 0); y <= std::min<int64_t>(cy + r, H - 1); ++y)
+          for (int64_t x = std::max<int64_t>(cx - r, 0); x <= std::min<int64_t>(cx + r, W - 1); ++x)
+            bits[(size_t)(y * wpr + x / 64)] |= 1ull << (x % 64);
+      }
+  }
+}
+
 }  // namespace
 
-int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
-                       const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
+int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>> &bits,
+                            const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
   const int capacity = g.max_levels() - 1;
   out.assign((size_t)capacity, {});
   for (int target = capacity; target >= 1; --target) {
     const int src = target - 1;
-    std::vector<Cell> flags;
-    if (src < g.num_levels()) {
-      for (uint32_t b = g.level_start[src]; b < g.level_start[src + 1]; ++b) {
-        const lgd_box &p = g.box[b];
-        for (int64_t j = 0; j < p.extent(1); ++j)
-          for (int64_t i = 0; i < p.extent(0); ++i) {
-            const double *v = tag_field.data();
-            double gx = std::fabs((v[g.index(b, i + 1, j)] - v[g.index(b, i - 1, j)]) * 0.5);
-            double gy = std::fabs((v[g.index(b, i, j + 1)] - v[g.index(b, i, j - 1)]) * 0.5);
-            double gm = std::fmax(std::fmax(0.0, gx), gy);
-            if (gm > pol.threshold) flags.push_back({p.lo[0] + i, p.lo[1] + j});
-          }
-      }
-    }
-    if (pol.buffer > 0) {
-      std::vector<Cell> seeds;
-      seeds.swap(flags);
-      for (const Cell &c : seeds)
-        for (int64_t y = c[1] - pol.buffer; y <= c[1] + pol.buffer; ++y)
-          for (int64_t x = c[0] - pol.buffer; x <= c[0] + pol.buffer; ++x) flags.push_back({x, y});
-    }
+    const lgd_box dom = g.domain(src);
+    const int64_t W = dom.hi[0], H = dom.hi[1], wpr = (W + 63) / 64;
+    std::vector<uint64_t> m;
+    if ((size_t)src < bits.size() && !bits[(size_t)src].empty())
+      m.swap(bits[(size_t)src]);
+    else
+      m.assign((size_t)(H * wpr), 0ull);
+    // Nesting flags: the coarse cells under the level being built above, grown by
+    // one cell (adapt.rs:194-211).
     if (target < capacity && !out[(size_t)target].empty()) {
       const int64_t rf = g.ratios[(size_t)target], rs = g.ratios[(size_t)target - 1];
       for (const lgd_box &bx : out[(size_t)target]) {
         int64_t nlo[2] = {bx.lo[0] / rf - 1, bx.lo[1] / rf - 1};
         int64_t nhi[2] = {bx.hi[0] / rf + 1, bx.hi[1] / rf + 1};
-        for (int64_t y = floor_div(nlo[1], rs); y < floor_div(nhi[1] + rs - 1, rs); ++y)
-          for (int64_t x = floor_div(nlo[0], rs); x < floor_div(nhi[0] + rs - 1, rs); ++x)
-            flags.push_back({x, y});
+        const int64_t y0 = std::max<int64_t>(floor_div(nlo[1], rs), 0);
+        const int64_t y1 = std::min<int64_t>(floor_div(nhi[1] + rs - 1, rs), H);
+        const int64_t x0 = std::max<int64_t>(floor_div(nlo[0], rs), 0);
+        const int64_t x1 = std::min<int64_t>(floor_div(nhi[0] + rs - 1, rs), W);
+        for (int64_t y = y0; y < y1; ++y)
+          for (int64_t x = x0; x < x1; ++x) m[(size_t)(y * wpr + x / 64)] |= 1ull << (x % 64);
       }
     }
-    const lgd_box dom = g.domain(src);
-    flags.erase(std::remove_if(flags.begin(), flags.end(),
-                               [&](const Cell &c) { return !dom.contains(c[0], c[1]); }),
-                flags.end());
-    std::sort(flags.begin(), flags.end());
-    flags.erase(std::unique(flags.begin(), flags.end()), flags.end());
+    // The flagged cells, each once (the set the reference clusters after its
+    // clip / sort / dedupe; clustering does not depend on their order).
+    std::vector<Cell> flags;
+    for (int64_t y = 0; y < H; ++y)
+      for (int64_t w = 0; w < wpr; ++w) {
+        uint64_t word = m[(size_t)(y * wpr + w)];
+        while (word) {
+          const int bit = __builtin_ctzll(word);
+          word &= word - 1;
+          flags.push_back({w * 64 + bit, y});
+        }
+      }
     if (flags.empty()) continue;
     std::vector<lgd_box> boxes;
     cluster_rec(flags.data(), flags.size(), pol.efficiency, (size_t)pol.min_width, boxes);
@@ -252,6 +274,14 @@ int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
   return 0;
 }
 
+int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
+                       const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
+  std::vector<std::vector<uint64_t>> bits((size_t)g.num_levels());
+  for (int src = 0; src < g.num_levels() && src < g.max_levels() - 1; ++src)
+    host_flag_bits(g, tag_field, pol, src, bits[(size_t)src]);
+  return lgd_regrid_propose_bits(g, bits, pol, out);
+}
+
 void lgd_regrid_migrate(const lgd_amr &old_g, const std::vector<double> &old_f, const lgd_amr &new_g,
                         std::vector<double> &new_f) {
   // Level 0 is identical in both hierarchies (the base tiling never changes) and
@@ -267,4 +297,243 @@ void lgd_regrid_migrate(const lgd_amr &old_g, const std::vector<double> &old_f, 
           new_f[(size_t)(new_g.index(b, i, j) - fine0)] =
               sample_old(old_g, old_f.data(), l, p.lo[0] + i, p.lo[1] + j);
     }
+}
+
+// ------------------------------------------------------------------ device path
+// The same regrid with the field-sized work on the device: the tagger and the
+// buffer dilation write a bitmask per level (so the host reads bits, not the
+// field), and the migration samples the old hierarchy where the state lives.
+namespace {
+
+constexpr int kRTX = 64, kRTY = 4;  // the level plans' tile shape
+constexpr int kMaxSampleDepth = 7;  // levels below the sampled one (8-level hierarchies)
+
+struct DevIndex {
+  const lgd_dev_level *levels;
+  const uint32_t *bstart;
+  const uint32_t *bids;
+  const int64_t *lo;
+  const lgd_block_desc *blocks;
+  int nlevels;
+};
+
+struct DevRatios {
+  int64_t cells[2];  // level-0 domain
+  uint32_t r[kMaxSampleDepth + 1];
+};
+
+__device__ int64_t dev_find_patch(const DevIndex &ix, int l, int64_t x, int64_t y) {
+  if (l >= ix.nlevels) return -1;
+  const lgd_dev_level L = ix.levels[l];
+  if (x < 0 || y < 0 || x >= L.w || y >= L.h) return -1;
+  const int64_t k = L.start0 + x / lgd_amr::kBucket + L.nbx * (y / lgd_amr::kBucket);
+  for (uint32_t i = ix.bstart[k]; i < ix.bstart[k + 1]; ++i) {
+    const uint32_t p = ix.bids[i];
+    const int64_t px = ix.lo[2 * p], py = ix.lo[2 * p + 1];
+    const lgd_block_desc bd = ix.blocks[p];
+    if (x >= px && x < px + bd.nx && y >= py && y < py + bd.ny) return (int64_t)p;
+  }
+  return -1;
+}
+
+__device__ int64_t dev_mirror(int64_t c, int64_t lo, int64_t hi) {
+  if (c < lo) return 2 * lo - 1 - c;
+  if (c >= hi) return 2 * hi - 1 - c;
+  return c;
+}
+
+// sample_old above, on the device: the same lookups, corner order and operation
+// order.  K bounds the recursion (level 0 always has a patch).
+template <int K>
+__device__ __attribute__((noinline)) double dev_sample(const DevIndex &ix, const DevRatios &R,
+                                                       const double *f, int level, int64_t x,
+                                                       int64_t y) {
+  const int64_t b = dev_find_patch(ix, level, x, y);
+  if (b >= 0) {
+    const lgd_block_desc bd = ix.blocks[b];
+    return f[bd.off + (y - ix.lo[2 * b + 1] + 1) * (int64_t)(bd.nx + 2) + (x - ix.lo[2 * b] + 1)];
+  }
+  if constexpr (K == 0) {
+    return 0.0;
+  } else {
+    const double r = (double)R.r[level - 1];
+    int64_t s = 1;
+    for (int i = 0; i < level - 1; ++i) s *= (int64_t)R.r[i];
+    const int64_t chi[2] = {R.cells[0] * s, R.cells[1] * s};
+    int64_t c0[2];
+    double w[2];
+    const int64_t cell[2] = {x, y};
+    for (int d = 0; d < 2; ++d) {
+      double pc = ((double)cell[d] + 0.5) / r - 0.5;
+      double fl = floor(pc);
+      c0[d] = (int64_t)fl;
+      w[d] = pc - fl;
+    }
+    double acc = 0.0;
+    for (unsigned corner = 0; corner < 4; ++corner) {
+      int64_t cc[2] = {c0[0], c0[1]};
+      double weight = 1.0;
+      for (int d = 0; d < 2; ++d) {
+        if ((corner >> d) & 1u) {
+          cc[d] += 1;
+          weight *= w[d];
+        } else {
+          weight *= 1.0 - w[d];
+        }
+      }
+      if (weight == 0.0) continue;
+      for (int d = 0; d < 2; ++d) cc[d] = dev_mirror(cc[d], 0, chi[d]);
+      acc = acc + dev_sample<K - 1>(ix, R, f, level - 1, cc[0], cc[1]) * weight;
+    }
+    return acc;
+  }
+}
+
+// New patch interiors of one level sampled from the old hierarchy.
+__global__ __launch_bounds__(kRTX *kRTY) void migrate_kernel(const lgd_tile *tiles,
+                                                             const lgd_block_desc *nblocks,
+                                                             const int64_t *nlo, DevIndex old,
+                                                             DevRatios R, const double *oldf,
+                                                             double *newf, int level) {
+  const lgd_tile tl = tiles[blockIdx.x];
+  const lgd_block_desc bd = nblocks[tl.block];
+  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
+  if (i >= bd.nx || j >= bd.ny) return;
+  const int64_t x = nlo[2 * tl.block] + i, y = nlo[2 * tl.block + 1] + j;
+  newf[bd.off + (int64_t)(j + 1) * (bd.nx + 2) + (i + 1)] =
+      dev_sample<kMaxSampleDepth>(old, R, oldf, level, x, y);
+}
+
+// GradientTagger on one level's patch interiors: a byte per flagged cell of the
+// level domain (row-major, width W).
+__global__ __launch_bounds__(kRTX *kRTY) void tag_kernel(const lgd_tile *tiles,
+                                                         const lgd_block_desc *blocks,
+                                                         const int64_t *lo, const double *v,
+                                                         double threshold, uint8_t *T, int64_t W) {
+  const lgd_tile tl = tiles[blockIdx.x];
+  const lgd_block_desc bd = blocks[tl.block];
+  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
+  if (i >= bd.nx || j >= bd.ny) return;
+  const int64_t sx = bd.nx + 2;
+  const double *c = v + bd.off + (int64_t)(j + 1) * sx + (i + 1);
+  const double gx = fabs((c[1] - c[-1]) * 0.5);
+  const double gy = fabs((c[sx] - c[-sx]) * 0.5);
+  const double gm = fmax(fmax(0.0, gx), gy);
+  if (gm > threshold) T[(lo[2 * tl.block + 1] + j) * W + lo[2 * tl.block] + i] = 1;
+}
+
+// Buffer dilation and packing: one wavefront per 64-cell word of the bitmask.
+__global__ __launch_bounds__(256) void dilate_pack_kernel(const uint8_t *T, int64_t W, int64_t H,
+                                                          int64_t wpr, int buffer,
+                                                          unsigned long long *bits) {
+  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
+  if (wave >= H * wpr) return;  // whole wavefronts
+  const int64_t y = wave / wpr, x = (wave - y * wpr) * 64 + lane;
+  bool f = false;
+  if (x < W)
+    for (int64_t yy = y - buffer; yy <= y + buffer; ++yy) {
+      if (yy < 0 || yy >= H) continue;
+      for (int64_t xx = x - buffer; xx <= x + buffer; ++xx)
+        if (xx >= 0 && xx < W && T[yy * W + xx]) f = true;
+    }
+  const unsigned long long m = __ballot(f);
+  if (lane == 0) bits[wave] = m;
+}
+
+DevIndex dev_index(const lgd_amr &g) {
+  return DevIndex{g.d_levels, g.d_bstart, g.d_bids, g.d_lo, g.d_blocks, g.num_levels()};
+}
+
+}  // namespace
+
+using namespace lgd;
+
+bool lgd_regrid_device_supported(const lgd_amr &g) { return g.max_levels() <= kMaxSampleDepth + 1; }
+
+int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field,
+                         const lgd_regrid_policy &pol, lgd_regrid_scratch &sc,
+                         std::vector<std::vector<uint64_t>> &bits) {
+  const int levels = std::min(g.num_levels(), g.max_levels() - 1);
+  bits.assign((size_t)g.num_levels(), {});
+  size_t tcap = 0, wtotal = 0;
+  std::vector<size_t> wofs;
+  for (int src = 0; src < levels; ++src) {
+    const lgd_box d = g.domain(src);
+    tcap = std::max(tcap, (size_t)(d.hi[0] * d.hi[1]));
+    wofs.push_back(wtotal);
+    wtotal += (size_t)(d.hi[1] * ((d.hi[0] + 63) / 64));
+  }
+  if (levels == 0) return LGD_OK;
+  hipError_t e = hipSuccess;
+  if (sc.tcap < tcap) {
+    (void)hipFree(sc.T);
+    sc.T = nullptr;
+    sc.tcap = 0;
+    e = hipMalloc((void **)&sc.T, tcap);
+    if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag scratch");
+    sc.tcap = tcap;
+  }
+  if (sc.wcap < wtotal) {
+    (void)hipFree(sc.bits);
+    sc.bits = nullptr;
+    sc.wcap = 0;
+    e = hipMalloc((void **)&sc.bits, wtotal * sizeof(uint64_t));
+    if (e != hipSuccess) return hip_fail(ctx, e, "regrid bitmask scratch");
+    sc.wcap = wtotal;
+  }
+  int rc;
+  for (int src = 0; src < levels; ++src) {
+    const lgd_box d = g.domain(src);
+    const int64_t W = d.hi[0], H = d.hi[1], wpr = (W + 63) / 64;
+    e = hipMemsetAsync(sc.T, 0, (size_t)(W * H), ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag clear");
+    const lgd_level_plan &P = g.plans[(size_t)src];
+    if (P.ntiles > 0) {
+      tag_kernel<<<(unsigned)P.ntiles, dim3(kRTX, kRTY), 0, ctx->stream>>>(
+          P.tiles, g.d_blocks, g.d_lo, tag_field, pol.threshold, sc.T, W);
+      if ((rc = launch_check(ctx, "regrid tag"))) return rc;
+    }
+    const int64_t waves = H * wpr;
+    dilate_pack_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, ctx->stream>>>(
+        sc.T, W, H, wpr, (int)std::max<int64_t>(pol.buffer, 0),
+        (unsigned long long *)(sc.bits + wofs[(size_t)src]));
+    if ((rc = launch_check(ctx, "regrid dilate"))) return rc;
+  }
+  std::vector<uint64_t> host(wtotal);
+  e = hipMemcpyAsync(host.data(), sc.bits, wtotal * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                     ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "regrid bitmask download");
+  for (int src = 0; src < levels; ++src) {
+    const size_t end = (size_t)src + 1 < wofs.size() ? wofs[(size_t)src + 1] : wtotal;
+    bits[(size_t)src].assign(host.begin() + (ptrdiff_t)wofs[(size_t)src], host.begin() + (ptrdiff_t)end);
+  }
+  return LGD_OK;
+}
+
+int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *old_f,
+                              const lgd_amr &new_g, double *new_f) {
+  if (!lgd_regrid_device_supported(new_g))
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "device migration supports at most 8 levels");
+  DevRatios R{};
+  R.cells[0] = old_g.cells[0];
+  R.cells[1] = old_g.cells[1];
+  for (size_t i = 0; i < old_g.ratios.size() && i <= (size_t)kMaxSampleDepth; ++i) R.r[i] = old_g.ratios[i];
+  const DevIndex old = dev_index(old_g);
+  int rc;
+  for (int l = 1; l < new_g.num_levels(); ++l) {
+    const lgd_level_plan &P = new_g.plans[(size_t)l];
+    if (P.ntiles == 0) continue;
+    migrate_kernel<<<(unsigned)P.ntiles, dim3(kRTX, kRTY), 0, ctx->stream>>>(
+        P.tiles, new_g.d_blocks, new_g.d_lo, old, R, old_f, new_f, l);
+    if ((rc = launch_check(ctx, "regrid migrate"))) return rc;
+  }
+  return LGD_OK;
+}
+
+void lgd_regrid_scratch_free(lgd_regrid_scratch &sc) {
+  (void)hipFree(sc.T);
+  (void)hipFree(sc.bits);
+  sc = lgd_regrid_scratch{};
 }

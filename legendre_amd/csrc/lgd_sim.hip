@@ -16,6 +16,7 @@
 // host's own arithmetic (t + dt, fma(k, dt_l, t)), so replays are bitwise equal to
 // eager steps.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -62,6 +63,11 @@ struct lgd_sim {
   bool capturing = false;     // noise launches read their salt from d_t
   bool d_t_known = false;     // d_t[0] holds d_t_value
   double d_t_value = 0.0;
+  // Regrid wall time by phase (seconds): tag field to the host, proposal (tagging
+  // and clustering), new hierarchy (plans and uploads), state migration; and count.
+  double regrid_seconds[4] = {0.0, 0.0, 0.0, 0.0};
+  int64_t regrids = 0;
+  lgd_regrid_scratch rscratch;  // device tagging scratch
 };
 
 using namespace lgd;
@@ -331,19 +337,45 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
 }
 
 // BergerOliger::regrid at the top of a step (src/core/simulation.rs:255-270).
+double seconds_since(std::chrono::steady_clock::time_point &t0) {
+  const auto t1 = std::chrono::steady_clock::now();
+  const double s = std::chrono::duration<double>(t1 - t0).count();
+  t0 = t1;
+  return s;
+}
+
 int amr_regrid(lgd_sim *s) {
   lgd_amr &g = *s->amr;
   const int tf = s->policy.tag_field;
   int rc;
+  auto clock = std::chrono::steady_clock::now();
+  ++s->regrids;
   // GradientTagger::prepare: restrict + fill the tagged field (adapt.rs:72-76).
   for (int l = g.num_levels() - 1; l >= 1; --l)
     if ((rc = lgd_amr_restrict(s->ctx, g, s->state[(size_t)tf]->data, l))) return rc;
   for (int l = 0; l < g.num_levels(); ++l)
     if ((rc = lgd_amr_fill(s->ctx, g, s->state[(size_t)tf]->data, nullptr, 0.0, l))) return rc;
-  std::vector<double> tag((size_t)g.off.back());
-  if ((rc = lgd_buf_download(s->ctx, s->state[(size_t)tf], 0, tag.data(), tag.size()))) return rc;
+  // Tag and dilate on the device (bitmasks come to the host), or bring the field
+  // to the host and do it there (the "regrid_host" knob).
+  const bool dev = !s->ctx->regrid_host && lgd_regrid_device_supported(g);
+  std::vector<double> tag;
   std::vector<std::vector<lgd_box>> levels;
-  if (lgd_regrid_propose(g, tag, s->policy, levels) == 0) return LGD_OK;
+  int changed;
+  if (dev) {
+    std::vector<std::vector<uint64_t>> bits;
+    if ((rc = lgd_regrid_flag_bits(s->ctx, g, s->state[(size_t)tf]->data, s->policy, s->rscratch,
+                                   bits)))
+      return rc;
+    s->regrid_seconds[0] += seconds_since(clock);
+    changed = lgd_regrid_propose_bits(g, bits, s->policy, levels);
+  } else {
+    tag.resize((size_t)g.off.back());
+    if ((rc = lgd_buf_download(s->ctx, s->state[(size_t)tf], 0, tag.data(), tag.size()))) return rc;
+    s->regrid_seconds[0] += seconds_since(clock);
+    changed = lgd_regrid_propose(g, tag, s->policy, levels);
+  }
+  s->regrid_seconds[1] += seconds_since(clock);
+  if (changed == 0) return LGD_OK;
   lgd_amr *ng = new (std::nothrow) lgd_amr();
   if (!ng) return fail(s->ctx, LGD_ERR_OUT_OF_MEMORY, "host allocation failed");
   if ((rc = lgd_amr_build(s->ctx, g.base, g.ratios, levels, *ng))) {
@@ -351,10 +383,21 @@ int amr_regrid(lgd_sim *s) {
     delete ng;
     return rc;
   }
+  s->regrid_seconds[2] += seconds_since(clock);
   std::vector<lgd_buf *> ns;
   std::vector<double> old_f, new_f;
   const int64_t fine0 = ng->off[ng->level_start[1]];  // level-0 elements (same in both)
   for (size_t f = 0; f < s->state.size(); ++f) {
+    lgd_buf *b = nullptr;
+    if ((rc = lgd_buf_alloc(s->ctx, (size_t)ng->off.back(), &b))) return rc;
+    ns.push_back(b);
+    hipError_t e = hipMemcpyAsync(b->data, s->state[f]->data, (size_t)fine0 * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s->ctx->stream);
+    if (e != hipSuccess) return hip_fail(s->ctx, e, "migrate level 0");
+    if (dev) {
+      if ((rc = lgd_regrid_migrate_device(s->ctx, g, s->state[f]->data, *ng, b->data))) return rc;
+      continue;
+    }
     if ((int)f == tf) {
       old_f.swap(tag);  // already downloaded
     } else {
@@ -362,24 +405,20 @@ int amr_regrid(lgd_sim *s) {
       if ((rc = lgd_buf_download(s->ctx, s->state[f], 0, old_f.data(), old_f.size()))) return rc;
     }
     lgd_regrid_migrate(g, old_f, *ng, new_f);
-    lgd_buf *b = nullptr;
-    if ((rc = lgd_buf_alloc(s->ctx, (size_t)ng->off.back(), &b))) return rc;
-    hipError_t e = hipMemcpyAsync(b->data, s->state[f]->data, (size_t)fine0 * sizeof(double),
-                                  hipMemcpyDeviceToDevice, s->ctx->stream);
-    if (e != hipSuccess) return hip_fail(s->ctx, e, "migrate level 0");
     if (!new_f.empty() && (rc = lgd_buf_upload(s->ctx, b, (size_t)fine0, new_f.data(), new_f.size())))
       return rc;
-    ns.push_back(b);
   }
   drop_graphs(s);  // they hold the old hierarchy's plans and stage buffers
   ++s->amr_epoch;
+  (void)hipStreamSynchronize(s->ctx->stream);
   for (auto *b : s->state) lgd_buf_free(b);
   s->state = ns;
-  (void)hipStreamSynchronize(s->ctx->stream);
   s->amr->free_device();
   delete s->amr;
   s->amr = ng;
-  return alloc_amr_stages(s);
+  rc = alloc_amr_stages(s);
+  s->regrid_seconds[3] += seconds_since(clock);
+  return rc;
 }
 
 // One step after any regrid due at its top.
@@ -602,6 +641,7 @@ int lgd_sim_destroy(lgd_sim *sim) {
   if (sim->ctx && sim->ctx->stream) (void)hipStreamSynchronize(sim->ctx->stream);
   drop_graphs(sim);
   if (sim->d_t) (void)hipFree(sim->d_t);
+  lgd_regrid_scratch_free(sim->rscratch);
   free_all(sim);
   if (sim->amr) lgd_amr_destroy(sim->amr);
   delete sim;
@@ -658,6 +698,14 @@ int lgd_sim_set_graph_steps(lgd_sim *sim, int64_t steps) {
 }
 
 double lgd_sim_time(const lgd_sim *sim) { return sim ? sim->t : 0.0; }
+
+int lgd_sim_regrid_stats(const lgd_sim *sim, int64_t *regrids, double *seconds) {
+  if (!sim) return LGD_ERR_INVALID_ARGUMENT;
+  if (regrids) *regrids = sim->regrids;
+  if (seconds)
+    for (int i = 0; i < 4; ++i) seconds[i] = sim->regrid_seconds[i];
+  return LGD_OK;
+}
 
 int64_t lgd_sim_step_index(const lgd_sim *sim) { return sim ? sim->step_index : -1; }
 

@@ -216,3 +216,41 @@ def test_per_block_dispatch_matches_batched_bitwise(ctx, orc):
         ctx.set_tuning("amr_per_block", 0)
     assert outs[0][0] == outs[1][0] and outs[0][0] > 4
     assert np.array_equal(outs[0][1], outs[1][1]) and np.array_equal(outs[0][2], outs[1][2])
+
+
+@pytest.mark.parametrize("subcycle", [False, True])
+def test_device_regrid_matches_host_regrid_bitwise(ctx, orc, subcycle):
+    """Tagging, dilation and migration on the device (the default) against the host
+    path (the "regrid_host" knob): the same hierarchy after every step and the same
+    bits, with noise and three levels."""
+    say(f"amr device vs host regrid subcycle={subcycle}")
+    base = orc.grid([96, 96], [24, 24], [0, 0], [0.4, 0.4])
+    phi = np.zeros(orc.field_len(base, 1))
+    u = np.zeros_like(phi)
+    orc.initialize_grains(base, phi, u, np.array([[0.4, 0.4, 4.0], [20.0, 15.0, 3.0], [30.0, 32.0, 2.0]]), 0.7)
+    pol = lg.RegridPolicy(3, 2, 0.75, 4, 0.12, 0)
+    dt = 0.2 * (0.4 if subcycle else 0.1) ** 2 / 2.0
+    runs = []
+    try:
+        for host in (0, 1):
+            ctx.set_tuning("regrid_host", host)
+            gg = lg.AmrGrid.from_patches(ctx, [96, 96], [24, 24], [0, 0], [0.4, 0.4], [2, 2], [])
+            sim = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.06, 3.19, 2.0, 0.01),
+                                    wiener=True, seed=9, subcycle=subcycle, policy=pol)
+            sim.field(0).upload(phi)
+            sim.field(1).upload(u)
+            trace = []
+            for s in range(13):
+                sim.step(dt)
+                a = sim.amr
+                trace.append(([a.patch(b) for b in range(a.num_blocks)], sim.field(0).download(),
+                              sim.field(1).download()))
+            runs.append(trace)
+            sim.close()
+    finally:
+        ctx.set_tuning("regrid_host", 0)
+    dev, host = runs
+    assert len(dev[-1][0]) > 16 and any(p[0] == 2 for p in dev[-1][0])
+    for s, (a, b) in enumerate(zip(dev, host)):
+        assert a[0] == b[0], f"step {s}: hierarchies differ"
+        assert np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), f"step {s}: fields differ"
