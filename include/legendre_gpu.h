@@ -308,7 +308,9 @@ int lgd_sim_create_amr(lgd_ctx *ctx, lgd_amr *amr, int32_t model, const lgd_mode
                        const lgd_regrid_policy *policy, lgd_sim **out);
 const lgd_amr *lgd_sim_amr(const lgd_sim *sim);
 int lgd_sim_destroy(lgd_sim *sim);
-/* The current state buffer of field `index` (valid until the next lgd_sim_step). */
+/* The current state buffer of field `index` (valid until the next lgd_sim_step).  Its
+ * interiors are the state; on an AmrGrid (as after the fused Cartesian step) ghost
+ * entries are scratch: every step fills the ghosts it reads. */
 int lgd_sim_field(lgd_sim *sim, int32_t index, lgd_buf **out);
 /* nsteps × Simulation::step(dt) (simulation.rs:255-290): t advances by t += dt on
  * the host in f64, exactly as the reference accumulates it (the noise salt). */

@@ -123,18 +123,40 @@ struct lgd_amr {
   void free_device();
 };
 
+// `reuse0`: a hierarchy on the same base whose level-0 plan (which depends on the
+// base tiling alone) g will take over instead of building its own (a regrid's old
+// hierarchy): g's level-0 plan is then left empty until lgd_amr_take_level0_plan,
+// called once the caller commits to g.
 int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t> &ratios,
-                  const std::vector<std::vector<lgd_box>> &levels, lgd_amr &g);
-int lgd_amr_build_plans(lgd_amr &g);
+                  const std::vector<std::vector<lgd_box>> &levels, lgd_amr &g,
+                  const lgd_amr *reuse0 = nullptr);
+int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0 = nullptr);
+void lgd_amr_take_level0_plan(lgd_amr &g, lgd_amr &from);
 
 // Device transfers and level kernels (lgd_amr.hip).
 int lgd_amr_fill(lgd_ctx *ctx, const lgd_amr &g, double *field, const double *old, double alpha,
                  int level);
 int lgd_amr_restrict(lgd_ctx *ctx, const lgd_amr &g, double *field, int fine_level);
-int lgd_amr_modelc_rhs(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
-                       const double *phi, const double *u, double *kphi, double *ku);
-int lgd_amr_heat_rhs(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, const double *u,
-                     double *k);
-int lgd_amr_apply(lgd_ctx *ctx, const lgd_amr &g, int level, double *x, const double *k, double dt);
-int lgd_amr_wiener(lgd_ctx *ctx, const lgd_amr &g, int level, double *phi, double noise_scale,
-                   uint64_t seed, uint64_t salt);
+
+
+// Ping-pong stepping (each level's current values in one of two buffers per
+// field).  nf <= 2 fields per launch.
+// Fill plan of `level`: lvl = the level's current buffers (copies read, ghosts
+// written); par / old = the parent's current and previous buffers (prolongation,
+// blended old·(1−α) + new·α when old is given; par may be null at level 0).
+int lgd_amr_fill_pp(lgd_ctx *ctx, const lgd_amr &g, int level, int nf, double *const *lvl,
+                    const double *const *par, const double *const *old, double alpha);
+// Restriction of fine_level into its parent, both in their current buffers.
+int lgd_amr_restrict_pp(lgd_ctx *ctx, const lgd_amr &g, int fine_level, int nf,
+                        const double *const *fine, double *const *coarse);
+// Whether level 0 steps through the Cartesian streaming kernel (which reads
+// neighbour interiors itself, so level 0 needs no ghost fill).
+bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g);
+// One fused Euler–Maruyama stage of `level`: (phi, u) → (phi_n, u_n) interiors.
+int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
+                        double dt, const double *phi, const double *u, double *phi_n, double *u_n,
+                        uint64_t seed, uint64_t salt, bool wiener);
+int lgd_amr_heat_step(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, double dt,
+                      const double *u, double *u_n);
+// Copy one level's range (interiors and ghosts) between buffers of the layout.
+int lgd_amr_copy_level(lgd_ctx *ctx, const lgd_amr &g, int level, const double *src, double *dst);

@@ -123,7 +123,8 @@ void lgd_amr::free_device() {
 // tiling; finer boxes are validated (non-empty, inside the domain, aligned to
 // the ratio, disjoint, properly nested) and appended level-major.
 int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t> &ratios,
-                  const std::vector<std::vector<lgd_box>> &levels, lgd_amr &g) {
+                  const std::vector<std::vector<lgd_box>> &levels, lgd_amr &g,
+                  const lgd_amr *reuse0) {
   if (base.dim != 2) return fail(ctx, LGD_ERR_UNSUPPORTED, "AMR hierarchies are 2-D");
   if (levels.size() > ratios.size())
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "more refined levels than ratios");
@@ -216,7 +217,7 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
   g.off.assign(g.box.size() + 1, 0);
   for (size_t b = 0; b < g.box.size(); ++b)
     g.off[b + 1] = g.off[b] + (g.box[b].extent(0) + 2) * (g.box[b].extent(1) + 2);
-  return lgd_amr_build_plans(g);
+  return lgd_amr_build_plans(g, reuse0);
 }
 
 // Bilinear prolongation stencil of fine cell (x, y) at `level` from level-1
@@ -256,9 +257,19 @@ static void prolong_stencil(const lgd_amr &g, int level, int64_t x, int64_t y,
   }
 }
 
-int lgd_amr_build_plans(lgd_amr &g) {
+void lgd_amr_take_level0_plan(lgd_amr &g, lgd_amr &from) {
+  if (g.plans.empty() || from.plans.empty() || g.plans[0].tiles) return;
+  g.plans[0] = from.plans[0];
+  from.plans[0] = lgd_level_plan{};
+}
+
+int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
   lgd_ctx *ctx = g.ctx;
   g.free_device();
+  const bool take0 = reuse0 && !reuse0->plans.empty() && reuse0->level_start.size() >= 2 &&
+                     reuse0->level_start[1] == g.level_start[1] &&
+                     reuse0->base.n[0] == g.base.n[0] && reuse0->base.n[1] == g.base.n[1] &&
+                     reuse0->base.nb[0] == g.base.nb[0] && reuse0->base.nb[1] == g.base.nb[1];
   std::vector<lgd_block_desc> blocks(g.box.size());
   for (size_t b = 0; b < g.box.size(); ++b) {
     blocks[b].nx = (int32_t)g.box[b].extent(0);
@@ -293,6 +304,7 @@ int lgd_amr_build_plans(lgd_amr &g) {
   g.plans.assign((size_t)g.num_levels(), lgd_level_plan{});
   for (int l = 0; l < g.num_levels(); ++l) {
     lgd_level_plan &P = g.plans[(size_t)l];
+    if (l == 0 && take0) continue;  // lgd_amr_take_level0_plan fills it in
     // Level kernels: one 64×4 tile of cells per workgroup.
     std::vector<lgd_tile> tiles;
     P.tile0.clear();
@@ -348,10 +360,24 @@ int lgd_amr_build_plans(lgd_amr &g) {
     std::vector<lgd_restrict_item> items;
     if (l >= 1) {
       const int64_t r = g.ratios[l - 1];
+      const lgd_amr::bucket_index &cix = g.lookup[(size_t)l - 1];
+      const int64_t B = lgd_amr::kBucket;
+      std::vector<uint32_t> cand;
       for (uint32_t fb = g.level_start[l]; fb < g.level_start[l + 1]; ++fb) {
         const lgd_box &fp = g.box[fb];
         lgd_box cov{{fp.lo[0] / r, fp.lo[1] / r}, {fp.hi[0] / r, fp.hi[1] / r}};
-        for (uint32_t cb = g.level_start[l - 1]; cb < g.level_start[l]; ++cb) {
+        // The coarse patches in the buckets under cov, in id order (each overlap is
+        // visited in the same order as a scan over every coarse patch).
+        cand.clear();
+        for (int64_t by = cov.lo[1] / B; by <= (cov.hi[1] - 1) / B; ++by)
+          for (int64_t bx = cov.lo[0] / B; bx <= (cov.hi[0] - 1) / B; ++bx) {
+            const int64_t k = bx + cix.nbx * by;
+            cand.insert(cand.end(), cix.ids.begin() + cix.start[(size_t)k],
+                        cix.ids.begin() + cix.start[(size_t)k + 1]);
+          }
+        std::sort(cand.begin(), cand.end());
+        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+        for (uint32_t cb : cand) {
           const lgd_box &cp = g.box[cb];
           lgd_box ov = cov.clipped(cp);
           for (int64_t cy = ov.lo[1]; cy < ov.hi[1]; ++cy)
@@ -418,11 +444,73 @@ __global__ void amr_restrict_kernel(const lgd_restrict_item *__restrict__ items,
   field[it.target] = sum * inv;
 }
 
-// Model C vector field (Time) on every cell of one level's patches, from
-// ghost-filled slabs (ModelC::vector_field_block on AmrGrid, phasefield.rs:452-475).
-__global__ __launch_bounds__(kTX *kTY) void amr_modelc_rhs_kernel(
+// ---- ping-pong stepping: each level's current values live in one of two buffers
+// per field; a substep reads the current buffer and writes the other, so the
+// values a level had before its step stay readable (the time-interpolation
+// source of the finer level) without a snapshot copy.
+
+// Fill plan of one level for nf fields: copies read the level's current buffer,
+// prolongation corners the parent's current (new) and previous (old) buffers.
+struct FieldPtrs {
+  double *lvl[2];
+  const double *par[2];
+  const double *old[2];
+};
+
+template <bool kTime>
+__global__ void amr_fill_pp_kernel(const lgd_fill_op *__restrict__ ops, int64_t nops,
+                                   const lgd_corner *__restrict__ corners, FieldPtrs F, int nf,
+                                   double one_minus_alpha, double alpha) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nops) return;
+  const lgd_fill_op op = ops[t];
+  for (int f = 0; f < nf; ++f) {
+    double v;
+    if (op.clen == 0) {
+      v = F.lvl[f][op.src];
+    } else {
+      double vn = 0.0, vo = 0.0;
+      for (int c = 0; c < op.clen; ++c) {
+        const lgd_corner k = corners[op.cstart + c];
+        vn = vn + F.par[f][k.src] * k.weight;
+        if (kTime) vo = vo + F.old[f][k.src] * k.weight;
+      }
+      v = kTime ? vo * one_minus_alpha + vn * alpha : vn;
+    }
+    F.lvl[f][op.target] = v;
+  }
+}
+
+// Restriction of nf fields: children from the fine level's current buffer into the
+// coarse level's current buffer.
+struct RestrictPtrs {
+  const double *fine[2];
+  double *coarse[2];
+};
+
+__global__ void amr_restrict_pp_kernel(const lgd_restrict_item *__restrict__ items, int64_t n,
+                                       int r, double inv, RestrictPtrs R, int nf) {
+  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const lgd_restrict_item it = items[t];
+  for (int f = 0; f < nf; ++f) {
+    double sum = 0.0;
+    for (int ky = 0; ky < r; ++ky)
+      for (int kx = 0; kx < r; ++kx) sum = sum + R.fine[f][it.child + ky * it.stride + kx];
+    R.coarse[f][it.target] = sum * inv;
+  }
+}
+
+// Fused Model C stage on one level's patch interiors, from ghost-filled slabs, in
+// the reference's order: the vector field (ModelC::vector_field_block on AmrGrid,
+// phasefield.rs:452-475), the Time update x + dt·k, then the Wiener(0) increment
+// (src/core/driver.rs:139-158) with the cell key per patch extent
+// (grid.rs:407-425).
+template <bool kNoise>
+__global__ __launch_bounds__(kTX *kTY) void amr_modelc_step_kernel(
     const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks,
-    lgd_kr_consts k, const double *phi, const double *u, double *kphi, double *ku) {
+    lgd_kr_consts k, double dt, const double *phi, const double *u, double *phi_n, double *u_n,
+    double noise_scale, uint64_t seed, uint64_t salt, const double *salt_src) {
   const lgd_tile tl = tiles[blockIdx.x];
   const lgd_block_desc bd = blocks[tl.block];
   const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
@@ -434,14 +522,21 @@ __global__ __launch_bounds__(kTX *kTY) void amr_modelc_rhs_kernel(
     for (int di = -1; di <= 1; ++di) p[dj + 1][di + 1] = phi[o + dj * sx + di];
   double dphi, du;
   lgd_modelc_cell(k, p, u[o], u[o + 1], u[o - 1], u[o + sx], u[o - sx], dphi, du);
-  kphi[o] = dphi;
-  ku[o] = du;
+  double pn = p[1][1] + dt * dphi;
+  if (kNoise) {
+    uint64_t bk = lgd_mix_key3(seed, lgd_salt(salt, salt_src), (1ull << 32) | 0ull, (uint64_t)tl.block);
+    uint64_t key = lgd_splitmix64(bk ^ (uint64_t)(i + (int64_t)j * bd.nx));
+    pn = pn + noise_scale * lgd_standard_normal(key);
+  }
+  phi_n[o] = pn;
+  u_n[o] = u[o] + dt * du;
 }
 
-// Heat vector field κ·∇²u on one level's patches (tests/amr.rs Heat model).
-__global__ __launch_bounds__(kTX *kTY) void amr_heat_rhs_kernel(
+// Fused Heat forward-Euler stage on one level's patch interiors (κ·∇²u, the Heat
+// model of tests/amr.rs, then u + dt·k).
+__global__ __launch_bounds__(kTX *kTY) void amr_heat_step_kernel(
     const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks, double ihx,
-    double ihy, double kappa, const double *u, double *k) {
+    double ihy, double kappa, double dt, const double *u, double *u_n) {
   const lgd_tile tl = tiles[blockIdx.x];
   const lgd_block_desc bd = blocks[tl.block];
   const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
@@ -452,29 +547,7 @@ __global__ __launch_bounds__(kTX *kTY) void amr_heat_rhs_kernel(
   double acc = 0.0;
   acc = acc + (((u[o + 1] - (c + c)) + u[o - 1]) * ihx);
   acc = acc + (((u[o + sx] - (c + c)) + u[o - sx]) * ihy);
-  k[o] = acc * kappa;
-}
-
-// Wiener(0) increment on the interior cells of one level's patches
-// (src/core/driver.rs:139-158; cell key per patch extent, grid.rs:407-425).
-__global__ __launch_bounds__(kTX *kTY) void amr_wiener_kernel(
-    const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks,
-    double noise_scale, uint64_t seed, uint64_t salt, const double *salt_src, uint64_t stream,
-    double *phi) {
-  const lgd_tile tl = tiles[blockIdx.x];
-  const lgd_block_desc bd = blocks[tl.block];
-  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
-  if (i >= bd.nx || j >= bd.ny) return;
-  const int64_t o = bd.off + (int64_t)(j + 1) * (bd.nx + 2) + (i + 1);
-  uint64_t bk = lgd_mix_key3(seed, lgd_salt(salt, salt_src), stream, (uint64_t)tl.block);
-  uint64_t key = lgd_splitmix64(bk ^ (uint64_t)(i + (int64_t)j * bd.nx));
-  phi[o] = phi[o] + noise_scale * lgd_standard_normal(key);
-}
-
-__global__ void amr_axpy_kernel(int64_t n, double alpha, double *x, const double *y) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  x[t] = x[t] + alpha * y[t];
+  u_n[o] = c + dt * (acc * kappa);
 }
 
 }  // namespace
@@ -503,75 +576,124 @@ int lgd_amr_restrict(lgd_ctx *ctx, const lgd_amr &g, double *field, int fine_lev
   return launch_check(ctx, "amr_restrict");
 }
 
-int lgd_amr_modelc_rhs(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
-                       const double *phi, const double *u, double *kphi, double *ku) {
+int lgd_amr_fill_pp(lgd_ctx *ctx, const lgd_amr &g, int level, int nf, double *const *lvl,
+                    const double *const *par, const double *const *old, double alpha) {
   const lgd_level_plan &P = g.plans[(size_t)level];
-  if (P.ntiles == 0) return LGD_OK;
+  if (P.nops == 0) return LGD_OK;
+  FieldPtrs F{};
+  for (int f = 0; f < nf; ++f) {
+    F.lvl[f] = lvl[f];
+    F.par[f] = par ? par[f] : nullptr;
+    F.old[f] = old ? old[f] : nullptr;
+  }
+  const unsigned nb = blocks_for(P.nops, 256);
+  if (old)
+    amr_fill_pp_kernel<true><<<nb, 256, 0, ctx->stream>>>(P.ops, P.nops, P.corners, F, nf,
+                                                         1.0 - alpha, alpha);
+  else
+    amr_fill_pp_kernel<false><<<nb, 256, 0, ctx->stream>>>(P.ops, P.nops, P.corners, F, nf, 0.0,
+                                                          0.0);
+  return launch_check(ctx, "amr_fill");
+}
+
+int lgd_amr_restrict_pp(lgd_ctx *ctx, const lgd_amr &g, int fine_level, int nf,
+                        const double *const *fine, double *const *coarse) {
+  const lgd_level_plan &P = g.plans[(size_t)fine_level];
+  if (P.nritems == 0) return LGD_OK;
+  const int r = (int)g.ratios[(size_t)fine_level - 1];
+  RestrictPtrs R{};
+  for (int f = 0; f < nf; ++f) {
+    R.fine[f] = fine[f];
+    R.coarse[f] = coarse[f];
+  }
+  amr_restrict_pp_kernel<<<blocks_for(P.nritems, 256), 256, 0, ctx->stream>>>(
+      P.ritems, P.nritems, r, 1.0 / (double)(r * r), R, nf);
+  return launch_check(ctx, "amr_restrict");
+}
+
+bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g) {
+  return ctx->modelc_kernel != 1 && g.base.n[0] + 2 <= 128;
+}
+
+int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
+                        double dt, const double *phi, const double *u, double *phi_n, double *u_n,
+                        uint64_t seed, uint64_t salt, bool wiener) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
   lgd_cart c = g.base;
   c.h[0] = g.spacing[(size_t)level][0];
   c.h[1] = g.spacing[(size_t)level][1];
-  lgd_kr_consts k = lgd_make_kr_consts(p, c);
-  if (ctx->amr_per_block) {
-    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) {
-      const int64_t n = P.tile0[b + 1] - P.tile0[b];
-      amr_modelc_rhs_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
-          P.tiles + P.tile0[b], g.d_blocks, k, phi, u, kphi, ku);
+  const lgd_kr_consts k = lgd_make_kr_consts(p, c);
+  const bool noise = wiener && p.noise_amplitude != 0.0;
+  const double scale = std::sqrt(dt) * p.noise_amplitude;
+  if (level == 0 && lgd_amr_level0_streams(ctx, g)) {
+    // The base tiling is a Cartesian grid with mirror faces: the streaming kernel
+    // reads neighbour interiors itself (no ghost fill) — the same values and order.
+    const int64_t nb0 = g.level_start[1];
+    const lgd_launch_shape shape = lgd_modelc_shape(ctx, g.base.n[0], g.base.n[1], nb0, noise, 4);
+    lgd_domain d0;
+    d0.k = k;
+    d0.noise_scale = noise ? scale : 0.0;
+    d0.seed = seed;
+    d0.block0 = 0;
+    d0.nbx = (int32_t)g.base.nb[0];
+    d0.nby = (int32_t)g.base.nb[1];
+    d0.per_x = 0;
+    d0.per_y = 0;
+    const uint64_t wiener0 = (1ull << 32) | 0ull;
+    if (ctx->amr_per_block) {
+      for (int64_t b = 0; b < nb0; ++b) {
+        int rc = lgd_modelc_stream_launch(ctx, g.base.n[0], g.base.n[1], d0, nullptr, nullptr, b, 1,
+                                          phi, u, phi_n, u_n, dt, noise, salt, wiener0,
+                                          (int)g.base.n[1], shape.prefetch);
+        if (rc) return rc;
+      }
+      return LGD_OK;
     }
-  } else {
-    amr_modelc_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
-        P.tiles, g.d_blocks, k, phi, u, kphi, ku);
+    return lgd_modelc_stream_launch(ctx, g.base.n[0], g.base.n[1], d0, nullptr, nullptr, 0, nb0, phi,
+                                    u, phi_n, u_n, dt, noise, salt, wiener0, shape.strips,
+                                    shape.prefetch);
   }
-  return launch_check(ctx, "amr_modelc_rhs");
+  if (P.ntiles == 0) return LGD_OK;
+  auto launch = [&](const lgd_tile *tiles, int64_t n) {
+    if (noise)
+      amr_modelc_step_kernel<true><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+          tiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, scale, seed, salt, ctx->salt_src);
+    else
+      amr_modelc_step_kernel<false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+          tiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, 0.0, seed, salt, nullptr);
+  };
+  if (ctx->amr_per_block) {
+    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) launch(P.tiles + P.tile0[b], P.tile0[b + 1] - P.tile0[b]);
+  } else {
+    launch(P.tiles, P.ntiles);
+  }
+  return launch_check(ctx, "amr_modelc_step");
 }
 
-int lgd_amr_heat_rhs(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, const double *u,
-                     double *k) {
+int lgd_amr_heat_step(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, double dt,
+                      const double *u, double *u_n) {
   const lgd_level_plan &P = g.plans[(size_t)level];
   if (P.ntiles == 0) return LGD_OK;
-  double hx = g.spacing[(size_t)level][0], hy = g.spacing[(size_t)level][1];
+  const double hx = g.spacing[(size_t)level][0], hy = g.spacing[(size_t)level][1];
+  auto launch = [&](const lgd_tile *tiles, int64_t n) {
+    amr_heat_step_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+        tiles, g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, dt, u, u_n);
+  };
   if (ctx->amr_per_block) {
-    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) {
-      const int64_t n = P.tile0[b + 1] - P.tile0[b];
-      amr_heat_rhs_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
-          P.tiles + P.tile0[b], g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, u, k);
-    }
+    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) launch(P.tiles + P.tile0[b], P.tile0[b + 1] - P.tile0[b]);
   } else {
-    amr_heat_rhs_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
-        P.tiles, g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, u, k);
+    launch(P.tiles, P.ntiles);
   }
-  return launch_check(ctx, "amr_heat_rhs");
+  return launch_check(ctx, "amr_heat_step");
 }
 
-int lgd_amr_apply(lgd_ctx *ctx, const lgd_amr &g, int level, double *x, const double *k, double dt) {
-  int64_t lo = g.off[g.level_start[level]], hi = g.off[g.level_start[level + 1]];
+int lgd_amr_copy_level(lgd_ctx *ctx, const lgd_amr &g, int level, const double *src, double *dst) {
+  const int64_t lo = g.off[g.level_start[level]], hi = g.off[g.level_start[level + 1]];
   if (hi <= lo) return LGD_OK;
-  if (ctx->amr_per_block) {
-    for (uint32_t b = g.level_start[level]; b < g.level_start[level + 1]; ++b) {
-      const int64_t b0 = g.off[b], n = g.off[b + 1] - g.off[b];
-      amr_axpy_kernel<<<blocks_for(n, 256), 256, 0, ctx->stream>>>(n, dt, x + b0, k + b0);
-    }
-  } else {
-    amr_axpy_kernel<<<blocks_for(hi - lo, 256), 256, 0, ctx->stream>>>(hi - lo, dt, x + lo, k + lo);
-  }
-  return launch_check(ctx, "amr_apply");
-}
-
-int lgd_amr_wiener(lgd_ctx *ctx, const lgd_amr &g, int level, double *phi, double noise_scale,
-                   uint64_t seed, uint64_t salt) {
-  const lgd_level_plan &P = g.plans[(size_t)level];
-  if (P.ntiles == 0) return LGD_OK;
-  if (ctx->amr_per_block) {
-    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) {
-      const int64_t n = P.tile0[b + 1] - P.tile0[b];
-      amr_wiener_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
-          P.tiles + P.tile0[b], g.d_blocks, noise_scale, seed, salt, ctx->salt_src,
-          (1ull << 32) | 0ull, phi);
-    }
-  } else {
-    amr_wiener_kernel<<<(unsigned)P.ntiles, dim3(kTX, kTY), 0, ctx->stream>>>(
-        P.tiles, g.d_blocks, noise_scale, seed, salt, ctx->salt_src, (1ull << 32) | 0ull, phi);
-  }
-  return launch_check(ctx, "amr_wiener");
+  hipError_t e = hipMemcpyAsync(dst + lo, src + lo, (size_t)(hi - lo) * sizeof(double),
+                                hipMemcpyDeviceToDevice, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "amr level copy");
+  return LGD_OK;
 }
 
 // ------------------------------------------------------------------- C ABI

@@ -157,6 +157,7 @@ int lgd_buf_alloc(lgd_ctx *ctx, size_t len, lgd_buf **out) {
   if (!b) return fail(ctx, LGD_ERR_OUT_OF_MEMORY, "host allocation failed");
   b->ctx = ctx;
   b->len = len;
+  b->cap = len;
   if (len > 0) {
     hipError_t e = hipMalloc((void **)&b->data, len * sizeof(double));
     if (e != hipSuccess) {

@@ -50,7 +50,8 @@ lgd_launch_shape lgd_modelc_shape(const lgd_ctx *ctx, int64_t nx, int64_t ny, in
 struct lgd_buf {
   lgd_ctx *ctx = nullptr;
   double *data = nullptr;
-  size_t len = 0;
+  size_t len = 0;  // elements in use
+  size_t cap = 0;  // elements allocated (>= len; an AMR simulation reuses buffers)
 };
 
 // Host and device view of a CartesianGrid<D> (src/geometry/cartesian.rs:28-35).

@@ -44,12 +44,14 @@ struct lgd_sim {
   int32_t subcycle = 0;
   bool adaptive = false;
   lgd_regrid_policy policy{};
-  std::vector<lgd_buf *> snaps;  // subcycling: per (level, field) time-interpolation snapshot
+  std::vector<int8_t> cur;       // AMR: per level, whether next[] holds its current values
   double t = 0.0;
   int64_t step_index = 0;
   // Graphs.
   struct graph {
     lgd_buf *state0;   // state[0] at capture (the fused step's ping-pong parity)
+    uint64_t cur0;     // AMR: the levels' buffer parities at capture, and after
+    uint64_t cur1;
     int64_t epoch;     // AMR hierarchy epoch
     int64_t tuning;    // context tuning epoch
     uint64_t dt_bits;
@@ -102,11 +104,9 @@ void drop_graphs(lgd_sim *s) {
 void free_stages(lgd_sim *s) {
   for (auto *b : s->next) lgd_buf_free(b);
   for (auto *b : s->k) lgd_buf_free(b);
-  for (auto *b : s->snaps) lgd_buf_free(b);
   lgd_buf_free(s->amp);
   s->next.clear();
   s->k.clear();
-  s->snaps.clear();
   s->amp = nullptr;
 }
 
@@ -197,43 +197,60 @@ inline void cell_center2(const lgd_cart &c, int64_t b, int64_t i, int64_t j, dou
 // ------------------------------------------------------------------ AMR path
 int nfields(const lgd_sim *s) { return s->kind == lgd_sim::kModelC ? 2 : 1; }
 
-// Stage buffers of a simulation on an AmrGrid: Tendency(Time) per field, the
-// Wiener(0) amplitude (Model C with noise), one snapshot per (level, field)
-// when subcycling (src/integrators/subcycling.rs:63-73).  After a regrid the
-// buffers are kept when they still hold the new field length (kernels address
-// them by block offsets, so a longer buffer is fine) and grown by a quarter
-// otherwise; tendencies are cleared (their ghost entries stay zero, as the
-// reference's zero-initialised stages), snapshots need no clearing (a level's
-// range is copied in before it is read).
+// Ping-pong state: state[f] and next[f] hold every field twice and cur[l] says
+// which one holds level l's current values (0: state, 1: next).  A substep of
+// level l reads its current buffer and writes the other, so the values before the
+// substep stay in place — the coarse source the finer level blends with in time
+// (subcycling.rs:63-73) — without a snapshot copy.  Ghost entries are scratch:
+// every substep fills the ghosts it reads first.
+lgd_buf *amr_cur(const lgd_sim *s, int f, int level) {
+  return s->cur[(size_t)level] ? s->next[(size_t)f] : s->state[(size_t)f];
+}
+lgd_buf *amr_prev(const lgd_sim *s, int f, int level) {
+  return s->cur[(size_t)level] ? s->state[(size_t)f] : s->next[(size_t)f];
+}
+
+// Bring every level's current values into state[] (copying the levels that live
+// in next[]; the larger part is moved by swapping the pair when level 0 is there).
+int amr_consolidate(lgd_sim *s) {
+  const lgd_amr &g = *s->amr;
+  if (s->cur[0]) {
+    std::swap(s->state, s->next);
+    for (auto &c : s->cur) c ^= 1;
+  }
+  int rc;
+  for (int l = 1; l < g.num_levels(); ++l) {
+    if (!s->cur[(size_t)l]) continue;
+    for (int f = 0; f < nfields(s); ++f)
+      if ((rc = lgd_amr_copy_level(s->ctx, g, l, s->next[(size_t)f]->data, s->state[(size_t)f]->data)))
+        return rc;
+    s->cur[(size_t)l] = 0;
+  }
+  return LGD_OK;
+}
+
+// The second buffer of every field, at least the state's length; kept while it is
+// large enough, else reallocated with a quarter of headroom.
 int alloc_amr_stages(lgd_sim *s) {
   const size_t len = (size_t)s->amr->off.back();
   const int nf = nfields(s);
-  const bool noise = s->kind == lgd_sim::kModelC && s->wiener;
-  const size_t nsnap = s->subcycle ? (size_t)(s->amr->max_levels() * nf) : 0;
-  bool fits = s->k.size() == (size_t)nf && s->snaps.size() == nsnap && (!noise || s->amp);
-  for (auto *b : s->k) fits = fits && b->len >= len;
-  for (auto *b : s->snaps) fits = fits && b->len >= len;
-  if (noise && s->amp) fits = fits && s->amp->len >= len;
   int rc;
-  if (fits) {
-    for (auto *b : s->k)
-      if ((rc = lgd_buf_fill_zero(s->ctx, b))) return rc;
-    if (noise && (rc = lgd_buf_fill_zero(s->ctx, s->amp))) return rc;
-    return LGD_OK;
+  if (s->next.size() != (size_t)nf) {
+    for (auto *b : s->next) lgd_buf_free(b);
+    s->next.assign((size_t)nf, nullptr);
   }
-  const size_t cap = s->k.empty() ? len : len + len / 4;  // grow with headroom after a regrid
-  free_stages(s);
   for (int f = 0; f < nf; ++f) {
-    lgd_buf *b = nullptr;
-    if ((rc = lgd_buf_alloc(s->ctx, cap, &b))) return rc;
-    s->k.push_back(b);
+    lgd_buf *&b = s->next[(size_t)f];
+    if (b && b->cap >= len) {
+      b->len = len;
+      continue;
+    }
+    lgd_buf_free(b);
+    b = nullptr;
+    if ((rc = lgd_buf_alloc(s->ctx, s->regrids ? len + len / 4 : len, &b))) return rc;
+    b->len = len;
   }
-  if (noise && (rc = lgd_buf_alloc(s->ctx, cap, &s->amp))) return rc;
-  for (size_t i = 0; i < nsnap; ++i) {
-    lgd_buf *b = nullptr;
-    if ((rc = lgd_buf_alloc(s->ctx, cap, &b))) return rc;
-    s->snaps.push_back(b);
-  }
+  s->cur.assign((size_t)s->amr->max_levels(), 0);
   return LGD_OK;
 }
 
@@ -243,79 +260,86 @@ double amr_stable_dt(const lgd_sim *s, int level) {
                                      : 0.25 * std::pow(h, 2) / s->kappa;
 }
 
-// Vector field + stage update on one level's patches (eval_drift_level,
-// eval_tendency_level, apply_step_level: src/integrators/mod.rs:205-240,
-// src/core/state.rs:706-753).
-int amr_level_update(lgd_sim *s, int level, double dt, uint64_t salt) {
-  lgd_ctx *ctx = s->ctx;
+// Fill the ghosts of `level` in its current buffers: same-level copies and
+// prolongation from the parent (blended between the parent's previous and current
+// values when `blend`).  Level 0 on the streaming kernel needs none.
+int amr_fill_level(lgd_sim *s, int level, bool blend, double alpha) {
   const lgd_amr &g = *s->amr;
-  int rc;
-  if (s->kind == lgd_sim::kModelC)
-    rc = lgd_amr_modelc_rhs(ctx, g, s->params, level, s->state[0]->data, s->state[1]->data,
-                            s->k[0]->data, s->k[1]->data);
-  else
-    rc = lgd_amr_heat_rhs(ctx, g, s->kappa, level, s->state[0]->data, s->k[0]->data);
-  return rc;
+  if (level == 0 && s->kind == lgd_sim::kModelC && lgd_amr_level0_streams(s->ctx, g)) return LGD_OK;
+  const int nf = nfields(s);
+  double *lvl[2] = {nullptr, nullptr};
+  const double *par[2] = {nullptr, nullptr}, *old[2] = {nullptr, nullptr};
+  for (int f = 0; f < nf; ++f) {
+    lvl[f] = amr_cur(s, f, level)->data;
+    if (level > 0) {
+      par[f] = amr_cur(s, f, level - 1)->data;
+      old[f] = amr_prev(s, f, level - 1)->data;
+    }
+  }
+  return lgd_amr_fill_pp(s->ctx, g, level, nf, lvl, level > 0 ? par : nullptr,
+                         level > 0 && blend ? old : nullptr, alpha);
 }
 
-// `tslot`: the device time slot holding this (sub)step's time while capturing.
-int amr_level_apply(lgd_sim *s, int level, double dt, uint64_t salt, int tslot) {
-  int rc;
-  for (int f = 0; f < nfields(s); ++f)
-    if ((rc = lgd_amr_apply(s->ctx, *s->amr, level, s->state[f]->data, s->k[f]->data, dt))) return rc;
-  if (s->kind == lgd_sim::kModelC && s->wiener && s->params.noise_amplitude != 0.0) {
-    double scale = std::sqrt(dt) * s->params.noise_amplitude;
-    s->ctx->salt_src = time_slot(s, tslot);
-    rc = lgd_amr_wiener(s->ctx, *s->amr, level, s->state[0]->data, scale, s->seed, salt);
-    s->ctx->salt_src = nullptr;
-    if (rc) return rc;
+// Restrict `fine_level` into its parent (both current).
+int amr_restrict_level(lgd_sim *s, int fine_level) {
+  const int nf = nfields(s);
+  const double *fine[2] = {nullptr, nullptr};
+  double *coarse[2] = {nullptr, nullptr};
+  for (int f = 0; f < nf; ++f) {
+    fine[f] = amr_cur(s, f, fine_level)->data;
+    coarse[f] = amr_cur(s, f, fine_level - 1)->data;
   }
+  return lgd_amr_restrict_pp(s->ctx, *s->amr, fine_level, nf, fine, coarse);
+}
+
+// One fused stage of `level` (eval_drift_level + apply_step_level,
+// src/integrators/mod.rs:205-240, src/core/state.rs:706-753): current → other
+// buffer, which becomes current.  `tslot`: the device time slot this substep's
+// noise reads while a graph is captured.
+int amr_level_step(lgd_sim *s, int level, double dt, uint64_t salt, int tslot) {
+  const lgd_amr &g = *s->amr;
+  int rc;
+  if (s->kind == lgd_sim::kModelC) {
+    s->ctx->salt_src = time_slot(s, tslot);
+    rc = lgd_amr_modelc_step(s->ctx, g, s->params, level, dt, amr_cur(s, 0, level)->data,
+                             amr_cur(s, 1, level)->data, amr_prev(s, 0, level)->data,
+                             amr_prev(s, 1, level)->data, s->seed, salt, s->wiener != 0);
+    s->ctx->salt_src = nullptr;
+  } else {
+    rc = lgd_amr_heat_step(s->ctx, g, s->kappa, level, dt, amr_cur(s, 0, level)->data,
+                           amr_prev(s, 0, level)->data);
+  }
+  if (rc) return rc;
+  s->cur[(size_t)level] ^= 1;
   return LGD_OK;
 }
 
 // EulerMaruyama on the AmrGrid with one global dt: ModelC/Heat fill_ghosts
-// (restrict every field finest→coarsest, then fill every level), every level's
-// vector field, then every level's update.
+// (restrict every field finest→coarsest, then fill every level), then every
+// level's stage.
 int amr_global_step(lgd_sim *s, double dt, uint64_t salt) {
   const lgd_amr &g = *s->amr;
-  int rc, nf = nfields(s);
-  for (int f = 0; f < nf; ++f)
-    for (int l = g.num_levels() - 1; l >= 1; --l)
-      if ((rc = lgd_amr_restrict(s->ctx, g, s->state[f]->data, l))) return rc;
-  for (int f = 0; f < nf; ++f)
-    for (int l = 0; l < g.num_levels(); ++l)
-      if ((rc = lgd_amr_fill(s->ctx, g, s->state[f]->data, nullptr, 0.0, l))) return rc;
+  int rc;
+  for (int l = g.num_levels() - 1; l >= 1; --l)
+    if ((rc = amr_restrict_level(s, l))) return rc;
   for (int l = 0; l < g.num_levels(); ++l)
-    if ((rc = amr_level_update(s, l, dt, salt))) return rc;
+    if ((rc = amr_fill_level(s, l, false, 0.0))) return rc;
   for (int l = 0; l < g.num_levels(); ++l)
-    if ((rc = amr_level_apply(s, l, dt, salt, 0))) return rc;
+    if ((rc = amr_level_step(s, l, dt, salt, 0))) return rc;
   return LGD_OK;
 }
 
-// Subcycle::advance (src/integrators/subcycling.rs:195-267), flattened into a
-// launch sequence: snapshot the level, fill its ghosts (coarse source blended in
-// time from the parent's snapshot), step it, recurse n = round(dt_l/dt_{l+1})
-// times into the finer level at t + s·dt_child (fma), then restrict it back.
+// Subcycle::advance (src/integrators/subcycling.rs:195-267) as a launch sequence:
+// fill the level's ghosts (coarse source blended in time between the parent's
+// previous and current values), step it, recurse n = round(dt_l/dt_{l+1}) times
+// into the finer level at t + s·dt_child (fma), then restrict it back.
 int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
   const lgd_amr &g = *s->amr;
-  const int nf = nfields(s);
   int rc;
-  // Only this level's cells of a snapshot are ever read (as prolongation
-  // donors of the finer level), and a level's patches are contiguous.
-  int64_t lo = g.off[g.level_start[level]], hi = g.off[g.level_start[level + 1]];
-  for (int f = 0; f < nf && hi > lo; ++f) {
-    copy_kernel<<<blocks_for(hi - lo, 256), 256, 0, s->ctx->stream>>>(
-        hi - lo, s->state[f]->data + lo, s->snaps[(size_t)(level * nf + f)]->data + lo);
-    if ((rc = launch_check(s->ctx, "snapshot"))) return rc;
-  }
-  for (int f = 0; f < nf; ++f) {
-    const double *old = level > 0 ? s->snaps[(size_t)((level - 1) * nf + f)]->data : nullptr;
-    if ((rc = lgd_amr_fill(s->ctx, g, s->state[f]->data, old, alpha, level))) return rc;
-  }
+  if ((rc = amr_fill_level(s, level, true, alpha))) return rc;
   uint64_t salt;
   std::memcpy(&salt, &t, sizeof salt);
-  if ((rc = amr_level_update(s, level, dt, salt))) return rc;
-  if ((rc = amr_level_apply(s, level, dt, salt, level))) return rc;
+  if ((rc = amr_level_step(s, level, dt, salt, level))) return rc;
   const int child = level + 1;
   if (child < g.num_levels()) {
     double n = std::fmax(std::round(amr_stable_dt(s, level) / amr_stable_dt(s, child)), 1.0);
@@ -330,13 +354,11 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
                             (double)k / (double)nn)))
         return rc;
     }
-    for (int f = 0; f < nf; ++f)
-      if ((rc = lgd_amr_restrict(s->ctx, g, s->state[f]->data, child))) return rc;
+    if ((rc = amr_restrict_level(s, child))) return rc;
   }
   return LGD_OK;
 }
 
-// BergerOliger::regrid at the top of a step (src/core/simulation.rs:255-270).
 double seconds_since(std::chrono::steady_clock::time_point &t0) {
   const auto t1 = std::chrono::steady_clock::now();
   const double s = std::chrono::duration<double>(t1 - t0).count();
@@ -344,10 +366,12 @@ double seconds_since(std::chrono::steady_clock::time_point &t0) {
   return s;
 }
 
+// BergerOliger::regrid at the top of a step (src/core/simulation.rs:255-270).
 int amr_regrid(lgd_sim *s) {
+  int rc;
+  if ((rc = amr_consolidate(s))) return rc;
   lgd_amr &g = *s->amr;
   const int tf = s->policy.tag_field;
-  int rc;
   auto clock = std::chrono::steady_clock::now();
   ++s->regrids;
   // GradientTagger::prepare: restrict + fill the tagged field (adapt.rs:72-76).
@@ -378,41 +402,67 @@ int amr_regrid(lgd_sim *s) {
   if (changed == 0) return LGD_OK;
   lgd_amr *ng = new (std::nothrow) lgd_amr();
   if (!ng) return fail(s->ctx, LGD_ERR_OUT_OF_MEMORY, "host allocation failed");
-  if ((rc = lgd_amr_build(s->ctx, g.base, g.ratios, levels, *ng))) {
+  if ((rc = lgd_amr_build(s->ctx, g.base, g.ratios, levels, *ng, &g))) {
     ng->free_device();
     delete ng;
     return rc;
   }
   s->regrid_seconds[2] += seconds_since(clock);
-  std::vector<lgd_buf *> ns;
+  // The new state goes into the spare buffers (next[]) when they hold the new
+  // length, else into fresh ones.
+  const size_t len = (size_t)ng->off.back();
+  std::vector<lgd_buf *> ns, fresh;
   std::vector<double> old_f, new_f;
   const int64_t fine0 = ng->off[ng->level_start[1]];  // level-0 elements (same in both)
+  // A failure from here on leaves the old hierarchy and state untouched.
+  auto abandon = [&](int code) {
+    (void)hipStreamSynchronize(s->ctx->stream);
+    for (auto *b : fresh) lgd_buf_free(b);
+    ng->free_device();
+    delete ng;
+    return code;
+  };
   for (size_t f = 0; f < s->state.size(); ++f) {
-    lgd_buf *b = nullptr;
-    if ((rc = lgd_buf_alloc(s->ctx, (size_t)ng->off.back(), &b))) return rc;
+    lgd_buf *b = s->next.size() > f ? s->next[f] : nullptr;
+    if (!b || b->cap < len) {
+      b = nullptr;
+      if ((rc = lgd_buf_alloc(s->ctx, len + len / 4, &b))) return abandon(rc);
+      fresh.push_back(b);
+    }
     ns.push_back(b);
     hipError_t e = hipMemcpyAsync(b->data, s->state[f]->data, (size_t)fine0 * sizeof(double),
                                   hipMemcpyDeviceToDevice, s->ctx->stream);
-    if (e != hipSuccess) return hip_fail(s->ctx, e, "migrate level 0");
+    if (e != hipSuccess) return abandon(hip_fail(s->ctx, e, "migrate level 0"));
     if (dev) {
-      if ((rc = lgd_regrid_migrate_device(s->ctx, g, s->state[f]->data, *ng, b->data))) return rc;
+      if ((rc = lgd_regrid_migrate_device(s->ctx, g, s->state[f]->data, *ng, b->data)))
+        return abandon(rc);
       continue;
     }
     if ((int)f == tf) {
       old_f.swap(tag);  // already downloaded
     } else {
       old_f.resize((size_t)g.off.back());
-      if ((rc = lgd_buf_download(s->ctx, s->state[f], 0, old_f.data(), old_f.size()))) return rc;
+      if ((rc = lgd_buf_download(s->ctx, s->state[f], 0, old_f.data(), old_f.size())))
+        return abandon(rc);
     }
     lgd_regrid_migrate(g, old_f, *ng, new_f);
-    if (!new_f.empty() && (rc = lgd_buf_upload(s->ctx, b, (size_t)fine0, new_f.data(), new_f.size())))
-      return rc;
+    if (!new_f.empty()) {
+      e = hipMemcpy(b->data + fine0, new_f.data(), new_f.size() * sizeof(double),
+                    hipMemcpyHostToDevice);
+      if (e != hipSuccess) return abandon(hip_fail(s->ctx, e, "migrate upload"));
+    }
   }
-  drop_graphs(s);  // they hold the old hierarchy's plans and stage buffers
+  drop_graphs(s);  // they hold the old hierarchy's plans and buffers
   ++s->amr_epoch;
   (void)hipStreamSynchronize(s->ctx->stream);
-  for (auto *b : s->state) lgd_buf_free(b);
+  // The old state buffers become the spares; spares that went unused are freed.
+  std::vector<lgd_buf *> spare = s->state;
+  for (size_t f = 0; f < s->next.size(); ++f)
+    if (s->next[f] != ns[f]) lgd_buf_free(s->next[f]);
+  for (auto *b : ns) b->len = len;
   s->state = ns;
+  s->next = spare;
+  lgd_amr_take_level0_plan(*ng, *s->amr);
   s->amr->free_device();
   delete s->amr;
   s->amr = ng;
@@ -444,9 +494,21 @@ void step_done(lgd_sim *s, double dt) {
 
 // Capture `n` steps from the current buffers into *out.  The host bookkeeping the
 // capture pass performs (buffer swaps) is undone: the graph has not run yet.
-int capture(lgd_sim *s, double dt, int64_t n, hipGraphExec_t *out) {
+uint64_t parity_bits(const lgd_sim *s) {
+  uint64_t m = 0;
+  for (size_t l = 0; l < s->cur.size() && l < 64; ++l)
+    if (s->cur[l]) m |= 1ull << l;
+  return m;
+}
+
+void set_parity_bits(lgd_sim *s, uint64_t m) {
+  for (size_t l = 0; l < s->cur.size() && l < 64; ++l) s->cur[l] = (int8_t)((m >> l) & 1u);
+}
+
+int capture(lgd_sim *s, double dt, int64_t n, hipGraphExec_t *out, uint64_t *cur_after) {
   lgd_ctx *ctx = s->ctx;
   const auto state = s->state, next = s->next;
+  const auto cur = s->cur;
   const double t = s->t;
   const int64_t step_index = s->step_index;
   hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
@@ -467,6 +529,8 @@ int capture(lgd_sim *s, double dt, int64_t n, hipGraphExec_t *out) {
   ctx->salt_src = nullptr;
   hipGraph_t graph = nullptr;
   e = hipStreamEndCapture(ctx->stream, &graph);
+  *cur_after = parity_bits(s);
+  s->cur = cur;
   s->state = state;
   s->next = next;
   s->t = t;
@@ -488,9 +552,10 @@ int graph_steps(lgd_sim *s, double dt, int64_t n) {
   uint64_t dt_bits;
   std::memcpy(&dt_bits, &dt, sizeof dt_bits);
   lgd_sim::graph *hit = nullptr;
+  const uint64_t cur0 = parity_bits(s);
   for (auto &g : s->graphs)
-    if (g.state0 == s->state[0] && g.epoch == s->amr_epoch && g.tuning == ctx->tuning_epoch &&
-        g.dt_bits == dt_bits && g.nsteps == n)
+    if (g.state0 == s->state[0] && g.cur0 == cur0 && g.epoch == s->amr_epoch &&
+        g.tuning == ctx->tuning_epoch && g.dt_bits == dt_bits && g.nsteps == n)
       hit = &g;
   int rc;
   if (!hit) {
@@ -505,8 +570,8 @@ int graph_steps(lgd_sim *s, double dt, int64_t n) {
         (void)hipGraphExecDestroy(g.exec);
     }
     s->graphs = keep;
-    lgd_sim::graph g{s->state[0], s->amr_epoch, ctx->tuning_epoch, dt_bits, n, nullptr};
-    if ((rc = capture(s, dt, n, &g.exec))) return rc;
+    lgd_sim::graph g{s->state[0], cur0, 0, s->amr_epoch, ctx->tuning_epoch, dt_bits, n, nullptr};
+    if ((rc = capture(s, dt, n, &g.exec, &g.cur1))) return rc;
     s->graphs.push_back(g);
     hit = &s->graphs.back();
   }
@@ -519,6 +584,7 @@ int graph_steps(lgd_sim *s, double dt, int64_t n) {
   }
   hipError_t e = hipGraphLaunch(hit->exec, ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "graph launch");
+  set_parity_bits(s, hit->cur1);
   for (int64_t k = 0; k < n; ++k) step_done(s, dt);
   s->d_t_known = true;
   s->d_t_value = s->t;
@@ -652,6 +718,8 @@ int lgd_sim_field(lgd_sim *sim, int32_t index, lgd_buf **out) {
   if (!sim || !out) return LGD_ERR_INVALID_ARGUMENT;
   if (index < 0 || index >= (int32_t)sim->state.size())
     return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT, "field index out of range");
+  int rc;
+  if (sim->amr && (rc = amr_consolidate(sim))) return rc;
   *out = sim->state[index];
   return LGD_OK;
 }

@@ -182,3 +182,15 @@ def within(got, want, rel=1e-10, eps=1e-12) -> bool:
     diff = np.abs(got - want)
     tol = np.maximum(eps, rel * np.maximum(np.abs(got), np.abs(want)))
     return bool(np.all(diff <= tol))
+
+
+def amr_interior_mask(amr) -> np.ndarray:
+    """Interior entries of every patch of a device AmrGrid's field layout (ghost
+    entries of a simulation's state are scratch after a step)."""
+    m = np.zeros(amr.field_len, dtype=bool)
+    for b in range(amr.num_blocks):
+        _, lo, hi = amr.patch(b)
+        nx, ny = hi[0] - lo[0], hi[1] - lo[1]
+        o = amr.offset(b)
+        m[o:o + (nx + 2) * (ny + 2)].reshape(ny + 2, nx + 2)[1:-1, 1:-1] = True
+    return m
