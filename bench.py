@@ -489,17 +489,21 @@ def run_amr(args) -> int:
     log(f"[amr] {nregrid} regrids, seconds by phase: tag field to host {phases[0]:.3f}, "
         f"proposal {phases[1]:.3f}, new hierarchy {phases[2]:.3f}, migration {phases[3]:.3f}")
     sim.close()
-    pb_chunks = min(nchunks, max(1, args.per_block_chunks))
-    ctx.set_tuning("amr_per_block", 1)
-    ctx_per_block[0] = True  # the baseline launches eagerly, one launch per patch
-    sim = build()
-    pb_times, pb_updates, pb_shapes, pb_first = run(sim, pb_chunks, "per-patch launches")
-    sim.close()
-    ctx.set_tuning("amr_per_block", 0)
-    same = all(pb_shapes[i] == shapes[i] for i in range(pb_chunks))
+    pb_chunks = min(nchunks, max(0, args.per_block_chunks))
+    pb_times, pb_first, same = [], [], None
+    if pb_chunks:
+        ctx.set_tuning("amr_per_block", 1)
+        ctx_per_block[0] = True  # the baseline launches eagerly, one launch per patch
+        sim = build()
+        pb_times, pb_updates, pb_shapes, pb_first = run(sim, pb_chunks, "per-patch launches")
+        sim.close()
+        ctx.set_tuning("amr_per_block", 0)
+        same = all(pb_shapes[i] == shapes[i] for i in range(pb_chunks))
     total = sum(times)
     value = sum(updates) / total / 1e9
-    speedup = sum(pb_times) / sum(times[:pb_chunks])
+    speedup = sum(pb_times) / sum(times[:pb_chunks]) if pb_chunks else None
+    speedup_steps = ((sum(pb_times) - sum(pb_first)) /
+                     (sum(times[:pb_chunks]) - sum(first[:pb_chunks]))) if pb_chunks else None
     ic.close()
     ctx.close()
     line = {
@@ -532,15 +536,14 @@ def run_amr(args) -> int:
             "per_patch_chunks_compared": pb_chunks,
             "batched_ms_first_chunks": round(sum(times[:pb_chunks]) * 1e3, 3),
             "per_patch_ms_first_chunks": round(sum(pb_times) * 1e3, 3),
-            "speedup_vs_per_patch_launches": round(speedup, 2),
+            "speedup_vs_per_patch_launches": round(speedup, 2) if speedup else None,
             "regrid_and_first_step_seconds": round(sum(first), 3),
             "regrids": nregrid,
             "regrid_phase_seconds": {"tag_field_to_host": round(float(phases[0]), 4),
                                      "proposal": round(float(phases[1]), 4),
                                      "new_hierarchy": round(float(phases[2]), 4),
                                      "migration": round(float(phases[3]), 4)},
-            "speedup_excluding_regrid_steps": round(
-                (sum(pb_times) - sum(pb_first)) / (sum(times[:pb_chunks]) - sum(first[:pb_chunks])), 2),
+            "speedup_excluding_regrid_steps": round(speedup_steps, 2) if speedup_steps else None,
             "same_hierarchy": same,
         },
     }
@@ -556,7 +559,7 @@ def main() -> int:
                          "ensemble: 512 batched realizations; amr: 3-level subcycled hierarchy")
     ap.add_argument("--amr-steps", type=int, default=0, help="AMR coarse steps (0 = 500)")
     ap.add_argument("--per-block-chunks", type=int, default=2,
-                    help="regrid intervals timed with one launch per patch (amr)")
+                    help="regrid intervals timed with one launch per patch (amr; 0 skips the baseline)")
     ap.add_argument("--ens-steps", type=int, default=0, help="ensemble steps (0 = 2000)")
     ap.add_argument("--per-block-steps", type=int, default=3,
                     help="steps timed with one launch per block (ensemble)")
