@@ -77,6 +77,7 @@ struct lgd_dev_level {
 };
 
 struct lgd_level_plan {
+  void *block = nullptr;  // one device allocation holding the four arrays below
   lgd_tile *tiles = nullptr;
   int64_t ntiles = 0;
   std::vector<int64_t> tile0;  // first tile of each of the level's patches, + end
@@ -106,6 +107,7 @@ struct lgd_amr {
   static constexpr int64_t kBucket = 16;
   std::vector<bucket_index> lookup;
   std::vector<int64_t> off;           // ghost-1 slab offsets, nblocks + 1
+  void *d_arena = nullptr;  // one device allocation: d_blocks and the patch index
   lgd_block_desc *d_blocks = nullptr;
   std::vector<lgd_level_plan> plans;  // per populated level
   // find_patch on the device: per-level headers, concatenated bucket starts (into

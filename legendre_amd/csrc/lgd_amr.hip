@@ -20,6 +20,7 @@
 #include <array>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
 #include <new>
 #include <vector>
 
@@ -43,16 +44,34 @@ inline int64_t floor_div(int64_t a, int64_t b) {
   return (a % b < 0) ? q - 1 : q;
 }
 
-template <class T>
-int upload_vec(lgd_ctx *ctx, const std::vector<T> &v, T **dptr) {
-  *dptr = nullptr;
-  if (v.empty()) return LGD_OK;
-  hipError_t e = hipMalloc((void **)dptr, v.size() * sizeof(T));
-  if (e != hipSuccess) return hip_fail(ctx, e, "hipMalloc(amr plan)");
-  e = hipMemcpy(*dptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice);
-  if (e != hipSuccess) return hip_fail(ctx, e, "upload(amr plan)");
-  return LGD_OK;
-}
+// Several host arrays uploaded as one device allocation (256-byte aligned parts),
+// so a hierarchy costs a handful of allocations, not one per array.
+struct Packer {
+  std::vector<char> host;
+  std::vector<std::pair<size_t, void **>> parts;
+  template <class T>
+  void add(const std::vector<T> &v, T **dptr) {
+    const size_t off = (host.size() + 255) & ~size_t(255);
+    host.resize(off + v.size() * sizeof(T));
+    if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
+    parts.push_back({off, (void **)dptr});
+  }
+  int upload(lgd_ctx *ctx, void **base) {
+    *base = nullptr;
+    for (auto &p : parts) *p.second = nullptr;
+    if (host.empty()) return LGD_OK;
+    hipError_t e = hipMalloc(base, host.size());
+    if (e != hipSuccess) {
+      *base = nullptr;
+      return hip_fail(ctx, e, "hipMalloc(amr plan)");
+    }
+    e = hipMemcpy(*base, host.data(), host.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(ctx, e, "upload(amr plan)");
+    for (auto &p : parts) *p.second = (char *)*base + p.first;
+    return LGD_OK;
+  }
+};
+
 
 // Whether every cell of `need` lies in a box of `cover` (run-skipping walk
 // along axis 0).
@@ -100,22 +119,14 @@ int64_t lgd_amr::index(int64_t b, int64_t i, int64_t j) const {
 }
 
 void lgd_amr::free_device() {
-  (void)hipFree(d_blocks);
+  (void)hipFree(d_arena);
+  d_arena = nullptr;
   d_blocks = nullptr;
-  (void)hipFree(d_levels);
-  (void)hipFree(d_bstart);
-  (void)hipFree(d_bids);
-  (void)hipFree(d_lo);
   d_levels = nullptr;
   d_bstart = nullptr;
   d_bids = nullptr;
   d_lo = nullptr;
-  for (auto &p : plans) {
-    (void)hipFree(p.tiles);
-    (void)hipFree(p.ops);
-    (void)hipFree(p.corners);
-    (void)hipFree(p.ritems);
-  }
+  for (auto &p : plans) (void)hipFree(p.block);
   plans.clear();
 }
 
@@ -277,8 +288,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     blocks[b].level = g.level[b];
     blocks[b].off = g.off[b];
   }
-  int rc = upload_vec(ctx, blocks, &g.d_blocks);
-  if (rc) return rc;
+  int rc;
   {
     // find_patch's index, concatenated over the levels.
     std::vector<lgd_dev_level> lv;
@@ -296,10 +306,13 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
       lo.push_back(b.lo[0]);
       lo.push_back(b.lo[1]);
     }
-    if ((rc = upload_vec(ctx, lv, &g.d_levels))) return rc;
-    if ((rc = upload_vec(ctx, bstart, &g.d_bstart))) return rc;
-    if ((rc = upload_vec(ctx, bids, &g.d_bids))) return rc;
-    if ((rc = upload_vec(ctx, lo, &g.d_lo))) return rc;
+    Packer pk;
+    pk.add(blocks, &g.d_blocks);
+    pk.add(lv, &g.d_levels);
+    pk.add(bstart, &g.d_bstart);
+    pk.add(bids, &g.d_bids);
+    pk.add(lo, &g.d_lo);
+    if ((rc = pk.upload(ctx, &g.d_arena))) return rc;
   }
   g.plans.assign((size_t)g.num_levels(), lgd_level_plan{});
   for (int l = 0; l < g.num_levels(); ++l) {
@@ -395,10 +408,12 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     P.nops = (int64_t)ops.size();
     P.ncorners = (int64_t)corners.size();
     P.nritems = (int64_t)items.size();
-    if ((rc = upload_vec(ctx, tiles, &P.tiles))) return rc;
-    if ((rc = upload_vec(ctx, ops, &P.ops))) return rc;
-    if ((rc = upload_vec(ctx, corners, &P.corners))) return rc;
-    if ((rc = upload_vec(ctx, items, &P.ritems))) return rc;
+    Packer pk;
+    pk.add(tiles, &P.tiles);
+    pk.add(ops, &P.ops);
+    pk.add(corners, &P.corners);
+    pk.add(items, &P.ritems);
+    if ((rc = pk.upload(ctx, &P.block))) return rc;
   }
   return LGD_OK;
 }

@@ -21,10 +21,9 @@ void lgd_regrid_migrate(const lgd_amr &old_g, const std::vector<double> &old_f,
 
 // Device scratch of the regrid's tagging pass (kept across regrids).
 struct lgd_regrid_scratch {
-  uint8_t *T = nullptr;
-  size_t tcap = 0;
-  uint64_t *bits = nullptr;
-  size_t wcap = 0;
+  uint64_t *T = nullptr;     // tag bitmasks
+  uint64_t *bits = nullptr;  // dilated bitmasks
+  size_t wcap = 0;           // words of each
 };
 void lgd_regrid_scratch_free(lgd_regrid_scratch &sc);
 // Whether the device path covers this hierarchy's depth.

@@ -404,12 +404,13 @@ __global__ __launch_bounds__(kRTX *kRTY) void migrate_kernel(const lgd_tile *til
       dev_sample<kMaxSampleDepth>(old, R, oldf, level, x, y);
 }
 
-// GradientTagger on one level's patch interiors: a byte per flagged cell of the
-// level domain (row-major, width W).
+// GradientTagger on one level's patch interiors: the flagged cells' bits in a
+// row-major bitmask of the level domain (wpr words of 64 cells per row).
 __global__ __launch_bounds__(kRTX *kRTY) void tag_kernel(const lgd_tile *tiles,
                                                          const lgd_block_desc *blocks,
                                                          const int64_t *lo, const double *v,
-                                                         double threshold, uint8_t *T, int64_t W) {
+                                                         double threshold,
+                                                         unsigned long long *T, int64_t wpr) {
   const lgd_tile tl = tiles[blockIdx.x];
   const lgd_block_desc bd = blocks[tl.block];
   const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
@@ -419,26 +420,36 @@ __global__ __launch_bounds__(kRTX *kRTY) void tag_kernel(const lgd_tile *tiles,
   const double gx = fabs((c[1] - c[-1]) * 0.5);
   const double gy = fabs((c[sx] - c[-sx]) * 0.5);
   const double gm = fmax(fmax(0.0, gx), gy);
-  if (gm > threshold) T[(lo[2 * tl.block + 1] + j) * W + lo[2 * tl.block] + i] = 1;
+  if (gm > threshold) {
+    const int64_t x = lo[2 * tl.block] + i, y = lo[2 * tl.block + 1] + j;
+    atomicOr(T + y * wpr + x / 64, 1ull << (x % 64));
+  }
 }
 
-// Buffer dilation and packing: one wavefront per 64-cell word of the bitmask.
-__global__ __launch_bounds__(256) void dilate_pack_kernel(const uint8_t *T, int64_t W, int64_t H,
-                                                          int64_t wpr, int buffer,
-                                                          unsigned long long *bits) {
-  const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int lane = threadIdx.x & 63;
-  if (wave >= H * wpr) return;  // whole wavefronts
-  const int64_t y = wave / wpr, x = (wave - y * wpr) * 64 + lane;
-  bool f = false;
-  if (x < W)
-    for (int64_t yy = y - buffer; yy <= y + buffer; ++yy) {
-      if (yy < 0 || yy >= H) continue;
-      for (int64_t xx = x - buffer; xx <= x + buffer; ++xx)
-        if (xx >= 0 && xx < W && T[yy * W + xx]) f = true;
+// Buffer dilation, one bitmask word (64 cells) per thread: the OR over rows
+// y−b … y+b of each row's word shifted by −b … b cells (bits carried in from the
+// neighbouring words), clipped to the domain (width W).
+__global__ __launch_bounds__(256) void dilate_kernel(const unsigned long long *T, int64_t W,
+                                                     int64_t H, int64_t wpr, int buffer,
+                                                     unsigned long long *bits) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= H * wpr) return;
+  const int64_t y = t / wpr, w = t - y * wpr;
+  unsigned long long out = 0;
+  for (int64_t yy = y - buffer; yy <= y + buffer; ++yy) {
+    if (yy < 0 || yy >= H) continue;
+    const unsigned long long *row = T + yy * wpr;
+    const unsigned long long c = row[w], L = w > 0 ? row[w - 1] : 0ull,
+                             R = w + 1 < wpr ? row[w + 1] : 0ull;
+    out |= c;
+    for (int d = 1; d <= buffer; ++d) {
+      out |= (c << d) | (L >> (64 - d));  // cell x takes the flag of x − d
+      out |= (c >> d) | (R << (64 - d));  // and of x + d
     }
-  const unsigned long long m = __ballot(f);
-  if (lane == 0) bits[wave] = m;
+  }
+  const int64_t valid = W - w * 64;  // cells of this word inside the domain
+  if (valid < 64) out &= (1ull << valid) - 1ull;
+  bits[t] = out;
 }
 
 DevIndex dev_index(const lgd_amr &g) {
@@ -456,48 +467,45 @@ int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field
                          std::vector<std::vector<uint64_t>> &bits) {
   const int levels = std::min(g.num_levels(), g.max_levels() - 1);
   bits.assign((size_t)g.num_levels(), {});
-  size_t tcap = 0, wtotal = 0;
+  if (levels <= 0) return LGD_OK;
+  if (pol.buffer > 63)
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "device regrid dilates by at most 63 cells");
+  size_t wtotal = 0;
   std::vector<size_t> wofs;
   for (int src = 0; src < levels; ++src) {
     const lgd_box d = g.domain(src);
-    tcap = std::max(tcap, (size_t)(d.hi[0] * d.hi[1]));
     wofs.push_back(wtotal);
     wtotal += (size_t)(d.hi[1] * ((d.hi[0] + 63) / 64));
   }
-  if (levels == 0) return LGD_OK;
+  // Tags (T) and dilated flags (bits) for every level, one region each.
   hipError_t e = hipSuccess;
-  if (sc.tcap < tcap) {
-    (void)hipFree(sc.T);
-    sc.T = nullptr;
-    sc.tcap = 0;
-    e = hipMalloc((void **)&sc.T, tcap);
-    if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag scratch");
-    sc.tcap = tcap;
-  }
   if (sc.wcap < wtotal) {
+    (void)hipFree(sc.T);
     (void)hipFree(sc.bits);
+    sc.T = nullptr;
     sc.bits = nullptr;
     sc.wcap = 0;
-    e = hipMalloc((void **)&sc.bits, wtotal * sizeof(uint64_t));
-    if (e != hipSuccess) return hip_fail(ctx, e, "regrid bitmask scratch");
+    e = hipMalloc((void **)&sc.T, wtotal * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&sc.bits, wtotal * sizeof(uint64_t));
+    if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag scratch");
     sc.wcap = wtotal;
   }
+  e = hipMemsetAsync(sc.T, 0, wtotal * sizeof(uint64_t), ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag clear");
   int rc;
+  const int buffer = (int)std::max<int64_t>(pol.buffer, 0);
   for (int src = 0; src < levels; ++src) {
     const lgd_box d = g.domain(src);
     const int64_t W = d.hi[0], H = d.hi[1], wpr = (W + 63) / 64;
-    e = hipMemsetAsync(sc.T, 0, (size_t)(W * H), ctx->stream);
-    if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag clear");
+    unsigned long long *T = (unsigned long long *)(sc.T + wofs[(size_t)src]);
     const lgd_level_plan &P = g.plans[(size_t)src];
     if (P.ntiles > 0) {
       tag_kernel<<<(unsigned)P.ntiles, dim3(kRTX, kRTY), 0, ctx->stream>>>(
-          P.tiles, g.d_blocks, g.d_lo, tag_field, pol.threshold, sc.T, W);
+          P.tiles, g.d_blocks, g.d_lo, tag_field, pol.threshold, T, wpr);
       if ((rc = launch_check(ctx, "regrid tag"))) return rc;
     }
-    const int64_t waves = H * wpr;
-    dilate_pack_kernel<<<(unsigned)((waves + 3) / 4), 256, 0, ctx->stream>>>(
-        sc.T, W, H, wpr, (int)std::max<int64_t>(pol.buffer, 0),
-        (unsigned long long *)(sc.bits + wofs[(size_t)src]));
+    dilate_kernel<<<blocks_for(H * wpr, 256), 256, 0, ctx->stream>>>(
+        T, W, H, wpr, buffer, (unsigned long long *)(sc.bits + wofs[(size_t)src]));
     if ((rc = launch_check(ctx, "regrid dilate"))) return rc;
   }
   std::vector<uint64_t> host(wtotal);
