@@ -172,6 +172,12 @@ int lgd_karma_rappel_flux(lgd_ctx *ctx, const lgd_grid *grid, const lgd_buf *phi
  * kphi, ku interiors from ghost-filled phi, u. */
 int lgd_modelc_rhs(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
                    const lgd_buf *phi, const lgd_buf *u, lgd_buf *kphi, lgd_buf *ku);
+/* The multi-grain vector field (KarmaRappelFlux::apply_oriented, finite_volume.rs:188-235;
+ * phasefield.rs:378-392): cos0, sin0 = cos θ₀, sin θ₀ per interior cell in ghost-0
+ * slabs (the caller takes them with libm). */
+int lgd_modelc_rhs_oriented(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                            const lgd_buf *phi, const lgd_buf *u, const lgd_buf *cos0,
+                            const lgd_buf *sin0, lgd_buf *kphi, lgd_buf *ku);
 /* eval_tendency for Driver::Wiener(0) (src/integrators/mod.rs:242-279 with
  * phasefield.rs:361-366): zero the slab, interior = noise_amplitude. */
 int lgd_modelc_amplitude(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
@@ -287,6 +293,10 @@ typedef enum {
   LGD_MODE_FUSED = 1
 } lgd_mode;
 
+/* Integrator of a simulation: EulerMaruyama / ForwardEuler (the default) or
+ * RungeKutta4 (src/integrators/rk4.rs:30-80; deterministic models only). */
+typedef enum { LGD_INTEGRATOR_EULER = 0, LGD_INTEGRATOR_RK4 = 1 } lgd_integrator;
+
 /* Simulation::new(grid, FiniteVolume{fv_tol}, ModelC, EulerMaruyama{seed}, GpuScheduler,
  * DeviceAllocator) (simulation.rs:108-183): registers phi (field 0, Time + Wiener(0)) and
  * u (field 1, Time), ghost 1, allocates state and stage buffers.  wiener = 0 selects
@@ -335,6 +345,21 @@ int64_t lgd_sim_step_index(const lgd_sim *sim);
 /* ModelC::initialize_grains with orientation 0 (phasefield.rs:227-286), computed on the
  * host (glibc tanh/hypot, as the reference) and uploaded: grains = [cx, cy, r] × n. */
 int lgd_sim_initialize_grains(lgd_sim *sim, const double *grains, size_t ngrains, double u0);
+/* Simulation::new with a RungeKutta4 integrator instead (rk4.rs:23-80): Cartesian
+ * Model C without a Wiener driver, or heat; stages k1..k4 and y_tmp are allocated
+ * here.  Bitwise the reference's operation order (whole-slab copy and axpy sweeps). */
+int lgd_sim_set_integrator(lgd_sim *sim, int32_t integrator);
+/* ModelC::with_orientations (phasefield.rs:118-125): θ₀ per interior cell in ghost-0
+ * slabs (nblocks·nx·ny values, NULL = all zero).  cos θ₀ and sin θ₀ are taken once
+ * here with libm, as the reference takes them per cell and step; the multi-grain
+ * vector field (finite_volume.rs:188-235, phasefield.rs:378-392) then runs in every
+ * step.  Cartesian Model C only. */
+int lgd_sim_set_orientation(lgd_sim *sim, const double *theta0, size_t n);
+/* ModelC::initialize_grains with orientations (phasefield.rs:227-314): grains =
+ * [cx, cy, r, θ₀] × n; φ and u as lgd_sim_initialize_grains, θ₀ by nearest-seed
+ * Voronoi, then lgd_sim_set_orientation. */
+int lgd_sim_initialize_grains_oriented(lgd_sim *sim, const double *grains, size_t ngrains,
+                                       double u0);
 
 /* -------------------------------------------------------------- Model C adjoint */
 /* The reverse-mode derivative of the explicit Model C step (Euler–Maruyama with

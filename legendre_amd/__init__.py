@@ -20,6 +20,8 @@ from ._abi import (
     LGD_BC_EXTRAPOLATE,
     LGD_BC_FLUX,
     LGD_BC_MIRROR,
+    LGD_INTEGRATOR_EULER,
+    LGD_INTEGRATOR_RK4,
     LGD_MODE_FUSED,
     LGD_MODE_PER_INTENT,
     LegendreError,
@@ -39,6 +41,8 @@ __all__ = [
     "time_bits",
     "LGD_MODE_FUSED",
     "LGD_MODE_PER_INTENT",
+    "LGD_INTEGRATOR_EULER",
+    "LGD_INTEGRATOR_RK4",
 ]
 
 # Karma–Rappel thin-interface constant a₂ (src/physics/phasefield.rs:44).
@@ -306,6 +310,16 @@ class CartesianGrid:
             self.ctx.handle,
         )
 
+    def modelc_rhs_oriented(self, params: "ModelCParams", phi, u, cos0, sin0, kphi, ku) -> None:
+        """The multi-grain vector field; cos0, sin0 = cos θ₀, sin θ₀ in ghost-0 slabs."""
+        p = params.c()
+        _abi.check(
+            self._lib.lgd_modelc_rhs_oriented(self.ctx.handle, self.handle, C.byref(p), phi.handle,
+                                              u.handle, cos0.handle, sin0.handle, kphi.handle,
+                                              ku.handle),
+            self.ctx.handle,
+        )
+
     def modelc_amplitude(self, params: "ModelCParams", amp: DeviceBuffer) -> None:
         p = params.c()
         _abi.check(
@@ -426,6 +440,27 @@ class Simulation:
         _abi.check(self._lib.lgd_sim_regrid_stats(self.handle, C.byref(n), sec.ctypes.data_as(_abi.DP)),
                    self.ctx.handle)
         return int(n.value), sec
+
+    def set_integrator(self, name: str) -> None:
+        """'euler' (EulerMaruyama / ForwardEuler) or 'rk4' (RungeKutta4, deterministic
+        models only)."""
+        code = {"euler": LGD_INTEGRATOR_EULER, "rk4": LGD_INTEGRATOR_RK4}[name]
+        _abi.check(self._lib.lgd_sim_set_integrator(self.handle, code), self.ctx.handle)
+
+    def set_orientation(self, theta0: Optional[np.ndarray]) -> None:
+        """Multi-grain Model C: θ₀ per interior cell (ghost-0 slabs; None = 0)."""
+        if theta0 is None:
+            _abi.check(self._lib.lgd_sim_set_orientation(self.handle, None, 0), self.ctx.handle)
+            return
+        t = np.ascontiguousarray(theta0, dtype=np.float64)
+        _abi.check(self._lib.lgd_sim_set_orientation(self.handle, t.ctypes.data_as(_abi.DP), t.size),
+                   self.ctx.handle)
+
+    def initialize_grains_oriented(self, grains: np.ndarray, u0: float) -> None:
+        """Grains [cx, cy, r, θ₀]: φ, u as initialize_grains, θ₀ by nearest seed."""
+        g = np.ascontiguousarray(grains, dtype=np.float64).reshape(-1, 4)
+        _abi.check(self._lib.lgd_sim_initialize_grains_oriented(self.handle, g.ctypes.data_as(_abi.DP),
+                                                                g.shape[0], u0), self.ctx.handle)
 
     def set_graph_steps(self, steps: int) -> None:
         """Run `steps` whole steps at a time as one replayed HIP graph (0: eager

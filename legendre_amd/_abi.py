@@ -25,6 +25,7 @@ STATUS_NAMES = {
 
 LGD_BC_MIRROR, LGD_BC_DIRICHLET, LGD_BC_FLUX, LGD_BC_EXTRAPOLATE = 0, 1, 2, 3
 LGD_MODE_PER_INTENT, LGD_MODE_FUSED = 0, 1
+LGD_INTEGRATOR_EULER, LGD_INTEGRATOR_RK4 = 0, 1
 
 
 class CartesianDesc(C.Structure):
@@ -120,6 +121,7 @@ SIGNATURES = {
     "lgd_central_laplacian": (C.c_int, [P, P, P, P, U32]),
     "lgd_karma_rappel_flux": (C.c_int, [P, P, P, P, DBL, DBL]),
     "lgd_modelc_rhs": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P, P, P, P]),
+    "lgd_modelc_rhs_oriented": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P, P, P, P, P, P]),
     "lgd_modelc_amplitude": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P]),
     "lgd_heat_rhs": (C.c_int, [P, P, DBL, P, P]),
     "lgd_axpy": (C.c_int, [P, P, P, DBL]),
@@ -158,6 +160,9 @@ SIGNATURES = {
     "lgd_sim_time": (DBL, [P]),
     "lgd_sim_step_index": (I64, [P]),
     "lgd_sim_initialize_grains": (C.c_int, [P, DP, SZ, DBL]),
+    "lgd_sim_set_integrator": (C.c_int, [P, C.c_int32]),
+    "lgd_sim_set_orientation": (C.c_int, [P, DP, SZ]),
+    "lgd_sim_initialize_grains_oriented": (C.c_int, [P, DP, SZ, DBL]),
     "lgd_adjoint_create": (C.c_int, [P, P, PP]),
     "lgd_adjoint_destroy": (C.c_int, [P]),
     "lgd_modelc_adjoint_step": (C.c_int, [P, C.POINTER(ModelCParamsC), P, P, DBL, P, P]),

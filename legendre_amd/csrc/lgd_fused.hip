@@ -9,6 +9,8 @@
 // the result goes to a second buffer pair (neighbours still read the old values).
 #include <cmath>
 
+#include <type_traits>
+
 #include "lgd_internal.h"
 
 namespace {
@@ -41,11 +43,13 @@ constexpr int kTY = 4;
 
 // Model C Euler–Maruyama step, one thread per interior cell (2-D).
 // Workgroups tile one block at a time: blockIdx.x = block * tiles + tile.
-template <bool kNoise>
+// kOriented: the multi-grain vector field with (cos θ₀, sin θ₀) per cell in ghost-0
+// slabs (cs0, sn0).
+template <bool kNoise, bool kOriented>
 __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
     lgd_cart c, lgd_kr_consts k, int64_t tiles_x, int64_t tiles_per_block, const double *phi,
     const double *u, double *phi_n, double *u_n, double dt, double noise_scale, uint64_t seed,
-    uint64_t salt, const double *salt_src, uint64_t stream) {
+    uint64_t salt, const double *salt_src, uint64_t stream, const double *cs0, const double *sn0) {
   int64_t b = blockIdx.x / tiles_per_block;
   int64_t tile = blockIdx.x - b * tiles_per_block;
   int64_t ty = tile / tiles_x, tx = tile - ty * tiles_x;
@@ -70,7 +74,12 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
   double un = u[cell_offset(c, gi, resolve_coord(gj + 1, NY, c.periodic[1]), 0)];
   double us = u[cell_offset(c, gi, resolve_coord(gj - 1, NY, c.periodic[1]), 0)];
   double dphi, du;
-  lgd_modelc_cell(k, p, uc, ue, uw, un, us, dphi, du);
+  if (kOriented) {
+    const int64_t ot = b * c.n[0] * c.n[1] + j * c.n[0] + i;
+    lgd_modelc_cell_oriented(k, p, uc, ue, uw, un, us, cs0[ot], sn0[ot], dphi, du);
+  } else {
+    lgd_modelc_cell(k, p, uc, ue, uw, un, us, dphi, du);
+  }
   // apply_step_with: Time first (x += dt*k), then Wiener(0) on phi
   // (src/core/state.rs:656-699, src/core/driver.rs:131-158).
   double pn = p[1][1] + dt * dphi;
@@ -119,11 +128,10 @@ __global__ void heat_step_kernel(lgd_cart c, double ih0, double ih1, double ih2,
 
 using namespace lgd;
 
-extern "C" {
-
-int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
-                    const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
-                    double dt, uint64_t seed, uint64_t salt, int32_t wiener) {
+int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                         const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
+                         double dt, uint64_t seed, uint64_t salt, int32_t wiener,
+                         const double *cs0, const double *sn0) {
   if (!ctx || !grid || !params) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
   const lgd_cart &c = grid->cart;
   if (c.dim != 2) return fail(ctx, LGD_ERR_UNSUPPORTED, "Model C is 2-D");
@@ -143,7 +151,8 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
   const double noise_scale = std::sqrt(dt) * params->noise_amplitude;
   const uint64_t wiener0 = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
   const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, noise_on, 4);
-  if (shape.kernel != 1) {
+  const bool oriented = cs0 != nullptr;
+  if (shape.kernel != 1 && !oriented) {
     lgd_domain d0;
     d0.k = k;
     d0.noise_scale = noise_on ? noise_scale : 0.0;
@@ -166,16 +175,35 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
   int64_t nwg = c.nblocks * tiles;
   if (nwg > (int64_t)INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "grid too large for one launch");
   dim3 wg(kTX, kTY);
-  if (noise_on)
-    modelc_step_kernel<true><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
-        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, noise_scale,
-        seed, salt, ctx->salt_src, wiener0);
+  auto go = [&](auto noise_t, auto orient_t) {
+    modelc_step_kernel<decltype(noise_t)::value, decltype(orient_t)::value>
+        <<<(unsigned)nwg, wg, 0, ctx->stream>>>(c, k, tiles_x, tiles, phi->data, u->data,
+                                                phi_next->data, u_next->data, dt, noise_scale,
+                                                seed, salt, noise_on ? ctx->salt_src : nullptr,
+                                                wiener0, cs0, sn0);
+  };
+  using T = std::true_type;
+  using F = std::false_type;
+  if (noise_on && oriented)
+    go(T{}, T{});
+  else if (noise_on)
+    go(T{}, F{});
+  else if (oriented)
+    go(F{}, T{});
   else
-    modelc_step_kernel<false><<<(unsigned)nwg, wg, 0, ctx->stream>>>(
-        c, k, tiles_x, tiles, phi->data, u->data, phi_next->data, u_next->data, dt, noise_scale,
-        seed, salt, nullptr, wiener0);
+    go(F{}, F{});
   return launch_check(ctx, "modelc_step");
 }
+
+extern "C" {
+
+int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                    const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
+                    double dt, uint64_t seed, uint64_t salt, int32_t wiener) {
+  return lgd_modelc_step_impl(ctx, grid, params, phi, u, phi_next, u_next, dt, seed, salt, wiener,
+                              nullptr, nullptr);
+}
+
 
 int lgd_heat_step(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_buf *u,
                   lgd_buf *u_next, double dt) {

@@ -164,6 +164,13 @@ struct lgd_domain {
   int32_t nbx, nby, per_x, per_y;
 };
 
+// The fused Model C step (lgd_modelc_step) with an optional orientation: cs0, sn0 =
+// (cos θ₀, sin θ₀) per cell in ghost-0 slabs (null: axis-aligned).
+int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                         const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
+                         double dt, uint64_t seed, uint64_t salt, int32_t wiener,
+                         const double *cs0, const double *sn0);
+
 // Streaming Model C step (lgd_modelc_stream.hip) over blocks
 // [block_begin, block_begin + nblocks); LGD_ERR_UNSUPPORTED when the block is
 // wider than 126 interior cells.
@@ -211,6 +218,63 @@ __device__ __forceinline__ double lgd_center_anisotropy(const lgd_kr_consts &k, 
   double mag2 = sum * sum;
   if (mag2 <= k.model_tol) return k.a_bar;
   return k.a_bar * (1.0 + (k.eps_prime * (dx2 * dx2 + dy2 * dy2)) / mag2);
+}
+
+// Multi-grain Model C at one interior cell (KarmaRappelFlux::apply_oriented,
+// finite_volume.rs:188-235, and the oriented branch of ModelC::vector_field_block,
+// phasefield.rs:378-392): every face's gradient, and the central difference, are
+// rotated into the crystal frame by θ₀ of this cell — given as (c0, s0) = (cos θ₀,
+// sin θ₀), computed once per cell on the host with libm as the reference computes
+// them per step — for (A, A′); the fluxes keep the lab-frame components; the centre
+// anisotropy is the flux's face() (FV tolerance) of the rotated difference.
+__device__ __forceinline__ void lgd_rotate(double c0, double s0, double dx, double dy,
+                                           double &rx, double &ry) {
+  rx = c0 * dx + s0 * dy;
+  ry = c0 * dy - s0 * dx;
+}
+
+__device__ __forceinline__ void lgd_modelc_cell_oriented(const lgd_kr_consts &k,
+                                                         const double p[3][3], double uc,
+                                                         double ue, double uw, double un,
+                                                         double us, double c0, double s0,
+                                                         double &dphi, double &du) {
+  const double q = 0.25;
+  double derx_r = p[1][2] - p[1][1];
+  double derx_l = p[1][1] - p[1][0];
+  double derx_t = q * (((p[2][2] - p[2][0]) + p[1][2]) - p[1][0]);
+  double derx_b = q * (((p[1][2] - p[1][0]) + p[0][2]) - p[0][0]);
+  double dery_t = p[2][1] - p[1][1];
+  double dery_b = p[1][1] - p[0][1];
+  double dery_r = q * (((p[2][2] - p[0][2]) + p[2][1]) - p[0][1]);
+  double dery_l = q * (((p[2][1] - p[0][1]) + p[2][0]) - p[0][0]);
+  double rx, ry, a_r, da_r, a_l, da_l, a_t, da_t, a_b, da_b;
+  lgd_rotate(c0, s0, derx_r, dery_r, rx, ry);
+  lgd_kr_face(k, rx, ry, a_r, da_r);
+  lgd_rotate(c0, s0, derx_l, dery_l, rx, ry);
+  lgd_kr_face(k, rx, ry, a_l, da_l);
+  lgd_rotate(c0, s0, derx_t, dery_t, rx, ry);
+  lgd_kr_face(k, rx, ry, a_t, da_t);
+  lgd_rotate(c0, s0, derx_b, dery_b, rx, ry);
+  lgd_kr_face(k, rx, ry, a_b, da_b);
+  double j_r = a_r * ((a_r * derx_r) - (da_r * dery_r));
+  double j_l = a_l * ((a_l * derx_l) - (da_l * dery_l));
+  double j_t = a_t * ((a_t * dery_t) + (da_t * derx_t));
+  double j_b = a_b * ((a_b * dery_b) + (da_b * derx_b));
+  double flux = (((j_r - j_l) + j_t) - j_b) * k.inv_h2;
+  double pc = p[1][1];
+  double g_prime = fma(pc * pc, pc, -pc);
+  double one_m_p2 = fma(pc, -pc, 1.0);
+  double p_prime = one_m_p2 * one_m_p2;
+  double cdx = 0.5 * (p[1][2] - p[1][0]);
+  double cdy = 0.5 * (p[2][1] - p[0][1]);
+  double crx, cry, a, da;
+  lgd_rotate(c0, s0, cdx, cdy, crx, cry);
+  lgd_kr_face(k, crx, cry, a, da);
+  dphi = fma(k.lambda * uc, -p_prime, flux - g_prime) / (a * a);
+  double acc = 0.0;
+  acc = acc + (((ue - (uc + uc)) + uw) * k.inv_hx2);
+  acc = acc + (((un - (uc + uc)) + us) * k.inv_hy2);
+  du = fma(acc, k.d_thermal, 0.5 * dphi);
 }
 
 // Model C right-hand side at one interior cell from its 3x3 phi neighbourhood

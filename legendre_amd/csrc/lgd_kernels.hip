@@ -182,8 +182,10 @@ __global__ void kr_flux_kernel(lgd_cart c, lgd_kr_consts k, const double *phi, d
 
 // ModelC::vector_field_block(Time) (src/physics/phasefield.rs:367-421): each
 // thread evaluates one interior cell's dφ and du from ghost-filled slabs.
+// cs0 / sn0 non-null: the multi-grain field, (cos θ₀, sin θ₀) per cell (ghost-0).
 __global__ void modelc_rhs_kernel(lgd_cart c, lgd_kr_consts k, const double *phi,
-                                  const double *u, double *kphi, double *ku) {
+                                  const double *u, double *kphi, double *ku, const double *cs0,
+                                  const double *sn0) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t ncell = c.n[0] * c.n[1];
   if (t >= c.nblocks * ncell) return;
@@ -196,7 +198,11 @@ __global__ void modelc_rhs_kernel(lgd_cart c, lgd_kr_consts k, const double *phi
   for (int dj = -1; dj <= 1; ++dj)
     for (int di = -1; di <= 1; ++di) p[dj + 1][di + 1] = phi[o + dj * sx + di];
   double dphi, du;
-  lgd_modelc_cell(k, p, u[o], u[o + 1], u[o - 1], u[o + sx], u[o - sx], dphi, du);
+  if (cs0)
+    lgd_modelc_cell_oriented(k, p, u[o], u[o + 1], u[o - 1], u[o + sx], u[o - sx], cs0[t], sn0[t],
+                             dphi, du);
+  else
+    lgd_modelc_cell(k, p, u[o], u[o + 1], u[o - 1], u[o + sx], u[o - sx], dphi, du);
   kphi[o] = dphi;
   ku[o] = du;
 }
@@ -364,8 +370,27 @@ int lgd_modelc_rhs(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *
   lgd_kr_consts k = lgd_make_kr_consts(*params, grid->cart);
   int64_t total = grid->cart.nblocks * grid->cart.n[0] * grid->cart.n[1];
   modelc_rhs_kernel<<<blocks_for(total, kWG), kWG, 0, ctx->stream>>>(
-      grid->cart, k, phi->data, u->data, kphi->data, ku->data);
+      grid->cart, k, phi->data, u->data, kphi->data, ku->data, nullptr, nullptr);
   return launch_check(ctx, "modelc_rhs");
+}
+
+int lgd_modelc_rhs_oriented(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                            const lgd_buf *phi, const lgd_buf *u, const lgd_buf *cos0,
+                            const lgd_buf *sin0, lgd_buf *kphi, lgd_buf *ku) {
+  if (!ctx || !grid || !params) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  if (grid->cart.dim != 2) return fail(ctx, LGD_ERR_UNSUPPORTED, "Model C is 2-D");
+  int rc;
+  if ((rc = check_field(ctx, grid, phi, 1, "phi"))) return rc;
+  if ((rc = check_field(ctx, grid, u, 1, "u"))) return rc;
+  if ((rc = check_field(ctx, grid, cos0, 0, "cos0"))) return rc;
+  if ((rc = check_field(ctx, grid, sin0, 0, "sin0"))) return rc;
+  if ((rc = check_field(ctx, grid, kphi, 1, "kphi"))) return rc;
+  if ((rc = check_field(ctx, grid, ku, 1, "ku"))) return rc;
+  lgd_kr_consts k = lgd_make_kr_consts(*params, grid->cart);
+  int64_t total = grid->cart.nblocks * grid->cart.n[0] * grid->cart.n[1];
+  modelc_rhs_kernel<<<blocks_for(total, kWG), kWG, 0, ctx->stream>>>(
+      grid->cart, k, phi->data, u->data, kphi->data, ku->data, cos0->data, sin0->data);
+  return launch_check(ctx, "modelc_rhs_oriented");
 }
 
 int lgd_modelc_amplitude(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,

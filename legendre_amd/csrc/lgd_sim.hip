@@ -19,6 +19,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <new>
 #include <vector>
 
@@ -45,6 +46,11 @@ struct lgd_sim {
   bool adaptive = false;
   lgd_regrid_policy policy{};
   std::vector<int8_t> cur;       // AMR: per level, whether next[] holds its current values
+  // Integrator (LGD_INTEGRATOR_*); RK4 stages: k1..k4 then y_tmp, one buffer per field each.
+  int32_t integrator = LGD_INTEGRATOR_EULER;
+  std::vector<lgd_buf *> rk;
+  // Multi-grain Model C: θ₀ per cell (ghost-0 slabs) as cos θ₀ and sin θ₀.
+  lgd_buf *cos0 = nullptr, *sin0 = nullptr;
   double t = 0.0;
   int64_t step_index = 0;
   // Graphs.
@@ -113,6 +119,11 @@ void free_stages(lgd_sim *s) {
 void free_all(lgd_sim *s) {
   for (auto *b : s->state) lgd_buf_free(b);
   s->state.clear();
+  for (auto *b : s->rk) lgd_buf_free(b);
+  s->rk.clear();
+  lgd_buf_free(s->cos0);
+  lgd_buf_free(s->sin0);
+  s->cos0 = s->sin0 = nullptr;
   free_stages(s);
 }
 
@@ -146,21 +157,74 @@ bool all_mirror(const lgd_sim *s) {
   return true;
 }
 
+// eval_drift of Model C: ModelC::fill_ghosts, then the per-block Time vector field
+// (axis-aligned or multi-grain) of (phi, u) into (kphi, ku).
+int modelc_eval(lgd_sim *s, lgd_buf *phi, lgd_buf *u, lgd_buf *kphi, lgd_buf *ku) {
+  lgd_ctx *ctx = s->ctx;
+  int rc;
+  if ((rc = lgd_fill_ghosts(ctx, s->grid, phi, 1, nullptr))) return rc;
+  if ((rc = lgd_fill_ghosts(ctx, s->grid, u, 1, nullptr))) return rc;
+  if (s->cos0) return lgd_modelc_rhs_oriented(ctx, s->grid, &s->params, phi, u, s->cos0, s->sin0, kphi, ku);
+  return lgd_modelc_rhs(ctx, s->grid, &s->params, phi, u, kphi, ku);
+}
+
+int heat_eval(lgd_sim *s, lgd_buf *u, lgd_buf *k) {
+  int rc;
+  if ((rc = lgd_fill_ghosts(s->ctx, s->grid, u, 1, s->has_bcs ? s->bcs : nullptr))) return rc;
+  return lgd_heat_rhs(s->ctx, s->grid, s->kappa, u, k);
+}
+
+int copy_buf(lgd_ctx *ctx, const lgd_buf *src, lgd_buf *dst) {
+  copy_kernel<<<blocks_for((int64_t)src->len, 256), 256, 0, ctx->stream>>>((int64_t)src->len,
+                                                                         src->data, dst->data);
+  return launch_check(ctx, "copy");
+}
+
+// RungeKutta4::step (src/integrators/rk4.rs:30-80): k1 = f(y); y_tmp = y + dt/2·k1,
+// k2 = f(y_tmp); y_tmp = y + dt/2·k2, k3; y_tmp = y + dt·k3, k4; then
+// y += dt/6·k1, += dt/3·k2, += dt/3·k3, += dt/6·k4, each a whole-slab sweep.
+int step_rk4(lgd_sim *s, double dt) {
+  const int nf = (int)s->state.size();
+  auto k = [&](int stage, int f) { return s->rk[(size_t)(stage * nf + f)]; };
+  auto y = [&](int f) { return s->rk[(size_t)(4 * nf + f)]; };
+  auto eval = [&](bool from_state, int stage) {
+    std::vector<lgd_buf *> in(nf);
+    for (int f = 0; f < nf; ++f) in[f] = from_state ? s->state[f] : y(f);
+    return s->kind == lgd_sim::kModelC ? modelc_eval(s, in[0], in[1], k(stage, 0), k(stage, 1))
+                                       : heat_eval(s, in[0], k(stage, 0));
+  };
+  int rc;
+  if ((rc = eval(true, 0))) return rc;
+  const double half_dt = 0.5 * dt;
+  const double c[3] = {half_dt, half_dt, dt};
+  for (int st = 1; st < 4; ++st) {
+    for (int f = 0; f < nf; ++f) {
+      if ((rc = copy_buf(s->ctx, s->state[f], y(f)))) return rc;
+      if ((rc = lgd_axpy(s->ctx, y(f), k(st - 1, f), c[st - 1]))) return rc;
+    }
+    if ((rc = eval(false, st))) return rc;
+  }
+  const double sixth = dt / 6.0, third = dt / 3.0;
+  const double w[4] = {sixth, third, third, sixth};
+  for (int st = 0; st < 4; ++st)
+    for (int f = 0; f < nf; ++f)
+      if ((rc = lgd_axpy(s->ctx, s->state[f], k(st, f), w[st]))) return rc;
+  return LGD_OK;
+}
+
 int step_modelc(lgd_sim *s, double dt, uint64_t salt) {
   lgd_ctx *ctx = s->ctx;
   int rc;
+  if (s->integrator == LGD_INTEGRATOR_RK4) return step_rk4(s, dt);
   if (s->mode == LGD_MODE_FUSED) {
     ctx->salt_src = time_slot(s, 0);
-    rc = lgd_modelc_step(ctx, s->grid, &s->params, s->state[0], s->state[1], s->next[0],
-                         s->next[1], dt, s->seed, salt, s->wiener);
+    rc = lgd_modelc_step_impl(ctx, s->grid, &s->params, s->state[0], s->state[1], s->next[0],
+                              s->next[1], dt, s->seed, salt, s->wiener,
+                              s->cos0 ? s->cos0->data : nullptr, s->sin0 ? s->sin0->data : nullptr);
     ctx->salt_src = nullptr;
     return rc;  // step_done swaps the ping-pong pair
   }
-  // eval_drift: ModelC::fill_ghosts then the per-block Time vector field.
-  if ((rc = lgd_fill_ghosts(ctx, s->grid, s->state[0], 1, nullptr))) return rc;
-  if ((rc = lgd_fill_ghosts(ctx, s->grid, s->state[1], 1, nullptr))) return rc;
-  if ((rc = lgd_modelc_rhs(ctx, s->grid, &s->params, s->state[0], s->state[1], s->k[0], s->k[1])))
-    return rc;
+  if ((rc = modelc_eval(s, s->state[0], s->state[1], s->k[0], s->k[1]))) return rc;
   if (s->wiener && (rc = lgd_modelc_amplitude(ctx, s->grid, &s->params, s->amp))) return rc;
   // apply_step_with: Time on phi and u, then Wiener(0) on phi.
   if ((rc = lgd_axpy(ctx, s->state[0], s->k[0], dt))) return rc;
@@ -177,6 +241,7 @@ int step_modelc(lgd_sim *s, double dt, uint64_t salt) {
 
 int step_heat(lgd_sim *s, double dt) {
   int rc;
+  if (s->integrator == LGD_INTEGRATOR_RK4) return step_rk4(s, dt);
   if (s->mode == LGD_MODE_FUSED) {
     return lgd_heat_step(s->ctx, s->grid, s->kappa, s->state[0], s->next[0], dt);
   }
@@ -487,7 +552,8 @@ bool regrid_due(const lgd_sim *s) {
 // Host-side effects of one step: the fused ping-pong swap and the time accumulation
 // (Simulation::step, src/core/simulation.rs:281-289).
 void step_done(lgd_sim *s, double dt) {
-  if (!s->amr && s->mode == LGD_MODE_FUSED) std::swap(s->state, s->next);
+  if (!s->amr && s->mode == LGD_MODE_FUSED && s->integrator == LGD_INTEGRATOR_EULER)
+    std::swap(s->state, s->next);
   s->t = s->t + dt;
   s->step_index += 1;
 }
@@ -820,6 +886,111 @@ int lgd_sim_initialize_grains(lgd_sim *sim, const double *grains, size_t ngrains
   int rc = lgd_buf_upload(sim->ctx, sim->state[0], 0, phi.data(), phi.size());
   if (rc) return rc;
   return lgd_buf_upload(sim->ctx, sim->state[1], 0, u.data(), u.size());
+}
+
+int lgd_sim_set_integrator(lgd_sim *sim, int32_t integrator) {
+  if (!sim) return LGD_ERR_INVALID_ARGUMENT;
+  if (integrator != LGD_INTEGRATOR_EULER && integrator != LGD_INTEGRATOR_RK4)
+    return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT, "unknown integrator");
+  if (integrator == sim->integrator) return LGD_OK;
+  if (integrator == LGD_INTEGRATOR_RK4) {
+    // RungeKutta4 implements Integrator<G, D, NoNoise> only (rk4.rs:23).
+    if (sim->amr) return fail(sim->ctx, LGD_ERR_UNSUPPORTED, "RK4 runs on Cartesian simulations");
+    if (sim->kind == lgd_sim::kModelC && sim->wiener)
+      return fail(sim->ctx, LGD_ERR_UNSUPPORTED, "RK4 pairs with deterministic models (no Wiener driver)");
+    const size_t len = sim->state[0]->len;
+    while (sim->rk.size() < 5 * sim->state.size()) {
+      lgd_buf *b = nullptr;
+      int rc = lgd_buf_alloc(sim->ctx, len, &b);
+      if (rc) return rc;
+      sim->rk.push_back(b);
+    }
+    // Fused ping-pong state is not used by RK4: keep the current values in state[].
+  }
+  drop_graphs(sim);
+  sim->integrator = integrator;
+  return LGD_OK;
+}
+
+int lgd_sim_set_orientation(lgd_sim *sim, const double *theta0, size_t n) {
+  if (!sim) return LGD_ERR_INVALID_ARGUMENT;
+  if (sim->kind != lgd_sim::kModelC || sim->amr)
+    return fail(sim->ctx, LGD_ERR_UNSUPPORTED, "orientations are a Cartesian Model C feature");
+  const lgd_cart &c = sim->grid->cart;
+  const size_t len = (size_t)(c.nblocks * c.n[0] * c.n[1]);
+  if (theta0 && n != len) return fail(sim->ctx, LGD_ERR_SIZE_MISMATCH, "theta0 needs nblocks·nx·ny values");
+  int rc;
+  if (!sim->cos0) {
+    if ((rc = lgd_buf_alloc(sim->ctx, len, &sim->cos0))) return rc;
+    if ((rc = lgd_buf_alloc(sim->ctx, len, &sim->sin0))) return rc;
+  }
+  // (sin θ₀, cos θ₀) with libm, as the reference's per-cell sin_cos (phasefield.rs:382).
+  std::vector<double> cs(len), sn(len);
+  for (size_t i = 0; i < len; ++i) {
+    const double th = theta0 ? theta0[i] : 0.0;
+    sn[i] = std::sin(th);
+    cs[i] = std::cos(th);
+  }
+  if ((rc = lgd_buf_upload(sim->ctx, sim->cos0, 0, cs.data(), len))) return rc;
+  if ((rc = lgd_buf_upload(sim->ctx, sim->sin0, 0, sn.data(), len))) return rc;
+  drop_graphs(sim);
+  return LGD_OK;
+}
+
+int lgd_sim_initialize_grains_oriented(lgd_sim *sim, const double *grains, size_t ngrains,
+                                       double u0) {
+  if (!sim || (!grains && ngrains)) return LGD_ERR_INVALID_ARGUMENT;
+  std::vector<double> g3(3 * ngrains);
+  for (size_t g = 0; g < ngrains; ++g)
+    for (int k = 0; k < 3; ++k) g3[3 * g + k] = grains[4 * g + k];
+  int rc = lgd_sim_initialize_grains(sim, g3.data(), ngrains, u0);
+  if (rc) return rc;
+  // θ₀ by nearest-seed Voronoi (phasefield.rs:286-314): per block, the grains whose
+  // closest approach beats every grain's farthest corner; the first nearest wins.
+  const lgd_cart &c = sim->grid->cart;
+  const int64_t nx = c.n[0], ny = c.n[1];
+  std::vector<double> theta((size_t)(c.nblocks * nx * ny), 0.0);
+  std::vector<size_t> cand;
+  for (int64_t b = 0; b < c.nblocks; ++b) {
+    double c0[2], c1[2];
+    cell_center2(c, b, 0, 0, c0);
+    cell_center2(c, b, nx - 1, ny - 1, c1);
+    const double lo[2] = {std::fma(0.5, -c.h[0], c0[0]), std::fma(0.5, -c.h[1], c0[1])};
+    const double hi[2] = {std::fma(0.5, c.h[0], c1[0]), std::fma(0.5, c.h[1], c1[1])};
+    double threshold = std::numeric_limits<double>::max();
+    for (size_t g = 0; g < ngrains; ++g) {
+      const double cx = grains[4 * g], cy = grains[4 * g + 1];
+      const double fx = std::fmax(std::fabs(cx - lo[0]), std::fabs(cx - hi[0]));
+      const double fy = std::fmax(std::fabs(cy - lo[1]), std::fabs(cy - hi[1]));
+      threshold = std::fmin(threshold, std::hypot(fx, fy));
+    }
+    cand.clear();
+    for (size_t g = 0; g < ngrains; ++g) {
+      const double cx = grains[4 * g], cy = grains[4 * g + 1];
+      const double dx = std::fmax(std::fmax(lo[0] - cx, cx - hi[0]), 0.0);
+      const double dy = std::fmax(std::fmax(lo[1] - cy, cy - hi[1]), 0.0);
+      if (std::hypot(dx, dy) <= threshold) cand.push_back(g);
+    }
+    double *tb = theta.data() + b * nx * ny;
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        double xy[2];
+        cell_center2(c, b, i, j, xy);
+        double best = 0.0, bestd = 0.0;
+        bool have = false;
+        for (size_t g : cand) {
+          const double dx = xy[0] - grains[4 * g], dy = xy[1] - grains[4 * g + 1];
+          const double d = std::fma(dy, dy, dx * dx);
+          if (!have || d < bestd) {
+            have = true;
+            bestd = d;
+            best = grains[4 * g + 3];
+          }
+        }
+        tb[j * nx + i] = best;
+      }
+  }
+  return lgd_sim_set_orientation(sim, theta.data(), theta.size());
 }
 
 }  // extern "C"

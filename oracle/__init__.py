@@ -79,6 +79,13 @@ _SIGS = {
     "orc_heat_run": (C.c_int, [GP, C.c_double, DP, C.c_int64, C.c_double, DP, C.POINTER(FaceBc), C.c_int32]),
     "orc_initialize_grains": (None, [GP, DP, DP, DP, C.c_size_t, C.c_double]),
     "orc_seed_positions": (None, [C.c_uint64, C.c_size_t, C.c_double, C.c_double, DP]),
+    "orc_modelc_rhs_oriented": (None, [GP, C.POINTER(ModelCParams), DP, DP, DP, DP, DP]),
+    "orc_modelc_oriented_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, DP, C.c_int64,
+                                          C.c_double, DP]),
+    "orc_modelc_rk4_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, DP, C.c_int64, C.c_double,
+                                     DP]),
+    "orc_heat_rk4_run": (C.c_int, [GP, C.c_double, DP, C.c_int64, C.c_double, DP, C.POINTER(FaceBc)]),
+    "orc_initialize_orientations": (None, [GP, DP, DP, C.c_size_t]),
     "orc_modelc_tlm_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_int32, DP, DP, DP,
                                      C.c_int64, C.c_double]),
     "orc_modelc_adjoint_step": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_double, DP, DP,
@@ -249,6 +256,48 @@ def heat_run(g: Grid, kappa, u, nsteps, dt, t0=0.0, bcs=None, nthreads=1) -> flo
 def initialize_grains(g: Grid, phi, u, grains: np.ndarray, u0: float) -> None:
     gr = np.ascontiguousarray(grains, dtype=np.float64).reshape(-1, 3)
     lib().orc_initialize_grains(C.byref(g), _dp(phi), _dp(u), _dp(gr), gr.shape[0], u0)
+
+
+def _dp_or_null(a):
+    return None if a is None else _dp(a)
+
+
+def modelc_rhs_oriented(g: Grid, p: ModelCParams, phi, u, theta, kphi, ku) -> None:
+    """Multi-grain vector field (finite_volume.rs:188-235, phasefield.rs:378-392);
+    theta in ghost-0 slabs."""
+    lib().orc_modelc_rhs_oriented(C.byref(g), C.byref(p), _dp(phi), _dp(u), _dp(theta), _dp(kphi),
+                                  _dp(ku))
+
+
+def modelc_oriented_run(g: Grid, p: ModelCParams, phi, u, theta, nsteps, dt, t0=0.0) -> float:
+    t = C.c_double(t0)
+    if lib().orc_modelc_oriented_run(C.byref(g), C.byref(p), _dp(phi), _dp(u), _dp(theta), nsteps,
+                                     dt, C.byref(t)) != 0:
+        raise ValueError("orc_modelc_oriented_run rejected the configuration")
+    return t.value
+
+
+def modelc_rk4_run(g: Grid, p: ModelCParams, phi, u, nsteps, dt, theta=None, t0=0.0) -> float:
+    """RungeKutta4 (src/integrators/rk4.rs) on Model C; theta (ghost-0 slabs) for the
+    multi-grain model."""
+    t = C.c_double(t0)
+    if lib().orc_modelc_rk4_run(C.byref(g), C.byref(p), _dp(phi), _dp(u), _dp_or_null(theta),
+                                nsteps, dt, C.byref(t)) != 0:
+        raise ValueError("orc_modelc_rk4_run rejected the configuration")
+    return t.value
+
+
+def heat_rk4_run(g: Grid, kappa, u, nsteps, dt, t0=0.0, bcs=None) -> float:
+    t = C.c_double(t0)
+    if lib().orc_heat_rk4_run(C.byref(g), kappa, _dp(u), nsteps, dt, C.byref(t), bcs_array(bcs)) != 0:
+        raise ValueError("orc_heat_rk4_run rejected the configuration")
+    return t.value
+
+
+def initialize_orientations(g: Grid, theta, grains: np.ndarray) -> None:
+    """The nearest-seed orientation pass of initialize_grains; grains [cx, cy, r, θ₀]."""
+    gr = np.ascontiguousarray(grains, dtype=np.float64).reshape(-1, 4)
+    lib().orc_initialize_orientations(C.byref(g), _dp(theta), _dp(gr), gr.shape[0])
 
 
 def seed_positions(seed: int, nseeds: int, extent: float, radius: float) -> np.ndarray:

@@ -578,6 +578,254 @@ void orc_seed_positions(uint64_t seed, size_t nseeds, double extent, double radi
 }  // extern "C"
 
 // ===========================================================================
+// Multi-grain orientations and the classic fourth-order Runge–Kutta.
+// ===========================================================================
+namespace {
+
+// rotate (src/discretization/finite_volume.rs:241-243): the crystal-frame
+// components (c0·dx + s0·dy, c0·dy − s0·dx).
+inline void rotate(double c0, double s0, double dx, double dy, double &rx, double &ry) {
+  rx = c0 * dx + s0 * dy;
+  ry = c0 * dy - s0 * dx;
+}
+
+// KarmaRappelFlux::apply_oriented (finite_volume.rs:188-235) and the multi-grain
+// branch of ModelC::vector_field_block (phasefield.rs:378-392) on one block:
+// θ₀ at the cell (ghost-0 slab) rotates every face's gradient into the crystal
+// frame for (A, A′) while the fluxes keep the lab-frame components; the centre
+// anisotropy is the flux's face() of the rotated central difference
+// (center_anisotropy_rotated, finite_volume.rs:163-177).
+void modelc_rhs_oriented_block(const orc_grid *g, const orc_modelc_params *p, const double *phi,
+                               const double *u, const double *theta, double *kphi, double *ku) {
+  KR k = kr_new(p->eps4, p->fv_tol);
+  const double inv_h2 = 1.0 / (g->spacing[0] * g->spacing[1]);
+  const double quarter = 0.25;
+  const int64_t nx = g->block_cells[0], sx = nx + 2;
+  auto pv = [&](int64_t i, int64_t j) { return phi[(j + 1) * sx + (i + 1)]; };
+  for_each_interior(g, [&](const int64_t *idx) {
+    const int64_t i = idx[0], j = idx[1];
+    const double th = theta[j * nx + i];
+    const double s0 = std::sin(th), c0 = std::cos(th);
+    double derx_r = pv(i + 1, j) - pv(i, j);
+    double derx_l = pv(i, j) - pv(i - 1, j);
+    double derx_t = quarter * (pv(i + 1, j + 1) - pv(i - 1, j + 1) + pv(i + 1, j) - pv(i - 1, j));
+    double derx_b = quarter * (pv(i + 1, j) - pv(i - 1, j) + pv(i + 1, j - 1) - pv(i - 1, j - 1));
+    double dery_t = pv(i, j + 1) - pv(i, j);
+    double dery_b = pv(i, j) - pv(i, j - 1);
+    double dery_r = quarter * (pv(i + 1, j + 1) - pv(i + 1, j - 1) + pv(i, j + 1) - pv(i, j - 1));
+    double dery_l = quarter * (pv(i, j + 1) - pv(i, j - 1) + pv(i - 1, j + 1) - pv(i - 1, j - 1));
+    double rx, ry, a_r, da_r, a_l, da_l, a_t, da_t, a_b, da_b;
+    rotate(c0, s0, derx_r, dery_r, rx, ry);
+    kr_face(k, rx, ry, a_r, da_r);
+    rotate(c0, s0, derx_l, dery_l, rx, ry);
+    kr_face(k, rx, ry, a_l, da_l);
+    rotate(c0, s0, derx_t, dery_t, rx, ry);
+    kr_face(k, rx, ry, a_t, da_t);
+    rotate(c0, s0, derx_b, dery_b, rx, ry);
+    kr_face(k, rx, ry, a_b, da_b);
+    double j_r = a_r * (a_r * derx_r - da_r * dery_r);
+    double j_l = a_l * (a_l * derx_l - da_l * dery_l);
+    double j_t = a_t * (a_t * dery_t + da_t * derx_t);
+    double j_b = a_b * (a_b * dery_b + da_b * derx_b);
+    const int64_t o = (j + 1) * sx + (i + 1);
+    const double flux = (j_r - j_l + j_t - j_b) * inv_h2;
+    const double pc = phi[o];
+    const double g_prime = std::fma(pc * pc, pc, -pc);
+    const double one_m_p2 = std::fma(pc, -pc, 1.0);
+    const double p_prime = one_m_p2 * one_m_p2;
+    const double cdx = 0.5 * (pv(i + 1, j) - pv(i - 1, j));
+    const double cdy = 0.5 * (pv(i, j + 1) - pv(i, j - 1));
+    double crx, cry, a, da;
+    rotate(c0, s0, cdx, cdy, crx, cry);
+    kr_face(k, crx, cry, a, da);
+    kphi[o] = std::fma(p->lambda * u[o], -p_prime, flux - g_prime) / (a * a);
+  });
+  laplacian_block(g, u, ku, 1);
+  for_each_interior(g, [&](const int64_t *idx) {
+    const int64_t o = (idx[1] + 1) * sx + (idx[0] + 1);
+    ku[o] = std::fma(ku[o], p->d_thermal, 0.5 * kphi[o]);
+  });
+}
+
+// eval_drift of Model C (fill_ghosts_mirror on φ and u, then the per-block vector
+// field, axis-aligned or oriented) into (kphi, ku).
+void modelc_eval(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                 const double *theta, double *kphi, double *ku) {
+  orc_fill_ghosts(g, phi, 1, nullptr);
+  orc_fill_ghosts(g, u, 1, nullptr);
+  const int64_t nb = orc_num_blocks(g), len = orc_block_len(g, 1), tl = orc_block_len(g, 0);
+  for (int64_t b = 0; b < nb; ++b) {
+    if (theta)
+      modelc_rhs_oriented_block(g, p, phi + b * len, u + b * len, theta + b * tl, kphi + b * len,
+                                ku + b * len);
+    else
+      modelc_rhs_block(g, p, phi + b * len, u + b * len, kphi + b * len, ku + b * len);
+  }
+}
+
+// eval_drift of Heat<D> (fill_ghosts with bcs, κ·∇²u per block) into k.
+void heat_eval(const orc_grid *g, double kappa, double *u, const orc_face_bc *bcs, double *k) {
+  orc_fill_ghosts(g, u, 1, bcs);
+  const int64_t nb = orc_num_blocks(g), len = orc_block_len(g, 1);
+  for (int64_t b = 0; b < nb; ++b) {
+    double *kb = k + b * len;
+    laplacian_block(g, u + b * len, kb, 1);
+    for_each_interior(g, [&](const int64_t *idx) {
+      const int64_t o = lin(g, idx, 1);
+      kb[o] = kb[o] * kappa;
+    });
+  }
+}
+
+// RungeKutta4::step (src/integrators/rk4.rs:30-80) over nf fields of `len` entries
+// (whole slabs: copy_from_with and axpy_with sweep ghost entries too; the
+// tendencies' ghost entries stay zero).
+template <class Eval>
+void rk4_step(int nf, size_t len, double *const *state, std::vector<double> (&k)[4][2],
+              std::vector<double> (&y)[2], double dt, Eval &&eval) {
+  const double half_dt = 0.5 * dt;
+  const double c[3] = {half_dt, half_dt, dt};
+  auto ptrs = [&](std::vector<double> *v, double **out) {
+    for (int f = 0; f < nf; ++f) out[f] = v[f].data();
+  };
+  double *kp[2], *yp[2];
+  ptrs(k[0], kp);
+  eval(state, kp);
+  for (int s = 1; s < 4; ++s) {
+    ptrs(y, yp);
+    for (int f = 0; f < nf; ++f) {
+      std::copy(state[f], state[f] + len, y[f].begin());
+      orc_axpy(y[f].data(), k[s - 1][f].data(), c[s - 1], len);
+    }
+    ptrs(k[s], kp);
+    eval(yp, kp);
+  }
+  const double sixth = dt / 6.0, third = dt / 3.0;
+  const double w[4] = {sixth, third, third, sixth};
+  for (int s = 0; s < 4; ++s)
+    for (int f = 0; f < nf; ++f) orc_axpy(state[f], k[s][f].data(), w[s], len);
+}
+
+}  // namespace
+
+extern "C" {
+
+void orc_modelc_rhs_oriented(const orc_grid *g, const orc_modelc_params *p, const double *phi,
+                             const double *u, const double *theta, double *kphi, double *ku) {
+  const int64_t nb = orc_num_blocks(g), len = orc_block_len(g, 1), tl = orc_block_len(g, 0);
+  for (int64_t b = 0; b < nb; ++b)
+    modelc_rhs_oriented_block(g, p, phi + b * len, u + b * len, theta + b * tl, kphi + b * len,
+                              ku + b * len);
+}
+
+int orc_modelc_rk4_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                       const double *theta, int64_t nsteps, double dt, double *t) {
+  if (g->dim != 2 || orc_grid_check(g) != 0) return -1;
+  const size_t len = (size_t)(orc_num_blocks(g) * orc_block_len(g, 1));
+  std::vector<double> k[4][2], y[2];
+  for (auto &ks : k)
+    for (auto &v : ks) v.assign(len, 0.0);
+  for (auto &v : y) v.assign(len, 0.0);
+  double *state[2] = {phi, u};
+  for (int64_t s = 0; s < nsteps; ++s) {
+    rk4_step(2, len, state, k, y, dt, [&](double *const *in, double *const *out) {
+      modelc_eval(g, p, in[0], in[1], theta, out[0], out[1]);
+    });
+    *t = *t + dt;
+  }
+  return 0;
+}
+
+int orc_heat_rk4_run(const orc_grid *g, double kappa, double *u, int64_t nsteps, double dt,
+                     double *t, const orc_face_bc *bcs) {
+  if (orc_grid_check(g) != 0) return -1;
+  const size_t len = (size_t)(orc_num_blocks(g) * orc_block_len(g, 1));
+  std::vector<double> k[4][2], y[2];
+  for (auto &ks : k) ks[0].assign(len, 0.0);
+  y[0].assign(len, 0.0);
+  double *state[2] = {u, nullptr};
+  for (int64_t s = 0; s < nsteps; ++s) {
+    rk4_step(1, len, state, k, y, dt, [&](double *const *in, double *const *out) {
+      heat_eval(g, kappa, in[0], bcs, out[0]);
+    });
+    *t = *t + dt;
+  }
+  return 0;
+}
+
+// Oriented Euler–Maruyama / forward-Euler run (b = 0): as orc_modelc_run with the
+// multi-grain vector field.
+int orc_modelc_oriented_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                            const double *theta, int64_t nsteps, double dt, double *t) {
+  if (g->dim != 2 || orc_grid_check(g) != 0) return -1;
+  const size_t len = (size_t)(orc_num_blocks(g) * orc_block_len(g, 1));
+  std::vector<double> kphi(len, 0.0), ku(len, 0.0);
+  for (int64_t s = 0; s < nsteps; ++s) {
+    modelc_eval(g, p, phi, u, theta, kphi.data(), ku.data());
+    orc_axpy(phi, kphi.data(), dt, len);
+    orc_axpy(u, ku.data(), dt, len);
+    *t = *t + dt;
+  }
+  return 0;
+}
+
+// The orientation pass of ModelC::initialize_grains (phasefield.rs:286-314):
+// nearest-seed Voronoi over the candidates whose closest approach to the block
+// beats every grain's farthest corner; first minimum on ties; 0 with no grain.
+// grains: [cx, cy, radius, θ₀] × n; theta: ghost-0 slabs.
+void orc_initialize_orientations(const orc_grid *g, double *theta, const double *grains,
+                                 size_t ngrains) {
+  const int64_t nx = g->block_cells[0], ny = g->block_cells[1];
+  const int64_t nb = orc_num_blocks(g);
+  for (int64_t b = 0; b < nb; ++b) {
+    const double hx = g->spacing[0], hy = g->spacing[1];
+    int64_t i00[3] = {0, 0, 0}, i11[3] = {nx - 1, ny - 1, 0};
+    double c0[3], c1[3];
+    orc_cell_center(g, b, i00, c0);
+    orc_cell_center(g, b, i11, c1);
+    const double lo[2] = {std::fma(0.5, -hx, c0[0]), std::fma(0.5, -hy, c0[1])};
+    const double hi[2] = {std::fma(0.5, hx, c1[0]), std::fma(0.5, hy, c1[1])};
+    auto dist_box = [&](double cx, double cy) {
+      double dx = std::fmax(std::fmax(lo[0] - cx, cx - hi[0]), 0.0);
+      double dy = std::fmax(std::fmax(lo[1] - cy, cy - hi[1]), 0.0);
+      return std::hypot(dx, dy);
+    };
+    auto dist_far = [&](double cx, double cy) {
+      double dx = std::fmax(std::fabs(cx - lo[0]), std::fabs(cx - hi[0]));
+      double dy = std::fmax(std::fabs(cy - lo[1]), std::fabs(cy - hi[1]));
+      return std::hypot(dx, dy);
+    };
+    double threshold = 1.7976931348623157e308;
+    for (size_t k = 0; k < ngrains; ++k)
+      threshold = std::fmin(threshold, dist_far(grains[4 * k], grains[4 * k + 1]));
+    std::vector<size_t> cand;
+    for (size_t k = 0; k < ngrains; ++k)
+      if (dist_box(grains[4 * k], grains[4 * k + 1]) <= threshold) cand.push_back(k);
+    double *tb = theta + b * nx * ny;
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        int64_t idx[3] = {i, j, 0};
+        double xy[3];
+        orc_cell_center(g, b, idx, xy);
+        double best = 0.0, bestd = 0.0;
+        bool have = false;
+        for (size_t k : cand) {
+          const double ddx = xy[0] - grains[4 * k], ddy = xy[1] - grains[4 * k + 1];
+          const double d = std::fma(ddy, ddy, ddx * ddx);
+          if (!have || d < bestd) {
+            have = true;
+            bestd = d;
+            best = grains[4 * k + 3];
+          }
+        }
+        tb[j * nx + i] = have ? best : 0.0;
+      }
+  }
+}
+
+}  // extern "C"
+
+// ===========================================================================
 // AMR (2-D): src/geometry/amr/{grid,intergrid,cluster,adapt}.rs and
 // src/integrators/subcycling.rs.
 // ===========================================================================

@@ -120,6 +120,25 @@ int orc_heat_run(const orc_grid *g, double kappa, double *u, int64_t nsteps, dou
 void orc_initialize_grains(const orc_grid *g, double *phi, double *u, const double *grains,
                            size_t ngrains, double u0);
 
+/* Multi-grain Model C (ModelC::with_orientations): the oriented vector field
+ * (finite_volume.rs:188-235, phasefield.rs:378-392) with θ₀ in ghost-0 slabs
+ * (nx·ny per block), sin/cos from libm per cell as the reference. */
+void orc_modelc_rhs_oriented(const orc_grid *g, const orc_modelc_params *p, const double *phi,
+                             const double *u, const double *theta, double *kphi, double *ku);
+/* Forward-Euler steps of the oriented model (b = 0). */
+int orc_modelc_oriented_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                            const double *theta, int64_t nsteps, double dt, double *t);
+/* RungeKutta4::step (src/integrators/rk4.rs:30-80) on Model C (theta NULL: axis-
+ * aligned) and on Heat<D> (bcs as in orc_fill_ghosts). */
+int orc_modelc_rk4_run(const orc_grid *g, const orc_modelc_params *p, double *phi, double *u,
+                       const double *theta, int64_t nsteps, double dt, double *t);
+int orc_heat_rk4_run(const orc_grid *g, double kappa, double *u, int64_t nsteps, double dt,
+                     double *t, const orc_face_bc *bcs);
+/* The orientation pass of ModelC::initialize_grains (phasefield.rs:286-314):
+ * grains as [cx, cy, radius, θ₀] quadruples, θ₀ into ghost-0 slabs. */
+void orc_initialize_orientations(const orc_grid *g, double *theta, const double *grains,
+                                 size_t ngrains);
+
 /* seed_positions recipe (examples/model_c.rs:175-206), multi-seed branch:
  * writes ngrains [cx, cy, radius] triples. */
 void orc_seed_positions(uint64_t seed, size_t nseeds, double extent, double radius,
