@@ -426,6 +426,105 @@ def run_ensemble(args) -> int:
     return 0
 
 
+def run_jump(args) -> int:
+    """The jump-diffusion model of the reference's jump-driver tests
+    (WienerJump<1, 1>: dw = μ dt + σ dW; on a fire count += 1, jsum += payoff, at a
+    per-path intensity λ) on 2^26 paths of one cell each, blocks of 2^16 paths,
+    100 Euler–Maruyama steps. A step is one lgd_apply_drivers launch: Time, Wiener(0)
+    and Jump(0) over the three state fields. The vector fields are constant and are
+    written once before the timed region, as are the intensities (λ ∈ U[0.5, 5))."""
+    import numpy as np
+    import torch
+
+    import legendre_amd as lg
+
+    paths, block, dt, seed = 1 << 26, 1 << 16, 0.01, 42
+    steps = args.jump_steps or 100
+    mu, sigma, payoff = 0.1, 0.5, 2.0
+    stream = torch.cuda.Stream()
+    ctx = lg.Context(0, stream.cuda_stream)
+    grid = lg.CartesianGrid(ctx, [paths], [block], [0.0], [1.0])
+    lam = 0.5 + 4.5 * np.random.default_rng(seed).random(paths)
+
+    def const(v):
+        b = lg.DeviceBuffer(ctx, paths)
+        b.upload(np.full(paths, v))
+        return b
+
+    w, count, jsum = (lg.DeviceBuffer(ctx, paths) for _ in range(3))
+    drift, amp, one, pay = const(mu), const(sigma), const(1.0), const(payoff)
+    rate = lg.DeviceBuffer(ctx, paths)
+    rate.upload(lam)
+    drivers = [lg.Driver(lg.LGD_DRIVER_TIME, 0, [drift, None, None]),
+               lg.Driver(lg.LGD_DRIVER_WIENER, 0, [amp, None, None]),
+               lg.Driver(lg.LGD_DRIVER_JUMP, 0, [None, one, pay], rate)]
+    fields = [w, count, jsum]
+    t = 0.0
+    for _ in range(args.warmup):
+        grid.apply_drivers(fields, [0, 0, 0], drivers, dt, seed, lg.time_bits(t))
+        t = t + dt
+    torch.cuda.synchronize()
+    events = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    for s in range(steps):
+        events[s].record(stream)
+        grid.apply_drivers(fields, [0, 0, 0], drivers, dt, seed, lg.time_bits(t))
+        t = t + dt
+    events[steps].record(stream)
+    torch.cuda.synchronize()
+    total_ms = events[0].elapsed_time(events[steps])
+    per = [events[s].elapsed_time(events[s + 1]) for s in range(steps)]
+    c = count.download()
+    fired = float(c.mean())
+    ctx.close()
+    avg_ms = sum(per) / len(per)
+    bytes_per_launch = 88 * paths  # 3 fields read + written, 5 vector-field / intensity reads
+    achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        sample = 1 << 22
+        g = oracle.grid([sample], [block], [0.0], [1.0])
+        z = [np.zeros(sample) for _ in range(3)]
+        tt = time.perf_counter()
+        oracle.jump_diffusion_run(g, lam[:sample].copy(), *z, 1, dt, seed, mu, sigma, payoff,
+                                  nthreads=threads)
+        first = time.perf_counter() - tt
+        n = int(max(1, min(100, 10.0 // max(first, 1e-3))))
+        tt = time.perf_counter()
+        oracle.jump_diffusion_run(g, lam[:sample].copy(), *z, n, dt, seed, mu, sigma, payoff,
+                                  nthreads=threads)
+        el = time.perf_counter() - tt
+        cpu = {"value": sample * n / el / 1e9, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
+               "sample": f"oracle jump-diffusion, {sample} paths x {n} steps, {el:.1f}s, {threads} threads"}
+    line = {
+        "metric": METRIC,
+        "value": round(paths * steps / (total_ms / 1e3) / 1e9, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(total_ms / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"jump_diffusion_{paths}paths_{block}blocks_{steps}steps",
+                   "paths": paths, "dt": dt, "mu": mu, "sigma": sigma, "payoff": payoff,
+                   "lambda": "U[0.5,5)"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "drivers_kernel", "bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": round(avg_ms, 5)},
+        "jump": {"mean_count": fired},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_amr(args) -> int:
     """Batched AMR: subcycled Model C on a 3-level Berger–Oliger hierarchy with
     regrids, every patch of a level advanced by one launch per stage, against one
@@ -554,9 +653,10 @@ def run_amr(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["modelc", "adjoint", "ensemble", "amr"], default="modelc",
+    ap.add_argument("--workload", choices=["modelc", "adjoint", "ensemble", "amr", "jump"], default="modelc",
                     help="modelc: the headline plain step; adjoint: the calibration gradient; "
                          "ensemble: 512 batched realizations; amr: 3-level subcycled hierarchy")
+    ap.add_argument("--jump-steps", type=int, default=0, help="jump workload steps (0 = 100)")
     ap.add_argument("--amr-steps", type=int, default=0, help="AMR coarse steps (0 = 500)")
     ap.add_argument("--per-block-chunks", type=int, default=2,
                     help="regrid intervals timed with one launch per patch (amr; 0 skips the baseline)")
@@ -604,6 +704,8 @@ def main() -> int:
         return run_ensemble(args)
     if args.workload == "amr":
         return run_amr(args)
+    if args.workload == "jump":
+        return run_jump(args)
 
     from legendre_amd.workloads import ModelCConfig
 

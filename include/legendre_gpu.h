@@ -193,6 +193,38 @@ int lgd_axpy(lgd_ctx *ctx, lgd_buf *x, const lgd_buf *y, double alpha);
 int lgd_apply_wiener(lgd_ctx *ctx, const lgd_grid *grid, lgd_buf *x, const lgd_buf *amp,
                      uint32_t ghost, double dt, uint64_t seed, uint64_t salt, uint64_t stream);
 
+/* Driver (src/core/driver.rs:35-50): what multiplies a vector field. */
+typedef enum { LGD_DRIVER_TIME = 0, LGD_DRIVER_WIENER = 1, LGD_DRIVER_JUMP = 2 } lgd_driver_kind;
+
+/* One driver of an explicit update: its kind and index j (Wiener(j) / Jump(j)), and
+ * per field the buffer the model evaluated for it — dY/dt for Time, the amplitude
+ * for Wiener, the increment ΔJ for Jump — congruent with that field, or NULL (or a
+ * zero-length buffer) where the driver does not move the field (the reference's
+ * zero-length slabs of State::like_for, src/core/state.rs:317-337).  A jump driver
+ * also carries its per-cell intensity λ in ghost-0 slabs (BlockStateMut::
+ * intensity_mut, state.rs:806-832). */
+typedef struct lgd_driver {
+  int32_t kind;              /* lgd_driver_kind */
+  uint32_t index;            /* j */
+  const lgd_buf *const *amp; /* nfields entries */
+  const lgd_buf *rate;       /* Jump only */
+} lgd_driver;
+
+/* State::apply_step_with (src/core/state.rs:646-699) for any driver set: x[f] (ghost
+ * width ghost[f]) takes every driver in the order given, each through
+ * Driver::apply_slab (driver.rs:115-182): Time x += dt·v on every entry; Wiener
+ * x += (√dt·v)·ξ on interior entries with v != 0; Jump x += ΔJ on interior cells
+ * that fire, each firing with probability 1 − e^{−λ·dt} against the intensity.  A
+ * stochastic driver's per-cell draw is keyed by (seed, salt, Driver::stream(),
+ * block, cell) and shared by every field it moves.  A WienerJump<W, J> set lists
+ * Time, Wiener(0..W), Jump(0..J) (driver.rs:239-252); EulerMaruyama passes
+ * salt = t.to_bits().  At most 8 fields and 8 drivers (LGD_ERR_UNSUPPORTED
+ * beyond); a jump driver without an intensity is LGD_ERR_INVALID_ARGUMENT (the
+ * reference's panic at driver.rs:160).  Jump fires use the device expm1. */
+int lgd_apply_drivers(lgd_ctx *ctx, const lgd_grid *grid, size_t nfields, lgd_buf *const *x,
+                      const uint32_t *ghost, size_t ndrivers, const lgd_driver *drivers, double dt,
+                      uint64_t seed, uint64_t salt);
+
 /* ------------------------------------------------------ fused hot path */
 /* One Euler–Maruyama step of Model C (src/integrators/euler_maruyama.rs:34-75 on
  * ModelC<NoNoise | Wiener<1>>, mirror ghosts, src/physics/phasefield.rs:431-439) in one
@@ -209,6 +241,13 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
  * examples/heat3d.rs:54-79) with mirror/periodic ghosts, fused the same way. */
 int lgd_heat_step(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_buf *u,
                   lgd_buf *u_next, double dt);
+/* The same step with a FaceBc per physical face (bcs as in lgd_fill_ghosts, NULL =
+ * Mirror everywhere): Dirichlet, Flux and Extrapolate halos are evaluated in the
+ * kernel from the edge cells (FaceBc::ghost_value, src/geometry/boundary.rs:63-73),
+ * bitwise what fill_ghosts would have written at the face-neighbour ghosts the
+ * stencil reads. */
+int lgd_heat_step_bc(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_face_bc *bcs,
+                     const lgd_buf *u, lgd_buf *u_next, double dt);
 
 /* ------------------------------------- AMR hierarchy (src/geometry/amr), 2-D */
 /* AmrGrid::from_patches(base, ratios, levels) (src/geometry/amr/grid.rs:90-196).
@@ -304,7 +343,7 @@ typedef enum { LGD_INTEGRATOR_EULER = 0, LGD_INTEGRATOR_RK4 = 1 } lgd_integrator
 int lgd_sim_create_modelc(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
                           int32_t wiener, uint64_t seed, int32_t mode, lgd_sim **out);
 /* Simulation::new(grid, FiniteDifference, Heat<D>{kappa}, ForwardEuler, ...): field 0 = u,
- * ghost 1; bcs as in lgd_fill_ghosts (fused mode needs Mirror/periodic). */
+ * ghost 1; bcs as in lgd_fill_ghosts (every FaceBc in both modes). */
 int lgd_sim_create_heat(lgd_ctx *ctx, const lgd_grid *grid, double kappa,
                         const lgd_face_bc *bcs, int32_t mode, lgd_sim **out);
 /* Simulation::adaptive on an AmrGrid (simulation.rs:137-183): model 0 = Heat (κ∇²u,

@@ -20,6 +20,9 @@ from ._abi import (
     LGD_BC_EXTRAPOLATE,
     LGD_BC_FLUX,
     LGD_BC_MIRROR,
+    LGD_DRIVER_JUMP,
+    LGD_DRIVER_TIME,
+    LGD_DRIVER_WIENER,
     LGD_INTEGRATOR_EULER,
     LGD_INTEGRATOR_RK4,
     LGD_MODE_FUSED,
@@ -43,6 +46,10 @@ __all__ = [
     "LGD_MODE_PER_INTENT",
     "LGD_INTEGRATOR_EULER",
     "LGD_INTEGRATOR_RK4",
+    "LGD_DRIVER_TIME",
+    "LGD_DRIVER_WIENER",
+    "LGD_DRIVER_JUMP",
+    "Driver",
 ]
 
 # Karma–Rappel thin-interface constant a₂ (src/physics/phasefield.rs:44).
@@ -348,11 +355,46 @@ class CartesianGrid:
             self.ctx.handle,
         )
 
-    def heat_step(self, kappa, u, u_next, dt) -> None:
+    def heat_step(self, kappa, u, u_next, dt, bcs=None) -> None:
+        if bcs is None:
+            rc = self._lib.lgd_heat_step(self.ctx.handle, self.handle, kappa, u.handle, u_next.handle, dt)
+        else:
+            rc = self._lib.lgd_heat_step_bc(self.ctx.handle, self.handle, kappa, _bcs_array(bcs),
+                                            u.handle, u_next.handle, dt)
+        _abi.check(rc, self.ctx.handle)
+
+    def apply_drivers(self, xs, ghosts, drivers, dt, seed, salt) -> None:
+        """State::apply_step_with over the field buffers xs (ghost widths ghosts):
+        drivers is a sequence of Driver, applied in order."""
+        nf = len(xs)
+        xarr = (C.c_void_p * nf)(*[x.handle for x in xs])
+        garr = (C.c_uint32 * nf)(*ghosts)
+        keep = []
+        darr = (_abi.DriverC * len(drivers))()
+        for i, d in enumerate(drivers):
+            if len(d.fields) != nf:
+                raise ValueError("a driver names one buffer (or None) per field")
+            a = (C.c_void_p * nf)(*[f.handle if f is not None else None for f in d.fields])
+            keep.append(a)
+            darr[i] = _abi.DriverC(d.kind, d.index, C.cast(a, _abi.PP),
+                                   d.rate.handle if d.rate is not None else None)
         _abi.check(
-            self._lib.lgd_heat_step(self.ctx.handle, self.handle, kappa, u.handle, u_next.handle, dt),
+            self._lib.lgd_apply_drivers(self.ctx.handle, self.handle, nf, C.cast(xarr, _abi.PP), garr,
+                                        len(drivers), darr, dt, seed, salt),
             self.ctx.handle,
         )
+
+
+@dataclasses.dataclass
+class Driver:
+    """One driver of an explicit update (src/core/driver.rs:35-50): kind LGD_DRIVER_*,
+    index j of Wiener(j) / Jump(j), per field the evaluated buffer (dY/dt, amplitude
+    or jump increment) or None, and a jump driver's per-cell intensity (ghost 0)."""
+
+    kind: int
+    index: int
+    fields: Sequence[Optional["DeviceBuffer"]]
+    rate: Optional["DeviceBuffer"] = None
 
 
 @dataclasses.dataclass

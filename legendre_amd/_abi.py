@@ -26,6 +26,7 @@ STATUS_NAMES = {
 LGD_BC_MIRROR, LGD_BC_DIRICHLET, LGD_BC_FLUX, LGD_BC_EXTRAPOLATE = 0, 1, 2, 3
 LGD_MODE_PER_INTENT, LGD_MODE_FUSED = 0, 1
 LGD_INTEGRATOR_EULER, LGD_INTEGRATOR_RK4 = 0, 1
+LGD_DRIVER_TIME, LGD_DRIVER_WIENER, LGD_DRIVER_JUMP = 0, 1, 2
 
 
 class CartesianDesc(C.Structure):
@@ -89,6 +90,11 @@ class TuningResultC(C.Structure):
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
+
+
+class DriverC(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_uint32), ("amp", PP), ("rate", P)]
+
 I32, I64, U32, U64, SZ, DBL = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_size_t, C.c_double
 DP = C.POINTER(C.c_double)
 
@@ -126,8 +132,10 @@ SIGNATURES = {
     "lgd_heat_rhs": (C.c_int, [P, P, DBL, P, P]),
     "lgd_axpy": (C.c_int, [P, P, P, DBL]),
     "lgd_apply_wiener": (C.c_int, [P, P, P, P, U32, DBL, U64, U64, U64]),
+    "lgd_apply_drivers": (C.c_int, [P, P, SZ, PP, C.POINTER(U32), SZ, C.POINTER(DriverC), DBL, U64, U64]),
     "lgd_modelc_step": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P, P, P, P, DBL, U64, U64, I32]),
     "lgd_heat_step": (C.c_int, [P, P, DBL, P, P, DBL]),
+    "lgd_heat_step_bc": (C.c_int, [P, P, DBL, C.POINTER(FaceBcC), P, P, DBL]),
     "lgd_amr_create": (C.c_int, [P, C.POINTER(CartesianDesc), C.POINTER(U32), SZ, C.POINTER(I64), SZ, PP]),
     "lgd_amr_destroy": (C.c_int, [P]),
     "lgd_amr_num_blocks": (I64, [P]),

@@ -92,10 +92,35 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
   u_n[oc] = uc + dt * du;
 }
 
+// The face rules of the six physical faces, low then high per dimension.
+struct FaceRules {
+  lgd_face_bc bc[6];
+};
+
+// FaceBc::ghost_value at layer 0 (src/geometry/boundary.rs:63-73) from the edge cell
+// i0 and the next cell inward i1 (read only by Extrapolate).
+__device__ __forceinline__ double face_rule(const lgd_face_bc &bc, double h, double i0,
+                                            const double *i1) {
+  switch (bc.kind) {
+    case LGD_BC_DIRICHLET:
+      return (2.0 * bc.value) - i0;
+    case LGD_BC_FLUX:
+      return i0 + ((1.0 * h) * bc.value);
+    case LGD_BC_EXTRAPOLATE:
+      return (2.0 * i0) - (1.0 * *i1);
+    default:
+      return i0;
+  }
+}
+
 // Heat<D> forward-Euler step (examples/heat3d.rs:63-79, src/integrators/euler.rs:128-129):
 // u' = u + dt*(κ·∇²u), one thread per interior cell, any dimension 1..3.
+// kRules: a neighbour across a non-periodic domain face is the face rule's ghost
+// value. The stencil reads face neighbours only, so the value is the one the
+// dimension sweep writes there from the edge cells along the normal.
+template <bool kRules>
 __global__ void heat_step_kernel(lgd_cart c, double ih0, double ih1, double ih2, double kappa,
-                                 double dt, const double *u, double *u_n) {
+                                 double dt, const double *u, double *u_n, FaceRules rules) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t ncell = c.n[0] * c.n[1] * c.n[2];
   if (t >= c.nblocks * ncell) return;
@@ -115,10 +140,23 @@ __global__ void heat_step_kernel(lgd_cart c, double ih0, double ih1, double ih2,
   for (int d = 0; d < c.dim; ++d) {
     int64_t gp[3] = {g[0], g[1], g[2]};
     int64_t gm[3] = {g[0], g[1], g[2]};
-    gp[d] = resolve_coord(g[d] + 1, N[d], c.periodic[d]);
-    gm[d] = resolve_coord(g[d] - 1, N[d], c.periodic[d]);
-    double up = u[cell_offset(c, gp[0], gp[1], gp[2])];
-    double um = u[cell_offset(c, gm[0], gm[1], gm[2])];
+    double up, um;
+    if (kRules && !c.periodic[d] && g[d] + 1 >= N[d]) {
+      gp[d] = g[d] - 1;  // the next cell inward from the high face
+      double i1 = rules.bc[2 * d + 1].kind == LGD_BC_EXTRAPOLATE ? u[cell_offset(c, gp[0], gp[1], gp[2])] : 0.0;
+      up = face_rule(rules.bc[2 * d + 1], c.h[d], center, &i1);
+    } else {
+      gp[d] = resolve_coord(g[d] + 1, N[d], c.periodic[d]);
+      up = u[cell_offset(c, gp[0], gp[1], gp[2])];
+    }
+    if (kRules && !c.periodic[d] && g[d] == 0) {
+      gm[d] = 1;  // the next cell inward from the low face
+      double i1 = rules.bc[2 * d].kind == LGD_BC_EXTRAPOLATE ? u[cell_offset(c, gm[0], gm[1], gm[2])] : 0.0;
+      um = face_rule(rules.bc[2 * d], c.h[d], center, &i1);
+    } else {
+      gm[d] = resolve_coord(g[d] - 1, N[d], c.periodic[d]);
+      um = u[cell_offset(c, gm[0], gm[1], gm[2])];
+    }
     acc = acc + (((up - (center + center)) + um) * inv[d]);
   }
   u_n[oc] = center + dt * (acc * kappa);
@@ -207,6 +245,11 @@ int lgd_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params 
 
 int lgd_heat_step(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_buf *u,
                   lgd_buf *u_next, double dt) {
+  return lgd_heat_step_bc(ctx, grid, kappa, nullptr, u, u_next, dt);
+}
+
+int lgd_heat_step_bc(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_face_bc *bcs,
+                     const lgd_buf *u, lgd_buf *u_next, double dt) {
   if (!ctx || !grid) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
   const lgd_cart &c = grid->cart;
   int rc;
@@ -217,9 +260,29 @@ int lgd_heat_step(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd_bu
     return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "the fused step needs a distinct output buffer");
   double ih[3] = {0.0, 0.0, 0.0};
   for (int d = 0; d < c.dim; ++d) ih[d] = 1.0 / (c.h[d] * c.h[d]);
+  FaceRules rules{};
+  bool any_rule = false;
+  if (bcs) {
+    for (int d = 0; d < c.dim; ++d)
+      for (int e = 0; e < 2; ++e) {
+        const lgd_face_bc &bc = bcs[2 * d + e];
+        if (bc.kind < LGD_BC_MIRROR || bc.kind > LGD_BC_EXTRAPOLATE)
+          return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "unknown FaceBc kind");
+        // Linear extrapolation reads two interior cells (cartesian.rs:418-421).
+        if (bc.kind == LGD_BC_EXTRAPOLATE && c.n[d] < 2)
+          return fail(ctx, LGD_ERR_INVALID_ARGUMENT,
+                      "linear extrapolation reads two interior cells per dimension");
+        rules.bc[2 * d + e] = bc;
+        any_rule = any_rule || (bc.kind != LGD_BC_MIRROR && !c.periodic[d]);
+      }
+  }
   int64_t total = c.nblocks * c.n[0] * c.n[1] * c.n[2];
-  heat_step_kernel<<<blocks_for(total, 256), 256, 0, ctx->stream>>>(c, ih[0], ih[1], ih[2], kappa,
-                                                                    dt, u->data, u_next->data);
+  if (any_rule)
+    heat_step_kernel<true><<<blocks_for(total, 256), 256, 0, ctx->stream>>>(
+        c, ih[0], ih[1], ih[2], kappa, dt, u->data, u_next->data, rules);
+  else
+    heat_step_kernel<false><<<blocks_for(total, 256), 256, 0, ctx->stream>>>(
+        c, ih[0], ih[1], ih[2], kappa, dt, u->data, u_next->data, rules);
   return launch_check(ctx, "heat_step");
 }
 

@@ -149,14 +149,6 @@ int alloc_fields(lgd_sim *s, int nfields) {
   return LGD_OK;
 }
 
-bool all_mirror(const lgd_sim *s) {
-  if (!s->has_bcs) return true;
-  for (int d = 0; d < s->grid->cart.dim; ++d)
-    for (int e = 0; e < 2; ++e)
-      if (s->bcs[d * 2 + e].kind != LGD_BC_MIRROR) return false;
-  return true;
-}
-
 // eval_drift of Model C: ModelC::fill_ghosts, then the per-block Time vector field
 // (axis-aligned or multi-grain) of (phi, u) into (kphi, ku).
 int modelc_eval(lgd_sim *s, lgd_buf *phi, lgd_buf *u, lgd_buf *kphi, lgd_buf *ku) {
@@ -243,7 +235,8 @@ int step_heat(lgd_sim *s, double dt) {
   int rc;
   if (s->integrator == LGD_INTEGRATOR_RK4) return step_rk4(s, dt);
   if (s->mode == LGD_MODE_FUSED) {
-    return lgd_heat_step(s->ctx, s->grid, s->kappa, s->state[0], s->next[0], dt);
+    return lgd_heat_step_bc(s->ctx, s->grid, s->kappa, s->has_bcs ? s->bcs : nullptr, s->state[0],
+                            s->next[0], dt);
   }
   if ((rc = lgd_fill_ghosts(s->ctx, s->grid, s->state[0], 1, s->has_bcs ? s->bcs : nullptr)))
     return rc;
@@ -707,11 +700,17 @@ int lgd_sim_create_heat(lgd_ctx *ctx, const lgd_grid *grid, double kappa,
     s->has_bcs = true;
     for (int i = 0; i < 2 * grid->cart.dim; ++i) s->bcs[i] = bcs[i];
   }
-  if (mode == LGD_MODE_FUSED && !all_mirror(s)) {
-    delete s;
-    return fail(ctx, LGD_ERR_UNSUPPORTED,
-                "the fused heat step implements mirror and periodic faces; use "
-                "LGD_MODE_PER_INTENT for Dirichlet/Flux/Extrapolate faces");
+  if (bcs) {
+    for (int d = 0; d < grid->cart.dim; ++d)
+      for (int e = 0; e < 2; ++e) {
+        const int k = bcs[2 * d + e].kind;
+        if (k < LGD_BC_MIRROR || k > LGD_BC_EXTRAPOLATE ||
+            (k == LGD_BC_EXTRAPOLATE && grid->cart.n[d] < 2)) {
+          delete s;
+          return fail(ctx, LGD_ERR_INVALID_ARGUMENT,
+                      "unknown FaceBc kind, or extrapolation across fewer than two cells");
+        }
+      }
   }
   if ((rc = alloc_fields(s, 1))) {
     free_all(s);

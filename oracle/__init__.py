@@ -54,6 +54,12 @@ class ModelCParams(C.Structure):
 DP = C.POINTER(C.c_double)
 GP = C.POINTER(Grid)
 
+DRIVER_TIME, DRIVER_WIENER, DRIVER_JUMP = 0, 1, 2
+
+
+class Driver(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("index", C.c_uint32), ("amp", C.POINTER(DP)), ("rate", DP)]
+
 _SIGS = {
     "orc_splitmix64": (C.c_uint64, [C.c_uint64]),
     "orc_mix_key": (C.c_uint64, [C.c_uint64, C.POINTER(C.c_uint64), C.c_size_t]),
@@ -86,6 +92,10 @@ _SIGS = {
                                      DP]),
     "orc_heat_rk4_run": (C.c_int, [GP, C.c_double, DP, C.c_int64, C.c_double, DP, C.POINTER(FaceBc)]),
     "orc_initialize_orientations": (None, [GP, DP, DP, C.c_size_t]),
+    "orc_apply_drivers": (C.c_int, [GP, C.c_size_t, C.POINTER(DP), C.POINTER(C.c_uint32), C.c_size_t,
+                                    C.POINTER(Driver), C.c_double, C.c_uint64, C.c_uint64, C.c_int32]),
+    "orc_jump_diffusion_run": (C.c_int, [GP, DP, DP, DP, DP, C.c_int64, C.c_double, DP, C.c_uint64,
+                                         C.c_double, C.c_double, C.c_double, C.c_int32]),
     "orc_modelc_tlm_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_int32, DP, DP, DP,
                                      C.c_int64, C.c_double]),
     "orc_modelc_adjoint_step": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_double, DP, DP,
@@ -298,6 +308,34 @@ def initialize_orientations(g: Grid, theta, grains: np.ndarray) -> None:
     """The nearest-seed orientation pass of initialize_grains; grains [cx, cy, r, θ₀]."""
     gr = np.ascontiguousarray(grains, dtype=np.float64).reshape(-1, 4)
     lib().orc_initialize_orientations(C.byref(g), _dp(theta), _dp(gr), gr.shape[0])
+
+
+def apply_drivers(g: Grid, xs, ghosts, drivers, dt, seed, salt, nthreads=1) -> None:
+    """State::apply_step_with over the field buffers xs. drivers: (kind, index, amps, rate)
+    with amps one array or None per field and rate the jump intensity (ghost-0 slabs)."""
+    nf = len(xs)
+    xarr = (DP * nf)(*[_dp(x) for x in xs])
+    garr = (C.c_uint32 * nf)(*ghosts)
+    keep = []
+    darr = (Driver * len(drivers))()
+    for i, (kind, index, amps, rate) in enumerate(drivers):
+        a = (DP * nf)(*[_dp(v) if v is not None else DP() for v in amps])
+        keep.append(a)
+        darr[i] = Driver(kind, index, a, _dp(rate) if rate is not None else DP())
+    rc = lib().orc_apply_drivers(C.byref(g), nf, xarr, garr, len(drivers), darr, dt, seed, salt,
+                                 nthreads)
+    if rc != 0:
+        raise ValueError("orc_apply_drivers rejected the drivers")
+
+
+def jump_diffusion_run(g: Grid, lam, w, count, jsum, nsteps, dt, seed, mu=0.1, sigma=0.5, payoff=2.0,
+                       t0=0.0, nthreads=1) -> float:
+    t = C.c_double(t0)
+    rc = lib().orc_jump_diffusion_run(C.byref(g), _dp(lam), _dp(w), _dp(count), _dp(jsum), nsteps, dt,
+                                      C.byref(t), seed, mu, sigma, payoff, nthreads)
+    if rc != 0:
+        raise ValueError("orc_jump_diffusion_run rejected the configuration")
+    return t.value
 
 
 def seed_positions(seed: int, nseeds: int, extent: float, radius: float) -> np.ndarray:

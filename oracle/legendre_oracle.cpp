@@ -1636,3 +1636,109 @@ extern "C" size_t orc_cluster(int64_t *flags, size_t n, double efficiency, size_
   }
   return boxes.size();
 }
+
+// ---- Driver::Jump and the multi-driver explicit update ----
+
+namespace {
+
+// Driver::Jump arm of Driver::apply_slab (src/core/driver.rs:159-182) over the
+// fields one jump driver moves, on one block: one fire per interior cell shared by
+// every field, thinned against the per-cell intensity (ghost-0 layout, indexed by
+// Grid::cell_key) as Bernoulli(1 − e^{−λ·dt}).
+void jump_block(const orc_grid *g, int64_t block, size_t nfields, double *const *x,
+                const double *const *inc, const uint32_t *ghost, const double *rate, double dt,
+                uint64_t seed, uint64_t salt, uint64_t stream) {
+  uint64_t parts[3] = {salt, stream, (uint64_t)block};
+  uint64_t block_key = orc_mix_key(seed, parts, 3);
+  for (size_t f = 0; f < nfields; ++f) {
+    if (!inc[f]) continue;
+    int64_t len = orc_block_len(g, ghost[f]);
+    for (int64_t i = 0; i < len; ++i) {
+      int64_t cell = orc_cell_key(g, ghost[f], i);
+      if (cell < 0) continue;
+      double lambda = rate[cell];
+      if (lambda <= 0.0) continue;
+      uint64_t key = orc_splitmix64(block_key ^ (uint64_t)cell);
+      double p = -std::expm1(-lambda * dt);
+      if (orc_unit_open(key) < p) x[f][i] = x[f][i] + inc[f][i];
+    }
+  }
+}
+
+// Driver::stream (src/core/driver.rs:55-61).
+uint64_t driver_stream(int32_t kind, uint32_t index) {
+  return kind == ORC_DRIVER_TIME ? 0ull : ((uint64_t)kind << 32) | (uint64_t)index;
+}
+
+}  // namespace
+
+extern "C" {
+
+int orc_apply_drivers(const orc_grid *g, size_t nfields, double *const *x, const uint32_t *ghost,
+                      size_t ndrivers, const orc_driver *drivers, double dt, uint64_t seed,
+                      uint64_t salt, int32_t nthreads) {
+  if (orc_grid_check(g) != 0) return -1;
+  for (size_t d = 0; d < ndrivers; ++d) {
+    const int32_t k = drivers[d].kind;
+    if (k != ORC_DRIVER_TIME && k != ORC_DRIVER_WIENER && k != ORC_DRIVER_JUMP) return -1;
+    if (k == ORC_DRIVER_JUMP && !drivers[d].rate) return -1;  // driver.rs:160
+  }
+  const int64_t nblocks = orc_num_blocks(g);
+  const int64_t ncell = orc_block_len(g, 0);
+  // State::apply_step_with (src/core/state.rs:646-699): per block, the drivers in
+  // order, each over every field it moves (block_apply_driver, 513-544).
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+  for (int64_t b = 0; b < nblocks; ++b) {
+    std::vector<double *> xs(nfields);
+    std::vector<const double *> as(nfields);
+    for (size_t d = 0; d < ndrivers; ++d) {
+      const orc_driver &dr = drivers[d];
+      for (size_t f = 0; f < nfields; ++f) {
+        const int64_t len = orc_block_len(g, ghost[f]);
+        xs[f] = x[f] + b * len;
+        as[f] = dr.amp[f] ? dr.amp[f] + b * len : nullptr;
+      }
+      const uint64_t stream = driver_stream(dr.kind, dr.index);
+      if (dr.kind == ORC_DRIVER_TIME) {
+        for (size_t f = 0; f < nfields; ++f)
+          if (as[f]) orc_axpy(xs[f], as[f], dt, (size_t)orc_block_len(g, ghost[f]));
+      } else if (dr.kind == ORC_DRIVER_WIENER) {
+        for (size_t f = 0; f < nfields; ++f)
+          if (as[f]) wiener_block(g, b, xs[f], as[f], ghost[f], dt, seed, salt, stream);
+      } else {
+        jump_block(g, b, nfields, xs.data(), as.data(), ghost, dr.rate + b * ncell, dt, seed, salt,
+                   stream);
+      }
+    }
+  }
+  return 0;
+}
+
+int orc_jump_diffusion_run(const orc_grid *g, const double *lam, double *w, double *count,
+                           double *jsum, int64_t nsteps, double dt, double *t, uint64_t seed,
+                           double mu, double sigma, double payoff, int32_t nthreads) {
+  if (orc_grid_check(g) != 0) return -1;
+  const size_t n = (size_t)(orc_num_blocks(g) * orc_block_len(g, 0));
+  // The vector fields of the jump-diffusion model (tests/jump.rs:48-86): drift μ
+  // and amplitude σ of w, increments 1 and the payoff of (count, jsum) on a fire,
+  // intensity λ from the static field. Every field has ghost width 0, so every
+  // slab entry is an interior cell.
+  std::vector<double> drift(n, mu), amp(n, sigma), one(n, 1.0), pay(n, payoff);
+  double *x[3] = {w, count, jsum};
+  const uint32_t ghost[3] = {0, 0, 0};
+  const double *time_f[3] = {drift.data(), nullptr, nullptr};
+  const double *wiener_f[3] = {amp.data(), nullptr, nullptr};
+  const double *jump_f[3] = {nullptr, one.data(), pay.data()};
+  const orc_driver drivers[3] = {{ORC_DRIVER_TIME, 0, time_f, nullptr},
+                                 {ORC_DRIVER_WIENER, 0, wiener_f, nullptr},
+                                 {ORC_DRIVER_JUMP, 0, jump_f, lam}};
+  for (int64_t s = 0; s < nsteps; ++s) {
+    uint64_t salt;
+    std::memcpy(&salt, t, sizeof salt);
+    if (orc_apply_drivers(g, 3, x, ghost, 3, drivers, dt, seed, salt, nthreads) != 0) return -1;
+    *t = *t + dt;
+  }
+  return 0;
+}
+
+}  // extern "C"

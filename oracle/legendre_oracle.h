@@ -139,6 +139,32 @@ int orc_heat_rk4_run(const orc_grid *g, double kappa, double *u, int64_t nsteps,
 void orc_initialize_orientations(const orc_grid *g, double *theta, const double *grains,
                                  size_t ngrains);
 
+/* Drivers (src/core/driver.rs:35-50) for the multi-driver explicit update. */
+enum { ORC_DRIVER_TIME = 0, ORC_DRIVER_WIENER = 1, ORC_DRIVER_JUMP = 2 };
+typedef struct {
+  int32_t kind;               /* ORC_DRIVER_* */
+  uint32_t index;             /* j of Wiener(j) / Jump(j) */
+  const double *const *amp;   /* per field: dY/dt, the Wiener amplitude or the jump
+                                 increment; NULL where the driver does not move it */
+  const double *rate;         /* Jump: per-cell intensity in ghost-0 slabs */
+} orc_driver;
+
+/* State::apply_step_with (src/core/state.rs:646-699) over nfields field buffers
+ * (ghost widths ghost[f]): per block, each driver in order through
+ * Driver::apply_slab (src/core/driver.rs:115-182). A WienerJump<W, J> set lists
+ * Time, Wiener(0..W), Jump(0..J). Returns -1 on an unknown driver kind or a jump
+ * driver without an intensity. nthreads > 1 spreads blocks over OpenMP threads. */
+int orc_apply_drivers(const orc_grid *g, size_t nfields, double *const *x, const uint32_t *ghost,
+                      size_t ndrivers, const orc_driver *drivers, double dt, uint64_t seed,
+                      uint64_t salt, int32_t nthreads);
+
+/* The jump-diffusion model of tests/jump.rs:36-86 (WienerJump<1, 1>, ghost-0
+ * fields on a CartesianGrid<1>) stepped by EulerMaruyama: dw = μ dt + σ dW; on a
+ * fire count += 1 and jsum += payoff, at intensity lam (ghost-0 slabs). */
+int orc_jump_diffusion_run(const orc_grid *g, const double *lam, double *w, double *count,
+                           double *jsum, int64_t nsteps, double dt, double *t, uint64_t seed,
+                           double mu, double sigma, double payoff, int32_t nthreads);
+
 /* seed_positions recipe (examples/model_c.rs:175-206), multi-seed branch:
  * writes ngrains [cx, cy, radius] triples. */
 void orc_seed_positions(uint64_t seed, size_t nseeds, double extent, double radius,

@@ -202,12 +202,51 @@ def test_heat_per_intent_with_rule_faces_matches_oracle(ctx, orc):
                           H.field_to_global(want, cells, block, 1))
 
 
-def test_heat_fused_rejects_rule_faces(ctx, orc):
-    say("heat fused rejects Dirichlet")
+RULE_CASES = [
+    # (cells, block, periodic, bcs): every FaceBc kind on a non-periodic face
+    ([16, 16], [8, 8], [False, False], [(1, 0.3), (1, 1.7), (2, 0.1), (3, 0.0)]),
+    ([24, 12], [8, 12], [False, True], [(3, 0.0), (2, -0.4), (0, 0.0), (0, 0.0)]),
+    ([12, 8, 10], [6, 4, 5], [False, True, False], [(2, 0.25), (1, -0.5), (0, 0.0), (0, 0.0),
+                                                    (3, 0.0), (1, 2.0)]),
+    ([2, 9], [2, 3], [False, False], [(3, 0.0), (3, 0.0), (1, 0.5), (2, 1.0)]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(RULE_CASES)))
+def test_heat_fused_with_rule_faces_matches_oracle(ctx, orc, case):
+    """The fused heat step evaluates Dirichlet / Flux / Extrapolate halos in the kernel:
+    bitwise the oracle's fill_ghosts + Laplacian + forward Euler."""
+    cells, block, periodic, bcs = RULE_CASES[case]
+    say(f"heat fused rule faces cells={cells} block={block} bcs={bcs}")
+    h = [0.5] * len(cells)
+    og, gg = grids(ctx, orc, cells, block, h, periodic)
+    kappa = 0.7
+    rng = np.random.default_rng(case)
+    shape = tuple(reversed(cells))
+    u0 = H.global_to_field(rng.random(shape), cells, block, 1)
+    dt = 0.2 * 0.25 / (kappa * len(cells))
+    want = u0.copy()
+    orc.heat_run(og, kappa, want, 40, dt, bcs=bcs)
+    for mode in (lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT):
+        sim = lg.Simulation.heat(gg, kappa, [lg.FaceBc(k, v) for k, v in bcs], mode=mode)
+        sim.field(0).upload(u0)
+        sim.step(dt, 40)
+        assert np.array_equal(H.field_to_global(sim.field(0).download(), cells, block, 1),
+                              H.field_to_global(want, cells, block, 1)), mode
+        sim.close()
+
+
+def test_heat_step_rejects_unknown_rules(ctx, orc):
+    say("heat fused rejects an unknown FaceBc and a one-cell extrapolation")
     _, gg = grids(ctx, orc, [8, 8], [4, 4], [0.5, 0.5])
     with pytest.raises(lg.LegendreError) as e:
-        lg.Simulation.heat(gg, 0.7, [lg.FaceBc.dirichlet(1.0)] * 4, mode=lg.LGD_MODE_FUSED)
-    assert e.value.status == 5
+        lg.Simulation.heat(gg, 0.7, [lg.FaceBc(7, 1.0)] * 4, mode=lg.LGD_MODE_FUSED)
+    assert e.value.status == 1
+    _, g1 = grids(ctx, orc, [8, 1], [4, 1], [0.5, 0.5])
+    with pytest.raises(lg.LegendreError) as e:
+        lg.Simulation.heat(g1, 0.7, [lg.FaceBc(0, 0.0)] * 2 + [lg.FaceBc(3, 0.0)] * 2,
+                           mode=lg.LGD_MODE_FUSED)
+    assert e.value.status == 1
 
 
 # ---------------------------------------------------------------- Model C
