@@ -525,6 +525,112 @@ def run_jump(args) -> int:
     return 0
 
 
+def run_observe(args) -> int:
+    """The observation seam on the headline workload: the 10080² Model C run with a
+    ParquetObserver (phi, u) on an asynchronous ring of 3 pinned buffers, snapshots on
+    step 1 and every 10 steps. The same steps are timed without and with the observer
+    (the solver-side cost: the gather kernel on the simulation's stream; the copy runs
+    on a copy stream and the files on a writer thread); then the writer is drained and
+    its throughput reported. Files go to a temporary directory removed afterwards."""
+    import shutil
+    import tempfile
+
+    import numpy as np
+    import torch
+
+    import legendre_amd as lg
+    from legendre_amd.workloads import ModelCConfig
+
+    C = ModelCConfig(cells=args.cells, block=args.block, noise=args.noise)
+    steps = args.steps
+    stream = torch.cuda.Stream()
+    ctx = lg.Context(0, stream.cuda_stream)
+    grid = lg.CartesianGrid(ctx, [C.cells] * 2, [C.block] * 2, [0.0, 0.0], [C.h, C.h])
+    params = lg.ModelCParams(C.eps4, C.lambda_, C.d_thermal, C.noise)
+    sim = lg.Simulation.model_c(grid, params, wiener=C.noise > 0.0, seed=C.seed)
+    sim.initialize_grains(C.grains(), C.delta)
+    sim.step(C.dt, args.warmup)
+
+    def timed(n):
+        torch.cuda.synchronize()
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(n + 1)]
+        for k in range(n):
+            ev[k].record(stream)
+            sim.step(C.dt, 1)
+        ev[n].record(stream)
+        torch.cuda.synchronize()
+        return [ev[k].elapsed_time(ev[k + 1]) for k in range(n)]
+
+    plain = timed(steps)
+    out = tempfile.mkdtemp(prefix="lgd_observe_")
+    try:
+        obs = lg.ParquetObserver(ctx, out, [("phi", 0), ("u", 1)], every=10, ring=3)
+        sim.attach_observer(obs)
+        t0 = time.perf_counter()
+        observed = timed(steps)
+        solver_s = time.perf_counter() - t0
+        obs.flush()
+        drain_s = time.perf_counter() - t0
+        st = obs.stats()
+        files = sorted(os.listdir(out))
+        nbytes = sum(os.path.getsize(os.path.join(out, f)) for f in files)
+        obs.close()
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+    sim.close()
+    ctx.close()
+    first = args.warmup + steps  # the step index before the observed steps
+    snap_steps = [k for k in range(steps) if first + k + 1 == 1 or (first + k + 1) % 10 == 0]
+    # Column bytes the writer encoded: the static file's x, y, level, patch, then two
+    # fields per snapshot file.
+    col_bytes = 8 * C.interior_cells * (4 + 2 * (st["written"] - 1)) if st["written"] else 0
+    base_ms = float(np.median(plain))
+    snap_ms = [observed[k] for k in snap_steps] if snap_steps else []
+    other_ms = [observed[k] for k in range(steps) if k not in snap_steps]
+    cols = 2
+    line = {
+        "metric": METRIC,
+        "value": round(C.interior_cells * steps / (sum(observed) / 1e3) / 1e9, 4),
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(sum(observed) / steps, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"model_c_{C.cells}sq_observed_every10_ring3", "cells": C.interior_cells,
+                   "fields": ["phi", "u"]},
+        "observe": {
+            "plain_ms_per_step": round(sum(plain) / steps, 5),
+            "observed_ms_per_step": round(sum(observed) / steps, 5),
+            "median_plain_step_ms": round(base_ms, 5),
+            "snapshot_step_ms": [round(x, 4) for x in snap_ms],
+            "median_other_step_ms": round(float(np.median(other_ms)), 5) if other_ms else None,
+            # The gather reads and writes 16 B per cell and field; its time is estimated
+            # as a snapshot step's excess over the median plain step (the static
+            # file's extra columns make the first snapshot step longer).
+            "gather_GBps_est": [round(32 * C.interior_cells / ((x - float(np.median(other_ms))) / 1e3) / 1e9, 1)
+                                for x in snap_ms[1:] if other_ms and x > float(np.median(other_ms)) * 1.2],
+            "snapshots": len(snap_steps),
+            "written_files": st["written"],
+            "dropped": st["dropped"],
+            "failed": st["failed"],
+            "column_bytes_per_snapshot": cols * 8 * C.interior_cells,
+            "file_bytes_total": nbytes,
+            "writer_seconds": round(st["write_seconds"], 3),
+            "writer_GBps": round(col_bytes / st["write_seconds"] / 1e9, 3) if st["write_seconds"] else None,
+            "solver_wall_s": round(solver_s, 3),
+            "drain_wall_s": round(drain_s, 3),
+        },
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_amr(args) -> int:
     """Batched AMR: subcycled Model C on a 3-level Berger–Oliger hierarchy with
     regrids, every patch of a level advanced by one launch per stage, against one
@@ -653,7 +759,7 @@ def run_amr(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["modelc", "adjoint", "ensemble", "amr", "jump"], default="modelc",
+    ap.add_argument("--workload", choices=["modelc", "adjoint", "ensemble", "amr", "jump", "observe"], default="modelc",
                     help="modelc: the headline plain step; adjoint: the calibration gradient; "
                          "ensemble: 512 batched realizations; amr: 3-level subcycled hierarchy")
     ap.add_argument("--jump-steps", type=int, default=0, help="jump workload steps (0 = 100)")
@@ -706,6 +812,8 @@ def main() -> int:
         return run_amr(args)
     if args.workload == "jump":
         return run_jump(args)
+    if args.workload == "observe":
+        return run_observe(args)
 
     from legendre_amd.workloads import ModelCConfig
 

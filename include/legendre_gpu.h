@@ -40,7 +40,9 @@ typedef enum {
   /* configuration the backend does not implement (message names it) */
   LGD_ERR_UNSUPPORTED = 5,
   LGD_ERR_OUT_OF_MEMORY = 6,
-  LGD_ERR_HIP = 7
+  LGD_ERR_HIP = 7,
+  /* a file could not be written (the reference's ObserverError::Io, src/io/mod.rs) */
+  LGD_ERR_IO = 8
 } lgd_status;
 
 typedef struct lgd_ctx lgd_ctx;
@@ -399,6 +401,75 @@ int lgd_sim_set_orientation(lgd_sim *sim, const double *theta0, size_t n);
  * Voronoi, then lgd_sim_set_orientation. */
 int lgd_sim_initialize_grains_oriented(lgd_sim *sim, const double *grains, size_t ngrains,
                                        double u0);
+
+/* ------------------------- observation (src/core/observer.rs, src/io/parquet.rs) */
+typedef struct lgd_observer lgd_observer;
+
+/* An observed field: its Parquet column name and, for lgd_sim_attach_observer, the
+ * simulation field it reads (observed fields: 0 = phi, 1 = u of Model C, 0 = u of
+ * heat; static fields: 0 = theta0 of a multi-grain Model C). */
+typedef struct {
+  const char *name;
+  int32_t field;
+} lgd_observed_field;
+
+/* ParquetObserver::new(grid, fields, every, dir).with_static(statics) fed by an
+ * AsyncObserver whose snapshot ring holds `ring` >= 1 buffers (src/io/parquet.rs:63-130,
+ * src/core/observer.rs:75-135), plus one buffer staging the static file of a grid
+ * epoch.  Each buffer is device staging plus pinned host memory, sized on
+ * lgd_sim_attach_observer (or on first use) and grown only when an AMR epoch has more
+ * rows.  Snapshots fire on step 1 and every `every` steps.  Files:
+ * dir/static_<epoch:07>.parquet (x, y[, z], level, patch, static fields) once per
+ * grid epoch and dir/snap_<step:07>.parquet (step, t, epoch, fields), rows block-major
+ * over interior cells with dimension 0 fastest, row groups of whole blocks cut at
+ * 2^22 rows, snappy pages (parquet.rs:1-35, 134-311).  Creates `dir`.  An empty field
+ * list is LGD_ERR_INVALID_ARGUMENT (parquet.rs:94-96). */
+int lgd_observer_create(lgd_ctx *ctx, const char *dir, uint64_t every, size_t ring,
+                        const lgd_observed_field *fields, size_t nfields,
+                        const lgd_observed_field *statics, size_t nstatics, lgd_observer **out);
+/* Writes every queued snapshot, stops the writer thread and frees the ring. */
+int lgd_observer_destroy(lgd_observer *obs);
+/* Observer::observe on a Cartesian grid (observer.rs:41-45, 156-176): `fields` and
+ * `statics` are device buffers in declaration order with their ghost widths.  At a
+ * snapshot step the interiors are gathered on the context's stream and copied to a
+ * pinned ring buffer on a copy stream; the call never waits for the disk.  With every
+ * ring buffer in flight the snapshot is dropped and counted.  `epoch` names the grid:
+ * a new epoch (or grid) writes a new static file before its first snapshot. */
+int lgd_observer_observe(lgd_observer *obs, const lgd_grid *grid, uint64_t step, double t,
+                         uint64_t epoch, const lgd_buf *const *fields, const uint32_t *field_ghost,
+                         const lgd_buf *const *statics, const uint32_t *static_ghost);
+/* The same on an AMR hierarchy: ghost-1 fields in patch order; the static file holds
+ * the AmrGrid cell centres, each cell's level and patch (parquet.rs:216-235). */
+int lgd_observer_observe_amr(lgd_observer *obs, const lgd_amr *amr, uint64_t step, double t,
+                             uint64_t epoch, const lgd_buf *const *fields);
+/* Blocks until every queued snapshot is on disk (resumes a paused writer). */
+int lgd_observer_flush(lgd_observer *obs);
+/* Blocking mode (1): with every ring buffer in flight observe() waits for the writer
+ * instead of dropping the snapshot — the synchronous ParquetObserver's guarantee that
+ * every snapshot step is written (parquet.rs:318-333).  Default 0 (AsyncObserver). */
+int lgd_observer_set_blocking(lgd_observer *obs, int32_t blocking);
+/* Holds (1) or releases (0) the writer thread: held, snapshots queue until the ring
+ * is full and are then dropped (also in blocking mode). */
+int lgd_observer_pause(lgd_observer *obs, int32_t paused);
+/* Files written, snapshots dropped, failed writes, seconds spent writing. */
+int lgd_observer_stats(lgd_observer *obs, uint64_t *written, uint64_t *dropped, uint64_t *failed,
+                       double *write_seconds);
+/* The last failed write's message (empty if none), truncated to len bytes. */
+int lgd_observer_last_error(lgd_observer *obs, char *buf, size_t len);
+/* Simulation::attach_observer (simulation.rs:204-208): after every completed step the
+ * simulation calls the observer with its step index, time, grid epoch (0, +1 per
+ * regrid that changes the hierarchy) and fields.  Graph-replayed runs end a graph at
+ * each snapshot step.  The observer must outlive the simulation's stepping. */
+int lgd_sim_attach_observer(lgd_sim *sim, lgd_observer *obs);
+
+/* Writes host columns as one Parquet file in the snapshot layout: types[c] 0 = DOUBLE,
+ * 1 = INT64; columns[c] holds nrows values, or is NULL with constants[c] holding the
+ * bits every row shares; row groups of group_rows[0..ngroups) rows (summing to
+ * nrows).  LGD_ERR_IO when the file cannot be written (message in err). */
+int lgd_parquet_write_file(const char *path, size_t ncols, const char *const *names,
+                           const int32_t *types, const void *const *columns,
+                           const uint64_t *constants, int64_t nrows, const int64_t *group_rows,
+                           size_t ngroups, char *err, size_t errlen);
 
 /* -------------------------------------------------------------- Model C adjoint */
 /* The reverse-mode derivative of the explicit Model C step (Euler–Maruyama with

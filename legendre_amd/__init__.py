@@ -50,6 +50,8 @@ __all__ = [
     "LGD_DRIVER_WIENER",
     "LGD_DRIVER_JUMP",
     "Driver",
+    "ParquetObserver",
+    "write_parquet",
 ]
 
 # Karma–Rappel thin-interface constant a₂ (src/physics/phasefield.rs:44).
@@ -504,6 +506,11 @@ class Simulation:
         _abi.check(self._lib.lgd_sim_initialize_grains_oriented(self.handle, g.ctypes.data_as(_abi.DP),
                                                                 g.shape[0], u0), self.ctx.handle)
 
+    def attach_observer(self, observer: "ParquetObserver") -> None:
+        """Simulation::attach_observer: the observer sees every completed step."""
+        _abi.check(self._lib.lgd_sim_attach_observer(self.handle, observer.handle), self.ctx.handle)
+        self._observers = getattr(self, "_observers", []) + [observer]
+
     def set_graph_steps(self, steps: int) -> None:
         """Run `steps` whole steps at a time as one replayed HIP graph (0: eager
         launches).  Results are bitwise those of eager steps."""
@@ -534,6 +541,121 @@ class Simulation:
             self.close()
         except Exception:
             pass
+
+
+class ParquetObserver:
+    """ParquetObserver behind an AsyncObserver snapshot ring (src/io/parquet.rs,
+    src/core/observer.rs): fields = [(column name, simulation field)], statics the
+    same for per-epoch static columns; snapshots on step 1 and every `every` steps."""
+
+    def __init__(self, ctx: Context, directory: str, fields, every: int = 1, ring: int = 2,
+                 statics=(), blocking: bool = False):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.directory = str(directory)
+        self.names = [n for n, _ in fields]
+        self.static_names = [n for n, _ in statics]
+        self._keep = [n.encode() for n, _ in fields] + [n.encode() for n, _ in statics]
+        farr = (_abi.ObservedFieldC * len(fields))(*[_abi.ObservedFieldC(n.encode(), int(i))
+                                                      for n, i in fields])
+        sarr = (_abi.ObservedFieldC * max(1, len(statics)))(*[_abi.ObservedFieldC(n.encode(), int(i))
+                                                              for n, i in statics])
+        h = C.c_void_p()
+        _abi.check(self._lib.lgd_observer_create(ctx.handle, self.directory.encode(), int(every),
+                                                 int(ring), farr, len(fields), sarr, len(statics),
+                                                 C.byref(h)), ctx.handle)
+        self.handle = h
+        ctx._adopt(self)
+        if blocking:
+            _abi.check(self._lib.lgd_observer_set_blocking(h, 1), ctx.handle)
+
+    def observe(self, grid: "CartesianGrid", step: int, t: float, epoch: int, fields, ghosts=None,
+                statics=(), static_ghosts=None) -> None:
+        """Observer::observe with device buffers in declaration order."""
+        nf, ns = len(fields), len(statics)
+        ghosts = list(ghosts) if ghosts is not None else [1] * nf
+        static_ghosts = list(static_ghosts) if static_ghosts is not None else [0] * ns
+        f = (C.c_void_p * max(1, nf))(*[b.handle for b in fields])
+        g = (C.c_uint32 * max(1, nf))(*ghosts)
+        sf = (C.c_void_p * max(1, ns))(*[b.handle for b in statics])
+        sg = (C.c_uint32 * max(1, ns))(*static_ghosts)
+        _abi.check(self._lib.lgd_observer_observe(self.handle, grid.handle, int(step), float(t), int(epoch),
+                                                  C.cast(f, _abi.PP), g, C.cast(sf, _abi.PP), sg),
+                   self.ctx.handle)
+
+    def observe_amr(self, amr: "AmrGrid", step: int, t: float, epoch: int, fields) -> None:
+        f = (C.c_void_p * max(1, len(fields)))(*[b.handle for b in fields])
+        _abi.check(self._lib.lgd_observer_observe_amr(self.handle, amr.handle, int(step), float(t),
+                                                      int(epoch), C.cast(f, _abi.PP)), self.ctx.handle)
+
+    def flush(self) -> None:
+        _abi.check(self._lib.lgd_observer_flush(self.handle), self.ctx.handle)
+
+    def pause(self, paused: bool = True) -> None:
+        _abi.check(self._lib.lgd_observer_pause(self.handle, int(paused)), self.ctx.handle)
+
+    def stats(self) -> dict:
+        w, d, f = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        sec = C.c_double()
+        _abi.check(self._lib.lgd_observer_stats(self.handle, C.byref(w), C.byref(d), C.byref(f),
+                                                C.byref(sec)), self.ctx.handle)
+        buf = C.create_string_buffer(512)
+        self._lib.lgd_observer_last_error(self.handle, buf, 512)
+        return {"written": w.value, "dropped": d.value, "failed": f.value, "write_seconds": sec.value,
+                "last_error": buf.value.decode()}
+
+    def close(self) -> None:
+        if self.handle:
+            self._lib.lgd_observer_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def write_parquet(path: str, columns, group_rows=None) -> None:
+    """Host columns (name -> float64 / int64 array, or a Python float / int for a
+    constant column) as one Parquet file in the snapshot layout."""
+    names = list(columns)
+    arrays, types, consts = [], [], []
+    nrows = None
+    for n in names:
+        v = columns[n]
+        if isinstance(v, (int, np.integer)) and not isinstance(v, bool):
+            types.append(1)
+            consts.append(int(v) & ((1 << 64) - 1))
+            arrays.append(None)
+        elif isinstance(v, (float, np.floating)):
+            types.append(0)
+            consts.append(struct.unpack("<Q", struct.pack("<d", float(v)))[0])
+            arrays.append(None)
+        else:
+            a = np.ascontiguousarray(v)
+            if a.dtype not in (np.float64, np.int64):
+                raise ValueError(f"column {n}: float64 or int64 expected")
+            types.append(0 if a.dtype == np.float64 else 1)
+            consts.append(0)
+            arrays.append(a)
+            nrows = a.size if nrows is None else nrows
+            if a.size != nrows:
+                raise ValueError("columns differ in length")
+    nrows = nrows or 0
+    groups = list(group_rows) if group_rows is not None else ([nrows] if nrows else [])
+    lib = _abi.lib()
+    nc = len(names)
+    name_arr = (C.c_char_p * nc)(*[n.encode() for n in names])
+    type_arr = (C.c_int32 * nc)(*types)
+    col_arr = (C.c_void_p * nc)(*[a.ctypes.data if a is not None else None for a in arrays])
+    const_arr = (C.c_uint64 * nc)(*consts)
+    g_arr = (C.c_int64 * max(1, len(groups)))(*groups)
+    err = C.create_string_buffer(512)
+    st = lib.lgd_parquet_write_file(str(path).encode(), nc, name_arr, type_arr, C.cast(col_arr, _abi.PP),
+                                    const_arr, nrows, g_arr, len(groups), err, 512)
+    if st != 0:
+        raise LegendreError(st, err.value.decode())
 
 
 @dataclasses.dataclass(frozen=True)

@@ -21,6 +21,7 @@ STATUS_NAMES = {
     5: "LGD_ERR_UNSUPPORTED",
     6: "LGD_ERR_OUT_OF_MEMORY",
     7: "LGD_ERR_HIP",
+    8: "LGD_ERR_IO",
 }
 
 LGD_BC_MIRROR, LGD_BC_DIRICHLET, LGD_BC_FLUX, LGD_BC_EXTRAPOLATE = 0, 1, 2, 3
@@ -90,6 +91,10 @@ class TuningResultC(C.Structure):
 
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
+
+
+class ObservedFieldC(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("field", C.c_int32)]
 
 
 class DriverC(C.Structure):
@@ -171,6 +176,19 @@ SIGNATURES = {
     "lgd_sim_set_integrator": (C.c_int, [P, C.c_int32]),
     "lgd_sim_set_orientation": (C.c_int, [P, DP, SZ]),
     "lgd_sim_initialize_grains_oriented": (C.c_int, [P, DP, SZ, DBL]),
+    "lgd_observer_create": (C.c_int, [P, C.c_char_p, U64, SZ, C.POINTER(ObservedFieldC), SZ,
+                                      C.POINTER(ObservedFieldC), SZ, PP]),
+    "lgd_observer_destroy": (C.c_int, [P]),
+    "lgd_observer_observe": (C.c_int, [P, P, U64, DBL, U64, PP, C.POINTER(U32), PP, C.POINTER(U32)]),
+    "lgd_observer_observe_amr": (C.c_int, [P, P, U64, DBL, U64, PP]),
+    "lgd_observer_flush": (C.c_int, [P]),
+    "lgd_observer_pause": (C.c_int, [P, C.c_int32]),
+    "lgd_observer_set_blocking": (C.c_int, [P, C.c_int32]),
+    "lgd_observer_stats": (C.c_int, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64), DP]),
+    "lgd_observer_last_error": (C.c_int, [P, C.c_char_p, SZ]),
+    "lgd_sim_attach_observer": (C.c_int, [P, P]),
+    "lgd_parquet_write_file": (C.c_int, [C.c_char_p, SZ, C.POINTER(C.c_char_p), C.POINTER(C.c_int32),
+                                         PP, C.POINTER(U64), I64, C.POINTER(I64), SZ, C.c_char_p, SZ]),
     "lgd_adjoint_create": (C.c_int, [P, P, PP]),
     "lgd_adjoint_destroy": (C.c_int, [P]),
     "lgd_modelc_adjoint_step": (C.c_int, [P, C.POINTER(ModelCParamsC), P, P, DBL, P, P]),

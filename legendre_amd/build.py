@@ -42,8 +42,15 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP backend cannot be built")
 
 
+# Host-only sources (*.cpp: the Parquet writer) build with the host compiler.
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-pthread"]
+
+
 def _compile(src: pathlib.Path, obj: pathlib.Path) -> None:
-    cmd = [hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
+    if src.suffix == ".cpp":
+        cmd = [os.environ.get("CXX", "g++"), *CXX_FLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [hipcc(), *HIPCC_FLAGS, "-c", str(src), "-o", str(obj)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"hipcc failed on {src.name}:\n{res.stderr}")
@@ -58,12 +65,13 @@ def _compile(src: pathlib.Path, obj: pathlib.Path) -> None:
 def build(verbose: bool = False) -> pathlib.Path:
     objdir = HERE / "_build"
     objdir.mkdir(exist_ok=True)
-    sources = sorted(CSRC.glob("*.hip"))
-    objs = [objdir / (s.stem + ".o") for s in sources]
+    sources = sorted(CSRC.glob("*.hip")) + sorted(CSRC.glob("*.cpp"))
+    objs = [objdir / (s.name + ".o") for s in sources]
     with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         list(ex.map(lambda so: _compile(*so), zip(sources, objs)))
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", *map(str, objs), "-o",
+           str(tmp)]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

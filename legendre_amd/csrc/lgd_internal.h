@@ -8,6 +8,7 @@
 #include <map>
 #include <string>
 #include <tuple>
+#include <vector>
 
 #include "../../include/legendre_gpu.h"
 
@@ -166,6 +167,17 @@ struct lgd_domain {
 
 // The fused Model C step (lgd_modelc_step) with an optional orientation: cs0, sn0 =
 // (cos θ₀, sin θ₀) per cell in ghost-0 slabs (null: axis-aligned).
+// The snapshot observer (lgd_observer.hip) as the simulation drives it: whether
+// `step` snapshots, steps from `step` to the next snapshot step, and the simulation
+// fields the observer reads.
+namespace lgd {
+bool observer_due(const lgd_observer *o, uint64_t step);
+int64_t observer_steps_to_due(const lgd_observer *o, uint64_t step);
+const std::vector<int32_t> &observer_fields(const lgd_observer *o);
+const std::vector<int32_t> &observer_statics(const lgd_observer *o);
+int observer_prepare(lgd_observer *o, const lgd_grid *grid, const lgd_amr *amr, uint64_t epoch);
+}  // namespace lgd
+
 int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
                          const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
                          double dt, uint64_t seed, uint64_t salt, int32_t wiener,

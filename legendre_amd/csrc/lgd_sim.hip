@@ -49,8 +49,10 @@ struct lgd_sim {
   // Integrator (LGD_INTEGRATOR_*); RK4 stages: k1..k4 then y_tmp, one buffer per field each.
   int32_t integrator = LGD_INTEGRATOR_EULER;
   std::vector<lgd_buf *> rk;
-  // Multi-grain Model C: θ₀ per cell (ghost-0 slabs) as cos θ₀ and sin θ₀.
-  lgd_buf *cos0 = nullptr, *sin0 = nullptr;
+  // Multi-grain Model C: θ₀ per cell (ghost-0 slabs) as cos θ₀ and sin θ₀, and θ₀
+  // itself for observers.
+  lgd_buf *cos0 = nullptr, *sin0 = nullptr, *theta0 = nullptr;
+  std::vector<lgd_observer *> observers;  // attached, not owned
   double t = 0.0;
   int64_t step_index = 0;
   // Graphs.
@@ -123,7 +125,8 @@ void free_all(lgd_sim *s) {
   s->rk.clear();
   lgd_buf_free(s->cos0);
   lgd_buf_free(s->sin0);
-  s->cos0 = s->sin0 = nullptr;
+  lgd_buf_free(s->theta0);
+  s->cos0 = s->sin0 = s->theta0 = nullptr;
   free_stages(s);
 }
 
@@ -650,6 +653,34 @@ int graph_steps(lgd_sim *s, double dt, int64_t n) {
   return LGD_OK;
 }
 
+// Simulation::step's observer loop (simulation.rs:286-289): every attached observer
+// sees the completed step; only snapshot steps touch the device.
+int notify_observers(lgd_sim *s) {
+  int rc;
+  for (auto *o : s->observers) {
+    if (!observer_due(o, (uint64_t)s->step_index)) continue;
+    std::vector<const lgd_buf *> f(observer_fields(o).size());
+    if (s->amr) {
+      if ((rc = amr_consolidate(s))) return rc;
+      for (size_t c = 0; c < f.size(); ++c) f[c] = s->state[(size_t)observer_fields(o)[c]];
+      rc = lgd_observer_observe_amr(o, s->amr, (uint64_t)s->step_index, s->t,
+                                    (uint64_t)s->amr_epoch, f.data());
+    } else {
+      std::vector<uint32_t> fg(f.size(), 1u);
+      for (size_t c = 0; c < f.size(); ++c) f[c] = s->state[(size_t)observer_fields(o)[c]];
+      const size_t ns = observer_statics(o).size();
+      if (ns && !s->theta0)
+        return fail(s->ctx, LGD_ERR_INVALID_ARGUMENT, "theta0 is observed but no orientation is set");
+      std::vector<const lgd_buf *> st(ns, s->theta0);
+      std::vector<uint32_t> sg(ns, 0u);
+      rc = lgd_observer_observe(o, s->grid, (uint64_t)s->step_index, s->t, 0, f.data(), fg.data(),
+                                st.data(), sg.data());
+    }
+    if (rc) return rc;
+  }
+  return LGD_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -799,8 +830,11 @@ int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps) {
       int64_t k = std::min(sim->graph_steps, nsteps - n);
       if (sim->amr && sim->adaptive)  // no regrid inside a graph
         k = std::min(k, sim->policy.every - sim->step_index % sim->policy.every);
+      for (auto *o : sim->observers)  // a graph ends at each snapshot step
+        k = std::min(k, observer_steps_to_due(o, (uint64_t)sim->step_index));
       if ((rc = graph_steps(sim, dt, k))) return rc;
       n += k;
+      if ((rc = notify_observers(sim))) return rc;
       continue;
     }
     uint64_t salt;
@@ -808,7 +842,26 @@ int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps) {
     if ((rc = step_body(sim, dt, salt))) return rc;
     step_done(sim, dt);
     ++n;
+    if ((rc = notify_observers(sim))) return rc;
   }
+  return LGD_OK;
+}
+
+int lgd_sim_attach_observer(lgd_sim *sim, lgd_observer *obs) {
+  if (!sim || !obs) return fail(sim ? sim->ctx : nullptr, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  for (int32_t f : observer_fields(obs))
+    if (f < 0 || f >= (int32_t)sim->state.size())
+      return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT, "observed field index out of range");
+  for (int32_t f : observer_statics(obs)) {
+    if (f != 0 || sim->kind != lgd_sim::kModelC || sim->amr)
+      return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT,
+                  "the only static field is theta0 (field 0) of a Cartesian Model C");
+  }
+  // Size the observer's buffers for this grid now: snapshots then allocate nothing.
+  int rc = observer_prepare(obs, sim->amr ? nullptr : sim->grid, sim->amr,
+                            sim->amr ? (uint64_t)sim->amr_epoch : 0);
+  if (rc) return rc;
+  sim->observers.push_back(obs);
   return LGD_OK;
 }
 
@@ -922,6 +975,12 @@ int lgd_sim_set_orientation(lgd_sim *sim, const double *theta0, size_t n) {
   if (!sim->cos0) {
     if ((rc = lgd_buf_alloc(sim->ctx, len, &sim->cos0))) return rc;
     if ((rc = lgd_buf_alloc(sim->ctx, len, &sim->sin0))) return rc;
+    if ((rc = lgd_buf_alloc(sim->ctx, len, &sim->theta0))) return rc;
+  }
+  if (theta0) {
+    if ((rc = lgd_buf_upload(sim->ctx, sim->theta0, 0, theta0, len))) return rc;
+  } else if ((rc = lgd_buf_fill_zero(sim->ctx, sim->theta0))) {
+    return rc;
   }
   // (sin θ₀, cos θ₀) with libm, as the reference's per-cell sin_cos (phasefield.rs:382).
   std::vector<double> cs(len), sn(len);
