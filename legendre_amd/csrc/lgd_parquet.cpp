@@ -181,7 +181,9 @@ void compress_block(const uint8_t *src, size_t n, std::vector<uint16_t> &table, 
 }
 
 // ------------------------------------------------------------ column encoding
-constexpr size_t kMaxDict = 1 << 17;       // 1 MiB of 8-byte dictionary values
+// A chunk with more distinct values than this is written PLAIN: Snappy then takes
+// the runs, and wide dictionary indices would cost more to pack than they save.
+constexpr size_t kMaxDict = 1 << 16;
 constexpr size_t kPlainPageValues = 1 << 17;  // 1 MiB PLAIN data pages
 
 struct Encoded {
@@ -194,15 +196,15 @@ struct Encoded {
   uint64_t min_bits = 0, max_bits = 0;
 };
 
-void page(Bytes &out, int32_t type, const Bytes &body, int32_t num_values, int32_t encoding,
-          int64_t &uncompressed) {
-  Bytes comp;
-  snappy_compress(body.data(), body.size(), comp);
+void page(Bytes &out, int32_t type, const uint8_t *body, size_t len, int32_t num_values,
+          int32_t encoding, int64_t &uncompressed) {
+  thread_local Bytes comp;  // reused across pages
+  snappy_compress(body, len, comp);
   Bytes hdr;
   Thrift t(hdr);
   t.begin();
   t.i32(1, type);
-  t.i32(2, (int32_t)body.size());
+  t.i32(2, (int32_t)len);
   t.i32(3, (int32_t)comp.size());
   if (type == kDictionaryPage) {
     t.struct_begin(7);
@@ -220,56 +222,59 @@ void page(Bytes &out, int32_t type, const Bytes &body, int32_t num_values, int32
   t.end();
   out.insert(out.end(), hdr.begin(), hdr.end());
   out.insert(out.end(), comp.begin(), comp.end());
-  uncompressed += (int64_t)(hdr.size() + body.size());
+  uncompressed += (int64_t)(hdr.size() + len);
 }
 
 // The RLE / bit-packed hybrid of `n` indices at `bw` bits: runs of >= 8 equal
 // values as RLE runs, everything else bit-packed in groups of 8 (LSB first).
 void rle_hybrid(const uint32_t *v, size_t n, int bw, Bytes &o) {
-  std::vector<uint32_t> pend;
   const size_t vbytes = (size_t)(bw + 7) / 8;
-  auto flush_packed = [&]() {
-    if (pend.empty()) return;
-    const size_t groups = (pend.size() + 7) / 8;
-    pend.resize(groups * 8, 0);
+  // Bit-pack v[from, to), zero-padded to whole groups of 8.
+  auto pack = [&](size_t from, size_t to) {
+    if (to == from) return;
+    const size_t groups = (to - from + 7) / 8;
     put_varint(o, (groups << 1) | 1);
+    const size_t base = o.size();
+    o.resize(base + groups * (size_t)bw, 0);  // 8 values of bw bits = bw bytes
+    uint8_t *out = o.data() + base;
     uint64_t acc = 0;
     int nb = 0;
-    for (uint32_t x : pend) {
-      acc |= (uint64_t)x << nb;
+    size_t w = 0;
+    for (size_t k = from; k < from + groups * 8; ++k) {
+      acc |= (uint64_t)(k < to ? v[k] : 0u) << nb;
       nb += bw;
       while (nb >= 8) {
-        o.push_back((uint8_t)acc);
+        out[w++] = (uint8_t)acc;
         acc >>= 8;
         nb -= 8;
       }
     }
-    if (nb > 0) o.push_back((uint8_t)acc);  // groups of 8 fill whole bytes; defensive
-    pend.clear();
   };
-  size_t i = 0;
+  size_t i = 0, lit = 0;  // v[lit, i) waits to be bit-packed
   while (i < n) {
     size_t j = i + 1;
     while (j < n && v[j] == v[i]) ++j;
     size_t run = j - i;
     if (run >= 8) {
-      const size_t top = (8 - pend.size() % 8) % 8;  // complete the pending group
-      for (size_t k = 0; k < top; ++k) pend.push_back(v[i]);
+      const size_t top = (8 - (i - lit) % 8) % 8;  // complete the pending group
       i += top;
       run -= top;
       if (run >= 8) {
-        flush_packed();
+        pack(lit, i);
         put_varint(o, (uint64_t)run << 1);
         for (size_t k = 0; k < vbytes; ++k) o.push_back((uint8_t)(v[i] >> (8 * k)));
         i += run;
+        lit = i;
       }
       continue;
     }
-    for (size_t k = 0; k < run; ++k) pend.push_back(v[i + k]);
     i = j;
-    if (pend.size() >= 4096 && pend.size() % 8 == 0) flush_packed();
+    if (i - lit >= 4096 && (i - lit) % 8 == 0) {
+      pack(lit, i);
+      lit = i;
+    }
   }
-  flush_packed();
+  pack(lit, n);
 }
 
 int bit_width(size_t max_index) {
@@ -319,6 +324,40 @@ void update_stats(Type type, uint64_t bits, Encoded &e) {
   e.has_stats = true;
 }
 
+// Min / max of n values (NaN skipped), merged into e. Zero signs are settled when
+// the statistics are written.
+void range_stats(Type type, const uint8_t *p, size_t n, Encoded &e) {
+  if (type == Type::kDouble) {
+    double lo = HUGE_VAL, hi = -HUGE_VAL;
+    bool any = false;
+    for (size_t i = 0; i < n; ++i) {
+      double x;
+      std::memcpy(&x, p + 8 * i, 8);
+      if (x != x) continue;
+      lo = x < lo ? x : lo;
+      hi = x > hi ? x : hi;
+      any = true;
+    }
+    if (!any) return;
+    uint64_t lb, hb;
+    std::memcpy(&lb, &lo, 8);
+    std::memcpy(&hb, &hi, 8);
+    update_stats(type, lb, e);
+    update_stats(type, hb, e);
+  } else {
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (size_t i = 0; i < n; ++i) {
+      int64_t x;
+      std::memcpy(&x, p + 8 * i, 8);
+      lo = x < lo ? x : lo;
+      hi = x > hi ? x : hi;
+    }
+    if (n == 0) return;
+    update_stats(type, (uint64_t)lo, e);
+    update_stats(type, (uint64_t)hi, e);
+  }
+}
+
 // One column chunk: rows [r0, r0 + n) of `c`.
 void encode_chunk(const Column &c, int64_t r0, int64_t n, Encoded &e) {
   auto value = [&](int64_t r) -> uint64_t {
@@ -327,6 +366,7 @@ void encode_chunk(const Column &c, int64_t r0, int64_t n, Encoded &e) {
     std::memcpy(&b, (const uint8_t *)c.data + 8 * (r0 + r), 8);
     return b;
   };
+  e.bytes.reserve((size_t)n * 8 / 2 + 4096);  // address space only until written
   std::vector<uint32_t> idx;
   Dict d;
   bool dict = true;
@@ -334,20 +374,30 @@ void encode_chunk(const Column &c, int64_t r0, int64_t n, Encoded &e) {
     d.index(c.constant);  // one dictionary entry, every index 0
     idx.assign((size_t)n, 0u);
   } else {
-    idx.resize((size_t)n);
-    for (int64_t r = 0; r < n && dict; ++r) {
-      const int32_t k = d.index(value(r));
-      if (k < 0) dict = false;
-      idx[(size_t)r] = (uint32_t)k;
+    uint64_t prev = 0;
+    int32_t prev_k = -1;
+    for (int64_t r = 0; r < n; ++r) {
+      const uint64_t v = value(r);
+      if (prev_k >= 0 && v == prev) {  // runs of one value skip the hash
+        idx.push_back((uint32_t)prev_k);
+        continue;
+      }
+      const int32_t k = d.index(v);
+      if (k < 0) {
+        dict = false;
+        break;
+      }
+      idx.push_back((uint32_t)k);
+      prev = v;
+      prev_k = k;
     }
   }
   e.dict = dict;
   if (dict) {
-    for (uint64_t v : d.values) update_stats(c.type, v, e);
-    Bytes body(d.values.size() * 8);
-    std::memcpy(body.data(), d.values.data(), body.size());
+    range_stats(c.type, (const uint8_t *)d.values.data(), d.values.size(), e);
     e.dict_offset = 0;
-    page(e.bytes, kDictionaryPage, body, (int32_t)d.values.size(), kPlain, e.uncompressed);
+    page(e.bytes, kDictionaryPage, (const uint8_t *)d.values.data(), d.values.size() * 8,
+         (int32_t)d.values.size(), kPlain, e.uncompressed);
     e.data_offset = (int64_t)e.bytes.size();
     const int bw = bit_width(d.values.empty() ? 0 : d.values.size() - 1);
     size_t per_page = std::max<size_t>(8192, ((size_t)1 << 23) / (size_t)std::max(bw, 1));
@@ -357,19 +407,17 @@ void encode_chunk(const Column &c, int64_t r0, int64_t n, Encoded &e) {
       Bytes b;
       b.push_back((uint8_t)bw);
       rle_hybrid(idx.data() + p, m, bw, b);
-      page(e.bytes, kDataPage, b, (int32_t)m, kRleDictionary, e.uncompressed);
+      page(e.bytes, kDataPage, b.data(), b.size(), (int32_t)m, kRleDictionary, e.uncompressed);
     }
   } else {
+    // PLAIN pages straight from the column (a constant column always takes the
+    // dictionary path, so data is set here).
     e.data_offset = 0;
+    const uint8_t *src = (const uint8_t *)c.data + 8 * r0;
+    range_stats(c.type, src, (size_t)n, e);
     for (int64_t p = 0; p < n; p += (int64_t)kPlainPageValues) {
       const size_t m = (size_t)std::min<int64_t>((int64_t)kPlainPageValues, n - p);
-      Bytes b(m * 8);
-      for (size_t k = 0; k < m; ++k) {
-        const uint64_t v = value(p + (int64_t)k);
-        update_stats(c.type, v, e);
-        std::memcpy(b.data() + 8 * k, &v, 8);
-      }
-      page(e.bytes, kDataPage, b, (int32_t)m, kPlain, e.uncompressed);
+      page(e.bytes, kDataPage, src + 8 * p, m * 8, (int32_t)m, kPlain, e.uncompressed);
     }
   }
 }
@@ -436,36 +484,49 @@ Status write_file(const std::string &path, const std::vector<Column> &cols, int6
   std::vector<GroupMeta> groups;
   const size_t ncol = cols.size();
   const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  int64_t r0 = 0;
+  std::vector<int64_t> gr, g0;  // non-empty row groups and their first rows
   for (int64_t rows : group_rows) {
-    if (rows == 0) continue;
-    // Encode the row group's column chunks in parallel, then write them in order.
-    std::vector<Encoded> enc(ncol);
+    if (rows > 0) {
+      g0.push_back(gr.empty() ? 0 : g0.back() + gr.back());
+      gr.push_back(rows);
+    }
+  }
+  // Column chunks are encoded in parallel over a window of row groups (bounding the
+  // encoded bytes held at once), then written in file order.
+  const size_t window = std::max<size_t>(1, hw / ncol);
+  for (size_t w0 = 0; w0 < gr.size(); w0 += window) {
+    const size_t w1 = std::min(gr.size(), w0 + window);
+    const size_t jobs = (w1 - w0) * ncol;
+    std::vector<Encoded> enc(jobs);
     std::vector<std::thread> pool;
-    const unsigned nt = std::min<unsigned>(hw, (unsigned)ncol);
-    for (unsigned w = 0; w < nt; ++w)
-      pool.emplace_back([&, w]() {
-        for (size_t c = w; c < ncol; c += nt) encode_chunk(cols[c], r0, rows, enc[c]);
+    const unsigned nt = (unsigned)std::min<size_t>(hw, jobs);
+    for (unsigned t0 = 0; t0 < nt; ++t0)
+      pool.emplace_back([&, t0]() {
+        for (size_t j = t0; j < jobs; j += nt) {
+          const size_t g = w0 + j / ncol, c = j % ncol;
+          encode_chunk(cols[c], g0[g], gr[g], enc[j]);
+        }
       });
     for (auto &th : pool) th.join();
-    GroupMeta gm{rows, pos, {}};
-    for (size_t c = 0; c < ncol; ++c) {
-      const Encoded &e = enc[c];
-      ChunkMeta m;
-      m.offset = pos;
-      m.dict = e.dict;
-      m.dict_offset = e.dict ? pos + e.dict_offset : -1;
-      m.data_offset = pos + e.data_offset;
-      m.uncompressed = e.uncompressed;
-      m.compressed = (int64_t)e.bytes.size();
-      m.has_stats = e.has_stats;
-      m.lo = e.min_bits;
-      m.hi = e.max_bits;
-      put(e.bytes.data(), e.bytes.size());
-      gm.chunks.push_back(m);
+    for (size_t g = w0; g < w1; ++g) {
+      GroupMeta gm{gr[g], pos, {}};
+      for (size_t c = 0; c < ncol; ++c) {
+        const Encoded &e = enc[(g - w0) * ncol + c];
+        ChunkMeta m;
+        m.offset = pos;
+        m.dict = e.dict;
+        m.dict_offset = e.dict ? pos + e.dict_offset : -1;
+        m.data_offset = pos + e.data_offset;
+        m.uncompressed = e.uncompressed;
+        m.compressed = (int64_t)e.bytes.size();
+        m.has_stats = e.has_stats;
+        m.lo = e.min_bits;
+        m.hi = e.max_bits;
+        put(e.bytes.data(), e.bytes.size());
+        gm.chunks.push_back(m);
+      }
+      groups.push_back(gm);
     }
-    groups.push_back(gm);
-    r0 += rows;
   }
 
   // FileMetaData.
