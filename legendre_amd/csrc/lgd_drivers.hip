@@ -38,7 +38,10 @@ struct DriverArgs {
   const double *salt_src;
 };
 
-__global__ __launch_bounds__(kWG) void drivers_kernel(DriverArgs a) {
+// Five waves per SIMD fit in 96 VGPRs without spills; the kernel is latency-bound
+// between its loads and its transcendental chains, so the extra wave pays.
+__global__ __launch_bounds__(kWG) __attribute__((amdgpu_waves_per_eu(5))) void drivers_kernel(
+    DriverArgs a) {
   const uint32_t b = blockIdx.x / a.tiles;
   int64_t r = (int64_t)(blockIdx.x - b * a.tiles) * kWG + threadIdx.x;
   if (r >= a.slab) return;

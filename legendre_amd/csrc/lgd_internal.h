@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/legendre_gpu.h"
+#include "lgd_trig.h"
 
 // A launch shape of the fused Model C step: kernel 1 = one thread per cell,
 // 2 = streaming (strips per block, rows loaded ahead).
@@ -128,12 +129,15 @@ __host__ __device__ inline double lgd_unit_open(uint64_t bits) {
   return (double)((bits >> 11) + 1) * (1.0 / 9007199254740992.0);
 }
 
+// The angle's cosine takes the reduced-range lgd_cos_angle (lgd_trig.h): the angle
+// never exceeds 2π, so the library cosine's large-argument path is dead code that
+// only costs registers.
 __device__ inline double lgd_standard_normal(uint64_t key) {
   uint64_t a = lgd_splitmix64(key);
   uint64_t b = lgd_splitmix64(a ^ 0xD1B54A32D192ED03ull);
   double u = lgd_unit_open(a);
   double v = lgd_unit_open(b);
-  return sqrt(-2.0 * log(u)) * cos(6.283185307179586 * v);
+  return sqrt(-2.0 * log(u)) * lgd_cos_angle(6.283185307179586 * v);
 }
 
 // --------------------------------------------------- Karma–Rappel constants
