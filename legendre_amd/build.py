@@ -81,5 +81,24 @@ def build(verbose: bool = False) -> pathlib.Path:
     return LIB
 
 
+HARNESS_SRC = HERE.parent / "tests" / "cabi" / "harness.cpp"
+HARNESS = HERE.parent / "tests" / "cabi" / "harness"
+
+
+def build_cabi_harness(verbose: bool = False) -> pathlib.Path:
+    """The C++ caller of the C ABI used by the tests (host code only, linked against
+    the in-tree library with a relative rpath so the tree can move)."""
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", str(HARNESS_SRC),
+           f"-I{HERE.parent / 'include'}", f"-L{HERE}", "-llegendre_gpu",
+           "-Wl,-rpath,$ORIGIN/../../legendre_amd", "-o", str(HARNESS)]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"C ABI harness build failed:\n{res.stderr}")
+    if verbose:
+        print(f"built {HARNESS}")
+    return HARNESS
+
+
 if __name__ == "__main__":
     build(verbose=True)
+    build_cabi_harness(verbose=True)
