@@ -461,6 +461,8 @@ int lgd_observer_last_error(lgd_observer *obs, char *buf, size_t len);
  * regrid that changes the hierarchy) and fields.  Graph-replayed runs end a graph at
  * each snapshot step.  The observer must outlive the simulation's stepping. */
 int lgd_sim_attach_observer(lgd_sim *sim, lgd_observer *obs);
+/* Stops notifying `obs` (before destroying an observer the simulation outlives). */
+int lgd_sim_detach_observer(lgd_sim *sim, lgd_observer *obs);
 
 /* Writes host columns as one Parquet file in the snapshot layout: types[c] 0 = DOUBLE,
  * 1 = INT64; columns[c] holds nrows values, or is NULL with constants[c] holding the

@@ -510,6 +510,11 @@ class Simulation:
         """Simulation::attach_observer: the observer sees every completed step."""
         _abi.check(self._lib.lgd_sim_attach_observer(self.handle, observer.handle), self.ctx.handle)
         self._observers = getattr(self, "_observers", []) + [observer]
+        observer._sims.append(weakref.ref(self))
+
+    def detach_observer(self, observer: "ParquetObserver") -> None:
+        _abi.check(self._lib.lgd_sim_detach_observer(self.handle, observer.handle), self.ctx.handle)
+        self._observers = [o for o in getattr(self, "_observers", []) if o is not observer]
 
     def set_graph_steps(self, steps: int) -> None:
         """Run `steps` whole steps at a time as one replayed HIP graph (0: eager
@@ -565,6 +570,7 @@ class ParquetObserver:
                                                  int(ring), farr, len(fields), sarr, len(statics),
                                                  C.byref(h)), ctx.handle)
         self.handle = h
+        self._sims = []  # simulations it is attached to (weak), detached on close
         ctx._adopt(self)
         if blocking:
             _abi.check(self._lib.lgd_observer_set_blocking(h, 1), ctx.handle)
@@ -606,6 +612,11 @@ class ParquetObserver:
 
     def close(self) -> None:
         if self.handle:
+            for ref in self._sims:
+                sim = ref()
+                if sim is not None and sim.handle:
+                    self._lib.lgd_sim_detach_observer(sim.handle, self.handle)
+            self._sims = []
             self._lib.lgd_observer_destroy(self.handle)
             self.handle = None
 

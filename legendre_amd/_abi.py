@@ -187,6 +187,7 @@ SIGNATURES = {
     "lgd_observer_stats": (C.c_int, [P, C.POINTER(U64), C.POINTER(U64), C.POINTER(U64), DP]),
     "lgd_observer_last_error": (C.c_int, [P, C.c_char_p, SZ]),
     "lgd_sim_attach_observer": (C.c_int, [P, P]),
+    "lgd_sim_detach_observer": (C.c_int, [P, P]),
     "lgd_parquet_write_file": (C.c_int, [C.c_char_p, SZ, C.POINTER(C.c_char_p), C.POINTER(C.c_int32),
                                          PP, C.POINTER(U64), I64, C.POINTER(I64), SZ, C.c_char_p, SZ]),
     "lgd_adjoint_create": (C.c_int, [P, P, PP]),

@@ -847,6 +847,17 @@ int lgd_sim_step(lgd_sim *sim, double dt, int64_t nsteps) {
   return LGD_OK;
 }
 
+int lgd_sim_detach_observer(lgd_sim *sim, lgd_observer *obs) {
+  if (!sim || !obs) return fail(sim ? sim->ctx : nullptr, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  auto &v = sim->observers;
+  for (size_t i = 0; i < v.size(); ++i)
+    if (v[i] == obs) {
+      v.erase(v.begin() + (long)i);
+      return LGD_OK;
+    }
+  return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT, "the observer is not attached to this simulation");
+}
+
 int lgd_sim_attach_observer(lgd_sim *sim, lgd_observer *obs) {
   if (!sim || !obs) return fail(sim ? sim->ctx : nullptr, LGD_ERR_INVALID_ARGUMENT, "null argument");
   for (int32_t f : observer_fields(obs))
@@ -861,6 +872,8 @@ int lgd_sim_attach_observer(lgd_sim *sim, lgd_observer *obs) {
   int rc = observer_prepare(obs, sim->amr ? nullptr : sim->grid, sim->amr,
                             sim->amr ? (uint64_t)sim->amr_epoch : 0);
   if (rc) return rc;
+  for (auto *o : sim->observers)
+    if (o == obs) return LGD_OK;  // attached once
   sim->observers.push_back(obs);
   return LGD_OK;
 }

@@ -234,6 +234,9 @@ def test_a_held_writer_drops_snapshots_without_stalling_the_solver(ctx, orc, tmp
     sim.step(0.008, 1)  # buffers are back: step 7 snapshots again
     obs.flush()
     assert obs.stats()["written"] == 5 and (tmp_path / "snap_0000007.parquet").exists()
+    obs.close()  # detaches itself: the simulation steps on without it
+    sim.step(0.008, 2)
+    assert not (tmp_path / "snap_0000008.parquet").exists()
     sim.close()
     obs.close()
     grid.close()
