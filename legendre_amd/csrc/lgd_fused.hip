@@ -118,44 +118,65 @@ __device__ __forceinline__ double face_rule(const lgd_face_bc &bc, double h, dou
 // kRules: a neighbour across a non-periodic domain face is the face rule's ghost
 // value. The stencil reads face neighbours only, so the value is the one the
 // dimension sweep writes there from the edge cells along the normal.
+// Workgroups tile one block (blockIdx.x = block · tiles + tile), so block-local indices
+// use 32-bit arithmetic and the block's coordinates are wave-uniform. A neighbour is
+// the slab entry one stride away inside the block; across a block face it is the
+// adjacent block's edge cell (the next block, or the wrapped one at a periodic face);
+// at a domain face it is the face rule's ghost value (the edge cell itself under
+// Mirror). These are the cells the reflected / wrapped global coordinate resolves to.
 template <bool kRules>
-__global__ void heat_step_kernel(lgd_cart c, double ih0, double ih1, double ih2, double kappa,
-                                 double dt, const double *u, double *u_n, FaceRules rules) {
-  int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int64_t ncell = c.n[0] * c.n[1] * c.n[2];
-  if (t >= c.nblocks * ncell) return;
-  int64_t b = t / ncell;
-  int64_t r = t - b * ncell;
-  int64_t i = r % c.n[0];
-  r /= c.n[0];
-  int64_t j = r % c.n[1];
-  int64_t kk = r / c.n[1];
-  int64_t bc[3] = {b % c.nb[0], (b / c.nb[0]) % c.nb[1], b / (c.nb[0] * c.nb[1])};
-  int64_t g[3] = {bc[0] * c.n[0] + i, bc[1] * c.n[1] + j, bc[2] * c.n[2] + kk};
-  int64_t N[3] = {c.nb[0] * c.n[0], c.nb[1] * c.n[1], c.nb[2] * c.n[2]};
-  int64_t oc = cell_offset(c, g[0], g[1], g[2]);
-  double center = u[oc];
-  double inv[3] = {ih0, ih1, ih2};
+__global__ __launch_bounds__(256) void heat_step_kernel(lgd_cart c, double ih0, double ih1,
+                                                        double ih2, double kappa, double dt,
+                                                        const double *u, double *u_n,
+                                                        FaceRules rules, uint32_t tiles) {
+  const uint32_t b = blockIdx.x / tiles;
+  const uint32_t n0 = (uint32_t)c.n[0], n1 = (uint32_t)c.n[1], n2 = (uint32_t)c.n[2];
+  const uint32_t r = (blockIdx.x - b * tiles) * 256u + threadIdx.x;
+  if (r >= n0 * n1 * n2) return;
+  const uint32_t q = r / n0;
+  const uint32_t l[3] = {r - q * n0, q % n1, q / n1};
+  // Slab strides of dimensions 0, 1, 2 (unused dimensions have no ghost layers).
+  const int64_t sx = c.dim > 0 ? (int64_t)n0 + 2 : 1;
+  const int64_t sy = c.dim > 1 ? (int64_t)n1 + 2 : 1;
+  const int64_t sz = c.dim > 2 ? (int64_t)n2 + 2 : 1;
+  const int64_t slab = sx * sy * sz;
+  const int64_t stride[3] = {1, sx, sx * sy};
+  const int64_t bstride[3] = {slab, c.nb[0] * slab, c.nb[0] * c.nb[1] * slab};  // one block over
+  int64_t oc = (int64_t)b * slab + (int64_t)(l[0] + 1);
+  if (c.dim > 1) oc += (int64_t)(l[1] + 1) * stride[1];
+  if (c.dim > 2) oc += (int64_t)(l[2] + 1) * stride[2];
+  const double center = u[oc];
+  const int64_t bc[3] = {(int64_t)b % c.nb[0], ((int64_t)b / c.nb[0]) % c.nb[1],
+                         (int64_t)b / (c.nb[0] * c.nb[1])};
+  const double inv[3] = {ih0, ih1, ih2};
   double acc = 0.0;
   for (int d = 0; d < c.dim; ++d) {
-    int64_t gp[3] = {g[0], g[1], g[2]};
-    int64_t gm[3] = {g[0], g[1], g[2]};
+    const int64_t nd = d == 0 ? n0 : d == 1 ? n1 : n2;
+    const int64_t span = (nd - 1) * stride[d];  // edge cell to edge cell across the block
     double up, um;
-    if (kRules && !c.periodic[d] && g[d] + 1 >= N[d]) {
-      gp[d] = g[d] - 1;  // the next cell inward from the high face
-      double i1 = rules.bc[2 * d + 1].kind == LGD_BC_EXTRAPOLATE ? u[cell_offset(c, gp[0], gp[1], gp[2])] : 0.0;
+    if (l[d] + 1 < nd) {
+      up = u[oc + stride[d]];
+    } else if (bc[d] + 1 < c.nb[d]) {
+      up = u[oc + bstride[d] - span];
+    } else if (c.periodic[d]) {
+      up = u[oc - (c.nb[d] - 1) * bstride[d] - span];
+    } else if (kRules) {
+      const double i1 = rules.bc[2 * d + 1].kind == LGD_BC_EXTRAPOLATE ? u[oc - stride[d]] : 0.0;
       up = face_rule(rules.bc[2 * d + 1], c.h[d], center, &i1);
     } else {
-      gp[d] = resolve_coord(g[d] + 1, N[d], c.periodic[d]);
-      up = u[cell_offset(c, gp[0], gp[1], gp[2])];
+      up = center;  // Mirror
     }
-    if (kRules && !c.periodic[d] && g[d] == 0) {
-      gm[d] = 1;  // the next cell inward from the low face
-      double i1 = rules.bc[2 * d].kind == LGD_BC_EXTRAPOLATE ? u[cell_offset(c, gm[0], gm[1], gm[2])] : 0.0;
+    if (l[d] > 0) {
+      um = u[oc - stride[d]];
+    } else if (bc[d] > 0) {
+      um = u[oc - bstride[d] + span];
+    } else if (c.periodic[d]) {
+      um = u[oc + (c.nb[d] - 1) * bstride[d] + span];
+    } else if (kRules) {
+      const double i1 = rules.bc[2 * d].kind == LGD_BC_EXTRAPOLATE ? u[oc + stride[d]] : 0.0;
       um = face_rule(rules.bc[2 * d], c.h[d], center, &i1);
     } else {
-      gm[d] = resolve_coord(g[d] - 1, N[d], c.periodic[d]);
-      um = u[cell_offset(c, gm[0], gm[1], gm[2])];
+      um = center;  // Mirror
     }
     acc = acc + (((up - (center + center)) + um) * inv[d]);
   }
@@ -276,13 +297,17 @@ int lgd_heat_step_bc(lgd_ctx *ctx, const lgd_grid *grid, double kappa, const lgd
         any_rule = any_rule || (bc.kind != LGD_BC_MIRROR && !c.periodic[d]);
       }
   }
-  int64_t total = c.nblocks * c.n[0] * c.n[1] * c.n[2];
+  const int64_t ncell = c.n[0] * c.n[1] * c.n[2];
+  if (ncell > INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "a block of more than 2^31 cells");
+  const uint32_t tiles = (uint32_t)((ncell + 255) / 256);
+  const int64_t wgs = c.nblocks * (int64_t)tiles;
+  if (wgs * 256 > (int64_t)UINT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "grid too large for one launch");
   if (any_rule)
-    heat_step_kernel<true><<<blocks_for(total, 256), 256, 0, ctx->stream>>>(
-        c, ih[0], ih[1], ih[2], kappa, dt, u->data, u_next->data, rules);
+    heat_step_kernel<true><<<(unsigned)wgs, 256, 0, ctx->stream>>>(c, ih[0], ih[1], ih[2], kappa, dt,
+                                                                  u->data, u_next->data, rules, tiles);
   else
-    heat_step_kernel<false><<<blocks_for(total, 256), 256, 0, ctx->stream>>>(
-        c, ih[0], ih[1], ih[2], kappa, dt, u->data, u_next->data, rules);
+    heat_step_kernel<false><<<(unsigned)wgs, 256, 0, ctx->stream>>>(c, ih[0], ih[1], ih[2], kappa, dt,
+                                                                   u->data, u_next->data, rules, tiles);
   return launch_check(ctx, "heat_step");
 }
 

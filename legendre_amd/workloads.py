@@ -118,6 +118,24 @@ def _fma(a: float, b: float, c: float) -> float:
     return float(Fraction(a) * Fraction(b) + Fraction(c))
 
 
+def to_slabs(u: np.ndarray, cells, block, ghost: int = 1) -> np.ndarray:
+    """A global interior array (numpy axes reversed: [z][y][x]) as a field buffer of
+    ghost-inclusive block slabs, block-major, dimension 0 fastest (ghosts zero)."""
+    dim = len(cells)
+    nbs = [c // b for c, b in zip(cells, block)]
+    shape = tuple(b + 2 * ghost for b in reversed(block))
+    out = np.zeros((int(np.prod(nbs)),) + shape)
+    for bid in range(out.shape[0]):
+        co, r = [], bid
+        for nb in nbs:
+            co.append(r % nb)
+            r //= nb
+        src = tuple(slice(co[d] * block[d], (co[d] + 1) * block[d]) for d in reversed(range(dim)))
+        dst = (bid,) + tuple(slice(ghost, ghost + block[d]) for d in reversed(range(dim)))
+        out[dst] = u[src]
+    return out.reshape(-1)
+
+
 HEAT3D = Heat3dConfig()
 MODEL_C = ModelCConfig()
 
