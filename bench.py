@@ -49,7 +49,17 @@ def run_gpu(args, C):
     if args.graph:
         sim.set_graph_steps(args.graph)
     t0 = time.time()
-    sim.initialize_grains(C.grains(), C.delta)
+    if args.oriented:
+        # Multi-grain Model C: each seed a grain of its own θ₀ ∈ U[−π/4, π/4), the same
+        # counter-keyed draws as the positions.
+        from legendre_amd.workloads import mix_key_np, unit_open_np
+
+        g3 = C.grains()
+        th = np.array([(unit_open_np(mix_key_np(C.seed, [i, 2]))[0] - 0.5) * (np.pi / 2)
+                       for i in range(len(g3))])
+        sim.initialize_grains_oriented(np.column_stack([g3, th]), C.delta)
+    else:
+        sim.initialize_grains(C.grains(), C.delta)
     log(f"[bench] initial condition on {C.cells}^2 in {time.time() - t0:.1f}s")
     if args.autotune:
         r = ctx.autotune_modelc_step(grid, params, sim.field(0), sim.field(1), C.dt,
@@ -862,6 +872,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=0, choices=[0, 1, 2],
                     help="rows the streaming kernel loads ahead (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--oriented", action="store_true",
+                    help="multi-grain Model C (per-grain θ₀, the per-cell oriented kernel)")
     ap.add_argument("--autotune", action="store_true",
                     help="time every launch shape of the Model C step first and use the fastest")
     ap.add_argument("--calibrate", action="store_true",
@@ -900,13 +912,15 @@ def main() -> int:
         f"per launch min {min(per):.4f} max {max(per):.4f} ms")
     traffic = read_traffic(os.path.join(ROOT, "profiles", "modelc_stream_kernel_10080sq.md"))
     if (args.cells != 10080 or args.block != 126 or args.noise != 0.0 or args.kernel == "percell"
-            or args.mode != "fused"):
+            or args.mode != "fused" or args.oriented):
         traffic = None  # the committed profile covers the default workload only
     cpu = None
     if not args.no_cpu:
         cpu = run_cpu_baseline(C, args.cpu_budget)
         log(f"[bench] cpu baseline: {cpu}")
     workload = f"model_c_{C.cells}sq_{C.block}blocks_b{C.noise:g}_{args.mode}"
+    if args.oriented:
+        workload += "_oriented"
     if args.kernel != "auto" or args.strips or args.prefetch:
         workload += f"_{args.kernel}_s{args.strips}_p{args.prefetch}"
     line = {
@@ -945,7 +959,8 @@ def main() -> int:
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
-            "kernel": "modelc_step_kernel" if args.kernel == "percell" else "modelc_stream_kernel",
+            "kernel": ("modelc_step_kernel" if args.kernel == "percell" or args.oriented
+                       else "modelc_stream_kernel"),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_launch_ms, 5),
         },

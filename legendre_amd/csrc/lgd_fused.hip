@@ -15,27 +15,24 @@
 
 namespace {
 
-// Global-coordinate resolution of one dimension: wrap (periodic) or reflect
-// (mirror, reflect_coord in src/geometry/boundary.rs:82-90) into [0, N).
-__device__ __forceinline__ int64_t resolve_coord(int64_t gi, int64_t N, int periodic) {
-  if (gi < 0) return periodic ? gi + N : -1 - gi;
-  if (gi >= N) return periodic ? gi - N : 2 * N - 1 - gi;
-  return gi;
-}
-
-// Element offset of global cell (gi, gj, gk) (already in range) in a ghost-1 field buffer.
-__device__ __forceinline__ int64_t cell_offset(const lgd_cart &c, int64_t gi, int64_t gj,
-                                               int64_t gk) {
-  int64_t bx = gi / c.n[0], by = gj / c.n[1], bz = gk / c.n[2];
-  int64_t i = gi - bx * c.n[0], j = gj - by * c.n[1], k = gk - bz * c.n[2];
-  int64_t ex = c.dim > 0 ? c.n[0] + 2 : 1;
-  int64_t ey = c.dim > 1 ? c.n[1] + 2 : 1;
-  int64_t ez = c.dim > 2 ? c.n[2] + 2 : 1;
-  int64_t b = bx + c.nb[0] * (by + c.nb[1] * bz);
-  int64_t o = (i + 1);
-  if (c.dim > 1) o += (j + 1) * ex;
-  if (c.dim > 2) o += (k + 1) * ex * ey;
-  return b * ex * ey * ez + o;
+// Offset from a cell to its neighbour one step along one dimension (dir = ±1): the
+// next slab entry inside the block, the adjacent block's edge cell across a block face,
+// the wrapped block's edge cell at a periodic face, or the cell itself at a mirror face
+// (reflect_coord, src/geometry/boundary.rs:82-90). `l` is the cell's index in its block
+// of n cells, `bc` the block's index among nb, `stride` / `bstride` the element
+// distances of one cell / one block along the dimension.
+__device__ __forceinline__ int64_t neighbour_delta(int64_t l, int64_t n, int64_t bc, int64_t nb,
+                                                   int periodic, int64_t stride, int64_t bstride,
+                                                   int dir) {
+  const int64_t span = (n - 1) * stride;
+  if (dir > 0) {
+    if (l + 1 < n) return stride;
+    if (bc + 1 < nb) return bstride - span;
+    return periodic ? -(nb - 1) * bstride - span : 0;
+  }
+  if (l > 0) return -stride;
+  if (bc > 0) return -bstride + span;
+  return periodic ? (nb - 1) * bstride + span : 0;
 }
 
 constexpr int kTX = 64;
@@ -56,23 +53,23 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
   int64_t i = tx * kTX + threadIdx.x;
   int64_t j = ty * kTY + threadIdx.y;
   if (i >= c.n[0] || j >= c.n[1]) return;
-  int64_t bx = b % c.nb[0], by = b / c.nb[0];
-  int64_t NX = c.nb[0] * c.n[0], NY = c.nb[1] * c.n[1];
-  int64_t gi = bx * c.n[0] + i, gj = by * c.n[1] + j;
+  const int64_t bx = b % c.nb[0], by = b / c.nb[0];
+  // Element offsets of the neighbours, one dimension at a time (the resolved global
+  // coordinates of the halo are per-dimension reflections / wraps, so a corner
+  // neighbour composes the two one-dimensional moves).
+  const int64_t sx = c.n[0] + 2, slab = sx * (c.n[1] + 2);
+  const int64_t oc = b * slab + (j + 1) * sx + (i + 1);
+  const int64_t dxm = neighbour_delta(i, c.n[0], bx, c.nb[0], c.periodic[0], 1, slab, -1);
+  const int64_t dxp = neighbour_delta(i, c.n[0], bx, c.nb[0], c.periodic[0], 1, slab, +1);
+  const int64_t dym = neighbour_delta(j, c.n[1], by, c.nb[1], c.periodic[1], sx, c.nb[0] * slab, -1);
+  const int64_t dyp = neighbour_delta(j, c.n[1], by, c.nb[1], c.periodic[1], sx, c.nb[0] * slab, +1);
+  const int64_t dx[3] = {dxm, 0, dxp}, dy[3] = {dym, 0, dyp};
   double p[3][3];
-  for (int dj = -1; dj <= 1; ++dj) {
-    int64_t rj = resolve_coord(gj + dj, NY, c.periodic[1]);
-    for (int di = -1; di <= 1; ++di) {
-      int64_t ri = resolve_coord(gi + di, NX, c.periodic[0]);
-      p[dj + 1][di + 1] = phi[cell_offset(c, ri, rj, 0)];
-    }
-  }
-  int64_t oc = cell_offset(c, gi, gj, 0);
-  double uc = u[oc];
-  double ue = u[cell_offset(c, resolve_coord(gi + 1, NX, c.periodic[0]), gj, 0)];
-  double uw = u[cell_offset(c, resolve_coord(gi - 1, NX, c.periodic[0]), gj, 0)];
-  double un = u[cell_offset(c, gi, resolve_coord(gj + 1, NY, c.periodic[1]), 0)];
-  double us = u[cell_offset(c, gi, resolve_coord(gj - 1, NY, c.periodic[1]), 0)];
+  for (int dj = 0; dj < 3; ++dj)
+    for (int di = 0; di < 3; ++di) p[dj][di] = phi[oc + dx[di] + dy[dj]];
+  const double uc = u[oc];
+  const double ue = u[oc + dxp], uw = u[oc + dxm];
+  const double un = u[oc + dyp], us = u[oc + dym];
   double dphi, du;
   if (kOriented) {
     const int64_t ot = b * c.n[0] * c.n[1] + j * c.n[0] + i;
