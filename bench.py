@@ -48,6 +48,8 @@ def run_gpu(args, C):
                                 mode=lg.LGD_MODE_FUSED if args.mode == "fused" else lg.LGD_MODE_PER_INTENT)
     if args.graph:
         sim.set_graph_steps(args.graph)
+    if args.integrator != "euler":
+        sim.set_integrator(args.integrator)
     t0 = time.time()
     if args.oriented:
         # Multi-grain Model C: each seed a grain of its own θ₀ ∈ U[−π/4, π/4), the same
@@ -872,6 +874,8 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=0, choices=[0, 1, 2],
                     help="rows the streaming kernel loads ahead (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--integrator", choices=["euler", "rk4"], default="euler",
+                    help="rk4: RungeKutta4 (deterministic runs), four stages of whole-slab sweeps")
     ap.add_argument("--oriented", action="store_true",
                     help="multi-grain Model C (per-grain θ₀, the per-cell oriented kernel)")
     ap.add_argument("--autotune", action="store_true",
@@ -912,7 +916,7 @@ def main() -> int:
         f"per launch min {min(per):.4f} max {max(per):.4f} ms")
     traffic = read_traffic(os.path.join(ROOT, "profiles", "modelc_stream_kernel_10080sq.md"))
     if (args.cells != 10080 or args.block != 126 or args.noise != 0.0 or args.kernel == "percell"
-            or args.mode != "fused" or args.oriented):
+            or args.mode != "fused" or args.oriented or args.integrator != "euler"):
         traffic = None  # the committed profile covers the default workload only
     cpu = None
     if not args.no_cpu:
@@ -921,6 +925,8 @@ def main() -> int:
     workload = f"model_c_{C.cells}sq_{C.block}blocks_b{C.noise:g}_{args.mode}"
     if args.oriented:
         workload += "_oriented"
+    if args.integrator != "euler":
+        workload += f"_{args.integrator}"
     if args.kernel != "auto" or args.strips or args.prefetch:
         workload += f"_{args.kernel}_s{args.strips}_p{args.prefetch}"
     line = {
@@ -949,7 +955,7 @@ def main() -> int:
             "u0": -C.delta,
             "seeds": C.nseeds,
             "noise_b": C.noise,
-            "integrator": "euler_maruyama",
+            "integrator": "euler_maruyama" if args.integrator == "euler" else args.integrator,
             "parallelism": "single_gpu",
         },
         "roofline": {
