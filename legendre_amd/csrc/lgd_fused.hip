@@ -35,6 +35,41 @@ __device__ __forceinline__ int64_t neighbour_delta(int64_t l, int64_t n, int64_t
   return periodic ? (nb - 1) * bstride + span : 0;
 }
 
+// The Model C vector field (dφ, du) of interior cell (i, j) of block b from (phi, u),
+// halos read in place; also the cell's element offset and its φ and u.
+template <bool kOriented>
+__device__ __forceinline__ void cell_vector_field(const lgd_cart &c, const lgd_kr_consts &k,
+                                                  int64_t b, int64_t i, int64_t j,
+                                                  const double *phi, const double *u,
+                                                  const double *cs0, const double *sn0,
+                                                  int64_t &oc, double &pc, double &uc,
+                                                  double &dphi, double &du) {
+  const int64_t bx = b % c.nb[0], by = b / c.nb[0];
+  // Element offsets of the neighbours, one dimension at a time (the resolved global
+  // coordinates of the halo are per-dimension reflections / wraps, so a corner
+  // neighbour composes the two one-dimensional moves).
+  const int64_t sx = c.n[0] + 2, slab = sx * (c.n[1] + 2);
+  oc = b * slab + (j + 1) * sx + (i + 1);
+  const int64_t dxm = neighbour_delta(i, c.n[0], bx, c.nb[0], c.periodic[0], 1, slab, -1);
+  const int64_t dxp = neighbour_delta(i, c.n[0], bx, c.nb[0], c.periodic[0], 1, slab, +1);
+  const int64_t dym = neighbour_delta(j, c.n[1], by, c.nb[1], c.periodic[1], sx, c.nb[0] * slab, -1);
+  const int64_t dyp = neighbour_delta(j, c.n[1], by, c.nb[1], c.periodic[1], sx, c.nb[0] * slab, +1);
+  const int64_t dx[3] = {dxm, 0, dxp}, dy[3] = {dym, 0, dyp};
+  double p[3][3];
+  for (int dj = 0; dj < 3; ++dj)
+    for (int di = 0; di < 3; ++di) p[dj][di] = phi[oc + dx[di] + dy[dj]];
+  uc = u[oc];
+  const double ue = u[oc + dxp], uw = u[oc + dxm];
+  const double un = u[oc + dyp], us = u[oc + dym];
+  pc = p[1][1];
+  if (kOriented) {
+    const int64_t ot = b * c.n[0] * c.n[1] + j * c.n[0] + i;
+    lgd_modelc_cell_oriented(k, p, uc, ue, uw, un, us, cs0[ot], sn0[ot], dphi, du);
+  } else {
+    lgd_modelc_cell(k, p, uc, ue, uw, un, us, dphi, du);
+  }
+}
+
 constexpr int kTX = 64;
 constexpr int kTY = 4;
 
@@ -53,33 +88,12 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
   int64_t i = tx * kTX + threadIdx.x;
   int64_t j = ty * kTY + threadIdx.y;
   if (i >= c.n[0] || j >= c.n[1]) return;
-  const int64_t bx = b % c.nb[0], by = b / c.nb[0];
-  // Element offsets of the neighbours, one dimension at a time (the resolved global
-  // coordinates of the halo are per-dimension reflections / wraps, so a corner
-  // neighbour composes the two one-dimensional moves).
-  const int64_t sx = c.n[0] + 2, slab = sx * (c.n[1] + 2);
-  const int64_t oc = b * slab + (j + 1) * sx + (i + 1);
-  const int64_t dxm = neighbour_delta(i, c.n[0], bx, c.nb[0], c.periodic[0], 1, slab, -1);
-  const int64_t dxp = neighbour_delta(i, c.n[0], bx, c.nb[0], c.periodic[0], 1, slab, +1);
-  const int64_t dym = neighbour_delta(j, c.n[1], by, c.nb[1], c.periodic[1], sx, c.nb[0] * slab, -1);
-  const int64_t dyp = neighbour_delta(j, c.n[1], by, c.nb[1], c.periodic[1], sx, c.nb[0] * slab, +1);
-  const int64_t dx[3] = {dxm, 0, dxp}, dy[3] = {dym, 0, dyp};
-  double p[3][3];
-  for (int dj = 0; dj < 3; ++dj)
-    for (int di = 0; di < 3; ++di) p[dj][di] = phi[oc + dx[di] + dy[dj]];
-  const double uc = u[oc];
-  const double ue = u[oc + dxp], uw = u[oc + dxm];
-  const double un = u[oc + dyp], us = u[oc + dym];
-  double dphi, du;
-  if (kOriented) {
-    const int64_t ot = b * c.n[0] * c.n[1] + j * c.n[0] + i;
-    lgd_modelc_cell_oriented(k, p, uc, ue, uw, un, us, cs0[ot], sn0[ot], dphi, du);
-  } else {
-    lgd_modelc_cell(k, p, uc, ue, uw, un, us, dphi, du);
-  }
+  int64_t oc;
+  double pc, uc, dphi, du;
+  cell_vector_field<kOriented>(c, k, b, i, j, phi, u, cs0, sn0, oc, pc, uc, dphi, du);
   // apply_step_with: Time first (x += dt*k), then Wiener(0) on phi
   // (src/core/state.rs:656-699, src/core/driver.rs:131-158).
-  double pn = p[1][1] + dt * dphi;
+  double pn = pc + dt * dphi;
   if (kNoise) {
     uint64_t block_key = lgd_mix_key3(seed, lgd_salt(salt, salt_src), stream, (uint64_t)b);
     uint64_t key = lgd_splitmix64(block_key ^ (uint64_t)(i + j * c.n[0]));
@@ -87,6 +101,54 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
   }
   phi_n[oc] = pn;
   u_n[oc] = uc + dt * du;
+}
+
+// One stage of RungeKutta4::step on Model C (src/integrators/rk4.rs:30-80) in one pass,
+// reading the stage input's halos in place. Stages 1-3 write k_s and the next stage's
+// input y + c_s·k_s (the reference's copy then axpy). Stage 4 evaluates k4 and writes
+// the new state ((((y + w1·k1) + w2·k2) + w3·k3) + w4·k4), the reference's four
+// whole-slab axpys in their order, into y in place (y is read only at the cell itself).
+struct Rk4Args {
+  const double *pin, *uin;    // the stage's input state
+  double *p0, *u0;            // y (read; stage 4 writes the result here)
+  double *kp, *ku;            // stages 1-3: k_s
+  double *pout, *uout;        // stages 1-3: the next stage's input
+  const double *k1p, *k1u, *k2p, *k2u, *k3p, *k3u;  // stage 4
+  double coef;                // stages 1-3: c_s
+  double w1, w2, w3, w4;      // stage 4: dt/6, dt/3, dt/3, dt/6
+};
+
+template <int kStage, bool kOriented>
+__global__ __launch_bounds__(kTX *kTY) void rk4_stage_kernel(lgd_cart c, lgd_kr_consts k,
+                                                             int64_t tiles_x, int64_t tiles_per_block,
+                                                             Rk4Args a, const double *cs0,
+                                                             const double *sn0) {
+  int64_t b = blockIdx.x / tiles_per_block;
+  int64_t tile = blockIdx.x - b * tiles_per_block;
+  int64_t ty = tile / tiles_x, tx = tile - ty * tiles_x;
+  int64_t i = tx * kTX + threadIdx.x;
+  int64_t j = ty * kTY + threadIdx.y;
+  if (i >= c.n[0] || j >= c.n[1]) return;
+  int64_t oc;
+  double pc, uc, dphi, du;
+  cell_vector_field<kOriented>(c, k, b, i, j, a.pin, a.uin, cs0, sn0, oc, pc, uc, dphi, du);
+  if (kStage < 4) {
+    a.kp[oc] = dphi;
+    a.ku[oc] = du;
+    a.pout[oc] = a.p0[oc] + a.coef * dphi;
+    a.uout[oc] = a.u0[oc] + a.coef * du;
+  } else {
+    double y = a.p0[oc];
+    y = y + a.w1 * a.k1p[oc];
+    y = y + a.w2 * a.k2p[oc];
+    y = y + a.w3 * a.k3p[oc];
+    a.p0[oc] = y + a.w4 * dphi;
+    double v = a.u0[oc];
+    v = v + a.w1 * a.k1u[oc];
+    v = v + a.w2 * a.k2u[oc];
+    v = v + a.w3 * a.k3u[oc];
+    a.u0[oc] = v + a.w4 * du;
+  }
 }
 
 // The face rules of the six physical faces, low then high per dimension.
@@ -249,6 +311,69 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
   else
     go(F{}, F{});
   return launch_check(ctx, "modelc_step");
+}
+
+// One RungeKutta4 step of Model C as four fused stage launches. y = (phi, u); k[0..2]
+// hold k1..k3 (per field); t0, t1 alternate as stage inputs. cs0 / sn0: orientations.
+int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                        lgd_buf *phi, lgd_buf *u, lgd_buf *const *kbuf, lgd_buf *const *tbuf,
+                        double dt, const double *cs0, const double *sn0) {
+  const lgd_cart &c = grid->cart;
+  if (c.dim != 2) return fail(ctx, LGD_ERR_UNSUPPORTED, "Model C is 2-D");
+  const lgd_kr_consts k = lgd_make_kr_consts(*params, c);
+  const int64_t tiles_x = (c.n[0] + kTX - 1) / kTX;
+  const int64_t tiles = tiles_x * ((c.n[1] + kTY - 1) / kTY);
+  const int64_t nwg = c.nblocks * tiles;
+  if (nwg > (int64_t)INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "grid too large for one launch");
+  const dim3 wg(kTX, kTY);
+  const double half_dt = 0.5 * dt;
+  const double coef[3] = {half_dt, half_dt, dt};
+  // kbuf: k1φ, k1u, k2φ, k2u, k3φ, k3u; tbuf: t0φ, t0u, t1φ, t1u.
+  Rk4Args a{};
+  a.p0 = phi->data;
+  a.u0 = u->data;
+  a.w1 = dt / 6.0;
+  a.w2 = dt / 3.0;
+  a.w3 = dt / 3.0;
+  a.w4 = dt / 6.0;
+  a.k1p = kbuf[0]->data;
+  a.k1u = kbuf[1]->data;
+  a.k2p = kbuf[2]->data;
+  a.k2u = kbuf[3]->data;
+  a.k3p = kbuf[4]->data;
+  a.k3u = kbuf[5]->data;
+  const bool oriented = cs0 != nullptr;
+  for (int st = 1; st <= 4; ++st) {
+    const lgd_buf *in_p = st == 1 ? phi : tbuf[(st % 2 == 0) ? 0 : 2];
+    const lgd_buf *in_u = st == 1 ? u : tbuf[(st % 2 == 0) ? 1 : 3];
+    a.pin = in_p->data;
+    a.uin = in_u->data;
+    if (st < 4) {
+      a.kp = kbuf[2 * (st - 1)]->data;
+      a.ku = kbuf[2 * (st - 1) + 1]->data;
+      a.pout = tbuf[(st % 2 == 1) ? 0 : 2]->data;
+      a.uout = tbuf[(st % 2 == 1) ? 1 : 3]->data;
+      a.coef = coef[st - 1];
+    }
+    auto go = [&](auto stage_t, auto orient_t) {
+      rk4_stage_kernel<decltype(stage_t)::value, decltype(orient_t)::value>
+          <<<(unsigned)nwg, wg, 0, ctx->stream>>>(c, k, tiles_x, tiles, a, cs0, sn0);
+    };
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    using S3 = std::integral_constant<int, 3>;
+    using S4 = std::integral_constant<int, 4>;
+    using T = std::true_type;
+    using F = std::false_type;
+    if (oriented) {
+      if (st == 1) go(S1{}, T{}); else if (st == 2) go(S2{}, T{}); else if (st == 3) go(S3{}, T{}); else go(S4{}, T{});
+    } else {
+      if (st == 1) go(S1{}, F{}); else if (st == 2) go(S2{}, F{}); else if (st == 3) go(S3{}, F{}); else go(S4{}, F{});
+    }
+    int rc = launch_check(ctx, "modelc_rk4_stage");
+    if (rc) return rc;
+  }
+  return LGD_OK;
 }
 
 extern "C" {

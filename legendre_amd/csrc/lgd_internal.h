@@ -182,6 +182,12 @@ const std::vector<int32_t> &observer_statics(const lgd_observer *o);
 int observer_prepare(lgd_observer *o, const lgd_grid *grid, const lgd_amr *amr, uint64_t epoch);
 }  // namespace lgd
 
+// One RungeKutta4 step of Model C in four fused stage launches (lgd_fused.hip):
+// kbuf = k1φ, k1u, k2φ, k2u, k3φ, k3u; tbuf = two alternating stage inputs (φ, u, φ, u).
+int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
+                        lgd_buf *phi, lgd_buf *u, lgd_buf *const *kbuf, lgd_buf *const *tbuf,
+                        double dt, const double *cs0, const double *sn0);
+
 int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
                          const lgd_buf *phi, const lgd_buf *u, lgd_buf *phi_next, lgd_buf *u_next,
                          double dt, uint64_t seed, uint64_t salt, int32_t wiener,
