@@ -181,3 +181,27 @@ def test_rk4_rejects_a_stochastic_model_and_replays_as_a_graph(ctx, orc):
     for sim in sims:
         sim.close()
     grid.close()
+
+
+@pytest.mark.parametrize("integrator", ["euler", "rk4"])
+def test_zero_orientation_is_bitwise_the_axis_aligned_path(ctx, orc, integrator):
+    """tests/model_c.rs (zero_orientation_is_bitwise_identical_to_legacy_path): θ₀ = 0
+    everywhere through the oriented kernels equals the axis-aligned model bit for bit."""
+    say(f"zero orientation == axis-aligned, {integrator}")
+    grid = lg.CartesianGrid(ctx, CELLS, BLOCK, [0.0, 0.0], [0.4, 0.4])
+    out = []
+    for oriented in (False, True):
+        for mode in (lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT):
+            sim = lg.Simulation.model_c(grid, lg.ModelCParams(0.05, 3.19, 2.0), mode=mode)
+            sim.set_integrator(integrator)
+            sim.initialize_grains(GRAINS[:, :3], 0.6)
+            if oriented:
+                sim.set_orientation(np.zeros(orc.num_blocks(orc.grid(CELLS, BLOCK, [0, 0], [0.4, 0.4]))
+                                             * BLOCK[0] * BLOCK[1]))
+            sim.step(0.008, 12)
+            m = interior_mask(CELLS, BLOCK)
+            out.append((sim.field(0).download()[m], sim.field(1).download()[m]))
+            sim.close()
+    for phi, u in out[1:]:
+        assert np.array_equal(phi, out[0][0]) and np.array_equal(u, out[0][1])
+    grid.close()
