@@ -129,15 +129,16 @@ __host__ __device__ inline double lgd_unit_open(uint64_t bits) {
   return (double)((bits >> 11) + 1) * (1.0 / 9007199254740992.0);
 }
 
-// The angle's cosine takes the reduced-range lgd_cos_angle (lgd_trig.h): the angle
-// never exceeds 2π, so the library cosine's large-argument path is dead code that
-// only costs registers.
+// The logarithm and the angle's cosine are the reduced-range lgd_log_unit and
+// lgd_cos_angle (lgd_trig.h): u is a positive normal number no greater than 1 and
+// the angle never exceeds 2π, so the library functions' special-case and
+// large-argument paths are dead code that costs instructions and registers.
 __device__ inline double lgd_standard_normal(uint64_t key) {
   uint64_t a = lgd_splitmix64(key);
   uint64_t b = lgd_splitmix64(a ^ 0xD1B54A32D192ED03ull);
   double u = lgd_unit_open(a);
   double v = lgd_unit_open(b);
-  return sqrt(-2.0 * log(u)) * lgd_cos_angle(6.283185307179586 * v);
+  return sqrt(-2.0 * lgd_log_unit(u)) * lgd_cos_angle(6.283185307179586 * v);
 }
 
 // --------------------------------------------------- Karma–Rappel constants

@@ -1,5 +1,9 @@
-// cos(x) for the Box–Muller angle x = fl(2π·v), v in (0, 1], so 0 < x <= 2π.
+// The two transcendental functions of the Box–Muller draw, written for the
+// arguments the draw forms (host and device compile the same code):
+// - cos(x) for the angle x = fl(2π·v), v in (0, 1], so 0 < x <= 2π;
+// - log(u) for the uniform u = k·2^-53, k in [1, 2^53], so 2^-53 <= u <= 1.
 //
+// cos:
 // Method (no large-argument reduction is needed in this range):
 // - Cody & Waite's additive reduction by π/2 with a three-part constant: with
 //   n = rint(2x/π) <= 4, x − n·P1 is exact (both are multiples of x's ulp or finer and
@@ -11,7 +15,21 @@
 //
 // The result is within about one ulp of the exact cosine. It is tested against the
 // host libm's cos over the whole range of angles, including every zero crossing.
-// Host and device compile the same code.
+//
+// log: FDLIBM's __ieee754_log (Sun Microsystems, 1993, as in FreeBSD's msun
+// e_log.c), specialised to positive normal arguments:
+// - x = 2^k·(1 + f) with √2/2 <= 1 + f < √2, taken from x's bits;
+// - s = f/(2 + f), so that log(1 + f) = 2s + s·R(s²), with FDLIBM's degree-7
+//   minimax R (Horner's rule with fused multiply-adds);
+// - log x = k·ln2_hi + (f − (hf² − s·(hf² + R) + k·ln2_lo)) for f near the ends of
+//   the reduced range (hf² = f²/2), else k·ln2_hi + (f − (s·(f − R) − k·ln2_lo)),
+//   with ln2_hi's trailing zeros making k·ln2_hi exact.
+// FDLIBM's shortcuts for k = 0 and |f| < 2^-20 give the same value or one within
+// its error bound, and are left out. Zero, negative, subnormal and non-finite
+// arguments are not handled: the draw never forms them. The result is within
+// one ulp of the exact logarithm; it is tested against the host libm's log over
+// random uniforms and every uniform near 1, near 2^-53 and near the reduction's
+// boundaries.
 #pragma once
 
 #include <cmath>
@@ -62,4 +80,41 @@ LGD_TRIG_HD inline double lgd_cos_angle(double x) {
   const int q = (int)n & 3;
   const double v = (q & 1) ? sinr : cosr;
   return (q == 1 || q == 2) ? -v : v;
+}
+
+LGD_TRIG_HD inline double lgd_log_unit(double x) {
+  uint64_t bits;
+  __builtin_memcpy(&bits, &x, 8);
+  const int32_t hi = (int32_t)(bits >> 32);
+  const int32_t hx = hi & 0x000fffff;             // top 20 mantissa bits
+  const int32_t half = (hx + 0x95f64) & 0x100000;  // set when 1.m >= √2: halve
+  const int k = ((hi >> 20) - 1023) + (half >> 20);
+  const uint64_t mbits = ((uint64_t)(uint32_t)(hx | (half ^ 0x3ff00000)) << 32) |
+                         (bits & 0xffffffffull);
+  const double f = lgd_bits_to_double(mbits) - 1.0;
+  const double lg1 = lgd_bits_to_double(0x3fe5555555555593ull);
+  const double lg2 = lgd_bits_to_double(0x3fd999999997fa04ull);
+  const double lg3 = lgd_bits_to_double(0x3fd2492494229359ull);
+  const double lg4 = lgd_bits_to_double(0x3fcc71c51d8e78afull);
+  const double lg5 = lgd_bits_to_double(0x3fc7466496cb03deull);
+  const double lg6 = lgd_bits_to_double(0x3fc39a09d078c69full);
+  const double lg7 = lgd_bits_to_double(0x3fc2f112df3e5244ull);
+  const double ln2_hi = lgd_bits_to_double(0x3fe62e42fee00000ull);
+  const double ln2_lo = lgd_bits_to_double(0x3dea39ef35793c76ull);
+  const double s = f / (2.0 + f);
+  const double dk = (double)k;
+  const double z = s * s;
+  const double w = z * z;
+  const double t1 = w * fma(w, fma(w, lg6, lg4), lg2);
+  const double t2 = z * fma(w, fma(w, fma(w, lg7, lg5), lg3), lg1);
+  const double r = t2 + t1;
+  // FDLIBM's two final forms on one evaluation path. Near the ends of the reduced
+  // range: q = p = hf², giving hf² − (s·(hf² + R) + k·ln2_lo). Elsewhere q = 0,
+  // p = −f, giving −(s·(R − f) + k·ln2_lo), which is s·(f − R) − k·ln2_lo exactly
+  // (rounding is symmetric under negation; a zero's sign is lost in the − f).
+  const bool ends = ((hx - 0x6147a) | (0x6b851 - hx)) > 0;
+  const double hfsq = 0.5 * f * f;
+  const double q = ends ? hfsq : 0.0;
+  const double p = ends ? hfsq : -f;
+  return dk * ln2_hi - ((q - (s * (p + r) + dk * ln2_lo)) - f);
 }

@@ -165,6 +165,29 @@ def test_wiener_increments_match_oracle(ctx, orc):
     assert np.mean(got == x) > 0.9
 
 
+def test_normal_draws_are_within_a_few_ulp_of_the_reference(ctx, orc):
+    """x = 0, amplitude 1, dt = 1: one Wiener update writes N(key) itself, for 2^20
+    cells. The device draw (the same key chain, the reduced-range log and cosine,
+    the correctly rounded sqrt) against the oracle's glibc draw: the log and the
+    cosine are each within one ulp, so the deviate is within a few."""
+    say("normal draws vs the reference's, 2^20 cells")
+    cells, block = [1024, 1024], [256, 256]
+    og, gg = grids(ctx, orc, cells, block, [0.4, 0.4])
+    x = np.zeros(orc.field_len(og, 1))
+    amp = np.ones_like(x)
+    gx, ga = upload(gg, x), upload(gg, amp)
+    salt = lg.time_bits(0.008 * 3)
+    orc.apply_wiener(og, x, amp, 1, 1.0, 42, salt, (1 << 32) | 0)
+    gg.apply_wiener(gx, ga, 1, 1.0, 42, salt, (1 << 32) | 0)
+    got = gx.download()
+    interior = x != 0.0
+    assert interior.sum() == 1024 * 1024 and np.array_equal(got[~interior], x[~interior])
+    ulps = np.abs(got[interior] - x[interior]) / np.spacing(np.abs(x[interior]))
+    print(f"max {ulps.max():.2f} ulp, mean {ulps.mean():.4f}, exact {np.mean(ulps == 0):.4f}", flush=True)
+    assert ulps.max() <= 4.0 and ulps.mean() < 0.3
+    assert np.mean(ulps == 0) > 0.75
+
+
 # ------------------------------------------------------------------- heat
 @pytest.mark.parametrize("mode", [lg.LGD_MODE_FUSED, lg.LGD_MODE_PER_INTENT])
 def test_heat3d_config_matches_oracle_bitwise(ctx, orc, mode):
