@@ -39,7 +39,7 @@ def run_gpu(args, C):
 
     stream = torch.cuda.Stream()  # default priority
     ctx = lg.Context(0, stream.cuda_stream)
-    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2}[args.kernel])
+    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2, "pair": 3}[args.kernel])
     ctx.set_tuning("modelc_strips", args.strips)
     ctx.set_tuning("modelc_prefetch", args.prefetch)
     grid = lg.CartesianGrid(ctx, [C.cells] * 2, [C.block] * 2, [0.0, 0.0], [C.h, C.h])
@@ -160,7 +160,7 @@ def run_adjoint(args) -> int:
     A = AdjointConfig(steps=args.adj_steps or AdjointConfig.steps,
                       snapshots=args.snapshots or AdjointConfig.snapshots)
     ctx = lg.Context(0)
-    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2}[args.kernel])
+    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2, "pair": 3}[args.kernel])
     ctx.set_tuning("modelc_strips", args.strips)
     ctx.set_tuning("adjoint_kernel", {"auto": 0, "passes": 1, "stream": 2}[args.adj_kernel])
     ctx.set_tuning("adjoint_rows", args.adj_rows)
@@ -325,8 +325,11 @@ def run_ensemble(args) -> int:
     E = EnsembleConfig(steps=args.ens_steps or EnsembleConfig.steps)
     members = E.members()
     ctx = lg.Context(0)
+    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2, "pair": 3}[args.kernel])
     if args.strips:
         ctx.set_tuning("modelc_strips", args.strips)
+    if args.prefetch:
+        ctx.set_tuning("modelc_prefetch", args.prefetch)
     reals = [lg.Realization((m["side"], m["side"]), m["eps4"], E.lambda_, E.d_thermal, m["b"],
                             seed=m["seed"]) for m in members]
     cells = sum(m["side"] ** 2 for m in members)
@@ -869,7 +872,7 @@ def main() -> int:
     ap.add_argument("--mode", choices=["fused", "per-intent"], default="fused")
     ap.add_argument("--cells", type=int, default=10080)
     ap.add_argument("--block", type=int, default=126)
-    ap.add_argument("--kernel", choices=["auto", "percell", "stream"], default="auto")
+    ap.add_argument("--kernel", choices=["auto", "percell", "stream", "pair"], default="auto")
     ap.add_argument("--strips", type=int, default=0, help="row strips per block (0 = default)")
     ap.add_argument("--prefetch", type=int, default=0, choices=[0, 1, 2],
                     help="rows the streaming kernel loads ahead (0 = default)")

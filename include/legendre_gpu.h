@@ -98,7 +98,9 @@ int lgd_ctx_sync(lgd_ctx *ctx);
 void *lgd_ctx_stream(lgd_ctx *ctx);
 const char *lgd_ctx_last_error(const lgd_ctx *ctx);
 /* Launch-shape knobs for this context (no reference counterpart: the reference has
- * no launch shapes).  Keys: "modelc_kernel" (0 auto, 1 per-cell, 2 streaming),
+ * no launch shapes).  Keys: "modelc_kernel" (0 auto, 1 per-cell, 2 streaming, 3
+ * streaming with two x-neighbouring blocks per wavefront where the blocks are at most
+ * 64 cells wide, both even in number per block row and in width, else as 2),
  * "modelc_strips" (row strips per block for the streaming kernel, 0 = default) and
  * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default) and
  * "amr_per_block" (1: the AMR level kernels launch once per patch instead of once per
@@ -116,7 +118,8 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value);
  * 1..32 strips per block and one or two rows loaded ahead — on (phi, u) over this
  * grid (outputs to scratch), `reps` launches each, and keeps the fastest for this
  * (block shape, block count, noise) in the context; lgd_modelc_step and the ensemble
- * step use it whenever no knob is set.  kernel: 1 per-cell, 2 streaming. */
+ * step use it whenever no knob is set.  kernel: 1 per-cell, 2 streaming, 3 streaming
+ * with block pairs. */
 typedef struct {
   int32_t kernel;
   int32_t strips;

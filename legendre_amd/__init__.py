@@ -101,7 +101,7 @@ class Context:
         _abi.check(self._lib.lgd_autotune_modelc_step(self.handle, grid.handle, C.byref(pc), phi.handle,
                                                       u.handle, dt, int(wiener), int(reps), C.byref(r)),
                    self.handle)
-        return {"kernel": "percell" if r.kernel == 1 else "stream", "strips": r.strips,
+        return {"kernel": {1: "percell", 2: "stream", 3: "pair"}[r.kernel], "strips": r.strips,
                 "prefetch": r.prefetch, "ms": r.ms, "default_ms": r.default_ms,
                 "candidates": r.candidates}
 

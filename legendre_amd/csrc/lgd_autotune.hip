@@ -15,9 +15,8 @@
 using namespace lgd;
 
 lgd_launch_shape lgd_modelc_shape(const lgd_ctx *ctx, int64_t nx, int64_t ny, int64_t nblocks,
-                                  bool noise, int32_t default_strips) {
-  lgd_launch_shape s;
-  s.strips = default_strips;
+                                  bool noise, const lgd_launch_shape &dflt) {
+  lgd_launch_shape s = dflt;
   const bool explicit_knobs =
       ctx->modelc_kernel != 0 || ctx->modelc_strips != 0 || ctx->modelc_prefetch != 0;
   if (!explicit_knobs) {
@@ -25,7 +24,7 @@ lgd_launch_shape lgd_modelc_shape(const lgd_ctx *ctx, int64_t nx, int64_t ny, in
     if (it != ctx->tuned.end()) return it->second;
     return s;
   }
-  if (ctx->modelc_kernel == 1) s.kernel = 1;
+  if (ctx->modelc_kernel != 0) s.kernel = (int32_t)ctx->modelc_kernel;
   if (ctx->modelc_strips > 0) s.strips = (int32_t)ctx->modelc_strips;
   if (ctx->modelc_prefetch > 0) s.prefetch = (int32_t)ctx->modelc_prefetch;
   return s;
@@ -79,10 +78,14 @@ int lgd_autotune_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_model
   const bool noise = wiener && params->noise_amplitude != 0.0;
   std::vector<lgd_launch_shape> cand;
   cand.push_back(lgd_launch_shape{1, 0, 0});
+  const bool pairable = lgd_stream_pairable(c.n[0], c.nb[0]);
   if (c.n[0] + 2 <= 128)
     for (int32_t strips : {1, 2, 4, 8, 16, 32})
       if (strips <= c.n[1])
-        for (int32_t pf : {1, 2}) cand.push_back(lgd_launch_shape{2, strips, pf});
+        for (int32_t pf : {1, 2}) {
+          cand.push_back(lgd_launch_shape{2, strips, pf});
+          if (pairable) cand.push_back(lgd_launch_shape{3, strips, pf});
+        }
   lgd_buf *pn = nullptr, *un = nullptr;
   const size_t len = phi->len;
   if ((rc = lgd_buf_alloc(ctx, len, &pn))) return rc;

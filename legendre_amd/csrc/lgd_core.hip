@@ -126,7 +126,7 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
   if (value < 0) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "tuning values are non-negative");
   ++ctx->tuning_epoch;
   if (k == "modelc_kernel") {
-    if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "modelc_kernel is 0, 1 or 2");
+    if (value > 3) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "modelc_kernel is 0, 1, 2 or 3");
     ctx->modelc_kernel = value;
   } else if (k == "modelc_strips") {
     ctx->modelc_strips = value;

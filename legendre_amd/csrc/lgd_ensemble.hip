@@ -225,7 +225,14 @@ int lgd_ensemble_step(lgd_ensemble *e, double dt, int64_t nsteps, int32_t per_bl
   bool noise = false;
   for (const lgd_realization &R : e->real) noise = noise || R.noise_amplitude != 0.0;
   const int64_t nblocks = e->block0.back();
-  const lgd_launch_shape shape = lgd_modelc_shape(e->ctx, e->nx, e->ny, nblocks, noise, 1);
+  // Block pairs (kernel 3) when every realization's block rows pair up (then every
+  // realization's first block is even too); by default with one strip per block
+  // and one row loaded ahead, else one block per wave, one strip, two rows ahead.
+  bool pairable = true;
+  for (const lgd_domain &D : e->domains) pairable = pairable && lgd_stream_pairable(e->nx, D.nbx);
+  const lgd_launch_shape dflt = pairable ? lgd_launch_shape{3, 1, 1} : lgd_launch_shape{2, 1, 2};
+  const lgd_launch_shape shape = lgd_modelc_shape(e->ctx, e->nx, e->ny, nblocks, noise, dflt);
+  const bool pair = shape.kernel == 3 && pairable;
   const uint64_t wiener0 = (1ull << 32) | 0ull;
   for (int64_t s = 0; s < nsteps; ++s) {
     uint64_t salt;
@@ -234,7 +241,7 @@ int lgd_ensemble_step(lgd_ensemble *e, double dt, int64_t nsteps, int32_t per_bl
       rc = lgd_modelc_stream_launch(e->ctx, e->nx, e->ny, e->domains[0], e->d_domains,
                                     e->d_block_domain, 0, nblocks, e->state[0]->data,
                                     e->state[1]->data, e->next[0]->data, e->next[1]->data, dt,
-                                    noise, salt, wiener0, shape.strips, shape.prefetch);
+                                    noise, salt, wiener0, shape.strips, shape.prefetch, pair);
     } else {
       // The per-block dispatch baseline: one launch per block, each in its
       // fastest single-block shape (one wave per row).

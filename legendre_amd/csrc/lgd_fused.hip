@@ -268,7 +268,7 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
   const bool noise_on = wiener && params->noise_amplitude != 0.0;
   const double noise_scale = std::sqrt(dt) * params->noise_amplitude;
   const uint64_t wiener0 = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
-  const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, noise_on, 4);
+  const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, noise_on, lgd_launch_shape{});
   const bool oriented = cs0 != nullptr;
   if (shape.kernel != 1 && !oriented) {
     lgd_domain d0;
@@ -280,11 +280,13 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
     d0.nby = (int32_t)c.nb[1];
     d0.per_x = c.periodic[0];
     d0.per_y = c.periodic[1];
+    // Block pairs where the grid allows them (kernel 3), else one block per wave.
+    const bool pair = shape.kernel == 3 && lgd_stream_pairable(c.n[0], c.nb[0]);
     rc = lgd_modelc_stream_launch(ctx, c.n[0], c.n[1], d0, nullptr, nullptr, 0, c.nblocks,
                                   phi->data, u->data, phi_next->data, u_next->data, dt, noise_on,
-                                  salt, wiener0, shape.strips, shape.prefetch);
+                                  salt, wiener0, shape.strips, shape.prefetch, pair);
     if (rc != LGD_ERR_UNSUPPORTED) return rc;
-    if (ctx->modelc_kernel == 2)
+    if (ctx->modelc_kernel >= 2)
       return fail(ctx, LGD_ERR_UNSUPPORTED, "streaming kernel needs blocks at most 126 cells wide");
   }
   int64_t tiles_x = (c.n[0] + kTX - 1) / kTX;

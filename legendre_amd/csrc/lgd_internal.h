@@ -44,10 +44,10 @@ struct lgd_ctx {
 };
 
 // The launch shape for a Model C step over `nblocks` blocks of nx × ny: explicit
-// knobs first, then the autotuned shape for this key, then the defaults
-// (streaming, `default_strips` strips, two rows ahead).
+// knobs first (over the fields of `dflt` they leave unset), then the autotuned
+// shape for this key, then `dflt`.
 lgd_launch_shape lgd_modelc_shape(const lgd_ctx *ctx, int64_t nx, int64_t ny, int64_t nblocks,
-                                  bool noise, int32_t default_strips);
+                                  bool noise, const lgd_launch_shape &dflt);
 
 struct lgd_buf {
   lgd_ctx *ctx = nullptr;
@@ -196,12 +196,18 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
 
 // Streaming Model C step (lgd_modelc_stream.hip) over blocks
 // [block_begin, block_begin + nblocks); LGD_ERR_UNSUPPORTED when the block is
-// wider than 126 interior cells.
+// wider than 126 interior cells.  `pair` runs two x-neighbouring blocks per wave
+// (blocks at most 64 wide; see lgd_stream_pairable).
 int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
                              const lgd_domain *domains, const int32_t *block_domain,
                              int64_t block_begin, int64_t nblocks, const double *phi,
                              const double *u, double *phi_n, double *u_n, double dt, bool noise,
-                             uint64_t salt, uint64_t stream, int strips, int prefetch);
+                             uint64_t salt, uint64_t stream, int strips, int prefetch,
+                             bool pair = false);
+
+// Whether a domain of nbx blocks of nx interior columns per row can run as block
+// pairs: nx even, 4 <= nx <= 64, nbx even.
+bool lgd_stream_pairable(int64_t nx, int64_t nbx);
 
 // Streaming reverse Model C step (lgd_adjoint_stream.hip): adjoint (APi, AUi) of the
 // state after the step → (APo, AUo) before it, global N0 × N1 arrays; θ partials

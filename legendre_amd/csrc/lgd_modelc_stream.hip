@@ -21,6 +21,15 @@
 // the block below/above, resolved per dimension in global coordinates — the values
 // fill_ghosts_mirror would have written (src/geometry/cartesian.rs:439-526).
 //
+// Block pairs: a block at most 64 cells wide leaves half the wave idle, so such
+// blocks can also run two to a wave: blocks b and b + 1, x-neighbours in one
+// block row, advanced over the same rows.  Lane L holds interior columns 2L and
+// 2L + 1 of the 2·nx-wide pair; the face between the two blocks comes from the
+// lane shift like any other, and the pair's two outer halo columns are loaded by
+// lanes 0 and nx - 1 into one extra register per row and field.  The x-face left
+// of the pair's first column is the one face no lane shift supplies: every lane
+// evaluates one more face per row, and lane 0 keeps it.
+//
 // Scheduling: workgroups of four independent waves; workgroup ids are remapped so
 // that each of the eight XCDs sweeps one contiguous range of blocks in order, so
 // neighbouring blocks (whose edge columns a wave reads) are in flight on the same
@@ -49,7 +58,7 @@ struct StreamArgs {
   uint64_t salt;
   const double *salt_src;  // time to take the salt from (graph capture), or null
   uint64_t stream;
-  int64_t nblocks;      // blocks in this launch
+  int64_t nblocks;      // blocks (block pairs) in this launch
   int64_t block_begin;  // first block of this launch
   int32_t nx, ny;      // block interior extent
   int32_t strips;      // strips per block
@@ -82,9 +91,10 @@ __device__ __forceinline__ int64_t row_base(const StreamArgs &a, const lgd_domai
 // Two consecutive f64 columns as one 16-byte vector (for nontemporal stores).
 typedef double F64x2 __attribute__((ext_vector_type(2)));
 
-// One row of a field as this lane holds it: two consecutive columns.
+// One row of a field as this lane holds it: two consecutive columns and, for a
+// block pair, the halo column value lanes 0 and nx - 1 need.
 struct RowVal {
-  double v0, v1;
+  double v0, v1, h;
 };
 
 // Cross-lane moves by one lane (DPP wavefront shifts: no LDS round trip).  The
@@ -153,7 +163,7 @@ __device__ __forceinline__ T load_uniform(const T *p) {
   return out;
 }
 
-template <bool kNoise, bool kVec, bool kTable, int kDepth>
+template <bool kNoise, bool kVec, bool kTable, int kDepth, bool kPair>
 __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamArgs a) {
   // XCD-aware workgroup order: XCD x (= blockIdx.x % 8) takes one contiguous
   // range of logical workgroups.
@@ -167,7 +177,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   const int lane = threadIdx.x & 63;
   const int64_t item = (int64_t)lw * kWavesPerWG + wave;
   if (item >= a.nblocks * a.strips) return;  // whole wave exits together
-  const int64_t b = a.block_begin + item / a.strips;
+  // One block, or (kPair) blocks b and b + 1 of one block row.
+  const int64_t b = a.block_begin + (kPair ? 2 : 1) * (item / a.strips);
   const int strip = (int)(item % a.strips);
   const int j0 = strip * a.rows;
   const int j1 = min(j0 + a.rows, a.ny);
@@ -180,60 +191,115 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   const int64_t slab = (int64_t)sx * (a.ny + 2);
   const lgd_kr_consts &k = D.k;
 
-  // Halo columns: global column just left / right of this block, resolved.
+  // Halo columns: global column just left / right of this block (pair), resolved.
   const int64_t NX = (int64_t)D.nbx * nx;
   int64_t gl = wrap_or_reflect(bx * nx - 1, NX, D.per_x);
-  int64_t gr = wrap_or_reflect(bx * nx + nx, NX, D.per_x);
+  int64_t gr = wrap_or_reflect((bx + (kPair ? 2 : 1)) * nx, NX, D.per_x);
   const int64_t bxl = gl / nx, bxr = gr / nx;
   const int il = (int)(gl - bxl * nx), ir = (int)(gr - bxr * nx);
-  const int hr_lane = (nx + 1) >> 1, hr_slot = (nx + 1) & 1;
+  const int64_t lcol = bxl * slab + (il + 1), rcol = bxr * slab + (ir + 1);
 
-  const int c0 = 2 * lane, c1 = c0 + 1;
-  const bool in0 = c0 >= 1 && c0 <= nx;
-  const bool in1 = c1 <= nx;
+  // Per-lane column offsets (from the start of a slab row in block column 0),
+  // valid for every row, the lane's output offset within a slab row (obase) and
+  // the interior column of its first cell (kc0, for the cell key).
+  int64_t col0, col1, hcol = 0, obase;
+  bool in0, in1;
+  int kc0, seg = 0;
+  if constexpr (kPair) {
+    // Lanes 0..nx-1 hold the pair's 2·nx interior columns (nx even, so a lane's
+    // two columns are in one block); lanes past them re-read block b's first
+    // columns and store nothing.  Lane 0 also reads the left halo column, lane
+    // nx - 1 the right one.
+    const bool act = lane < nx;
+    seg = 2 * lane >= nx ? 1 : 0;
+    const int ic = act ? 2 * lane - seg * nx : 0;
+    col0 = (bx + seg) * slab + (ic + 1);
+    col1 = col0 + 1;
+    hcol = lane == 0 ? lcol : (lane == nx - 1 ? rcol : col0);
+    obase = (b + seg) * slab + (ic + 1);
+    in0 = act;
+    in1 = act;
+    kc0 = ic;
+  } else {
+    // Lane 0 reads the left halo cell in its first column, lane (nx+1)/2 the
+    // right halo cell in its first (nx odd) or second (nx even) column, lanes
+    // past the slab row re-read its last column (their values are never stored
+    // and never an interior column's neighbour).
+    const int hr_lane = (nx + 1) >> 1, hr_slot = (nx + 1) & 1;
+    const int c0 = 2 * lane, c1 = c0 + 1;
+    col0 = lane == 0 ? lcol
+           : (lane == hr_lane && hr_slot == 0) ? rcol
+                                               : bx * slab + min(c0, sx - 1);
+    col1 = (lane == hr_lane && hr_slot == 1) ? rcol : bx * slab + min(c1, sx - 1);
+    obase = b * slab + c0;
+    in0 = c0 >= 1 && c0 <= nx;
+    in1 = c1 <= nx;
+    kc0 = c0 - 1;
+  }
+  const bool st0 = kPair ? in0 : obase - b * slab < sx;      // column inside the slab row
+  const bool st1 = kPair ? in1 : obase - b * slab + 1 < sx;
 
   // Wiener(0) amplitude b on interior cells; zero amplitude draws nothing
   // (src/core/driver.rs:144-146).
   const bool noisy = kNoise && D.noise_scale != 0.0;
   uint64_t block_key = 0;
-  if (noisy) block_key = lgd_mix_key3(D.seed, lgd_salt(a.salt, a.salt_src), a.stream, (uint64_t)rb);
+  if (noisy) {
+    const uint64_t salt = lgd_salt(a.salt, a.salt_src);
+    block_key = lgd_mix_key3(D.seed, salt, a.stream, (uint64_t)rb);
+    if constexpr (kPair) {
+      const uint64_t key1 = lgd_mix_key3(D.seed, salt, a.stream, (uint64_t)(rb + 1));
+      block_key = seg ? key1 : block_key;
+    }
+  }
 
-  // Per-lane column offsets (from the start of a slab row in block column 0),
-  // valid for every row: lane 0 reads the left halo cell in its first column,
-  // lane (nx+1)/2 the right halo cell in its first (nx odd) or second (nx even)
-  // column, lanes past the slab row re-read its last column (their values are
-  // never stored and never an interior column's neighbour).
-  const int64_t lcol = bxl * slab + (il + 1), rcol = bxr * slab + (ir + 1);
-  const int64_t col0 = lane == 0 ? lcol
-                       : (lane == hr_lane && hr_slot == 0) ? rcol
-                                                           : bx * slab + min(c0, sx - 1);
-  const int64_t col1 = (lane == hr_lane && hr_slot == 1) ? rcol : bx * slab + min(c1, sx - 1);
-  auto fetch = [&](const double *f, int r) {
+  auto fetch = [&](const double *f, int r, bool edge) {
     const double *row = f + row_base(a, D, by, r, slab, sx);
-    return RowVal{row[col0], row[col1]};
+    RowVal v{row[col0], row[col1], 0.0};
+    if constexpr (kPair) {
+      if (edge) v.h = row[hcol];
+    }
+    return v;
+  };
+  // The neighbour column's value: from the lane to the left / right, or (block
+  // pair) the halo column at the pair's edges.  The shift runs on every lane.
+  auto lft = [&](double v, double edge) {
+    const double s = from_left(v);
+    if constexpr (kPair) return lane == 0 ? edge : s;
+    return s;
+  };
+  auto rgt = [&](double v, double edge) {
+    const double s = from_right(v);
+    if constexpr (kPair) return lane == nx - 1 ? edge : s;
+    return s;
   };
 
-  // Rows j0-1, j0, j0+1 of phi and u.
+  // Rows j0-1, j0, j0+1 of phi and u (h*: the pair's halo column, rows m, c, p).
   double pm0, pm1, pc0, pc1, pp0, pp1, um0, um1, uc0, uc1, up0, up1;
+  double hm = 0.0, hc = 0.0, hp = 0.0, huc = 0.0, hup = 0.0;
   {
-    RowVal r0 = fetch(a.phi, j0 - 1), r1 = fetch(a.phi, j0), r2 = fetch(a.phi, j0 + 1);
-    RowVal r3 = fetch(a.u, j0 - 1), r4 = fetch(a.u, j0), r5 = fetch(a.u, j0 + 1);
+    RowVal r0 = fetch(a.phi, j0 - 1, true), r1 = fetch(a.phi, j0, true),
+           r2 = fetch(a.phi, j0 + 1, true);
+    RowVal r3 = fetch(a.u, j0 - 1, false), r4 = fetch(a.u, j0, true), r5 = fetch(a.u, j0 + 1, true);
     pm0 = r0.v0; pm1 = r0.v1;
     pc0 = r1.v0; pc1 = r1.v1;
     pp0 = r2.v0; pp1 = r2.v1;
     um0 = r3.v0; um1 = r3.v1;
     uc0 = r4.v0; uc1 = r4.v1;
     up0 = r5.v0; up1 = r5.v1;
+    if constexpr (kPair) {
+      hm = r0.h; hc = r1.h; hp = r2.h;
+      huc = r4.h; hup = r5.h;
+    }
   }
   const double q = 0.25;
-  double pmR = from_right(pm0);
-  double pcL = from_left(pc1), pcR = from_right(pc0);
-  double ppL = from_left(pp1), ppR = from_right(pp0);
-  double ucL = from_left(uc1), ucR = from_right(uc0);
+  double pmR = rgt(pm0, hm);
+  double pcL = lft(pc1, hc), pcR = rgt(pc0, hc);
+  double ppL = lft(pp1, hp), ppR = rgt(pp0, hp);
+  double ucL = lft(uc1, huc), ucR = rgt(uc0, huc);
   // Flux through the bottom face of row j0 (the top face of row j0-1).
   double jb0, jb1;
   {
-    double pmL = from_left(pm1);
+    double pmL = lft(pm1, hm);
     jb0 = flux_y(k, q * (((pc1 - pcL) + pm1) - pmL), pc0 - pm0);
     jb1 = flux_y(k, q * (((pcR - pc0) + pmR) - pm0), pc1 - pm1);
   }
@@ -248,12 +314,18 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     // Issued on every row (past the strip's last needed row it re-reads row j1,
     // unused) so the wait for the consumed row never depends on a branch.
     const int rn = min(j + 1 + kDepth, j1);
-    ip = fetch(a.phi, rn);
-    iu = fetch(a.u, rn);
+    ip = fetch(a.phi, rn, true);
+    iu = fetch(a.u, rn, true);
     // x-faces right of c0 and c1; the face left of c0 is the left lane's right-of-c1.
     double jx0 = flux_x(k, pc1 - pc0, q * (((pp1 - pm1) + pp0) - pm0));
     double jx1 = flux_x(k, pcR - pc1, q * (((ppR - pmR) + pp1) - pm1));
     double jxl0 = from_left(jx1);
+    if constexpr (kPair) {
+      // The face between the left halo column and the pair's first column
+      // (derx_l, dery_l of src/discretization/finite_volume.rs:261-280).
+      const double je = flux_x(k, pc0 - hc, q * (((pp0 - pm0) + hp) - hm));
+      jxl0 = lane == 0 ? je : jxl0;
+    }
     // y-faces above c0 and c1.
     double jt0 = flux_y(k, q * (((pp1 - ppL) + pc1) - pcL), pp0 - pc0);
     double jt1 = flux_y(k, q * (((ppR - pp0) + pcR) - pc0), pp1 - pc1);
@@ -266,8 +338,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
       // Wiener(0) on phi after the drift (src/core/driver.rs:139-158); cell key =
       // interior linear index (src/geometry/cartesian.rs:599-619).
       uint64_t rowkey = (uint64_t)j * (uint64_t)nx;
-      if (in0) n0 = n0 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c0 - 1))));
-      if (in1) n1 = n1 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(c1 - 1))));
+      if (in0) n0 = n0 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)kc0)));
+      if (in1) n1 = n1 + D.noise_scale * lgd_standard_normal(lgd_splitmix64(block_key ^ (rowkey + (uint64_t)(kc0 + 1))));
     }
     jb0 = jt0;
     jb1 = jt1;
@@ -283,41 +355,75 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
       n1 = pc1;
       v1 = uc1;
     }
+    // Block pair: the ghost columns of both output slabs take this row's halo
+    // values, as the single-block form stores them (lane 0: left halo; lane
+    // nx/2 - 1: block b's last column, block b + 1's left ghost; lane nx/2: block
+    // b + 1's first column, block b's right ghost; lane nx - 1: right halo).
+    int64_t go = -1;
+    double gp = 0.0, gu = 0.0;
+    if constexpr (kPair) {
+      const int64_t ro = (int64_t)(j + 1) * sx;
+      if (lane == 0) {
+        go = b * slab + ro;
+        gp = hc;
+        gu = huc;
+      } else if (lane == nx / 2 - 1) {
+        go = (b + 1) * slab + ro;
+        gp = pc1;
+        gu = uc1;
+      } else if (lane == nx / 2) {
+        go = b * slab + ro + (nx + 1);
+        gp = pc0;
+        gu = uc0;
+      } else if (lane == nx - 1) {
+        go = (b + 1) * slab + ro + (nx + 1);
+        gp = hc;
+        gu = huc;
+      }
+    }
     if (j + 1 < j1) {
       pm0 = pc0; pm1 = pc1; pmR = pcR;
       pc0 = pp0; pc1 = pp1; pcL = ppL; pcR = ppR;
       pp0 = cp.v0; pp1 = cp.v1;
-      ppL = from_left(pp1);
-      ppR = from_right(pp0);
+      if constexpr (kPair) {
+        hm = hc; hc = hp; hp = cp.h;
+        huc = hup; hup = cu.h;
+      }
+      ppL = lft(pp1, hp);
+      ppR = rgt(pp0, hp);
       um0 = uc0; um1 = uc1;
       uc0 = up0; uc1 = up1;
       up0 = cu.v0; up1 = cu.v1;
-      ucL = from_left(uc1);
-      ucR = from_right(uc0);
+      ucL = lft(uc1, huc);
+      ucR = rgt(uc0, huc);
     }
-    const int64_t o = b * slab + (int64_t)(j + 1) * sx + c0;
+    const int64_t o = obase + (int64_t)(j + 1) * sx;
     // Streaming (nontemporal) stores: the new state is not read again in this
     // launch, so it need not displace the halo lines neighbouring waves re-read.
     if (kVec) {
-      if (c1 < sx) {
+      if (st1) {
         __builtin_nontemporal_store(F64x2{n0, n1}, reinterpret_cast<F64x2 *>(a.phi_n + o));
         __builtin_nontemporal_store(F64x2{v0, v1}, reinterpret_cast<F64x2 *>(a.u_n + o));
       }
     } else {
-      if (c0 < sx) {
+      if (st0) {
         __builtin_nontemporal_store(n0, a.phi_n + o);
         __builtin_nontemporal_store(v0, a.u_n + o);
       }
-      if (c1 < sx) {
+      if (st1) {
         __builtin_nontemporal_store(n1, a.phi_n + o + 1);
         __builtin_nontemporal_store(v1, a.u_n + o + 1);
       }
     }
+    if (kPair && go >= 0) {
+      __builtin_nontemporal_store(gp, a.phi_n + go);
+      __builtin_nontemporal_store(gu, a.u_n + go);
+    }
   };
   RowVal ap{}, au{}, bp{}, bu{};
   if (kDepth == 2) {
-    ap = fetch(a.phi, min(j0 + 2, j1));
-    au = fetch(a.u, min(j0 + 2, j1));
+    ap = fetch(a.phi, min(j0 + 2, j1), true);
+    au = fetch(a.u, min(j0 + 2, j1), true);
   }
   for (int j = j0; j < j1; j += 2) {
     if (kDepth == 2) {
@@ -332,8 +438,18 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
 
 template <int kDepth>
 void launch_variant(dim3 grid, dim3 block, hipStream_t st, const StreamArgs &a, bool noise,
-                    bool vec, bool table) {
-#define LGD_STREAM_LAUNCH(N, V, T) modelc_stream_kernel<N, V, T, kDepth><<<grid, block, 0, st>>>(a)
+                    bool vec, bool table, bool pair) {
+  if (pair) {  // block pairs store column by column (odd slab columns start a lane's pair)
+#define LGD_PAIR_LAUNCH(N, T) modelc_stream_kernel<N, false, T, kDepth, true><<<grid, block, 0, st>>>(a)
+    if (noise) {
+      if (table) LGD_PAIR_LAUNCH(true, true); else LGD_PAIR_LAUNCH(true, false);
+    } else {
+      if (table) LGD_PAIR_LAUNCH(false, true); else LGD_PAIR_LAUNCH(false, false);
+    }
+#undef LGD_PAIR_LAUNCH
+    return;
+  }
+#define LGD_STREAM_LAUNCH(N, V, T) modelc_stream_kernel<N, V, T, kDepth, false><<<grid, block, 0, st>>>(a)
   if (noise) {
     if (vec) {
       if (table) LGD_STREAM_LAUNCH(true, true, true); else LGD_STREAM_LAUNCH(true, true, false);
@@ -352,17 +468,26 @@ void launch_variant(dim3 grid, dim3 block, hipStream_t st, const StreamArgs &a, 
 
 }  // namespace
 
+bool lgd_stream_pairable(int64_t nx, int64_t nbx) {
+  return nx % 2 == 0 && nx >= 4 && nx <= 64 && nbx % 2 == 0;
+}
+
 // Streaming launch over blocks [block_begin, block_begin + nblocks) of a set of
 // domains sharing the block shape nx × ny.  Returns LGD_ERR_UNSUPPORTED when the
 // block is wider than one wavefront row (126 interior cells); the caller then
-// uses the per-cell kernel.
+// uses the per-cell kernel.  With `pair`, two x-neighbouring blocks share a wave:
+// the caller guarantees lgd_stream_pairable for every domain, an even block_begin
+// and nblocks, and even first blocks of every domain, so that blocks 2i and 2i + 1
+// of the launch are neighbours in one block row.
 int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
                              const lgd_domain *domains, const int32_t *block_domain,
                              int64_t block_begin, int64_t nblocks, const double *phi,
                              const double *u, double *phi_n, double *u_n, double dt, bool noise,
-                             uint64_t salt, uint64_t stream, int strips, int prefetch) {
+                             uint64_t salt, uint64_t stream, int strips, int prefetch, bool pair) {
   if (nx + 2 > 128 || nx < 1 || ny < 1) return LGD_ERR_UNSUPPORTED;
   if (nblocks <= 0) return LGD_OK;
+  if (pair && (!lgd_stream_pairable(nx, 2) || block_begin % 2 != 0 || nblocks % 2 != 0))
+    return lgd::fail(ctx, LGD_ERR_INVALID_ARGUMENT, "block pairs need an even block count and width");
   StreamArgs a;
   a.d0 = d0;
   a.domains = domains;
@@ -375,7 +500,7 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   a.salt = salt;
   a.salt_src = ctx->salt_src;
   a.stream = stream;
-  a.nblocks = nblocks;
+  a.nblocks = pair ? nblocks / 2 : nblocks;
   a.block_begin = block_begin;
   a.nx = (int32_t)nx;
   a.ny = (int32_t)ny;
@@ -383,7 +508,7 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   if (strips > a.ny) strips = a.ny;
   a.rows = (a.ny + strips - 1) / strips;
   a.strips = (a.ny + a.rows - 1) / a.rows;
-  int64_t items = nblocks * a.strips;
+  int64_t items = a.nblocks * a.strips;
   int64_t nwg = (items + kWavesPerWG - 1) / kWavesPerWG;
   if (nwg > INT32_MAX) return LGD_ERR_UNSUPPORTED;
   a.nwg = (int32_t)nwg;
@@ -392,8 +517,8 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   const int depth = prefetch == 1 ? 1 : 2;
   dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
   if (depth == 2)
-    launch_variant<2>(grid, block, ctx->stream, a, noise, vec, table);
+    launch_variant<2>(grid, block, ctx->stream, a, noise, vec, table, pair);
   else
-    launch_variant<1>(grid, block, ctx->stream, a, noise, vec, table);
+    launch_variant<1>(grid, block, ctx->stream, a, noise, vec, table, pair);
   return lgd::launch_check(ctx, "modelc_stream");
 }

@@ -73,3 +73,38 @@ def test_batched_equals_per_block_launches(ctx, orc):
     b.step(0.008, 20, per_block=True)
     assert np.array_equal(a.field(0).download(), b.field(0).download())
     assert np.array_equal(a.field(1).download(), b.field(1).download())
+
+
+PAIRED_SIDES = [(64, 64), (128, 32), (64, 128), (192, 96), (64, 32)]  # even blocks per row
+
+
+def test_block_pairs_equal_single_blocks_and_the_oracle(ctx, orc):
+    """Two x-neighbouring blocks per wave (every realization's block rows pair up):
+    bitwise equal to one block per wave with noise, and to the oracle without."""
+    say("ensemble block pairs")
+    outs = []
+    try:
+        for kernel in (3, 2):
+            ctx.set_tuning("modelc_kernel", kernel)
+            ens, _, _ = make_ensemble(ctx, PAIRED_SIDES, True)
+            ens.step(0.008, 30)
+            outs.append((ens.field(0).download(), ens.field(1).download()))
+            ens.close()
+        assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+        ctx.set_tuning("modelc_kernel", 3)
+        ens, reals, deltas = make_ensemble(ctx, PAIRED_SIDES, False)
+        ens.step(0.008, 40)
+        for i, (s, r) in enumerate(zip(PAIRED_SIDES, reals)):
+            g = orc.grid(list(s), [32, 32], [0, 0], [0.4, 0.4])
+            phi = np.zeros(orc.field_len(g, 1))
+            u = np.zeros_like(phi)
+            orc.initialize_grains(g, phi, u, np.array([[s[0] * 0.2, s[1] * 0.2, 2.0]]), float(deltas[i]))
+            p = orc.params(r.eps4, r.lambda_, r.d_thermal, 0.0)
+            orc.modelc_run(g, p, phi, u, 40, 0.008, nthreads=4)
+            assert np.array_equal(H.field_to_global(ens.realization_field(i, 0), list(s), [32, 32], 1),
+                                  H.field_to_global(phi, list(s), [32, 32], 1))
+            assert np.array_equal(H.field_to_global(ens.realization_field(i, 1), list(s), [32, 32], 1),
+                                  H.field_to_global(u, list(s), [32, 32], 1))
+        ens.close()
+    finally:
+        ctx.set_tuning("modelc_kernel", 0)

@@ -386,6 +386,84 @@ def test_fused_and_per_intent_agree_and_runs_are_deterministic(ctx, orc, tuned):
         assert np.array_equal(_interiors(o[1], cells, block), _interiors(outs[0][1], cells, block))
 
 
+PAIR_CASES = [
+    # (cells, block, periodic, grains): block rows of an even number of blocks at
+    # most 64 wide, so two x-neighbouring blocks share a wave
+    ([128, 64], [64, 32], [False, False], [[30.0, 12.0, 3.0]]),
+    ([256, 96], [64, 32], [True, False], [[0.8, 10.0, 3.0], [60.0, 30.0, 2.0]]),
+    ([64, 40], [16, 20], [True, True], [[12.0, 7.6, 2.0]]),
+    ([8, 12], [4, 3], [False, True], [[1.6, 2.4, 1.0]]),
+    ([128, 128], [32, 64], [False, False], [[12.8, 25.6, 3.0], [25.6, 12.0, 2.0]]),
+]
+PAIR_SHAPES = [(1, 2), (3, 1), (4, 2)]  # (strips, prefetch)
+
+
+@pytest.mark.parametrize("shape", PAIR_SHAPES)
+@pytest.mark.parametrize("case", range(len(PAIR_CASES)))
+def test_block_pairs_match_oracle_and_single_blocks_bitwise(ctx, orc, tuned, case, shape):
+    """Two blocks per wave: interiors bitwise equal to the oracle, and whole slabs
+    (the ghost columns the kernel stores included) bitwise equal to the
+    one-block-per-wave kernel, without and with noise."""
+    cells, block, periodic, grains = PAIR_CASES[case]
+    strips, prefetch = shape
+    say(f"block pairs cells={cells} block={block} periodic={periodic} strips={strips} pf={prefetch}")
+    dt = 0.008
+    outs = {}
+    for kernel in (3, 2):
+        tuned(kernel, strips, prefetch)
+        og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, periodic, np.array(grains),
+                                               0.0, False, lg.LGD_MODE_FUSED)
+        sim.step(dt, 60)
+        outs[kernel] = (sim.field(0).download(), sim.field(1).download())
+        sim.close()
+    orc.modelc_run(og, op, phi, u, 60, dt, nthreads=8)
+    assert np.array_equal(_interiors(outs[3][0], cells, block), _interiors(phi, cells, block))
+    assert np.array_equal(_interiors(outs[3][1], cells, block), _interiors(u, cells, block))
+    assert np.array_equal(outs[3][0], outs[2][0]) and np.array_equal(outs[3][1], outs[2][1])
+    noisy = {}
+    for kernel in (3, 2):
+        tuned(kernel, strips, prefetch)
+        _, _, sim, *_ = _modelc_pair(ctx, orc, cells, block, periodic, np.array(grains), 0.02, True,
+                                     lg.LGD_MODE_FUSED)
+        sim.step(dt, 25)
+        noisy[kernel] = (sim.field(0).download(), sim.field(1).download())
+        sim.close()
+    assert np.array_equal(noisy[3][0], noisy[2][0]) and np.array_equal(noisy[3][1], noisy[2][1])
+    assert not np.array_equal(noisy[3][0], outs[3][0])
+
+
+def test_block_pairs_noise_matches_oracle_within_tolerance(ctx, orc, tuned):
+    cells, block = [256, 128], [64, 64]
+    grains = np.array([[50.0, 25.0, 3.0]])
+    tuned(3, 2, 2)
+    say(f"block pairs b=0.01 cells={cells} block={block}")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], grains, 0.01, True,
+                                           lg.LGD_MODE_FUSED)
+    dt = 0.008
+    sim.step(dt, 1)
+    orc.modelc_run(og, op, phi, u, 1, dt, seed=42, wiener=True, nthreads=8)
+    g = _interiors(sim.field(0).download(), cells, block)
+    w = _interiors(phi, cells, block)
+    assert np.max(np.abs(g - w) / np.maximum(np.abs(w), 1e-300)) <= 1e-12
+    sim.close()
+
+
+def test_block_pairs_fall_back_where_blocks_do_not_pair(ctx, orc, tuned):
+    """An odd number of blocks per row, or blocks wider than 64: kernel 3 runs one
+    block per wave, with the same bits."""
+    for cells, block in (([192, 64], [64, 32]), ([252, 126], [126, 63]), ([30, 20], [6, 10])):
+        say(f"block pairs fallback cells={cells} block={block}")
+        outs = []
+        for kernel in (3, 2):
+            tuned(kernel, 2, 2)
+            _, _, sim, *_ = _modelc_pair(ctx, orc, cells, block, [False, False],
+                                         np.array([[10.0, 8.0, 2.0]]), 0.01, True, lg.LGD_MODE_FUSED)
+            sim.step(0.008, 10)
+            outs.append((sim.field(0).download(), sim.field(1).download()))
+            sim.close()
+        assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+
+
 @pytest.mark.slow
 def test_modelc_reduced_config_1000_steps_bitwise(ctx, orc):
     """The benchmark physics at 1008² (8×8 blocks of 126²), 64 seeds, 1000 steps."""
@@ -434,7 +512,7 @@ def test_autotuner_picks_a_shape_and_results_do_not_change(ctx, orc, tuned):
     params = lg.ModelCParams(0.05, 3.19, 2.0)
     r = ctx.autotune_modelc_step(gg, params, sim.field(0), sim.field(1), 0.008, reps=2)
     assert r["candidates"] >= 10 and r["ms"] > 0.0 and r["ms"] <= r["default_ms"] * 1.5
-    assert r["kernel"] in ("percell", "stream")
+    assert r["kernel"] in ("percell", "stream", "pair")
     sim.step(0.008, 30)
     tuned_out = (sim.field(0).download(), sim.field(1).download())
     tuned(2, 4, 2)  # the explicit default shape
