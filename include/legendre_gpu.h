@@ -115,7 +115,8 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
 int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value);
 /* Autotuner (no reference counterpart: the reference has no launch shapes).  Times
  * every launch shape of lgd_modelc_step — one thread per cell, or streaming with
- * 1..32 strips per block and one or two rows loaded ahead — on (phi, u) over this
+ * 1..32 strips per block and one or two rows loaded ahead, one block per wavefront
+ * or (where the grid allows) two — on (phi, u) over this
  * grid (outputs to scratch), `reps` launches each, and keeps the fastest for this
  * (block shape, block count, noise) in the context; lgd_modelc_step and the ensemble
  * step use it whenever no knob is set.  kernel: 1 per-cell, 2 streaming, 3 streaming
