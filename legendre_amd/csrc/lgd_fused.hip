@@ -345,6 +345,52 @@ int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_par
   a.k3p = kbuf[4]->data;
   a.k3u = kbuf[5]->data;
   const bool oriented = cs0 != nullptr;
+  // Unoriented grids with blocks at most 126 wide run the stages on the streaming
+  // kernel (face sharing; bitwise the same vector field), unless the per-cell
+  // kernel is asked for.
+  const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, false,
+                                                  lgd_launch_shape{});
+  if (!oriented && shape.kernel != 1 && c.n[0] + 2 <= 128) {
+    lgd_domain d0;
+    d0.k = k;
+    d0.noise_scale = 0.0;
+    d0.seed = 0;
+    d0.block0 = 0;
+    d0.nbx = (int32_t)c.nb[0];
+    d0.nby = (int32_t)c.nb[1];
+    d0.per_x = c.periodic[0];
+    d0.per_y = c.periodic[1];
+    lgd_rk4_stage r{};
+    r.y_phi = phi->data;
+    r.y_u = u->data;
+    r.k1p = kbuf[0]->data;
+    r.k1u = kbuf[1]->data;
+    r.k2p = kbuf[2]->data;
+    r.k2u = kbuf[3]->data;
+    r.k3p = kbuf[4]->data;
+    r.k3u = kbuf[5]->data;
+    r.w1 = a.w1;
+    r.w2 = a.w2;
+    r.w3 = a.w3;
+    r.w4 = a.w4;
+    for (int st = 1; st <= 4; ++st) {
+      const lgd_buf *in_p = st == 1 ? phi : tbuf[(st % 2 == 0) ? 0 : 2];
+      const lgd_buf *in_u = st == 1 ? u : tbuf[(st % 2 == 0) ? 1 : 3];
+      double *out_p = nullptr, *out_u = nullptr;
+      if (st < 4) {
+        r.k_phi = kbuf[2 * (st - 1)]->data;
+        r.k_u = kbuf[2 * (st - 1) + 1]->data;
+        out_p = tbuf[(st % 2 == 1) ? 0 : 2]->data;
+        out_u = tbuf[(st % 2 == 1) ? 1 : 3]->data;
+        r.coef = coef[st - 1];
+      }
+      int rc = lgd_modelc_stream_rk4_launch(ctx, c.n[0], c.n[1], d0, c.nblocks, in_p->data,
+                                            in_u->data, out_p, out_u, st, r, shape.strips,
+                                            shape.prefetch);
+      if (rc) return rc;
+    }
+    return LGD_OK;
+  }
   for (int st = 1; st <= 4; ++st) {
     const lgd_buf *in_p = st == 1 ? phi : tbuf[(st % 2 == 0) ? 0 : 2];
     const lgd_buf *in_u = st == 1 ? u : tbuf[(st % 2 == 0) ? 1 : 3];

@@ -209,6 +209,26 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
 // pairs: nx even, 4 <= nx <= 64, nbx even.
 bool lgd_stream_pairable(int64_t nx, int64_t nbx);
 
+// Operands of one RungeKutta4 stage on the streaming kernel: y (read; stage 4
+// writes the result into it), k_s (stages 1-3), k1..k3 (stage 4), the stage
+// coefficient c_s and the weights dt/6, dt/3, dt/3, dt/6.
+struct lgd_rk4_stage {
+  double *y_phi, *y_u;
+  double *k_phi, *k_u;
+  const double *k1p, *k1u, *k2p, *k2u, *k3p, *k3u;
+  double coef;
+  double w1, w2, w3, w4;
+};
+
+// RK4 stage `stage` (1..4) of the Model C vector field on the streaming kernel
+// over every block of one domain (blocks at most 126 wide, no orientation): the
+// stencil reads (pin, uin); stages 1-3 write k_s and the next input to (pout,
+// uout). Interior cells only. LGD_ERR_UNSUPPORTED when the block is too wide.
+int lgd_modelc_stream_rk4_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
+                                 int64_t nblocks, const double *pin, const double *uin,
+                                 double *pout, double *uout, int stage, const lgd_rk4_stage &rk,
+                                 int strips, int prefetch);
+
 // Streaming reverse Model C step (lgd_adjoint_stream.hip): adjoint (APi, AUi) of the
 // state after the step → (APo, AUo) before it, global N0 × N1 arrays; θ partials
 // (λ, D, ε₄ centre, ε₄ faces) added to acc[4·wave + n]; `rows` rows per strip (at

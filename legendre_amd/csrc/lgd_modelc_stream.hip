@@ -43,6 +43,12 @@ namespace {
 constexpr int kWavesPerWG = 4;
 constexpr int kXcds = 8;
 
+// A RungeKutta4 stage on the streaming kernel (kStage 1..4; src/integrators/rk4.rs):
+// the stencil reads the stage input (phi, u of StreamArgs); stages 1-3 write
+// k_s = f(input) and the next input y + c_s·k_s (to phi_n, u_n), stage 4 folds
+// y + w1·k1 + w2·k2 + w3·k3 + w4·f(input) in that order into y in place.
+using Rk4Stage = lgd_rk4_stage;
+
 struct StreamArgs {
   // The domains (independent Cartesian grids sharing the block shape) and,
   // per block, which domain it belongs to; a single grid passes its domain in
@@ -64,6 +70,7 @@ struct StreamArgs {
   int32_t strips;      // strips per block
   int32_t rows;        // rows per strip
   int32_t nwg;         // workgroups in the launch
+  Rk4Stage rk;         // kStage >= 1 only
 };
 
 __device__ __forceinline__ int64_t wrap_or_reflect(int64_t g, int64_t n, int periodic) {
@@ -134,7 +141,7 @@ __device__ __forceinline__ void cell_update(const lgd_kr_consts &k, double flux,
                                             double pe, double pw, double pn, double ps, double uc,
                                             double ue, double uw, double un, double us,
                                             double dt, double &phi_new, double &u_new,
-                                            double &dphi_out) {
+                                            double &dphi_out, double &du_out) {
   double g_prime = fma(pc * pc, pc, -pc);
   double one_m_p2 = fma(pc, -pc, 1.0);
   double p_prime = one_m_p2 * one_m_p2;
@@ -147,6 +154,7 @@ __device__ __forceinline__ void cell_update(const lgd_kr_consts &k, double flux,
   phi_new = pc + dt * dphi;
   u_new = uc + dt * du;
   dphi_out = dphi;
+  du_out = du;
 }
 
 // Read a table entry at a wave-uniform address into scalar registers (the
@@ -163,8 +171,9 @@ __device__ __forceinline__ T load_uniform(const T *p) {
   return out;
 }
 
-template <bool kNoise, bool kVec, bool kTable, int kDepth, bool kPair>
+template <bool kNoise, bool kVec, bool kTable, int kDepth, bool kPair, int kStage>
 __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamArgs a) {
+  static_assert(kStage == 0 || (!kNoise && !kPair && !kVec), "RK4 stages: no noise, one block");
   // XCD-aware workgroup order: XCD x (= blockIdx.x % 8) takes one contiguous
   // range of logical workgroups.
   const int w = blockIdx.x;
@@ -311,6 +320,30 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   // ago.  Two buffers alternate (the loop is unrolled by two) so no register
   // with a load in flight is copied.
   auto body = [&](int j, RowVal &ip, RowVal &iu, RowVal &cp, RowVal &cu) {
+    // RK4 stage: the pointwise operands at row j (interior columns only), used
+    // once the row's vector field is known.  They are issued before the rows
+    // loaded ahead: loads complete in order, so waiting for them then never waits
+    // for the look-ahead.
+    const int64_t o = obase + (int64_t)(j + 1) * sx;
+    double ry[4] = {0.0, 0.0, 0.0, 0.0}, rk[12] = {};
+    if constexpr (kStage >= 1) {
+      if (in0) {
+        ry[0] = a.rk.y_phi[o];
+        ry[2] = a.rk.y_u[o];
+      }
+      if (in1) {
+        ry[1] = a.rk.y_phi[o + 1];
+        ry[3] = a.rk.y_u[o + 1];
+      }
+    }
+    if constexpr (kStage == 4) {
+      const double *src[6] = {a.rk.k1p, a.rk.k2p, a.rk.k3p, a.rk.k1u, a.rk.k2u, a.rk.k3u};
+#pragma unroll
+      for (int q = 0; q < 6; ++q) {
+        if (in0) rk[2 * q] = src[q][o];
+        if (in1) rk[2 * q + 1] = src[q][o + 1];
+      }
+    }
     // Issued on every row (past the strip's last needed row it re-reads row j1,
     // unused) so the wait for the consumed row never depends on a branch.
     const int rn = min(j + 1 + kDepth, j1);
@@ -331,9 +364,9 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     double jt1 = flux_y(k, q * (((ppR - pp0) + pcR) - pc0), pp1 - pc1);
     double f0 = (((jx0 - jxl0) + jt0) - jb0) * k.inv_h2;
     double f1 = (((jx1 - jx0) + jt1) - jb1) * k.inv_h2;
-    double n0, n1, v0, v1, d0, d1;
-    cell_update(k, f0, pc0, pc1, pcL, pp0, pm0, uc0, uc1, ucL, up0, um0, a.dt, n0, v0, d0);
-    cell_update(k, f1, pc1, pcR, pc0, pp1, pm1, uc1, ucR, uc0, up1, um1, a.dt, n1, v1, d1);
+    double n0, n1, v0, v1, d0, d1, e0, e1;
+    cell_update(k, f0, pc0, pc1, pcL, pp0, pm0, uc0, uc1, ucL, up0, um0, a.dt, n0, v0, d0, e0);
+    cell_update(k, f1, pc1, pcR, pc0, pp1, pm1, uc1, ucR, uc0, up1, um1, a.dt, n1, v1, d1, e1);
     if (noisy) {
       // Wiener(0) on phi after the drift (src/core/driver.rs:139-158); cell key =
       // interior linear index (src/geometry/cartesian.rs:599-619).
@@ -397,7 +430,54 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
       ucL = lft(uc1, huc);
       ucR = rgt(uc0, huc);
     }
-    const int64_t o = obase + (int64_t)(j + 1) * sx;
+    if constexpr (kStage >= 1 && kStage <= 3) {
+      // k_s and the next stage's input y + c_s·k_s, interior cells only (the
+      // ghost ring of these buffers is never read).
+      const double c = a.rk.coef;
+      if (in0) {
+        a.rk.k_phi[o] = d0;
+        a.rk.k_u[o] = e0;
+        a.phi_n[o] = ry[0] + c * d0;
+        a.u_n[o] = ry[2] + c * e0;
+      }
+      if (in1) {
+        a.rk.k_phi[o + 1] = d1;
+        a.rk.k_u[o + 1] = e1;
+        a.phi_n[o + 1] = ry[1] + c * d1;
+        a.u_n[o + 1] = ry[3] + c * e1;
+      }
+      return;
+    }
+    if constexpr (kStage == 4) {
+      // y + (dt/6)·k1 + (dt/3)·k2 + (dt/3)·k3 + (dt/6)·k4, one axpy after another
+      // (src/integrators/rk4.rs), in place: y is read only at its own cell.
+      const Rk4Stage &r = a.rk;
+      if (in0) {
+        double y = ry[0];
+        y = y + r.w1 * rk[0];
+        y = y + r.w2 * rk[2];
+        y = y + r.w3 * rk[4];
+        r.y_phi[o] = y + r.w4 * d0;
+        double v = ry[2];
+        v = v + r.w1 * rk[6];
+        v = v + r.w2 * rk[8];
+        v = v + r.w3 * rk[10];
+        r.y_u[o] = v + r.w4 * e0;
+      }
+      if (in1) {
+        double y = ry[1];
+        y = y + r.w1 * rk[1];
+        y = y + r.w2 * rk[3];
+        y = y + r.w3 * rk[5];
+        r.y_phi[o + 1] = y + r.w4 * d1;
+        double v = ry[3];
+        v = v + r.w1 * rk[7];
+        v = v + r.w2 * rk[9];
+        v = v + r.w3 * rk[11];
+        r.y_u[o + 1] = v + r.w4 * e1;
+      }
+      return;
+    }
     // Streaming (nontemporal) stores: the new state is not read again in this
     // launch, so it need not displace the halo lines neighbouring waves re-read.
     if (kVec) {
@@ -440,7 +520,7 @@ template <int kDepth>
 void launch_variant(dim3 grid, dim3 block, hipStream_t st, const StreamArgs &a, bool noise,
                     bool vec, bool table, bool pair) {
   if (pair) {  // block pairs store column by column (odd slab columns start a lane's pair)
-#define LGD_PAIR_LAUNCH(N, T) modelc_stream_kernel<N, false, T, kDepth, true><<<grid, block, 0, st>>>(a)
+#define LGD_PAIR_LAUNCH(N, T) modelc_stream_kernel<N, false, T, kDepth, true, 0><<<grid, block, 0, st>>>(a)
     if (noise) {
       if (table) LGD_PAIR_LAUNCH(true, true); else LGD_PAIR_LAUNCH(true, false);
     } else {
@@ -449,7 +529,7 @@ void launch_variant(dim3 grid, dim3 block, hipStream_t st, const StreamArgs &a, 
 #undef LGD_PAIR_LAUNCH
     return;
   }
-#define LGD_STREAM_LAUNCH(N, V, T) modelc_stream_kernel<N, V, T, kDepth, false><<<grid, block, 0, st>>>(a)
+#define LGD_STREAM_LAUNCH(N, V, T) modelc_stream_kernel<N, V, T, kDepth, false, 0><<<grid, block, 0, st>>>(a)
   if (noise) {
     if (vec) {
       if (table) LGD_STREAM_LAUNCH(true, true, true); else LGD_STREAM_LAUNCH(true, true, false);
@@ -521,4 +601,49 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   else
     launch_variant<1>(grid, block, ctx->stream, a, noise, vec, table, pair);
   return lgd::launch_check(ctx, "modelc_stream");
+}
+
+int lgd_modelc_stream_rk4_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
+                                 int64_t nblocks, const double *pin, const double *uin,
+                                 double *pout, double *uout, int stage, const lgd_rk4_stage &rk,
+                                 int strips, int prefetch) {
+  if (nx + 2 > 128 || nx < 1 || ny < 1) return LGD_ERR_UNSUPPORTED;
+  if (stage < 1 || stage > 4) return lgd::fail(ctx, LGD_ERR_INVALID_ARGUMENT, "RK4 stage is 1..4");
+  if (nblocks <= 0) return LGD_OK;
+  StreamArgs a{};
+  a.d0 = d0;
+  a.domains = nullptr;
+  a.block_domain = nullptr;
+  a.phi = pin;
+  a.u = uin;
+  a.phi_n = pout;
+  a.u_n = uout;
+  a.dt = 0.0;  // the stage update is the caller's (rk), not x + dt·f
+  a.salt = 0;
+  a.salt_src = nullptr;
+  a.stream = 0;
+  a.nblocks = nblocks;
+  a.block_begin = 0;
+  a.nx = (int32_t)nx;
+  a.ny = (int32_t)ny;
+  a.rk = rk;
+  if (strips < 1) strips = 1;
+  if (strips > a.ny) strips = a.ny;
+  a.rows = (a.ny + strips - 1) / strips;
+  a.strips = (a.ny + a.rows - 1) / a.rows;
+  const int64_t nwg = (nblocks * a.strips + kWavesPerWG - 1) / kWavesPerWG;
+  if (nwg > INT32_MAX) return LGD_ERR_UNSUPPORTED;
+  a.nwg = (int32_t)nwg;
+  const dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
+  hipStream_t st = ctx->stream;
+#define LGD_RK4_LAUNCH(D, S) modelc_stream_kernel<false, false, false, D, false, S><<<grid, block, 0, st>>>(a)
+  if (prefetch == 1) {
+    if (stage == 1) LGD_RK4_LAUNCH(1, 1); else if (stage == 2) LGD_RK4_LAUNCH(1, 2);
+    else if (stage == 3) LGD_RK4_LAUNCH(1, 3); else LGD_RK4_LAUNCH(1, 4);
+  } else {
+    if (stage == 1) LGD_RK4_LAUNCH(2, 1); else if (stage == 2) LGD_RK4_LAUNCH(2, 2);
+    else if (stage == 3) LGD_RK4_LAUNCH(2, 3); else LGD_RK4_LAUNCH(2, 4);
+  }
+#undef LGD_RK4_LAUNCH
+  return lgd::launch_check(ctx, "modelc_stream_rk4");
 }

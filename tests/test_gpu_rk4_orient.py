@@ -205,3 +205,52 @@ def test_zero_orientation_is_bitwise_the_axis_aligned_path(ctx, orc, integrator)
     for phi, u in out[1:]:
         assert np.array_equal(phi, out[0][0]) and np.array_equal(u, out[0][1])
     grid.close()
+
+
+RK4_STREAM_CASES = [
+    # cells, block, periodic, (strips, rows ahead)
+    ([64, 48], [32, 24], [False, False], (4, 2)),
+    ([90, 60], [30, 20], [True, False], (3, 1)),
+    ([252, 126], [126, 63], [False, True], (4, 2)),
+    ([12, 8], [1, 2], [True, True], (1, 2)),
+    ([256, 40], [128, 20], [False, False], (2, 2)),  # 128 wide: one thread per cell
+]
+
+
+@pytest.mark.parametrize("case", range(len(RK4_STREAM_CASES)))
+def test_modelc_rk4_stream_stages_match_per_cell_and_oracle(ctx, orc, case):
+    """The fused RK4 stages on the streaming kernel (face sharing, two columns per
+    lane) against the one-thread-per-cell stages, whole slabs bit for bit, and
+    against the oracle's RungeKutta4 on the interiors."""
+    cells, block, periodic, (strips, prefetch) = RK4_STREAM_CASES[case]
+    say(f"rk4 streaming stages cells={cells} block={block} periodic={periodic}")
+    h = 0.4
+    grains = np.array([[0.3 * cells[0] * h, 0.45 * cells[1] * h, 2.0],
+                       [0.75 * cells[0] * h, 0.2 * cells[1] * h, 1.2]])
+    og = orc.grid(cells, block, [0, 0], [h, h], periodic=periodic)
+    n = orc.field_len(og, 1)
+    phi, u = np.zeros(n), np.zeros(n)
+    orc.initialize_grains(og, phi, u, grains, 0.6)
+    wp, wu = phi.copy(), u.copy()
+    orc.modelc_rk4_run(og, orc.params(0.05, 3.19, 2.0), wp, wu, 8, 0.008)
+    grid = lg.CartesianGrid(ctx, cells, block, [0.0, 0.0], [h, h], periodic=periodic)
+    outs = []
+    try:
+        for kernel, s, pf in ((2, strips, prefetch), (1, 0, 0)):
+            ctx.set_tuning("modelc_kernel", kernel)
+            ctx.set_tuning("modelc_strips", s)
+            ctx.set_tuning("modelc_prefetch", pf)
+            sim = lg.Simulation.model_c(grid, lg.ModelCParams(0.05, 3.19, 2.0), mode=lg.LGD_MODE_FUSED)
+            sim.set_integrator("rk4")
+            sim.field(0).upload(phi)
+            sim.field(1).upload(u)
+            sim.step(0.008, 8)
+            outs.append((sim.field(0).download(), sim.field(1).download()))
+            sim.close()
+    finally:
+        for key in ("modelc_kernel", "modelc_strips", "modelc_prefetch"):
+            ctx.set_tuning(key, 0)
+    grid.close()
+    m = interior_mask(cells, block)
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert np.array_equal(outs[0][0][m], wp[m]) and np.array_equal(outs[0][1][m], wu[m])
