@@ -212,7 +212,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
   // valid for every row, the lane's output offset within a slab row (obase) and
   // the interior column of its first cell (kc0, for the cell key).
   int64_t col0, col1, hcol = 0, obase;
-  bool in0, in1;
+  bool in0, in1;    // the lane's columns are interior cells
+  bool st0, st1;    // the lane's columns lie in the slab row (stored by the step)
   int kc0, seg = 0;
   if constexpr (kPair) {
     // Lanes 0..nx-1 hold the pair's 2·nx interior columns (nx even, so a lane's
@@ -228,6 +229,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     obase = (b + seg) * slab + (ic + 1);
     in0 = act;
     in1 = act;
+    st0 = act;
+    st1 = act;
     kc0 = ic;
   } else {
     // Lane 0 reads the left halo cell in its first column, lane (nx+1)/2 the
@@ -243,10 +246,10 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     obase = b * slab + c0;
     in0 = c0 >= 1 && c0 <= nx;
     in1 = c1 <= nx;
+    st0 = c0 < sx;
+    st1 = c1 < sx;
     kc0 = c0 - 1;
   }
-  const bool st0 = kPair ? in0 : obase - b * slab < sx;      // column inside the slab row
-  const bool st1 = kPair ? in1 : obase - b * slab + 1 < sx;
 
   // Wiener(0) amplitude b on interior cells; zero amplitude draws nothing
   // (src/core/driver.rs:144-146).
