@@ -483,7 +483,7 @@ int lgd_parquet_write_file(const char *path, size_t ncols, const char *const *na
  * tests/convergence.rs:100-107) for calibrating θ = (λ, D, ε₄) and the initial
  * fields.  No counterpart in the reference, which has no derivative code; the
  * callers are a forward/backward trait in the autograd-Function role on the host
- * side (INTEGRATION.md) and the Python ModelCCalibration. */
+ * side and the Python ModelCCalibration. */
 typedef struct lgd_adjoint lgd_adjoint;
 /* Workspace for reverse steps on `grid` (2-D; mirror or periodic faces). */
 int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out);
