@@ -91,17 +91,20 @@ __device__ __forceinline__ FD fderiv(const lgd_kr_consts &k, double tol, double 
   const double d2 = dx2 - dy2;
   const double qm = Q * im;
   f.a = k.a_bar * (1.0 + k.eps_prime * qm);
-  const double c4 = (4.0 * k.a_bar * k.eps_prime) * is3;
-  f.a_x = c4 * (dx * dy2 * d2);
-  f.a_y = -c4 * (dy * dx2 * d2);
+  // dx·dy²·(dx² − dy²) = dy·H and dy·dx²·(dx² − dy²) = dx·H.
+  const double H = dx * dy * d2;
+  const double cH = ((4.0 * k.a_bar * k.eps_prime) * is3) * H;
+  f.a_x = cH * dy;
+  f.a_y = -cH * dx;
   f.a_e = -3.0 * (1.0 + k.eps_prime * qm) + (3.0 / k.a_bar) * qm;
   if (kAngular) {
-    const double H = dx * dy * d2;
-    f.da = -k.a12 * H * im;
+    const double Him = H * im;
+    f.da = -k.a12 * Him;
     const double H_x = dy * (3.0 * dx2 - dy2), H_y = dx * (dx2 - 3.0 * dy2);
-    f.da_x = -k.a12 * (H_x * s - 4.0 * dx * H) * is3;
-    f.da_y = -k.a12 * (H_y * s - 4.0 * dy * H) * is3;
-    f.da_e = -12.0 * H * im;
+    const double K = -k.a12 * is3;
+    f.da_x = K * (H_x * s - 4.0 * dx * H);
+    f.da_y = K * (H_y * s - 4.0 * dy * H);
+    f.da_e = -12.0 * Him;
   } else {
     f.da = f.da_x = f.da_y = f.da_e = 0.0;
   }
@@ -230,11 +233,14 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
       const double derx = PcR - Pc;
       const double dery = q * (((PpR - PmR) + Pp) - Pm);
       const FD h = fderiv<true>(k, k.fv_tol, derx, dery);
+      // With P = inner + a·derx: ∂J/∂derx = a_x·inner + a·(a_x·derx + a − a′_x·dery)
+      // = a_x·P + a·(a − a′_x·dery), and likewise for dery and ε₄.
       const double inner = h.a * derx - h.da * dery;
+      const double P = inner + h.a * derx;
       JX = h.a * ((h.a * derx) - (h.da * dery));
-      FXx = h.a_x * inner + h.a * (h.a_x * derx + h.a - h.da_x * dery);
-      FXy = h.a_y * inner + h.a * (h.a_y * derx - h.da_y * dery - h.da);
-      FXe = h.a_e * inner + h.a * (h.a_e * derx - h.da_e * dery);
+      FXx = h.a_x * P + h.a * (h.a - h.da_x * dery);
+      FXy = h.a_y * P - h.a * (h.da_y * dery + h.da);
+      FXe = h.a_e * P - h.a * (h.da_e * dery);
     }
     // y-face between rows r and r+1.
     double JT, NYx, NYy, NYe;
@@ -242,11 +248,13 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
       const double derx = q * (((PpR - PpL) + PcR) - PcL);
       const double dery = Pp - Pc;
       const FD g = fderiv<true>(k, k.fv_tol, derx, dery);
+      // With Q = inner + a·dery, as for the x-face.
       const double inner = g.a * dery + g.da * derx;
+      const double Q = inner + g.a * dery;
       JT = g.a * ((g.a * dery) + (g.da * derx));
-      NYx = g.a_x * inner + g.a * (g.a_x * dery + g.da_x * derx + g.da);
-      NYy = g.a_y * inner + g.a * (g.a_y * dery + g.a + g.da_y * derx);
-      NYe = g.a_e * inner + g.a * (g.a_e * dery + g.da_e * derx);
+      NYx = g.a_x * Q + g.a * (g.da_x * derx + g.da);
+      NYy = g.a_y * Q + g.a * (g.a + g.da_y * derx);
+      NYe = g.a_e * Q + g.a * (g.da_e * derx);
     }
     // Cell quantities at row r.
     const double JL = from_left(JX);
