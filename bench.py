@@ -727,6 +727,7 @@ def run_amr(args) -> int:
 
     A = AmrConfig(coarse_steps=args.amr_steps or AmrConfig.coarse_steps)
     ctx = lg.Context(0)
+    ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2, "pair": 3}[args.kernel])
     params = lg.ModelCParams(A.eps4, A.lambda_, A.d_thermal)
     pol = lg.RegridPolicy(A.every, A.buffer, A.efficiency, A.min_width, A.threshold, 0)
     grid = lg.CartesianGrid(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])
