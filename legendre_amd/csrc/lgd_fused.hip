@@ -108,14 +108,18 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
 // input y + c_s·k_s (the reference's copy then axpy). Stage 4 evaluates k4 and writes
 // the new state ((((y + w1·k1) + w2·k2) + w3·k3) + w4·k4), the reference's four
 // whole-slab axpys in their order, into y in place (y is read only at the cell itself).
+// One RK4 stage.  The four axpys y + w1·k1 + w2·k2 + w3·k3 + w4·k4 (w = dt/6, dt/3,
+// dt/3, dt/6) are summed as the stages go, in their order, in a running sum A:
+// stage 1 writes A = y + w1·k1, stages 2-3 add w_s·k_s, stage 4 writes
+// y = A + w4·k4.  That is the reference's fold, operation for operation, without
+// keeping k1..k3.
 struct Rk4Args {
-  const double *pin, *uin;    // the stage's input state
-  double *p0, *u0;            // y (read; stage 4 writes the result here)
-  double *kp, *ku;            // stages 1-3: k_s
-  double *pout, *uout;        // stages 1-3: the next stage's input
-  const double *k1p, *k1u, *k2p, *k2u, *k3p, *k3u;  // stage 4
+  const double *pin, *uin;    // the stage's input state (stage 1: y itself)
+  double *p0, *u0;            // y (read by stages 2-3; stage 4 writes the result here)
+  double *ap, *au;            // the running sum A
+  double *pout, *uout;        // stages 1-3: the next stage's input y + c_s·k_s
   double coef;                // stages 1-3: c_s
-  double w1, w2, w3, w4;      // stage 4: dt/6, dt/3, dt/3, dt/6
+  double w;                   // w_s
 };
 
 template <int kStage, bool kOriented>
@@ -132,22 +136,19 @@ __global__ __launch_bounds__(kTX *kTY) void rk4_stage_kernel(lgd_cart c, lgd_kr_
   int64_t oc;
   double pc, uc, dphi, du;
   cell_vector_field<kOriented>(c, k, b, i, j, a.pin, a.uin, cs0, sn0, oc, pc, uc, dphi, du);
-  if (kStage < 4) {
-    a.kp[oc] = dphi;
-    a.ku[oc] = du;
+  if (kStage == 1) {  // the input is y: (pc, uc) are y at this cell
+    a.ap[oc] = pc + a.w * dphi;
+    a.au[oc] = uc + a.w * du;
+    a.pout[oc] = pc + a.coef * dphi;
+    a.uout[oc] = uc + a.coef * du;
+  } else if (kStage < 4) {
+    a.ap[oc] = a.ap[oc] + a.w * dphi;
+    a.au[oc] = a.au[oc] + a.w * du;
     a.pout[oc] = a.p0[oc] + a.coef * dphi;
     a.uout[oc] = a.u0[oc] + a.coef * du;
   } else {
-    double y = a.p0[oc];
-    y = y + a.w1 * a.k1p[oc];
-    y = y + a.w2 * a.k2p[oc];
-    y = y + a.w3 * a.k3p[oc];
-    a.p0[oc] = y + a.w4 * dphi;
-    double v = a.u0[oc];
-    v = v + a.w1 * a.k1u[oc];
-    v = v + a.w2 * a.k2u[oc];
-    v = v + a.w3 * a.k3u[oc];
-    a.u0[oc] = v + a.w4 * du;
+    a.p0[oc] = a.ap[oc] + a.w * dphi;
+    a.u0[oc] = a.au[oc] + a.w * du;
   }
 }
 
@@ -330,20 +331,13 @@ int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_par
   const dim3 wg(kTX, kTY);
   const double half_dt = 0.5 * dt;
   const double coef[3] = {half_dt, half_dt, dt};
-  // kbuf: k1φ, k1u, k2φ, k2u, k3φ, k3u; tbuf: t0φ, t0u, t1φ, t1u.
+  const double w[4] = {dt / 6.0, dt / 3.0, dt / 3.0, dt / 6.0};
+  // kbuf[0], kbuf[1]: the running sum A (φ, u); tbuf: t0φ, t0u, t1φ, t1u.
   Rk4Args a{};
   a.p0 = phi->data;
   a.u0 = u->data;
-  a.w1 = dt / 6.0;
-  a.w2 = dt / 3.0;
-  a.w3 = dt / 3.0;
-  a.w4 = dt / 6.0;
-  a.k1p = kbuf[0]->data;
-  a.k1u = kbuf[1]->data;
-  a.k2p = kbuf[2]->data;
-  a.k2u = kbuf[3]->data;
-  a.k3p = kbuf[4]->data;
-  a.k3u = kbuf[5]->data;
+  a.ap = kbuf[0]->data;
+  a.au = kbuf[1]->data;
   const bool oriented = cs0 != nullptr;
   // Unoriented grids with blocks at most 126 wide run the stages on the streaming
   // kernel (face sharing; bitwise the same vector field), unless the per-cell
@@ -363,23 +357,14 @@ int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_par
     lgd_rk4_stage r{};
     r.y_phi = phi->data;
     r.y_u = u->data;
-    r.k1p = kbuf[0]->data;
-    r.k1u = kbuf[1]->data;
-    r.k2p = kbuf[2]->data;
-    r.k2u = kbuf[3]->data;
-    r.k3p = kbuf[4]->data;
-    r.k3u = kbuf[5]->data;
-    r.w1 = a.w1;
-    r.w2 = a.w2;
-    r.w3 = a.w3;
-    r.w4 = a.w4;
+    r.acc_phi = kbuf[0]->data;
+    r.acc_u = kbuf[1]->data;
     for (int st = 1; st <= 4; ++st) {
       const lgd_buf *in_p = st == 1 ? phi : tbuf[(st % 2 == 0) ? 0 : 2];
       const lgd_buf *in_u = st == 1 ? u : tbuf[(st % 2 == 0) ? 1 : 3];
       double *out_p = nullptr, *out_u = nullptr;
+      r.w = w[st - 1];
       if (st < 4) {
-        r.k_phi = kbuf[2 * (st - 1)]->data;
-        r.k_u = kbuf[2 * (st - 1) + 1]->data;
         out_p = tbuf[(st % 2 == 1) ? 0 : 2]->data;
         out_u = tbuf[(st % 2 == 1) ? 1 : 3]->data;
         r.coef = coef[st - 1];
@@ -396,9 +381,8 @@ int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_par
     const lgd_buf *in_u = st == 1 ? u : tbuf[(st % 2 == 0) ? 1 : 3];
     a.pin = in_p->data;
     a.uin = in_u->data;
+    a.w = w[st - 1];
     if (st < 4) {
-      a.kp = kbuf[2 * (st - 1)]->data;
-      a.ku = kbuf[2 * (st - 1) + 1]->data;
       a.pout = tbuf[(st % 2 == 1) ? 0 : 2]->data;
       a.uout = tbuf[(st % 2 == 1) ? 1 : 3]->data;
       a.coef = coef[st - 1];

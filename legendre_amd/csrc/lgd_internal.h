@@ -184,7 +184,8 @@ int observer_prepare(lgd_observer *o, const lgd_grid *grid, const lgd_amr *amr, 
 }  // namespace lgd
 
 // One RungeKutta4 step of Model C in four fused stage launches (lgd_fused.hip):
-// kbuf = k1φ, k1u, k2φ, k2u, k3φ, k3u; tbuf = two alternating stage inputs (φ, u, φ, u).
+// kbuf = the running sum of the four axpys (φ, u); tbuf = two alternating stage
+// inputs (φ, u, φ, u).
 int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
                         lgd_buf *phi, lgd_buf *u, lgd_buf *const *kbuf, lgd_buf *const *tbuf,
                         double dt, const double *cs0, const double *sn0);
@@ -209,21 +210,22 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
 // pairs: nx even, 4 <= nx <= 64, nbx even.
 bool lgd_stream_pairable(int64_t nx, int64_t nbx);
 
-// Operands of one RungeKutta4 stage on the streaming kernel: y (read; stage 4
-// writes the result into it), k_s (stages 1-3), k1..k3 (stage 4), the stage
-// coefficient c_s and the weights dt/6, dt/3, dt/3, dt/6.
+// Operands of one RungeKutta4 stage on the streaming kernel: y (read by stages
+// 2-3; stage 4 writes the result into it), the running sum A of the four axpys
+// (stage 1 writes y + w1·k1, stages 2-3 add w_s·k_s, stage 4 reads it), the stage
+// coefficient c_s and the stage weight w_s (dt/6, dt/3, dt/3, dt/6).
 struct lgd_rk4_stage {
   double *y_phi, *y_u;
-  double *k_phi, *k_u;
-  const double *k1p, *k1u, *k2p, *k2u, *k3p, *k3u;
+  double *acc_phi, *acc_u;
   double coef;
-  double w1, w2, w3, w4;
+  double w;
 };
 
 // RK4 stage `stage` (1..4) of the Model C vector field on the streaming kernel
 // over every block of one domain (blocks at most 126 wide, no orientation): the
-// stencil reads (pin, uin); stages 1-3 write k_s and the next input to (pout,
-// uout). Interior cells only. LGD_ERR_UNSUPPORTED when the block is too wide.
+// stencil reads (pin, uin; stage 1: y); stages 1-3 update the running sum and
+// write the next input to (pout, uout). Interior cells only. LGD_ERR_UNSUPPORTED
+// when the block is too wide.
 int lgd_modelc_stream_rk4_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
                                  int64_t nblocks, const double *pin, const double *uin,
                                  double *pout, double *uout, int stage, const lgd_rk4_stage &rk,

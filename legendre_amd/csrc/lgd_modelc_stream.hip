@@ -44,9 +44,10 @@ constexpr int kWavesPerWG = 4;
 constexpr int kXcds = 8;
 
 // A RungeKutta4 stage on the streaming kernel (kStage 1..4; src/integrators/rk4.rs):
-// the stencil reads the stage input (phi, u of StreamArgs); stages 1-3 write
-// k_s = f(input) and the next input y + c_s·k_s (to phi_n, u_n), stage 4 folds
-// y + w1·k1 + w2·k2 + w3·k3 + w4·f(input) in that order into y in place.
+// the stencil reads the stage input (phi, u of StreamArgs; stage 1: y itself);
+// with k_s = f(input), stages 1-3 write the next input y + c_s·k_s (to phi_n, u_n)
+// and the running sum A of y + w1·k1 + w2·k2 + w3·k3 (stage 1 starts it from y,
+// stages 2-3 add to it in place), and stage 4 writes y = A + w4·k4.
 using Rk4Stage = lgd_rk4_stage;
 
 struct StreamArgs {
@@ -328,8 +329,8 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     // loaded ahead: loads complete in order, so waiting for them then never waits
     // for the look-ahead.
     const int64_t o = obase + (int64_t)(j + 1) * sx;
-    double ry[4] = {0.0, 0.0, 0.0, 0.0}, rk[12] = {};
-    if constexpr (kStage >= 1) {
+    double ry[4] = {0.0, 0.0, 0.0, 0.0}, ra[4] = {0.0, 0.0, 0.0, 0.0};
+    if constexpr (kStage == 2 || kStage == 3) {
       if (in0) {
         ry[0] = a.rk.y_phi[o];
         ry[2] = a.rk.y_u[o];
@@ -339,12 +340,14 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
         ry[3] = a.rk.y_u[o + 1];
       }
     }
-    if constexpr (kStage == 4) {
-      const double *src[6] = {a.rk.k1p, a.rk.k2p, a.rk.k3p, a.rk.k1u, a.rk.k2u, a.rk.k3u};
-#pragma unroll
-      for (int q = 0; q < 6; ++q) {
-        if (in0) rk[2 * q] = src[q][o];
-        if (in1) rk[2 * q + 1] = src[q][o + 1];
+    if constexpr (kStage >= 2) {
+      if (in0) {
+        ra[0] = a.rk.acc_phi[o];
+        ra[2] = a.rk.acc_u[o];
+      }
+      if (in1) {
+        ra[1] = a.rk.acc_phi[o + 1];
+        ra[3] = a.rk.acc_u[o + 1];
       }
     }
     // Issued on every row (past the strip's last needed row it re-reads row j1,
@@ -370,6 +373,12 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
     double n0, n1, v0, v1, d0, d1, e0, e1;
     cell_update(k, f0, pc0, pc1, pcL, pp0, pm0, uc0, uc1, ucL, up0, um0, a.dt, n0, v0, d0, e0);
     cell_update(k, f1, pc1, pcR, pc0, pp1, pm1, uc1, ucR, uc0, up1, um1, a.dt, n1, v1, d1, e1);
+    if constexpr (kStage == 1) {  // the input is y: row j's own values (before the rotation)
+      ry[0] = pc0;
+      ry[1] = pc1;
+      ry[2] = uc0;
+      ry[3] = uc1;
+    }
     if (noisy) {
       // Wiener(0) on phi after the drift (src/core/driver.rs:139-158); cell key =
       // interior linear index (src/geometry/cartesian.rs:599-619).
@@ -434,50 +443,37 @@ __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamA
       ucR = rgt(uc0, huc);
     }
     if constexpr (kStage >= 1 && kStage <= 3) {
-      // k_s and the next stage's input y + c_s·k_s, interior cells only (the
-      // ghost ring of these buffers is never read).
-      const double c = a.rk.coef;
+      // The running sum (started from y at stage 1) and the next stage's input
+      // y + c_s·k_s, interior cells only (the ghost ring of these buffers is
+      // never read).
+      const double c = a.rk.coef, w = a.rk.w;
+      const double s00 = kStage == 1 ? ry[0] : ra[0], s01 = kStage == 1 ? ry[1] : ra[1];
+      const double s10 = kStage == 1 ? ry[2] : ra[2], s11 = kStage == 1 ? ry[3] : ra[3];
       if (in0) {
-        a.rk.k_phi[o] = d0;
-        a.rk.k_u[o] = e0;
+        a.rk.acc_phi[o] = s00 + w * d0;
+        a.rk.acc_u[o] = s10 + w * e0;
         a.phi_n[o] = ry[0] + c * d0;
         a.u_n[o] = ry[2] + c * e0;
       }
       if (in1) {
-        a.rk.k_phi[o + 1] = d1;
-        a.rk.k_u[o + 1] = e1;
+        a.rk.acc_phi[o + 1] = s01 + w * d1;
+        a.rk.acc_u[o + 1] = s11 + w * e1;
         a.phi_n[o + 1] = ry[1] + c * d1;
         a.u_n[o + 1] = ry[3] + c * e1;
       }
       return;
     }
     if constexpr (kStage == 4) {
-      // y + (dt/6)·k1 + (dt/3)·k2 + (dt/3)·k3 + (dt/6)·k4, one axpy after another
-      // (src/integrators/rk4.rs), in place: y is read only at its own cell.
-      const Rk4Stage &r = a.rk;
+      // y = A + (dt/6)·k4, A = y + (dt/6)·k1 + (dt/3)·k2 + (dt/3)·k3 summed in that
+      // order (src/integrators/rk4.rs), in place: y is written only at its own cell.
+      const double w = a.rk.w;
       if (in0) {
-        double y = ry[0];
-        y = y + r.w1 * rk[0];
-        y = y + r.w2 * rk[2];
-        y = y + r.w3 * rk[4];
-        r.y_phi[o] = y + r.w4 * d0;
-        double v = ry[2];
-        v = v + r.w1 * rk[6];
-        v = v + r.w2 * rk[8];
-        v = v + r.w3 * rk[10];
-        r.y_u[o] = v + r.w4 * e0;
+        a.rk.y_phi[o] = ra[0] + w * d0;
+        a.rk.y_u[o] = ra[2] + w * e0;
       }
       if (in1) {
-        double y = ry[1];
-        y = y + r.w1 * rk[1];
-        y = y + r.w2 * rk[3];
-        y = y + r.w3 * rk[5];
-        r.y_phi[o + 1] = y + r.w4 * d1;
-        double v = ry[3];
-        v = v + r.w1 * rk[7];
-        v = v + r.w2 * rk[9];
-        v = v + r.w3 * rk[11];
-        r.y_u[o + 1] = v + r.w4 * e1;
+        a.rk.y_phi[o + 1] = ra[1] + w * d1;
+        a.rk.y_u[o + 1] = ra[3] + w * e1;
       }
       return;
     }

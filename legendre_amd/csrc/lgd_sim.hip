@@ -182,7 +182,7 @@ int step_rk4(lgd_sim *s, double dt) {
   const int nf = (int)s->state.size();
   if (s->kind == lgd_sim::kModelC && s->mode == LGD_MODE_FUSED) {
     // Fused stages: the same operations in the same order, half the field passes.
-    lgd_buf *kb[6] = {s->rk[0], s->rk[1], s->rk[2], s->rk[3], s->rk[4], s->rk[5]};
+    lgd_buf *kb[2] = {s->rk[0], s->rk[1]};
     lgd_buf *tb[4] = {s->rk[8], s->rk[9], s->rk[6], s->rk[7]};
     return lgd_modelc_rk4_step(s->ctx, s->grid, &s->params, s->state[0], s->state[1], kb, tb, dt,
                                s->cos0 ? s->cos0->data : nullptr, s->sin0 ? s->sin0->data : nullptr);
