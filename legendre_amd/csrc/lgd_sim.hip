@@ -183,7 +183,7 @@ int step_rk4(lgd_sim *s, double dt) {
   if (s->kind == lgd_sim::kModelC && s->mode == LGD_MODE_FUSED) {
     // Fused stages: the same operations in the same order, half the field passes.
     lgd_buf *kb[2] = {s->rk[0], s->rk[1]};
-    lgd_buf *tb[4] = {s->rk[8], s->rk[9], s->rk[6], s->rk[7]};
+    lgd_buf *tb[4] = {s->rk[2], s->rk[3], s->rk[4], s->rk[5]};
     return lgd_modelc_rk4_step(s->ctx, s->grid, &s->params, s->state[0], s->state[1], kb, tb, dt,
                                s->cos0 ? s->cos0->data : nullptr, s->sin0 ? s->sin0->data : nullptr);
   }
@@ -971,7 +971,12 @@ int lgd_sim_set_integrator(lgd_sim *sim, int32_t integrator) {
     if (sim->kind == lgd_sim::kModelC && sim->wiener)
       return fail(sim->ctx, LGD_ERR_UNSUPPORTED, "RK4 pairs with deterministic models (no Wiener driver)");
     const size_t len = sim->state[0]->len;
-    while (sim->rk.size() < 5 * sim->state.size()) {
+    // Fused Model C stages: the running sum and two stage inputs (φ, u each); the
+    // reference's sweeps: k1..k4 and y + c·k per field.
+    const size_t need = (sim->kind == lgd_sim::kModelC && sim->mode == LGD_MODE_FUSED)
+                            ? 6
+                            : 5 * sim->state.size();
+    while (sim->rk.size() < need) {
       lgd_buf *b = nullptr;
       int rc = lgd_buf_alloc(sim->ctx, len, &b);
       if (rc) return rc;
