@@ -269,7 +269,10 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
   const bool noise_on = wiener && params->noise_amplitude != 0.0;
   const double noise_scale = std::sqrt(dt) * params->noise_amplitude;
   const uint64_t wiener0 = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
-  const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, noise_on, lgd_launch_shape{});
+  // Untuned default: 4 strips; two rows ahead, or one with noise (VALU-bound: the
+  // registers of the second row are worth more as headroom).
+  const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, noise_on,
+                                                  lgd_launch_shape{2, 4, noise_on ? 1 : 2});
   const bool oriented = cs0 != nullptr;
   if (shape.kernel != 1 && !oriented) {
     lgd_domain d0;
