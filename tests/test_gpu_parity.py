@@ -394,6 +394,9 @@ PAIR_CASES = [
     ([64, 40], [16, 20], [True, True], [[12.0, 7.6, 2.0]]),
     ([8, 12], [4, 3], [False, True], [[1.6, 2.4, 1.0]]),
     ([128, 128], [32, 64], [False, False], [[12.8, 25.6, 3.0], [25.6, 12.0, 2.0]]),
+    # one pair spans the periodic row: each outer halo wraps to the pair's other block
+    ([128, 64], [64, 32], [True, False], [[0.4, 12.0, 3.0], [50.0, 20.0, 2.0]]),
+    ([64, 16], [32, 16], [True, True], [[1.0, 0.4, 2.0]]),
 ]
 PAIR_SHAPES = [(1, 2), (3, 1), (4, 2)]  # (strips, prefetch)
 
