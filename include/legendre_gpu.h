@@ -387,6 +387,20 @@ double lgd_sim_time(const lgd_sim *sim);
  * new hierarchy (patches, plans, uploads), [3] the state migrated. */
 int lgd_sim_regrid_stats(const lgd_sim *sim, int64_t *regrids, double *seconds);
 int64_t lgd_sim_step_index(const lgd_sim *sim);
+/* TagCells (src/geometry/amr/adapt.rs:32-46) as a host callback, in place of the
+ * policy's GradientTagger: at each regrid BergerOliger::regrid calls tag_level for
+ * every existing source level, finest first, and the callback appends the cells of
+ * `level` needing refinement (level cell coordinates, n × [x, y]) through
+ * lgd_tag_sink_push; the state is readable through lgd_sim_field.  prepare is the
+ * trait's default (nothing), so the tag field is not restricted or filled.  Buffer
+ * dilation, nesting flags, clustering, hysteresis and migration follow the policy as
+ * for the GradientTagger.  A nonzero return fails the step with
+ * LGD_ERR_INVALID_ARGUMENT.  fn = NULL restores the GradientTagger. */
+typedef struct lgd_tag_sink lgd_tag_sink;
+typedef int (*lgd_tag_level_fn)(void *user, const lgd_amr *grid, int32_t level,
+                                lgd_tag_sink *sink);
+int lgd_sim_set_tagger(lgd_sim *sim, lgd_tag_level_fn fn, void *user);
+int lgd_tag_sink_push(lgd_tag_sink *sink, const int64_t *cells, size_t ncells);
 /* ModelC::initialize_grains with orientation 0 (phasefield.rs:227-286), computed on the
  * host (glibc tanh/hypot, as the reference) and uploaded: grains = [cx, cy, r] × n. */
 int lgd_sim_initialize_grains(lgd_sim *sim, const double *grains, size_t ngrains, double u0);

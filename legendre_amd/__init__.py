@@ -485,6 +485,30 @@ class Simulation:
                    self.ctx.handle)
         return int(n.value), sec
 
+    def set_tagger(self, fn) -> None:
+        """A caller's TagCells::tag_level in place of the policy's GradientTagger
+        (adaptive AMR only; prepare is the trait default): fn(amr: AmrGrid, level)
+        returns the level's cells needing refinement as (x, y) pairs.  None restores
+        the GradientTagger."""
+        if fn is None:
+            self._tagger = None
+            _abi.check(self._lib.lgd_sim_set_tagger(self.handle, None, None), self.ctx.handle)
+            return
+        lib = self._lib
+        ctx = self.ctx
+
+        def cb(user, grid, level, sink):
+            try:
+                cells = [c for xy in fn(AmrGrid(ctx, C.c_void_p(grid), owned=False), int(level)) for c in xy]
+            except Exception:  # noqa: BLE001 - a failing tagger fails the step
+                return 1
+            buf = (C.c_int64 * max(1, len(cells)))(*cells)
+            return lib.lgd_tag_sink_push(sink, buf, len(cells) // 2)
+
+        self._tagger = _abi.TAG_LEVEL_FN(cb)  # kept alive with the simulation
+        _abi.check(self._lib.lgd_sim_set_tagger(self.handle, C.cast(self._tagger, C.c_void_p), None),
+                   self.ctx.handle)
+
     def set_integrator(self, name: str) -> None:
         """'euler' (EulerMaruyama / ForwardEuler) or 'rk4' (RungeKutta4, deterministic
         models only)."""

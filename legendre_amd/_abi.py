@@ -103,6 +103,9 @@ class DriverC(C.Structure):
 I32, I64, U32, U64, SZ, DBL = C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_size_t, C.c_double
 DP = C.POINTER(C.c_double)
 
+# lgd_tag_level_fn: int (*)(void *user, const lgd_amr *grid, int32_t level, lgd_tag_sink *sink)
+TAG_LEVEL_FN = C.CFUNCTYPE(C.c_int, P, P, I32, P)
+
 # name -> (restype, argtypes); the exact list include/legendre_gpu.h declares.
 SIGNATURES = {
     "lgd_abi_version": (U32, []),
@@ -172,6 +175,8 @@ SIGNATURES = {
     "lgd_sim_regrid_stats": (C.c_int, [P, C.POINTER(C.c_int64), DP]),
     "lgd_sim_time": (DBL, [P]),
     "lgd_sim_step_index": (I64, [P]),
+    "lgd_sim_set_tagger": (C.c_int, [P, P, P]),
+    "lgd_tag_sink_push": (C.c_int, [P, C.POINTER(C.c_int64), C.c_size_t]),
     "lgd_sim_initialize_grains": (C.c_int, [P, DP, SZ, DBL]),
     "lgd_sim_set_integrator": (C.c_int, [P, C.c_int32]),
     "lgd_sim_set_orientation": (C.c_int, [P, DP, SZ]),

@@ -13,6 +13,11 @@ int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
 // each level's domain, 64 cells per word; consumed).
 int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>> &bits,
                             const lgd_regrid_policy &policy, std::vector<std::vector<lgd_box>> &out);
+// The bitmask of level `src` from a caller's tagged cells (n × [x, y], level cell
+// coordinates, as TagCells::tag_level appends them), buffer-dilated and clipped to
+// the level domain like the GradientTagger's.
+void lgd_regrid_bits_from_cells(const lgd_amr &g, int src, const std::vector<int64_t> &cells,
+                                int64_t buffer, std::vector<uint64_t> &bits);
 // Migrate one field from the old hierarchy onto the new one: new_f receives the
 // levels >= 1 of the new field (elements from new_g.off[level_start[1]] on); level 0
 // is the same in both hierarchies and is copied on the device.

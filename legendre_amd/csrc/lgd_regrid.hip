@@ -274,6 +274,21 @@ int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>>
   return 0;
 }
 
+void lgd_regrid_bits_from_cells(const lgd_amr &g, int src, const std::vector<int64_t> &cells,
+                                int64_t buffer, std::vector<uint64_t> &bits) {
+  const lgd_box dom = g.domain(src);
+  const int64_t W = dom.hi[0], H = dom.hi[1], wpr = (W + 63) / 64;
+  bits.assign((size_t)(H * wpr), 0ull);
+  const int64_t r = buffer > 0 ? buffer : 0;
+  for (size_t k = 0; k + 1 < cells.size(); k += 2) {
+    const int64_t cx = cells[k], cy = cells[k + 1];
+    // The ball of the buffer dilation (adapt.rs:181-193), clipped to the domain.
+    for (int64_t y = std::max<int64_t>(cy - r, 0); y <= std::min<int64_t>(cy + r, H - 1); ++y)
+      for (int64_t x = std::max<int64_t>(cx - r, 0); x <= std::min<int64_t>(cx + r, W - 1); ++x)
+        bits[(size_t)(y * wpr + x / 64)] |= 1ull << (x % 64);
+  }
+}
+
 int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
                        const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
   std::vector<std::vector<uint64_t>> bits((size_t)g.num_levels());

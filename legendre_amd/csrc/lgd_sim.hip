@@ -78,6 +78,14 @@ struct lgd_sim {
   double regrid_seconds[4] = {0.0, 0.0, 0.0, 0.0};
   int64_t regrids = 0;
   lgd_regrid_scratch rscratch;  // device tagging scratch
+  // A caller's TagCells::tag_level in place of the GradientTagger (NULL: gradient).
+  lgd_tag_level_fn tagger = nullptr;
+  void *tagger_user = nullptr;
+};
+
+// The flags a TagCells::tag_level call appends to (n × [x, y]).
+struct lgd_tag_sink {
+  std::vector<int64_t> cells;
 };
 
 using namespace lgd;
@@ -434,6 +442,18 @@ double seconds_since(std::chrono::steady_clock::time_point &t0) {
   return s;
 }
 
+// GradientTagger::prepare: restrict + fill the tagged field (adapt.rs:72-76).
+int gradient_prepare(lgd_sim *s) {
+  const lgd_amr &g = *s->amr;
+  double *f = s->state[(size_t)s->policy.tag_field]->data;
+  int rc;
+  for (int l = g.num_levels() - 1; l >= 1; --l)
+    if ((rc = lgd_amr_restrict(s->ctx, g, f, l))) return rc;
+  for (int l = 0; l < g.num_levels(); ++l)
+    if ((rc = lgd_amr_fill(s->ctx, g, f, nullptr, 0.0, l))) return rc;
+  return LGD_OK;
+}
+
 // BergerOliger::regrid at the top of a step (src/core/simulation.rs:255-270).
 int amr_regrid(lgd_sim *s) {
   int rc;
@@ -442,18 +462,30 @@ int amr_regrid(lgd_sim *s) {
   const int tf = s->policy.tag_field;
   auto clock = std::chrono::steady_clock::now();
   ++s->regrids;
-  // GradientTagger::prepare: restrict + fill the tagged field (adapt.rs:72-76).
-  for (int l = g.num_levels() - 1; l >= 1; --l)
-    if ((rc = lgd_amr_restrict(s->ctx, g, s->state[(size_t)tf]->data, l))) return rc;
-  for (int l = 0; l < g.num_levels(); ++l)
-    if ((rc = lgd_amr_fill(s->ctx, g, s->state[(size_t)tf]->data, nullptr, 0.0, l))) return rc;
-  // Tag and dilate on the device (bitmasks come to the host), or bring the field
-  // to the host and do it there (the "regrid_host" knob).
+  // The tags come from a caller's TagCells, or the GradientTagger tags and dilates
+  // on the device (bitmasks come to the host), or the field comes to the host and
+  // is tagged there (the "regrid_host" knob, which also migrates on the host).
   const bool dev = !s->ctx->regrid_host && lgd_regrid_device_supported(g);
   std::vector<double> tag;
   std::vector<std::vector<lgd_box>> levels;
   int changed;
-  if (dev) {
+  if (s->tagger) {
+    // A caller's TagCells: the trait's default prepare (nothing), then tag_level
+    // per existing source level, finest first (adapt.rs:165-175).
+    (void)hipStreamSynchronize(s->ctx->stream);
+    const int capacity = g.max_levels() - 1;
+    std::vector<std::vector<uint64_t>> bits((size_t)g.num_levels());
+    for (int src = capacity - 1; src >= 0; --src) {
+      if (src >= g.num_levels()) continue;
+      lgd_tag_sink sink;
+      if (s->tagger(s->tagger_user, &g, src, &sink) != 0)
+        return fail(s->ctx, LGD_ERR_INVALID_ARGUMENT, "the tagger failed");
+      lgd_regrid_bits_from_cells(g, src, sink.cells, s->policy.buffer, bits[(size_t)src]);
+    }
+    s->regrid_seconds[0] += seconds_since(clock);
+    changed = lgd_regrid_propose_bits(g, bits, s->policy, levels);
+  } else if (dev) {
+    if ((rc = gradient_prepare(s))) return rc;
     std::vector<std::vector<uint64_t>> bits;
     if ((rc = lgd_regrid_flag_bits(s->ctx, g, s->state[(size_t)tf]->data, s->policy, s->rscratch,
                                    bits)))
@@ -461,6 +493,7 @@ int amr_regrid(lgd_sim *s) {
     s->regrid_seconds[0] += seconds_since(clock);
     changed = lgd_regrid_propose_bits(g, bits, s->policy, levels);
   } else {
+    if ((rc = gradient_prepare(s))) return rc;
     tag.resize((size_t)g.off.back());
     if ((rc = lgd_buf_download(s->ctx, s->state[(size_t)tf], 0, tag.data(), tag.size()))) return rc;
     s->regrid_seconds[0] += seconds_since(clock);
@@ -506,7 +539,7 @@ int amr_regrid(lgd_sim *s) {
         return abandon(rc);
       continue;
     }
-    if ((int)f == tf) {
+    if ((int)f == tf && !tag.empty()) {
       old_f.swap(tag);  // already downloaded
     } else {
       old_f.resize((size_t)g.off.back());
@@ -914,6 +947,21 @@ int lgd_sim_regrid_stats(const lgd_sim *sim, int64_t *regrids, double *seconds) 
 }
 
 int64_t lgd_sim_step_index(const lgd_sim *sim) { return sim ? sim->step_index : -1; }
+
+int lgd_sim_set_tagger(lgd_sim *sim, lgd_tag_level_fn fn, void *user) {
+  if (!sim) return LGD_ERR_INVALID_ARGUMENT;
+  if (!sim->amr || !sim->adaptive)
+    return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT, "a tagger needs an adaptive AMR simulation");
+  sim->tagger = fn;
+  sim->tagger_user = fn ? user : nullptr;
+  return LGD_OK;
+}
+
+int lgd_tag_sink_push(lgd_tag_sink *sink, const int64_t *cells, size_t ncells) {
+  if (!sink || (!cells && ncells)) return LGD_ERR_INVALID_ARGUMENT;
+  sink->cells.insert(sink->cells.end(), cells, cells + 2 * ncells);
+  return LGD_OK;
+}
 
 int lgd_sim_initialize_grains(lgd_sim *sim, const double *grains, size_t ngrains, double u0) {
   if (!sim || (!grains && ngrains)) return LGD_ERR_INVALID_ARGUMENT;

@@ -422,7 +422,23 @@ _AMR_SIGS = {
     "orc_amr_sim_field": (DP, [C.c_void_p, C.c_int32]),
     "orc_amr_sim_time": (C.c_double, [C.c_void_p]),
     "orc_amr_sim_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int64]),
+    "orc_amr_sim_set_tagger": (None, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    "orc_tag_sink_push": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.c_size_t]),
+    "orc_nesting_parents": (None, [C.POINTER(C.c_int64), C.c_int64, C.c_int64, C.POINTER(C.c_int64)]),
 }
+
+# int tag_level(void *user, const orc_amr *grid, int32_t level, orc_tag_sink *sink)
+TAG_LEVEL_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p)
+
+
+def nesting_parents(box, rf, rs):
+    """The source-level cells the nesting flags of one box cover (adapt.rs:194-211),
+    unclipped: box = ((lo0, lo1), (hi0, hi1)) in its own level's cells."""
+    (lo, hi) = box
+    b = (C.c_int64 * 4)(lo[0], lo[1], hi[0], hi[1])
+    out = (C.c_int64 * 4)()
+    _amr_lib().orc_nesting_parents(b, rf, rs, out)
+    return (int(out[0]), int(out[1])), (int(out[2]), int(out[3]))
 
 
 def _amr_lib():
@@ -574,4 +590,24 @@ class AmrSim:
         return float(_amr_lib().orc_amr_sim_time(self.h))
 
     def step(self, dt, n=1):
-        _amr_lib().orc_amr_sim_step(self.h, dt, n)
+        if _amr_lib().orc_amr_sim_step(self.h, dt, n) != 0:
+            raise ValueError("regrid failed: the tagger failed or the proposal is not a valid hierarchy")
+
+    def set_tagger(self, fn):
+        """A caller's TagCells::tag_level in place of the GradientTagger (prepare is
+        the trait default): fn(grid: AmrGrid, level) -> iterable of (x, y) cells."""
+        if fn is None:
+            self._tagger = None
+            _amr_lib().orc_amr_sim_set_tagger(self.h, None, None)
+            return
+
+        def cb(user, grid, level, sink):
+            try:
+                cells = [c for xy in fn(AmrGrid(C.c_void_p(grid), owned=False), int(level)) for c in xy]
+            except Exception:  # noqa: BLE001 - a failing tagger aborts the regrid
+                return 1
+            buf = (C.c_int64 * max(1, len(cells)))(*cells)
+            return _amr_lib().orc_tag_sink_push(sink, buf, len(cells) // 2)
+
+        self._tagger = TAG_LEVEL_FN(cb)  # kept alive with the simulation
+        _amr_lib().orc_amr_sim_set_tagger(self.h, C.cast(self._tagger, C.c_void_p), None)

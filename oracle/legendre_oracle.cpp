@@ -1280,6 +1280,16 @@ void push_cells(const Box2 &b, std::vector<Cell> &out) {
     for (int64_t x = b.lo[0]; x < b.hi[0]; ++x) out.push_back({x, y});
 }
 
+// The nesting flags of one box of the level just built (adapt.rs:194-211): the box
+// coarsened to the level being built (ratio rf, truncating division) and grown by
+// one cell, then the source-level cells under that (ratio rs, floor / ceiling
+// division).  Not clipped: the caller clips to the source domain.
+Box2 nesting_parents(const Box2 &bx, int64_t rf, int64_t rs) {
+  Box2 need{{bx.lo[0] / rf - 1, bx.lo[1] / rf - 1}, {bx.hi[0] / rf + 1, bx.hi[1] / rf + 1}};
+  return Box2{{div_euclid(need.lo[0], rs), div_euclid(need.lo[1], rs)},
+              {div_euclid(need.hi[0] + rs - 1, rs), div_euclid(need.hi[1] + rs - 1, rs)}};
+}
+
 // hierarchy_acceptable (adapt.rs:269-290).
 bool hierarchy_acceptable(const orc_amr &g, const std::vector<std::vector<Box2>> &nl) {
   if ((int)nl.size() != g.num_levels() - 1) return false;
@@ -1335,6 +1345,10 @@ double amr_sample(const orc_amr &g, const double *f, uint32_t ghost, int level, 
 }  // namespace
 
 // ---- simulation on an AMR hierarchy ----
+struct orc_tag_sink {
+  std::vector<std::array<int64_t, 2>> *flags;
+};
+
 struct orc_amr_sim {
   std::unique_ptr<orc_amr> grid;
   int32_t model = 0;  // 0 heat, 1 Model C
@@ -1345,6 +1359,10 @@ struct orc_amr_sim {
   int32_t wiener = 0;
   bool adaptive = false;
   orc_regrid_policy policy{};
+  // A caller's TagCells (adapt.rs:32-46) in place of the GradientTagger: its
+  // prepare is the trait's default (nothing) and tag_level is the callback.
+  orc_tag_level_fn tagger = nullptr;
+  void *tagger_user = nullptr;
   std::vector<std::vector<double>> state;  // one buffer per field (ghost 1)
   std::vector<std::vector<double>> k, amp, snaps;
   double t = 0.0;
@@ -1444,21 +1462,31 @@ void sim_advance(orc_amr_sim &s, int level, double t, double dt, double alpha) {
   }
 }
 
-// BergerOliger::regrid (adapt.rs:152-237) with GradientTagger; returns whether
-// the hierarchy was replaced.
-bool sim_regrid(orc_amr_sim &s) {
+// BergerOliger::regrid (adapt.rs:152-237) with GradientTagger; returns 1 when the
+// hierarchy was replaced, 0 when kept, -1 when the tagger failed or the proposal is
+// not a valid hierarchy (the reference panics at adapt.rs:227-230).
+int sim_regrid(orc_amr_sim &s) {
   const orc_regrid_policy &pol = s.policy;
-  if (s.step_index % pol.every != 0) return false;
+  if (s.step_index % pol.every != 0) return 0;
   orc_amr &g = *s.grid;
   double *tf = s.state[(size_t)pol.tag_field].data();
-  amr_restrict_all(g, tf, 1);
-  amr_fill_all(g, tf, 1);
+  if (!s.tagger) {  // GradientTagger::prepare (adapt.rs:72-76)
+    amr_restrict_all(g, tf, 1);
+    amr_fill_all(g, tf, 1);
+  }
   int capacity = g.max_levels() - 1;
   std::vector<std::vector<Box2>> nl((size_t)capacity);
   for (int target = capacity; target >= 1; --target) {
     int src = target - 1;
     std::vector<Cell> flags;
-    if (src < g.num_levels()) tag_level(g, tf, src, pol.threshold, flags);
+    if (src < g.num_levels()) {
+      if (s.tagger) {
+        orc_tag_sink sink{&flags};
+        if (s.tagger(s.tagger_user, &g, src, &sink) != 0) return -1;
+      } else {
+        tag_level(g, tf, src, pol.threshold, flags);
+      }
+    }
     if (pol.buffer > 0) {
       std::vector<Cell> seeds;
       seeds.swap(flags);
@@ -1469,12 +1497,7 @@ bool sim_regrid(orc_amr_sim &s) {
     }
     if (target < capacity && !nl[(size_t)target].empty()) {
       int64_t rf = g.ratios[(size_t)target], rs = g.ratios[(size_t)target - 1];
-      for (const Box2 &bx : nl[(size_t)target]) {
-        Box2 need{{bx.lo[0] / rf - 1, bx.lo[1] / rf - 1}, {bx.hi[0] / rf + 1, bx.hi[1] / rf + 1}};
-        Box2 par{{div_euclid(need.lo[0], rs), div_euclid(need.lo[1], rs)},
-                 {div_euclid(need.hi[0] + rs - 1, rs), div_euclid(need.hi[1] + rs - 1, rs)}};
-        push_cells(par, flags);
-      }
+      for (const Box2 &bx : nl[(size_t)target]) push_cells(nesting_parents(bx, rf, rs), flags);
     }
     Box2 dom = g.domain(src);
     std::vector<Cell> kept;
@@ -1494,9 +1517,9 @@ bool sim_regrid(orc_amr_sim &s) {
     nl[(size_t)target - 1] = boxes;
   }
   while (!nl.empty() && nl.back().empty()) nl.pop_back();
-  if (hierarchy_acceptable(g, nl)) return false;
+  if (hierarchy_acceptable(g, nl)) return 0;
   auto ng = std::make_unique<orc_amr>();
-  if (amr_build(g.base, g.ratios, nl, *ng) != 0) std::abort();
+  if (amr_build(g.base, g.ratios, nl, *ng) != 0) return -1;
   // State::reshaped + migrate (adapt.rs:323-347).
   std::vector<std::vector<double>> ns;
   for (int f = 0; f < s.nfields(); ++f) {
@@ -1518,7 +1541,7 @@ bool sim_regrid(orc_amr_sim &s) {
   s.grid = std::move(ng);
   s.state = std::move(ns);
   sim_alloc_stages(s);
-  return true;
+  return 1;
 }
 
 }  // namespace
@@ -1603,9 +1626,27 @@ const orc_amr *orc_amr_sim_grid(const orc_amr_sim *s) { return s->grid.get(); }
 double *orc_amr_sim_field(orc_amr_sim *s, int32_t f) { return s->state[(size_t)f].data(); }
 double orc_amr_sim_time(const orc_amr_sim *s) { return s->t; }
 
+void orc_amr_sim_set_tagger(orc_amr_sim *s, orc_tag_level_fn fn, void *user) {
+  s->tagger = fn;
+  s->tagger_user = user;
+}
+
+int orc_tag_sink_push(orc_tag_sink *sink, const int64_t *cells, size_t n) {
+  for (size_t i = 0; i < n; ++i) sink->flags->push_back({cells[2 * i], cells[2 * i + 1]});
+  return 0;
+}
+
+void orc_nesting_parents(const int64_t *box4, int64_t rf, int64_t rs, int64_t *out4) {
+  Box2 p = nesting_parents(Box2{{box4[0], box4[1]}, {box4[2], box4[3]}}, rf, rs);
+  out4[0] = p.lo[0];
+  out4[1] = p.lo[1];
+  out4[2] = p.hi[0];
+  out4[3] = p.hi[1];
+}
+
 int orc_amr_sim_step(orc_amr_sim *s, double dt, int64_t nsteps) {
   for (int64_t n = 0; n < nsteps; ++n) {
-    if (s->adaptive) sim_regrid(*s);
+    if (s->adaptive && sim_regrid(*s) < 0) return -1;
     if (s->integ == 1)
       sim_advance(*s, 0, s->t, dt, 0.0);
     else

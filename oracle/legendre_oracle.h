@@ -220,6 +220,16 @@ const orc_amr *orc_amr_sim_grid(const orc_amr_sim *s);
 double *orc_amr_sim_field(orc_amr_sim *s, int32_t field);
 double orc_amr_sim_time(const orc_amr_sim *s);
 int orc_amr_sim_step(orc_amr_sim *s, double dt, int64_t nsteps);
+/* A caller's TagCells (adapt.rs:32-46): tag_level(user, grid, level, sink) appends
+ * the level-`level` cells needing refinement through orc_tag_sink_push (n × [x, y]);
+ * prepare is the trait default (nothing).  NULL restores the GradientTagger. */
+typedef struct orc_tag_sink orc_tag_sink;
+typedef int (*orc_tag_level_fn)(void *user, const orc_amr *grid, int32_t level, orc_tag_sink *sink);
+void orc_amr_sim_set_tagger(orc_amr_sim *s, orc_tag_level_fn fn, void *user);
+int orc_tag_sink_push(orc_tag_sink *sink, const int64_t *cells, size_t n);
+/* The nesting flags of one box of the level just built (adapt.rs:194-211), box4 =
+ * [lo0, lo1, hi0, hi1] in its level's cells; out4 = the source-level box, unclipped. */
+void orc_nesting_parents(const int64_t *box4, int64_t rf, int64_t rs, int64_t *out4);
 
 /* ---- Model C derivatives (legendre_oracle_adjoint.cpp; no reference counterpart:
  *      the checkers for the device adjoint).  θ = (λ, D, ε₄); b = 0 only. ---- */

@@ -194,3 +194,59 @@ def amr_interior_mask(amr) -> np.ndarray:
         o = amr.offset(b)
         m[o:o + (nx + 2) * (ny + 2)].reshape(ny + 2, nx + 2)[1:-1, 1:-1] = True
     return m
+
+
+# A 3-level case whose level-1 boxes come from the nesting flags alone
+# (src/geometry/amr/adapt.rs:194-211): base 32² in 16² blocks (h = 0.5), ratios
+# [2, 2], one level-1 patch [16, 48)²; level 1 holds x-stripes of period 4
+# (1, 1, 0, 0) on [24, 40) × [24, 36) and 0.5 elsewhere, level 0 holds 0.5.
+# Undivided central differences are 0.5 inside the stripes and at most 0.25 at
+# their edges, and the restriction of the stripes is a period-2 pattern whose
+# central differences vanish, so at threshold 0.3 only level-1 cells are tagged:
+# [25, 39) × [24, 36).  With buffer 1 they cluster into the level-1 box
+# [24, 40) × [23, 37), i.e. level 2 = [48, 80) × [46, 74).  Coarsened to level 1
+# and grown by one that is [23, 41) × [22, 38), whose level-0 parents are
+# [11, 21) × [11, 19): level 1 = [22, 42) × [22, 38).
+STRIPES = dict(cells=[32, 32], block=[16, 16], h=0.5, ratios=[2, 2],
+               levels=[[((16, 16), (48, 48))]], policy=(4, 1, 0.7, 2, 0.3, 0),
+               expect=[(1, (22, 22), (42, 38)), (2, (48, 46), (80, 74))])
+
+
+def stripes_field(amr, n: int) -> np.ndarray:
+    """The striped field on a hierarchy with AmrGrid's patch/offset (ghost 1)."""
+    f = np.zeros(n)
+    for b in range(amr.num_blocks):
+        lev, lo, hi = amr.patch(b)
+        nx, ny = hi[0] - lo[0], hi[1] - lo[1]
+        o = amr.offset(b)
+        v = f[o:o + (nx + 2) * (ny + 2)].reshape(ny + 2, nx + 2)
+        v[:] = 0.5
+        if lev == 1:
+            for j in range(ny):
+                for i in range(nx):
+                    x, y = lo[0] + i, lo[1] + j
+                    if 24 <= x < 40 and 24 <= y < 36:
+                        v[j + 1, i + 1] = 1.0 if x % 4 < 2 else 0.0
+    return f
+
+
+def proper_nesting_violations(amr) -> list:
+    """Cells of every level-l box (l >= 2) coarsened to level l-1 and grown by one,
+    clipped to the level domain, that no level-(l-1) patch covers — the proper
+    nesting the regrid's nesting flags guarantee (adapt.rs:194-211).  ratios are
+    read off the hierarchy's boxes: this helper takes r = 2 between levels."""
+    bad = []
+    boxes = [amr.patch(b) for b in range(amr.num_blocks)]
+    dom0 = None
+    for lev, lo, hi in boxes:
+        if lev == 0:
+            dom0 = (max(dom0[0], hi[0]), max(dom0[1], hi[1])) if dom0 else hi
+    for lev, lo, hi in boxes:
+        if lev < 2:
+            continue
+        w, h = dom0[0] * 2 ** (lev - 1), dom0[1] * 2 ** (lev - 1)
+        for y in range(max(lo[1] // 2 - 1, 0), min(hi[1] // 2 + 1, h)):
+            for x in range(max(lo[0] // 2 - 1, 0), min(hi[0] // 2 + 1, w)):
+                if amr.find_patch(lev - 1, (x, y)) is None:
+                    bad.append((lev, x, y))
+    return bad

@@ -144,7 +144,9 @@ def gaussian(c):
 @pytest.mark.parametrize("subcycle", [False, True])
 def test_adaptive_heat_matches_oracle_and_golden_numbers(ctx, orc, subcycle):
     """The reference's adaptive test (tests/amr.rs:682-788) on the device: the
-    same regrid decisions (hierarchy after every step) and the same bits."""
+    same regrid decisions (hierarchy after every step) and the same bits; with a
+    global dt (the reference's configuration) the device state reproduces the
+    reference's golden numbers err_adaptive = 0.001680900 and refined_cells = 400."""
     say(f"amr adaptive heat subcycle={subcycle}")
     kappa = 0.7
     dt = 0.2 * 0.25 * 0.25 / kappa
@@ -156,6 +158,7 @@ def test_adaptive_heat_matches_oracle_and_golden_numbers(ctx, orc, subcycle):
     gg = lg.AmrGrid.from_patches(ctx, [16, 16], [8, 8], [0, 0], [0.5, 0.5], [2], [])
     sim_g = lg.amr_simulation(gg, "heat", kappa=kappa, subcycle=subcycle, policy=lg.RegridPolicy(*pol))
     sim_g.field(0).upload(sim_o.field(0))
+    refined = 0
     for s in range(60):
         sim_o.step(dt)
         sim_g.step(dt)
@@ -163,8 +166,36 @@ def test_adaptive_heat_matches_oracle_and_golden_numbers(ctx, orc, subcycle):
         assert og.num_blocks == ga.num_blocks, f"step {s}"
         for b in range(og.num_blocks):
             assert og.patch(b) == ga.patch(b), f"step {s} block {b}"
+        refined = max(refined, sum((hi[0] - lo[0]) * (hi[1] - lo[1])
+                                   for lev, lo, hi in (ga.patch(b) for b in range(ga.num_blocks)) if lev == 1))
     m = H.amr_interior_mask(sim_g.amr)
     assert np.array_equal(sim_g.field(0).download()[m], sim_o.field(0)[m])
+    if subcycle:
+        return
+    # The golden numbers from the device state: restrict on the device, gather
+    # level 0, compare with the fine uniform run averaged onto the coarse cells.
+    ga = sim_g.amr
+    f = sim_g.field(0)
+    ga.restrict_level(f, 1)
+    v = f.download()
+    amr0 = np.zeros((16, 16))
+    for b in range(ga.num_blocks):
+        lev, lo, hi = ga.patch(b)
+        if lev == 0:
+            o = ga.offset(b)
+            amr0[lo[1]:hi[1], lo[0]:hi[0]] = v[o:o + 100].reshape(10, 10)[1:-1, 1:-1]
+    ug = orc.grid([32, 32], [16, 16], [0, 0], [0.25, 0.25])
+    xs = (np.arange(32) + 0.5) * 0.25
+    uf = H.global_to_field(np.array([[gaussian((x, y)) for x in xs] for y in xs]), [32, 32], [16, 16], 1)
+    orc.heat_run(ug, kappa, uf, 60, dt)
+    fine = H.field_to_global(uf, [32, 32], [16, 16], 1)
+    ref = np.zeros((16, 16))
+    for y in range(32):
+        for x in range(32):
+            ref[y // 2, x // 2] = H._fma(fine[y, x], 0.25, ref[y // 2, x // 2])
+    err_adaptive = float(np.max(np.abs(amr0 - ref)))
+    assert abs(err_adaptive - 0.001680900) <= 1e-6, err_adaptive
+    assert refined == 400
 
 
 def test_adaptive_subcycled_model_c_matches_oracle_bitwise(ctx, orc):
