@@ -235,12 +235,14 @@ def run_adjoint(args) -> int:
         fd[name.rstrip("_")] = errs
     log(f"[adjoint] finite-difference relative errors: {fd}")
     tlm = None
+    crossings = None
     if not args.no_cpu:
         import oracle
 
         g = oracle.grid([A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])
         op = oracle.params(A.eps4, A.lambda_, A.d_thermal)
         ph, uu, tg = phi0.download(), u0.download(), target.download()
+        ph0, uu0 = ph.copy(), uu.copy()
         n = ph.size
         rng = np.random.default_rng(0)
         vu = rng.standard_normal(n)
@@ -270,6 +272,25 @@ def run_adjoint(args) -> int:
             "seconds": round(time.perf_counter() - t, 1),
         }
         log(f"[adjoint] tangent-linear check: {tlm}")
+        # Face evaluations whose flat branch |∇φ|⁴ <= tol differs between the θ − h
+        # and θ + h runs (finite_volume.rs:129-142): a central difference with any
+        # carries a jump of L, whatever h is.
+        crossings = {}
+        t = time.perf_counter()
+        cores = tlm["cores"]
+        for name in ("lambda_", "d_thermal", "eps4"):
+            per = {}
+            for h in (1e-6, 1e-7):
+                lo, hi = dict(base), dict(base)
+                lo[name] -= h
+                hi[name] += h
+                per[f"h={h:g}"] = oracle.modelc_branch_crossings(
+                    g, oracle.params(lo["eps4"], lo["lambda_"], lo["d_thermal"]),
+                    oracle.params(hi["eps4"], hi["lambda_"], hi["d_thermal"]), ph0, uu0, A.steps, A.dt,
+                    cores)
+                log(f"[adjoint] branch crossings {name} h={h:g}: {per[f'h={h:g}']} "
+                    f"({time.perf_counter() - t:.0f}s)")
+            crossings[name.rstrip("_")] = per
     cal.close()
     sim0.close()
     ctx.close()
@@ -305,6 +326,7 @@ def run_adjoint(args) -> int:
             "loss": loss,
             "grad": [float(x) for x in grad],
             "fd_rel_err": fd,
+            "fd_branch_crossings": crossings,
             "tlm": tlm,
         },
     }

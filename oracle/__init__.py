@@ -83,6 +83,8 @@ _SIGS = {
     "orc_modelc_run": (C.c_int, [GP, C.POINTER(ModelCParams), DP, DP, C.c_int64, C.c_double, DP,
                                  C.c_uint64, C.c_int32, C.c_int32]),
     "orc_heat_run": (C.c_int, [GP, C.c_double, DP, C.c_int64, C.c_double, DP, C.POINTER(FaceBc), C.c_int32]),
+    "orc_modelc_branch_crossings": (C.c_int, [GP, C.POINTER(ModelCParams), C.POINTER(ModelCParams), DP, DP,
+                                              C.c_int64, C.c_double, C.c_int32, C.POINTER(C.c_int64)]),
     "orc_initialize_grains": (None, [GP, DP, DP, DP, C.c_size_t, C.c_double]),
     "orc_seed_positions": (None, [C.c_uint64, C.c_size_t, C.c_double, C.c_double, DP]),
     "orc_modelc_rhs_oriented": (None, [GP, C.POINTER(ModelCParams), DP, DP, DP, DP, DP]),
@@ -253,6 +255,18 @@ def modelc_run(g: Grid, p: ModelCParams, phi, u, nsteps, dt, t0=0.0, seed=0, wie
     if rc != 0:
         raise ValueError("orc_modelc_run rejected the configuration")
     return t.value
+
+
+def modelc_branch_crossings(g: Grid, p_minus: ModelCParams, p_plus: ModelCParams, phi0, u0, nsteps,
+                            dt, nthreads=1) -> int:
+    """Face evaluations (four faces and the centre per cell and step) whose flat
+    branch |∇φ|⁴ <= tol differs between the runs at p_minus and p_plus (b = 0)."""
+    n = C.c_int64()
+    rc = lib().orc_modelc_branch_crossings(C.byref(g), C.byref(p_minus), C.byref(p_plus), _dp(phi0),
+                                           _dp(u0), nsteps, dt, nthreads, C.byref(n))
+    if rc != 0:
+        raise ValueError("orc_modelc_branch_crossings rejected the configuration")
+    return int(n.value)
 
 
 def heat_run(g: Grid, kappa, u, nsteps, dt, t0=0.0, bcs=None, nthreads=1) -> float:

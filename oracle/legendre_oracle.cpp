@@ -498,6 +498,69 @@ int orc_modelc_run(const orc_grid *g, const orc_modelc_params *p, double *phi, d
   return 0;
 }
 
+// The branch face() (finite_volume.rs:129-142) takes at each of a cell's four faces
+// (r, l, t, b: bits 0..3) and the centre anisotropy (phasefield.rs:159-175, bit 4)
+// for a ghost-filled phi, 1 = the flat branch (|∇φ|⁴ <= tol); one byte per interior
+// cell, block-major.
+void modelc_branch_mask(const orc_grid *g, const orc_modelc_params *p, const double *phi,
+                        std::vector<uint8_t> &mask, int32_t nthreads) {
+  const int64_t nblocks = orc_num_blocks(g), nx = g->block_cells[0], ny = g->block_cells[1];
+  const int64_t sx = nx + 2, len = orc_block_len(g, 1);
+  mask.assign((size_t)(nblocks * nx * ny), 0);
+  auto flat = [](double dx, double dy, double tol) {
+    double dx2 = dx * dx, dy2 = dy * dy, sum = dx2 + dy2;
+    return sum * sum <= tol;
+  };
+#pragma omp parallel for schedule(static) num_threads(nthreads > 0 ? nthreads : 1) if (nthreads > 1)
+  for (int64_t b = 0; b < nblocks; ++b) {
+    const double *in = phi + b * len;
+    auto q = [&](int64_t i, int64_t j) { return in[(j + 1) * sx + (i + 1)]; };
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t i = 0; i < nx; ++i) {
+        const double qq = 0.25;
+        double derx_r = q(i + 1, j) - q(i, j), derx_l = q(i, j) - q(i - 1, j);
+        double derx_t = qq * (q(i + 1, j + 1) - q(i - 1, j + 1) + q(i + 1, j) - q(i - 1, j));
+        double derx_b = qq * (q(i + 1, j) - q(i - 1, j) + q(i + 1, j - 1) - q(i - 1, j - 1));
+        double dery_t = q(i, j + 1) - q(i, j), dery_b = q(i, j) - q(i, j - 1);
+        double dery_r = qq * (q(i + 1, j + 1) - q(i + 1, j - 1) + q(i, j + 1) - q(i, j - 1));
+        double dery_l = qq * (q(i, j + 1) - q(i, j - 1) + q(i - 1, j + 1) - q(i - 1, j - 1));
+        double cx = 0.5 * (q(i + 1, j) - q(i - 1, j)), cy = 0.5 * (q(i, j + 1) - q(i, j - 1));
+        uint8_t m = (uint8_t)(flat(derx_r, dery_r, p->fv_tol) | flat(derx_l, dery_l, p->fv_tol) << 1 |
+                              flat(derx_t, dery_t, p->fv_tol) << 2 | flat(derx_b, dery_b, p->fv_tol) << 3 |
+                              flat(cx, cy, p->anisotropy_tol) << 4);
+        mask[(size_t)((b * ny + j) * nx + i)] = m;
+      }
+  }
+}
+
+int orc_modelc_branch_crossings(const orc_grid *g, const orc_modelc_params *pm,
+                                const orc_modelc_params *pp, const double *phi0, const double *u0,
+                                int64_t nsteps, double dt, int32_t nthreads, int64_t *crossings) {
+  if (g->dim != 2 || orc_grid_check(g) != 0) return -1;
+  const size_t total = (size_t)(orc_num_blocks(g) * orc_block_len(g, 1));
+  std::vector<double> phi[2] = {std::vector<double>(phi0, phi0 + total),
+                                std::vector<double>(phi0, phi0 + total)};
+  std::vector<double> u[2] = {std::vector<double>(u0, u0 + total), std::vector<double>(u0, u0 + total)};
+  const orc_modelc_params *ps[2] = {pm, pp};
+  std::vector<uint8_t> mask[2];
+  int64_t count = 0;
+  for (int64_t s = 0; s < nsteps; ++s) {
+    for (int r = 0; r < 2; ++r) {
+      if (orc_fill_ghosts(g, phi[r].data(), 1, nullptr) != 0) return -1;
+      modelc_branch_mask(g, ps[r], phi[r].data(), mask[r], nthreads);
+    }
+    for (size_t c = 0; c < mask[0].size(); ++c)
+      count += __builtin_popcount((unsigned)(mask[0][c] ^ mask[1][c]));
+    for (int r = 0; r < 2; ++r) {
+      double t = 0.0;
+      if (orc_modelc_run(g, ps[r], phi[r].data(), u[r].data(), 1, dt, &t, 0, 0, nthreads) != 0)
+        return -1;
+    }
+  }
+  *crossings = count;
+  return 0;
+}
+
 int orc_heat_run(const orc_grid *g, double kappa, double *u, int64_t nsteps, double dt, double *t,
                  const orc_face_bc *bcs, int32_t nthreads) {
   if (orc_grid_check(g) != 0) return -1;

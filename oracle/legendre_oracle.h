@@ -112,6 +112,15 @@ int orc_modelc_run(const orc_grid *g, const orc_modelc_params *p, double *phi, d
 
 /* ForwardEuler::step (src/integrators/euler.rs:104-130) on Heat<D>; bcs as in
  * orc_fill_ghosts. */
+/* Face evaluations whose branch differs between two runs from the same state (params
+ * pm and pp, b = 0, nsteps steps): at every step's right-hand side, each interior
+ * cell's four face() calls (finite_volume.rs:129-142) and its centre anisotropy
+ * (phasefield.rs:159-175) take the flat branch (|∇φ|⁴ <= tol) or not; the count sums
+ * the evaluations where the two runs disagree.  Zero means L(θ) is smooth between
+ * the two parameter sets along the trajectory. */
+int orc_modelc_branch_crossings(const orc_grid *g, const orc_modelc_params *pm,
+                                const orc_modelc_params *pp, const double *phi0, const double *u0,
+                                int64_t nsteps, double dt, int32_t nthreads, int64_t *crossings);
 int orc_heat_run(const orc_grid *g, double kappa, double *u, int64_t nsteps, double dt, double *t,
                  const orc_face_bc *bcs, int32_t nthreads);
 

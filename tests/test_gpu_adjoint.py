@@ -173,35 +173,62 @@ def test_calibration_streaming_and_three_pass_agree(ctx, orc, case):
     assert rel_close(p2, p1, 1e-11) and rel_close(u2, u1, 1e-11)
 
 
-def test_calibration_gradient_matches_central_differences(ctx, orc):
-    case, steps = 0, 60
-    say(f"calibration FD steps={steps}")
+def _fd_check(ctx, orc, case, steps, hs_by_param):
+    """Central differences of the device loss against the device gradient at the step
+    sizes given per parameter, each with the oracle's count of face evaluations whose
+    flat branch |∇φ|⁴ <= tol (finite_volume.rs:129-142) differs between the θ − h and
+    θ + h runs: returns [(name, h, crossings, rel err)]."""
     og, op, phi, u, target = _calibration_inputs(orc, case, steps)
     grid = gpu_grid(ctx, case)
     dphi, du, dt_ = upload(ctx, phi), upload(ctx, u), upload(ctx, target)
     cal = lg.ModelCCalibration(grid, steps, DT, snapshots=8)
     base = dict(eps4=0.05, lambda_=3.19, d_thermal=2.0)
 
-    def loss(**kw):
+    def at(**kw):
         pr = dict(base)
         pr.update(kw)
+        return pr
+
+    def loss(pr):
         return cal.forward(lg.ModelCParams(pr["eps4"], pr["lambda_"], pr["d_thermal"]), dphi, du, dt_)
 
-    loss()
+    loss(base)
     grad = cal.backward()
-    # The face function switches to its flat branch where |∇φ|⁴ <= tol
-    # (finite_volume.rs:129-142), so L is only piecewise smooth: a central
-    # difference whose ±h runs put one face on different branches carries that
-    # jump (here ε₄ at h = 1e-6: 3e-5 off, while h <= 3e-7 agrees to 2e-10 and the
-    # tangent-linear run to 1e-15).  The adjoint is the derivative of the branch
-    # taken; the check takes the better of h = 1e-6 and 1e-7.
+    out = []
     for k, name in enumerate(["lambda_", "d_thermal", "eps4"]):
-        errs = []
-        for h in (1e-6, 1e-7):
-            fd = (loss(**{name: base[name] + h}) - loss(**{name: base[name] - h})) / (2 * h)
-            errs.append(abs(fd - grad[k]) / abs(grad[k]))
-        assert min(errs) <= 1e-6, (name, errs, grad[k])
+        for h in hs_by_param[name]:
+            lo, hi = at(**{name: base[name] - h}), at(**{name: base[name] + h})
+            cross = orc.modelc_branch_crossings(og, orc.params(lo["eps4"], lo["lambda_"], lo["d_thermal"]),
+                                                orc.params(hi["eps4"], hi["lambda_"], hi["d_thermal"]),
+                                                phi, u, steps, DT, 8)
+            fd = (loss(hi) - loss(lo)) / (2 * h)
+            out.append((name, h, cross, abs(fd - grad[k]) / abs(grad[k])))
     cal.close()
+    return out
+
+
+def test_calibration_gradient_matches_central_differences_at_h_1e_6(ctx, orc):
+    """FD at h = 1e-6 (no other step size) on a run where no face changes branch
+    between θ ± h: every parameter within 1e-6."""
+    say("calibration FD h=1e-6 steps=30")
+    res = _fd_check(ctx, orc, 0, 30, {n: [1e-6] for n in ("lambda_", "d_thermal", "eps4")})
+    for name, h, cross, err in res:
+        assert cross == 0, (name, cross)
+        assert err <= 1e-6, (name, h, err)
+
+
+def test_central_differences_fail_only_where_a_face_changes_branch(ctx, orc):
+    """60 steps: at h = 1e-6 one ε₄ face evaluation takes the other branch of the
+    face function between θ − h and θ + h, and that central difference carries the
+    jump; every step size with no crossing agrees within 1e-6."""
+    say("calibration FD branch crossings steps=60")
+    res = _fd_check(ctx, orc, 0, 60, {"lambda_": [1e-6], "d_thermal": [1e-6], "eps4": [1e-6, 1e-7]})
+    for name, h, cross, err in res:
+        print(f"{name} h={h:g}: crossings {cross}, FD rel err {err:.2e}", flush=True)
+        if cross == 0:
+            assert err <= 1e-6, (name, h, err)
+    assert any(cross > 0 for name, h, cross, err in res if name == "eps4" and h == 1e-6)
+    assert any(cross == 0 for name, h, cross, err in res if name == "eps4" and h == 1e-7)
 
 
 def test_closing_the_context_first_releases_everything():

@@ -176,7 +176,8 @@ def test_adaptive_heat_matches_oracle_and_golden_numbers(ctx, orc, subcycle):
     # level 0, compare with the fine uniform run averaged onto the coarse cells.
     ga = sim_g.amr
     f = sim_g.field(0)
-    ga.restrict_level(f, 1)
+    if ga.num_levels > 1:  # the bump has usually diffused below the threshold by now
+        ga.restrict_level(f, 1)
     v = f.download()
     amr0 = np.zeros((16, 16))
     for b in range(ga.num_blocks):

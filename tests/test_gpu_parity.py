@@ -505,6 +505,60 @@ def test_modelc_full_scale_first_steps_bitwise(ctx, orc):
     assert np.array_equal(gu[:, 1:-1, 1:-1], uv[:, 1:-1, 1:-1])
 
 
+def _max_rel(got, want):
+    return float(np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300)))
+
+
+@pytest.mark.slow
+def test_modelc_noise_reduced_config_1000_steps(ctx, orc):
+    """The benchmark physics with Wiener noise b = 0.01 at 1008² (8×8 blocks of 126²),
+    64 seeds, 1000 steps: within 1e-9 of the oracle after 1000 steps
+    (src/core/driver.rs:139-158), and within 1e-12 relative after the first."""
+    from legendre_amd.workloads import ModelCConfig
+
+    C = ModelCConfig(cells=1008)
+    cells, block = [C.cells] * 2, [C.block] * 2
+    say("modelc 1008^2 x 1000 steps b=0.01 fused")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], C.grains(), 0.01,
+                                           True, lg.LGD_MODE_FUSED, u0=C.delta, eps4=C.eps4,
+                                           lam=C.lambda_, D=C.d_thermal)
+    sim.step(C.dt, 1)
+    t = orc.modelc_run(og, op, phi, u, 1, C.dt, seed=42, wiener=True, nthreads=16)
+    for f, w in ((0, phi), (1, u)):
+        assert _max_rel(_interiors(sim.field(f).download(), cells, block), _interiors(w, cells, block)) <= 1e-12
+    sim.step(C.dt, C.steps - 1)
+    t = orc.modelc_run(og, op, phi, u, C.steps - 1, C.dt, t0=t, seed=42, wiener=True, nthreads=16)
+    assert sim.time == t
+    for f, w in ((0, phi), (1, u)):
+        err = float(np.max(np.abs(_interiors(sim.field(f).download(), cells, block) - _interiors(w, cells, block))))
+        print(f"field {f}: max |GPU - oracle| after {C.steps} steps = {err:.3e}", flush=True)
+        assert err <= 1e-9
+
+
+@pytest.mark.slow
+def test_modelc_noise_full_scale_first_steps(ctx, orc):
+    """10080² with Wiener noise b = 0.01: each of the first 3 steps within 1e-12
+    relative of the oracle."""
+    from legendre_amd.workloads import MODEL_C as C
+
+    cells, block = [C.cells] * 2, [C.block] * 2
+    say("modelc 10080^2 x 3 steps b=0.01 fused")
+    og, gg, sim, op, phi, u = _modelc_pair(ctx, orc, cells, block, [False, False], C.grains(), 0.01,
+                                           True, lg.LGD_MODE_FUSED, u0=C.delta, eps4=C.eps4,
+                                           lam=C.lambda_, D=C.d_thermal)
+    t = 0.0
+    for step in range(3):
+        sim.step(C.dt, 1)
+        t = orc.modelc_run(og, op, phi, u, 1, C.dt, t0=t, seed=42, wiener=True, nthreads=16)
+        for f, w in ((0, phi), (1, u)):
+            got = H.field_view(sim.field(f).download(), cells, block, 1)[:, 1:-1, 1:-1]
+            want = H.field_view(w, cells, block, 1)[:, 1:-1, 1:-1]
+            rel = _max_rel(got, want)
+            del got
+            print(f"step {step + 1} field {f}: max relative difference {rel:.3e}", flush=True)
+            assert rel <= 1e-12
+
+
 def test_autotuner_picks_a_shape_and_results_do_not_change(ctx, orc, tuned):
     """The autotuner times every launch shape and keeps one; the step it then
     uses computes the same bits as the default shape."""

@@ -209,3 +209,24 @@ def test_tlm_several_directions_equal_single_runs():
         assert np.array_equal(interior(b, cells, block), interior(du[k], cells, block))
         assert np.array_equal(interior(c1, cells, block), interior(c, cells, block))
     assert n == dp.shape[1]
+
+
+def test_branch_crossings_count_the_faces_between_two_tolerances(orc):
+    """One spike φ = c at cell (1, 1) of a 4×4 block: the four faces of that cell and
+    the facing face of each of its four neighbours have |∇φ|⁴ = c⁴ (normal difference
+    ±c, transverse 0); every other face has at most c⁴/256.  Between fv_tol = c⁴/2 and
+    2c⁴ exactly those 8 evaluations change branch in the first step's right-hand
+    side; tolerances below c⁴/256 on both sides change none."""
+    g = orc.grid([4, 4], [4, 4], [0, 0], [0.4, 0.4])
+    n = orc.field_len(g, 1)
+    phi, u = np.zeros(n), np.zeros(n)
+    c = 0.1
+    phi[(1 + 1) * 6 + (1 + 1)] = c
+    c4 = c ** 4
+
+    def p(tol):
+        return orc.params(0.05, 3.19, 2.0, 0.0, anisotropy_tol=1e-12, fv_tol=tol)
+
+    assert orc.modelc_branch_crossings(g, p(0.5 * c4), p(2.0 * c4), phi, u, 1, 0.008) == 8
+    assert orc.modelc_branch_crossings(g, p(1e-3 * c4), p(2e-3 * c4), phi, u, 1, 0.008) == 0
+    assert orc.modelc_branch_crossings(g, p(0.5 * c4), p(0.5 * c4), phi, u, 5, 0.008) == 0
