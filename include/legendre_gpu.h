@@ -513,6 +513,25 @@ int lgd_modelc_adjoint_step(lgd_adjoint *adj, const lgd_modelc_params *params,
 int lgd_adjoint_reset(lgd_adjoint *adj);
 int lgd_adjoint_grad(lgd_adjoint *adj, double *grad);
 
+/* The reverse step without allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory — code under HIP graph capture, tight
+ * calibration loops, harnesses that forbid hipMalloc.  lgd_modelc_adjoint_step on the
+ * streaming reverse step with no lgd_adjoint: its workspace (the flattened adjoint in
+ * and out and the per-wave θ partials) is a static device array of the library, one
+ * per GPU, held until the process ends (about 34 MB).  (pphi, pu) as for
+ * lgd_modelc_adjoint_step, bitwise the same; grad (>= 3 f64 on the device) receives
+ * this step's dL/dθ = (dL/dλ, dL/dD, dL/dε₄), bitwise what lgd_adjoint_grad reports
+ * after lgd_adjoint_reset and the same one step.  Calls no allocator.  Grids above
+ * LGD_NOALLOC_ADJOINT_CELLS interior cells, grids the streaming reverse step does not
+ * cover, and the "adjoint_kernel" = 1 setting return LGD_ERR_UNSUPPORTED.  Every call
+ * on one GPU shares the array: calls that overlap on one GPU (other streams or
+ * threads) race. */
+#define LGD_NOALLOC_ADJOINT_CELLS (1u << 20)
+int lgd_modelc_adjoint_step_noalloc(lgd_ctx *ctx, const lgd_grid *grid,
+                                    const lgd_modelc_params *params, const lgd_buf *phi,
+                                    const lgd_buf *u, double dt, lgd_buf *pphi, lgd_buf *pu,
+                                    lgd_buf *grad);
+
 /* Calibration: L(θ, φ₀, u₀) = ½‖φ_T − target‖² over the interiors after nsteps steps,
  * as a forward and a backward call, reversed by binomial checkpointing in `snapshots`
  * state slots (the initial state included; >= 1).  Results are bitwise independent of

@@ -198,6 +198,7 @@ SIGNATURES = {
     "lgd_adjoint_create": (C.c_int, [P, P, PP]),
     "lgd_adjoint_destroy": (C.c_int, [P]),
     "lgd_modelc_adjoint_step": (C.c_int, [P, C.POINTER(ModelCParamsC), P, P, DBL, P, P]),
+    "lgd_modelc_adjoint_step_noalloc": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P, P, DBL, P, P, P]),
     "lgd_adjoint_reset": (C.c_int, [P]),
     "lgd_adjoint_grad": (C.c_int, [P, DP]),
     "lgd_calibration_create": (C.c_int, [P, P, I64, DBL, I32, PP]),

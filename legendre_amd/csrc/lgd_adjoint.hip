@@ -31,6 +31,7 @@
 // slots; the schedule's first chain of checkpoints is laid down by the forward sweep.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "lgd_internal.h"
@@ -442,6 +443,24 @@ struct lgd_adjoint {
 
 namespace {
 
+// The reverse step's view of a 2-D Cartesian grid; false when the flat indices of
+// the global arrays would not fit 32 bits.
+bool adj_grid_of(const lgd_cart &c, AdjGrid &G) {
+  const int64_t N0 = c.n[0] * c.nb[0], N1 = c.n[1] * c.nb[1];
+  if ((N0 + 2) * (N1 + 2) >= ((int64_t)1 << 31)) return false;
+  G.n0 = (int)c.n[0];
+  G.n1 = (int)c.n[1];
+  G.nb0 = (int)c.nb[0];
+  G.N0 = (int)N0;
+  G.N1 = (int)N1;
+  G.sx = (int)(c.n[0] + 2);
+  G.slab = (int64_t)G.sx * (c.n[1] + 2);
+  G.per0 = c.periodic[0];
+  G.per1 = c.periodic[1];
+  G.P0 = G.N0 + 2;
+  return true;
+}
+
 int adj_alloc(lgd_adjoint *a, int64_t n, double **p) {
   hipError_t e = hipMalloc((void **)p, (size_t)std::max<int64_t>(n, 1) * sizeof(double));
   if (e != hipSuccess) {
@@ -537,22 +556,10 @@ int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out) {
   a->ctx = ctx;
   a->grid = grid;
   AdjGrid &G = a->G;
-  // Flat indices of the global scratch arrays are 32-bit.
-  const int64_t N0 = c.n[0] * c.nb[0], N1 = c.n[1] * c.nb[1];
-  if ((N0 + 2) * (N1 + 2) >= ((int64_t)1 << 31)) {
+  if (!adj_grid_of(c, G)) {
     delete a;
     return fail(ctx, LGD_ERR_UNSUPPORTED, "adjoint domain above 2^31 cells");
   }
-  G.n0 = (int)c.n[0];
-  G.n1 = (int)c.n[1];
-  G.nb0 = (int)c.nb[0];
-  G.N0 = (int)N0;
-  G.N1 = (int)N1;
-  G.sx = (int)(c.n[0] + 2);
-  G.slab = (int64_t)G.sx * (c.n[1] + 2);
-  G.per0 = c.periodic[0];
-  G.per1 = c.periodic[1];
-  G.P0 = G.N0 + 2;
   a->ncell = (int64_t)G.N0 * G.N1;
   a->nfx = (int64_t)(G.N0 + 1) * G.N1;
   a->nfy = (int64_t)G.N0 * (G.N1 + 1);
@@ -643,6 +650,106 @@ int lgd_adjoint_grad(lgd_adjoint *a, double *grad) {
 }
 
 }  // extern "C"
+
+// ------------------------------------------------- the reverse step without allocation
+// lgd_modelc_adjoint_step_noalloc keeps its workspace in one static device array per
+// GPU instead of an lgd_adjoint: the flattened incoming and outgoing adjoint (four
+// arrays of kNoAllocCells), the per-wave θ partials of the streaming reverse step and
+// the reduced sums.  Host code never uses the array's name as an address: each
+// device's copy is looked up once with hipGetSymbolAddress and kept in
+// g_scratch_ptr.
+constexpr int64_t kNoAllocCells = (int64_t)1 << 20;  // LGD_NOALLOC_ADJOINT_CELLS
+constexpr int64_t kNoAllocAcc = (int64_t)1 << 16;    // θ partials: 4 per wave
+constexpr int64_t kNoAllocDoubles = 4 * kNoAllocCells + kNoAllocAcc + 8;
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int noalloc_scratch(lgd_ctx *ctx, double **out) {
+  const int dev = ctx->device;
+  if (dev < 0 || dev >= kMaxDevices) return fail(ctx, LGD_ERR_UNSUPPORTED, "device index");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    int prev = -1;
+    hipError_t e = hipGetDevice(&prev);
+    if (e == hipSuccess && prev != dev) e = hipSetDevice(dev);
+    double *p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (prev >= 0 && prev != dev) (void)hipSetDevice(prev);
+    if (e != hipSuccess || !p) return hip_fail(ctx, e, "adjoint scratch symbol address");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return LGD_OK;
+}
+
+// grad = (0 + Σλ, 0 + ΣD, (0 + 0) + (Σε₄ centre + Σε₄ faces)): lgd_adjoint_grad's
+// operations with the three-pass accumulators at zero, so both report the same bits.
+__global__ void adj_noalloc_grad_kernel(const double *sums, double *grad) {
+  if (threadIdx.x == 0) {
+    grad[0] = 0.0 + sums[0];
+    grad[1] = 0.0 + sums[1];
+    grad[2] = (0.0 + 0.0) + (sums[2] + sums[3]);
+  }
+}
+
+}  // namespace
+
+extern "C" int lgd_modelc_adjoint_step_noalloc(lgd_ctx *ctx, const lgd_grid *grid,
+                                               const lgd_modelc_params *params, const lgd_buf *phi,
+                                               const lgd_buf *u, double dt, lgd_buf *pphi,
+                                               lgd_buf *pu, lgd_buf *grad) {
+  if (!ctx || !grid || !params || !grad) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "null argument");
+  const lgd_cart &c = grid->cart;
+  if (c.dim != 2) return fail(ctx, LGD_ERR_UNSUPPORTED, "the Model C adjoint is 2-D");
+  int rc;
+  if ((rc = check_field(ctx, grid, phi, 1, "phi"))) return rc;
+  if ((rc = check_field(ctx, grid, u, 1, "u"))) return rc;
+  if ((rc = check_field(ctx, grid, pphi, 1, "pphi"))) return rc;
+  if ((rc = check_field(ctx, grid, pu, 1, "pu"))) return rc;
+  if (pphi == pu || pphi == phi || pphi == u || pu == phi || pu == u)
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "adjoint buffers must be distinct from the state");
+  if (grad->len < 3) return fail(ctx, LGD_ERR_SIZE_MISMATCH, "grad holds fewer than 3 values");
+  if (ctx->adjoint_kernel == 1)
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "the three-pass reverse step needs an lgd_adjoint workspace");
+  AdjGrid G{};
+  const int64_t ncell = c.n[0] * c.nb[0] * c.n[1] * c.nb[1];
+  if (ncell > kNoAllocCells || !adj_grid_of(c, G))
+    return fail(ctx, LGD_ERR_UNSUPPORTED,
+                "grid above the allocation-free reverse step's 2^20 cells (use lgd_adjoint_create)");
+  const int rows = (int)std::min<int64_t>(ctx->adjoint_rows, 1 << 20);  // 0: by occupancy
+  const lgd_kr_consts k = lgd_make_kr_consts(*params, c);
+  int64_t nw = 0;
+  if (lgd_adjoint_stream_launch(ctx, c, k, dt, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                nullptr, nullptr, rows, &nw) != LGD_OK)
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "streaming reverse step: unsupported block shape");
+  if (4 * nw > kNoAllocAcc)
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "too many reverse-step waves for the static scratch");
+  double *base = nullptr;
+  if ((rc = noalloc_scratch(ctx, &base))) return rc;
+  double *AP = base, *AU = base + kNoAllocCells, *AP2 = base + 2 * kNoAllocCells,
+         *AU2 = base + 3 * kNoAllocCells, *acc = base + 4 * kNoAllocCells,
+         *sums = acc + kNoAllocAcc;
+  // The previous call's partials are still there: clear them with a kernel (a
+  // memset node misbehaves under graph capture).
+  if ((rc = zero(ctx, acc, 4 * nw))) return rc;
+  adj_flatten_kernel<<<blocks_for(ncell, kWG), kWG, 0, ctx->stream>>>(G, pphi->data, pu->data, AP, AU);
+  if ((rc = launch_check(ctx, "adjoint flatten"))) return rc;
+  rc = lgd_adjoint_stream_launch(ctx, c, k, dt, phi->data, u->data, AP, AU, AP2, AU2, acc, rows,
+                                 nullptr);
+  if (rc) return rc;
+  adj_unflatten_kernel<<<blocks_for(ncell, kWG), kWG, 0, ctx->stream>>>(G, AP2, AU2, pphi->data,
+                                                                       pu->data);
+  if ((rc = launch_check(ctx, "adjoint unflatten"))) return rc;
+  if ((rc = sum_strided(ctx, nw, acc, 4, sums))) return rc;
+  adj_noalloc_grad_kernel<<<1, 64, 0, ctx->stream>>>(sums, grad->data);
+  return launch_check(ctx, "adjoint gradient");
+}
 
 // ------------------------------------------------------------------ calibration
 struct lgd_calibration {
