@@ -24,7 +24,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Model C Gcell-steps/s (FP64, % HBM roofline); batched & adjoint vs plain"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_CELL_STEP = 32  # phi, u read + phi, u written, f64
 
 
 def log(msg: str) -> None:
@@ -85,18 +84,41 @@ def run_gpu(args, C):
     if not np.all(np.isfinite(phi)):
         raise RuntimeError("non-finite state after the timed steps")
     sim.close()
-    if args.calibrate:
-        # A kernel of known traffic for scaling profiler byte counters: one
-        # axpy over 2^27 f64 reads 2 GiB and writes 1 GiB.
-        n = 1 << 27
-        x, y = lg.DeviceBuffer(ctx, n), lg.DeviceBuffer(ctx, n)
-        grid.axpy(x, y, 0.5)
-        ctx.sync()
-        log(f"[bench] calibration axpy: n={n}, algorithmic fetch {16 * n} B, write {8 * n} B")
-        x.free()
-        y.free()
+    ceiling = measure_ceiling(lg, ctx, stream)
+    log(f"[bench] streaming ceiling (copy, 16-byte nontemporal): {ceiling}")
     ctx.close()
-    return total_ms, per, wall
+    return total_ms, per, wall, ceiling
+
+
+def measure_ceiling(lg, ctx, stream, n: int = 1 << 28, reps: int = 5):
+    """The device's streaming rate: lgd_stream_copy over n f64 (reads 8n B, writes 8n
+    B) with 1, 2, 4 and 8 16-byte vectors in flight per lane, each the best of `reps`
+    launches timed by HIP events on the launch stream.  One launch of the kVec = 4
+    form also serves profiler byte-counter calibration (known traffic)."""
+    import ctypes as C
+
+    import torch
+
+    from legendre_amd import _abi
+
+    x, y = lg.DeviceBuffer(ctx, n), lg.DeviceBuffer(ctx, n)
+    rates = {}
+    for vec in (1, 2, 4, 8):
+        best = None
+        for r in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            _abi.check(ctx._lib.lgd_stream_copy(ctx.handle, x.handle, y.handle, vec), ctx.handle)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            if r > 0:  # the first launch warms the pages
+                best = ms if best is None else min(best, ms)
+        rates[vec] = round(16 * n / (best / 1e3) / 1e9, 1)
+    x.free()
+    y.free()
+    vec = max(rates, key=rates.get)
+    return {"gbs": rates[vec], "vec": vec, "by_vec": rates, "bytes": 16 * n}
 
 
 def read_traffic(profile: str):
@@ -111,41 +133,141 @@ def read_traffic(profile: str):
     return None
 
 
+def roofline(nbytes: float, seconds: float, kernel: str, unit_bytes: str) -> dict:
+    """HBM roofline of a whole workload: algorithmic bytes over wall seconds."""
+    achieved = nbytes / seconds / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "algorithmic_bytes": unit_bytes}
+
+
+def cpu_threads() -> int:
+    """Every core of this process's affinity mask, within OMP_NUM_THREADS when the
+    host sets it (the CPU share the machine grants this process)."""
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(cores, int(env))) if env.isdigit() and int(env) > 0 else cores
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def native_oracle():
+    """Load the oracle built for this host's CPU (-march=native, same bits) for timing."""
+    import tempfile
+
+    import oracle
+
+    t = time.perf_counter()
+    path = oracle.build_native(tempfile.mkdtemp(prefix="lgd_oracle_native_"))
+    oracle.load(path)
+    log(f"[bench] oracle built with -march=native in {time.perf_counter() - t:.1f}s")
+    return oracle
+
+
+def run_extras(args, plain_value: float) -> dict:
+    """The metric's "batched & adjoint vs plain" legs at reduced length, each with its
+    own roofline: batched AMR (60 coarse steps, one per-patch chunk), the ensemble
+    (100 steps, 2 per-block steps) and the adjoint gradient (200 steps, 45 slots)."""
+    import argparse
+
+    base = dict(vars(args))
+
+    def ns(**kw):
+        d = dict(base)
+        d.update(kw)
+        return argparse.Namespace(**d)
+
+    legs = [
+        ("batched_amr", run_amr, ns(amr_steps=60, per_block_chunks=1, graph=0, kernel="auto")),
+        ("batched_ensemble", run_ensemble,
+         ns(ens_steps=100, per_block_steps=2, no_cpu=True, kernel="auto", strips=0, prefetch=0)),
+        ("adjoint", run_adjoint, ns(adj_steps=200, snapshots=0, no_cpu=True, adj_kernel="auto", adj_rows=0)),
+    ]
+    out = {}
+    for name, fn, a in legs:
+        t = time.perf_counter()
+        line = fn(a)
+        ratio = {"batched_amr": ("speedup_vs_per_patch_launches", line.get("amr", {})),
+                 "batched_ensemble": ("speedup_vs_per_block", line.get("ensemble", {})),
+                 "adjoint": ("gradient_over_forward", line.get("adjoint", {}))}[name]
+        out[name] = {
+            "workload": line["config"]["workload"],
+            "value": line["value"],
+            "unit": line["unit"],
+            "ms_per_step": line["ms_per_step"],
+            "vs_plain": round(line["value"] / plain_value, 4),
+            ratio[0]: ratio[1].get(ratio[0]),
+            "roofline": {k: line["roofline"][k] for k in ("achieved", "peak", "unit", "frac", "kernel",
+                                                           "algorithmic_bytes")},
+            "seconds": round(time.perf_counter() - t, 1),
+        }
+        log(f"[bench] {name}: {out[name]}")
+    return out
+
+
 def run_cpu_baseline(C, budget_s: float):
     """The oracle (a C++ port of the reference's serial semantics, OpenMP over
     blocks like the RayonScheduler) on this host, on a bounded number of steps
-    of the same workload."""
+    of the same workload, in both of its builds (-march=x86-64-v3, and -march=native
+    built here); the faster is the baseline."""
     import numpy as np
 
     import oracle
 
-    cores = len(os.sched_getaffinity(0))
-    threads = max(1, min(16, cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    threads = cpu_threads()
+    builds = [("x86-64-v3", oracle.LIB_PATH)]
+    try:
+        import tempfile
+
+        t = time.perf_counter()
+        builds.append(("native", oracle.build_native(tempfile.mkdtemp(prefix="lgd_oracle_native_"))))
+        log(f"[bench] oracle built with -march=native in {time.perf_counter() - t:.1f}s")
+    except Exception as e:  # noqa: BLE001 - the portable build still times the baseline
+        log(f"[bench] -march=native build failed ({e}); timing the portable build only")
     g = oracle.grid([C.cells] * 2, [C.block] * 2, [0.0, 0.0], [C.h, C.h])
     p = oracle.params(C.eps4, C.lambda_, C.d_thermal, C.noise)
     n = oracle.field_len(g, 1)
-    phi = np.zeros(n)
-    u = np.zeros(n)
-    oracle.initialize_grains(g, phi, u, C.grains(), C.delta)
-    t = time.perf_counter()
-    tt = oracle.modelc_run(g, p, phi, u, 1, C.dt, seed=C.seed, wiener=C.noise > 0, nthreads=threads)
-    first = time.perf_counter() - t
-    steps = int(max(1, min(50, budget_s // max(first, 1e-3))))
-    t = time.perf_counter()
-    oracle.modelc_run(g, p, phi, u, steps, C.dt, t0=tt, seed=C.seed, wiener=C.noise > 0, nthreads=threads)
-    el = time.perf_counter() - t
+    runs = {}
+    for name, path in builds:
+        oracle.load(path)
+        phi = np.zeros(n)
+        u = np.zeros(n)
+        oracle.initialize_grains(g, phi, u, C.grains(), C.delta)
+        t = time.perf_counter()
+        tt = oracle.modelc_run(g, p, phi, u, 1, C.dt, seed=C.seed, wiener=C.noise > 0, nthreads=threads)
+        first = time.perf_counter() - t
+        steps = int(max(1, min(50, (budget_s / len(builds)) // max(first, 1e-3))))
+        t = time.perf_counter()
+        oracle.modelc_run(g, p, phi, u, steps, C.dt, t0=tt, seed=C.seed, wiener=C.noise > 0, nthreads=threads)
+        el = time.perf_counter() - t
+        runs[name] = (C.interior_cells * steps / el / 1e9, steps, el)
+        log(f"[bench] cpu baseline, {name} build: {runs[name][0]:.4f} Gcell-steps/s ({steps} steps, {el:.1f}s)")
+    best = max(runs, key=lambda k: runs[k][0])
+    value, steps, el = runs[best]
+    others = ", ".join(f"{k}: {v[0]:.3f}" for k, v in runs.items() if k != best)
     return {
-        "value": C.interior_cells * steps / el / 1e9,
+        "value": value,
         "unit": "Gcell-steps/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"oracle (C++ restatement of SerialScheduler semantics, OpenMP over blocks) on the "
-                  f"same {C.cells}^2 Model C state, {steps} steps after 1 warm-up step, "
-                  f"{el:.1f}s, {threads} threads",
+        "cpu_model": cpu_model(),
+        "sample": f"oracle (C++ restatement of SerialScheduler semantics, OpenMP over blocks, -O3 "
+                  f"-march={best}) on the same {C.cells}^2 Model C state, {steps} steps after 1 "
+                  f"warm-up step, {el:.1f}s, {threads} threads on {cpu_model()}"
+                  + (f"; other build {others} Gcell-steps/s" if others else ""),
     }
 
 
-def run_adjoint(args) -> int:
+def run_adjoint(args) -> dict:
     """Calibration gradient (the adjoint workload): L = ½‖φ_T − φ_target‖² after
     2000 steps on 1024² (16×16 blocks of 64²), dL/d(λ, D, ε₄, φ₀, u₀) by the device
     adjoint with binomial checkpointing, timed against one plain forward run of the
@@ -268,7 +390,7 @@ def run_adjoint(args) -> int:
         tlm = {
             "rel_err": [float(abs(adj[k] - dl[k]) / abs(dl[k])) for k in range(4)],
             "directions": ["lambda", "D", "eps4", "u0 (random field)"],
-            "cores": int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0)))),
+            "cores": cpu_threads(),
             "seconds": round(time.perf_counter() - t, 1),
         }
         log(f"[adjoint] tangent-linear check: {tlm}")
@@ -315,6 +437,9 @@ def run_adjoint(args) -> int:
             "target_theta": [A.target_lambda, A.target_d, A.target_eps4],
             "seeds": A.nseeds,
         },
+        "roofline": roofline(32.0 * A.interior_cells * nfwd + 48.0 * A.interior_cells * nrev, t_grad,
+                             "the gradient (forward steps and adj_stream_kernel reverse steps)",
+                             "32 B per forward cell-step, 48 B per reverse cell-step"),
         "adjoint": {
             "forward_ms": round(t_fwd * 1e3, 3),
             "gradient_ms": round(t_grad * 1e3, 3),
@@ -330,11 +455,10 @@ def run_adjoint(args) -> int:
             "tlm": tlm,
         },
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
-def run_ensemble(args) -> int:
+def run_ensemble(args) -> dict:
     """Batched stochastic ensemble: 512 Model C realizations of uneven sizes
     (blocks of 64²) advanced by one launch per step over every block of every
     realization, against one launch per block; solid-fraction statistics checked
@@ -408,7 +532,7 @@ def run_ensemble(args) -> int:
             phi, u = np.zeros(n), np.zeros(n)
             oracle.initialize_grains(g, phi, u, E.grains(m["side"]), m["delta"])
             oracle.modelc_run(g, p, phi, u, E.steps, E.dt, seed=m["seed"], wiener=True,
-                              nthreads=min(16, len(os.sched_getaffinity(0))))
+                              nthreads=cpu_threads())
             b = E.block + 2
             inner = phi.reshape(-1, b, b)[:, 1:-1, 1:-1]
             ref.append(float(np.count_nonzero(inner > 0.0)) / m["side"] ** 2)
@@ -449,6 +573,8 @@ def run_ensemble(args) -> int:
             "blocks": int(sum((m["side"] // E.block) ** 2 for m in members)),
             "noise_b": "U[0.005,0.02]",
         },
+        "roofline": roofline(32.0 * cells, t_batched, "modelc_stream_kernel (block pairs, device domain table)",
+                             "32 B per cell-step"),
         "ensemble": {
             "batched_ms_per_step": round(t_batched * 1e3, 4),
             "per_block_ms_per_step": round(t_per_block * 1e3, 4),
@@ -459,8 +585,7 @@ def run_ensemble(args) -> int:
             "oracle": cpu,
         },
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 def run_jump(args) -> int:
@@ -518,9 +643,8 @@ def run_jump(args) -> int:
     achieved = bytes_per_launch / (avg_ms / 1e3) / 1e9
     cpu = None
     if not args.no_cpu:
-        import oracle
-
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        oracle = native_oracle()
+        threads = cpu_threads()
         sample = 1 << 22
         g = oracle.grid([sample], [block], [0.0], [1.0])
         z = [np.zeros(sample) for _ in range(3)]
@@ -534,7 +658,9 @@ def run_jump(args) -> int:
                                   nthreads=threads)
         el = time.perf_counter() - tt
         cpu = {"value": sample * n / el / 1e9, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
-               "sample": f"oracle jump-diffusion, {sample} paths x {n} steps, {el:.1f}s, {threads} threads"}
+               "cpu_model": cpu_model(),
+               "sample": f"oracle jump-diffusion (-march=native), {sample} paths x {n} steps, {el:.1f}s, "
+                         f"{threads} threads"}
     line = {
         "metric": METRIC,
         "value": round(paths * steps / (total_ms / 1e3) / 1e9, 4),
@@ -703,16 +829,17 @@ def run_heat3d(args) -> int:
     avg = sum(per) / len(per)
     cpu = None
     if not args.no_cpu:
-        import oracle
-
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+        oracle = native_oracle()
+        threads = cpu_threads()
         og = oracle.grid(cells, block, [0.0] * 3, [C.dx] * 3, [True] * 3)
         u = u0.copy()
         t0 = time.perf_counter()
         oracle.heat_run(og, C.kappa, u, steps, C.dt, nthreads=threads)
         el = time.perf_counter() - t0
         cpu = {"value": n * steps / el / 1e9, "unit": "Gcell-steps/s", "cores": threads, "kind": "port",
-               "sample": f"oracle heat run, the whole 128^3 workload ({steps} steps), {el:.2f}s, {threads} threads"}
+               "cpu_model": cpu_model(),
+               "sample": f"oracle heat run (-march=native), the whole 128^3 workload ({steps} steps), "
+                         f"{el:.2f}s, {threads} threads"}
     line = {
         "metric": METRIC,
         "value": round(n * steps / (total / 1e3) / 1e9, 4),
@@ -738,7 +865,7 @@ def run_heat3d(args) -> int:
     return 0
 
 
-def run_amr(args) -> int:
+def run_amr(args) -> dict:
     """Batched AMR: subcycled Model C on a 3-level Berger–Oliger hierarchy with
     regrids, every patch of a level advanced by one launch per stage, against one
     launch per patch on the same schedule.  Throughput counts cell updates:
@@ -841,6 +968,8 @@ def run_amr(args) -> int:
             "seeds": A.nseeds,
             "noise_b": 0.0,
         },
+        "roofline": roofline(32.0 * sum(updates), total, "every level stage, fill, restriction and regrid",
+                             "32 B per cell update"),
         "amr": {
             "cell_updates": int(sum(updates)),
             "seconds": round(total, 3),
@@ -860,8 +989,7 @@ def run_amr(args) -> int:
             "same_hierarchy": same,
         },
     }
-    print(json.dumps(line), flush=True)
-    return 0
+    return line
 
 
 def main() -> int:
@@ -900,14 +1028,14 @@ def main() -> int:
     ap.add_argument("--prefetch", type=int, default=0, choices=[0, 1, 2],
                     help="rows the streaming kernel loads ahead (0 = default)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the reduced batched-AMR, ensemble and adjoint legs of the default run")
     ap.add_argument("--integrator", choices=["euler", "rk4"], default="euler",
                     help="rk4: RungeKutta4 (deterministic runs), four stages of whole-slab sweeps")
     ap.add_argument("--oriented", action="store_true",
                     help="multi-grain Model C (per-grain θ₀, the per-cell oriented kernel)")
     ap.add_argument("--autotune", action="store_true",
                     help="time every launch shape of the Model C step first and use the fastest")
-    ap.add_argument("--calibrate", action="store_true",
-                    help="also run one axpy of known traffic (for profiler byte-counter scaling)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -916,12 +1044,10 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    if args.workload == "adjoint":
-        return run_adjoint(args)
-    if args.workload == "ensemble":
-        return run_ensemble(args)
-    if args.workload == "amr":
-        return run_amr(args)
+    if args.workload in ("adjoint", "ensemble", "amr"):
+        fn = {"adjoint": run_adjoint, "ensemble": run_ensemble, "amr": run_amr}[args.workload]
+        print(json.dumps(fn(args)), flush=True)
+        return 0
     if args.workload == "jump":
         return run_jump(args)
     if args.workload == "observe":
@@ -932,12 +1058,22 @@ def main() -> int:
     from legendre_amd.workloads import ModelCConfig
 
     C = ModelCConfig(cells=args.cells, block=args.block, noise=args.noise)
-    total_ms, per, wall = run_gpu(args, C)
+    total_ms, per, wall, ceiling = run_gpu(args, C)
     ms_per_step = total_ms / args.steps
     value = C.interior_cells * args.steps / (total_ms / 1e3) / 1e9
     avg_launch_ms = sum(per) / len(per)
-    bytes_per_launch = BYTES_PER_CELL_STEP * C.interior_cells
-    achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9
+    # Field passes per step of the fused kernels (8 B per cell each): the Euler step
+    # reads φ, u and writes them (4); the multi-grain cell also reads cos θ₀ and
+    # sin θ₀ (6); the four RK4 stages move 16 passes per field (32), 40 oriented.
+    # The per-intent mode runs many kernels per step: no single-kernel figure.
+    passes = None
+    if args.mode == "fused":
+        if args.integrator == "rk4":
+            passes = 40 if args.oriented else 32
+        else:
+            passes = 6 if args.oriented else 4
+    bytes_per_launch = passes * 8 * C.interior_cells if passes else None
+    achieved = bytes_per_launch / (avg_launch_ms / 1e3) / 1e9 if passes else None
     log(f"[bench] {args.steps} steps: {total_ms:.3f} ms on-stream, {wall * 1e3:.3f} ms wall; "
         f"per launch min {min(per):.4f} max {max(per):.4f} ms")
     traffic = read_traffic(os.path.join(ROOT, "profiles", "modelc_stream_kernel_10080sq.md"))
@@ -984,7 +1120,7 @@ def main() -> int:
             "integrator": "euler_maruyama" if args.integrator == "euler" else args.integrator,
             "parallelism": "single_gpu",
         },
-        "roofline": {
+        "roofline": None if passes is None else {
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -992,12 +1128,20 @@ def main() -> int:
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "kernel": ("modelc_step_kernel" if args.kernel == "percell" or args.oriented
-                       else "modelc_stream_kernel"),
+                       else "modelc_stream_kernel")
+                      + (" (four RK4 stage launches per step)" if args.integrator == "rk4" else ""),
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": round(avg_launch_ms, 5),
+            "ceiling": ceiling,
+            "frac_of_ceiling": round(achieved / ceiling["gbs"], 4),
         },
         "cpu_baseline": cpu,
     }
+    default_variant = (args.cells == 10080 and args.block == 126 and args.noise == 0.0 and args.mode == "fused"
+                       and args.integrator == "euler" and not args.oriented and args.kernel == "auto"
+                       and not args.strips and not args.prefetch and not args.graph and not args.autotune)
+    if default_variant and not args.no_extras:
+        line["batched_and_adjoint"] = run_extras(args, value)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -127,11 +127,17 @@ typedef struct {
   int32_t prefetch;
   int32_t candidates;
   double ms;          /* the chosen shape's mean launch time */
-  double default_ms;  /* the untuned default's (streaming, 4 strips, 2 rows ahead) */
+  double default_ms;  /* the untuned default's (streaming, 4 strips, two rows ahead,
+                         one with noise) */
 } lgd_tuning_result;
 int lgd_autotune_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
                              const lgd_buf *phi, const lgd_buf *u, double dt, int32_t wiener,
                              int32_t reps, lgd_tuning_result *out);
+
+/* The device's streaming rate, for rooflines (no reference counterpart: the reference
+ * launches no kernels): y = x over len f64 (equal, even lengths) with 16-byte
+ * nontemporal loads and stores, vec (1, 2, 4 or 8) vectors in flight per lane. */
+int lgd_stream_copy(lgd_ctx *ctx, const lgd_buf *x, lgd_buf *y, int32_t vec);
 
 /* ------------------------------------- Allocator / StorageBackend (storage.rs) */
 /* Allocator::allocate(len): zero-initialised storage of len f64

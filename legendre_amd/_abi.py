@@ -117,6 +117,7 @@ SIGNATURES = {
     "lgd_ctx_set_tuning": (C.c_int, [P, C.c_char_p, I64]),
     "lgd_autotune_modelc_step": (C.c_int, [P, P, C.POINTER(ModelCParamsC), P, P, DBL, I32, I32,
                                            C.POINTER(TuningResultC)]),
+    "lgd_stream_copy": (C.c_int, [P, P, P, I32]),
     "lgd_buf_alloc": (C.c_int, [P, SZ, PP]),
     "lgd_buf_free": (C.c_int, [P]),
     "lgd_buf_len": (SZ, [P]),

@@ -644,7 +644,8 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
     // The base tiling is a Cartesian grid with mirror faces: the streaming kernel
     // reads neighbour interiors itself (no ghost fill) — the same values and order.
     const int64_t nb0 = g.level_start[1];
-    const lgd_launch_shape shape = lgd_modelc_shape(ctx, g.base.n[0], g.base.n[1], nb0, noise, lgd_launch_shape{});
+    const lgd_launch_shape shape = lgd_modelc_shape(ctx, g.base.n[0], g.base.n[1], nb0, noise,
+                                                    lgd_modelc_default_shape(noise));
     lgd_domain d0;
     d0.k = k;
     d0.noise_scale = noise ? scale : 0.0;

@@ -97,7 +97,7 @@ int lgd_autotune_modelc_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_model
   hipError_t e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e != hipSuccess) rc = hip_fail(ctx, e, "autotune events");
-  const lgd_launch_shape dflt{};
+  const lgd_launch_shape dflt = lgd_modelc_default_shape(noise);
   double best = 1e300, dflt_ms = -1.0;
   lgd_launch_shape pick = dflt;
   for (const lgd_launch_shape &s : cand) {

@@ -104,12 +104,9 @@ __global__ __launch_bounds__(kTX *kTY) void modelc_step_kernel(
 }
 
 // One stage of RungeKutta4::step on Model C (src/integrators/rk4.rs:30-80) in one pass,
-// reading the stage input's halos in place. Stages 1-3 write k_s and the next stage's
-// input y + c_s·k_s (the reference's copy then axpy). Stage 4 evaluates k4 and writes
-// the new state ((((y + w1·k1) + w2·k2) + w3·k3) + w4·k4), the reference's four
-// whole-slab axpys in their order, into y in place (y is read only at the cell itself).
-// One RK4 stage.  The four axpys y + w1·k1 + w2·k2 + w3·k3 + w4·k4 (w = dt/6, dt/3,
-// dt/3, dt/6) are summed as the stages go, in their order, in a running sum A:
+// reading the stage input's halos in place.  The four axpys y + w1·k1 + w2·k2 +
+// w3·k3 + w4·k4 (w = dt/6, dt/3, dt/3, dt/6) are summed as the stages go, in their
+// order, in a running sum A:
 // stage 1 writes A = y + w1·k1, stages 2-3 add w_s·k_s, stage 4 writes
 // y = A + w4·k4.  That is the reference's fold, operation for operation, without
 // keeping k1..k3.
@@ -269,10 +266,8 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
   const bool noise_on = wiener && params->noise_amplitude != 0.0;
   const double noise_scale = std::sqrt(dt) * params->noise_amplitude;
   const uint64_t wiener0 = (1ull << 32) | 0ull;  // Driver::Wiener(0).stream()
-  // Untuned default: 4 strips; two rows ahead, or one with noise (VALU-bound: the
-  // registers of the second row are worth more as headroom).
   const lgd_launch_shape shape = lgd_modelc_shape(ctx, c.n[0], c.n[1], c.nblocks, noise_on,
-                                                  lgd_launch_shape{2, 4, noise_on ? 1 : 2});
+                                                  lgd_modelc_default_shape(noise_on));
   const bool oriented = cs0 != nullptr;
   if (shape.kernel != 1 && !oriented) {
     lgd_domain d0;
@@ -319,8 +314,9 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
   return launch_check(ctx, "modelc_step");
 }
 
-// One RungeKutta4 step of Model C as four fused stage launches. y = (phi, u); k[0..2]
-// hold k1..k3 (per field); t0, t1 alternate as stage inputs. cs0 / sn0: orientations.
+// One RungeKutta4 step of Model C as four fused stage launches. y = (phi, u);
+// kbuf[0..1] hold the running sum A of the four axpys (φ, u); t0, t1 alternate as
+// stage inputs. cs0 / sn0: orientations.
 int lgd_modelc_rk4_step(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_params *params,
                         lgd_buf *phi, lgd_buf *u, lgd_buf *const *kbuf, lgd_buf *const *tbuf,
                         double dt, const double *cs0, const double *sn0) {

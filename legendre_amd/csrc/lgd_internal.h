@@ -48,6 +48,12 @@ struct lgd_ctx {
 // shape for this key, then `dflt`.
 lgd_launch_shape lgd_modelc_shape(const lgd_ctx *ctx, int64_t nx, int64_t ny, int64_t nblocks,
                                   bool noise, const lgd_launch_shape &dflt);
+// The untuned shape of the single-grid Model C step: streaming, 4 strips per block,
+// two rows loaded ahead, or one with noise (VALU-bound: the registers of the second
+// row are worth more as headroom).
+inline lgd_launch_shape lgd_modelc_default_shape(bool noise) {
+  return lgd_launch_shape{2, 4, noise ? 1 : 2};
+}
 
 struct lgd_buf {
   lgd_ctx *ctx = nullptr;

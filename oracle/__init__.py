@@ -115,17 +115,31 @@ def build() -> pathlib.Path:
     return LIB_PATH
 
 
-def lib() -> C.CDLL:
+def build_native(outdir: str) -> pathlib.Path:
+    """The same sources built for this host's CPU (-march=native; still
+    -ffp-contract=off, so the same bits) into outdir, for timing the CPU baseline on
+    the machine that runs it."""
+    out = pathlib.Path(outdir) / "liblegendre_oracle.so"
+    subprocess.run(["make", "-s", "-C", str(HERE), "native", f"NATIVE_OUT={out}"], check=True)
+    return out
+
+
+def load(path) -> None:
+    """Use the oracle library at `path` for every later call (before or after lib())."""
     global _lib
+    h = C.CDLL(str(path))
+    for name, (res, args) in _SIGS.items():
+        f = getattr(h, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = h
+
+
+def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             build()
-        h = C.CDLL(str(LIB_PATH))
-        for name, (res, args) in _SIGS.items():
-            f = getattr(h, name)
-            f.restype = res
-            f.argtypes = args
-        _lib = h
+        load(LIB_PATH)
     return _lib
 
 
