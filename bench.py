@@ -877,6 +877,7 @@ def run_amr(args) -> dict:
     A = AmrConfig(coarse_steps=args.amr_steps or AmrConfig.coarse_steps)
     ctx = lg.Context(0)
     ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2, "pair": 3}[args.kernel])
+    ctx.set_tuning("amr_split_fill", getattr(args, "amr_split_fill", 0))
     params = lg.ModelCParams(A.eps4, A.lambda_, A.d_thermal)
     pol = lg.RegridPolicy(A.every, A.buffer, A.efficiency, A.min_width, A.threshold, 0)
     grid = lg.CartesianGrid(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])
@@ -1000,6 +1001,8 @@ def main() -> int:
                          "ensemble: 512 batched realizations; amr: 3-level subcycled hierarchy")
     ap.add_argument("--jump-steps", type=int, default=0, help="jump workload steps (0 = 100)")
     ap.add_argument("--amr-steps", type=int, default=0, help="AMR coarse steps (0 = 500)")
+    ap.add_argument("--amr-split-fill", type=int, default=0, choices=[0, 1],
+                    help="1: the finest level's ghost fills in launches of their own")
     ap.add_argument("--per-block-chunks", type=int, default=2,
                     help="regrid intervals timed with one launch per patch (amr; 0 skips the baseline)")
     ap.add_argument("--ens-steps", type=int, default=0, help="ensemble steps (0 = 2000)")

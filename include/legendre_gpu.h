@@ -104,7 +104,9 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
  * "modelc_strips" (row strips per block for the streaming kernel, 0 = default) and
  * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default) and
  * "amr_per_block" (1: the AMR level kernels launch once per patch instead of once per
- * level — the per-block dispatch baseline).
+ * level — the per-block dispatch baseline) and "amr_split_fill" (1: every substep of the
+ * finest level fills its ghosts in a launch of its own; 0, the default: a substep's
+ * stage launch also fills the ghosts the next substep reads).
  * Results are bitwise independent of these knobs.  The adjoint has two more:
  * "adjoint_kernel" (0: the streaming reverse step wherever the grid allows, else the
  * three-pass form; 1: always the three passes; 2: streaming, error if the grid does

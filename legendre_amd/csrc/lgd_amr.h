@@ -110,6 +110,7 @@ struct lgd_amr {
   void *d_arena = nullptr;  // one device allocation: d_blocks and the patch index
   lgd_block_desc *d_blocks = nullptr;
   std::vector<lgd_level_plan> plans;  // per populated level
+  std::vector<std::vector<char>> staging;  // host bytes of the device arrays' uploads
   // find_patch on the device: per-level headers, concatenated bucket starts (into
   // the concatenated ids), patch ids, and each patch's lower corner (x, y).
   lgd_dev_level *d_levels = nullptr;
@@ -154,11 +155,21 @@ int lgd_amr_restrict_pp(lgd_ctx *ctx, const lgd_amr &g, int fine_level, int nf,
 // Whether level 0 steps through the Cartesian streaming kernel (which reads
 // neighbour interiors itself, so level 0 needs no ghost fill).
 bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g);
-// One fused Euler–Maruyama stage of `level`: (phi, u) → (phi_n, u_n) interiors.
+// The ghosts of the next substep of a level without children, written by the stage
+// launch that produces that substep's input: the parent's current and previous
+// buffers (fields 0, 1) and the next substep's α.  Bitwise the separate fill.
+struct lgd_amr_next_fill {
+  const double *par[2];
+  const double *old[2];
+  double alpha;
+};
+// One fused Euler–Maruyama stage of `level`: (phi, u) → (phi_n, u_n) interiors, and
+// with `nf` the ghosts of (phi_n, u_n) the level's next substep reads.
 int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
                         double dt, const double *phi, const double *u, double *phi_n, double *u_n,
-                        uint64_t seed, uint64_t salt, bool wiener);
+                        uint64_t seed, uint64_t salt, bool wiener,
+                        const lgd_amr_next_fill *nf = nullptr);
 int lgd_amr_heat_step(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, double dt,
-                      const double *u, double *u_n);
+                      const double *u, double *u_n, const lgd_amr_next_fill *nf = nullptr);
 // Copy one level's range (interiors and ghosts) between buffers of the layout.
 int lgd_amr_copy_level(lgd_ctx *ctx, const lgd_amr &g, int level, const double *src, double *dst);

@@ -56,18 +56,22 @@ struct Packer {
     if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
     parts.push_back({off, (void **)dptr});
   }
-  int upload(lgd_ctx *ctx, void **base) {
+  // One stream-ordered allocation from the device's memory pool and one copy on the
+  // context's stream, so a regrid neither synchronises the device nor pays for a
+  // fresh allocation; the host bytes move into `keep`, which outlives the copy.
+  int upload(lgd_ctx *ctx, void **base, std::vector<std::vector<char>> &keep) {
     *base = nullptr;
     for (auto &p : parts) *p.second = nullptr;
     if (host.empty()) return LGD_OK;
-    hipError_t e = hipMalloc(base, host.size());
+    hipError_t e = hipMallocAsync(base, host.size(), ctx->stream);
     if (e != hipSuccess) {
       *base = nullptr;
-      return hip_fail(ctx, e, "hipMalloc(amr plan)");
+      return hip_fail(ctx, e, "hipMallocAsync(amr plan)");
     }
-    e = hipMemcpy(*base, host.data(), host.size(), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(*base, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "upload(amr plan)");
     for (auto &p : parts) *p.second = (char *)*base + p.first;
+    keep.push_back(std::move(host));
     return LGD_OK;
   }
 };
@@ -119,15 +123,19 @@ int64_t lgd_amr::index(int64_t b, int64_t i, int64_t j) const {
 }
 
 void lgd_amr::free_device() {
-  (void)hipFree(d_arena);
+  // Returned to the pool in stream order, after the kernels that read them.
+  hipStream_t st = ctx ? ctx->stream : nullptr;
+  if (d_arena) (void)hipFreeAsync(d_arena, st);
   d_arena = nullptr;
   d_blocks = nullptr;
   d_levels = nullptr;
   d_bstart = nullptr;
   d_bids = nullptr;
   d_lo = nullptr;
-  for (auto &p : plans) (void)hipFree(p.block);
+  for (auto &p : plans)
+    if (p.block) (void)hipFreeAsync(p.block, st);
   plans.clear();
+  staging.clear();
 }
 
 // AmrGrid::from_patches (src/geometry/amr/grid.rs:90-196): level 0 is the base
@@ -312,7 +320,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     pk.add(bstart, &g.d_bstart);
     pk.add(bids, &g.d_bids);
     pk.add(lo, &g.d_lo);
-    if ((rc = pk.upload(ctx, &g.d_arena))) return rc;
+    if ((rc = pk.upload(ctx, &g.d_arena, g.staging))) return rc;
   }
   g.plans.assign((size_t)g.num_levels(), lgd_level_plan{});
   for (int l = 0; l < g.num_levels(); ++l) {
@@ -342,7 +350,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
         int64_t src = g.find_patch(l, tx, ty);
         if (src >= 0) {
           op.src = g.index(src, tx - g.box[src].lo[0], ty - g.box[src].lo[1]);
-          op.cstart = 0;
+          op.cstart = (int32_t)src;  // the source patch (a fused stage recomputes the cell)
           op.clen = 0;
         } else {
           op.src = -1;
@@ -413,7 +421,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     pk.add(ops, &P.ops);
     pk.add(corners, &P.corners);
     pk.add(items, &P.ritems);
-    if ((rc = pk.upload(ctx, &P.block))) return rc;
+    if ((rc = pk.upload(ctx, &P.block, g.staging))) return rc;
   }
   return LGD_OK;
 }
@@ -516,53 +524,139 @@ __global__ void amr_restrict_pp_kernel(const lgd_restrict_item *__restrict__ ite
   }
 }
 
-// Fused Model C stage on one level's patch interiors, from ghost-filled slabs, in
-// the reference's order: the vector field (ModelC::vector_field_block on AmrGrid,
-// phasefield.rs:452-475), the Time update x + dt·k, then the Wiener(0) increment
-// (src/core/driver.rs:139-158) with the cell key per patch extent
-// (grid.rs:407-425).
+// Fused Model C stage at one interior cell (element o of block b, interior (i, j)),
+// from ghost-filled slabs, in the reference's order: the vector field
+// (ModelC::vector_field_block on AmrGrid, phasefield.rs:452-475), the Time update
+// x + dt·k, then the Wiener(0) increment (src/core/driver.rs:139-158) with the cell
+// key per patch extent (grid.rs:407-425).
 template <bool kNoise>
-__global__ __launch_bounds__(kTX *kTY) void amr_modelc_step_kernel(
-    const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks,
-    lgd_kr_consts k, double dt, const double *phi, const double *u, double *phi_n, double *u_n,
-    double noise_scale, uint64_t seed, uint64_t salt, const double *salt_src) {
-  const lgd_tile tl = tiles[blockIdx.x];
-  const lgd_block_desc bd = blocks[tl.block];
-  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
-  if (i >= bd.nx || j >= bd.ny) return;
-  const int64_t sx = bd.nx + 2;
-  const int64_t o = bd.off + (int64_t)(j + 1) * sx + (i + 1);
+__device__ __forceinline__ void modelc_cell_stage(const lgd_kr_consts &k, double dt,
+                                                  const double *phi, const double *u, int64_t o,
+                                                  int64_t sx, double noise_scale, uint64_t seed,
+                                                  uint64_t salt, int64_t b, int64_t cell,
+                                                  double &pn, double &un) {
   double p[3][3];
   for (int dj = -1; dj <= 1; ++dj)
     for (int di = -1; di <= 1; ++di) p[dj + 1][di + 1] = phi[o + dj * sx + di];
   double dphi, du;
   lgd_modelc_cell(k, p, u[o], u[o + 1], u[o - 1], u[o + sx], u[o - sx], dphi, du);
-  double pn = p[1][1] + dt * dphi;
+  pn = p[1][1] + dt * dphi;
   if (kNoise) {
-    uint64_t bk = lgd_mix_key3(seed, lgd_salt(salt, salt_src), (1ull << 32) | 0ull, (uint64_t)tl.block);
-    uint64_t key = lgd_splitmix64(bk ^ (uint64_t)(i + (int64_t)j * bd.nx));
+    uint64_t bk = lgd_mix_key3(seed, salt, (1ull << 32) | 0ull, (uint64_t)b);
+    uint64_t key = lgd_splitmix64(bk ^ (uint64_t)cell);
     pn = pn + noise_scale * lgd_standard_normal(key);
   }
-  phi_n[o] = pn;
-  u_n[o] = u[o] + dt * du;
+  un = u[o] + dt * du;
 }
 
-// Fused Heat forward-Euler stage on one level's patch interiors (κ·∇²u, the Heat
-// model of tests/amr.rs, then u + dt·k).
-__global__ __launch_bounds__(kTX *kTY) void amr_heat_step_kernel(
-    const lgd_tile *__restrict__ tiles, const lgd_block_desc *__restrict__ blocks, double ihx,
-    double ihy, double kappa, double dt, const double *u, double *u_n) {
-  const lgd_tile tl = tiles[blockIdx.x];
-  const lgd_block_desc bd = blocks[tl.block];
-  const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
-  if (i >= bd.nx || j >= bd.ny) return;
-  const int64_t sx = bd.nx + 2;
-  const int64_t o = bd.off + (int64_t)(j + 1) * sx + (i + 1);
+// Fused Heat forward-Euler stage at one interior cell (κ·∇²u, the Heat model of
+// tests/amr.rs, then u + dt·k).
+__device__ __forceinline__ double heat_cell_stage(double ihx, double ihy, double kappa, double dt,
+                                                  const double *u, int64_t o, int64_t sx) {
   const double c = u[o];
   double acc = 0.0;
   acc = acc + (((u[o + 1] - (c + c)) + u[o - 1]) * ihx);
   acc = acc + (((u[o + sx] - (c + c)) + u[o - sx]) * ihy);
-  u_n[o] = c + dt * (acc * kappa);
+  return c + dt * (acc * kappa);
+}
+
+// The ghost ring the next substep of the finest level reads, written by the stage
+// launch that produces the substep's input (the fill of intergrid.rs:183-223, one
+// launch earlier): a copy ghost takes its source cell's new value, which the
+// ghost's thread computes itself from the stage's input (the same operations and
+// noise key as the thread that owns the cell, so the same bits); a prolonged ghost
+// blends the parent's previous and current values at the next substep's α, which
+// no substep of this level changes.
+struct NextFill {
+  const lgd_fill_op *ops;
+  const lgd_corner *corners;
+  int64_t nops;
+  const double *par[2];
+  const double *old[2];
+  double one_minus_alpha, alpha;
+};
+
+__device__ __forceinline__ double prolong_blend(const NextFill &N, const lgd_fill_op &op, int f) {
+  double vn = 0.0, vo = 0.0;
+  for (int c = 0; c < op.clen; ++c) {
+    const lgd_corner kc = N.corners[op.cstart + c];
+    vn = vn + N.par[f][kc.src] * kc.weight;
+    vo = vo + N.old[f][kc.src] * kc.weight;
+  }
+  return vo * N.one_minus_alpha + vn * N.alpha;
+}
+
+// The element's interior coordinates in its block (o is an interior element of b).
+__device__ __forceinline__ void slab_coords(const lgd_block_desc &bd, int64_t o, int &i, int &j) {
+  const int64_t local = o - bd.off, sx = bd.nx + 2;
+  j = (int)(local / sx) - 1;
+  i = (int)(local - (local / sx) * sx) - 1;
+}
+
+// One fused Model C stage of a level: workgroups [0, ntiles) advance the tiles'
+// cells; with kFill, workgroups from ntiles on write the next substep's ghosts.
+template <bool kNoise, bool kFill>
+__global__ __launch_bounds__(kTX *kTY) void amr_modelc_step_kernel(
+    const lgd_tile *__restrict__ tiles, int64_t ntiles, const lgd_block_desc *__restrict__ blocks,
+    lgd_kr_consts k, double dt, const double *phi, const double *u, double *phi_n, double *u_n,
+    double noise_scale, uint64_t seed, uint64_t salt, const double *salt_src, NextFill N) {
+  const uint64_t st = kNoise ? lgd_salt(salt, salt_src) : 0;
+  if (!kFill || blockIdx.x < ntiles) {
+    const lgd_tile tl = tiles[blockIdx.x];
+    const lgd_block_desc bd = blocks[tl.block];
+    const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
+    if (i >= bd.nx || j >= bd.ny) return;
+    const int64_t sx = bd.nx + 2;
+    const int64_t o = bd.off + (int64_t)(j + 1) * sx + (i + 1);
+    double pn, un;
+    modelc_cell_stage<kNoise>(k, dt, phi, u, o, sx, noise_scale, seed, st, tl.block,
+                              i + (int64_t)j * bd.nx, pn, un);
+    phi_n[o] = pn;
+    u_n[o] = un;
+    return;
+  }
+  const int64_t t = (int64_t)(blockIdx.x - ntiles) * (kTX * kTY) + threadIdx.y * kTX + threadIdx.x;
+  if (t >= N.nops) return;
+  const lgd_fill_op op = N.ops[t];
+  if (op.clen == 0) {
+    const lgd_block_desc bd = blocks[op.cstart];
+    int i, j;
+    slab_coords(bd, op.src, i, j);
+    double pn, un;
+    modelc_cell_stage<kNoise>(k, dt, phi, u, op.src, bd.nx + 2, noise_scale, seed, st, op.cstart,
+                              i + (int64_t)j * bd.nx, pn, un);
+    phi_n[op.target] = pn;
+    u_n[op.target] = un;
+  } else {
+    phi_n[op.target] = prolong_blend(N, op, 0);
+    u_n[op.target] = prolong_blend(N, op, 1);
+  }
+}
+
+// One fused Heat stage of a level, with the next substep's ghosts as above.
+template <bool kFill>
+__global__ __launch_bounds__(kTX *kTY) void amr_heat_step_kernel(
+    const lgd_tile *__restrict__ tiles, int64_t ntiles, const lgd_block_desc *__restrict__ blocks,
+    double ihx, double ihy, double kappa, double dt, const double *u, double *u_n, NextFill N) {
+  if (!kFill || blockIdx.x < ntiles) {
+    const lgd_tile tl = tiles[blockIdx.x];
+    const lgd_block_desc bd = blocks[tl.block];
+    const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
+    if (i >= bd.nx || j >= bd.ny) return;
+    const int64_t sx = bd.nx + 2;
+    const int64_t o = bd.off + (int64_t)(j + 1) * sx + (i + 1);
+    u_n[o] = heat_cell_stage(ihx, ihy, kappa, dt, u, o, sx);
+    return;
+  }
+  const int64_t t = (int64_t)(blockIdx.x - ntiles) * (kTX * kTY) + threadIdx.y * kTX + threadIdx.x;
+  if (t >= N.nops) return;
+  const lgd_fill_op op = N.ops[t];
+  if (op.clen == 0) {
+    const lgd_block_desc bd = blocks[op.cstart];
+    u_n[op.target] = heat_cell_stage(ihx, ihy, kappa, dt, u, op.src, bd.nx + 2);
+  } else {
+    u_n[op.target] = prolong_blend(N, op, 0);
+  }
 }
 
 }  // namespace
@@ -630,9 +724,29 @@ bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g) {
   return ctx->modelc_kernel != 1 && g.base.n[0] + 2 <= 128;
 }
 
+namespace {
+
+// The next-fill arguments of a stage launch (nops = 0: none).
+NextFill next_fill_args(const lgd_level_plan &P, const lgd_amr_next_fill *nf, int nfields) {
+  NextFill N{};
+  if (!nf) return N;
+  N.ops = P.ops;
+  N.corners = P.corners;
+  N.nops = P.nops;
+  for (int f = 0; f < nfields; ++f) {
+    N.par[f] = nf->par[f];
+    N.old[f] = nf->old[f];
+  }
+  N.one_minus_alpha = 1.0 - nf->alpha;
+  N.alpha = nf->alpha;
+  return N;
+}
+
+}  // namespace
+
 int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
                         double dt, const double *phi, const double *u, double *phi_n, double *u_n,
-                        uint64_t seed, uint64_t salt, bool wiener) {
+                        uint64_t seed, uint64_t salt, bool wiener, const lgd_amr_next_fill *nf) {
   const lgd_level_plan &P = g.plans[(size_t)level];
   lgd_cart c = g.base;
   c.h[0] = g.spacing[(size_t)level][0];
@@ -669,17 +783,29 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
                                     u, phi_n, u_n, dt, noise, salt, wiener0, shape.strips,
                                     shape.prefetch);
   }
-  if (P.ntiles == 0) return LGD_OK;
+  if (P.ntiles == 0 && !(nf && P.nops)) return LGD_OK;
+  const NextFill none{};
   auto launch = [&](const lgd_tile *tiles, int64_t n) {
     if (noise)
-      amr_modelc_step_kernel<true><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
-          tiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, scale, seed, salt, ctx->salt_src);
+      amr_modelc_step_kernel<true, false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+          tiles, n, g.d_blocks, k, dt, phi, u, phi_n, u_n, scale, seed, salt, ctx->salt_src, none);
     else
-      amr_modelc_step_kernel<false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
-          tiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, 0.0, seed, salt, nullptr);
+      amr_modelc_step_kernel<false, false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+          tiles, n, g.d_blocks, k, dt, phi, u, phi_n, u_n, 0.0, seed, salt, nullptr, none);
   };
+  if (ctx->amr_per_block && nf)
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "per-patch launches fill ghosts in a launch of their own");
   if (ctx->amr_per_block) {
     for (size_t b = 0; b + 1 < P.tile0.size(); ++b) launch(P.tiles + P.tile0[b], P.tile0[b + 1] - P.tile0[b]);
+  } else if (nf && P.nops) {
+    const NextFill N = next_fill_args(P, nf, 2);
+    const unsigned nwg = (unsigned)(P.ntiles + (P.nops + kTX * kTY - 1) / (kTX * kTY));
+    if (noise)
+      amr_modelc_step_kernel<true, true><<<nwg, dim3(kTX, kTY), 0, ctx->stream>>>(
+          P.tiles, P.ntiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, scale, seed, salt, ctx->salt_src, N);
+    else
+      amr_modelc_step_kernel<false, true><<<nwg, dim3(kTX, kTY), 0, ctx->stream>>>(
+          P.tiles, P.ntiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, 0.0, seed, salt, nullptr, N);
   } else {
     launch(P.tiles, P.ntiles);
   }
@@ -687,16 +813,25 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
 }
 
 int lgd_amr_heat_step(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, double dt,
-                      const double *u, double *u_n) {
+                      const double *u, double *u_n, const lgd_amr_next_fill *nf) {
   const lgd_level_plan &P = g.plans[(size_t)level];
-  if (P.ntiles == 0) return LGD_OK;
+  if (P.ntiles == 0 && !(nf && P.nops)) return LGD_OK;
   const double hx = g.spacing[(size_t)level][0], hy = g.spacing[(size_t)level][1];
+  const double ihx = 1.0 / (hx * hx), ihy = 1.0 / (hy * hy);
+  const NextFill none{};
   auto launch = [&](const lgd_tile *tiles, int64_t n) {
-    amr_heat_step_kernel<<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
-        tiles, g.d_blocks, 1.0 / (hx * hx), 1.0 / (hy * hy), kappa, dt, u, u_n);
+    amr_heat_step_kernel<false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+        tiles, n, g.d_blocks, ihx, ihy, kappa, dt, u, u_n, none);
   };
+  if (ctx->amr_per_block && nf)
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "per-patch launches fill ghosts in a launch of their own");
   if (ctx->amr_per_block) {
     for (size_t b = 0; b + 1 < P.tile0.size(); ++b) launch(P.tiles + P.tile0[b], P.tile0[b + 1] - P.tile0[b]);
+  } else if (nf && P.nops) {
+    const NextFill N = next_fill_args(P, nf, 1);
+    const unsigned nwg = (unsigned)(P.ntiles + (P.nops + kTX * kTY - 1) / (kTX * kTY));
+    amr_heat_step_kernel<true><<<nwg, dim3(kTX, kTY), 0, ctx->stream>>>(
+        P.tiles, P.ntiles, g.d_blocks, ihx, ihy, kappa, dt, u, u_n, N);
   } else {
     launch(P.tiles, P.ntiles);
   }

@@ -95,6 +95,14 @@ int lgd_ctx_create(int32_t device, void *stream, lgd_ctx **out) {
   int cus = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess)
     ctx->num_cus = cus;
+  // The AMR hierarchy's device arrays come from the device's stream-ordered memory
+  // pool: keep freed blocks in the pool for the next regrid instead of returning
+  // them to the driver at every synchronisation.
+  hipMemPool_t pool = nullptr;
+  if (hipDeviceGetDefaultMemPool(&pool, device) == hipSuccess && pool) {
+    uint64_t keep = UINT64_MAX;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+  }
   *out = ctx;
   return LGD_OK;
 }
@@ -136,6 +144,9 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
   } else if (k == "amr_per_block") {
     if (value > 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_per_block is 0 or 1");
     ctx->amr_per_block = value;
+  } else if (k == "amr_split_fill") {
+    if (value > 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_split_fill is 0 or 1");
+    ctx->amr_split_fill = value;
   } else if (k == "adjoint_kernel") {
     if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "adjoint_kernel is 0, 1 or 2");
     ctx->adjoint_kernel = value;

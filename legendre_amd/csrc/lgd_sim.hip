@@ -372,18 +372,30 @@ int amr_restrict_level(lgd_sim *s, int fine_level) {
 // src/integrators/mod.rs:205-240, src/core/state.rs:706-753): current → other
 // buffer, which becomes current.  `tslot`: the device time slot this substep's
 // noise reads while a graph is captured.
-int amr_level_step(lgd_sim *s, int level, double dt, uint64_t salt, int tslot) {
+// With next_alpha >= 0 the same launch also fills the ghosts of the level's next
+// substep (a level without children; the parent's buffers blended at next_alpha).
+int amr_level_step(lgd_sim *s, int level, double dt, uint64_t salt, int tslot,
+                   double next_alpha = -1.0) {
   const lgd_amr &g = *s->amr;
   int rc;
+  lgd_amr_next_fill nf{};
+  if (next_alpha >= 0.0) {
+    for (int f = 0; f < nfields(s); ++f) {
+      nf.par[f] = amr_cur(s, f, level - 1)->data;
+      nf.old[f] = amr_prev(s, f, level - 1)->data;
+    }
+    nf.alpha = next_alpha;
+  }
+  const lgd_amr_next_fill *nfp = next_alpha >= 0.0 ? &nf : nullptr;
   if (s->kind == lgd_sim::kModelC) {
     s->ctx->salt_src = time_slot(s, tslot);
     rc = lgd_amr_modelc_step(s->ctx, g, s->params, level, dt, amr_cur(s, 0, level)->data,
                              amr_cur(s, 1, level)->data, amr_prev(s, 0, level)->data,
-                             amr_prev(s, 1, level)->data, s->seed, salt, s->wiener != 0);
+                             amr_prev(s, 1, level)->data, s->seed, salt, s->wiener != 0, nfp);
     s->ctx->salt_src = nullptr;
   } else {
     rc = lgd_amr_heat_step(s->ctx, g, s->kappa, level, dt, amr_cur(s, 0, level)->data,
-                           amr_prev(s, 0, level)->data);
+                           amr_prev(s, 0, level)->data, nfp);
   }
   if (rc) return rc;
   s->cur[(size_t)level] ^= 1;
@@ -421,14 +433,26 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
     double n = std::fmax(std::round(amr_stable_dt(s, level) / amr_stable_dt(s, child)), 1.0);
     uint64_t nn = (uint64_t)n;
     double dt_child = dt / (double)nn;
+    // The finest level's substeps after the first take their ghosts from the stage
+    // launch before them (one fill and nn stage launches instead of nn of each).
+    const bool fused = child + 1 == g.num_levels() && !s->ctx->amr_per_block &&
+                       !s->ctx->amr_split_fill && nn > 1;
     for (uint64_t k = 0; k < nn; ++k) {
       if (s->capturing) {
         time_substep_kernel<<<1, 1, 0, s->ctx->stream>>>(s->d_t, child, level, (double)k, dt_child);
         if ((rc = launch_check(s->ctx, "substep time"))) return rc;
       }
-      if ((rc = amr_advance(s, child, std::fma((double)k, dt_child, t), dt_child,
-                            (double)k / (double)nn)))
-        return rc;
+      const double tk = std::fma((double)k, dt_child, t), ak = (double)k / (double)nn;
+      if (!fused) {
+        if ((rc = amr_advance(s, child, tk, dt_child, ak))) return rc;
+        continue;
+      }
+      // Subcycle::advance of the finest level (no recursion below it).
+      if (k == 0 && (rc = amr_fill_level(s, child, true, ak))) return rc;
+      uint64_t salt_k;
+      std::memcpy(&salt_k, &tk, sizeof salt_k);
+      const double next = k + 1 < nn ? (double)(k + 1) / (double)nn : -1.0;
+      if ((rc = amr_level_step(s, child, dt_child, salt_k, child, next))) return rc;
     }
     if ((rc = amr_restrict_level(s, child))) return rc;
   }

@@ -287,3 +287,47 @@ def test_a_patch_steps_like_its_uniform_twin_bitwise(ctx, orc, model):
         want = x.download().reshape(8, 8)[sl]
         assert np.array_equal(got, want), f"field {i}"
     sa.close()
+
+
+@pytest.mark.parametrize("model", ["modelc", "heat"])
+def test_fused_next_fill_matches_split_fill_bitwise(ctx, orc, model):
+    """The finest level's stage launch filling the next substep's ghosts (the
+    default) computes the bits of a separate fill launch per substep: three-level
+    adaptive subcycled runs, Model C with noise and heat, hierarchy and fields after
+    every step."""
+    say(f"amr fused next fill vs split fill ({model})")
+    h, n, blk = 0.4, 96, 24
+    base = orc.grid([n, n], [blk, blk], [0, 0], [h, h])
+    phi = np.zeros(orc.field_len(base, 1))
+    u = np.zeros_like(phi)
+    orc.initialize_grains(base, phi, u, np.array([[10.0, 9.0, 3.0], [22.0, 25.0, 2.0]]), 0.55)
+    pol = lg.RegridPolicy(4, 2, 0.8, 4, 0.1, 0)
+    runs = []
+    try:
+        for split in (0, 1):
+            ctx.set_tuning("amr_split_fill", split)
+            gg = lg.AmrGrid.from_patches(ctx, [n, n], [blk, blk], [0, 0], [h, h], [2, 2], [])
+            if model == "modelc":
+                sim = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.05, 3.19, 2.0, 0.01),
+                                        wiener=True, seed=4, subcycle=True, policy=pol)
+                sim.field(1).upload(u)
+                dt = 0.2 * h * h / 2.0
+            else:
+                sim = lg.amr_simulation(gg, "heat", kappa=0.7, subcycle=True, policy=pol)
+                dt = 0.25 * h * h / 0.7
+            sim.field(0).upload(phi)
+            trace = []
+            for s in range(16):
+                sim.step(dt)
+                a = sim.amr
+                m = H.amr_interior_mask(a)
+                trace.append((patches(a), [sim.field(f).download()[m] for f in range(sim.nfields)]))
+            runs.append(trace)
+            sim.close()
+    finally:
+        ctx.set_tuning("amr_split_fill", 0)
+    assert any(p[0] == 2 for tr in runs[0] for p in tr[0])  # level 2 existed
+    for s, (a, b) in enumerate(zip(*runs)):
+        assert a[0] == b[0], f"step {s}: hierarchies differ"
+        for fa, fb in zip(a[1], b[1]):
+            assert np.array_equal(fa, fb), f"step {s}: fields differ"
