@@ -895,6 +895,16 @@ def run_amr(args) -> dict:
         ctx.sync()
         return sim
 
+    tuning = None
+    if args.autotune:
+        # Tune on a 3-level hierarchy of this run: a copy advanced past its second
+        # regrid (the timed run starts from the initial state as usual).
+        probe = build()
+        probe.step(A.dt, A.every + 1)
+        tuning = probe.autotune_amr(A.dt, reps=5)
+        probe.close()
+        log(f"[amr] autotuned launch shapes: {tuning}")
+
     def level_cells(sim):
         g = sim.amr
         cells = [0] * g.num_levels
@@ -988,6 +998,7 @@ def run_amr(args) -> dict:
                                      "migration": round(float(phases[3]), 4)},
             "speedup_excluding_regrid_steps": round(speedup_steps, 2) if speedup_steps else None,
             "same_hierarchy": same,
+            "autotuned": tuning,
         },
     }
     return line

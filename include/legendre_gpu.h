@@ -104,7 +104,9 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
  * "modelc_strips" (row strips per block for the streaming kernel, 0 = default) and
  * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default) and
  * "amr_per_block" (1: the AMR level kernels launch once per patch instead of once per
- * level — the per-block dispatch baseline) and "amr_split_fill" (1: every substep of the
+ * level — the per-block dispatch baseline), "amr_tile" and "amr_wg" (the AMR launch
+ * shapes lgd_sim_autotune_amr chooses; 0 = 64×4 tiles and 256-wide fills) and
+ * "amr_split_fill" (1: every substep of the
  * finest level fills its ghosts in a launch of its own; 0, the default: a substep's
  * stage launch also fills the ghosts the next substep reads).
  * Results are bitwise independent of these knobs.  The adjoint has two more:
@@ -395,6 +397,28 @@ double lgd_sim_time(const lgd_sim *sim);
  * new hierarchy (patches, plans, uploads), [3] the state migrated. */
 int lgd_sim_regrid_stats(const lgd_sim *sim, int64_t *regrids, double *seconds);
 int64_t lgd_sim_step_index(const lgd_sim *sim);
+/* Autotuner for an AMR simulation's launches (no reference counterpart: the
+ * reference has no launch shapes).  Times, on the simulation's current hierarchy and
+ * state, every level-stage tile shape (the "amr_tile" knob: 64×4, 32×8, 16×16, 128×2,
+ * 64×2, 32×4, 256×1 threads), every fill / restriction workgroup width ("amr_wg": 64
+ * to 1024) and, for level 0 on the streaming kernel, 1, 2, 4 or 8 strips per block
+ * with one or two rows loaded ahead, `reps` launches each, weighted by each level's
+ * substeps per coarse step of `dt`; keeps the fastest in the context.  The timed
+ * launches write only entries the next step overwrites before reading them (the
+ * other buffer of a level, ghost entries), so the simulation's results are bitwise
+ * those it would have had untuned; results are bitwise independent of every shape.
+ * ms / default_ms: the kernel time per coarse step the timings predict, tuned and
+ * untuned (64×4 tiles, 256-wide fills, the untuned level-0 shape). */
+typedef struct {
+  int32_t tile_x, tile_y;
+  int32_t wg;
+  int32_t level0_strips, level0_prefetch; /* 0 when level 0 is not on the streaming kernel */
+  int32_t candidates;
+  double ms;
+  double default_ms;
+} lgd_amr_tuning_result;
+int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_result *out);
+
 /* TagCells (src/geometry/amr/adapt.rs:32-46) as a host callback, in place of the
  * policy's GradientTagger: at each regrid BergerOliger::regrid calls tag_level for
  * every existing source level, finest first, and the callback appends the cells of

@@ -485,6 +485,16 @@ class Simulation:
                    self.ctx.handle)
         return int(n.value), sec
 
+    def autotune_amr(self, dt: float, reps: int = 5) -> dict:
+        """Time the AMR launch shapes on the current hierarchy and state (level-stage
+        tiles, fill / restriction widths, level-0 strips) and keep the fastest in the
+        context; the simulation's results do not change."""
+        r = _abi.AmrTuningResultC()
+        _abi.check(self._lib.lgd_sim_autotune_amr(self.handle, dt, int(reps), C.byref(r)), self.ctx.handle)
+        return {"tile": (r.tile_x, r.tile_y), "wg": r.wg, "level0_strips": r.level0_strips,
+                "level0_prefetch": r.level0_prefetch, "candidates": r.candidates, "ms": r.ms,
+                "default_ms": r.default_ms}
+
     def set_tagger(self, fn) -> None:
         """A caller's TagCells::tag_level in place of the policy's GradientTagger
         (adaptive AMR only; prepare is the trait default): fn(amr: AmrGrid, level)

@@ -89,6 +89,19 @@ class TuningResultC(C.Structure):
     ]
 
 
+class AmrTuningResultC(C.Structure):
+    _fields_ = [
+        ("tile_x", C.c_int32),
+        ("tile_y", C.c_int32),
+        ("wg", C.c_int32),
+        ("level0_strips", C.c_int32),
+        ("level0_prefetch", C.c_int32),
+        ("candidates", C.c_int32),
+        ("ms", C.c_double),
+        ("default_ms", C.c_double),
+    ]
+
+
 P = C.c_void_p
 PP = C.POINTER(C.c_void_p)
 
@@ -177,6 +190,7 @@ SIGNATURES = {
     "lgd_sim_time": (DBL, [P]),
     "lgd_sim_step_index": (I64, [P]),
     "lgd_sim_set_tagger": (C.c_int, [P, P, P]),
+    "lgd_sim_autotune_amr": (C.c_int, [P, DBL, I32, C.POINTER(AmrTuningResultC)]),
     "lgd_tag_sink_push": (C.c_int, [P, C.POINTER(C.c_int64), C.c_size_t]),
     "lgd_sim_initialize_grains": (C.c_int, [P, DP, SZ, DBL]),
     "lgd_sim_set_integrator": (C.c_int, [P, C.c_int32]),

@@ -87,7 +87,21 @@ struct lgd_level_plan {
   int64_t ncorners = 0;
   lgd_restrict_item *ritems = nullptr;  // restriction of this level onto level-1
   int64_t nritems = 0;
+  // The level-stage kernels' tiling in a tuned shape other than the plan's 64×4
+  // (allocated on first use of that shape; the regrid kernels keep `tiles`).
+  mutable lgd_tile *stiles = nullptr;
+  mutable int64_t nstiles = 0;
+  mutable int32_t stx = 0, sty = 0;
+  mutable std::vector<int64_t> stile0;
 };
+
+// The level-stage tile shapes the "amr_tile" knob selects (threads x, y per workgroup,
+// one cell per thread); shape 0 is the plans' own 64×4 tiling, which the regrid
+// kernels share.
+constexpr int kAmrTileShapes[][2] = {{64, 4}, {32, 8}, {16, 16}, {128, 2}, {64, 2}, {32, 4}, {256, 1}};
+
+// Workgroup width of the AMR fill and restriction launches ("amr_wg"; 256 by default).
+inline int lgd_amr_wg(const lgd_ctx *ctx) { return ctx->amr_wg > 0 ? (int)ctx->amr_wg : 256; }
 
 struct lgd_amr {
   lgd_ctx *ctx = nullptr;

@@ -132,8 +132,10 @@ void lgd_amr::free_device() {
   d_bstart = nullptr;
   d_bids = nullptr;
   d_lo = nullptr;
-  for (auto &p : plans)
+  for (auto &p : plans) {
     if (p.block) (void)hipFreeAsync(p.block, st);
+    if (p.stiles) (void)hipFreeAsync(p.stiles, st);
+  }
   plans.clear();
   staging.clear();
 }
@@ -615,7 +617,8 @@ __global__ __launch_bounds__(kTX *kTY) void amr_modelc_step_kernel(
     u_n[o] = un;
     return;
   }
-  const int64_t t = (int64_t)(blockIdx.x - ntiles) * (kTX * kTY) + threadIdx.y * kTX + threadIdx.x;
+  const int64_t t = (int64_t)(blockIdx.x - ntiles) * (blockDim.x * blockDim.y) +
+                    threadIdx.y * blockDim.x + threadIdx.x;
   if (t >= N.nops) return;
   const lgd_fill_op op = N.ops[t];
   if (op.clen == 0) {
@@ -648,7 +651,8 @@ __global__ __launch_bounds__(kTX *kTY) void amr_heat_step_kernel(
     u_n[o] = heat_cell_stage(ihx, ihy, kappa, dt, u, o, sx);
     return;
   }
-  const int64_t t = (int64_t)(blockIdx.x - ntiles) * (kTX * kTY) + threadIdx.y * kTX + threadIdx.x;
+  const int64_t t = (int64_t)(blockIdx.x - ntiles) * (blockDim.x * blockDim.y) +
+                    threadIdx.y * blockDim.x + threadIdx.x;
   if (t >= N.nops) return;
   const lgd_fill_op op = N.ops[t];
   if (op.clen == 0) {
@@ -695,12 +699,13 @@ int lgd_amr_fill_pp(lgd_ctx *ctx, const lgd_amr &g, int level, int nf, double *c
     F.par[f] = par ? par[f] : nullptr;
     F.old[f] = old ? old[f] : nullptr;
   }
-  const unsigned nb = blocks_for(P.nops, 256);
+  const int wg = lgd_amr_wg(ctx);
+  const unsigned nb = blocks_for(P.nops, wg);
   if (old)
-    amr_fill_pp_kernel<true><<<nb, 256, 0, ctx->stream>>>(P.ops, P.nops, P.corners, F, nf,
+    amr_fill_pp_kernel<true><<<nb, wg, 0, ctx->stream>>>(P.ops, P.nops, P.corners, F, nf,
                                                          1.0 - alpha, alpha);
   else
-    amr_fill_pp_kernel<false><<<nb, 256, 0, ctx->stream>>>(P.ops, P.nops, P.corners, F, nf, 0.0,
+    amr_fill_pp_kernel<false><<<nb, wg, 0, ctx->stream>>>(P.ops, P.nops, P.corners, F, nf, 0.0,
                                                           0.0);
   return launch_check(ctx, "amr_fill");
 }
@@ -715,7 +720,8 @@ int lgd_amr_restrict_pp(lgd_ctx *ctx, const lgd_amr &g, int fine_level, int nf,
     R.fine[f] = fine[f];
     R.coarse[f] = coarse[f];
   }
-  amr_restrict_pp_kernel<<<blocks_for(P.nritems, 256), 256, 0, ctx->stream>>>(
+  const int wg = lgd_amr_wg(ctx);
+  amr_restrict_pp_kernel<<<blocks_for(P.nritems, wg), wg, 0, ctx->stream>>>(
       P.ritems, P.nritems, r, 1.0 / (double)(r * r), R, nf);
   return launch_check(ctx, "amr_restrict");
 }
@@ -725,6 +731,62 @@ bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g) {
 }
 
 namespace {
+
+constexpr int kNumTileShapes = (int)(sizeof(kAmrTileShapes) / sizeof(kAmrTileShapes[0]));
+
+struct StageTiles {
+  const lgd_tile *tiles = nullptr;
+  int64_t n = 0;
+  const std::vector<int64_t> *tile0 = nullptr;
+  dim3 blk{kTX, kTY};
+};
+
+// The tiles the stage kernels of `level` run over in the context's shape: the plan's
+// own, or a retiling of the level's patches kept with the plan until the shape
+// changes.
+int stage_tiles(lgd_ctx *ctx, const lgd_amr &g, int level, StageTiles &T) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  const int sh = (int)ctx->amr_tile;
+  if (sh <= 0 || sh >= kNumTileShapes) {
+    T.tiles = P.tiles;
+    T.n = P.ntiles;
+    T.tile0 = &P.tile0;
+    T.blk = dim3(kTX, kTY);
+    return LGD_OK;
+  }
+  const int tx = kAmrTileShapes[sh][0], ty = kAmrTileShapes[sh][1];
+  if (P.stx != tx || P.sty != ty) {
+    std::vector<lgd_tile> tiles;
+    P.stile0.clear();
+    for (uint32_t b = g.level_start[level]; b < g.level_start[level + 1]; ++b) {
+      P.stile0.push_back((int64_t)tiles.size());
+      for (int64_t y0 = 0; y0 < g.box[b].extent(1); y0 += ty)
+        for (int64_t x0 = 0; x0 < g.box[b].extent(0); x0 += tx)
+          tiles.push_back(lgd_tile{(int32_t)b, (int32_t)x0, (int32_t)y0, 0});
+    }
+    P.stile0.push_back((int64_t)tiles.size());
+    if (P.stiles) (void)hipFreeAsync(P.stiles, ctx->stream);
+    P.stiles = nullptr;
+    P.nstiles = 0;
+    P.stx = P.sty = 0;
+    if (!tiles.empty()) {
+      hipError_t e = hipMallocAsync((void **)&P.stiles, tiles.size() * sizeof(lgd_tile), ctx->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(P.stiles, tiles.data(), tiles.size() * sizeof(lgd_tile),
+                           hipMemcpyHostToDevice, ctx->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // `tiles` leaves scope
+      if (e != hipSuccess) return hip_fail(ctx, e, "amr stage tiles");
+    }
+    P.nstiles = (int64_t)tiles.size();
+    P.stx = tx;
+    P.sty = ty;
+  }
+  T.tiles = P.stiles;
+  T.n = P.nstiles;
+  T.tile0 = &P.stile0;
+  T.blk = dim3(tx, ty);
+  return LGD_OK;
+}
 
 // The next-fill arguments of a stage launch (nops = 0: none).
 NextFill next_fill_args(const lgd_level_plan &P, const lgd_amr_next_fill *nf, int nfields) {
@@ -783,31 +845,36 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
                                     u, phi_n, u_n, dt, noise, salt, wiener0, shape.strips,
                                     shape.prefetch);
   }
-  if (P.ntiles == 0 && !(nf && P.nops)) return LGD_OK;
+  StageTiles T;
+  int rc;
+  if ((rc = stage_tiles(ctx, g, level, T))) return rc;
+  if (T.n == 0 && !(nf && P.nops)) return LGD_OK;
   const NextFill none{};
   auto launch = [&](const lgd_tile *tiles, int64_t n) {
     if (noise)
-      amr_modelc_step_kernel<true, false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+      amr_modelc_step_kernel<true, false><<<(unsigned)n, T.blk, 0, ctx->stream>>>(
           tiles, n, g.d_blocks, k, dt, phi, u, phi_n, u_n, scale, seed, salt, ctx->salt_src, none);
     else
-      amr_modelc_step_kernel<false, false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+      amr_modelc_step_kernel<false, false><<<(unsigned)n, T.blk, 0, ctx->stream>>>(
           tiles, n, g.d_blocks, k, dt, phi, u, phi_n, u_n, 0.0, seed, salt, nullptr, none);
   };
   if (ctx->amr_per_block && nf)
     return fail(ctx, LGD_ERR_UNSUPPORTED, "per-patch launches fill ghosts in a launch of their own");
   if (ctx->amr_per_block) {
-    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) launch(P.tiles + P.tile0[b], P.tile0[b + 1] - P.tile0[b]);
+    for (size_t b = 0; b + 1 < T.tile0->size(); ++b)
+      launch(T.tiles + (*T.tile0)[b], (*T.tile0)[b + 1] - (*T.tile0)[b]);
   } else if (nf && P.nops) {
     const NextFill N = next_fill_args(P, nf, 2);
-    const unsigned nwg = (unsigned)(P.ntiles + (P.nops + kTX * kTY - 1) / (kTX * kTY));
+    const int64_t per = (int64_t)T.blk.x * T.blk.y;
+    const unsigned nwg = (unsigned)(T.n + (P.nops + per - 1) / per);
     if (noise)
-      amr_modelc_step_kernel<true, true><<<nwg, dim3(kTX, kTY), 0, ctx->stream>>>(
-          P.tiles, P.ntiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, scale, seed, salt, ctx->salt_src, N);
+      amr_modelc_step_kernel<true, true><<<nwg, T.blk, 0, ctx->stream>>>(
+          T.tiles, T.n, g.d_blocks, k, dt, phi, u, phi_n, u_n, scale, seed, salt, ctx->salt_src, N);
     else
-      amr_modelc_step_kernel<false, true><<<nwg, dim3(kTX, kTY), 0, ctx->stream>>>(
-          P.tiles, P.ntiles, g.d_blocks, k, dt, phi, u, phi_n, u_n, 0.0, seed, salt, nullptr, N);
+      amr_modelc_step_kernel<false, true><<<nwg, T.blk, 0, ctx->stream>>>(
+          T.tiles, T.n, g.d_blocks, k, dt, phi, u, phi_n, u_n, 0.0, seed, salt, nullptr, N);
   } else {
-    launch(P.tiles, P.ntiles);
+    launch(T.tiles, T.n);
   }
   return launch_check(ctx, "amr_modelc_step");
 }
@@ -815,25 +882,30 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
 int lgd_amr_heat_step(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, double dt,
                       const double *u, double *u_n, const lgd_amr_next_fill *nf) {
   const lgd_level_plan &P = g.plans[(size_t)level];
-  if (P.ntiles == 0 && !(nf && P.nops)) return LGD_OK;
+  StageTiles T;
+  int rc;
+  if ((rc = stage_tiles(ctx, g, level, T))) return rc;
+  if (T.n == 0 && !(nf && P.nops)) return LGD_OK;
   const double hx = g.spacing[(size_t)level][0], hy = g.spacing[(size_t)level][1];
   const double ihx = 1.0 / (hx * hx), ihy = 1.0 / (hy * hy);
   const NextFill none{};
   auto launch = [&](const lgd_tile *tiles, int64_t n) {
-    amr_heat_step_kernel<false><<<(unsigned)n, dim3(kTX, kTY), 0, ctx->stream>>>(
+    amr_heat_step_kernel<false><<<(unsigned)n, T.blk, 0, ctx->stream>>>(
         tiles, n, g.d_blocks, ihx, ihy, kappa, dt, u, u_n, none);
   };
   if (ctx->amr_per_block && nf)
     return fail(ctx, LGD_ERR_UNSUPPORTED, "per-patch launches fill ghosts in a launch of their own");
   if (ctx->amr_per_block) {
-    for (size_t b = 0; b + 1 < P.tile0.size(); ++b) launch(P.tiles + P.tile0[b], P.tile0[b + 1] - P.tile0[b]);
+    for (size_t b = 0; b + 1 < T.tile0->size(); ++b)
+      launch(T.tiles + (*T.tile0)[b], (*T.tile0)[b + 1] - (*T.tile0)[b]);
   } else if (nf && P.nops) {
     const NextFill N = next_fill_args(P, nf, 1);
-    const unsigned nwg = (unsigned)(P.ntiles + (P.nops + kTX * kTY - 1) / (kTX * kTY));
-    amr_heat_step_kernel<true><<<nwg, dim3(kTX, kTY), 0, ctx->stream>>>(
-        P.tiles, P.ntiles, g.d_blocks, ihx, ihy, kappa, dt, u, u_n, N);
+    const int64_t per = (int64_t)T.blk.x * T.blk.y;
+    const unsigned nwg = (unsigned)(T.n + (P.nops + per - 1) / per);
+    amr_heat_step_kernel<true><<<nwg, T.blk, 0, ctx->stream>>>(
+        T.tiles, T.n, g.d_blocks, ihx, ihy, kappa, dt, u, u_n, N);
   } else {
-    launch(P.tiles, P.ntiles);
+    launch(T.tiles, T.n);
   }
   return launch_check(ctx, "amr_heat_step");
 }

@@ -147,6 +147,13 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
   } else if (k == "amr_split_fill") {
     if (value > 1) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_split_fill is 0 or 1");
     ctx->amr_split_fill = value;
+  } else if (k == "amr_tile") {
+    if (value > 6) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_tile is 0..6");
+    ctx->amr_tile = value;
+  } else if (k == "amr_wg") {
+    if (value != 0 && value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
+      return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_wg is 0, 64, 128, 256, 512 or 1024");
+    ctx->amr_wg = value;
   } else if (k == "adjoint_kernel") {
     if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "adjoint_kernel is 0, 1 or 2");
     ctx->adjoint_kernel = value;

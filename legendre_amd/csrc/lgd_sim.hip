@@ -972,6 +972,164 @@ int lgd_sim_regrid_stats(const lgd_sim *sim, int64_t *regrids, double *seconds) 
 
 int64_t lgd_sim_step_index(const lgd_sim *sim) { return sim ? sim->step_index : -1; }
 
+int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_result *out) {
+  if (!sim || !out || reps < 1) return LGD_ERR_INVALID_ARGUMENT;
+  if (!sim->amr) return fail(sim->ctx, LGD_ERR_INVALID_ARGUMENT, "not an AMR simulation");
+  lgd_ctx *ctx = sim->ctx;
+  const lgd_amr &g = *sim->amr;
+  const int L = g.num_levels(), nf = nfields(sim);
+  const bool modelc = sim->kind == lgd_sim::kModelC;
+  int rc = LGD_OK;
+  // Substeps of each level per coarse step (Subcycle::advance's n = round(dt_l/dt_{l+1})),
+  // one each with a global dt.
+  std::vector<double> N((size_t)L, 1.0);
+  for (int l = 1; l < L; ++l)
+    N[(size_t)l] = N[(size_t)l - 1] *
+                   (sim->subcycle ? std::fmax(std::round(amr_stable_dt(sim, l - 1) / amr_stable_dt(sim, l)), 1.0)
+                                  : 1.0);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return hip_fail(ctx, e, "autotune events");
+  }
+  // Mean ms of `reps` launches of f after one untimed launch.
+  auto timed = [&](auto &&f, double *ms) -> int {
+    int r = f();
+    if (r) return r;
+    hipError_t er = hipEventRecord(e0, ctx->stream);
+    for (int i = 0; i < reps && !r; ++i) r = f();
+    if (r) return r;
+    if (er == hipSuccess) er = hipEventRecord(e1, ctx->stream);
+    if (er == hipSuccess) er = hipEventSynchronize(e1);
+    float t = 0.0f;
+    if (er == hipSuccess) er = hipEventElapsedTime(&t, e0, e1);
+    if (er != hipSuccess) return hip_fail(ctx, er, "autotune timing");
+    *ms = (double)t / reps;
+    return LGD_OK;
+  };
+  uint64_t salt;
+  std::memcpy(&salt, &sim->t, sizeof salt);
+  // Every timed launch writes only what the next coarse step overwrites before it
+  // reads it: a level's other buffer (its stage output), the ghost entries of the
+  // current one (fills), the parent's other buffer (restrictions).
+  auto stage = [&](int l) {
+    if (modelc)
+      return lgd_amr_modelc_step(ctx, g, sim->params, l, dt, amr_cur(sim, 0, l)->data,
+                                 amr_cur(sim, 1, l)->data, amr_prev(sim, 0, l)->data,
+                                 amr_prev(sim, 1, l)->data, sim->seed, salt, sim->wiener != 0);
+    return lgd_amr_heat_step(ctx, g, sim->kappa, l, dt, amr_cur(sim, 0, l)->data,
+                             amr_prev(sim, 0, l)->data);
+  };
+  auto fill = [&](int l) { return amr_fill_level(sim, l, l > 0, 0.5); };
+  auto restrict_prev = [&](int l) {
+    const double *fine[2] = {nullptr, nullptr};
+    double *coarse[2] = {nullptr, nullptr};
+    for (int f = 0; f < nf; ++f) {
+      fine[f] = amr_cur(sim, f, l)->data;
+      coarse[f] = amr_prev(sim, f, l - 1)->data;
+    }
+    return lgd_amr_restrict_pp(ctx, g, l, nf, fine, coarse);
+  };
+  const bool streams0 = modelc && lgd_amr_level0_streams(ctx, g);
+  const int64_t k0 = ctx->modelc_kernel, s0 = ctx->modelc_strips, p0 = ctx->modelc_prefetch;
+  const int64_t t0 = ctx->amr_tile, w0 = ctx->amr_wg;
+  int32_t candidates = 0;
+  // Level-stage tiles (the levels the per-cell stage runs on).
+  double best_tiles = 1e300, dflt_tiles = 0.0;
+  int best_tile = 0;
+  for (int sh = 0; sh <= 6 && !rc; ++sh) {
+    ctx->amr_tile = sh;
+    double tot = 0.0;
+    for (int l = streams0 ? 1 : 0; l < L && !rc; ++l) {
+      double ms = 0.0;
+      rc = timed([&] { return stage(l); }, &ms);
+      tot += N[(size_t)l] * ms;
+    }
+    ++candidates;
+    if (sh == 0) dflt_tiles = tot;
+    if (tot < best_tiles) {
+      best_tiles = tot;
+      best_tile = sh;
+    }
+  }
+  // Fill and restriction widths.
+  double best_wg_ms = 1e300, dflt_wg = 0.0;
+  int best_wg = 256;
+  const int widths[] = {64, 128, 256, 512, 1024};
+  for (int w : widths) {
+    if (rc) break;
+    ctx->amr_wg = w;
+    double tot = 0.0;
+    for (int l = 0; l < L && !rc; ++l) {
+      double ms = 0.0;
+      if (!(l == 0 && streams0)) {
+        rc = timed([&] { return fill(l); }, &ms);
+        tot += N[(size_t)l] * ms;
+      }
+      if (l >= 1 && !rc) {
+        rc = timed([&] { return restrict_prev(l); }, &ms);
+        tot += N[(size_t)l - 1] * ms;
+      }
+    }
+    ++candidates;
+    if (w == 256) dflt_wg = tot;
+    if (tot < best_wg_ms) {
+      best_wg_ms = tot;
+      best_wg = w;
+    }
+  }
+  // Level 0 on the streaming kernel: strips and rows loaded ahead.
+  lgd_launch_shape best0 = lgd_modelc_default_shape(modelc && sim->wiener && sim->params.noise_amplitude != 0.0);
+  double best0_ms = 0.0, dflt0_ms = 0.0;
+  const bool noise0 = modelc && sim->wiener != 0 && sim->params.noise_amplitude != 0.0;
+  if (streams0 && !rc) {
+    const lgd_launch_shape d = lgd_modelc_default_shape(noise0);
+    best0_ms = 1e300;
+    const int strips[] = {1, 2, 4, 8};
+    for (int st : strips)
+      for (int pf = 1; pf <= 2 && !rc; ++pf) {
+        ctx->modelc_kernel = 2;
+        ctx->modelc_strips = st;
+        ctx->modelc_prefetch = pf;
+        double ms = 0.0;
+        rc = timed([&] { return stage(0); }, &ms);
+        ++candidates;
+        if (st == d.strips && pf == d.prefetch) dflt0_ms = ms;
+        if (ms < best0_ms) {
+          best0_ms = ms;
+          best0 = lgd_launch_shape{2, st, pf};
+        }
+      }
+  }
+  ctx->modelc_kernel = k0;
+  ctx->modelc_strips = s0;
+  ctx->modelc_prefetch = p0;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) {
+    ctx->amr_tile = t0;
+    ctx->amr_wg = w0;
+    return rc;
+  }
+  ctx->amr_tile = best_tile;
+  ctx->amr_wg = best_wg;
+  if (streams0)
+    ctx->tuned[std::make_tuple(g.base.n[0], g.base.n[1], (int64_t)g.level_start[1], noise0 ? 1 : 0)] =
+        best0;
+  ++ctx->tuning_epoch;  // captured graphs hold the old shapes
+  out->tile_x = kAmrTileShapes[best_tile][0];
+  out->tile_y = kAmrTileShapes[best_tile][1];
+  out->wg = best_wg;
+  out->level0_strips = streams0 ? best0.strips : 0;
+  out->level0_prefetch = streams0 ? best0.prefetch : 0;
+  out->candidates = candidates;
+  out->ms = best_tiles + best_wg_ms + best0_ms;
+  out->default_ms = dflt_tiles + dflt_wg + dflt0_ms;
+  return LGD_OK;
+}
+
 int lgd_sim_set_tagger(lgd_sim *sim, lgd_tag_level_fn fn, void *user) {
   if (!sim) return LGD_ERR_INVALID_ARGUMENT;
   if (!sim->amr || !sim->adaptive)

@@ -331,3 +331,56 @@ def test_fused_next_fill_matches_split_fill_bitwise(ctx, orc, model):
         assert a[0] == b[0], f"step {s}: hierarchies differ"
         for fa, fb in zip(a[1], b[1]):
             assert np.array_equal(fa, fb), f"step {s}: fields differ"
+
+
+def test_amr_launch_shapes_change_no_bit(ctx, orc):
+    """Every level-stage tile shape, fill / restriction width and the autotuner's
+    choice give the bits of the default shapes: three-level adaptive subcycled Model
+    C with noise, hierarchy and fields after every step; the autotuner leaves the
+    simulation it times bitwise unchanged."""
+    say("amr launch shapes")
+    h, n, blk = 0.4, 96, 24
+    base = orc.grid([n, n], [blk, blk], [0, 0], [h, h])
+    phi = np.zeros(orc.field_len(base, 1))
+    u = np.zeros_like(phi)
+    orc.initialize_grains(base, phi, u, np.array([[10.0, 9.0, 3.0], [22.0, 25.0, 2.0]]), 0.55)
+    pol = lg.RegridPolicy(4, 2, 0.8, 4, 0.1, 0)
+    dt = 0.2 * h * h / 2.0
+
+    def run(knobs, tune=False):
+        for k, v in knobs.items():
+            ctx.set_tuning(k, v)
+        gg = lg.AmrGrid.from_patches(ctx, [n, n], [blk, blk], [0, 0], [h, h], [2, 2], [])
+        sim = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.05, 3.19, 2.0, 0.01),
+                                wiener=True, seed=4, subcycle=True, policy=pol)
+        sim.field(0).upload(phi)
+        sim.field(1).upload(u)
+        trace = []
+        for s in range(12):
+            sim.step(dt)
+            if tune and s == 5:
+                r = sim.autotune_amr(dt, reps=2)
+                assert r["candidates"] >= 12 and r["ms"] > 0.0
+            a = sim.amr
+            m = H.amr_interior_mask(a)
+            trace.append((patches(a), sim.field(0).download()[m], sim.field(1).download()[m]))
+        sim.close()
+        return trace
+
+    shapes = [{}, {"amr_tile": 1, "amr_wg": 64}, {"amr_tile": 2, "amr_wg": 1024},
+              {"amr_tile": 6, "amr_wg": 128}, {"amr_tile": 3, "modelc_strips": 1, "modelc_prefetch": 1}]
+    try:
+        ref = run({})
+        assert any(p[0] == 2 for tr in ref for p in tr[0])
+        for knobs in shapes[1:]:
+            got = run(knobs)
+            for s, (a, b) in enumerate(zip(ref, got)):
+                assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), (knobs, s)
+            for k in knobs:
+                ctx.set_tuning(k, 0)
+        got = run({}, tune=True)
+        for s, (a, b) in enumerate(zip(ref, got)):
+            assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), ("tuned", s)
+    finally:
+        for k in ("amr_tile", "amr_wg", "modelc_strips", "modelc_prefetch"):
+            ctx.set_tuning(k, 0)
