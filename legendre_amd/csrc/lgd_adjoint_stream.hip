@@ -30,6 +30,7 @@
 // multiply-adds here (this file only): the derivative needs no bitwise match with
 // the forward, and agrees with the three-pass form to rounding.
 #include <cmath>
+#include <type_traits>
 
 #include "lgd_internal.h"
 
@@ -126,6 +127,9 @@ struct RevArgs {
 
 constexpr int kWaves = 4;  // waves per workgroup
 
+// Row-iteration phases of the reverse kernel.
+constexpr int kGeneral = 0, kHead = 1, kFirst = 2, kMid = 3, kLast = 4;
+
 __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
 #pragma clang fp contract(fast)
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -197,19 +201,40 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
   double Vmm = 0.0, Vm = 0.0, PUm = 0.0;
   double part_l = 0.0, part_d = 0.0, part_ec = 0.0, part_ef = 0.0;
 
-  for (int r = y0 - 1; r <= y1; ++r) {
-    // Row r+2 of the state and the adjoint at row r+1 (rows y1+1 and y1 again at
-    // the last row, where they go unused), then the offsets of the next ones.
-    const double Pn = a.phi[offS + colofs], Un = a.u[offS + colofs];
-    const double apn = a.APi[offA + rx], aun = a.AUi[offA + rx];
-    {
+  // One row iteration.  kGeneral evaluates every ownership and mirror condition at
+  // run time.  The other phases serve interior strips, whose rows y0−2 .. y1+1 all
+  // lie inside the domain (no ghost row resolves and no mirror fold applies): the
+  // halo row y0−1 (kHead), the first owned row (kFirst, row y0−1 is not written),
+  // the owned rows after it (kMid) and the halo row y1 (kLast, which loads nothing).
+  const int64_t bjump = bstride - (int64_t)a.n1 * a.sx;  // slab row n1 → next block row
+  auto row_step = [&](int r, auto phase) {
+    constexpr int kPhase = decltype(phase)::value;
+    constexpr bool kGen = kPhase == kGeneral;
+    // Row r+2 of the state and the adjoint at row r+1 (in the general form, rows
+    // y1+1 and y1 again at the last row, where they go unused), then the offsets of
+    // the next ones.
+    double Pn = 0.0, Un = 0.0, apn = 0.0, aun = 0.0;
+    if (kPhase != kLast) {
+      Pn = a.phi[offS + colofs];
+      Un = a.u[offS + colofs];
+      apn = a.APi[offA + rx];
+      aun = a.AUi[offA + rx];
+    }
+    if (!kGen) {
+      offS += a.sx;
+      if (++jS == a.n1) {
+        jS = 0;
+        offS += bjump;
+      }
+      offA += N0;
+    } else {
       const int g = min(r + 3, y1 + 1);
       if (g != gS) {
         if (g >= 1 && g < N1) {
           offS += a.sx;
           if (++jS == a.n1) {
             jS = 0;
-            offS += bstride - (int64_t)a.n1 * a.sx;
+            offS += bjump;
           }
         } else {
           offS = slab_row(g, &jS);
@@ -222,8 +247,13 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
         gA = ga;
       }
     }
-    const bool real = colreal && (a.per1 || (r >= 0 && r < N1));
-    const bool row_own = r >= y0 && r < y1;
+    const bool real = kGen ? colreal && (a.per1 || (r >= 0 && r < N1)) : colreal;
+    const bool row_own = kGen ? r >= y0 && r < y1 : kPhase == kFirst || kPhase == kMid;
+    // Row r−1 is owned (complete after this row, and its y-face (r−1|r) is owned).
+    const bool prev_own = kGen ? r - 1 >= y0 && r - 1 < y1 : kPhase == kMid || kPhase == kLast;
+    // The mirror folds (never in an interior strip).
+    const bool fold_lo = kGen && !a.per1 && r == 0;   // ghost row −1 onto row 0
+    const bool fold_hi = kGen && !a.per1 && r == N1;  // ghost row N1 onto row N1−1
     const double PpL = from_left(Pp), PpR = from_right(Pp);
     const double UpL = from_left(Up), UpR = from_right(Up);
 
@@ -290,8 +320,7 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
     if (row_own && (colown || face_left_ghost)) part_ef += G * FXe;
     const double Hy = (Rprev - R) * k.inv_h2;
     const double hys = Hy * FYy, hxs = q * (Hy * FYx);
-    if (colown && ((r - 1 >= y0 && r - 1 < y1) || (r == y0 && !a.per1 && r == 0)))
-      part_ef += Hy * FYe;
+    if (colown && (prev_own || (fold_lo && r == y0))) part_ef += Hy * FYe;
     // x-faces: −gx to the left cell, +gx to the right cell, ±q·gy to rows r±1;
     // y-face (r−1|r): +gy to row r, −gy to row r−1, ±q·gx to columns c±1;
     // centres: ±SX to columns c±1, ±SY to rows r±1.
@@ -303,10 +332,10 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
     accp += gy_in + SY;
     accm += (hx_in - hys) - (gy_in + SY);
     // Row r−1 is complete.
-    if (!a.per1 && r == 0) {
+    if (fold_lo) {
       accc += accm;  // the mirror ghost row −1 is an image of row 0
-    } else if (r - 1 >= y0 && r - 1 < y1) {
-      if (!a.per1 && r == N1) accm += accc;  // the mirror ghost row N1 is an image of row N1−1
+    } else if (prev_own) {
+      if (fold_hi) accm += accc;  // the mirror ghost row N1 is an image of row N1−1
       const double lv = from_left(accm), rv = from_right(accm);
       const double vl = from_left(Vm), vr = from_right(Vm);
       if (colown) {
@@ -321,8 +350,8 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
           lx += Vm;
         }
         double ly = Vmm + V;
-        if (!a.per1 && r - 1 == 0) ly += Vm;
-        if (!a.per1 && r - 1 == N1 - 1) ly += Vm;
+        if (kGen && !a.per1 && r - 1 == 0) ly += Vm;
+        if (kGen && !a.per1 && r - 1 == N1 - 1) ly += Vm;
         const double lap = (lx - 2.0 * Vm) * k.inv_hx2 + (ly - 2.0 * Vm) * k.inv_hy2;
         const int64_t o = (int64_t)gx + (int64_t)N0 * (r - 1);
         a.APo[o] = out;
@@ -354,6 +383,15 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
     UcR = UpR;
     ap = apn;
     au = aun;
+  };
+  using GenT = std::integral_constant<int, kGeneral>;
+  if (y0 >= 2 && y1 + 1 <= N1 - 1) {
+    row_step(y0 - 1, std::integral_constant<int, kHead>());
+    row_step(y0, std::integral_constant<int, kFirst>());
+    for (int r = y0 + 1; r < y1; ++r) row_step(r, std::integral_constant<int, kMid>());
+    row_step(y1, std::integral_constant<int, kLast>());
+  } else {
+    for (int r = y0 - 1; r <= y1; ++r) row_step(r, GenT());
   }
   // Fixed-order wave sums of the θ partials.
   double p4[4] = {part_l, part_d, part_ec, part_ef};
