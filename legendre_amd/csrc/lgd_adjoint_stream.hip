@@ -207,9 +207,13 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
   // halo row y0−1 (kHead), the first owned row (kFirst, row y0−1 is not written),
   // the owned rows after it (kMid) and the halo row y1 (kLast, which loads nothing).
   const int64_t bjump = bstride - (int64_t)a.n1 * a.sx;  // slab row n1 → next block row
-  auto row_step = [&](int r, auto phase) {
+  // kXEdge: the tile reaches within two columns of a mirror edge in x, so some lane
+  // is a ghost column (colreal false) or folds one (left_edge, right_edge,
+  // face_left_ghost); without it those flags are false on every lane.
+  auto row_step = [&](int r, auto phase, auto xedge) {
     constexpr int kPhase = decltype(phase)::value;
     constexpr bool kGen = kPhase == kGeneral;
+    constexpr bool kXEdge = decltype(xedge)::value;
     // Row r+2 of the state and the adjoint at row r+1 (in the general form, rows
     // y1+1 and y1 again at the last row, where they go unused), then the offsets of
     // the next ones.
@@ -247,7 +251,8 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
         gA = ga;
       }
     }
-    const bool real = kGen ? colreal && (a.per1 || (r >= 0 && r < N1)) : colreal;
+    const bool row_real = !kGen || a.per1 || (r >= 0 && r < N1);
+    const bool real = kXEdge ? colreal && row_real : row_real;
     const bool row_own = kGen ? r >= y0 && r < y1 : kPhase == kFirst || kPhase == kMid;
     // Row r−1 is owned (complete after this row, and its y-face (r−1|r) is owned).
     const bool prev_own = kGen ? r - 1 >= y0 && r - 1 < y1 : kPhase == kMid || kPhase == kLast;
@@ -317,7 +322,7 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
     // x-face weights (R(c) − R(c+1))/(hx·hy); y-face (r−1|r) weights (R(r−1) − R(r))/(hx·hy).
     const double G = (R - from_right(R)) * k.inv_h2;
     const double gxs = G * FXx, gys = q * (G * FXy);
-    if (row_own && (colown || face_left_ghost)) part_ef += G * FXe;
+    if (row_own && (colown || (kXEdge && face_left_ghost))) part_ef += G * FXe;
     const double Hy = (Rprev - R) * k.inv_h2;
     const double hys = Hy * FYy, hxs = q * (Hy * FYx);
     if (colown && (prev_own || (fold_lo && r == y0))) part_ef += Hy * FYe;
@@ -336,16 +341,20 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
       accc += accm;  // the mirror ghost row −1 is an image of row 0
     } else if (prev_own) {
       if (fold_hi) accm += accc;  // the mirror ghost row N1 is an image of row N1−1
-      const double lv = from_left(accm), rv = from_right(accm);
       const double vl = from_left(Vm), vr = from_right(Vm);
+      double lv = 0.0, rv = 0.0;
+      if (kXEdge) {
+        lv = from_left(accm);
+        rv = from_right(accm);
+      }
       if (colown) {
         double out = accm;
         double lx = vl + vr;
-        if (left_edge) {
+        if (kXEdge && left_edge) {
           out += lv;
           lx += Vm;
         }
-        if (right_edge) {
+        if (kXEdge && right_edge) {
           out += rv;
           lx += Vm;
         }
@@ -384,15 +393,20 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
     ap = apn;
     au = aun;
   };
-  using GenT = std::integral_constant<int, kGeneral>;
-  if (y0 >= 2 && y1 + 1 <= N1 - 1) {
-    row_step(y0 - 1, std::integral_constant<int, kHead>());
-    row_step(y0, std::integral_constant<int, kFirst>());
-    for (int r = y0 + 1; r < y1; ++r) row_step(r, std::integral_constant<int, kMid>());
-    row_step(y1, std::integral_constant<int, kLast>());
-  } else {
-    for (int r = y0 - 1; r <= y1; ++r) row_step(r, GenT());
-  }
+  auto run = [&](auto xedge) {
+    if (y0 >= 2 && y1 + 1 <= N1 - 1) {
+      row_step(y0 - 1, std::integral_constant<int, kHead>(), xedge);
+      row_step(y0, std::integral_constant<int, kFirst>(), xedge);
+      for (int r = y0 + 1; r < y1; ++r) row_step(r, std::integral_constant<int, kMid>(), xedge);
+      row_step(y1, std::integral_constant<int, kLast>(), xedge);
+    } else {
+      for (int r = y0 - 1; r <= y1; ++r) row_step(r, std::integral_constant<int, kGeneral>(), xedge);
+    }
+  };
+  if (!a.per0 && (x0 <= 2 || x1 + 2 >= N0))
+    run(std::true_type());
+  else
+    run(std::false_type());
   // Fixed-order wave sums of the θ partials.
   double p4[4] = {part_l, part_d, part_ec, part_ef};
   for (int n = 0; n < 4; ++n)
