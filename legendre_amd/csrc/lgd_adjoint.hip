@@ -567,9 +567,12 @@ int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out) {
   a->wg_cell = (a->ncell + kWG - 1) / kWG;
   a->wg_face = ((int64_t)(G.N0 + 1) * (G.N1 + 1) + kWG - 1) / kWG;
   int rc = LGD_OK;
-  double **cells[] = {&a->AP, &a->AU, &a->R, &a->SX, &a->SY, &a->PW, &a->PU, &a->V};
+  double **cells[] = {&a->R, &a->SX, &a->SY, &a->PW, &a->PU, &a->V};
   for (double **p : cells)
     if (!rc) rc = adj_alloc(a, a->ncell, p);
+  // The adjoint pairs swap roles each streaming step: both carry its output pad.
+  if (!rc) rc = adj_alloc(a, a->ncell + kAdjStreamPad, &a->AP);
+  if (!rc) rc = adj_alloc(a, a->ncell + kAdjStreamPad, &a->AU);
   if (!rc) rc = adj_alloc(a, a->npad, &a->Pp);
   if (!rc) rc = adj_alloc(a, a->npad, &a->Up);
   if (!rc) rc = adj_alloc(a, a->nfx, &a->GXx);
@@ -587,8 +590,8 @@ int lgd_adjoint_create(lgd_ctx *ctx, const lgd_grid *grid, lgd_adjoint **out) {
                                              nullptr, nullptr, nullptr, 2, &nw) == LGD_OK;
     if (a->stream_ok) {
       a->nacc_stream = 4 * nw;
-      if (!rc) rc = adj_alloc(a, a->ncell, &a->AP2);
-      if (!rc) rc = adj_alloc(a, a->ncell, &a->AU2);
+      if (!rc) rc = adj_alloc(a, a->ncell + kAdjStreamPad, &a->AP2);
+      if (!rc) rc = adj_alloc(a, a->ncell + kAdjStreamPad, &a->AU2);
       if (!rc) rc = adj_alloc(a, a->nacc_stream, &a->acc_stream);
     }
   }
@@ -654,13 +657,14 @@ int lgd_adjoint_grad(lgd_adjoint *a, double *grad) {
 // ------------------------------------------------- the reverse step without allocation
 // lgd_modelc_adjoint_step_noalloc keeps its workspace in one static device array per
 // GPU instead of an lgd_adjoint: the flattened incoming and outgoing adjoint (four
-// arrays of kNoAllocCells), the per-wave θ partials of the streaming reverse step and
-// the reduced sums.  Host code never uses the array's name as an address: each
+// arrays of kNoAllocCells, each followed by the streaming step's output pad), the
+// per-wave θ partials of the streaming reverse step and the reduced sums.  Host code never uses the array's name as an address: each
 // device's copy is looked up once with hipGetSymbolAddress and kept in
 // g_scratch_ptr.
 constexpr int64_t kNoAllocCells = (int64_t)1 << 20;  // LGD_NOALLOC_ADJOINT_CELLS
 constexpr int64_t kNoAllocAcc = (int64_t)1 << 16;    // θ partials: 4 per wave
-constexpr int64_t kNoAllocDoubles = 4 * kNoAllocCells + kNoAllocAcc + 8;
+constexpr int64_t kNoAllocArray = kNoAllocCells + kAdjStreamPad;
+constexpr int64_t kNoAllocDoubles = 4 * kNoAllocArray + kNoAllocAcc + 8;
 
 static __device__ __align__(256) double scratch[kNoAllocDoubles];
 
@@ -732,8 +736,8 @@ extern "C" int lgd_modelc_adjoint_step_noalloc(lgd_ctx *ctx, const lgd_grid *gri
     return fail(ctx, LGD_ERR_UNSUPPORTED, "too many reverse-step waves for the static scratch");
   double *base = nullptr;
   if ((rc = noalloc_scratch(ctx, &base))) return rc;
-  double *AP = base, *AU = base + kNoAllocCells, *AP2 = base + 2 * kNoAllocCells,
-         *AU2 = base + 3 * kNoAllocCells, *acc = base + 4 * kNoAllocCells,
+  double *AP = base, *AU = base + kNoAllocArray, *AP2 = base + 2 * kNoAllocArray,
+         *AU2 = base + 3 * kNoAllocArray, *acc = base + 4 * kNoAllocArray,
          *sums = acc + kNoAllocAcc;
   // The previous call's partials are still there: clear them with a kernel (a
   // memset node misbehaves under graph capture).

@@ -347,7 +347,11 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
         lv = from_left(accm);
         rv = from_right(accm);
       }
-      if (colown) {
+      // Outside the general form every lane stores, the unowned ones into the pad past
+      // the N0 × N1 outputs: no branch around the stores, so the wait for a row's
+      // loads never has to count in a store that a skipped branch might not have
+      // issued, and so never waits for this row's stores.
+      if (!kGen || colown) {
         double out = accm;
         double lx = vl + vr;
         if (kXEdge && left_edge) {
@@ -362,7 +366,8 @@ __global__ __launch_bounds__(64 * kWaves) void adj_stream_kernel(RevArgs a) {
         if (kGen && !a.per1 && r - 1 == 0) ly += Vm;
         if (kGen && !a.per1 && r - 1 == N1 - 1) ly += Vm;
         const double lap = (lx - 2.0 * Vm) * k.inv_hx2 + (ly - 2.0 * Vm) * k.inv_hy2;
-        const int64_t o = (int64_t)gx + (int64_t)N0 * (r - 1);
+        const int64_t o = kGen || colown ? (int64_t)gx + (int64_t)N0 * (r - 1)
+                                         : (int64_t)N0 * N1 + lane;
         a.APo[o] = out;
         a.AUo[o] = PUm + lap;
       }

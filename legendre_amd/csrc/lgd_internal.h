@@ -241,10 +241,12 @@ int lgd_modelc_stream_rk4_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd
                                  int strips, int prefetch);
 
 // Streaming reverse Model C step (lgd_adjoint_stream.hip): adjoint (APi, AUi) of the
-// state after the step → (APo, AUo) before it, global N0 × N1 arrays; θ partials
+// state after the step → (APo, AUo) before it, global N0 × N1 arrays, each output
+// with kAdjStreamPad more doubles after them that the kernel overwrites; θ partials
 // (λ, D, ε₄ centre, ε₄ faces) added to acc[4·wave + n]; `rows` rows per strip (at
 // least 2; 0 = the height that balances the waves over the device's SIMDs).  With acc == nullptr only *nwaves is set.  LGD_ERR_UNSUPPORTED for a
 // grid that is not 2-D or has fewer than 2 cells along a dimension.
+constexpr int64_t kAdjStreamPad = 64;
 int lgd_adjoint_stream_launch(lgd_ctx *ctx, const lgd_cart &c, const lgd_kr_consts &k, double dt,
                               const double *phi, const double *u, const double *APi,
                               const double *AUi, double *APo, double *AUo, double *acc,
