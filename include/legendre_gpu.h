@@ -104,8 +104,11 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
  * "modelc_strips" (row strips per block for the streaming kernel, 0 = default) and
  * "modelc_prefetch" (rows the streaming kernel loads ahead: 1 or 2, 0 = default) and
  * "amr_per_block" (1: the AMR level kernels launch once per patch instead of once per
- * level — the per-block dispatch baseline), "amr_tile" and "amr_wg" (the AMR launch
- * shapes lgd_sim_autotune_amr chooses; 0 = 64×4 tiles and 256-wide fills) and
+ * level — the per-block dispatch baseline), "amr_kernel" (the AMR Model C level
+ * stage: 0 or 2 streaming — one wavefront per strip of rows of up to 62 columns of a
+ * patch —, 1 one thread per cell in tiles), "amr_rows" (rows per strip of the
+ * streaming stage, 0 = spread over the device's SIMDs), "amr_tile" and "amr_wg" (the
+ * AMR launch shapes lgd_sim_autotune_amr chooses; 0 = 64×4 tiles and 256-wide fills) and
  * "amr_split_fill" (1: every substep of the
  * finest level fills its ghosts in a launch of its own; 0, the default: a substep's
  * stage launch also fills the ghosts the next substep reads).
@@ -399,10 +402,13 @@ int lgd_sim_regrid_stats(const lgd_sim *sim, int64_t *regrids, double *seconds);
 int64_t lgd_sim_step_index(const lgd_sim *sim);
 /* Autotuner for an AMR simulation's launches (no reference counterpart: the
  * reference has no launch shapes).  Times, on the simulation's current hierarchy and
- * state, every level-stage tile shape (the "amr_tile" knob: 64×4, 32×8, 16×16, 128×2,
- * 64×2, 32×4, 256×1 threads), every fill / restriction workgroup width ("amr_wg": 64
- * to 1024) and, for level 0 on the streaming kernel, 1, 2, 4 or 8 strips per block
- * with one or two rows loaded ahead, `reps` launches each, weighted by each level's
+ * state, the Model C level stage in its streaming form at 2 to 16 rows per strip
+ * (the "amr_rows" knob; 0 = the level's own height) and in its per-cell form at every
+ * tile shape ("amr_kernel" 1 with "amr_tile": 64×4, 32×8, 16×16, 128×2, 64×2, 32×4,
+ * 256×1 threads), every fill / restriction workgroup width ("amr_wg": 64 to 1024)
+ * and, for level 0 on the streaming kernel, one block or two per wavefront, 1, 2, 4
+ * or 8 strips per block with one or two rows loaded ahead, `reps` launches each,
+ * weighted by each level's
  * substeps per coarse step of `dt`; keeps the fastest in the context.  The timed
  * launches write only entries the next step overwrites before reading them (the
  * other buffer of a level, ghost entries), so the simulation's results are bitwise
@@ -410,10 +416,12 @@ int64_t lgd_sim_step_index(const lgd_sim *sim);
  * ms / default_ms: the kernel time per coarse step the timings predict, tuned and
  * untuned (64×4 tiles, 256-wide fills, the untuned level-0 shape). */
 typedef struct {
-  int32_t tile_x, tile_y;
+  int32_t tile_x, tile_y; /* the per-cell stage's tile, 0 when the streaming stage won */
   int32_t wg;
   int32_t level0_strips, level0_prefetch; /* 0 when level 0 is not on the streaming kernel */
   int32_t candidates;
+  int32_t stage_rows;    /* rows per strip of the streaming stage (finest level), 0 if per-cell */
+  int32_t level0_kernel; /* 2: one block per wavefront, 3: block pairs, 0: not streamed */
   double ms;
   double default_ms;
 } lgd_amr_tuning_result;

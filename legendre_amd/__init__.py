@@ -486,14 +486,16 @@ class Simulation:
         return int(n.value), sec
 
     def autotune_amr(self, dt: float, reps: int = 5) -> dict:
-        """Time the AMR launch shapes on the current hierarchy and state (level-stage
-        tiles, fill / restriction widths, level-0 strips) and keep the fastest in the
-        context; the simulation's results do not change."""
+        """Time the AMR launch shapes on the current hierarchy and state (streaming
+        level-stage strip heights and per-cell tiles, fill / restriction widths, level-0
+        kernel and strips) and keep the fastest in the context; the simulation's results
+        do not change."""
         r = _abi.AmrTuningResultC()
         _abi.check(self._lib.lgd_sim_autotune_amr(self.handle, dt, int(reps), C.byref(r)), self.ctx.handle)
-        return {"tile": (r.tile_x, r.tile_y), "wg": r.wg, "level0_strips": r.level0_strips,
-                "level0_prefetch": r.level0_prefetch, "candidates": r.candidates, "ms": r.ms,
-                "default_ms": r.default_ms}
+        return {"stage": "streaming" if r.stage_rows else "per-cell", "stage_rows": r.stage_rows,
+                "tile": (r.tile_x, r.tile_y), "wg": r.wg, "level0_kernel": r.level0_kernel,
+                "level0_strips": r.level0_strips, "level0_prefetch": r.level0_prefetch,
+                "candidates": r.candidates, "ms": r.ms, "default_ms": r.default_ms}
 
     def set_tagger(self, fn) -> None:
         """A caller's TagCells::tag_level in place of the policy's GradientTagger

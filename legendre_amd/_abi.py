@@ -97,6 +97,8 @@ class AmrTuningResultC(C.Structure):
         ("level0_strips", C.c_int32),
         ("level0_prefetch", C.c_int32),
         ("candidates", C.c_int32),
+        ("stage_rows", C.c_int32),
+        ("level0_kernel", C.c_int32),
         ("ms", C.c_double),
         ("default_ms", C.c_double),
     ]

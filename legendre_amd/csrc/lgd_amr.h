@@ -47,6 +47,20 @@ struct lgd_tile {
   int32_t block, x0, y0, pad;
 };
 
+// One wavefront's share of a level on the streaming level stage: rows [j0, j1) of
+// the interior columns [x0, x0 + w) of one patch (w <= 62: lanes 1..w hold them,
+// lanes 0 and w + 1 the columns either side).
+struct lgd_sseg {
+  int64_t off;     // the patch's ghost-1 slab offset
+  int32_t block;   // patch id (noise key)
+  int32_t nx;      // patch interior width (slab rows hold nx + 2)
+  int32_t x0, w;
+  int32_t j0, j1;
+};
+
+// Interior columns per wavefront of the streaming level stage.
+constexpr int kSegCols = 62;
+
 // One ghost cell of the fill plan: copy `src`, or prolong from corners
 // [cstart, cstart + clen).
 struct lgd_fill_op {
@@ -87,6 +101,19 @@ struct lgd_level_plan {
   int64_t ncorners = 0;
   lgd_restrict_item *ritems = nullptr;  // restriction of this level onto level-1
   int64_t nritems = 0;
+  // The streaming level stage: column segments of every patch cut into strips of
+  // seg_rows rows (patch-major; seg0 = first item of each patch, + end).
+  lgd_sseg *segs = nullptr;
+  int64_t nsegs = 0;
+  int32_t seg_rows = 0;
+  int32_t auto_rows = 0;  // lgd_amr_default_seg_rows of the level
+  std::vector<int64_t> seg0;
+  // The same in a tuned strip height other than seg_rows (allocated on first use).
+  mutable lgd_sseg *tsegs = nullptr;
+  mutable int64_t ntsegs = 0;
+  mutable int32_t trows = 0;
+  mutable std::vector<int64_t> tseg0;
+  mutable std::vector<lgd_sseg> tsegs_host;  // host copy kept until the plan is freed
   // The level-stage kernels' tiling in a tuned shape other than the plan's 64×4
   // (allocated on first use of that shape; the regrid kernels keep `tiles`).
   mutable lgd_tile *stiles = nullptr;
@@ -99,6 +126,14 @@ struct lgd_level_plan {
 // one cell per thread); shape 0 is the plans' own 64×4 tiling, which the regrid
 // kernels share.
 constexpr int kAmrTileShapes[][2] = {{64, 4}, {32, 8}, {16, 16}, {128, 2}, {64, 2}, {32, 4}, {256, 1}};
+
+// Rows per strip of the streaming level stage when no "amr_rows" knob is set: the
+// level's segment rows spread to about two wavefronts per SIMD of the device, 2 to
+// 32 rows per strip.
+int lgd_amr_default_seg_rows(const lgd_amr &g, int level);
+// Whether `level` steps through the streaming level stage (Model C, batched
+// launches, the "amr_kernel" knob not asking for the per-cell tiles).
+bool lgd_amr_level_streams(const lgd_ctx *ctx, const lgd_amr &g, int level);
 
 // Workgroup width of the AMR fill and restriction launches ("amr_wg"; 256 by default).
 inline int lgd_amr_wg(const lgd_ctx *ctx) { return ctx->amr_wg > 0 ? (int)ctx->amr_wg : 256; }
@@ -169,6 +204,10 @@ int lgd_amr_restrict_pp(lgd_ctx *ctx, const lgd_amr &g, int fine_level, int nf,
 // Whether level 0 steps through the Cartesian streaming kernel (which reads
 // neighbour interiors itself, so level 0 needs no ghost fill).
 bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g);
+// The untuned shape of level 0 on the streaming kernel: two x-neighbouring blocks
+// per wavefront where the base tiling allows it (blocks at most 64 wide leave half
+// of a wavefront idle otherwise), else one.
+lgd_launch_shape lgd_amr_level0_default_shape(const lgd_amr &g, bool noise);
 // The ghosts of the next substep of a level without children, written by the stage
 // launch that produces that substep's input: the parent's current and previous
 // buffers (fields 0, 1) and the next substep's α.  Bitwise the separate fill.

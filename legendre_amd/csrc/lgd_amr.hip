@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "lgd_amr.h"
+#include "lgd_stream_ops.h"
 
 using namespace lgd;
 
@@ -42,6 +43,26 @@ inline int64_t reflect_coord(int64_t c, int64_t lo, int64_t hi) {
 inline int64_t floor_div(int64_t a, int64_t b) {
   int64_t q = a / b;
   return (a % b < 0) ? q - 1 : q;
+}
+
+// The streaming level stage's items of `level`: each patch cut into balanced column
+// segments of at most kSegCols columns and strips of `rows` rows, patch-major.
+void build_segs(const lgd_amr &g, int level, int rows, std::vector<lgd_sseg> &out,
+                std::vector<int64_t> &seg0) {
+  out.clear();
+  seg0.clear();
+  for (uint32_t b = g.level_start[level]; b < g.level_start[level + 1]; ++b) {
+    seg0.push_back((int64_t)out.size());
+    const int64_t nx = g.box[b].extent(0), ny = g.box[b].extent(1);
+    const int64_t nseg = (nx + kSegCols - 1) / kSegCols;
+    for (int64_t s = 0; s < nseg; ++s) {
+      const int64_t x0 = s * nx / nseg, x1 = (s + 1) * nx / nseg;
+      for (int64_t j0 = 0; j0 < ny; j0 += rows)
+        out.push_back(lgd_sseg{g.off[b], (int32_t)b, (int32_t)nx, (int32_t)x0, (int32_t)(x1 - x0),
+                               (int32_t)j0, (int32_t)std::min<int64_t>(j0 + rows, ny)});
+    }
+  }
+  seg0.push_back((int64_t)out.size());
 }
 
 // Several host arrays uploaded as one device allocation (256-byte aligned parts),
@@ -135,9 +156,19 @@ void lgd_amr::free_device() {
   for (auto &p : plans) {
     if (p.block) (void)hipFreeAsync(p.block, st);
     if (p.stiles) (void)hipFreeAsync(p.stiles, st);
+    if (p.tsegs) (void)hipFreeAsync(p.tsegs, st);
   }
   plans.clear();
   staging.clear();
+}
+
+int lgd_amr_default_seg_rows(const lgd_amr &g, int level) {
+  int64_t rows = 0;
+  for (uint32_t b = g.level_start[level]; b < g.level_start[level + 1]; ++b)
+    rows += (g.box[b].extent(0) + kSegCols - 1) / kSegCols * g.box[b].extent(1);
+  const int64_t cus = g.ctx && g.ctx->num_cus > 0 ? g.ctx->num_cus : 256;
+  const int64_t waves = 2 * 4 * cus;
+  return (int)std::min<int64_t>(32, std::max<int64_t>(2, (rows + waves - 1) / waves));
 }
 
 // AmrGrid::from_patches (src/geometry/amr/grid.rs:90-196): level 0 is the base
@@ -414,6 +445,12 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
         }
       }
     }
+    // Streaming level-stage items at the level's default strip height.
+    std::vector<lgd_sseg> segs;
+    P.auto_rows = lgd_amr_default_seg_rows(g, l);
+    P.seg_rows = ctx->amr_rows > 0 ? (int32_t)ctx->amr_rows : P.auto_rows;
+    build_segs(g, l, P.seg_rows, segs, P.seg0);
+    P.nsegs = (int64_t)segs.size();
     P.ntiles = (int64_t)tiles.size();
     P.nops = (int64_t)ops.size();
     P.ncorners = (int64_t)corners.size();
@@ -423,6 +460,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     pk.add(ops, &P.ops);
     pk.add(corners, &P.corners);
     pk.add(items, &P.ritems);
+    pk.add(segs, &P.segs);
     if ((rc = pk.upload(ctx, &P.block, g.staging))) return rc;
   }
   return LGD_OK;
@@ -636,6 +674,149 @@ __global__ __launch_bounds__(kTX *kTY) void amr_modelc_step_kernel(
   }
 }
 
+// The streaming form of the fused Model C stage: one wavefront advances rows
+// [j0, j1) of one column segment of one patch, lane L holding slab column x0 + L
+// (interior column x0 + L - 1; lanes 1..w are the segment's cells, lane 0 and lane
+// w + 1 the columns either side, which are ghost cells or another segment's
+// cells).  Every operand comes from the patch's own ghost-filled slab.  The wave
+// marches up the strip with rows j-1, j, j+1 of both fields in registers while the
+// loads of the row kDepth ahead are in flight; each x-face flux is evaluated once,
+// by the lane left of it, and reaches its right neighbour by a DPP shift, and each
+// y-face flux once, as the top face of one row and the bottom face of the next.
+// These are the operands, in the order, of the per-cell evaluation
+// (src/discretization/finite_volume.rs:261-280; src/physics/phasefield.rs:395-421),
+// so the results are bitwise those of amr_modelc_step_kernel.  With kFill,
+// workgroups from nwg_seg on write the next substep's ghosts as that kernel does.
+constexpr int kSegWaves = 4;
+
+struct SegArgs {
+  const lgd_sseg *segs;
+  int64_t nsegs;
+  int32_t nwg_seg;  // workgroups of the streaming part
+  lgd_kr_consts k;
+  double dt;
+  const double *phi;
+  const double *u;
+  double *phi_n;
+  double *u_n;
+  double noise_scale;
+  uint64_t seed;
+  uint64_t salt;
+  const double *salt_src;
+};
+
+template <bool kNoise, bool kFill, int kDepth>
+__global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
+    SegArgs a, const lgd_block_desc *__restrict__ blocks, NextFill N) {
+  using namespace lgd_stream;
+  if (kFill && (int)blockIdx.x >= a.nwg_seg) {
+    const int64_t t = (int64_t)(blockIdx.x - a.nwg_seg) * blockDim.x + threadIdx.x;
+    if (t >= N.nops) return;
+    const uint64_t st = kNoise ? lgd_salt(a.salt, a.salt_src) : 0;
+    const lgd_fill_op op = N.ops[t];
+    if (op.clen == 0) {
+      const lgd_block_desc bd = blocks[op.cstart];
+      int i, j;
+      slab_coords(bd, op.src, i, j);
+      double pn, un;
+      modelc_cell_stage<kNoise>(a.k, a.dt, a.phi, a.u, op.src, bd.nx + 2, a.noise_scale, a.seed,
+                                st, op.cstart, i + (int64_t)j * bd.nx, pn, un);
+      a.phi_n[op.target] = pn;
+      a.u_n[op.target] = un;
+    } else {
+      a.phi_n[op.target] = prolong_blend(N, op, 0);
+      a.u_n[op.target] = prolong_blend(N, op, 1);
+    }
+    return;
+  }
+  // XCD-aware order: XCD x (= blockIdx.x % 8) takes one contiguous range of items,
+  // so the strips of a segment, which share their halo rows, meet in one L2.
+  const int wg = blockIdx.x;
+  const int xcd = wg % 8, idx = wg / 8;
+  const int per = a.nwg_seg / 8, rem = a.nwg_seg % 8;
+  const int lw = xcd * per + min(xcd, rem) + idx;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t item = (int64_t)lw * kSegWaves + wave;
+  if (item >= a.nsegs) return;  // whole wave exits together
+  const lgd_sseg S = load_uniform(a.segs + item);
+  const lgd_kr_consts &k = a.k;
+  const int64_t sx = (int64_t)S.nx + 2;
+  // Lanes past w + 1 re-read column w + 1: never stored, never a stored cell's
+  // neighbour (w <= 62, so lane 63 is at most column w + 1).
+  const int64_t cbase = S.off + S.x0 + min(lane, S.w + 1);
+  const bool in = lane >= 1 && lane <= S.w;
+  const int j0 = S.j0, j1 = S.j1;
+  auto ld = [&](const double *f, int r) { return f[cbase + (int64_t)(r + 1) * sx]; };
+
+  uint64_t block_key = 0;
+  if (kNoise)
+    block_key = lgd_mix_key3(a.seed, lgd_salt(a.salt, a.salt_src), (1ull << 32) | 0ull,
+                             (uint64_t)S.block);
+  const uint64_t kc = (uint64_t)(S.x0 + lane - 1);  // interior column (cell key)
+
+  double pm = ld(a.phi, j0 - 1), pc = ld(a.phi, j0), pp = ld(a.phi, j0 + 1);
+  double um = ld(a.u, j0 - 1), uc = ld(a.u, j0), up = ld(a.u, j0 + 1);
+  const double q = 0.25;
+  double pmR = from_right(pm);
+  double pcL = from_left(pc), pcR = from_right(pc);
+  double ppL = from_left(pp), ppR = from_right(pp);
+  double ucL = from_left(uc), ucR = from_right(uc);
+  // Flux through the bottom face of row j0 (the top face of row j0 - 1).
+  double jb = flux_y(k, q * (((pcR - pcL) + pmR) - from_left(pm)), pc - pm);
+
+  // At row j the loads of row j + 1 + kDepth are issued; the row that becomes the
+  // top neighbour of row j + 1 is moved into place after row j is computed (two
+  // alternating buffers for kDepth 2, so no register with a load in flight is
+  // copied).
+  auto body = [&](int j, double &ip, double &iu, double &cp, double &cu) {
+    const int rn = min(j + 1 + kDepth, j1);
+    ip = ld(a.phi, rn);
+    iu = ld(a.u, rn);
+    const double jx = flux_x(k, pcR - pc, q * (((ppR - pmR) + pp) - pm));
+    const double jxl = from_left(jx);
+    const double jt = flux_y(k, q * (((ppR - ppL) + pcR) - pcL), pp - pc);
+    const double f = (((jx - jxl) + jt) - jb) * k.inv_h2;
+    double n, v, d, e;
+    cell_update(k, f, pc, pcR, pcL, pp, pm, uc, ucR, ucL, up, um, a.dt, n, v, d, e);
+    if (kNoise && in)
+      n = n + a.noise_scale *
+                  lgd_standard_normal(lgd_splitmix64(block_key ^ ((uint64_t)j * (uint64_t)S.nx + kc)));
+    const int64_t o = cbase + (int64_t)(j + 1) * sx;
+    jb = jt;
+    if (j + 1 < j1) {
+      pm = pc; pmR = pcR;
+      pc = pp; pcL = ppL; pcR = ppR;
+      pp = cp;
+      ppL = from_left(pp);
+      ppR = from_right(pp);
+      um = uc;
+      uc = up;
+      up = cu;
+      ucL = from_left(uc);
+      ucR = from_right(uc);
+    }
+    if (in) {
+      a.phi_n[o] = n;
+      a.u_n[o] = v;
+    }
+  };
+  double ap = 0.0, au = 0.0, bp = 0.0, bu = 0.0;
+  if (kDepth == 2) {
+    ap = ld(a.phi, min(j0 + 2, j1));
+    au = ld(a.u, min(j0 + 2, j1));
+  }
+  for (int j = j0; j < j1; j += 2) {
+    if (kDepth == 2) {
+      body(j, bp, bu, ap, au);
+      if (j + 1 < j1) body(j + 1, ap, au, bp, bu);
+    } else {
+      body(j, ap, au, ap, au);
+      if (j + 1 < j1) body(j + 1, ap, au, ap, au);
+    }
+  }
+}
+
 // One fused Heat stage of a level, with the next substep's ghosts as above.
 template <bool kFill>
 __global__ __launch_bounds__(kTX *kTY) void amr_heat_step_kernel(
@@ -730,6 +911,17 @@ bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g) {
   return ctx->modelc_kernel != 1 && g.base.n[0] + 2 <= 128;
 }
 
+lgd_launch_shape lgd_amr_level0_default_shape(const lgd_amr &g, bool noise) {
+  lgd_launch_shape s = lgd_modelc_default_shape(noise);
+  if (lgd_stream_pairable(g.base.n[0], g.base.nb[0])) s.kernel = 3;
+  return s;
+}
+
+bool lgd_amr_level_streams(const lgd_ctx *ctx, const lgd_amr &g, int level) {
+  if (ctx->amr_kernel == 1) return false;
+  return !(level == 0 && lgd_amr_level0_streams(ctx, g));
+}
+
 namespace {
 
 constexpr int kNumTileShapes = (int)(sizeof(kAmrTileShapes) / sizeof(kAmrTileShapes[0]));
@@ -788,6 +980,48 @@ int stage_tiles(lgd_ctx *ctx, const lgd_amr &g, int level, StageTiles &T) {
   return LGD_OK;
 }
 
+struct StageSegs {
+  const lgd_sseg *segs = nullptr;
+  int64_t n = 0;
+  const std::vector<int64_t> *seg0 = nullptr;
+};
+
+// The streaming level stage's items of `level` at the context's strip height: the
+// plan's own, or a recut kept with the plan until the height changes.
+int stage_segs(lgd_ctx *ctx, const lgd_amr &g, int level, StageSegs &T) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  const int rows = ctx->amr_rows > 0 ? (int)ctx->amr_rows : P.auto_rows;
+  if (rows == P.seg_rows) {
+    T.segs = P.segs;
+    T.n = P.nsegs;
+    T.seg0 = &P.seg0;
+    return LGD_OK;
+  }
+  if (P.trows != rows) {
+    // The previous recut's upload may still read its host copy.
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return hip_fail(ctx, e, "amr stage items");
+    build_segs(g, level, rows, P.tsegs_host, P.tseg0);
+    if (P.tsegs) (void)hipFreeAsync(P.tsegs, ctx->stream);
+    P.tsegs = nullptr;
+    P.ntsegs = 0;
+    P.trows = 0;
+    if (!P.tsegs_host.empty()) {
+      const size_t bytes = P.tsegs_host.size() * sizeof(lgd_sseg);
+      e = hipMallocAsync((void **)&P.tsegs, bytes, ctx->stream);
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(P.tsegs, P.tsegs_host.data(), bytes, hipMemcpyHostToDevice, ctx->stream);
+      if (e != hipSuccess) return hip_fail(ctx, e, "amr stage items");
+    }
+    P.ntsegs = (int64_t)P.tsegs_host.size();
+    P.trows = rows;
+  }
+  T.segs = P.tsegs;
+  T.n = P.ntsegs;
+  T.seg0 = &P.tseg0;
+  return LGD_OK;
+}
+
 // The next-fill arguments of a stage launch (nops = 0: none).
 NextFill next_fill_args(const lgd_level_plan &P, const lgd_amr_next_fill *nf, int nfields) {
   NextFill N{};
@@ -821,7 +1055,8 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
     // reads neighbour interiors itself (no ghost fill) — the same values and order.
     const int64_t nb0 = g.level_start[1];
     const lgd_launch_shape shape = lgd_modelc_shape(ctx, g.base.n[0], g.base.n[1], nb0, noise,
-                                                    lgd_modelc_default_shape(noise));
+                                                    lgd_amr_level0_default_shape(g, noise));
+    const bool pair = shape.kernel == 3 && lgd_stream_pairable(g.base.n[0], g.base.nb[0]);
     lgd_domain d0;
     d0.k = k;
     d0.noise_scale = noise ? scale : 0.0;
@@ -843,10 +1078,56 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
     }
     return lgd_modelc_stream_launch(ctx, g.base.n[0], g.base.n[1], d0, nullptr, nullptr, 0, nb0, phi,
                                     u, phi_n, u_n, dt, noise, salt, wiener0, shape.strips,
-                                    shape.prefetch);
+                                    shape.prefetch, pair);
+  }
+  int rc;
+  if (lgd_amr_level_streams(ctx, g, level)) {
+    StageSegs T;
+    if ((rc = stage_segs(ctx, g, level, T))) return rc;
+    const bool fill = nf && P.nops;
+    if (T.n == 0 && !fill) return LGD_OK;
+    if (ctx->amr_per_block && nf)
+      return fail(ctx, LGD_ERR_UNSUPPORTED, "per-patch launches fill ghosts in a launch of their own");
+    SegArgs A{};
+    A.k = k;
+    A.dt = dt;
+    A.phi = phi;
+    A.u = u;
+    A.phi_n = phi_n;
+    A.u_n = u_n;
+    A.noise_scale = noise ? scale : 0.0;
+    A.seed = seed;
+    A.salt = salt;
+    A.salt_src = noise ? ctx->salt_src : nullptr;
+    const NextFill N = fill ? next_fill_args(P, nf, 2) : NextFill{};
+    auto go = [&](const lgd_sseg *segs, int64_t n, bool with_fill) -> int {
+      const int64_t nwg = (n + kSegWaves - 1) / kSegWaves;
+      const int64_t total = nwg + (with_fill ? (P.nops + 64 * kSegWaves - 1) / (64 * kSegWaves) : 0);
+      if (total == 0) return LGD_OK;
+      if (total > INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "AMR level too large for one launch");
+      A.segs = segs;
+      A.nsegs = n;
+      A.nwg_seg = (int32_t)nwg;
+      const dim3 grid((unsigned)total), blk(64 * kSegWaves);
+      hipStream_t st = ctx->stream;
+      if (noise) {
+        if (with_fill) amr_stream_kernel<true, true, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
+        else amr_stream_kernel<true, false, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
+      } else {
+        if (with_fill) amr_stream_kernel<false, true, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
+        else amr_stream_kernel<false, false, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
+      }
+      return LGD_OK;
+    };
+    if (ctx->amr_per_block) {
+      for (size_t b = 0; b + 1 < T.seg0->size(); ++b)
+        if ((rc = go(T.segs + (*T.seg0)[b], (*T.seg0)[b + 1] - (*T.seg0)[b], false))) return rc;
+    } else if ((rc = go(T.segs, T.n, fill))) {
+      return rc;
+    }
+    return launch_check(ctx, "amr_stream");
   }
   StageTiles T;
-  int rc;
   if ((rc = stage_tiles(ctx, g, level, T))) return rc;
   if (T.n == 0 && !(nf && P.nops)) return LGD_OK;
   const NextFill none{};

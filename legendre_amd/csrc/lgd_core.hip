@@ -154,6 +154,11 @@ int lgd_ctx_set_tuning(lgd_ctx *ctx, const char *key, int64_t value) {
     if (value != 0 && value != 64 && value != 128 && value != 256 && value != 512 && value != 1024)
       return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_wg is 0, 64, 128, 256, 512 or 1024");
     ctx->amr_wg = value;
+  } else if (k == "amr_kernel") {
+    if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "amr_kernel is 0, 1 or 2");
+    ctx->amr_kernel = value;
+  } else if (k == "amr_rows") {
+    ctx->amr_rows = value;
   } else if (k == "adjoint_kernel") {
     if (value > 2) return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "adjoint_kernel is 0, 1 or 2");
     ctx->adjoint_kernel = value;

@@ -39,6 +39,8 @@ struct lgd_ctx {
   int64_t amr_split_fill = 0;   // 1 = the finest level's ghost fills in launches of their own
   int64_t amr_tile = 0;         // AMR level-stage tile shape (0 = the plans' 64×4)
   int64_t amr_wg = 0;           // AMR fill / restriction workgroup width (0 = 256)
+  int64_t amr_kernel = 0;       // AMR Model C level stage: 0/2 = streaming, 1 = per-cell tiles
+  int64_t amr_rows = 0;         // rows per strip of the streaming level stage (0 = by level)
   int64_t adjoint_kernel = 0;   // 1 = three-pass reverse step, 2 = streaming required
   int64_t adjoint_rows = 0;     // rows per strip of the streaming reverse step (0 = by device)
   int64_t regrid_host = 0;      // 1 = AMR regrid tags and migrates on the host

@@ -1034,13 +1034,24 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
   };
   const bool streams0 = modelc && lgd_amr_level0_streams(ctx, g);
   const int64_t k0 = ctx->modelc_kernel, s0 = ctx->modelc_strips, p0 = ctx->modelc_prefetch;
-  const int64_t t0 = ctx->amr_tile, w0 = ctx->amr_wg;
+  const int64_t t0 = ctx->amr_tile, w0 = ctx->amr_wg, ak0 = ctx->amr_kernel, r0 = ctx->amr_rows;
   int32_t candidates = 0;
-  // Level-stage tiles (the levels the per-cell stage runs on).
+  // Level stages (the levels not on the Cartesian streaming kernel): the streaming
+  // stage at several strip heights (Model C), then the per-cell stage's tile shapes.
+  struct StageShape {
+    int kernel, rows, tile;
+  };
+  std::vector<StageShape> stage_cand;
+  if (modelc)
+    for (int rows : {0, 2, 3, 4, 6, 8, 12, 16}) stage_cand.push_back({2, rows, 0});
+  for (int sh = 0; sh <= 6; ++sh) stage_cand.push_back({1, 0, sh});
   double best_tiles = 1e300, dflt_tiles = 0.0;
-  int best_tile = 0;
-  for (int sh = 0; sh <= 6 && !rc; ++sh) {
-    ctx->amr_tile = sh;
+  StageShape best_stage = stage_cand.front();
+  for (size_t c = 0; c < stage_cand.size() && !rc; ++c) {
+    const StageShape &sc = stage_cand[c];
+    ctx->amr_kernel = sc.kernel;
+    ctx->amr_rows = sc.rows;
+    ctx->amr_tile = sc.tile;
     double tot = 0.0;
     for (int l = streams0 ? 1 : 0; l < L && !rc; ++l) {
       double ms = 0.0;
@@ -1048,12 +1059,15 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
       tot += N[(size_t)l] * ms;
     }
     ++candidates;
-    if (sh == 0) dflt_tiles = tot;
+    if (c == 0) dflt_tiles = tot;
     if (tot < best_tiles) {
       best_tiles = tot;
-      best_tile = sh;
+      best_stage = sc;
     }
   }
+  ctx->amr_kernel = ak0;
+  ctx->amr_rows = r0;
+  ctx->amr_tile = t0;
   // Fill and restriction widths.
   double best_wg_ms = 1e300, dflt_wg = 0.0;
   int best_wg = 256;
@@ -1080,28 +1094,31 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
       best_wg = w;
     }
   }
-  // Level 0 on the streaming kernel: strips and rows loaded ahead.
-  lgd_launch_shape best0 = lgd_modelc_default_shape(modelc && sim->wiener && sim->params.noise_amplitude != 0.0);
-  double best0_ms = 0.0, dflt0_ms = 0.0;
+  // Level 0 on the streaming kernel: one block or two per wavefront, strips and rows
+  // loaded ahead.
   const bool noise0 = modelc && sim->wiener != 0 && sim->params.noise_amplitude != 0.0;
+  lgd_launch_shape best0 = lgd_amr_level0_default_shape(g, noise0);
+  double best0_ms = 0.0, dflt0_ms = 0.0;
   if (streams0 && !rc) {
-    const lgd_launch_shape d = lgd_modelc_default_shape(noise0);
+    const lgd_launch_shape d = lgd_amr_level0_default_shape(g, noise0);
+    const bool pairable = lgd_stream_pairable(g.base.n[0], g.base.nb[0]);
     best0_ms = 1e300;
     const int strips[] = {1, 2, 4, 8};
-    for (int st : strips)
-      for (int pf = 1; pf <= 2 && !rc; ++pf) {
-        ctx->modelc_kernel = 2;
-        ctx->modelc_strips = st;
-        ctx->modelc_prefetch = pf;
-        double ms = 0.0;
-        rc = timed([&] { return stage(0); }, &ms);
-        ++candidates;
-        if (st == d.strips && pf == d.prefetch) dflt0_ms = ms;
-        if (ms < best0_ms) {
-          best0_ms = ms;
-          best0 = lgd_launch_shape{2, st, pf};
+    for (int kern = 2; kern <= (pairable ? 3 : 2); ++kern)
+      for (int st : strips)
+        for (int pf = 1; pf <= 2 && !rc; ++pf) {
+          ctx->modelc_kernel = kern;
+          ctx->modelc_strips = st;
+          ctx->modelc_prefetch = pf;
+          double ms = 0.0;
+          rc = timed([&] { return stage(0); }, &ms);
+          ++candidates;
+          if (kern == d.kernel && st == d.strips && pf == d.prefetch) dflt0_ms = ms;
+          if (ms < best0_ms) {
+            best0_ms = ms;
+            best0 = lgd_launch_shape{kern, st, pf};
+          }
         }
-      }
   }
   ctx->modelc_kernel = k0;
   ctx->modelc_strips = s0;
@@ -1111,16 +1128,28 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
   if (rc) {
     ctx->amr_tile = t0;
     ctx->amr_wg = w0;
+    ctx->amr_kernel = ak0;
+    ctx->amr_rows = r0;
     return rc;
   }
-  ctx->amr_tile = best_tile;
+  ctx->amr_kernel = best_stage.kernel == 1 ? 1 : 0;
+  ctx->amr_rows = best_stage.kernel == 1 ? 0 : best_stage.rows;
+  ctx->amr_tile = best_stage.kernel == 1 ? best_stage.tile : 0;
   ctx->amr_wg = best_wg;
   if (streams0)
     ctx->tuned[std::make_tuple(g.base.n[0], g.base.n[1], (int64_t)g.level_start[1], noise0 ? 1 : 0)] =
         best0;
   ++ctx->tuning_epoch;  // captured graphs hold the old shapes
-  out->tile_x = kAmrTileShapes[best_tile][0];
-  out->tile_y = kAmrTileShapes[best_tile][1];
+  out->stage_rows = 0;
+  if (best_stage.kernel == 1) {
+    out->tile_x = kAmrTileShapes[best_stage.tile][0];
+    out->tile_y = kAmrTileShapes[best_stage.tile][1];
+  } else {
+    out->tile_x = out->tile_y = 0;
+    // The strip height the finest streamed level runs at.
+    out->stage_rows = best_stage.rows > 0 ? best_stage.rows : (L > 1 ? g.plans[(size_t)L - 1].auto_rows : 0);
+  }
+  out->level0_kernel = streams0 ? best0.kernel : 0;
   out->wg = best_wg;
   out->level0_strips = streams0 ? best0.strips : 0;
   out->level0_prefetch = streams0 ? best0.prefetch : 0;

@@ -60,3 +60,45 @@ def test_library_links_no_oracle_code():
     out = subprocess.run(["nm", "-D", str(_abi.LIB_PATH)], capture_output=True, text=True,
                          check=True).stdout
     assert "orc_" not in out
+
+
+# ctypes mirror of each struct the header declares.
+STRUCTS = {
+    "lgd_cartesian_desc": "CartesianDesc",
+    "lgd_face_bc": "FaceBcC",
+    "lgd_modelc_params": "ModelCParamsC",
+    "lgd_regrid_policy": "RegridPolicyC",
+    "lgd_realization": "RealizationC",
+    "lgd_tuning_result": "TuningResultC",
+    "lgd_amr_tuning_result": "AmrTuningResultC",
+    "lgd_observed_field": "ObservedFieldC",
+    "lgd_driver": "DriverC",
+}
+
+
+def test_ctypes_structs_match_the_header_layout(tmp_path):
+    """Every struct of the header has a ctypes mirror with the same size and field
+    offsets as the C compiler lays it out."""
+    from legendre_amd import _abi
+
+    lines = ['#include <stddef.h>', '#include <stdio.h>', f'#include "{HEADER}"', "int main(void) {"]
+    for cname, pyname in STRUCTS.items():
+        st = getattr(_abi, pyname)
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for field, _ in st._fields_:
+            # a Python keyword in C (lambda) is mirrored with a trailing underscore
+            lines.append(f'  printf("{cname} {field} %zu\\n", offsetof({cname}, {field.rstrip("_")}));')
+    lines.append("  return 0;\n}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    res = subprocess.run(["gcc", "-std=c11", "-I", str(ROOT / "include"), str(src),
+                          "-o", str(exe)], capture_output=True, text=True)
+    assert res.returncode == 0, res.stderr
+    got = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    c = {tuple(line.split()[:2]): int(line.split()[2]) for line in got if line}
+    for cname, pyname in STRUCTS.items():
+        st = getattr(_abi, pyname)
+        assert c[(cname, "size")] == __import__("ctypes").sizeof(st), cname
+        for field, _ in st._fields_:
+            assert c[(cname, field)] == getattr(st, field).offset, (cname, field)

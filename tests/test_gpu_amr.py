@@ -99,12 +99,28 @@ def test_heat_on_static_hierarchy_matches_oracle_bitwise(ctx, orc, name, subcycl
     assert np.array_equal(sim_g.field(0).download()[m], sim_o.field(0)[m])
 
 
+# Static Model C hierarchies: (base cells, base block, level-1 boxes, seeds (x, y, r)).
+# "wide" has level-1 patches 160 and 64 cells wide, which the streaming level stage
+# cuts into three and two column segments.
+MODELC_LAYOUTS = {
+    "small": ([48, 48], [24, 24], [[((8, 8), (40, 32)), ((40, 8), (56, 24))]], [(6.0, 5.0, 3.0)]),
+    "wide": ([96, 64], [48, 32], [[((8, 8), (168, 40)), ((8, 48), (72, 120))]],
+             [(12.0, 5.0, 3.0), (6.0, 14.0, 2.0)]),
+}
+
+
 @pytest.mark.parametrize("subcycle", [False, True])
 @pytest.mark.parametrize("noise", [0.0, 0.02])
-def test_modelc_on_static_hierarchy_matches_oracle(ctx, orc, subcycle, noise):
-    say(f"amr modelc subcycle={subcycle} noise={noise}")
-    cells, block, h = [48, 48], [24, 24], 0.4
-    levels = [[((8, 8), (40, 32)), ((40, 8), (56, 24))]]
+@pytest.mark.parametrize("layout", list(MODELC_LAYOUTS))
+@pytest.mark.parametrize("form", ["streaming", "per-cell"])
+def test_modelc_on_static_hierarchy_matches_oracle(ctx, orc, subcycle, noise, layout, form):
+    """Model C on a static two-level hierarchy against the oracle, through the
+    streaming level stage and the per-cell one: bitwise without noise, within 1e-10
+    with it (the noise draws agree to a few ulp)."""
+    say(f"amr modelc subcycle={subcycle} noise={noise} layout={layout} form={form}")
+    cells, block, levels, seeds = MODELC_LAYOUTS[layout]
+    h = 0.4
+    ctx.set_tuning("amr_kernel", 1 if form == "per-cell" else 0)
     og = orc.AmrGrid.from_patches(orc.grid(cells, block, [0, 0], [h, h]), [2], levels)
     gg = lg.AmrGrid.from_patches(ctx, cells, block, [0, 0], [h, h], [2], levels)
     p = orc.params(0.05, 3.19, 2.0, noise)
@@ -118,7 +134,8 @@ def test_modelc_on_static_hierarchy_matches_oracle(ctx, orc, subcycle, noise):
         for j in range(ny):
             for i in range(nx):
                 x, y = gr.cell_center(b, i, j)
-                vp[j + 1, i + 1] = -math.tanh((math.hypot(x - 6.0, y - 5.0) - 3.0) / math.sqrt(2.0))
+                d = min(math.hypot(x - sx, y - sy) - r for sx, sy, r in seeds)
+                vp[j + 1, i + 1] = -math.tanh(d / math.sqrt(2.0))
                 vu[j + 1, i + 1] = -0.55
     sim_g = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.05, 3.19, 2.0, noise),
                               wiener=noise > 0, seed=11, subcycle=subcycle)
@@ -126,7 +143,10 @@ def test_modelc_on_static_hierarchy_matches_oracle(ctx, orc, subcycle, noise):
     sim_g.field(1).upload(u)
     dt = 0.2 * (h if subcycle else h / 2) ** 2 / 2.0
     sim_o.step(dt, 20)
-    sim_g.step(dt, 20)
+    try:
+        sim_g.step(dt, 20)
+    finally:
+        ctx.set_tuning("amr_kernel", 0)
     gp, gu = sim_g.field(0).download(), sim_g.field(1).download()
     m = H.amr_interior_mask(sim_g.amr)  # ghost entries are scratch after a step
     if noise == 0.0:

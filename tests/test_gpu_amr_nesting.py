@@ -334,10 +334,12 @@ def test_fused_next_fill_matches_split_fill_bitwise(ctx, orc, model):
 
 
 def test_amr_launch_shapes_change_no_bit(ctx, orc):
-    """Every level-stage tile shape, fill / restriction width and the autotuner's
-    choice give the bits of the default shapes: three-level adaptive subcycled Model
-    C with noise, hierarchy and fields after every step; the autotuner leaves the
-    simulation it times bitwise unchanged."""
+    """Every level-stage form (streaming at several strip heights, per-cell at several
+    tile shapes, per-patch launches), fill / restriction width, level-0 kernel (one
+    block or two per wavefront) and the autotuner's choice give the bits of the
+    default shapes: three-level adaptive subcycled Model C with noise, hierarchy and
+    fields after every step; the autotuner leaves the simulation it times bitwise
+    unchanged."""
     say("amr launch shapes")
     h, n, blk = 0.4, 96, 24
     base = orc.grid([n, n], [blk, blk], [0, 0], [h, h])
@@ -367,8 +369,12 @@ def test_amr_launch_shapes_change_no_bit(ctx, orc):
         sim.close()
         return trace
 
-    shapes = [{}, {"amr_tile": 1, "amr_wg": 64}, {"amr_tile": 2, "amr_wg": 1024},
-              {"amr_tile": 6, "amr_wg": 128}, {"amr_tile": 3, "modelc_strips": 1, "modelc_prefetch": 1}]
+    shapes = [{}, {"amr_kernel": 1, "amr_tile": 1, "amr_wg": 64},
+              {"amr_kernel": 1, "amr_tile": 2, "amr_wg": 1024},
+              {"amr_kernel": 1, "amr_tile": 6, "modelc_kernel": 3},
+              {"amr_rows": 2, "amr_wg": 128}, {"amr_rows": 5, "modelc_kernel": 2},
+              {"amr_rows": 16, "modelc_kernel": 2, "modelc_strips": 1, "modelc_prefetch": 1},
+              {"amr_per_block": 1}, {"amr_per_block": 1, "amr_kernel": 1}]
     try:
         ref = run({})
         assert any(p[0] == 2 for tr in ref for p in tr[0])
@@ -382,5 +388,6 @@ def test_amr_launch_shapes_change_no_bit(ctx, orc):
         for s, (a, b) in enumerate(zip(ref, got)):
             assert a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2]), ("tuned", s)
     finally:
-        for k in ("amr_tile", "amr_wg", "modelc_strips", "modelc_prefetch"):
+        for k in ("amr_tile", "amr_wg", "amr_kernel", "amr_rows", "amr_per_block", "modelc_kernel",
+                  "modelc_strips", "modelc_prefetch"):
             ctx.set_tuning(k, 0)
