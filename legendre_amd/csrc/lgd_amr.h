@@ -156,16 +156,16 @@ struct lgd_amr {
   static constexpr int64_t kBucket = 16;
   std::vector<bucket_index> lookup;
   std::vector<int64_t> off;           // ghost-1 slab offsets, nblocks + 1
-  void *d_arena = nullptr;  // one device allocation: d_blocks and the patch index
+  void *d_arena = nullptr;  // one device allocation: the patch index and the plans of levels >= 1
   lgd_block_desc *d_blocks = nullptr;
   std::vector<lgd_level_plan> plans;  // per populated level
-  std::vector<std::vector<char>> staging;  // host bytes of the device arrays' uploads
   // find_patch on the device: per-level headers, concatenated bucket starts (into
   // the concatenated ids), patch ids, and each patch's lower corner (x, y).
   lgd_dev_level *d_levels = nullptr;
   uint32_t *d_bstart = nullptr;
   uint32_t *d_bids = nullptr;
   int64_t *d_lo = nullptr;
+  int32_t *d_err = nullptr;  // set by the device plan builders when a ghost has no donor
 
   int num_levels() const { return (int)level_start.size() - 1; }
   int max_levels() const { return (int)ratios.size() + 1; }
@@ -175,6 +175,31 @@ struct lgd_amr {
   void free_device();
 };
 
+// find_patch on the device (the hierarchy's d_levels / d_bstart / d_bids / d_lo /
+// d_blocks): the patch of level l holding cell (x, y), -1 if none.
+struct lgd_dev_index {
+  const lgd_dev_level *levels;
+  const uint32_t *bstart;
+  const uint32_t *bids;
+  const int64_t *lo;
+  const lgd_block_desc *blocks;
+  int nlevels;
+};
+
+__device__ inline int64_t lgd_dev_find_patch(const lgd_dev_index &ix, int l, int64_t x, int64_t y) {
+  if (l >= ix.nlevels) return -1;
+  const lgd_dev_level L = ix.levels[l];
+  if (x < 0 || y < 0 || x >= L.w || y >= L.h) return -1;
+  const int64_t k = L.start0 + x / lgd_amr::kBucket + L.nbx * (y / lgd_amr::kBucket);
+  for (uint32_t i = ix.bstart[k]; i < ix.bstart[k + 1]; ++i) {
+    const uint32_t p = ix.bids[i];
+    const int64_t px = ix.lo[2 * p], py = ix.lo[2 * p + 1];
+    const lgd_block_desc bd = ix.blocks[p];
+    if (x >= px && x < px + bd.nx && y >= py && y < py + bd.ny) return (int64_t)p;
+  }
+  return -1;
+}
+
 // `reuse0`: a hierarchy on the same base whose level-0 plan (which depends on the
 // base tiling alone) g will take over instead of building its own (a regrid's old
 // hierarchy): g's level-0 plan is then left empty until lgd_amr_take_level0_plan,
@@ -183,6 +208,10 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
                   const std::vector<std::vector<lgd_box>> &levels, lgd_amr &g,
                   const lgd_amr *reuse0 = nullptr);
 int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0 = nullptr);
+// Queues the copy of the device plan builders' error flag into *host_flag on the
+// context's stream (valid once the stream has synchronised): nonzero when a ghost
+// cell of the hierarchy found no donor.
+int lgd_amr_plan_error(lgd_ctx *ctx, const lgd_amr &g, int32_t *host_flag);
 void lgd_amr_take_level0_plan(lgd_amr &g, lgd_amr &from);
 
 // Device transfers and level kernels (lgd_amr.hip).

@@ -66,57 +66,121 @@ void build_segs(const lgd_amr &g, int level, int rows, std::vector<lgd_sseg> &ou
 }
 
 // Several host arrays uploaded as one device allocation (256-byte aligned parts),
-// so a hierarchy costs a handful of allocations, not one per array.
+// so a hierarchy costs a handful of allocations, not one per array.  The arrays are
+// gathered into the context's pinned staging buffer and copied with one
+// stream-ordered copy into one stream-ordered allocation from the device's memory
+// pool: a regrid neither synchronises the device nor pays for a fresh allocation,
+// and the host returns once the bytes are staged.  The vectors added must live
+// until upload returns.
 struct Packer {
-  std::vector<char> host;
-  std::vector<std::pair<size_t, void **>> parts;
+  struct Part {
+    const void *src;
+    size_t bytes, off;
+    void **dptr;
+  };
+  std::vector<Part> parts, space;  // uploaded arrays; device-only arrays (no host bytes)
+  size_t total = 0;                // bytes of the uploaded arrays
   template <class T>
   void add(const std::vector<T> &v, T **dptr) {
-    const size_t off = (host.size() + 255) & ~size_t(255);
-    host.resize(off + v.size() * sizeof(T));
-    if (!v.empty()) std::memcpy(host.data() + off, v.data(), v.size() * sizeof(T));
-    parts.push_back({off, (void **)dptr});
+    const size_t off = (total + 255) & ~size_t(255);
+    parts.push_back({v.data(), v.size() * sizeof(T), off, (void **)dptr});
+    total = off + v.size() * sizeof(T);
   }
-  // One stream-ordered allocation from the device's memory pool and one copy on the
-  // context's stream, so a regrid neither synchronises the device nor pays for a
-  // fresh allocation; the host bytes move into `keep`, which outlives the copy.
-  int upload(lgd_ctx *ctx, void **base, std::vector<std::vector<char>> &keep) {
+  // n elements the device fills in, placed after every uploaded array.
+  template <class T>
+  void reserve(size_t n, T **dptr) {
+    space.push_back({nullptr, n * sizeof(T), 0, (void **)dptr});
+  }
+  int upload(lgd_ctx *ctx, void **base) {
     *base = nullptr;
-    for (auto &p : parts) *p.second = nullptr;
-    if (host.empty()) return LGD_OK;
-    hipError_t e = hipMallocAsync(base, host.size(), ctx->stream);
+    for (auto &p : parts) *p.dptr = nullptr;
+    for (auto &p : space) *p.dptr = nullptr;
+    size_t end = total;
+    for (auto &p : space) {
+      p.off = (end + 255) & ~size_t(255);
+      end = p.off + p.bytes;
+    }
+    if (end == 0) return LGD_OK;
+    if (total == 0) {
+      hipError_t e = hipMallocAsync(base, end, ctx->stream);
+      if (e != hipSuccess) {
+        *base = nullptr;
+        return hip_fail(ctx, e, "hipMallocAsync(amr plan)");
+      }
+      for (auto &p : space) *p.dptr = (char *)*base + p.off;
+      return LGD_OK;
+    }
+    hipError_t e = hipSuccess;
+    // The staging buffer may still feed the previous upload's copy.
+    if (ctx->pinned_done) e = hipEventSynchronize(ctx->pinned_done);
+    else e = hipEventCreateWithFlags(&ctx->pinned_done, hipEventDisableTiming);
+    if (e != hipSuccess) return hip_fail(ctx, e, "amr plan staging event");
+    if (ctx->pinned_cap < total) {
+      if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+      ctx->pinned = nullptr;
+      ctx->pinned_cap = 0;
+      const size_t cap = std::max(total + total / 2, (size_t)1 << 20);
+      e = hipHostMalloc((void **)&ctx->pinned, cap, hipHostMallocDefault);
+      if (e != hipSuccess) {
+        ctx->pinned = nullptr;
+        return hip_fail(ctx, e, "hipHostMalloc(amr plan staging)");
+      }
+      ctx->pinned_cap = cap;
+    }
+    for (const auto &p : parts)
+      if (p.bytes) std::memcpy(ctx->pinned + p.off, p.src, p.bytes);
+    e = hipMallocAsync(base, end, ctx->stream);
     if (e != hipSuccess) {
       *base = nullptr;
       return hip_fail(ctx, e, "hipMallocAsync(amr plan)");
     }
-    e = hipMemcpyAsync(*base, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpyAsync(*base, ctx->pinned, total, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipEventRecord(ctx->pinned_done, ctx->stream);
     if (e != hipSuccess) return hip_fail(ctx, e, "upload(amr plan)");
-    for (auto &p : parts) *p.second = (char *)*base + p.first;
-    keep.push_back(std::move(host));
+    for (auto &p : parts) *p.dptr = (char *)*base + p.off;
+    for (auto &p : space) *p.dptr = (char *)*base + p.off;
     return LGD_OK;
   }
 };
 
-
-// Whether every cell of `need` lies in a box of `cover` (run-skipping walk
-// along axis 0).
-bool covered(const lgd_box &need, const std::vector<lgd_box> &cover) {
+// Whether every cell of `need` lies in a patch of level l (run-skipping walk along
+// axis 0 through the level's bucket index).
+bool covered(const lgd_amr &g, int l, const lgd_box &need) {
   if (need.cells() == 0) return true;
-  int64_t x = need.lo[0], y = need.lo[1];
-  while (true) {
-    const lgd_box *hit = nullptr;
-    for (const lgd_box &b : cover)
-      if (b.contains(x, y)) {
-        hit = &b;
-        break;
-      }
-    if (!hit) return false;
-    x = hit->hi[0];
-    if (x >= need.hi[0]) {
-      x = need.lo[0];
-      if (++y >= need.hi[1]) return true;
+  for (int64_t y = need.lo[1]; y < need.hi[1]; ++y) {
+    int64_t x = need.lo[0];
+    while (x < need.hi[0]) {
+      const int64_t p = g.find_patch(l, x, y);
+      if (p < 0) return false;
+      x = g.box[(size_t)p].hi[0];
     }
   }
+  return true;
+}
+
+// Patches of one level do not overlap; each 16×16-cell bucket of the level domain
+// lists the few that touch it (find_patch's index for level l, appended to lookup).
+void build_bucket_index(lgd_amr &g, int l) {
+  const int64_t B = lgd_amr::kBucket;
+  const lgd_box d = g.domain(l);
+  lgd_amr::bucket_index ix;
+  ix.nbx = (d.hi[0] + B - 1) / B;
+  ix.nby = (d.hi[1] + B - 1) / B;
+  ix.start.assign((size_t)(ix.nbx * ix.nby + 1), 0);
+  auto each_bucket = [&](uint32_t p, auto &&f) {
+    const lgd_box &bx = g.box[p];
+    for (int64_t by = bx.lo[1] / B; by <= (bx.hi[1] - 1) / B; ++by)
+      for (int64_t bx0 = bx.lo[0] / B; bx0 <= (bx.hi[0] - 1) / B; ++bx0) f(bx0 + ix.nbx * by);
+  };
+  for (uint32_t p = g.level_start[l]; p < g.level_start[l + 1]; ++p)
+    each_bucket(p, [&](int64_t k) { ++ix.start[(size_t)k + 1]; });
+  for (size_t k = 1; k < ix.start.size(); ++k) ix.start[k] += ix.start[k - 1];
+  ix.ids.resize(ix.start.back());
+  std::vector<uint32_t> fillpos(ix.start.begin(), ix.start.end() - 1);
+  for (uint32_t p = g.level_start[l]; p < g.level_start[l + 1]; ++p)
+    each_bucket(p, [&](int64_t k) { ix.ids[fillpos[(size_t)k]++] = p; });
+  if ((int)g.lookup.size() <= l) g.lookup.resize((size_t)l + 1);
+  g.lookup[(size_t)l] = std::move(ix);
 }
 
 }  // namespace
@@ -153,13 +217,13 @@ void lgd_amr::free_device() {
   d_bstart = nullptr;
   d_bids = nullptr;
   d_lo = nullptr;
+  d_err = nullptr;
   for (auto &p : plans) {
     if (p.block) (void)hipFreeAsync(p.block, st);
     if (p.stiles) (void)hipFreeAsync(p.stiles, st);
     if (p.tsegs) (void)hipFreeAsync(p.tsegs, st);
   }
   plans.clear();
-  staging.clear();
 }
 
 int lgd_amr_default_seg_rows(const lgd_amr &g, int level) {
@@ -196,7 +260,8 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
                               {(bx + 1) * base.n[0], (by + 1) * base.n[1]}});
     }
   g.level_start = {0, (uint32_t)g.box.size()};
-  std::vector<lgd_box> coarser(g.box);
+  g.lookup.clear();
+  build_bucket_index(g, 0);
   for (size_t l = 0; l < levels.size(); ++l) {
     const int lev = (int)l + 1;
     const int64_t r = ratios[l];
@@ -226,7 +291,7 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
       lgd_box need{{floor_div(b.lo[0], r) - 1, floor_div(b.lo[1], r) - 1},
                    {floor_div(b.hi[0] + r - 1, r) + 1, floor_div(b.hi[1] + r - 1, r) + 1}};
       need = need.clipped(cdom);
-      if (!covered(need, coarser)) {
+      if (!covered(g, lev - 1, need)) {
         snprintf(m, sizeof m, "level-%d patch is not properly nested in level %d", lev, lev - 1);
         return fail(ctx, LGD_ERR_INVALID_ARGUMENT, m);
       }
@@ -236,35 +301,12 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
       g.box.push_back(b);
     }
     g.level_start.push_back((uint32_t)g.box.size());
-    coarser = boxes;
+    build_bucket_index(g, lev);
   }
   g.spacing.assign(1, {base.h[0], base.h[1]});
   for (uint32_t r : ratios) {
     auto p = g.spacing.back();
     g.spacing.push_back({p[0] / (double)r, p[1] / (double)r});
-  }
-  // Patches of one level do not overlap; each bucket lists the few that touch it.
-  const int64_t B = lgd_amr::kBucket;
-  g.lookup.clear();
-  for (int l = 0; l < g.num_levels(); ++l) {
-    lgd_box d = g.domain(l);
-    lgd_amr::bucket_index ix;
-    ix.nbx = (d.hi[0] + B - 1) / B;
-    ix.nby = (d.hi[1] + B - 1) / B;
-    ix.start.assign((size_t)(ix.nbx * ix.nby + 1), 0);
-    auto each_bucket = [&](uint32_t p, auto &&f) {
-      const lgd_box &bx = g.box[p];
-      for (int64_t by = bx.lo[1] / B; by <= (bx.hi[1] - 1) / B; ++by)
-        for (int64_t bx0 = bx.lo[0] / B; bx0 <= (bx.hi[0] - 1) / B; ++bx0) f(bx0 + ix.nbx * by);
-    };
-    for (uint32_t p = g.level_start[l]; p < g.level_start[l + 1]; ++p)
-      each_bucket(p, [&](int64_t k) { ++ix.start[(size_t)k + 1]; });
-    for (size_t k = 1; k < ix.start.size(); ++k) ix.start[k] += ix.start[k - 1];
-    ix.ids.resize(ix.start.back());
-    std::vector<uint32_t> fillpos(ix.start.begin(), ix.start.end() - 1);
-    for (uint32_t p = g.level_start[l]; p < g.level_start[l + 1]; ++p)
-      each_bucket(p, [&](int64_t k) { ix.ids[fillpos[(size_t)k]++] = p; });
-    g.lookup.push_back(std::move(ix));
   }
   g.off.assign(g.box.size() + 1, 0);
   for (size_t b = 0; b < g.box.size(); ++b)
@@ -272,47 +314,159 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
   return lgd_amr_build_plans(g, reuse0);
 }
 
-// Bilinear prolongation stencil of fine cell (x, y) at `level` from level-1
-// interiors (prolong_corners, intergrid.rs:307-351): corners in bitmask order,
-// zero weights skipped, corners outside the domain mirrored.
-static void prolong_stencil(const lgd_amr &g, int level, int64_t x, int64_t y,
-                            std::vector<lgd_corner> &out) {
-  const double r = (double)g.ratios[level - 1];
-  const lgd_box cd = g.domain(level - 1);
-  int64_t c0[2];
-  double w[2];
-  const int64_t cell[2] = {x, y};
-  for (int d = 0; d < 2; ++d) {
-    double pc = ((double)cell[d] + 0.5) / r - 0.5;
-    double f = std::floor(pc);
-    c0[d] = (int64_t)f;
-    w[d] = pc - f;
-  }
-  for (unsigned corner = 0; corner < 4; ++corner) {
-    int64_t cc[2] = {c0[0], c0[1]};
-    double weight = 1.0;
-    for (int d = 0; d < 2; ++d) {
-      if ((corner >> d) & 1u) {
-        cc[d] += 1;
-        weight *= w[d];
-      } else {
-        weight *= 1.0 - w[d];
-      }
-    }
-    if (weight == 0.0) continue;
-    for (int d = 0; d < 2; ++d) cc[d] = reflect_coord(cc[d], cd.lo[d], cd.hi[d]);
-    int64_t q = g.find_patch(level - 1, cc[0], cc[1]);
-    lgd_corner c;
-    c.src = q < 0 ? -1 : g.index(q, cc[0] - g.box[q].lo[0], cc[1] - g.box[q].lo[1]);
-    c.weight = weight;
-    out.push_back(c);
-  }
-}
-
 void lgd_amr_take_level0_plan(lgd_amr &g, lgd_amr &from) {
   if (g.plans.empty() || from.plans.empty() || g.plans[0].tiles) return;
   g.plans[0] = from.plans[0];
   from.plans[0] = lgd_level_plan{};
+}
+
+namespace {
+
+__device__ inline int64_t plan_mirror(int64_t c, int64_t hi) {
+  if (c < 0) return -1 - c;
+  if (c >= hi) return 2 * hi - 1 - c;
+  return c;
+}
+
+// The entry of a sorted prefix array (n + 1 entries, first 0) whose range holds t.
+__device__ inline int plan_bucket(const int64_t *pre, int n, int64_t t) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (pre[mid] <= t) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+// Level l's fill plan on the device, one thread per ghost-ring cell: patch b0 + p's
+// ring cells are entries [ring0[p], ring0[p + 1]) in the host's visiting order (the
+// rows below and above, left to right, then the columns left and right, bottom to
+// top).  Each entry is the host plan's: a copy from the same-level patch holding the
+// mirrored cell, or the bilinear prolongation stencil of prolong_corners
+// (intergrid.rs:307-351) in corner bitmask order with zero weights skipped and the
+// corners mirrored into the coarse domain, with the same floating-point operations —
+// at most four corners, in the op's own slots 4t..4t+3.  A ghost with no donor (a
+// hierarchy that is not properly nested) sets *err.
+__global__ void plan_fill_kernel(lgd_dev_index ix, int l, uint32_t b0, int32_t np,
+                                 const int64_t *__restrict__ ring0, int64_t total, int64_t dw,
+                                 int64_t dh, int64_t cw, int64_t ch, double r, lgd_fill_op *ops,
+                                 lgd_corner *corners, int32_t *err) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int p = plan_bucket(ring0, np, t);
+  const uint32_t b = b0 + (uint32_t)p;
+  const int64_t k = t - ring0[p];
+  const lgd_block_desc bd = ix.blocks[b];
+  const int64_t px = ix.lo[2 * b], py = ix.lo[2 * b + 1];
+  const int64_t W = (int64_t)bd.nx + 2;
+  int64_t x, y;
+  if (k < 2 * W) {
+    x = px - 1 + k / 2;
+    y = (k & 1) ? py + bd.ny : py - 1;
+  } else {
+    const int64_t k2 = k - 2 * W;
+    y = py + k2 / 2;
+    x = (k2 & 1) ? px + bd.nx : px - 1;
+  }
+  lgd_fill_op op;
+  op.target = bd.off + (y - py + 1) * W + (x - px + 1);
+  const int64_t tx = plan_mirror(x, dw), ty = plan_mirror(y, dh);
+  const int64_t src = lgd_dev_find_patch(ix, l, tx, ty);
+  if (src >= 0) {
+    const lgd_block_desc sd = ix.blocks[src];
+    op.src = sd.off + (ty - ix.lo[2 * src + 1] + 1) * ((int64_t)sd.nx + 2) + (tx - ix.lo[2 * src] + 1);
+    op.cstart = (int32_t)src;
+    op.clen = 0;
+    ops[t] = op;
+    return;
+  }
+  op.src = -1;
+  op.cstart = (int32_t)(4 * t);
+  int n = 0;
+  if (l == 0) {
+    *err = 1;  // a level-0 ghost always has a same-level donor
+  } else {
+    int64_t c0[2];
+    double w[2];
+    const int64_t cell[2] = {tx, ty};
+    for (int d = 0; d < 2; ++d) {
+      const double pc = ((double)cell[d] + 0.5) / r - 0.5;
+      const double f = floor(pc);
+      c0[d] = (int64_t)f;
+      w[d] = pc - f;
+    }
+    for (unsigned corner = 0; corner < 4; ++corner) {
+      int64_t cc[2] = {c0[0], c0[1]};
+      double weight = 1.0;
+      for (int d = 0; d < 2; ++d) {
+        if ((corner >> d) & 1u) {
+          cc[d] += 1;
+          weight *= w[d];
+        } else {
+          weight *= 1.0 - w[d];
+        }
+      }
+      if (weight == 0.0) continue;
+      cc[0] = plan_mirror(cc[0], cw);
+      cc[1] = plan_mirror(cc[1], ch);
+      const int64_t q = lgd_dev_find_patch(ix, l - 1, cc[0], cc[1]);
+      lgd_corner c;
+      c.weight = weight;
+      if (q < 0) {
+        c.src = -1;
+        *err = 1;
+      } else {
+        const lgd_block_desc qd = ix.blocks[q];
+        c.src = qd.off + (cc[1] - ix.lo[2 * q + 1] + 1) * ((int64_t)qd.nx + 2) +
+                (cc[0] - ix.lo[2 * q] + 1);
+      }
+      corners[4 * t + n++] = c;
+    }
+  }
+  op.clen = n;
+  ops[t] = op;
+}
+
+// Level l's restriction onto level l - 1 on the device, one thread per coarse cell
+// under a patch: patch b0 + p covers the coarse cells [lo / r, hi / r) (aligned to
+// r), entries [rit0[p], rit0[p + 1]) row by row; the coarse patch holding each is
+// unique (patches of a level do not overlap).  Items write distinct coarse cells, so
+// their order does not matter.  A coarse cell outside level l - 1 sets *err.
+__global__ void plan_restrict_kernel(lgd_dev_index ix, int l, uint32_t b0, int32_t np,
+                                     const int64_t *__restrict__ rit0, int64_t total, int64_t r,
+                                     lgd_restrict_item *items, int32_t *err) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const int p = plan_bucket(rit0, np, t);
+  const uint32_t fb = b0 + (uint32_t)p;
+  const int64_t k = t - rit0[p];
+  const lgd_block_desc fd = ix.blocks[fb];
+  const int64_t fx = ix.lo[2 * fb], fy = ix.lo[2 * fb + 1];
+  const int64_t cwid = fd.nx / r;
+  const int64_t cx = fx / r + k % cwid, cy = fy / r + k / cwid;
+  const int64_t cb = lgd_dev_find_patch(ix, l - 1, cx, cy);
+  lgd_restrict_item it;
+  it.stride = (int64_t)fd.nx + 2;
+  it.child = fd.off + (cy * r - fy + 1) * it.stride + (cx * r - fx + 1);
+  if (cb < 0) {
+    *err = 1;
+    it.target = -1;
+  } else {
+    const lgd_block_desc cd = ix.blocks[cb];
+    it.target = cd.off + (cy - ix.lo[2 * cb + 1] + 1) * ((int64_t)cd.nx + 2) + (cx - ix.lo[2 * cb] + 1);
+  }
+  items[t] = it;
+}
+
+}  // namespace
+
+int lgd_amr_plan_error(lgd_ctx *ctx, const lgd_amr &g, int32_t *host_flag) {
+  *host_flag = 0;
+  if (!g.d_err) return LGD_OK;
+  hipError_t e = hipMemcpyAsync(host_flag, g.d_err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e != hipSuccess) return hip_fail(ctx, e, "amr plan status");
+  return LGD_OK;
 }
 
 int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
@@ -330,11 +484,14 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     blocks[b].off = g.off[b];
   }
   int rc;
+  // One device allocation for the patch index and the plans of levels >= 1; level 0's
+  // plan (the base tiling's, which a regrid's next hierarchy takes over) gets its own.
+  Packer pk, pk0;
+  // find_patch's index, concatenated over the levels.
+  std::vector<lgd_dev_level> lv;
+  std::vector<uint32_t> bstart, bids;
+  std::vector<int64_t> lo;
   {
-    // find_patch's index, concatenated over the levels.
-    std::vector<lgd_dev_level> lv;
-    std::vector<uint32_t> bstart, bids;
-    std::vector<int64_t> lo;
     for (int l = 0; l < g.num_levels(); ++l) {
       const lgd_amr::bucket_index &ix = g.lookup[(size_t)l];
       const lgd_box d = g.domain(l);
@@ -347,22 +504,34 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
       lo.push_back(b.lo[0]);
       lo.push_back(b.lo[1]);
     }
-    Packer pk;
     pk.add(blocks, &g.d_blocks);
     pk.add(lv, &g.d_levels);
     pk.add(bstart, &g.d_bstart);
     pk.add(bids, &g.d_bids);
     pk.add(lo, &g.d_lo);
-    if ((rc = pk.upload(ctx, &g.d_arena, g.staging))) return rc;
   }
   g.plans.assign((size_t)g.num_levels(), lgd_level_plan{});
+  // Host arrays of every level's plan, alive until the uploads: the stage tilings
+  // and the fill / restriction plans' per-patch prefix sums (the plans themselves are
+  // built on the device, in arrays that are reserved, not uploaded).
+  struct LevelHost {
+    std::vector<lgd_tile> tiles;
+    std::vector<lgd_sseg> segs;
+    std::vector<int64_t> ring0, rit0;
+    int64_t *d_ring0 = nullptr, *d_rit0 = nullptr;
+  };
+  std::vector<LevelHost> host((size_t)g.num_levels());
+  const std::vector<int32_t> zero{0};
+  pk.add(zero, &g.d_err);
   for (int l = 0; l < g.num_levels(); ++l) {
     lgd_level_plan &P = g.plans[(size_t)l];
     if (l == 0 && take0) continue;  // lgd_amr_take_level0_plan fills it in
+    LevelHost &Hl = host[(size_t)l];
+    const uint32_t b0 = g.level_start[l], b1 = g.level_start[l + 1];
     // Level kernels: one 64×4 tile of cells per workgroup.
-    std::vector<lgd_tile> tiles;
+    std::vector<lgd_tile> &tiles = Hl.tiles;
     P.tile0.clear();
-    for (uint32_t b = g.level_start[l]; b < g.level_start[l + 1]; ++b) {
+    for (uint32_t b = b0; b < b1; ++b) {
       P.tile0.push_back((int64_t)tiles.size());
       for (int64_t y0 = 0; y0 < g.box[b].extent(1); y0 += kTY)
         for (int64_t x0 = 0; x0 < g.box[b].extent(0); x0 += kTX)
@@ -370,98 +539,58 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     }
     P.tile0.push_back((int64_t)tiles.size());
     // Fill plan: every ghost-ring cell (ghost width 1) of every patch.
-    std::vector<lgd_fill_op> ops;
-    std::vector<lgd_corner> corners;
-    const lgd_box dom = g.domain(l);
-    for (uint32_t b = g.level_start[l]; b < g.level_start[l + 1]; ++b) {
-      const lgd_box &p = g.box[b];
-      auto visit = [&](int64_t x, int64_t y) {
-        int64_t tx = reflect_coord(x, dom.lo[0], dom.hi[0]);
-        int64_t ty = reflect_coord(y, dom.lo[1], dom.hi[1]);
-        lgd_fill_op op;
-        op.target = g.index(b, x - p.lo[0], y - p.lo[1]);
-        int64_t src = g.find_patch(l, tx, ty);
-        if (src >= 0) {
-          op.src = g.index(src, tx - g.box[src].lo[0], ty - g.box[src].lo[1]);
-          op.cstart = (int32_t)src;  // the source patch (a fused stage recomputes the cell)
-          op.clen = 0;
-        } else {
-          op.src = -1;
-          op.cstart = (int32_t)corners.size();
-          prolong_stencil(g, l, tx, ty, corners);
-          op.clen = (int32_t)(corners.size() - (size_t)op.cstart);
-        }
-        ops.push_back(op);
-      };
-      for (int64_t x = p.lo[0] - 1; x < p.hi[0] + 1; ++x) {
-        visit(x, p.lo[1] - 1);
-        visit(x, p.hi[1]);
-      }
-      for (int64_t y = p.lo[1]; y < p.hi[1]; ++y) {
-        visit(p.lo[0] - 1, y);
-        visit(p.hi[0], y);
-      }
-    }
-    for (const lgd_corner &c : corners)
-      if (c.src < 0) {
-        return fail(ctx, LGD_ERR_INVALID_ARGUMENT,
-                    "no coarse donor for a prolongation corner (hierarchy not properly nested)");
-      }
-    for (const lgd_fill_op &op : ops)
-      if (op.src < 0 && op.clen == 0)
-        return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "level-0 ghost without a same-level donor");
-    // Restriction of this level onto the coarser one.
-    std::vector<lgd_restrict_item> items;
+    Hl.ring0.assign((size_t)(b1 - b0) + 1, 0);
+    for (uint32_t b = b0; b < b1; ++b)
+      Hl.ring0[b - b0 + 1] = Hl.ring0[b - b0] + 2 * (g.box[b].extent(0) + g.box[b].extent(1)) + 4;
+    // Restriction of this level onto the coarser one: every coarse cell under a patch.
+    Hl.rit0.assign((size_t)(b1 - b0) + 1, 0);
     if (l >= 1) {
-      const int64_t r = g.ratios[l - 1];
-      const lgd_amr::bucket_index &cix = g.lookup[(size_t)l - 1];
-      const int64_t B = lgd_amr::kBucket;
-      std::vector<uint32_t> cand;
-      for (uint32_t fb = g.level_start[l]; fb < g.level_start[l + 1]; ++fb) {
-        const lgd_box &fp = g.box[fb];
-        lgd_box cov{{fp.lo[0] / r, fp.lo[1] / r}, {fp.hi[0] / r, fp.hi[1] / r}};
-        // The coarse patches in the buckets under cov, in id order (each overlap is
-        // visited in the same order as a scan over every coarse patch).
-        cand.clear();
-        for (int64_t by = cov.lo[1] / B; by <= (cov.hi[1] - 1) / B; ++by)
-          for (int64_t bx = cov.lo[0] / B; bx <= (cov.hi[0] - 1) / B; ++bx) {
-            const int64_t k = bx + cix.nbx * by;
-            cand.insert(cand.end(), cix.ids.begin() + cix.start[(size_t)k],
-                        cix.ids.begin() + cix.start[(size_t)k + 1]);
-          }
-        std::sort(cand.begin(), cand.end());
-        cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
-        for (uint32_t cb : cand) {
-          const lgd_box &cp = g.box[cb];
-          lgd_box ov = cov.clipped(cp);
-          for (int64_t cy = ov.lo[1]; cy < ov.hi[1]; ++cy)
-            for (int64_t cx = ov.lo[0]; cx < ov.hi[0]; ++cx) {
-              lgd_restrict_item it;
-              it.target = g.index(cb, cx - cp.lo[0], cy - cp.lo[1]);
-              it.child = g.index(fb, cx * r - fp.lo[0], cy * r - fp.lo[1]);
-              it.stride = fp.extent(0) + 2;
-              items.push_back(it);
-            }
-        }
-      }
+      const int64_t r = g.ratios[(size_t)l - 1];
+      for (uint32_t b = b0; b < b1; ++b)
+        Hl.rit0[b - b0 + 1] = Hl.rit0[b - b0] + (g.box[b].extent(0) / r) * (g.box[b].extent(1) / r);
     }
     // Streaming level-stage items at the level's default strip height.
-    std::vector<lgd_sseg> segs;
+    std::vector<lgd_sseg> &segs = Hl.segs;
     P.auto_rows = lgd_amr_default_seg_rows(g, l);
     P.seg_rows = ctx->amr_rows > 0 ? (int32_t)ctx->amr_rows : P.auto_rows;
     build_segs(g, l, P.seg_rows, segs, P.seg0);
     P.nsegs = (int64_t)segs.size();
     P.ntiles = (int64_t)tiles.size();
-    P.nops = (int64_t)ops.size();
-    P.ncorners = (int64_t)corners.size();
-    P.nritems = (int64_t)items.size();
-    Packer pk;
-    pk.add(tiles, &P.tiles);
-    pk.add(ops, &P.ops);
-    pk.add(corners, &P.corners);
-    pk.add(items, &P.ritems);
-    pk.add(segs, &P.segs);
-    if ((rc = pk.upload(ctx, &P.block, g.staging))) return rc;
+    P.nops = Hl.ring0.back();
+    P.ncorners = 4 * P.nops;
+    P.nritems = Hl.rit0.back();
+    Packer &to = l == 0 ? pk0 : pk;
+    to.add(tiles, &P.tiles);
+    to.add(segs, &P.segs);
+    to.add(Hl.ring0, &Hl.d_ring0);
+    to.add(Hl.rit0, &Hl.d_rit0);
+    to.reserve((size_t)P.nops, &P.ops);
+    to.reserve((size_t)P.ncorners, &P.corners);
+    to.reserve((size_t)P.nritems, &P.ritems);
+  }
+  if (!take0 && (rc = pk0.upload(ctx, &g.plans[0].block))) return rc;
+  if ((rc = pk.upload(ctx, &g.d_arena))) return rc;
+  const lgd_dev_index ix{g.d_levels, g.d_bstart, g.d_bids, g.d_lo, g.d_blocks, g.num_levels()};
+  for (int l = 0; l < g.num_levels(); ++l) {
+    if (l == 0 && take0) continue;
+    const lgd_level_plan &P = g.plans[(size_t)l];
+    const LevelHost &Hl = host[(size_t)l];
+    const uint32_t b0 = g.level_start[l];
+    const int32_t np = (int32_t)(g.level_start[l + 1] - b0);
+    const lgd_box d = g.domain(l);
+    if (P.nops > 0) {
+      const lgd_box cd = l > 0 ? g.domain(l - 1) : d;
+      const double r = l > 0 ? (double)g.ratios[(size_t)l - 1] : 1.0;
+      plan_fill_kernel<<<blocks_for(P.nops, 256), 256, 0, ctx->stream>>>(
+          ix, l, b0, np, Hl.d_ring0, P.nops, d.hi[0], d.hi[1], cd.hi[0], cd.hi[1], r, P.ops,
+          P.corners, g.d_err);
+      if ((rc = launch_check(ctx, "amr fill plan"))) return rc;
+    }
+    if (l >= 1 && P.nritems > 0) {
+      plan_restrict_kernel<<<blocks_for(P.nritems, 256), 256, 0, ctx->stream>>>(
+          ix, l, b0, np, Hl.d_rit0, P.nritems, (int64_t)g.ratios[(size_t)l - 1], P.ritems, g.d_err);
+      if ((rc = launch_check(ctx, "amr restriction plan"))) return rc;
+    }
   }
   return LGD_OK;
 }
@@ -1225,6 +1354,15 @@ int lgd_amr_create(lgd_ctx *ctx, const lgd_cartesian_desc *base, const uint32_t 
   lgd_amr *g = new (std::nothrow) lgd_amr();
   if (!g) return fail(ctx, LGD_ERR_OUT_OF_MEMORY, "host allocation failed");
   rc = lgd_amr_build(ctx, c, r, levels, *g);
+  int32_t bad = 0;
+  if (!rc) rc = lgd_amr_plan_error(ctx, *g, &bad);
+  if (!rc) {
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) rc = hip_fail(ctx, e, "amr plan status");
+  }
+  if (!rc && bad)
+    rc = fail(ctx, LGD_ERR_INVALID_ARGUMENT,
+              "a ghost cell without a donor (hierarchy not properly nested)");
   if (rc) {
     g->free_device();
     delete g;

@@ -110,6 +110,8 @@ int lgd_ctx_create(int32_t device, void *stream, lgd_ctx **out) {
 int lgd_ctx_destroy(lgd_ctx *ctx) {
   if (!ctx) return LGD_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->pinned_done) (void)hipEventDestroy(ctx->pinned_done);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->owns_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return LGD_OK;

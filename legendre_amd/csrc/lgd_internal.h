@@ -44,6 +44,11 @@ struct lgd_ctx {
   int64_t adjoint_kernel = 0;   // 1 = three-pass reverse step, 2 = streaming required
   int64_t adjoint_rows = 0;     // rows per strip of the streaming reverse step (0 = by device)
   int64_t regrid_host = 0;      // 1 = AMR regrid tags and migrates on the host
+  // Pinned host staging of AMR plan uploads (grown on demand) and the event recorded
+  // after the last copy out of it.
+  char *pinned = nullptr;
+  size_t pinned_cap = 0;
+  hipEvent_t pinned_done = nullptr;
   // Shapes chosen by lgd_autotune_modelc_step, keyed by (nx, ny, blocks, noise).
   std::map<std::tuple<int64_t, int64_t, int64_t, int>, lgd_launch_shape> tuned;
 };

@@ -12,6 +12,7 @@
 // the reference's bit for bit.
 #include <algorithm>
 #include <cmath>
+#include <thread>
 #include <vector>
 
 #include "lgd_regrid.h"
@@ -91,7 +92,13 @@ bool bisect_split(const lgd_box &bx, size_t minw, Split &s) {
   return true;
 }
 
-void cluster_rec(Cell *f, size_t n, double eff, size_t minw, std::vector<lgd_box> &out) {
+// Flag sets from this size up split their two halves over two host threads while
+// `depth` allows (the halves are disjoint ranges of f; their boxes join left first,
+// the order of the one-thread recursion).
+constexpr size_t kParallelFlags = 4096;
+
+void cluster_rec(Cell *f, size_t n, double eff, size_t minw, std::vector<lgd_box> &out,
+                 int depth) {
   const lgd_box bx = bounding_box(f, n);
   if ((double)n / (double)bx.cells() >= eff) {
     out.push_back(bx);
@@ -113,8 +120,25 @@ void cluster_rec(Cell *f, size_t n, double eff, size_t minw, std::vector<lgd_box
   size_t lt = 0;
   for (size_t j = 0; j < n; ++j)
     if (f[j][s.axis] < s.plane) std::swap(f[lt++], f[j]);
-  cluster_rec(f, lt, eff, minw, out);
-  cluster_rec(f + lt, n - lt, eff, minw, out);
+  if (depth > 0 && n >= kParallelFlags) {
+    std::vector<lgd_box> left;
+    std::thread t;
+    bool spawned = false;
+    try {
+      t = std::thread([&] { cluster_rec(f, lt, eff, minw, left, depth - 1); });
+      spawned = true;
+    } catch (...) {
+      cluster_rec(f, lt, eff, minw, left, depth - 1);
+    }
+    std::vector<lgd_box> right;
+    cluster_rec(f + lt, n - lt, eff, minw, right, depth - 1);
+    if (spawned) t.join();
+    out.insert(out.end(), left.begin(), left.end());
+    out.insert(out.end(), right.begin(), right.end());
+    return;
+  }
+  cluster_rec(f, lt, eff, minw, out, 0);
+  cluster_rec(f + lt, n - lt, eff, minw, out, 0);
 }
 
 bool box_covered(const lgd_box &need, const std::vector<lgd_box> &cover) {
@@ -250,7 +274,7 @@ int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>>
       }
     if (flags.empty()) continue;
     std::vector<lgd_box> boxes;
-    cluster_rec(flags.data(), flags.size(), pol.efficiency, (size_t)pol.min_width, boxes);
+    cluster_rec(flags.data(), flags.size(), pol.efficiency, (size_t)pol.min_width, boxes, 3);
     const int64_t r = g.ratios[(size_t)target - 1];
     for (lgd_box &b : boxes)
       for (int d = 0; d < 2; ++d) {
@@ -323,33 +347,12 @@ namespace {
 constexpr int kRTX = 64, kRTY = 4;  // the level plans' tile shape
 constexpr int kMaxSampleDepth = 7;  // levels below the sampled one (8-level hierarchies)
 
-struct DevIndex {
-  const lgd_dev_level *levels;
-  const uint32_t *bstart;
-  const uint32_t *bids;
-  const int64_t *lo;
-  const lgd_block_desc *blocks;
-  int nlevels;
-};
+using DevIndex = lgd_dev_index;
 
 struct DevRatios {
   int64_t cells[2];  // level-0 domain
   uint32_t r[kMaxSampleDepth + 1];
 };
-
-__device__ int64_t dev_find_patch(const DevIndex &ix, int l, int64_t x, int64_t y) {
-  if (l >= ix.nlevels) return -1;
-  const lgd_dev_level L = ix.levels[l];
-  if (x < 0 || y < 0 || x >= L.w || y >= L.h) return -1;
-  const int64_t k = L.start0 + x / lgd_amr::kBucket + L.nbx * (y / lgd_amr::kBucket);
-  for (uint32_t i = ix.bstart[k]; i < ix.bstart[k + 1]; ++i) {
-    const uint32_t p = ix.bids[i];
-    const int64_t px = ix.lo[2 * p], py = ix.lo[2 * p + 1];
-    const lgd_block_desc bd = ix.blocks[p];
-    if (x >= px && x < px + bd.nx && y >= py && y < py + bd.ny) return (int64_t)p;
-  }
-  return -1;
-}
 
 __device__ int64_t dev_mirror(int64_t c, int64_t lo, int64_t hi) {
   if (c < lo) return 2 * lo - 1 - c;
@@ -363,7 +366,7 @@ template <int K>
 __device__ __attribute__((noinline)) double dev_sample(const DevIndex &ix, const DevRatios &R,
                                                        const double *f, int level, int64_t x,
                                                        int64_t y) {
-  const int64_t b = dev_find_patch(ix, level, x, y);
+  const int64_t b = lgd_dev_find_patch(ix, level, x, y);
   if (b >= 0) {
     const lgd_block_desc bd = ix.blocks[b];
     return f[bd.off + (y - ix.lo[2 * b + 1] + 1) * (int64_t)(bd.nx + 2) + (x - ix.lo[2 * b] + 1)];

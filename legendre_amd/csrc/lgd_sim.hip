@@ -577,9 +577,17 @@ int amr_regrid(lgd_sim *s) {
       if (e != hipSuccess) return abandon(hip_fail(s->ctx, e, "migrate upload"));
     }
   }
+  int32_t bad = 0;
+  if ((rc = lgd_amr_plan_error(s->ctx, *ng, &bad))) return abandon(rc);
+  {
+    hipError_t e = hipStreamSynchronize(s->ctx->stream);
+    if (e != hipSuccess) return abandon(hip_fail(s->ctx, e, "regrid"));
+  }
+  if (bad)
+    return abandon(fail(s->ctx, LGD_ERR_INVALID_ARGUMENT,
+                        "regrid proposal has a ghost cell without a donor"));
   drop_graphs(s);  // they hold the old hierarchy's plans and buffers
   ++s->amr_epoch;
-  (void)hipStreamSynchronize(s->ctx->stream);
   // The old state buffers become the spares; spares that went unused are freed.
   std::vector<lgd_buf *> spare = s->state;
   for (size_t f = 0; f < s->next.size(); ++f)
