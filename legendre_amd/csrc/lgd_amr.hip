@@ -18,6 +18,7 @@
 // and visit order cannot change results (intergrid.rs:23-26).
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -251,6 +252,13 @@ int lgd_amr_build(lgd_ctx *ctx, const lgd_cart &base, const std::vector<uint32_t
   g.cells[0] = base.n[0] * base.nb[0];
   g.cells[1] = base.n[1] * base.nb[1];
   g.ratios = ratios;
+  // Level domains stay below 2^31 cells per axis (the regrid's cell coordinates are
+  // 32-bit).
+  for (int l = 0; l <= (int)ratios.size(); ++l) {
+    const lgd_box d = g.domain(l);
+    if (d.hi[0] > INT32_MAX || d.hi[1] > INT32_MAX)
+      return fail(ctx, LGD_ERR_UNSUPPORTED, "an AMR level domain of 2^31 cells or more per axis");
+  }
   g.level.clear();
   g.box.clear();
   for (int64_t by = 0; by < base.nb[1]; ++by)

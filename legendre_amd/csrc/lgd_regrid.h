@@ -10,8 +10,9 @@
 int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
                        const lgd_regrid_policy &policy, std::vector<std::vector<lgd_box>> &out);
 // The same from per-level bitmasks of tagged, buffer-dilated cells (row-major over
-// each level's domain, 64 cells per word; consumed).
-int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>> &bits,
+// each level's domain, 64 cells per word; null = none; the nesting flags are added
+// in place).
+int lgd_regrid_propose_bits(const lgd_amr &g, const std::vector<uint64_t *> &bits,
                             const lgd_regrid_policy &policy, std::vector<std::vector<lgd_box>> &out);
 // The bitmask of level `src` from a caller's tagged cells (n × [x, y], level cell
 // coordinates, as TagCells::tag_level appends them), buffer-dilated and clipped to
@@ -28,16 +29,18 @@ void lgd_regrid_migrate(const lgd_amr &old_g, const std::vector<double> &old_f,
 struct lgd_regrid_scratch {
   uint64_t *T = nullptr;     // tag bitmasks
   uint64_t *bits = nullptr;  // dilated bitmasks
+  uint64_t *host = nullptr;  // pinned host copy of the dilated bitmasks
   size_t wcap = 0;           // words of each
 };
 void lgd_regrid_scratch_free(lgd_regrid_scratch &sc);
 // Whether the device path covers this hierarchy's depth.
 bool lgd_regrid_device_supported(const lgd_amr &g);
 // Tag and dilate every level that can be refined on the device (the tag field is
-// the ghost-filled device field); bits[level] receives its bitmask.
+// the ghost-filled device field); bits[level] points at its bitmask in the scratch's
+// pinned host copy (valid until the next call).
 int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field,
                          const lgd_regrid_policy &policy, lgd_regrid_scratch &sc,
-                         std::vector<std::vector<uint64_t>> &bits);
+                         std::vector<uint64_t *> &bits);
 // Device migration: the interiors of new_g's levels >= 1 in new_f (a buffer of the
 // new layout) sampled from old_f on old_g; level 0 and ghosts are not written.
 int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *old_f,

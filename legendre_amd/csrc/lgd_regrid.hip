@@ -11,6 +11,7 @@
 // All of it is integer work plus exact-order sampling, so regrid decisions are
 // the reference's bit for bit.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <thread>
 #include <vector>
@@ -19,31 +20,38 @@
 
 namespace {
 
-using Cell = std::array<int64_t, 2>;
+// A flagged cell in level coordinates (level domains are far below 2^31 cells wide).
+using Cell = std::array<int32_t, 2>;
 
-lgd_box bounding_box(const Cell *c, size_t n) {
-  lgd_box b{{c[0][0], c[0][1]}, {c[0][0] + 1, c[0][1] + 1}};
-  for (size_t i = 1; i < n; ++i)
-    for (int d = 0; d < 2; ++d) {
-      if (c[i][d] < b.lo[d]) b.lo[d] = c[i][d];
-      if (c[i][d] + 1 > b.hi[d]) b.hi[d] = c[i][d] + 1;
-    }
-  return b;
+// Grows b (empty: lo > hi) to hold cell c.
+inline void grow(lgd_box &b, const Cell &c) {
+  for (int d = 0; d < 2; ++d) {
+    if (c[d] < b.lo[d]) b.lo[d] = c[d];
+    if (c[d] + 1 > b.hi[d]) b.hi[d] = c[d] + 1;
+  }
 }
+
+inline lgd_box empty_box() { return lgd_box{{INT64_MAX, INT64_MAX}, {INT64_MIN, INT64_MIN}}; }
 
 struct Split {
   int axis;
   int64_t plane;
 };
 
+// The two signatures of a box: flagged cells per column (axis 0) and per row (axis 1).
+struct Sig {
+  const uint32_t *v[2];
+  size_t n[2];
+};
+
 // Interior zero of a signature nearest the middle (first found wins ties).
-bool hole_split(const lgd_box &bx, const std::vector<uint32_t> (&sig)[2], Split &s) {
+bool hole_split(const lgd_box &bx, const Sig &sig, Split &s) {
   bool have = false;
   uint64_t best = 0;
   for (int d = 0; d < 2; ++d) {
-    const size_t n = sig[d].size();
+    const size_t n = sig.n[d];
     for (size_t i = 1; i + 1 < n; ++i) {
-      if (sig[d][i] != 0) continue;
+      if (sig.v[d][i] != 0) continue;
       uint64_t c = std::min<uint64_t>(i, n - 1 - i);
       if (!have || c > best) {
         have = true;
@@ -57,17 +65,17 @@ bool hole_split(const lgd_box &bx, const std::vector<uint32_t> (&sig)[2], Split 
 
 // Strongest sign change of the signature's second difference leaving
 // min_width cells on each side (stronger, then more central, wins).
-bool inflection_split(const lgd_box &bx, const std::vector<uint32_t> (&sig)[2], size_t minw,
-                      Split &s) {
+bool inflection_split(const lgd_box &bx, const Sig &sig, size_t minw, Split &s) {
   bool have = false;
   int64_t best_s = 0;
   uint64_t best_c = 0;
   for (int d = 0; d < 2; ++d) {
-    const size_t n = sig[d].size();
+    const size_t n = sig.n[d];
+    const uint32_t *v = sig.v[d];
     if (n < 4) continue;
     for (size_t i = 1; i + 2 < n; ++i) {
-      int64_t a = (int64_t)sig[d][i + 1] - 2 * (int64_t)sig[d][i] + (int64_t)sig[d][i - 1];
-      int64_t b = (int64_t)sig[d][i + 2] - 2 * (int64_t)sig[d][i + 1] + (int64_t)sig[d][i];
+      int64_t a = (int64_t)v[i + 1] - 2 * (int64_t)v[i] + (int64_t)v[i - 1];
+      int64_t b = (int64_t)v[i + 2] - 2 * (int64_t)v[i + 1] + (int64_t)v[i];
       if (!(a * b < 0)) continue;
       size_t plane = i + 1;
       if (plane < minw || n - plane < minw) continue;
@@ -97,48 +105,65 @@ bool bisect_split(const lgd_box &bx, size_t minw, Split &s) {
 // the order of the one-thread recursion).
 constexpr size_t kParallelFlags = 4096;
 
-void cluster_rec(Cell *f, size_t n, double eff, size_t minw, std::vector<lgd_box> &out,
-                 int depth) {
-  const lgd_box bx = bounding_box(f, n);
+// Berger–Rigoutsos over the n flagged cells f (in any order) whose bounding box is
+// bx (cluster.rs:157-281): accept the box at the efficiency, else split at a
+// signature hole, an inflection or the midpoint and recurse (left half first).  Each
+// half's bounding box is gathered by the partition pass; `sig` is scratch.
+void cluster_rec(Cell *f, size_t n, const lgd_box &bx, double eff, size_t minw,
+                 std::vector<lgd_box> &out, int depth, std::vector<uint32_t> &sig) {
   if ((double)n / (double)bx.cells() >= eff) {
     out.push_back(bx);
     return;
   }
-  std::vector<uint32_t> sig[2];
-  sig[0].assign((size_t)bx.extent(0), 0u);
-  sig[1].assign((size_t)bx.extent(1), 0u);
+  const size_t nx = (size_t)bx.extent(0), ny = (size_t)bx.extent(1);
+  sig.assign(nx + ny, 0u);
+  uint32_t *sx = sig.data(), *sy = sig.data() + nx;
+  const int64_t lx = bx.lo[0], ly = bx.lo[1];
   for (size_t i = 0; i < n; ++i) {
-    sig[0][(size_t)(f[i][0] - bx.lo[0])]++;
-    sig[1][(size_t)(f[i][1] - bx.lo[1])]++;
+    sx[(size_t)(f[i][0] - lx)]++;
+    sy[(size_t)(f[i][1] - ly)]++;
   }
+  const Sig sg{{sx, sy}, {nx, ny}};
   Split s{0, 0};
-  if (!hole_split(bx, sig, s) && !inflection_split(bx, sig, minw, s) && !bisect_split(bx, minw, s)) {
+  if (!hole_split(bx, sg, s) && !inflection_split(bx, sg, minw, s) && !bisect_split(bx, minw, s)) {
     out.push_back(bx);
     return;
   }
   // Two-pointer partition (cluster.rs:270-281): cells below the plane move to the front.
+  lgd_box lb = empty_box(), rb = empty_box();
   size_t lt = 0;
-  for (size_t j = 0; j < n; ++j)
-    if (f[j][s.axis] < s.plane) std::swap(f[lt++], f[j]);
+  for (size_t j = 0; j < n; ++j) {
+    const Cell c = f[j];
+    if (c[s.axis] < s.plane) {
+      grow(lb, c);
+      f[j] = f[lt];
+      f[lt++] = c;
+    } else {
+      grow(rb, c);
+    }
+  }
   if (depth > 0 && n >= kParallelFlags) {
     std::vector<lgd_box> left;
     std::thread t;
     bool spawned = false;
     try {
-      t = std::thread([&] { cluster_rec(f, lt, eff, minw, left, depth - 1); });
+      t = std::thread([&] {
+        std::vector<uint32_t> tsig;
+        cluster_rec(f, lt, lb, eff, minw, left, depth - 1, tsig);
+      });
       spawned = true;
     } catch (...) {
-      cluster_rec(f, lt, eff, minw, left, depth - 1);
+      cluster_rec(f, lt, lb, eff, minw, left, depth - 1, sig);
     }
     std::vector<lgd_box> right;
-    cluster_rec(f + lt, n - lt, eff, minw, right, depth - 1);
+    cluster_rec(f + lt, n - lt, rb, eff, minw, right, depth - 1, sig);
     if (spawned) t.join();
     out.insert(out.end(), left.begin(), left.end());
     out.insert(out.end(), right.begin(), right.end());
     return;
   }
-  cluster_rec(f, lt, eff, minw, out, 0);
-  cluster_rec(f + lt, n - lt, eff, minw, out, 0);
+  cluster_rec(f, lt, lb, eff, minw, out, 0, sig);
+  cluster_rec(f + lt, n - lt, rb, eff, minw, out, 0, sig);
 }
 
 bool box_covered(const lgd_box &need, const std::vector<lgd_box> &cover) {
@@ -232,7 +257,7 @@ void host_flag_bits(const lgd_amr &g, const std::vector<double> &tag_field,
 
 }  // namespace
 
-int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>> &bits,
+int lgd_regrid_propose_bits(const lgd_amr &g, const std::vector<uint64_t *> &bits,
                             const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
   const int capacity = g.max_levels() - 1;
   out.assign((size_t)capacity, {});
@@ -240,11 +265,13 @@ int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>>
     const int src = target - 1;
     const lgd_box dom = g.domain(src);
     const int64_t W = dom.hi[0], H = dom.hi[1], wpr = (W + 63) / 64;
-    std::vector<uint64_t> m;
-    if ((size_t)src < bits.size() && !bits[(size_t)src].empty())
-      m.swap(bits[(size_t)src]);
-    else
-      m.assign((size_t)(H * wpr), 0ull);
+    const size_t words = (size_t)(H * wpr);
+    std::vector<uint64_t> none;
+    uint64_t *m = (size_t)src < bits.size() ? bits[(size_t)src] : nullptr;
+    if (!m) {
+      none.assign(words, 0ull);
+      m = none.data();
+    }
     // Nesting flags: the coarse cells under the level being built above, grown by
     // one cell (adapt.rs:194-211).
     if (target < capacity && !out[(size_t)target].empty()) {
@@ -261,20 +288,36 @@ int lgd_regrid_propose_bits(const lgd_amr &g, std::vector<std::vector<uint64_t>>
       }
     }
     // The flagged cells, each once (the set the reference clusters after its
-    // clip / sort / dedupe; clustering does not depend on their order).
-    std::vector<Cell> flags;
-    for (int64_t y = 0; y < H; ++y)
+    // clip / sort / dedupe; clustering does not depend on their order), and their
+    // bounding box.
+    size_t nflags = 0;
+    for (size_t i = 0; i < words; ++i) nflags += (size_t)__builtin_popcountll(m[i]);
+    if (nflags == 0) continue;
+    std::vector<Cell> flags(nflags);
+    lgd_box fb = empty_box();
+    size_t k = 0;
+    for (int64_t y = 0; y < H; ++y) {
+      const uint64_t *row = m + (size_t)(y * wpr);
+      bool any = false;
       for (int64_t w = 0; w < wpr; ++w) {
-        uint64_t word = m[(size_t)(y * wpr + w)];
+        uint64_t word = row[w];
+        if (!word) continue;
+        any = true;
+        fb.lo[0] = std::min<int64_t>(fb.lo[0], w * 64 + __builtin_ctzll(word));
+        fb.hi[0] = std::max<int64_t>(fb.hi[0], w * 64 + 64 - __builtin_clzll(word));
         while (word) {
-          const int bit = __builtin_ctzll(word);
+          flags[k++] = Cell{(int32_t)(w * 64 + __builtin_ctzll(word)), (int32_t)y};
           word &= word - 1;
-          flags.push_back({w * 64 + bit, y});
         }
       }
-    if (flags.empty()) continue;
+      if (any) {
+        fb.lo[1] = std::min<int64_t>(fb.lo[1], y);
+        fb.hi[1] = y + 1;
+      }
+    }
     std::vector<lgd_box> boxes;
-    cluster_rec(flags.data(), flags.size(), pol.efficiency, (size_t)pol.min_width, boxes, 3);
+    std::vector<uint32_t> sig;
+    cluster_rec(flags.data(), flags.size(), fb, pol.efficiency, (size_t)pol.min_width, boxes, 3, sig);
     const int64_t r = g.ratios[(size_t)target - 1];
     for (lgd_box &b : boxes)
       for (int d = 0; d < 2; ++d) {
@@ -316,9 +359,12 @@ void lgd_regrid_bits_from_cells(const lgd_amr &g, int src, const std::vector<int
 int lgd_regrid_propose(const lgd_amr &g, const std::vector<double> &tag_field,
                        const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
   std::vector<std::vector<uint64_t>> bits((size_t)g.num_levels());
-  for (int src = 0; src < g.num_levels() && src < g.max_levels() - 1; ++src)
+  std::vector<uint64_t *> ptr((size_t)g.num_levels(), nullptr);
+  for (int src = 0; src < g.num_levels() && src < g.max_levels() - 1; ++src) {
     host_flag_bits(g, tag_field, pol, src, bits[(size_t)src]);
-  return lgd_regrid_propose_bits(g, bits, pol, out);
+    ptr[(size_t)src] = bits[(size_t)src].data();
+  }
+  return lgd_regrid_propose_bits(g, ptr, pol, out);
 }
 
 void lgd_regrid_migrate(const lgd_amr &old_g, const std::vector<double> &old_f, const lgd_amr &new_g,
@@ -482,9 +528,9 @@ bool lgd_regrid_device_supported(const lgd_amr &g) { return g.max_levels() <= kM
 
 int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field,
                          const lgd_regrid_policy &pol, lgd_regrid_scratch &sc,
-                         std::vector<std::vector<uint64_t>> &bits) {
+                         std::vector<uint64_t *> &bits) {
   const int levels = std::min(g.num_levels(), g.max_levels() - 1);
-  bits.assign((size_t)g.num_levels(), {});
+  bits.assign((size_t)g.num_levels(), nullptr);
   if (levels <= 0) return LGD_OK;
   if (pol.buffer > 63)
     return fail(ctx, LGD_ERR_UNSUPPORTED, "device regrid dilates by at most 63 cells");
@@ -500,12 +546,18 @@ int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field
   if (sc.wcap < wtotal) {
     (void)hipFree(sc.T);
     (void)hipFree(sc.bits);
+    if (sc.host) (void)hipHostFree(sc.host);
     sc.T = nullptr;
     sc.bits = nullptr;
+    sc.host = nullptr;
     sc.wcap = 0;
     e = hipMalloc((void **)&sc.T, wtotal * sizeof(uint64_t));
     if (e == hipSuccess) e = hipMalloc((void **)&sc.bits, wtotal * sizeof(uint64_t));
-    if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag scratch");
+    if (e == hipSuccess) e = hipHostMalloc((void **)&sc.host, wtotal * sizeof(uint64_t), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      sc.host = nullptr;
+      return hip_fail(ctx, e, "regrid tag scratch");
+    }
     sc.wcap = wtotal;
   }
   e = hipMemsetAsync(sc.T, 0, wtotal * sizeof(uint64_t), ctx->stream);
@@ -526,15 +578,11 @@ int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field
         T, W, H, wpr, buffer, (unsigned long long *)(sc.bits + wofs[(size_t)src]));
     if ((rc = launch_check(ctx, "regrid dilate"))) return rc;
   }
-  std::vector<uint64_t> host(wtotal);
-  e = hipMemcpyAsync(host.data(), sc.bits, wtotal * sizeof(uint64_t), hipMemcpyDeviceToHost,
+  e = hipMemcpyAsync(sc.host, sc.bits, wtotal * sizeof(uint64_t), hipMemcpyDeviceToHost,
                      ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "regrid bitmask download");
-  for (int src = 0; src < levels; ++src) {
-    const size_t end = (size_t)src + 1 < wofs.size() ? wofs[(size_t)src + 1] : wtotal;
-    bits[(size_t)src].assign(host.begin() + (ptrdiff_t)wofs[(size_t)src], host.begin() + (ptrdiff_t)end);
-  }
+  for (int src = 0; src < levels; ++src) bits[(size_t)src] = sc.host + wofs[(size_t)src];
   return LGD_OK;
 }
 
@@ -561,5 +609,6 @@ int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *
 void lgd_regrid_scratch_free(lgd_regrid_scratch &sc) {
   (void)hipFree(sc.T);
   (void)hipFree(sc.bits);
+  if (sc.host) (void)hipHostFree(sc.host);
   sc = lgd_regrid_scratch{};
 }

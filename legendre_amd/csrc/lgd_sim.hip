@@ -507,10 +507,13 @@ int amr_regrid(lgd_sim *s) {
       lgd_regrid_bits_from_cells(g, src, sink.cells, s->policy.buffer, bits[(size_t)src]);
     }
     s->regrid_seconds[0] += seconds_since(clock);
-    changed = lgd_regrid_propose_bits(g, bits, s->policy, levels);
+    std::vector<uint64_t *> ptr(bits.size(), nullptr);
+    for (size_t l = 0; l < bits.size(); ++l)
+      if (!bits[l].empty()) ptr[l] = bits[l].data();
+    changed = lgd_regrid_propose_bits(g, ptr, s->policy, levels);
   } else if (dev) {
     if ((rc = gradient_prepare(s))) return rc;
-    std::vector<std::vector<uint64_t>> bits;
+    std::vector<uint64_t *> bits;
     if ((rc = lgd_regrid_flag_bits(s->ctx, g, s->state[(size_t)tf]->data, s->policy, s->rscratch,
                                    bits)))
       return rc;
