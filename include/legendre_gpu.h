@@ -109,9 +109,10 @@ const char *lgd_ctx_last_error(const lgd_ctx *ctx);
  * patch —, 1 one thread per cell in tiles), "amr_rows" (rows per strip of the
  * streaming stage, 0 = spread over the device's SIMDs), "amr_tile" and "amr_wg" (the
  * AMR launch shapes lgd_sim_autotune_amr chooses; 0 = 64×4 tiles and 256-wide fills) and
- * "amr_split_fill" (1: every substep of the
- * finest level fills its ghosts in a launch of its own; 0, the default: a substep's
- * stage launch also fills the ghosts the next substep reads).
+ * "amr_split_fill" (1: every fill and every restriction is a launch of its own; 0,
+ * the default: a substep's stage launch on the finest level also fills the ghosts
+ * the next substep reads, and on a level with a finer one the restriction ending a
+ * substep shares a launch with the next substep's fill).
  * Results are bitwise independent of these knobs.  The adjoint has two more:
  * "adjoint_kernel" (0: the streaming reverse step wherever the grid allows, else the
  * three-pass form; 1: always the three passes; 2: streaming, error if the grid does

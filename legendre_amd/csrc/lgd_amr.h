@@ -75,6 +75,14 @@ struct lgd_corner {
   double weight;
 };
 
+// Of a copy op whose source cell is covered by the next finer level: the element
+// of that cell's first child (child < 0: not covered) and the children's slab row
+// stride, so a launch that restricts the finer level can also fill the ghost.
+struct lgd_copy_child {
+  int64_t child;
+  int64_t stride;
+};
+
 // One covered coarse cell: target element, child (0,0) element, fine row stride.
 struct lgd_restrict_item {
   int64_t target;
@@ -97,8 +105,9 @@ struct lgd_level_plan {
   std::vector<int64_t> tile0;  // first tile of each of the level's patches, + end
   lgd_fill_op *ops = nullptr;
   int64_t nops = 0;
-  lgd_corner *corners = nullptr;
+  lgd_corner *corners = nullptr;  // four slots per op: op t's corners start at 4t
   int64_t ncorners = 0;
+  lgd_copy_child *cchild = nullptr;  // per op, on levels with a finer level (else null)
   lgd_restrict_item *ritems = nullptr;  // restriction of this level onto level-1
   int64_t nritems = 0;
   // The streaming level stage: column segments of every patch cut into strips of
@@ -230,6 +239,13 @@ int lgd_amr_fill_pp(lgd_ctx *ctx, const lgd_amr &g, int level, int nf, double *c
 // Restriction of fine_level into its parent, both in their current buffers.
 int lgd_amr_restrict_pp(lgd_ctx *ctx, const lgd_amr &g, int fine_level, int nf,
                         const double *const *fine, double *const *coarse);
+// Both in one launch: restriction of level + 1 (`fine`, current) into `level`
+// (lvl, current) and the fill of `level` as lgd_amr_fill_pp; a ghost copied from a
+// cell the restriction writes takes the restricted value, computed the same way.
+// Bitwise the two launches in sequence.
+int lgd_amr_restrict_fill_pp(lgd_ctx *ctx, const lgd_amr &g, int level, int nf, double *const *lvl,
+                             const double *const *par, const double *const *old, double alpha,
+                             const double *const *fine);
 // Whether level 0 steps through the Cartesian streaming kernel (which reads
 // neighbour interiors itself, so level 0 needs no ghost fill).
 bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g);

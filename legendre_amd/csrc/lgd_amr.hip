@@ -359,7 +359,8 @@ __device__ inline int plan_bucket(const int64_t *pre, int n, int64_t t) {
 __global__ void plan_fill_kernel(lgd_dev_index ix, int l, uint32_t b0, int32_t np,
                                  const int64_t *__restrict__ ring0, int64_t total, int64_t dw,
                                  int64_t dh, int64_t cw, int64_t ch, double r, lgd_fill_op *ops,
-                                 lgd_corner *corners, int32_t *err) {
+                                 lgd_corner *corners, int64_t rf, lgd_copy_child *cchild,
+                                 int32_t *err) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const int p = plan_bucket(ring0, np, t);
@@ -387,8 +388,20 @@ __global__ void plan_fill_kernel(lgd_dev_index ix, int l, uint32_t b0, int32_t n
     op.cstart = (int32_t)src;
     op.clen = 0;
     ops[t] = op;
+    if (cchild) {
+      // The source cell's children at level l + 1 (patches there are aligned to rf).
+      const int64_t fb = lgd_dev_find_patch(ix, l + 1, tx * rf, ty * rf);
+      lgd_copy_child cc{-1, 0};
+      if (fb >= 0) {
+        const lgd_block_desc fd = ix.blocks[fb];
+        cc.stride = (int64_t)fd.nx + 2;
+        cc.child = fd.off + (ty * rf - ix.lo[2 * fb + 1] + 1) * cc.stride + (tx * rf - ix.lo[2 * fb] + 1);
+      }
+      cchild[t] = cc;
+    }
     return;
   }
+  if (cchild) cchild[t] = lgd_copy_child{-1, 0};
   op.src = -1;
   op.cstart = (int32_t)(4 * t);
   int n = 0;
@@ -575,6 +588,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     to.reserve((size_t)P.nops, &P.ops);
     to.reserve((size_t)P.ncorners, &P.corners);
     to.reserve((size_t)P.nritems, &P.ritems);
+    if (l + 1 < g.num_levels()) to.reserve((size_t)P.nops, &P.cchild);
   }
   if (!take0 && (rc = pk0.upload(ctx, &g.plans[0].block))) return rc;
   if ((rc = pk.upload(ctx, &g.d_arena))) return rc;
@@ -591,7 +605,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
       const double r = l > 0 ? (double)g.ratios[(size_t)l - 1] : 1.0;
       plan_fill_kernel<<<blocks_for(P.nops, 256), 256, 0, ctx->stream>>>(
           ix, l, b0, np, Hl.d_ring0, P.nops, d.hi[0], d.hi[1], cd.hi[0], cd.hi[1], r, P.ops,
-          P.corners, g.d_err);
+          P.corners, l + 1 < g.num_levels() ? (int64_t)g.ratios[(size_t)l] : 0, P.cchild, g.d_err);
       if ((rc = launch_check(ctx, "amr fill plan"))) return rc;
     }
     if (l >= 1 && P.nritems > 0) {
@@ -609,25 +623,43 @@ namespace {
 // Apply one level's fill plan (fill_level, intergrid.rs:183-223): copy or
 // prolong (acc from 0 in corner order), optionally blended in time between the
 // parent's snapshot `old` and its current state: old·(1−α) + new·α.
+// Op t's four corner slots (4t .. 4t + 3; the plan's cstart), read beside the op
+// itself so the corner loads do not wait for it; slots past clen hold no corner
+// and are never followed.
+__device__ __forceinline__ void load_corners(const lgd_corner *__restrict__ corners, int64_t t,
+                                             lgd_corner (&k)[4]) {
+#pragma unroll
+  for (int c = 0; c < 4; ++c) k[c] = corners[4 * t + c];
+}
+
+// The prolongation of one field at an op (corners in order, acc from 0), blended
+// in time: old·(1−α) + new·α.
+template <bool kTime>
+__device__ __forceinline__ double prolong_value(const lgd_corner (&k)[4], int clen, const double *nw,
+                                                const double *old, double one_minus_alpha,
+                                                double alpha) {
+  double vn = 0.0, vo = 0.0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    if (c < clen) {
+      vn = vn + nw[k[c].src] * k[c].weight;
+      if (kTime) vo = vo + old[k[c].src] * k[c].weight;
+    }
+  }
+  return kTime ? vo * one_minus_alpha + vn * alpha : vn;
+}
+
 template <bool kTime>
 __global__ void amr_fill_kernel(const lgd_fill_op *__restrict__ ops, int64_t nops,
                                 const lgd_corner *__restrict__ corners, double *field,
                                 const double *old, double one_minus_alpha, double alpha) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nops) return;
+  lgd_corner k[4];
+  load_corners(corners, t, k);
   const lgd_fill_op op = ops[t];
-  double v;
-  if (op.clen == 0) {
-    v = field[op.src];
-  } else {
-    double vn = 0.0, vo = 0.0;
-    for (int c = 0; c < op.clen; ++c) {
-      const lgd_corner k = corners[op.cstart + c];
-      vn = vn + field[k.src] * k.weight;
-      if (kTime) vo = vo + old[k.src] * k.weight;
-    }
-    v = kTime ? vo * one_minus_alpha + vn * alpha : vn;
-  }
+  const double v = op.clen == 0 ? field[op.src]
+                                : prolong_value<kTime>(k, op.clen, field, old, one_minus_alpha, alpha);
   field[op.target] = v;
 }
 
@@ -663,20 +695,13 @@ __global__ void amr_fill_pp_kernel(const lgd_fill_op *__restrict__ ops, int64_t 
                                    double one_minus_alpha, double alpha) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nops) return;
+  lgd_corner k[4];
+  load_corners(corners, t, k);
   const lgd_fill_op op = ops[t];
   for (int f = 0; f < nf; ++f) {
-    double v;
-    if (op.clen == 0) {
-      v = F.lvl[f][op.src];
-    } else {
-      double vn = 0.0, vo = 0.0;
-      for (int c = 0; c < op.clen; ++c) {
-        const lgd_corner k = corners[op.cstart + c];
-        vn = vn + F.par[f][k.src] * k.weight;
-        if (kTime) vo = vo + F.old[f][k.src] * k.weight;
-      }
-      v = kTime ? vo * one_minus_alpha + vn * alpha : vn;
-    }
+    const double v = op.clen == 0
+                         ? F.lvl[f][op.src]
+                         : prolong_value<kTime>(k, op.clen, F.par[f], F.old[f], one_minus_alpha, alpha);
     F.lvl[f][op.target] = v;
   }
 }
@@ -688,16 +713,57 @@ struct RestrictPtrs {
   double *coarse[2];
 };
 
+// restrict_level's value at one coarse cell: (Σ children, dimension 0 fastest,
+// from 0) · 1/r².
+__device__ __forceinline__ double restricted(const double *fine, int64_t child, int64_t stride,
+                                             int r, double inv) {
+  double sum = 0.0;
+  for (int ky = 0; ky < r; ++ky)
+    for (int kx = 0; kx < r; ++kx) sum = sum + fine[child + ky * stride + kx];
+  return sum * inv;
+}
+
 __global__ void amr_restrict_pp_kernel(const lgd_restrict_item *__restrict__ items, int64_t n,
                                        int r, double inv, RestrictPtrs R, int nf) {
   int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const lgd_restrict_item it = items[t];
+  for (int f = 0; f < nf; ++f) R.coarse[f][it.target] = restricted(R.fine[f], it.child, it.stride, r, inv);
+}
+
+// Restriction of level + 1 into `level` (threads [0, nitems)) and the fill of
+// `level` (threads from nitems on) in one launch.  The restriction writes interior
+// cells under the finer level and the fill writes ghost cells, so the two never
+// write one element; a ghost copied from a cell under the finer level takes the
+// restricted value from the children itself instead of reading the cell.
+template <bool kTime>
+__global__ void amr_restrict_fill_pp_kernel(const lgd_restrict_item *__restrict__ items,
+                                            int64_t nitems, int r, double inv,
+                                            const lgd_fill_op *__restrict__ ops, int64_t nops,
+                                            const lgd_corner *__restrict__ corners,
+                                            const lgd_copy_child *__restrict__ cchild, FieldPtrs F,
+                                            RestrictPtrs R, int nf, double one_minus_alpha,
+                                            double alpha) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nitems) {
+    const lgd_restrict_item it = items[t];
+    for (int f = 0; f < nf; ++f)
+      R.coarse[f][it.target] = restricted(R.fine[f], it.child, it.stride, r, inv);
+    return;
+  }
+  const int64_t o = t - nitems;
+  if (o >= nops) return;
+  lgd_corner k[4];
+  load_corners(corners, o, k);
+  const lgd_copy_child cc = cchild[o];
+  const lgd_fill_op op = ops[o];
   for (int f = 0; f < nf; ++f) {
-    double sum = 0.0;
-    for (int ky = 0; ky < r; ++ky)
-      for (int kx = 0; kx < r; ++kx) sum = sum + R.fine[f][it.child + ky * it.stride + kx];
-    R.coarse[f][it.target] = sum * inv;
+    double v;
+    if (op.clen == 0)
+      v = cc.child >= 0 ? restricted(R.fine[f], cc.child, cc.stride, r, inv) : F.lvl[f][op.src];
+    else
+      v = prolong_value<kTime>(k, op.clen, F.par[f], F.old[f], one_minus_alpha, alpha);
+    F.lvl[f][op.target] = v;
   }
 }
 
@@ -1042,6 +1108,37 @@ int lgd_amr_restrict_pp(lgd_ctx *ctx, const lgd_amr &g, int fine_level, int nf,
   amr_restrict_pp_kernel<<<blocks_for(P.nritems, wg), wg, 0, ctx->stream>>>(
       P.ritems, P.nritems, r, 1.0 / (double)(r * r), R, nf);
   return launch_check(ctx, "amr_restrict");
+}
+
+int lgd_amr_restrict_fill_pp(lgd_ctx *ctx, const lgd_amr &g, int level, int nf, double *const *lvl,
+                             const double *const *par, const double *const *old, double alpha,
+                             const double *const *fine) {
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  const lgd_level_plan &Q = g.plans[(size_t)level + 1];
+  if (P.nops > 0 && !P.cchild)
+    return fail(ctx, LGD_ERR_INVALID_ARGUMENT, "restrict_fill: the level has no finer level");
+  const int64_t total = Q.nritems + P.nops;
+  if (total == 0) return LGD_OK;
+  FieldPtrs F{};
+  RestrictPtrs R{};
+  for (int f = 0; f < nf; ++f) {
+    F.lvl[f] = lvl[f];
+    F.par[f] = par ? par[f] : nullptr;
+    F.old[f] = old ? old[f] : nullptr;
+    R.fine[f] = fine[f];
+    R.coarse[f] = lvl[f];
+  }
+  const int r = (int)g.ratios[(size_t)level];
+  const double inv = 1.0 / (double)(r * r);
+  const int wg = lgd_amr_wg(ctx);
+  const unsigned nb = blocks_for(total, wg);
+  if (old)
+    amr_restrict_fill_pp_kernel<true><<<nb, wg, 0, ctx->stream>>>(
+        Q.ritems, Q.nritems, r, inv, P.ops, P.nops, P.corners, P.cchild, F, R, nf, 1.0 - alpha, alpha);
+  else
+    amr_restrict_fill_pp_kernel<false><<<nb, wg, 0, ctx->stream>>>(
+        Q.ritems, Q.nritems, r, inv, P.ops, P.nops, P.corners, P.cchild, F, R, nf, 0.0, 0.0);
+  return launch_check(ctx, "amr_restrict_fill");
 }
 
 bool lgd_amr_level0_streams(const lgd_ctx *ctx, const lgd_amr &g) {

@@ -36,7 +36,7 @@ struct lgd_ctx {
   int64_t modelc_strips = 0;  // strips (row ranges) per block for the streaming kernel
   int64_t modelc_prefetch = 0;  // rows the streaming kernel loads ahead (1 or 2; 0 = default)
   int64_t amr_per_block = 0;    // 1 = AMR level kernels launched once per patch (baseline)
-  int64_t amr_split_fill = 0;   // 1 = the finest level's ghost fills in launches of their own
+  int64_t amr_split_fill = 0;   // 1 = every AMR fill and restriction in a launch of its own
   int64_t amr_tile = 0;         // AMR level-stage tile shape (0 = the plans' 64×4)
   int64_t amr_wg = 0;           // AMR fill / restriction workgroup width (0 = 256)
   int64_t amr_kernel = 0;       // AMR Model C level stage: 0/2 = streaming, 1 = per-cell tiles

@@ -356,6 +356,21 @@ int amr_fill_level(lgd_sim *s, int level, bool blend, double alpha) {
                          level > 0 && blend ? old : nullptr, alpha);
 }
 
+// Restrict level + 1 into `level` and fill `level`'s ghosts (as amr_fill_level with
+// blending) in one launch.
+int amr_restrict_fill_level(lgd_sim *s, int level, double alpha) {
+  const int nf = nfields(s);
+  double *lvl[2] = {nullptr, nullptr};
+  const double *par[2] = {nullptr, nullptr}, *old[2] = {nullptr, nullptr}, *fine[2] = {nullptr, nullptr};
+  for (int f = 0; f < nf; ++f) {
+    lvl[f] = amr_cur(s, f, level)->data;
+    par[f] = amr_cur(s, f, level - 1)->data;
+    old[f] = amr_prev(s, f, level - 1)->data;
+    fine[f] = amr_cur(s, f, level + 1)->data;
+  }
+  return lgd_amr_restrict_fill_pp(s->ctx, *s->amr, level, nf, lvl, par, old, alpha, fine);
+}
+
 // Restrict `fine_level` into its parent (both current).
 int amr_restrict_level(lgd_sim *s, int fine_level) {
   const int nf = nfields(s);
@@ -421,10 +436,18 @@ int amr_global_step(lgd_sim *s, double dt, uint64_t salt) {
 // fill the level's ghosts (coarse source blended in time between the parent's
 // previous and current values), step it, recurse n = round(dt_l/dt_{l+1}) times
 // into the finer level at t + s·dt_child (fma), then restrict it back.
-int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
+// Between two substeps of a level that has a finer level, the restriction that
+// ends one substep (defer_out) and the fill that starts the next (fuse_in) share a
+// launch; the last substep restricts on its own.
+int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha, bool fuse_in = false,
+                bool defer_out = false) {
   const lgd_amr &g = *s->amr;
   int rc;
-  if ((rc = amr_fill_level(s, level, true, alpha))) return rc;
+  if (fuse_in) {
+    if ((rc = amr_restrict_fill_level(s, level, alpha))) return rc;
+  } else if ((rc = amr_fill_level(s, level, true, alpha))) {
+    return rc;
+  }
   uint64_t salt;
   std::memcpy(&salt, &t, sizeof salt);
   if ((rc = amr_level_step(s, level, dt, salt, level))) return rc;
@@ -437,6 +460,9 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
     // launch before them (one fill and nn stage launches instead of nn of each).
     const bool fused = child + 1 == g.num_levels() && !s->ctx->amr_per_block &&
                        !s->ctx->amr_split_fill && nn > 1;
+    // A child with a finer level: its restriction and its next fill share launches.
+    const bool chain = child + 1 < g.num_levels() && !s->ctx->amr_per_block &&
+                       !s->ctx->amr_split_fill && nn > 1;
     for (uint64_t k = 0; k < nn; ++k) {
       if (s->capturing) {
         time_substep_kernel<<<1, 1, 0, s->ctx->stream>>>(s->d_t, child, level, (double)k, dt_child);
@@ -444,7 +470,8 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
       }
       const double tk = std::fma((double)k, dt_child, t), ak = (double)k / (double)nn;
       if (!fused) {
-        if ((rc = amr_advance(s, child, tk, dt_child, ak))) return rc;
+        if ((rc = amr_advance(s, child, tk, dt_child, ak, chain && k > 0, chain && k + 1 < nn)))
+          return rc;
         continue;
       }
       // Subcycle::advance of the finest level (no recursion below it).
@@ -454,7 +481,7 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha) {
       const double next = k + 1 < nn ? (double)(k + 1) / (double)nn : -1.0;
       if ((rc = amr_level_step(s, child, dt_child, salt_k, child, next))) return rc;
     }
-    if ((rc = amr_restrict_level(s, child))) return rc;
+    if (!defer_out && (rc = amr_restrict_level(s, child))) return rc;
   }
   return LGD_OK;
 }
