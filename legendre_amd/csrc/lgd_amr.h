@@ -96,6 +96,7 @@ struct lgd_dev_level {
   int64_t w, h;
   int64_t nbx, nby;
   int64_t start0;
+  int64_t bucket;  // bucket side in cells
 };
 
 struct lgd_level_plan {
@@ -156,9 +157,12 @@ struct lgd_amr {
   std::vector<lgd_box> box;           // per patch, level cell coordinates
   std::vector<uint32_t> level_start;  // populated levels + 1
   std::vector<std::array<double, 2>> spacing;  // per level, ratio capacity
-  // Per populated level: patches bucketed by 16×16-cell squares of the level domain
-  // (CSR: bucket k holds ids[start[k] .. start[k+1])), for find_patch.
+  // Per populated level: patches bucketed by b×b-cell squares of the level domain
+  // (CSR: bucket k holds ids[start[k] .. start[k+1])), for find_patch.  b is kBucket
+  // at level 0 and grows with the refinement ratios, so every level's buckets cover
+  // the same area of the domain.
   struct bucket_index {
+    int64_t b = 16;
     int64_t nbx = 0, nby = 0;
     std::vector<uint32_t> start, ids;
   };
@@ -199,7 +203,7 @@ __device__ inline int64_t lgd_dev_find_patch(const lgd_dev_index &ix, int l, int
   if (l >= ix.nlevels) return -1;
   const lgd_dev_level L = ix.levels[l];
   if (x < 0 || y < 0 || x >= L.w || y >= L.h) return -1;
-  const int64_t k = L.start0 + x / lgd_amr::kBucket + L.nbx * (y / lgd_amr::kBucket);
+  const int64_t k = L.start0 + x / L.bucket + L.nbx * (y / L.bucket);
   for (uint32_t i = ix.bstart[k]; i < ix.bstart[k + 1]; ++i) {
     const uint32_t p = ix.bids[i];
     const int64_t px = ix.lo[2 * p], py = ix.lo[2 * p + 1];

@@ -162,9 +162,11 @@ bool covered(const lgd_amr &g, int l, const lgd_box &need) {
 // Patches of one level do not overlap; each 16×16-cell bucket of the level domain
 // lists the few that touch it (find_patch's index for level l, appended to lookup).
 void build_bucket_index(lgd_amr &g, int l) {
-  const int64_t B = lgd_amr::kBucket;
+  int64_t B = lgd_amr::kBucket;
+  for (int i = 0; i < l; ++i) B *= (int64_t)g.ratios[(size_t)i];
   const lgd_box d = g.domain(l);
   lgd_amr::bucket_index ix;
+  ix.b = B;
   ix.nbx = (d.hi[0] + B - 1) / B;
   ix.nby = (d.hi[1] + B - 1) / B;
   ix.start.assign((size_t)(ix.nbx * ix.nby + 1), 0);
@@ -197,7 +199,7 @@ int64_t lgd_amr::find_patch(int l, int64_t x, int64_t y) const {
   lgd_box d = domain(l);
   if (!d.contains(x, y)) return -1;
   const bucket_index &ix = lookup[(size_t)l];
-  const int64_t k = (x / kBucket) + ix.nbx * (y / kBucket);
+  const int64_t k = (x / ix.b) + ix.nbx * (y / ix.b);
   for (uint32_t i = ix.start[(size_t)k]; i < ix.start[(size_t)k + 1]; ++i)
     if (box[ix.ids[i]].contains(x, y)) return (int64_t)ix.ids[i];
   return -1;
@@ -516,7 +518,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     for (int l = 0; l < g.num_levels(); ++l) {
       const lgd_amr::bucket_index &ix = g.lookup[(size_t)l];
       const lgd_box d = g.domain(l);
-      lv.push_back(lgd_dev_level{d.hi[0], d.hi[1], ix.nbx, ix.nby, (int64_t)bstart.size()});
+      lv.push_back(lgd_dev_level{d.hi[0], d.hi[1], ix.nbx, ix.nby, (int64_t)bstart.size(), ix.b});
       const uint32_t base = (uint32_t)bids.size();
       for (uint32_t s : ix.start) bstart.push_back(base + s);
       bids.insert(bids.end(), ix.ids.begin(), ix.ids.end());

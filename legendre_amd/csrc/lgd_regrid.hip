@@ -166,18 +166,15 @@ void cluster_rec(Cell *f, size_t n, const lgd_box &bx, double eff, size_t minw,
   cluster_rec(f + lt, n - lt, rb, eff, minw, out, 0, sig);
 }
 
-bool box_covered(const lgd_box &need, const std::vector<lgd_box> &cover) {
+// Whether every cell of `need` lies in a patch of level l of g (run-skipping walk
+// along axis 0 through the level's bucket index).
+bool box_covered(const lgd_amr &g, int l, const lgd_box &need) {
   for (int64_t y = need.lo[1]; y < need.hi[1]; ++y) {
     int64_t x = need.lo[0];
     while (x < need.hi[0]) {
-      const lgd_box *hit = nullptr;
-      for (const lgd_box &b : cover)
-        if (b.contains(x, y)) {
-          hit = &b;
-          break;
-        }
-      if (!hit) return false;
-      x = hit->hi[0];
+      const int64_t p = g.find_patch(l, x, y);
+      if (p < 0) return false;
+      x = g.box[(size_t)p].hi[0];
     }
   }
   return true;
@@ -330,13 +327,12 @@ int lgd_regrid_propose_bits(const lgd_amr &g, const std::vector<uint64_t *> &bit
   // Hysteresis: keep the current hierarchy while it still serves.
   if ((int)out.size() != g.num_levels() - 1) return 1;
   for (size_t l = 0; l < out.size(); ++l) {
-    std::vector<lgd_box> cur(g.box.begin() + g.level_start[l + 1], g.box.begin() + g.level_start[l + 2]);
     uint64_t pc = 0, cc = 0;
     for (const lgd_box &b : out[l]) pc += b.cells();
-    for (const lgd_box &b : cur) cc += b.cells();
+    for (uint32_t p = g.level_start[l + 1]; p < g.level_start[l + 2]; ++p) cc += g.box[p].cells();
     if (cc > pc + pc / 2) return 1;
     for (const lgd_box &b : out[l])
-      if (!box_covered(b, cur)) return 1;
+      if (!box_covered(g, (int)l + 1, b)) return 1;
   }
   return 0;
 }
