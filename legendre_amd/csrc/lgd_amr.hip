@@ -890,14 +890,15 @@ __global__ __launch_bounds__(kTX *kTY) void amr_modelc_step_kernel(
 // y-face flux once, as the top face of one row and the bottom face of the next.
 // These are the operands, in the order, of the per-cell evaluation
 // (src/discretization/finite_volume.rs:261-280; src/physics/phasefield.rs:395-421),
-// so the results are bitwise those of amr_modelc_step_kernel.  With kFill,
-// workgroups from nwg_seg on write the next substep's ghosts as that kernel does.
+// so the results are bitwise those of amr_modelc_step_kernel.  With kFill, the
+// first fill_wg workgroups write the next substep's ghosts as that kernel does.
 constexpr int kSegWaves = 4;
 
 struct SegArgs {
   const lgd_sseg *segs;
   int64_t nsegs;
   int32_t nwg_seg;  // workgroups of the streaming part
+  int32_t fill_wg;  // workgroups of the next fill before it (a multiple of 8; 0: none)
   lgd_kr_consts k;
   double dt;
   const double *phi;
@@ -914,8 +915,10 @@ template <bool kNoise, bool kFill, int kDepth>
 __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
     SegArgs a, const lgd_block_desc *__restrict__ blocks, NextFill N) {
   using namespace lgd_stream;
-  if (kFill && (int)blockIdx.x >= a.nwg_seg) {
-    const int64_t t = (int64_t)(blockIdx.x - a.nwg_seg) * blockDim.x + threadIdx.x;
+  // The fill's workgroups come first, so that they run beside the streaming waves
+  // rather than after them (their copy ghosts each recompute a whole cell).
+  if (kFill && (int)blockIdx.x < a.fill_wg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N.nops) return;
     const uint64_t st = kNoise ? lgd_salt(a.salt, a.salt_src) : 0;
     const lgd_fill_op op = N.ops[t];
@@ -934,9 +937,9 @@ __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
     }
     return;
   }
-  // XCD-aware order: XCD x (= blockIdx.x % 8) takes one contiguous range of items,
-  // so the strips of a segment, which share their halo rows, meet in one L2.
-  const int wg = blockIdx.x;
+  // XCD-aware order: XCD x (= blockIdx.x % 8, also wg % 8) takes one contiguous range
+  // of items, so the strips of a segment, which share their halo rows, meet in one L2.
+  const int wg = (int)blockIdx.x - (kFill ? a.fill_wg : 0);
   const int xcd = wg % 8, idx = wg / 8;
   const int per = a.nwg_seg / 8, rem = a.nwg_seg % 8;
   const int lw = xcd * per + min(xcd, rem) + idx;
@@ -1338,12 +1341,14 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
     const NextFill N = fill ? next_fill_args(P, nf, 2) : NextFill{};
     auto go = [&](const lgd_sseg *segs, int64_t n, bool with_fill) -> int {
       const int64_t nwg = (n + kSegWaves - 1) / kSegWaves;
-      const int64_t total = nwg + (with_fill ? (P.nops + 64 * kSegWaves - 1) / (64 * kSegWaves) : 0);
+      const int64_t fwg = with_fill ? ((P.nops + 64 * kSegWaves - 1) / (64 * kSegWaves) + 7) / 8 * 8 : 0;
+      const int64_t total = nwg + fwg;
       if (total == 0) return LGD_OK;
       if (total > INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "AMR level too large for one launch");
       A.segs = segs;
       A.nsegs = n;
       A.nwg_seg = (int32_t)nwg;
+      A.fill_wg = (int32_t)fwg;
       const dim3 grid((unsigned)total), blk(64 * kSegWaves);
       hipStream_t st = ctx->stream;
       if (noise) {
