@@ -144,6 +144,10 @@ int lgd_amr_default_seg_rows(const lgd_amr &g, int level);
 // Whether `level` steps through the streaming level stage (Model C, batched
 // launches, the "amr_kernel" knob not asking for the per-cell tiles).
 bool lgd_amr_level_streams(const lgd_ctx *ctx, const lgd_amr &g, int level);
+// Whether the first fill of level + 1 may share the stage launch of `level`
+// (lgd_amr_modelc_step's `cf`): a streamed level with a finer one, batched launches,
+// the "amr_split_fill" knob unset.
+bool lgd_amr_child_fill_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level);
 
 // Workgroup width of the AMR fill and restriction launches ("amr_wg"; 256 by default).
 inline int lgd_amr_wg(const lgd_ctx *ctx) { return ctx->amr_wg > 0 ? (int)ctx->amr_wg : 256; }
@@ -265,12 +269,23 @@ struct lgd_amr_next_fill {
   const double *old[2];
   double alpha;
 };
+// The first fill of level + 1 at α = 0 in the stage launch of `level`: the child's
+// current buffers (fields 0, 1), whose ghosts are written.  A prolonged ghost is
+// old·1 + new·0 of the parent; the corner sum of the old values is never −0 (it
+// starts from +0), so that is the old sum bit for bit whenever the new values are
+// finite: the bits of the fill launched after the stage for every finite state.
+struct lgd_amr_child_fill {
+  double *lvl[2];
+};
 // One fused Euler–Maruyama stage of `level`: (phi, u) → (phi_n, u_n) interiors, and
-// with `nf` the ghosts of (phi_n, u_n) the level's next substep reads.
+// with `nf` the ghosts of (phi_n, u_n) the level's next substep reads, or with `cf`
+// (a streamed level with a finer one; LGD_ERR_UNSUPPORTED otherwise) the first fill of
+// level + 1.
 int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
                         double dt, const double *phi, const double *u, double *phi_n, double *u_n,
                         uint64_t seed, uint64_t salt, bool wiener,
-                        const lgd_amr_next_fill *nf = nullptr);
+                        const lgd_amr_next_fill *nf = nullptr,
+                        const lgd_amr_child_fill *cf = nullptr);
 int lgd_amr_heat_step(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, double dt,
                       const double *u, double *u_n, const lgd_amr_next_fill *nf = nullptr);
 // Copy one level's range (interiors and ghosts) between buffers of the layout.

@@ -838,6 +838,15 @@ __device__ __forceinline__ void slab_coords(const lgd_block_desc &bd, int64_t o,
   i = (int)(local - (local / sx) * sx) - 1;
 }
 
+// The first fill of the next finer level at α = 0, written by the stage launch of
+// its parent: the child's plan and current buffers.
+struct ChildFill {
+  const lgd_fill_op *ops;
+  const lgd_corner *corners;
+  int64_t nops;
+  double *lvl[2];
+};
+
 // One fused Model C stage of a level: workgroups [0, ntiles) advance the tiles'
 // cells; with kFill, workgroups from ntiles on write the next substep's ghosts.
 template <bool kNoise, bool kFill>
@@ -911,13 +920,38 @@ struct SegArgs {
   const double *salt_src;
 };
 
-template <bool kNoise, bool kFill, int kDepth>
+// kFill: 0 none, 1 the next substep's ghosts (N), 2 the child level's first fill (C).
+template <bool kNoise, int kFill, int kDepth>
 __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
-    SegArgs a, const lgd_block_desc *__restrict__ blocks, NextFill N) {
+    SegArgs a, const lgd_block_desc *__restrict__ blocks, NextFill N, ChildFill C) {
   using namespace lgd_stream;
   // The fill's workgroups come first, so that they run beside the streaming waves
   // rather than after them (their copy ghosts each recompute a whole cell).
-  if (kFill && (int)blockIdx.x < a.fill_wg) {
+  if (kFill == 2 && (int)blockIdx.x < a.fill_wg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= C.nops) return;
+    const lgd_fill_op op = C.ops[t];
+    if (op.clen == 0) {  // a copy within the child level
+      C.lvl[0][op.target] = C.lvl[0][op.src];
+      C.lvl[1][op.target] = C.lvl[1][op.src];
+      return;
+    }
+    // old·(1 − 0) + new·0 is the old value bit for bit whenever the new one is
+    // finite: the corner sum starts from +0, so it is never −0, and adding ±0 to it
+    // changes nothing.  The stage being computed is not read.
+    lgd_corner k[4];
+    load_corners(C.corners, t, k);
+    for (int f = 0; f < 2; ++f) {
+      const double *old = f == 0 ? a.phi : a.u;
+      double vo = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c < op.clen) vo = vo + old[k[c].src] * k[c].weight;
+      C.lvl[f][op.target] = vo;
+    }
+    return;
+  }
+  if (kFill == 1 && (int)blockIdx.x < a.fill_wg) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N.nops) return;
     const uint64_t st = kNoise ? lgd_salt(a.salt, a.salt_src) : 0;
@@ -939,7 +973,7 @@ __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
   }
   // XCD-aware order: XCD x (= blockIdx.x % 8, also wg % 8) takes one contiguous range
   // of items, so the strips of a segment, which share their halo rows, meet in one L2.
-  const int wg = (int)blockIdx.x - (kFill ? a.fill_wg : 0);
+  const int wg = (int)blockIdx.x - (kFill != 0 ? a.fill_wg : 0);
   const int xcd = wg % 8, idx = wg / 8;
   const int per = a.nwg_seg / 8, rem = a.nwg_seg % 8;
   const int lw = xcd * per + min(xcd, rem) + idx;
@@ -1161,6 +1195,11 @@ bool lgd_amr_level_streams(const lgd_ctx *ctx, const lgd_amr &g, int level) {
   return !(level == 0 && lgd_amr_level0_streams(ctx, g));
 }
 
+bool lgd_amr_child_fill_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level) {
+  return level + 1 < g.num_levels() && lgd_amr_level_streams(ctx, g, level) && !ctx->amr_per_block &&
+         !ctx->amr_split_fill;
+}
+
 namespace {
 
 constexpr int kNumTileShapes = (int)(sizeof(kAmrTileShapes) / sizeof(kAmrTileShapes[0]));
@@ -1281,8 +1320,11 @@ NextFill next_fill_args(const lgd_level_plan &P, const lgd_amr_next_fill *nf, in
 
 int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
                         double dt, const double *phi, const double *u, double *phi_n, double *u_n,
-                        uint64_t seed, uint64_t salt, bool wiener, const lgd_amr_next_fill *nf) {
+                        uint64_t seed, uint64_t salt, bool wiener, const lgd_amr_next_fill *nf,
+                        const lgd_amr_child_fill *cf) {
   const lgd_level_plan &P = g.plans[(size_t)level];
+  if (cf && (nf || !lgd_amr_child_fill_fuses(ctx, g, level)))
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "the child level's fill shares only a streamed stage launch");
   lgd_cart c = g.base;
   c.h[0] = g.spacing[(size_t)level][0];
   c.h[1] = g.spacing[(size_t)level][1];
@@ -1323,7 +1365,9 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
   if (lgd_amr_level_streams(ctx, g, level)) {
     StageSegs T;
     if ((rc = stage_segs(ctx, g, level, T))) return rc;
-    const bool fill = nf && P.nops;
+    const lgd_level_plan *FP = cf ? &g.plans[(size_t)level + 1] : &P;
+    const int mode = cf ? (FP->nops ? 2 : 0) : (nf && P.nops ? 1 : 0);
+    const bool fill = mode != 0;
     if (T.n == 0 && !fill) return LGD_OK;
     if (ctx->amr_per_block && nf)
       return fail(ctx, LGD_ERR_UNSUPPORTED, "per-patch launches fill ghosts in a launch of their own");
@@ -1338,10 +1382,19 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
     A.seed = seed;
     A.salt = salt;
     A.salt_src = noise ? ctx->salt_src : nullptr;
-    const NextFill N = fill ? next_fill_args(P, nf, 2) : NextFill{};
+    const NextFill N = mode == 1 ? next_fill_args(P, nf, 2) : NextFill{};
+    ChildFill C{};
+    if (mode == 2) {
+      C.ops = FP->ops;
+      C.corners = FP->corners;
+      C.nops = FP->nops;
+      C.lvl[0] = cf->lvl[0];
+      C.lvl[1] = cf->lvl[1];
+    }
     auto go = [&](const lgd_sseg *segs, int64_t n, bool with_fill) -> int {
       const int64_t nwg = (n + kSegWaves - 1) / kSegWaves;
-      const int64_t fwg = with_fill ? ((P.nops + 64 * kSegWaves - 1) / (64 * kSegWaves) + 7) / 8 * 8 : 0;
+      const int64_t fwg =
+          with_fill ? ((FP->nops + 64 * kSegWaves - 1) / (64 * kSegWaves) + 7) / 8 * 8 : 0;
       const int64_t total = nwg + fwg;
       if (total == 0) return LGD_OK;
       if (total > INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "AMR level too large for one launch");
@@ -1351,12 +1404,15 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
       A.fill_wg = (int32_t)fwg;
       const dim3 grid((unsigned)total), blk(64 * kSegWaves);
       hipStream_t st = ctx->stream;
+      const int m = with_fill ? mode : 0;
       if (noise) {
-        if (with_fill) amr_stream_kernel<true, true, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
-        else amr_stream_kernel<true, false, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
+        if (m == 1) amr_stream_kernel<true, 1, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
+        else if (m == 2) amr_stream_kernel<true, 2, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
+        else amr_stream_kernel<true, 0, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
       } else {
-        if (with_fill) amr_stream_kernel<false, true, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
-        else amr_stream_kernel<false, false, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N);
+        if (m == 1) amr_stream_kernel<false, 1, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
+        else if (m == 2) amr_stream_kernel<false, 2, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
+        else amr_stream_kernel<false, 0, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
       }
       return LGD_OK;
     };

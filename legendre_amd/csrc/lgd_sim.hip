@@ -388,11 +388,15 @@ int amr_restrict_level(lgd_sim *s, int fine_level) {
 // buffer, which becomes current.  `tslot`: the device time slot this substep's
 // noise reads while a graph is captured.
 // With next_alpha >= 0 the same launch also fills the ghosts of the level's next
-// substep (a level without children; the parent's buffers blended at next_alpha).
+// substep (a level without children; the parent's buffers blended at next_alpha);
+// with child_fill the ghosts of level + 1 for its first substep (blended at α = 0).
 int amr_level_step(lgd_sim *s, int level, double dt, uint64_t salt, int tslot,
-                   double next_alpha = -1.0) {
+                   double next_alpha = -1.0, bool child_fill = false) {
   const lgd_amr &g = *s->amr;
   int rc;
+  lgd_amr_child_fill cf{};
+  if (child_fill)
+    for (int f = 0; f < nfields(s); ++f) cf.lvl[f] = amr_cur(s, f, level + 1)->data;
   lgd_amr_next_fill nf{};
   if (next_alpha >= 0.0) {
     for (int f = 0; f < nfields(s); ++f) {
@@ -406,8 +410,11 @@ int amr_level_step(lgd_sim *s, int level, double dt, uint64_t salt, int tslot,
     s->ctx->salt_src = time_slot(s, tslot);
     rc = lgd_amr_modelc_step(s->ctx, g, s->params, level, dt, amr_cur(s, 0, level)->data,
                              amr_cur(s, 1, level)->data, amr_prev(s, 0, level)->data,
-                             amr_prev(s, 1, level)->data, s->seed, salt, s->wiener != 0, nfp);
+                             amr_prev(s, 1, level)->data, s->seed, salt, s->wiener != 0, nfp,
+                             child_fill ? &cf : nullptr);
     s->ctx->salt_src = nullptr;
+  } else if (child_fill) {
+    return fail(s->ctx, LGD_ERR_UNSUPPORTED, "the finer level's fill shares only a Model C stage");
   } else {
     rc = lgd_amr_heat_step(s->ctx, g, s->kappa, level, dt, amr_cur(s, 0, level)->data,
                            amr_prev(s, 0, level)->data, nfp);
@@ -438,19 +445,22 @@ int amr_global_step(lgd_sim *s, double dt, uint64_t salt) {
 // into the finer level at t + s·dt_child (fma), then restrict it back.
 // Between two substeps of a level that has a finer level, the restriction that
 // ends one substep (defer_out) and the fill that starts the next (fuse_in) share a
-// launch; the last substep restricts on its own.
+// launch; the last substep restricts on its own.  The child's first fill shares the
+// level's stage launch where that streams (filled_in: the caller's launch wrote
+// this level's ghosts).
 int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha, bool fuse_in = false,
-                bool defer_out = false) {
+                bool defer_out = false, bool filled_in = false) {
   const lgd_amr &g = *s->amr;
   int rc;
   if (fuse_in) {
     if ((rc = amr_restrict_fill_level(s, level, alpha))) return rc;
-  } else if ((rc = amr_fill_level(s, level, true, alpha))) {
+  } else if (!filled_in && (rc = amr_fill_level(s, level, true, alpha))) {
     return rc;
   }
   uint64_t salt;
   std::memcpy(&salt, &t, sizeof salt);
-  if ((rc = amr_level_step(s, level, dt, salt, level))) return rc;
+  const bool child_fill = s->kind == lgd_sim::kModelC && lgd_amr_child_fill_fuses(s->ctx, g, level);
+  if ((rc = amr_level_step(s, level, dt, salt, level, -1.0, child_fill))) return rc;
   const int child = level + 1;
   if (child < g.num_levels()) {
     double n = std::fmax(std::round(amr_stable_dt(s, level) / amr_stable_dt(s, child)), 1.0);
@@ -470,12 +480,13 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha, bool f
       }
       const double tk = std::fma((double)k, dt_child, t), ak = (double)k / (double)nn;
       if (!fused) {
-        if ((rc = amr_advance(s, child, tk, dt_child, ak, chain && k > 0, chain && k + 1 < nn)))
+        if ((rc = amr_advance(s, child, tk, dt_child, ak, chain && k > 0, chain && k + 1 < nn,
+                              child_fill && k == 0)))
           return rc;
         continue;
       }
       // Subcycle::advance of the finest level (no recursion below it).
-      if (k == 0 && (rc = amr_fill_level(s, child, true, ak))) return rc;
+      if (k == 0 && !child_fill && (rc = amr_fill_level(s, child, true, ak))) return rc;
       uint64_t salt_k;
       std::memcpy(&salt_k, &tk, sizeof salt_k);
       const double next = k + 1 < nn ? (double)(k + 1) / (double)nn : -1.0;
@@ -1052,13 +1063,27 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
   // Every timed launch writes only what the next coarse step overwrites before it
   // reads it: a level's other buffer (its stage output), the ghost entries of the
   // current one (fills), the parent's other buffer (restrictions).
-  auto stage = [&](int l) {
+  // A level's stage as a coarse step issues it: with the next substep's ghosts
+  // (next_fill, the finest level) or with the finer level's first fill (child_fill,
+  // which writes only ghosts of that level's current buffers).
+  auto stage = [&](int l, bool next_fill = false, bool child_fill = false) {
+    lgd_amr_next_fill nfa{};
+    lgd_amr_child_fill cfa{};
+    for (int f = 0; f < nf; ++f) {
+      if (next_fill) {
+        nfa.par[f] = amr_cur(sim, f, l - 1)->data;
+        nfa.old[f] = amr_prev(sim, f, l - 1)->data;
+      }
+      if (child_fill) cfa.lvl[f] = amr_cur(sim, f, l + 1)->data;
+    }
+    nfa.alpha = 0.5;
     if (modelc)
       return lgd_amr_modelc_step(ctx, g, sim->params, l, dt, amr_cur(sim, 0, l)->data,
                                  amr_cur(sim, 1, l)->data, amr_prev(sim, 0, l)->data,
-                                 amr_prev(sim, 1, l)->data, sim->seed, salt, sim->wiener != 0);
+                                 amr_prev(sim, 1, l)->data, sim->seed, salt, sim->wiener != 0,
+                                 next_fill ? &nfa : nullptr, child_fill ? &cfa : nullptr);
     return lgd_amr_heat_step(ctx, g, sim->kappa, l, dt, amr_cur(sim, 0, l)->data,
-                             amr_prev(sim, 0, l)->data);
+                             amr_prev(sim, 0, l)->data, next_fill ? &nfa : nullptr);
   };
   auto fill = [&](int l) { return amr_fill_level(sim, l, l > 0, 0.5); };
   auto restrict_prev = [&](int l) {
@@ -1093,8 +1118,28 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
     double tot = 0.0;
     for (int l = streams0 ? 1 : 0; l < L && !rc; ++l) {
       double ms = 0.0;
-      rc = timed([&] { return stage(l); }, &ms);
-      tot += N[(size_t)l] * ms;
+      const bool batched = !ctx->amr_per_block && !ctx->amr_split_fill;
+      if (l + 1 < L) {
+        // A level with a finer one: its stage, and the finer level's first fill in
+        // the same launch or in one of its own.
+        const bool cf = modelc && lgd_amr_child_fill_fuses(ctx, g, l);
+        rc = timed([&] { return stage(l, false, cf); }, &ms);
+        tot += N[(size_t)l] * ms;
+        if (!cf && !rc) {
+          rc = timed([&] { return fill(l + 1); }, &ms);
+          tot += N[(size_t)l] * ms;
+        }
+      } else if (l > 0 && batched && N[(size_t)l] > N[(size_t)l - 1]) {
+        // The finest level: all but the last substep of each parent substep also
+        // write the next substep's ghosts.
+        rc = timed([&] { return stage(l, true); }, &ms);
+        tot += (N[(size_t)l] - N[(size_t)l - 1]) * ms;
+        if (!rc) rc = timed([&] { return stage(l); }, &ms);
+        tot += N[(size_t)l - 1] * ms;
+      } else {
+        rc = timed([&] { return stage(l); }, &ms);
+        tot += N[(size_t)l] * ms;
+      }
     }
     ++candidates;
     if (c == 0) dflt_tiles = tot;
