@@ -895,15 +895,18 @@ def run_amr(args) -> dict:
         ctx.sync()
         return sim
 
+    # Untimed warmup on a copy of the run, advanced past its second regrid (3 levels:
+    # every kernel and pool size of the timed run used once); with --autotune the
+    # launch shapes are tuned on that hierarchy.  The timed run starts from the
+    # initial state as usual.
     tuning = None
+    warmup_steps = A.every + 1
+    probe = build()
+    probe.step(A.dt, warmup_steps)
     if args.autotune:
-        # Tune on a 3-level hierarchy of this run: a copy advanced past its second
-        # regrid (the timed run starts from the initial state as usual).
-        probe = build()
-        probe.step(A.dt, A.every + 1)
         tuning = probe.autotune_amr(A.dt, reps=5)
-        probe.close()
         log(f"[amr] autotuned launch shapes: {tuning}")
+    probe.close()
 
     def level_cells(sim):
         g = sim.amr
@@ -963,7 +966,7 @@ def run_amr(args) -> dict:
         "unit": "Gcell-steps/s",
         "n_gpus": 1,
         "steps": nchunks * A.every,
-        "warmup": 0,
+        "warmup": warmup_steps,
         "ms_per_step": round(total / (nchunks * A.every) * 1e3, 5),
         "higher_is_better": True,
         "scaling": "weak",

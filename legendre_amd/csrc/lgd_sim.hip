@@ -338,22 +338,26 @@ double amr_stable_dt(const lgd_sim *s, int level) {
 
 // Fill the ghosts of `level` in its current buffers: same-level copies and
 // prolongation from the parent (blended between the parent's previous and current
-// values when `blend`).  Level 0 on the streaming kernel needs none.
+// values when `blend`).  Level 0 on the streaming kernel needs none.  At α = 0 the
+// blend old·1 + new·0 is the corner sum of the old values bit for bit (that sum
+// starts from +0, so it is never −0) whenever the new ones are finite: the fill
+// reads the previous buffer alone (half the corner loads).
 int amr_fill_level(lgd_sim *s, int level, bool blend, double alpha) {
   const lgd_amr &g = *s->amr;
   if (level == 0 && s->kind == lgd_sim::kModelC && lgd_amr_level0_streams(s->ctx, g)) return LGD_OK;
   const int nf = nfields(s);
+  const bool at_old = blend && alpha == 0.0;
   double *lvl[2] = {nullptr, nullptr};
   const double *par[2] = {nullptr, nullptr}, *old[2] = {nullptr, nullptr};
   for (int f = 0; f < nf; ++f) {
     lvl[f] = amr_cur(s, f, level)->data;
     if (level > 0) {
-      par[f] = amr_cur(s, f, level - 1)->data;
+      par[f] = at_old ? amr_prev(s, f, level - 1)->data : amr_cur(s, f, level - 1)->data;
       old[f] = amr_prev(s, f, level - 1)->data;
     }
   }
   return lgd_amr_fill_pp(s->ctx, g, level, nf, lvl, level > 0 ? par : nullptr,
-                         level > 0 && blend ? old : nullptr, alpha);
+                         level > 0 && blend && !at_old ? old : nullptr, alpha);
 }
 
 // Restrict level + 1 into `level` and fill `level`'s ghosts (as amr_fill_level with

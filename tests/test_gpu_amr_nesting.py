@@ -291,10 +291,11 @@ def test_a_patch_steps_like_its_uniform_twin_bitwise(ctx, orc, model):
 
 @pytest.mark.parametrize("model", ["modelc", "heat"])
 def test_fused_next_fill_matches_split_fill_bitwise(ctx, orc, model):
-    """The finest level's stage launch filling the next substep's ghosts (the
-    default) computes the bits of a separate fill launch per substep: three-level
-    adaptive subcycled runs, Model C with noise and heat, hierarchy and fields after
-    every step."""
+    """The finest level's stage launch filling the next substep's ghosts, and a level's
+    stage launch filling its finer level's first substep's ghosts (the defaults),
+    compute the bits of a separate fill launch per substep: three-level adaptive
+    subcycled runs, Model C with noise and heat, hierarchy and fields after every
+    step."""
     say(f"amr fused next fill vs split fill ({model})")
     h, n, blk = 0.4, 96, 24
     base = orc.grid([n, n], [blk, blk], [0, 0], [h, h])
@@ -336,7 +337,8 @@ def test_fused_next_fill_matches_split_fill_bitwise(ctx, orc, model):
 def test_amr_launch_shapes_change_no_bit(ctx, orc):
     """Every level-stage form (streaming at several strip heights, per-cell at several
     tile shapes, per-patch launches), fill / restriction width, level-0 kernel (one
-    block or two per wavefront) and the autotuner's choice give the bits of the
+    block or two per wavefront, or per-cell with its own fill) and the autotuner's
+    choice give the bits of the
     default shapes: three-level adaptive subcycled Model C with noise, hierarchy and
     fields after every step; the autotuner leaves the simulation it times bitwise
     unchanged."""
@@ -374,7 +376,8 @@ def test_amr_launch_shapes_change_no_bit(ctx, orc):
               {"amr_kernel": 1, "amr_tile": 6, "modelc_kernel": 3},
               {"amr_rows": 2, "amr_wg": 128}, {"amr_rows": 5, "modelc_kernel": 2},
               {"amr_rows": 16, "modelc_kernel": 2, "modelc_strips": 1, "modelc_prefetch": 1},
-              {"amr_per_block": 1}, {"amr_per_block": 1, "amr_kernel": 1}]
+              {"amr_per_block": 1}, {"amr_per_block": 1, "amr_kernel": 1},
+              {"amr_kernel": 1, "modelc_kernel": 1}, {"modelc_kernel": 1}]
     try:
         ref = run({})
         assert any(p[0] == 2 for tr in ref for p in tr[0])
