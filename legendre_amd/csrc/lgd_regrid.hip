@@ -254,25 +254,24 @@ void host_flag_bits(const lgd_amr &g, const std::vector<double> &tag_field,
 
 }  // namespace
 
-int lgd_regrid_propose_bits(const lgd_amr &g, const std::vector<uint64_t *> &bits,
-                            const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
+int lgd_regrid_propose_words(const lgd_amr &g, const std::vector<lgd_flag_words> &flags,
+                             const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
   const int capacity = g.max_levels() - 1;
   out.assign((size_t)capacity, {});
+  std::vector<std::pair<int64_t, uint64_t>> nest;
+  std::vector<int64_t> midx;
+  std::vector<uint64_t> mbits;
   for (int target = capacity; target >= 1; --target) {
     const int src = target - 1;
     const lgd_box dom = g.domain(src);
     const int64_t W = dom.hi[0], H = dom.hi[1], wpr = (W + 63) / 64;
-    const size_t words = (size_t)(H * wpr);
-    std::vector<uint64_t> none;
-    uint64_t *m = (size_t)src < bits.size() ? bits[(size_t)src] : nullptr;
-    if (!m) {
-      none.assign(words, 0ull);
-      m = none.data();
-    }
+    lgd_flag_words fw = (size_t)src < flags.size() ? flags[(size_t)src] : lgd_flag_words{};
     // Nesting flags: the coarse cells under the level being built above, grown by
-    // one cell (adapt.rs:194-211).
+    // one cell (adapt.rs:194-211), as words in row-major order merged into the
+    // level's own (OR where both have a word).
     if (target < capacity && !out[(size_t)target].empty()) {
       const int64_t rf = g.ratios[(size_t)target], rs = g.ratios[(size_t)target - 1];
+      nest.clear();
       for (const lgd_box &bx : out[(size_t)target]) {
         int64_t nlo[2] = {bx.lo[0] / rf - 1, bx.lo[1] / rf - 1};
         int64_t nhi[2] = {bx.hi[0] / rf + 1, bx.hi[1] / rf + 1};
@@ -280,41 +279,61 @@ int lgd_regrid_propose_bits(const lgd_amr &g, const std::vector<uint64_t *> &bit
         const int64_t y1 = std::min<int64_t>(floor_div(nhi[1] + rs - 1, rs), H);
         const int64_t x0 = std::max<int64_t>(floor_div(nlo[0], rs), 0);
         const int64_t x1 = std::min<int64_t>(floor_div(nhi[0] + rs - 1, rs), W);
+        if (x0 >= x1) continue;
         for (int64_t y = y0; y < y1; ++y)
-          for (int64_t x = x0; x < x1; ++x) m[(size_t)(y * wpr + x / 64)] |= 1ull << (x % 64);
+          for (int64_t w = x0 / 64; w <= (x1 - 1) / 64; ++w) {
+            const int64_t a = std::max<int64_t>(x0 - w * 64, 0), b = std::min<int64_t>(x1 - w * 64, 64);
+            const uint64_t m = (b == 64 ? ~0ull : (1ull << b) - 1ull) & ~((1ull << a) - 1ull);
+            nest.emplace_back(y * wpr + w, m);
+          }
       }
+      std::sort(nest.begin(), nest.end(),
+                [](const std::pair<int64_t, uint64_t> &p, const std::pair<int64_t, uint64_t> &q) {
+                  return p.first < q.first;
+                });
+      midx.clear();
+      mbits.clear();
+      size_t i = 0, j = 0;
+      while (i < fw.n || j < nest.size()) {
+        int64_t k;
+        uint64_t m = 0;
+        if (j >= nest.size() || (i < fw.n && fw.idx[i] <= nest[j].first)) {
+          k = fw.idx[i];
+        } else {
+          k = nest[j].first;
+        }
+        while (i < fw.n && fw.idx[i] == k) m |= fw.bits[i++];
+        while (j < nest.size() && nest[j].first == k) m |= nest[j++].second;
+        midx.push_back(k);
+        mbits.push_back(m);
+      }
+      fw = lgd_flag_words{midx.data(), mbits.data(), midx.size()};
     }
     // The flagged cells, each once (the set the reference clusters after its
     // clip / sort / dedupe; clustering does not depend on their order), and their
     // bounding box.
     size_t nflags = 0;
-    for (size_t i = 0; i < words; ++i) nflags += (size_t)__builtin_popcountll(m[i]);
+    for (size_t i = 0; i < fw.n; ++i) nflags += (size_t)__builtin_popcountll(fw.bits[i]);
     if (nflags == 0) continue;
-    std::vector<Cell> flags(nflags);
+    std::vector<Cell> flags_c(nflags);
     lgd_box fb = empty_box();
     size_t k = 0;
-    for (int64_t y = 0; y < H; ++y) {
-      const uint64_t *row = m + (size_t)(y * wpr);
-      bool any = false;
-      for (int64_t w = 0; w < wpr; ++w) {
-        uint64_t word = row[w];
-        if (!word) continue;
-        any = true;
-        fb.lo[0] = std::min<int64_t>(fb.lo[0], w * 64 + __builtin_ctzll(word));
-        fb.hi[0] = std::max<int64_t>(fb.hi[0], w * 64 + 64 - __builtin_clzll(word));
-        while (word) {
-          flags[k++] = Cell{(int32_t)(w * 64 + __builtin_ctzll(word)), (int32_t)y};
-          word &= word - 1;
-        }
-      }
-      if (any) {
-        fb.lo[1] = std::min<int64_t>(fb.lo[1], y);
-        fb.hi[1] = y + 1;
+    for (size_t i = 0; i < fw.n; ++i) {
+      uint64_t word = fw.bits[i];
+      if (!word) continue;
+      const int64_t y = fw.idx[i] / wpr, w = fw.idx[i] - y * wpr;
+      fb.lo[0] = std::min<int64_t>(fb.lo[0], w * 64 + __builtin_ctzll(word));
+      fb.hi[0] = std::max<int64_t>(fb.hi[0], w * 64 + 64 - __builtin_clzll(word));
+      fb.lo[1] = std::min<int64_t>(fb.lo[1], y);
+      fb.hi[1] = std::max<int64_t>(fb.hi[1], y + 1);
+      while (word) {
+        flags_c[k++] = Cell{(int32_t)(w * 64 + __builtin_ctzll(word)), (int32_t)y};
+        word &= word - 1;
       }
     }
     std::vector<lgd_box> boxes;
     std::vector<uint32_t> sig;
-    cluster_rec(flags.data(), flags.size(), fb, pol.efficiency, (size_t)pol.min_width, boxes, 3, sig);
+    cluster_rec(flags_c.data(), flags_c.size(), fb, pol.efficiency, (size_t)pol.min_width, boxes, 3, sig);
     const int64_t r = g.ratios[(size_t)target - 1];
     for (lgd_box &b : boxes)
       for (int d = 0; d < 2; ++d) {
@@ -335,6 +354,25 @@ int lgd_regrid_propose_bits(const lgd_amr &g, const std::vector<uint64_t *> &bit
       if (!box_covered(g, (int)l + 1, b)) return 1;
   }
   return 0;
+}
+
+int lgd_regrid_propose_bits(const lgd_amr &g, const std::vector<uint64_t *> &bits,
+                            const lgd_regrid_policy &pol, std::vector<std::vector<lgd_box>> &out) {
+  std::vector<std::vector<int64_t>> idx(bits.size());
+  std::vector<std::vector<uint64_t>> words(bits.size());
+  std::vector<lgd_flag_words> flags(bits.size());
+  for (size_t l = 0; l < bits.size() && (int)l < g.max_levels() - 1; ++l) {
+    if (!bits[l]) continue;
+    const lgd_box dom = g.domain((int)l);
+    const int64_t n = dom.hi[1] * ((dom.hi[0] + 63) / 64);
+    for (int64_t k = 0; k < n; ++k)
+      if (bits[l][k]) {
+        idx[l].push_back(k);
+        words[l].push_back(bits[l][k]);
+      }
+    flags[l] = lgd_flag_words{idx[l].data(), words[l].data(), idx[l].size()};
+  }
+  return lgd_regrid_propose_words(g, flags, pol, out);
 }
 
 void lgd_regrid_bits_from_cells(const lgd_amr &g, int src, const std::vector<int64_t> &cells,
@@ -486,30 +524,118 @@ __global__ __launch_bounds__(kRTX *kRTY) void tag_kernel(const lgd_tile *tiles,
   }
 }
 
+// The bitmask words of every refinable level, one after another: level l's
+// H[l]·wpr[l] words start at word wofs[l] and at workgroup first[l] of the launches
+// below (kWordWG words per workgroup).
+constexpr int kWordWG = 256;
+constexpr int kMaxWordLevels = kMaxSampleDepth + 1;
+struct LevelWords {
+  int64_t first[kMaxWordLevels + 1];
+  int64_t wofs[kMaxWordLevels];
+  int64_t W[kMaxWordLevels], H[kMaxWordLevels], wpr[kMaxWordLevels];
+  int n;
+};
+
+// The level of workgroup b and the level-local word of its thread.
+__device__ __forceinline__ int word_level(const LevelWords &L, int64_t b, int64_t &t) {
+  int l = 0;
+  while (l + 1 < L.n && b >= L.first[l + 1]) ++l;
+  t = (b - L.first[l]) * kWordWG + threadIdx.x;
+  return l;
+}
+
+// The nonzero flags of this workgroup's words, summed over its wavefronts.
+__device__ __forceinline__ int block_count(bool nz, int *lds) {
+  const unsigned long long m = __ballot(nz);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  int c = 0;
+  for (int w = 0; w < kWordWG / 64; ++w) c += lds[w];
+  return c;
+}
+
 // Buffer dilation, one bitmask word (64 cells) per thread: the OR over rows
 // y−b … y+b of each row's word shifted by −b … b cells (bits carried in from the
-// neighbouring words), clipped to the domain (width W).
-__global__ __launch_bounds__(256) void dilate_kernel(const unsigned long long *T, int64_t W,
-                                                     int64_t H, int64_t wpr, int buffer,
-                                                     unsigned long long *bits) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= H * wpr) return;
-  const int64_t y = t / wpr, w = t - y * wpr;
+// neighbouring words), clipped to the domain (width W); each workgroup also counts
+// its nonzero words.
+__global__ __launch_bounds__(kWordWG) void dilate_kernel(LevelWords L, const unsigned long long *T0,
+                                                         int buffer, unsigned long long *bits0,
+                                                         int32_t *counts) {
+  __shared__ int lds[kWordWG / 64];
+  int64_t t;
+  const int l = word_level(L, blockIdx.x, t);
+  const int64_t W = L.W[l], H = L.H[l], wpr = L.wpr[l];
+  const unsigned long long *T = T0 + L.wofs[l];
   unsigned long long out = 0;
-  for (int64_t yy = y - buffer; yy <= y + buffer; ++yy) {
-    if (yy < 0 || yy >= H) continue;
-    const unsigned long long *row = T + yy * wpr;
-    const unsigned long long c = row[w], L = w > 0 ? row[w - 1] : 0ull,
-                             R = w + 1 < wpr ? row[w + 1] : 0ull;
-    out |= c;
-    for (int d = 1; d <= buffer; ++d) {
-      out |= (c << d) | (L >> (64 - d));  // cell x takes the flag of x − d
-      out |= (c >> d) | (R << (64 - d));  // and of x + d
+  const bool valid = t < H * wpr;
+  if (valid) {
+    const int64_t y = t / wpr, w = t - y * wpr;
+    for (int64_t yy = y - buffer; yy <= y + buffer; ++yy) {
+      if (yy < 0 || yy >= H) continue;
+      const unsigned long long *row = T + yy * wpr;
+      const unsigned long long c = row[w], Lw = w > 0 ? row[w - 1] : 0ull,
+                               Rw = w + 1 < wpr ? row[w + 1] : 0ull;
+      out |= c;
+      for (int d = 1; d <= buffer; ++d) {
+        out |= (c << d) | (Lw >> (64 - d));  // cell x takes the flag of x − d
+        out |= (c >> d) | (Rw << (64 - d));  // and of x + d
+      }
     }
+    const int64_t vcells = W - w * 64;  // cells of this word inside the domain
+    if (vcells < 64) out &= (1ull << vcells) - 1ull;
+    bits0[L.wofs[l] + t] = out;
   }
-  const int64_t valid = W - w * 64;  // cells of this word inside the domain
-  if (valid < 64) out &= (1ull << valid) - 1ull;
-  bits[t] = out;
+  const int c = block_count(out != 0ull, lds);
+  if (threadIdx.x == 0) counts[blockIdx.x] = c;
+}
+
+// Exclusive prefix sums of the workgroups' counts (one workgroup of kScanWG
+// threads), and where each level's nonzero words start (starts[n]: their total).
+constexpr int kScanWG = 1024;
+__global__ __launch_bounds__(kScanWG) void count_scan_kernel(const int32_t *counts, int64_t nblocks,
+                                                             LevelWords L, int64_t *offsets,
+                                                             int64_t *starts) {
+  __shared__ int64_t part[kScanWG];
+  int64_t carry = 0;
+  for (int64_t base = 0; base < nblocks; base += kScanWG) {
+    const int64_t i = base + threadIdx.x;
+    const int64_t v = i < nblocks ? counts[i] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < kScanWG; d <<= 1) {  // Hillis–Steele inclusive scan
+      const int64_t add = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+      __syncthreads();
+      part[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (i < nblocks) offsets[i] = carry + part[threadIdx.x] - v;
+    carry += part[kScanWG - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int l = 0; l < L.n; ++l) starts[l] = L.first[l] < nblocks ? offsets[L.first[l]] : carry;
+    starts[L.n] = carry;
+  }
+}
+
+// Each nonzero word with its level-local index, in increasing word order per level.
+__global__ __launch_bounds__(kWordWG) void compact_kernel(LevelWords L, const unsigned long long *bits0,
+                                                          const int64_t *offsets, int64_t *idx,
+                                                          unsigned long long *words) {
+  __shared__ int lds[kWordWG / 64];
+  int64_t t;
+  const int l = word_level(L, blockIdx.x, t);
+  const unsigned long long v = t < L.H[l] * L.wpr[l] ? bits0[L.wofs[l] + t] : 0ull;
+  const unsigned long long m = __ballot(v != 0ull);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = __popcll(m);
+  __syncthreads();
+  if (v == 0ull) return;
+  int64_t o = offsets[blockIdx.x];
+  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) o += lds[w];
+  const unsigned lane = threadIdx.x & 63;
+  o += __popcll(m & ((1ull << lane) - 1ull));
+  idx[o] = t;
+  words[o] = v;
 }
 
 DevIndex dev_index(const lgd_amr &g) {
@@ -522,63 +648,105 @@ using namespace lgd;
 
 bool lgd_regrid_device_supported(const lgd_amr &g) { return g.max_levels() <= kMaxSampleDepth + 1; }
 
-int lgd_regrid_flag_bits(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field,
-                         const lgd_regrid_policy &pol, lgd_regrid_scratch &sc,
-                         std::vector<uint64_t *> &bits) {
+int lgd_regrid_flag_words(lgd_ctx *ctx, const lgd_amr &g, const double *tag_field,
+                          const lgd_regrid_policy &pol, lgd_regrid_scratch &sc,
+                          std::vector<lgd_flag_words> &flags) {
   const int levels = std::min(g.num_levels(), g.max_levels() - 1);
-  bits.assign((size_t)g.num_levels(), nullptr);
+  flags.assign((size_t)g.num_levels(), lgd_flag_words{});
   if (levels <= 0) return LGD_OK;
+  if (levels > kMaxWordLevels) return fail(ctx, LGD_ERR_UNSUPPORTED, "device regrid tags at most 8 levels");
   if (pol.buffer > 63)
     return fail(ctx, LGD_ERR_UNSUPPORTED, "device regrid dilates by at most 63 cells");
+  LevelWords L{};
+  L.n = levels;
   size_t wtotal = 0;
-  std::vector<size_t> wofs;
+  int64_t nblocks = 0;
   for (int src = 0; src < levels; ++src) {
     const lgd_box d = g.domain(src);
-    wofs.push_back(wtotal);
-    wtotal += (size_t)(d.hi[1] * ((d.hi[0] + 63) / 64));
+    L.W[src] = d.hi[0];
+    L.H[src] = d.hi[1];
+    L.wpr[src] = (d.hi[0] + 63) / 64;
+    L.wofs[src] = (int64_t)wtotal;
+    L.first[src] = nblocks;
+    const int64_t nw = L.H[src] * L.wpr[src];
+    wtotal += (size_t)nw;
+    nblocks += (nw + kWordWG - 1) / kWordWG;
   }
-  // Tags (T) and dilated flags (bits) for every level, one region each.
+  L.first[levels] = nblocks;
+  if (nblocks > INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "regrid bitmask too large for one launch");
+  // Tags (T), dilated flags (bits), per-workgroup counts and offsets, the compacted
+  // words (idx, words) and where each level's start; a pinned host copy of the
+  // starts and the compacted words.
   hipError_t e = hipSuccess;
-  if (sc.wcap < wtotal) {
-    (void)hipFree(sc.T);
-    (void)hipFree(sc.bits);
-    if (sc.host) (void)hipHostFree(sc.host);
-    sc.T = nullptr;
-    sc.bits = nullptr;
-    sc.host = nullptr;
-    sc.wcap = 0;
-    e = hipMalloc((void **)&sc.T, wtotal * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc((void **)&sc.bits, wtotal * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipHostMalloc((void **)&sc.host, wtotal * sizeof(uint64_t), hipHostMallocDefault);
+  if (sc.wcap < wtotal || sc.bcap < (size_t)nblocks) {
+    lgd_regrid_scratch_free(sc);
+    const size_t wc = wtotal, bc = (size_t)nblocks;
+    e = hipMalloc((void **)&sc.T, wc * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&sc.bits, wc * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&sc.idx, wc * sizeof(int64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&sc.words, wc * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&sc.counts, bc * sizeof(int32_t));
+    if (e == hipSuccess) e = hipMalloc((void **)&sc.offsets, (bc + kMaxWordLevels + 1) * sizeof(int64_t));
+    if (e == hipSuccess)
+      e = hipHostMalloc((void **)&sc.host, (2 * wc + kMaxWordLevels + 1) * sizeof(uint64_t),
+                        hipHostMallocDefault);
     if (e != hipSuccess) {
-      sc.host = nullptr;
+      lgd_regrid_scratch_free(sc);
       return hip_fail(ctx, e, "regrid tag scratch");
     }
-    sc.wcap = wtotal;
+    sc.wcap = wc;
+    sc.bcap = bc;
   }
+  int64_t *starts = sc.offsets + sc.bcap;
   e = hipMemsetAsync(sc.T, 0, wtotal * sizeof(uint64_t), ctx->stream);
   if (e != hipSuccess) return hip_fail(ctx, e, "regrid tag clear");
   int rc;
-  const int buffer = (int)std::max<int64_t>(pol.buffer, 0);
   for (int src = 0; src < levels; ++src) {
-    const lgd_box d = g.domain(src);
-    const int64_t W = d.hi[0], H = d.hi[1], wpr = (W + 63) / 64;
-    unsigned long long *T = (unsigned long long *)(sc.T + wofs[(size_t)src]);
+    unsigned long long *T = (unsigned long long *)(sc.T + L.wofs[src]);
     const lgd_level_plan &P = g.plans[(size_t)src];
     if (P.ntiles > 0) {
       tag_kernel<<<(unsigned)P.ntiles, dim3(kRTX, kRTY), 0, ctx->stream>>>(
-          P.tiles, g.d_blocks, g.d_lo, tag_field, pol.threshold, T, wpr);
+          P.tiles, g.d_blocks, g.d_lo, tag_field, pol.threshold, T, L.wpr[src]);
       if ((rc = launch_check(ctx, "regrid tag"))) return rc;
     }
-    dilate_kernel<<<blocks_for(H * wpr, 256), 256, 0, ctx->stream>>>(
-        T, W, H, wpr, buffer, (unsigned long long *)(sc.bits + wofs[(size_t)src]));
-    if ((rc = launch_check(ctx, "regrid dilate"))) return rc;
   }
-  e = hipMemcpyAsync(sc.host, sc.bits, wtotal * sizeof(uint64_t), hipMemcpyDeviceToHost,
+  const int buffer = (int)std::max<int64_t>(pol.buffer, 0);
+  dilate_kernel<<<(unsigned)nblocks, kWordWG, 0, ctx->stream>>>(
+      L, (const unsigned long long *)sc.T, buffer, (unsigned long long *)sc.bits, sc.counts);
+  if ((rc = launch_check(ctx, "regrid dilate"))) return rc;
+  count_scan_kernel<<<1, kScanWG, 0, ctx->stream>>>(sc.counts, nblocks, L, sc.offsets, starts);
+  if ((rc = launch_check(ctx, "regrid count scan"))) return rc;
+  compact_kernel<<<(unsigned)nblocks, kWordWG, 0, ctx->stream>>>(
+      L, (const unsigned long long *)sc.bits, sc.offsets, sc.idx, (unsigned long long *)sc.words);
+  if ((rc = launch_check(ctx, "regrid compact"))) return rc;
+  // Download the starts and, in the same round trip, as many words as the last
+  // regrid had (with room to grow); the rest, if any, after the total is known.
+  int64_t *h_starts = (int64_t *)sc.host;
+  int64_t *h_idx = h_starts + kMaxWordLevels + 1;
+  uint64_t *h_words = (uint64_t *)(h_idx + sc.wcap);
+  const size_t guess = std::min(wtotal, std::max<size_t>(4096, 2 * sc.last_total));
+  e = hipMemcpyAsync(h_starts, starts, (size_t)(levels + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
                      ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_idx, sc.idx, guess * sizeof(int64_t), hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(h_words, sc.words, guess * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-  if (e != hipSuccess) return hip_fail(ctx, e, "regrid bitmask download");
-  for (int src = 0; src < levels; ++src) bits[(size_t)src] = sc.host + wofs[(size_t)src];
+  const size_t total = e == hipSuccess ? (size_t)h_starts[levels] : 0;
+  if (e == hipSuccess && total > guess) {
+    e = hipMemcpyAsync(h_idx + guess, sc.idx + guess, (total - guess) * sizeof(int64_t),
+                       hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(h_words + guess, sc.words + guess, (total - guess) * sizeof(uint64_t),
+                         hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  }
+  if (e != hipSuccess) return hip_fail(ctx, e, "regrid flag download");
+  sc.last_total = total;
+  for (int src = 0; src < levels; ++src) {
+    const int64_t a = h_starts[src], b = h_starts[src + 1];
+    flags[(size_t)src] = lgd_flag_words{h_idx + a, h_words + a, (size_t)(b - a)};
+  }
   return LGD_OK;
 }
 
@@ -605,6 +773,10 @@ int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *
 void lgd_regrid_scratch_free(lgd_regrid_scratch &sc) {
   (void)hipFree(sc.T);
   (void)hipFree(sc.bits);
+  (void)hipFree(sc.idx);
+  (void)hipFree(sc.words);
+  (void)hipFree(sc.counts);
+  (void)hipFree(sc.offsets);
   if (sc.host) (void)hipHostFree(sc.host);
   sc = lgd_regrid_scratch{};
 }

@@ -555,12 +555,12 @@ int amr_regrid(lgd_sim *s) {
     changed = lgd_regrid_propose_bits(g, ptr, s->policy, levels);
   } else if (dev) {
     if ((rc = gradient_prepare(s))) return rc;
-    std::vector<uint64_t *> bits;
-    if ((rc = lgd_regrid_flag_bits(s->ctx, g, s->state[(size_t)tf]->data, s->policy, s->rscratch,
-                                   bits)))
+    std::vector<lgd_flag_words> flags;
+    if ((rc = lgd_regrid_flag_words(s->ctx, g, s->state[(size_t)tf]->data, s->policy, s->rscratch,
+                                    flags)))
       return rc;
     s->regrid_seconds[0] += seconds_since(clock);
-    changed = lgd_regrid_propose_bits(g, bits, s->policy, levels);
+    changed = lgd_regrid_propose_words(g, flags, s->policy, levels);
   } else {
     if ((rc = gradient_prepare(s))) return rc;
     tag.resize((size_t)g.off.back());
