@@ -13,23 +13,11 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
-#include <thread>
 #include <vector>
 
 #include "lgd_regrid.h"
 
 namespace {
-
-// A flagged cell in level coordinates (level domains are far below 2^31 cells wide).
-using Cell = std::array<int32_t, 2>;
-
-// Grows b (empty: lo > hi) to hold cell c.
-inline void grow(lgd_box &b, const Cell &c) {
-  for (int d = 0; d < 2; ++d) {
-    if (c[d] < b.lo[d]) b.lo[d] = c[d];
-    if (c[d] + 1 > b.hi[d]) b.hi[d] = c[d] + 1;
-  }
-}
 
 inline lgd_box empty_box() { return lgd_box{{INT64_MAX, INT64_MAX}, {INT64_MIN, INT64_MIN}}; }
 
@@ -64,7 +52,10 @@ bool hole_split(const lgd_box &bx, const Sig &sig, Split &s) {
 }
 
 // Strongest sign change of the signature's second difference leaving
-// min_width cells on each side (stronger, then more central, wins).
+// min_width cells on each side (stronger, then more central, wins; the first
+// found, axis 0 before axis 1 and lower planes first, on a full tie).  Planes
+// closer than min_width to either end are never candidates, so only the second
+// differences around the others are formed.
 bool inflection_split(const lgd_box &bx, const Sig &sig, size_t minw, Split &s) {
   bool have = false;
   int64_t best_s = 0;
@@ -73,20 +64,26 @@ bool inflection_split(const lgd_box &bx, const Sig &sig, size_t minw, Split &s) 
     const size_t n = sig.n[d];
     const uint32_t *v = sig.v[d];
     if (n < 4) continue;
-    for (size_t i = 1; i + 2 < n; ++i) {
-      int64_t a = (int64_t)v[i + 1] - 2 * (int64_t)v[i] + (int64_t)v[i - 1];
-      int64_t b = (int64_t)v[i + 2] - 2 * (int64_t)v[i + 1] + (int64_t)v[i];
-      if (!(a * b < 0)) continue;
-      size_t plane = i + 1;
-      if (plane < minw || n - plane < minw) continue;
-      int64_t strength = b - a < 0 ? a - b : b - a;
-      uint64_t c = std::min<uint64_t>(plane, n - plane);
-      if (!have || strength > best_s || (strength == best_s && c > best_c)) {
-        have = true;
-        best_s = strength;
-        best_c = c;
-        s = Split{d, bx.lo[d] + (int64_t)plane};
+    // plane = i + 1 with 1 <= i <= n - 3, minw <= plane <= n - minw.
+    const size_t i0 = std::max<size_t>(1, minw > 0 ? minw - 1 : 0);
+    if (n < minw + 1) continue;
+    const size_t i1 = std::min<size_t>(n - 3, n - minw - 1);  // inclusive
+    if (i0 > i1) continue;
+    int64_t a = (int64_t)v[i0 + 1] - 2 * (int64_t)v[i0] + (int64_t)v[i0 - 1];
+    for (size_t i = i0; i <= i1; ++i) {
+      const int64_t b = (int64_t)v[i + 2] - 2 * (int64_t)v[i + 1] + (int64_t)v[i];
+      if ((a < 0 && b > 0) || (a > 0 && b < 0)) {
+        const size_t plane = i + 1;
+        const int64_t strength = b - a < 0 ? a - b : b - a;
+        const uint64_t c = std::min<uint64_t>(plane, n - plane);
+        if (!have || strength > best_s || (strength == best_s && c > best_c)) {
+          have = true;
+          best_s = strength;
+          best_c = c;
+          s = Split{d, bx.lo[d] + (int64_t)plane};
+        }
       }
+      a = b;
     }
   }
   return have;
@@ -100,70 +97,157 @@ bool bisect_split(const lgd_box &bx, size_t minw, Split &s) {
   return true;
 }
 
-// Flag sets from this size up split their two halves over two host threads while
-// `depth` allows (the halves are disjoint ranges of f; their boxes join left first,
-// the order of the one-thread recursion).
-constexpr size_t kParallelFlags = 4096;
+// A run of flagged cells: the set bits of `bits` are cells (64·w + bit, y) of the
+// level.  A node's runs are in increasing (y, w) order, with no bits outside its box.
+struct Run {
+  int32_t y, w;
+  uint64_t bits;
+};
 
-// Berger–Rigoutsos over the n flagged cells f (in any order) whose bounding box is
-// bx (cluster.rs:157-281): accept the box at the efficiency, else split at a
-// signature hole, an inflection or the midpoint and recurse (left half first).  Each
-// half's bounding box is gathered by the partition pass; `sig` is scratch.
-void cluster_rec(Cell *f, size_t n, const lgd_box &bx, double eff, size_t minw,
-                 std::vector<lgd_box> &out, int depth, std::vector<uint32_t> &sig) {
+// The bits of each byte value as eight 0/1 counts (column signatures add a byte of
+// a run at a time).
+struct ByteBits {
+  uint32_t v[256][8];
+  ByteBits() {
+    for (int x = 0; x < 256; ++x)
+      for (int j = 0; j < 8; ++j) v[x][j] = (uint32_t)((x >> j) & 1);
+  }
+};
+const ByteBits kByteBits;
+
+// Column (axis 0) or row (axis 1) counts of the runs' bits into v, indexed from lo.
+void count_runs(const Run *r, size_t m, int axis, int64_t lo, size_t len, uint32_t *v) {
+  if (axis == 1) {
+    for (size_t i = 0; i < m; ++i) v[(size_t)(r[i].y - lo)] += (uint32_t)__builtin_popcountll(r[i].bits);
+    return;
+  }
+  for (size_t i = 0; i < m; ++i) {
+    // A byte of the run at a time (the run's bits lie inside [lo, lo + len)).
+    uint64_t b = r[i].bits;
+    const int64_t base = (int64_t)r[i].w * 64 - lo;
+    while (b) {
+      const int k = __builtin_ctzll(b) >> 3;
+      const uint32_t *lut = kByteBits.v[(b >> (8 * k)) & 0xffu];
+      const int64_t c0 = base + 8 * k;
+      if (c0 >= 0 && c0 + 8 <= (int64_t)len) {
+        for (int j = 0; j < 8; ++j) v[c0 + j] += lut[j];
+      } else {
+        for (int j = 0; j < 8; ++j)
+          if (lut[j]) v[c0 + j] += 1;
+      }
+      b &= ~(0xffull << (8 * k));
+    }
+  }
+}
+
+// Berger–Rigoutsos over the n flagged cells of the m runs r whose bounding box is bx
+// (cluster.rs:157-281): accept the box at the efficiency, else split at a signature
+// hole, an inflection or the midpoint and recurse (left half first).  sig[0] and
+// sig[1] are the box's column and row counts.  A split across rows divides the run
+// list in two, a split across columns masks each run at the plane.  The halves'
+// counts along the split axis are slices of the box's; across it, the half with
+// fewer runs counts its own and the other's are the box's less those.  Each half's
+// bounding box comes from the counts along the split axis and from its runs across
+// it.  One host thread: per regrid the clustering is a few hundred microseconds, less
+// than starting threads for it costs.
+void cluster_rec(const Run *r, size_t m, size_t n, const lgd_box &bx, std::vector<uint32_t> sig[2],
+                 double eff, size_t minw, std::vector<lgd_box> &out) {
   if ((double)n / (double)bx.cells() >= eff) {
     out.push_back(bx);
     return;
   }
   const size_t nx = (size_t)bx.extent(0), ny = (size_t)bx.extent(1);
-  sig.assign(nx + ny, 0u);
-  uint32_t *sx = sig.data(), *sy = sig.data() + nx;
-  const int64_t lx = bx.lo[0], ly = bx.lo[1];
-  for (size_t i = 0; i < n; ++i) {
-    sx[(size_t)(f[i][0] - lx)]++;
-    sy[(size_t)(f[i][1] - ly)]++;
-  }
-  const Sig sg{{sx, sy}, {nx, ny}};
+  const Sig sg{{sig[0].data(), sig[1].data()}, {nx, ny}};
   Split s{0, 0};
   if (!hole_split(bx, sg, s) && !inflection_split(bx, sg, minw, s) && !bisect_split(bx, minw, s)) {
     out.push_back(bx);
     return;
   }
-  // Two-pointer partition (cluster.rs:270-281): cells below the plane move to the front.
+  // The halves: flag counts and extents along the split axis from its counts.
+  const int ax = s.axis, ox = 1 - ax;
+  const uint32_t *sa = sig[ax].data();
+  const size_t na = ax == 0 ? nx : ny, cut = (size_t)(s.plane - bx.lo[ax]);
+  size_t nl = 0;
+  for (size_t i = 0; i < cut; ++i) nl += sa[i];
   lgd_box lb = empty_box(), rb = empty_box();
-  size_t lt = 0;
-  for (size_t j = 0; j < n; ++j) {
-    const Cell c = f[j];
-    if (c[s.axis] < s.plane) {
-      grow(lb, c);
-      f[j] = f[lt];
-      f[lt++] = c;
-    } else {
-      grow(rb, c);
+  for (size_t i = 0; i < cut; ++i)
+    if (sa[i]) {
+      lb.lo[ax] = std::min<int64_t>(lb.lo[ax], bx.lo[ax] + (int64_t)i);
+      lb.hi[ax] = bx.lo[ax] + (int64_t)i + 1;
+    }
+  for (size_t i = cut; i < na; ++i)
+    if (sa[i]) {
+      rb.lo[ax] = std::min<int64_t>(rb.lo[ax], bx.lo[ax] + (int64_t)i);
+      rb.hi[ax] = bx.lo[ax] + (int64_t)i + 1;
+    }
+  // Across the split axis, from the runs of each half.
+  auto extent_of = [&](const Run *q, size_t k, lgd_box &b) {
+    if (ox == 1) {
+      if (k) {
+        b.lo[1] = q[0].y;
+        b.hi[1] = (int64_t)q[k - 1].y + 1;
+      }
+      return;
+    }
+    for (size_t i = 0; i < k; ++i) {
+      const int64_t x0 = (int64_t)q[i].w * 64;
+      b.lo[0] = std::min<int64_t>(b.lo[0], x0 + __builtin_ctzll(q[i].bits));
+      b.hi[0] = std::max<int64_t>(b.hi[0], x0 + 64 - __builtin_clzll(q[i].bits));
+    }
+  };
+  const Run *left, *right;
+  size_t ml, mr;
+  std::vector<Run> lv, rv;
+  if (ax == 1) {
+    // Rows below the plane: a prefix of the runs.
+    size_t k = 0;
+    while (k < m && r[k].y < s.plane) ++k;
+    left = r;
+    ml = k;
+    right = r + k;
+    mr = m - k;
+  } else {
+    lv.reserve(m);
+    rv.reserve(m);
+    for (size_t i = 0; i < m; ++i) {
+      const int64_t c = s.plane - (int64_t)r[i].w * 64;  // bits below c go left
+      const uint64_t lm = c >= 64 ? ~0ull : c <= 0 ? 0ull : (1ull << c) - 1ull;
+      const uint64_t lbits = r[i].bits & lm, rbits = r[i].bits & ~lm;
+      if (lbits) lv.push_back(Run{r[i].y, r[i].w, lbits});
+      if (rbits) rv.push_back(Run{r[i].y, r[i].w, rbits});
+    }
+    left = lv.data();
+    ml = lv.size();
+    right = rv.data();
+    mr = rv.size();
+  }
+  extent_of(left, ml, lb);
+  extent_of(right, mr, rb);
+  const size_t nr = n - nl;
+  // The halves' counts, each over its own box.
+  const size_t no = ox == 0 ? nx : ny;
+  std::vector<uint32_t> cross((size_t)no, 0u);
+  const bool left_counts = ml <= mr;
+  count_runs(left_counts ? left : right, left_counts ? ml : mr, ox, bx.lo[ox], no, cross.data());
+  std::vector<uint32_t> ls[2], rs[2];
+  auto slice = [](const uint32_t *v, int64_t from, int64_t to, std::vector<uint32_t> &o) {
+    o.assign(v + from, v + to);
+  };
+  slice(sa, lb.lo[ax] - bx.lo[ax], lb.hi[ax] - bx.lo[ax], ls[ax]);
+  slice(sa, rb.lo[ax] - bx.lo[ax], rb.hi[ax] - bx.lo[ax], rs[ax]);
+  {
+    const uint32_t *so = sig[ox].data();
+    std::vector<uint32_t> &mine = left_counts ? ls[ox] : rs[ox], &other = left_counts ? rs[ox] : ls[ox];
+    const lgd_box &mb = left_counts ? lb : rb, &obx = left_counts ? rb : lb;
+    slice(cross.data(), mb.lo[ox] - bx.lo[ox], mb.hi[ox] - bx.lo[ox], mine);
+    other.resize((size_t)(obx.hi[ox] - obx.lo[ox]));
+    for (size_t i = 0; i < other.size(); ++i) {
+      const size_t j = (size_t)(obx.lo[ox] - bx.lo[ox]) + i;
+      other[i] = so[j] - cross[j];
     }
   }
-  if (depth > 0 && n >= kParallelFlags) {
-    std::vector<lgd_box> left;
-    std::thread t;
-    bool spawned = false;
-    try {
-      t = std::thread([&] {
-        std::vector<uint32_t> tsig;
-        cluster_rec(f, lt, lb, eff, minw, left, depth - 1, tsig);
-      });
-      spawned = true;
-    } catch (...) {
-      cluster_rec(f, lt, lb, eff, minw, left, depth - 1, sig);
-    }
-    std::vector<lgd_box> right;
-    cluster_rec(f + lt, n - lt, rb, eff, minw, right, depth - 1, sig);
-    if (spawned) t.join();
-    out.insert(out.end(), left.begin(), left.end());
-    out.insert(out.end(), right.begin(), right.end());
-    return;
-  }
-  cluster_rec(f, lt, lb, eff, minw, out, 0, sig);
-  cluster_rec(f + lt, n - lt, rb, eff, minw, out, 0, sig);
+  cluster_rec(left, ml, nl, lb, ls, eff, minw, out);
+  cluster_rec(right, mr, nr, rb, rs, eff, minw, out);
 }
 
 // Whether every cell of `need` lies in a patch of level l of g (run-skipping walk
@@ -310,30 +394,31 @@ int lgd_regrid_propose_words(const lgd_amr &g, const std::vector<lgd_flag_words>
       fw = lgd_flag_words{midx.data(), mbits.data(), midx.size()};
     }
     // The flagged cells, each once (the set the reference clusters after its
-    // clip / sort / dedupe; clustering does not depend on their order), and their
-    // bounding box.
+    // clip / sort / dedupe; clustering does not depend on their order), as runs, and
+    // their bounding box.
     size_t nflags = 0;
-    for (size_t i = 0; i < fw.n; ++i) nflags += (size_t)__builtin_popcountll(fw.bits[i]);
-    if (nflags == 0) continue;
-    std::vector<Cell> flags_c(nflags);
+    std::vector<Run> runs;
+    runs.reserve(fw.n);
     lgd_box fb = empty_box();
-    size_t k = 0;
     for (size_t i = 0; i < fw.n; ++i) {
-      uint64_t word = fw.bits[i];
+      const uint64_t word = fw.bits[i];
       if (!word) continue;
       const int64_t y = fw.idx[i] / wpr, w = fw.idx[i] - y * wpr;
+      nflags += (size_t)__builtin_popcountll(word);
       fb.lo[0] = std::min<int64_t>(fb.lo[0], w * 64 + __builtin_ctzll(word));
       fb.hi[0] = std::max<int64_t>(fb.hi[0], w * 64 + 64 - __builtin_clzll(word));
       fb.lo[1] = std::min<int64_t>(fb.lo[1], y);
       fb.hi[1] = std::max<int64_t>(fb.hi[1], y + 1);
-      while (word) {
-        flags_c[k++] = Cell{(int32_t)(w * 64 + __builtin_ctzll(word)), (int32_t)y};
-        word &= word - 1;
-      }
+      runs.push_back(Run{(int32_t)y, (int32_t)w, word});
     }
+    if (nflags == 0) continue;
     std::vector<lgd_box> boxes;
-    std::vector<uint32_t> sig;
-    cluster_rec(flags_c.data(), flags_c.size(), fb, pol.efficiency, (size_t)pol.min_width, boxes, 3, sig);
+    std::vector<uint32_t> sig[2];
+    for (int d = 0; d < 2; ++d) {
+      sig[d].assign((size_t)fb.extent(d), 0u);
+      count_runs(runs.data(), runs.size(), d, fb.lo[d], sig[d].size(), sig[d].data());
+    }
+    cluster_rec(runs.data(), runs.size(), nflags, fb, sig, pol.efficiency, (size_t)pol.min_width, boxes);
     const int64_t r = g.ratios[(size_t)target - 1];
     for (lgd_box &b : boxes)
       for (int d = 0; d < 2; ++d) {
