@@ -174,6 +174,7 @@ struct lgd_amr {
   std::vector<bucket_index> lookup;
   std::vector<int64_t> off;           // ghost-1 slab offsets, nblocks + 1
   void *d_arena = nullptr;  // one device allocation: the patch index and the plans of levels >= 1
+  size_t d_arena_cap = 0;   // its bytes (it returns to the context's retired arenas)
   lgd_block_desc *d_blocks = nullptr;
   std::vector<lgd_level_plan> plans;  // per populated level
   // find_patch on the device: per-level headers, concatenated bucket starts (into

@@ -92,7 +92,13 @@ struct Packer {
   void reserve(size_t n, T **dptr) {
     space.push_back({nullptr, n * sizeof(T), 0, (void **)dptr});
   }
-  int upload(lgd_ctx *ctx, void **base) {
+  // With `cap`, the device block comes from the context's retired arenas (its size
+  // in *cap), else from the stream-ordered pool.
+  hipError_t alloc(lgd_ctx *ctx, size_t bytes, void **base, size_t *cap) {
+    if (cap) return lgd_arena_take(ctx, bytes, base, cap);
+    return hipMallocAsync(base, bytes, ctx->stream);
+  }
+  int upload(lgd_ctx *ctx, void **base, size_t *cap = nullptr) {
     *base = nullptr;
     for (auto &p : parts) *p.dptr = nullptr;
     for (auto &p : space) *p.dptr = nullptr;
@@ -103,7 +109,7 @@ struct Packer {
     }
     if (end == 0) return LGD_OK;
     if (total == 0) {
-      hipError_t e = hipMallocAsync(base, end, ctx->stream);
+      hipError_t e = alloc(ctx, end, base, cap);
       if (e != hipSuccess) {
         *base = nullptr;
         return hip_fail(ctx, e, "hipMallocAsync(amr plan)");
@@ -130,7 +136,7 @@ struct Packer {
     }
     for (const auto &p : parts)
       if (p.bytes) std::memcpy(ctx->pinned + p.off, p.src, p.bytes);
-    e = hipMallocAsync(base, end, ctx->stream);
+    e = alloc(ctx, end, base, cap);
     if (e != hipSuccess) {
       *base = nullptr;
       return hip_fail(ctx, e, "hipMallocAsync(amr plan)");
@@ -213,8 +219,12 @@ int64_t lgd_amr::index(int64_t b, int64_t i, int64_t j) const {
 void lgd_amr::free_device() {
   // Returned to the pool in stream order, after the kernels that read them.
   hipStream_t st = ctx ? ctx->stream : nullptr;
-  if (d_arena) (void)hipFreeAsync(d_arena, st);
+  if (d_arena) {
+    if (ctx) lgd_arena_give(ctx, d_arena, d_arena_cap);
+    else (void)hipFreeAsync(d_arena, st);
+  }
   d_arena = nullptr;
+  d_arena_cap = 0;
   d_blocks = nullptr;
   d_levels = nullptr;
   d_bstart = nullptr;
@@ -593,7 +603,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
     if (l + 1 < g.num_levels()) to.reserve((size_t)P.nops, &P.cchild);
   }
   if (!take0 && (rc = pk0.upload(ctx, &g.plans[0].block))) return rc;
-  if ((rc = pk.upload(ctx, &g.d_arena))) return rc;
+  if ((rc = pk.upload(ctx, &g.d_arena, &g.d_arena_cap))) return rc;
   const lgd_dev_index ix{g.d_levels, g.d_bstart, g.d_bids, g.d_lo, g.d_blocks, g.num_levels()};
   for (int l = 0; l < g.num_levels(); ++l) {
     if (l == 0 && take0) continue;

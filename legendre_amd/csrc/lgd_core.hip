@@ -110,6 +110,7 @@ int lgd_ctx_create(int32_t device, void *stream, lgd_ctx **out) {
 int lgd_ctx_destroy(lgd_ctx *ctx) {
   if (!ctx) return LGD_OK;
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto &a : ctx->arenas) (void)hipFree(a.first);
   if (ctx->pinned_done) (void)hipEventDestroy(ctx->pinned_done);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->owns_stream && ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -344,3 +345,39 @@ int64_t lgd_grid_cell_key(const lgd_grid *grid, uint32_t ghost, int64_t offset) 
 }
 
 }  // extern "C"
+
+hipError_t lgd_arena_take(lgd_ctx *ctx, size_t bytes, void **out, size_t *cap) {
+  size_t best = ctx->arenas.size();
+  for (size_t i = 0; i < ctx->arenas.size(); ++i)
+    if (ctx->arenas[i].second >= bytes &&
+        (best == ctx->arenas.size() || ctx->arenas[i].second < ctx->arenas[best].second))
+      best = i;
+  if (best < ctx->arenas.size()) {
+    *out = ctx->arenas[best].first;
+    *cap = ctx->arenas[best].second;
+    ctx->arenas.erase(ctx->arenas.begin() + (std::ptrdiff_t)best);
+    return hipSuccess;
+  }
+  const size_t want = bytes + bytes / 4;
+  hipError_t e = hipMallocAsync(out, want, ctx->stream);
+  if (e != hipSuccess) {
+    *out = nullptr;
+    *cap = 0;
+    return e;
+  }
+  *cap = want;
+  return hipSuccess;
+}
+
+void lgd_arena_give(lgd_ctx *ctx, void *p, size_t cap) {
+  if (!p) return;
+  ctx->arenas.emplace_back(p, cap);
+  // Keep the three largest.
+  if (ctx->arenas.size() > 3) {
+    size_t small = 0;
+    for (size_t i = 1; i < ctx->arenas.size(); ++i)
+      if (ctx->arenas[i].second < ctx->arenas[small].second) small = i;
+    (void)hipFreeAsync(ctx->arenas[small].first, ctx->stream);
+    ctx->arenas.erase(ctx->arenas.begin() + (std::ptrdiff_t)small);
+  }
+}

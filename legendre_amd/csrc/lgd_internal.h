@@ -49,9 +49,18 @@ struct lgd_ctx {
   char *pinned = nullptr;
   size_t pinned_cap = 0;
   hipEvent_t pinned_done = nullptr;
+  // Device arenas of retired AMR hierarchies (pointer, bytes), reused by the next
+  // hierarchy's plans in stream order instead of allocating again.
+  std::vector<std::pair<void *, size_t>> arenas;
   // Shapes chosen by lgd_autotune_modelc_step, keyed by (nx, ny, blocks, noise).
   std::map<std::tuple<int64_t, int64_t, int64_t, int>, lgd_launch_shape> tuned;
 };
+
+// A device arena of at least `bytes` for work on the context's stream: a retired
+// one (stream-ordered after its last reader) or a new one with a quarter to spare;
+// *cap receives its size.  Returned with lgd_arena_give.
+hipError_t lgd_arena_take(lgd_ctx *ctx, size_t bytes, void **out, size_t *cap);
+void lgd_arena_give(lgd_ctx *ctx, void *p, size_t cap);
 
 // The launch shape for a Model C step over `nblocks` blocks of nx × ny: explicit
 // knobs first (over the fields of `dflt` they leave unset), then the autotuned

@@ -58,5 +58,5 @@ int lgd_regrid_flag_words(lgd_ctx *ctx, const lgd_amr &g, const double *tag_fiel
                           std::vector<lgd_flag_words> &flags);
 // Device migration: the interiors of new_g's levels >= 1 in new_f (a buffer of the
 // new layout) sampled from old_f on old_g; level 0 and ghosts are not written.
-int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *old_f,
-                              const lgd_amr &new_g, double *new_f);
+int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *const *old_f,
+                              const lgd_amr &new_g, double *const *new_f, int nf);

@@ -572,19 +572,47 @@ __device__ __attribute__((noinline)) double dev_sample(const DevIndex &ix, const
   }
 }
 
-// New patch interiors of one level sampled from the old hierarchy.
+// The fields a migration moves: old hierarchy's values in, new patch interiors out.
+struct MigrateFields {
+  const double *old[2];
+  double *nw[2];
+  int nf;
+};
+
+// New patch interiors of one level sampled from the old hierarchy, one 64×4 tile of
+// cells per workgroup.  A tile inside one patch of the same level in the old
+// hierarchy (where dev_sample's first lookup ends for each of its cells) copies from
+// that patch after one lookup; any other tile samples cell by cell.
 __global__ __launch_bounds__(kRTX *kRTY) void migrate_kernel(const lgd_tile *tiles,
                                                              const lgd_block_desc *nblocks,
                                                              const int64_t *nlo, DevIndex old,
-                                                             DevRatios R, const double *oldf,
-                                                             double *newf, int level) {
+                                                             DevRatios R, MigrateFields F, int level) {
+  __shared__ int64_t src_patch;
   const lgd_tile tl = tiles[blockIdx.x];
   const lgd_block_desc bd = nblocks[tl.block];
+  const int64_t X0 = nlo[2 * tl.block] + tl.x0, Y0 = nlo[2 * tl.block + 1] + tl.y0;
+  if (threadIdx.x == 0 && threadIdx.y == 0) {
+    const int64_t tw = min(kRTX, bd.nx - tl.x0), th = min(kRTY, bd.ny - tl.y0);
+    int64_t p = lgd_dev_find_patch(old, level, X0, Y0);
+    if (p >= 0) {
+      const lgd_block_desc ob = old.blocks[p];
+      if (X0 + tw > old.lo[2 * p] + ob.nx || Y0 + th > old.lo[2 * p + 1] + ob.ny) p = -1;
+    }
+    src_patch = p;
+  }
+  __syncthreads();
   const int i = tl.x0 + threadIdx.x, j = tl.y0 + threadIdx.y;
   if (i >= bd.nx || j >= bd.ny) return;
-  const int64_t x = nlo[2 * tl.block] + i, y = nlo[2 * tl.block + 1] + j;
-  newf[bd.off + (int64_t)(j + 1) * (bd.nx + 2) + (i + 1)] =
-      dev_sample<kMaxSampleDepth>(old, R, oldf, level, x, y);
+  const int64_t x = X0 + threadIdx.x, y = Y0 + threadIdx.y;
+  const int64_t dst = bd.off + (int64_t)(j + 1) * (bd.nx + 2) + (i + 1);
+  const int64_t p = src_patch;
+  if (p >= 0) {
+    const lgd_block_desc ob = old.blocks[p];
+    const int64_t o = ob.off + (y - old.lo[2 * p + 1] + 1) * (int64_t)(ob.nx + 2) + (x - old.lo[2 * p] + 1);
+    for (int f = 0; f < F.nf; ++f) F.nw[f][dst] = F.old[f][o];
+    return;
+  }
+  for (int f = 0; f < F.nf; ++f) F.nw[f][dst] = dev_sample<kMaxSampleDepth>(old, R, F.old[f], level, x, y);
 }
 
 // GradientTagger on one level's patch interiors: the flagged cells' bits in a
@@ -835,8 +863,8 @@ int lgd_regrid_flag_words(lgd_ctx *ctx, const lgd_amr &g, const double *tag_fiel
   return LGD_OK;
 }
 
-int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *old_f,
-                              const lgd_amr &new_g, double *new_f) {
+int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *const *old_f,
+                              const lgd_amr &new_g, double *const *new_f, int nf) {
   if (!lgd_regrid_device_supported(new_g))
     return fail(ctx, LGD_ERR_UNSUPPORTED, "device migration supports at most 8 levels");
   DevRatios R{};
@@ -845,12 +873,21 @@ int lgd_regrid_migrate_device(lgd_ctx *ctx, const lgd_amr &old_g, const double *
   for (size_t i = 0; i < old_g.ratios.size() && i <= (size_t)kMaxSampleDepth; ++i) R.r[i] = old_g.ratios[i];
   const DevIndex old = dev_index(old_g);
   int rc;
-  for (int l = 1; l < new_g.num_levels(); ++l) {
-    const lgd_level_plan &P = new_g.plans[(size_t)l];
-    if (P.ntiles == 0) continue;
-    migrate_kernel<<<(unsigned)P.ntiles, dim3(kRTX, kRTY), 0, ctx->stream>>>(
-        P.tiles, new_g.d_blocks, new_g.d_lo, old, R, old_f, new_f, l);
-    if ((rc = launch_check(ctx, "regrid migrate"))) return rc;
+  // Two fields per launch.
+  for (int f0 = 0; f0 < nf; f0 += 2) {
+    MigrateFields F{};
+    F.nf = std::min(2, nf - f0);
+    for (int f = 0; f < F.nf; ++f) {
+      F.old[f] = old_f[f0 + f];
+      F.nw[f] = new_f[f0 + f];
+    }
+    for (int l = 1; l < new_g.num_levels(); ++l) {
+      const lgd_level_plan &P = new_g.plans[(size_t)l];
+      if (P.ntiles == 0) continue;
+      migrate_kernel<<<(unsigned)P.ntiles, dim3(kRTX, kRTY), 0, ctx->stream>>>(
+          P.tiles, new_g.d_blocks, new_g.d_lo, old, R, F, l);
+      if ((rc = launch_check(ctx, "regrid migrate"))) return rc;
+    }
   }
   return LGD_OK;
 }

@@ -603,11 +603,7 @@ int amr_regrid(lgd_sim *s) {
     hipError_t e = hipMemcpyAsync(b->data, s->state[f]->data, (size_t)fine0 * sizeof(double),
                                   hipMemcpyDeviceToDevice, s->ctx->stream);
     if (e != hipSuccess) return abandon(hip_fail(s->ctx, e, "migrate level 0"));
-    if (dev) {
-      if ((rc = lgd_regrid_migrate_device(s->ctx, g, s->state[f]->data, *ng, b->data)))
-        return abandon(rc);
-      continue;
-    }
+    if (dev) continue;
     if ((int)f == tf && !tag.empty()) {
       old_f.swap(tag);  // already downloaded
     } else {
@@ -621,6 +617,16 @@ int amr_regrid(lgd_sim *s) {
                     hipMemcpyHostToDevice);
       if (e != hipSuccess) return abandon(hip_fail(s->ctx, e, "migrate upload"));
     }
+  }
+  if (dev) {
+    std::vector<const double *> of;
+    std::vector<double *> nfp;
+    for (size_t f = 0; f < s->state.size(); ++f) {
+      of.push_back(s->state[f]->data);
+      nfp.push_back(ns[f]->data);
+    }
+    if ((rc = lgd_regrid_migrate_device(s->ctx, g, of.data(), *ng, nfp.data(), (int)of.size())))
+      return abandon(rc);
   }
   int32_t bad = 0;
   if ((rc = lgd_amr_plan_error(s->ctx, *ng, &bad))) return abandon(rc);
