@@ -111,6 +111,9 @@ struct lgd_level_plan {
   lgd_copy_child *cchild = nullptr;  // per op, on levels with a finer level (else null)
   lgd_restrict_item *ritems = nullptr;  // restriction of this level onto level-1
   int64_t nritems = 0;
+  // First restriction item of each of the level's patches, + end (device): patch p's
+  // items are its coarse cells row by row, (nx / r) per row.
+  int64_t *rit0 = nullptr;
   // The streaming level stage: column segments of every patch cut into strips of
   // seg_rows rows (patch-major; seg0 = first item of each patch, + end).
   lgd_sseg *segs = nullptr;
@@ -148,6 +151,11 @@ bool lgd_amr_level_streams(const lgd_ctx *ctx, const lgd_amr &g, int level);
 // (lgd_amr_modelc_step's `cf`): a streamed level with a finer one, batched launches,
 // the "amr_split_fill" knob unset.
 bool lgd_amr_child_fill_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level);
+// Whether the restriction of `level` into its parent (and the parent's next fill) may
+// share the level's stage launch (lgd_amr_modelc_step's `ps`): a streamed level
+// >= 1 refined by 2 over its parent, an even strip height, batched launches, the
+// "amr_split_fill" knob unset.
+bool lgd_amr_restrict_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level);
 
 // Workgroup width of the AMR fill and restriction launches ("amr_wg"; 256 by default).
 inline int lgd_amr_wg(const lgd_ctx *ctx) { return ctx->amr_wg > 0 ? (int)ctx->amr_wg : 256; }
@@ -278,6 +286,22 @@ struct lgd_amr_next_fill {
 struct lgd_amr_child_fill {
   double *lvl[2];
 };
+// The end of a level's last substep within its parent's substep, in the stage
+// launch of that substep: the restriction into the parent (coarse: the parent's
+// buffers that take it, fields 0, 1) and, with `fill`, the parent's ghosts for its
+// next substep in those buffers (copies, the copies from cells under this level
+// restricted from the stage's own new values, and prolongation from the
+// grandparent's current (par) and previous (old) buffers blended at alpha).  Each
+// wavefront restricts the 2×2 blocks of its strip (ratio 2, an even strip height);
+// the fill's threads recompute the four children of a covered source from the
+// stage's inputs.  Bitwise the restriction and fill launched after the stage.
+struct lgd_amr_parent_step {
+  double *coarse[2];
+  bool fill;
+  const double *par[2];
+  const double *old[2];
+  double alpha;
+};
 // One fused Euler–Maruyama stage of `level`: (phi, u) → (phi_n, u_n) interiors, and
 // with `nf` the ghosts of (phi_n, u_n) the level's next substep reads, or with `cf`
 // (a streamed level with a finer one; LGD_ERR_UNSUPPORTED otherwise) the first fill of
@@ -286,7 +310,8 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
                         double dt, const double *phi, const double *u, double *phi_n, double *u_n,
                         uint64_t seed, uint64_t salt, bool wiener,
                         const lgd_amr_next_fill *nf = nullptr,
-                        const lgd_amr_child_fill *cf = nullptr);
+                        const lgd_amr_child_fill *cf = nullptr,
+                        const lgd_amr_parent_step *ps = nullptr);
 int lgd_amr_heat_step(lgd_ctx *ctx, const lgd_amr &g, double kappa, int level, double dt,
                       const double *u, double *u_n, const lgd_amr_next_fill *nf = nullptr);
 // Copy one level's range (interiors and ghosts) between buffers of the layout.

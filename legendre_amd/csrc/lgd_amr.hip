@@ -56,8 +56,11 @@ void build_segs(const lgd_amr &g, int level, int rows, std::vector<lgd_sseg> &ou
     seg0.push_back((int64_t)out.size());
     const int64_t nx = g.box[b].extent(0), ny = g.box[b].extent(1);
     const int64_t nseg = (nx + kSegCols - 1) / kSegCols;
+    // Balanced widths; an even-width patch cuts at even columns (at most kSegCols
+    // wide still), so a ratio-2 block of 2×2 cells never spans two segments.
+    const int64_t even = nx % 2 == 0 ? ~int64_t(1) : ~int64_t(0);
     for (int64_t s = 0; s < nseg; ++s) {
-      const int64_t x0 = s * nx / nseg, x1 = (s + 1) * nx / nseg;
+      const int64_t x0 = (s * nx / nseg) & even, x1 = ((s + 1) * nx / nseg) & even;
       for (int64_t j0 = 0; j0 < ny; j0 += rows)
         out.push_back(lgd_sseg{g.off[b], (int32_t)b, (int32_t)nx, (int32_t)x0, (int32_t)(x1 - x0),
                                (int32_t)j0, (int32_t)std::min<int64_t>(j0 + rows, ny)});
@@ -245,7 +248,9 @@ int lgd_amr_default_seg_rows(const lgd_amr &g, int level) {
     rows += (g.box[b].extent(0) + kSegCols - 1) / kSegCols * g.box[b].extent(1);
   const int64_t cus = g.ctx && g.ctx->num_cus > 0 ? g.ctx->num_cus : 256;
   const int64_t waves = 2 * 4 * cus;
-  return (int)std::min<int64_t>(32, std::max<int64_t>(2, (rows + waves - 1) / waves));
+  // Even, so that a ratio-2 level's 2×2 blocks never straddle two strips.
+  const int64_t r = std::min<int64_t>(32, std::max<int64_t>(2, (rows + waves - 1) / waves));
+  return (int)(r + (r & 1));
 }
 
 // AmrGrid::from_patches (src/geometry/amr/grid.rs:90-196): level 0 is the base
@@ -620,6 +625,7 @@ int lgd_amr_build_plans(lgd_amr &g, const lgd_amr *reuse0) {
           P.corners, l + 1 < g.num_levels() ? (int64_t)g.ratios[(size_t)l] : 0, P.cchild, g.d_err);
       if ((rc = launch_check(ctx, "amr fill plan"))) return rc;
     }
+    g.plans[(size_t)l].rit0 = Hl.d_rit0;
     if (l >= 1 && P.nritems > 0) {
       plan_restrict_kernel<<<blocks_for(P.nritems, 256), 256, 0, ctx->stream>>>(
           ix, l, b0, np, Hl.d_rit0, P.nritems, (int64_t)g.ratios[(size_t)l - 1], P.ritems, g.d_err);
@@ -857,6 +863,37 @@ struct ChildFill {
   double *lvl[2];
 };
 
+// The end of a level's substep in its stage launch (lgd_amr_parent_step): the
+// restriction into the parent through the level's restriction plan, and the
+// parent's fill plan.  b0 / nb: the stepped level's patches.
+struct ParentStep {
+  double *coarse[2];
+  const lgd_restrict_item *items;
+  const int64_t *rit0;
+  double inv;
+  const lgd_fill_op *ops;
+  const lgd_corner *corners;
+  const lgd_copy_child *cchild;
+  int64_t nops;
+  const double *par[2];
+  const double *old[2];
+  double one_minus_alpha, alpha;
+  uint32_t b0, nb;
+};
+
+// The stepped level's patch holding element o (patches are laid out in order of
+// their slab offsets).
+__device__ __forceinline__ uint32_t patch_of(const lgd_block_desc *blocks, uint32_t b0, uint32_t nb,
+                                             int64_t o) {
+  uint32_t lo = b0, hi = b0 + nb;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (blocks[mid].off <= o) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // One fused Model C stage of a level: workgroups [0, ntiles) advance the tiles'
 // cells; with kFill, workgroups from ntiles on write the next substep's ghosts.
 template <bool kNoise, bool kFill>
@@ -930,10 +967,11 @@ struct SegArgs {
   const double *salt_src;
 };
 
-// kFill: 0 none, 1 the next substep's ghosts (N), 2 the child level's first fill (C).
+// kFill: 0 none, 1 the next substep's ghosts (N), 2 the child level's first fill (C),
+// 3 the restriction into the parent and its next fill (Q).
 template <bool kNoise, int kFill, int kDepth>
 __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
-    SegArgs a, const lgd_block_desc *__restrict__ blocks, NextFill N, ChildFill C) {
+    SegArgs a, const lgd_block_desc *__restrict__ blocks, NextFill N, ChildFill C, ParentStep Q) {
   using namespace lgd_stream;
   // The fill's workgroups come first, so that they run beside the streaming waves
   // rather than after them (their copy ghosts each recompute a whole cell).
@@ -959,6 +997,45 @@ __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
         if (c < op.clen) vo = vo + old[k[c].src] * k[c].weight;
       C.lvl[f][op.target] = vo;
     }
+    return;
+  }
+  if (kFill == 3 && (int)blockIdx.x < a.fill_wg) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Q.nops) return;
+    lgd_corner kk[4];
+    load_corners(Q.corners, t, kk);
+    const lgd_fill_op op = Q.ops[t];
+    if (op.clen != 0) {
+      for (int f = 0; f < 2; ++f)
+        Q.coarse[f][op.target] =
+            prolong_value<true>(kk, op.clen, Q.par[f], Q.old[f], Q.one_minus_alpha, Q.alpha);
+      return;
+    }
+    const lgd_copy_child cc = Q.cchild[t];
+    if (cc.child < 0) {
+      Q.coarse[0][op.target] = Q.coarse[0][op.src];
+      Q.coarse[1][op.target] = Q.coarse[1][op.src];
+      return;
+    }
+    // A source cell under this level: its children's new values, recomputed from
+    // the stage's inputs, restricted (children row by row, from 0) · 1/4.
+    const uint64_t st = kNoise ? lgd_salt(a.salt, a.salt_src) : 0;
+    double sp = 0.0, su = 0.0;
+#pragma unroll 1
+    for (int c = 0; c < 4; ++c) {  // (kx, ky) = (c & 1, c >> 1)
+      const int64_t o = cc.child + (c >> 1) * cc.stride + (c & 1);
+      const uint32_t b = patch_of(blocks, Q.b0, Q.nb, o);
+      const lgd_block_desc bd = blocks[b];
+      int i, j;
+      slab_coords(bd, o, i, j);
+      double pn, un;
+      modelc_cell_stage<kNoise>(a.k, a.dt, a.phi, a.u, o, bd.nx + 2, a.noise_scale, a.seed, st, b,
+                                i + (int64_t)j * bd.nx, pn, un);
+      sp = sp + pn;
+      su = su + un;
+    }
+    Q.coarse[0][op.target] = sp * Q.inv;
+    Q.coarse[1][op.target] = su * Q.inv;
     return;
   }
   if (kFill == 1 && (int)blockIdx.x < a.fill_wg) {
@@ -1021,6 +1098,7 @@ __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
   // top neighbour of row j + 1 is moved into place after row j is computed (two
   // alternating buffers for kDepth 2, so no register with a load in flight is
   // copied).
+  double rp = 0.0, ru = 0.0;  // kFill 3: the even row's new values
   auto body = [&](int j, double &ip, double &iu, double &cp, double &cu) {
     const int rn = min(j + 1 + kDepth, j1);
     ip = ld(a.phi, rn);
@@ -1035,6 +1113,32 @@ __global__ __launch_bounds__(64 * kSegWaves) void amr_stream_kernel(
       n = n + a.noise_scale *
                   lgd_standard_normal(lgd_splitmix64(block_key ^ ((uint64_t)j * (uint64_t)S.nx + kc)));
     const int64_t o = cbase + (int64_t)(j + 1) * sx;
+    if constexpr (kFill == 3) {
+      // Rows j − 1 (even) and j of a 2×2 block: the even column's lane restricts the
+      // block (children row by row, from 0) into its parent cell.
+      if (j & 1) {
+        const double rpR = from_right(rp), ruR = from_right(ru), nR = from_right(n), vR = from_right(v);
+        const int64_t ci = (int64_t)S.x0 + lane - 1;
+        if (in && (ci & 1) == 0) {
+          double sp = 0.0, su = 0.0;
+          sp = sp + rp;
+          sp = sp + rpR;
+          sp = sp + n;
+          sp = sp + nR;
+          su = su + ru;
+          su = su + ruR;
+          su = su + v;
+          su = su + vR;
+          const int64_t t = Q.rit0[S.block - (int32_t)Q.b0] + (int64_t)(j >> 1) * (S.nx >> 1) + (ci >> 1);
+          const int64_t tg = Q.items[t].target;
+          Q.coarse[0][tg] = sp * Q.inv;
+          Q.coarse[1][tg] = su * Q.inv;
+        }
+      } else {
+        rp = n;
+        ru = v;
+      }
+    }
     jb = jt;
     if (j + 1 < j1) {
       pm = pc; pmR = pcR;
@@ -1205,6 +1309,14 @@ bool lgd_amr_level_streams(const lgd_ctx *ctx, const lgd_amr &g, int level) {
   return !(level == 0 && lgd_amr_level0_streams(ctx, g));
 }
 
+bool lgd_amr_restrict_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level) {
+  if (level < 1 || level >= g.num_levels() || !lgd_amr_level_streams(ctx, g, level)) return false;
+  if (ctx->amr_per_block || ctx->amr_split_fill || g.ratios[(size_t)level - 1] != 2) return false;
+  const lgd_level_plan &P = g.plans[(size_t)level];
+  const int rows = ctx->amr_rows > 0 ? (int)ctx->amr_rows : P.auto_rows;
+  return rows % 2 == 0;
+}
+
 bool lgd_amr_child_fill_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level) {
   return level + 1 < g.num_levels() && lgd_amr_level_streams(ctx, g, level) && !ctx->amr_per_block &&
          !ctx->amr_split_fill;
@@ -1331,10 +1443,12 @@ NextFill next_fill_args(const lgd_level_plan &P, const lgd_amr_next_fill *nf, in
 int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params &p, int level,
                         double dt, const double *phi, const double *u, double *phi_n, double *u_n,
                         uint64_t seed, uint64_t salt, bool wiener, const lgd_amr_next_fill *nf,
-                        const lgd_amr_child_fill *cf) {
+                        const lgd_amr_child_fill *cf, const lgd_amr_parent_step *ps) {
   const lgd_level_plan &P = g.plans[(size_t)level];
-  if (cf && (nf || !lgd_amr_child_fill_fuses(ctx, g, level)))
+  if (cf && (nf || ps || !lgd_amr_child_fill_fuses(ctx, g, level)))
     return fail(ctx, LGD_ERR_UNSUPPORTED, "the child level's fill shares only a streamed stage launch");
+  if (ps && (nf || !lgd_amr_restrict_fuses(ctx, g, level)))
+    return fail(ctx, LGD_ERR_UNSUPPORTED, "the restriction shares only a streamed ratio-2 stage launch");
   lgd_cart c = g.base;
   c.h[0] = g.spacing[(size_t)level][0];
   c.h[1] = g.spacing[(size_t)level][1];
@@ -1375,10 +1489,22 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
   if (lgd_amr_level_streams(ctx, g, level)) {
     StageSegs T;
     if ((rc = stage_segs(ctx, g, level, T))) return rc;
-    const lgd_level_plan *FP = cf ? &g.plans[(size_t)level + 1] : &P;
-    const int mode = cf ? (FP->nops ? 2 : 0) : (nf && P.nops ? 1 : 0);
+    // The fused work: 1 the next substep's ghosts, 2 the finer level's first fill,
+    // 3 the restriction into the parent (and its next fill); fops: the fill ops.
+    int mode = 0;
+    int64_t fops = 0;
+    if (ps) {
+      mode = 3;
+      fops = ps->fill ? g.plans[(size_t)level - 1].nops : 0;
+    } else if (cf) {
+      fops = g.plans[(size_t)level + 1].nops;
+      mode = fops ? 2 : 0;
+    } else if (nf && P.nops) {
+      mode = 1;
+      fops = P.nops;
+    }
     const bool fill = mode != 0;
-    if (T.n == 0 && !fill) return LGD_OK;
+    if (T.n == 0 && fops == 0) return LGD_OK;
     if (ctx->amr_per_block && nf)
       return fail(ctx, LGD_ERR_UNSUPPORTED, "per-patch launches fill ghosts in a launch of their own");
     SegArgs A{};
@@ -1395,16 +1521,39 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
     const NextFill N = mode == 1 ? next_fill_args(P, nf, 2) : NextFill{};
     ChildFill C{};
     if (mode == 2) {
-      C.ops = FP->ops;
-      C.corners = FP->corners;
-      C.nops = FP->nops;
+      const lgd_level_plan &FP = g.plans[(size_t)level + 1];
+      C.ops = FP.ops;
+      C.corners = FP.corners;
+      C.nops = FP.nops;
       C.lvl[0] = cf->lvl[0];
       C.lvl[1] = cf->lvl[1];
     }
+    ParentStep Q{};
+    if (mode == 3) {
+      const lgd_level_plan &PP = g.plans[(size_t)level - 1];
+      Q.coarse[0] = ps->coarse[0];
+      Q.coarse[1] = ps->coarse[1];
+      Q.items = P.ritems;
+      Q.rit0 = P.rit0;
+      Q.inv = 1.0 / (double)(2 * 2);
+      if (ps->fill) {
+        Q.ops = PP.ops;
+        Q.corners = PP.corners;
+        Q.cchild = PP.cchild;
+        Q.nops = PP.nops;
+        for (int f = 0; f < 2; ++f) {
+          Q.par[f] = ps->par[f];
+          Q.old[f] = ps->old[f];
+        }
+        Q.one_minus_alpha = 1.0 - ps->alpha;
+        Q.alpha = ps->alpha;
+      }
+      Q.b0 = (uint32_t)g.level_start[(size_t)level];
+      Q.nb = (uint32_t)(g.level_start[(size_t)level + 1] - g.level_start[(size_t)level]);
+    }
     auto go = [&](const lgd_sseg *segs, int64_t n, bool with_fill) -> int {
       const int64_t nwg = (n + kSegWaves - 1) / kSegWaves;
-      const int64_t fwg =
-          with_fill ? ((FP->nops + 64 * kSegWaves - 1) / (64 * kSegWaves) + 7) / 8 * 8 : 0;
+      const int64_t fwg = with_fill ? ((fops + 64 * kSegWaves - 1) / (64 * kSegWaves) + 7) / 8 * 8 : 0;
       const int64_t total = nwg + fwg;
       if (total == 0) return LGD_OK;
       if (total > INT32_MAX) return fail(ctx, LGD_ERR_UNSUPPORTED, "AMR level too large for one launch");
@@ -1416,13 +1565,15 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
       hipStream_t st = ctx->stream;
       const int m = with_fill ? mode : 0;
       if (noise) {
-        if (m == 1) amr_stream_kernel<true, 1, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
-        else if (m == 2) amr_stream_kernel<true, 2, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
-        else amr_stream_kernel<true, 0, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
+        if (m == 1) amr_stream_kernel<true, 1, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
+        else if (m == 2) amr_stream_kernel<true, 2, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
+        else if (m == 3) amr_stream_kernel<true, 3, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
+        else amr_stream_kernel<true, 0, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
       } else {
-        if (m == 1) amr_stream_kernel<false, 1, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
-        else if (m == 2) amr_stream_kernel<false, 2, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
-        else amr_stream_kernel<false, 0, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C);
+        if (m == 1) amr_stream_kernel<false, 1, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
+        else if (m == 2) amr_stream_kernel<false, 2, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
+        else if (m == 3) amr_stream_kernel<false, 3, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
+        else amr_stream_kernel<false, 0, 1><<<grid, blk, 0, st>>>(A, g.d_blocks, N, C, Q);
       }
       return LGD_OK;
     };

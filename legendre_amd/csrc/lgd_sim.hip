@@ -393,14 +393,30 @@ int amr_restrict_level(lgd_sim *s, int fine_level) {
 // noise reads while a graph is captured.
 // With next_alpha >= 0 the same launch also fills the ghosts of the level's next
 // substep (a level without children; the parent's buffers blended at next_alpha);
-// with child_fill the ghosts of level + 1 for its first substep (blended at α = 0).
+// with child_fill the ghosts of level + 1 for its first substep (blended at α = 0);
+// with parent_step 1 the restriction of the level's new values into its parent, 2
+// also the parent's ghosts for its next substep (the grandparent blended at
+// parent_alpha).
 int amr_level_step(lgd_sim *s, int level, double dt, uint64_t salt, int tslot,
-                   double next_alpha = -1.0, bool child_fill = false) {
+                   double next_alpha = -1.0, bool child_fill = false, int parent_step = 0,
+                   double parent_alpha = 0.0) {
   const lgd_amr &g = *s->amr;
   int rc;
   lgd_amr_child_fill cf{};
   if (child_fill)
     for (int f = 0; f < nfields(s); ++f) cf.lvl[f] = amr_cur(s, f, level + 1)->data;
+  lgd_amr_parent_step ps{};
+  if (parent_step) {
+    ps.fill = parent_step == 2;
+    ps.alpha = parent_alpha;
+    for (int f = 0; f < nfields(s); ++f) {
+      ps.coarse[f] = amr_cur(s, f, level - 1)->data;
+      if (ps.fill) {
+        ps.par[f] = amr_cur(s, f, level - 2)->data;
+        ps.old[f] = amr_prev(s, f, level - 2)->data;
+      }
+    }
+  }
   lgd_amr_next_fill nf{};
   if (next_alpha >= 0.0) {
     for (int f = 0; f < nfields(s); ++f) {
@@ -415,10 +431,10 @@ int amr_level_step(lgd_sim *s, int level, double dt, uint64_t salt, int tslot,
     rc = lgd_amr_modelc_step(s->ctx, g, s->params, level, dt, amr_cur(s, 0, level)->data,
                              amr_cur(s, 1, level)->data, amr_prev(s, 0, level)->data,
                              amr_prev(s, 1, level)->data, s->seed, salt, s->wiener != 0, nfp,
-                             child_fill ? &cf : nullptr);
+                             child_fill ? &cf : nullptr, parent_step ? &ps : nullptr);
     s->ctx->salt_src = nullptr;
-  } else if (child_fill) {
-    return fail(s->ctx, LGD_ERR_UNSUPPORTED, "the finer level's fill shares only a Model C stage");
+  } else if (child_fill || parent_step) {
+    return fail(s->ctx, LGD_ERR_UNSUPPORTED, "fused fills and restrictions share only a Model C stage");
   } else {
     rc = lgd_amr_heat_step(s->ctx, g, s->kappa, level, dt, amr_cur(s, 0, level)->data,
                            amr_prev(s, 0, level)->data, nfp);
@@ -451,9 +467,14 @@ int amr_global_step(lgd_sim *s, double dt, uint64_t salt) {
 // ends one substep (defer_out) and the fill that starts the next (fuse_in) share a
 // launch; the last substep restricts on its own.  The child's first fill shares the
 // level's stage launch where that streams (filled_in: the caller's launch wrote
-// this level's ghosts).
+// this level's ghosts).  A finest child whose restriction shares its last stage
+// launch restricts there, and with defer_out also fills this level's ghosts for
+// its next substep (at next_alpha): *carry tells the caller which (1 restricted,
+// 2 restricted and filled; 0 neither).
 int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha, bool fuse_in = false,
-                bool defer_out = false, bool filled_in = false) {
+                bool defer_out = false, bool filled_in = false, double next_alpha = -1.0,
+                int *carry = nullptr) {
+  if (carry) *carry = 0;
   const lgd_amr &g = *s->amr;
   int rc;
   if (fuse_in) {
@@ -477,6 +498,10 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha, bool f
     // A child with a finer level: its restriction and its next fill share launches.
     const bool chain = child + 1 < g.num_levels() && !s->ctx->amr_per_block &&
                        !s->ctx->amr_split_fill && nn > 1;
+    // The finest child's last substep restricts into this level in its stage launch.
+    const bool fuse_restrict =
+        fused && s->kind == lgd_sim::kModelC && lgd_amr_restrict_fuses(s->ctx, g, child);
+    int from_child = 0;  // what the child's last substep did for the next one
     for (uint64_t k = 0; k < nn; ++k) {
       if (s->capturing) {
         time_substep_kernel<<<1, 1, 0, s->ctx->stream>>>(s->d_t, child, level, (double)k, dt_child);
@@ -484,8 +509,11 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha, bool f
       }
       const double tk = std::fma((double)k, dt_child, t), ak = (double)k / (double)nn;
       if (!fused) {
-        if ((rc = amr_advance(s, child, tk, dt_child, ak, chain && k > 0, chain && k + 1 < nn,
-                              child_fill && k == 0)))
+        const double na = k + 1 < nn ? (double)(k + 1) / (double)nn : -1.0;
+        const int prev = from_child;
+        if ((rc = amr_advance(s, child, tk, dt_child, ak, chain && k > 0 && prev == 0,
+                              chain && k + 1 < nn, (child_fill && k == 0) || prev == 2, na,
+                              &from_child)))
           return rc;
         continue;
       }
@@ -494,9 +522,16 @@ int amr_advance(lgd_sim *s, int level, double t, double dt, double alpha, bool f
       uint64_t salt_k;
       std::memcpy(&salt_k, &tk, sizeof salt_k);
       const double next = k + 1 < nn ? (double)(k + 1) / (double)nn : -1.0;
+      if (fuse_restrict && k + 1 == nn) {
+        const int ps = defer_out && next_alpha >= 0.0 && level >= 1 ? 2 : 1;
+        if ((rc = amr_level_step(s, child, dt_child, salt_k, child, -1.0, false, ps, next_alpha)))
+          return rc;
+        if (carry) *carry = ps;
+        continue;
+      }
       if ((rc = amr_level_step(s, child, dt_child, salt_k, child, next))) return rc;
     }
-    if (!defer_out && (rc = amr_restrict_level(s, child))) return rc;
+    if (!defer_out && !fuse_restrict && (rc = amr_restrict_level(s, child))) return rc;
   }
   return LGD_OK;
 }
@@ -1141,11 +1176,50 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
         }
       } else if (l > 0 && batched && N[(size_t)l] > N[(size_t)l - 1]) {
         // The finest level: all but the last substep of each parent substep also
-        // write the next substep's ghosts.
+        // write the next substep's ghosts; the last one restricts into the parent
+        // (and fills the parent's next ghosts) in its own launch or before the
+        // parent's next fill.  Into the parent's other buffers, which its next stage
+        // overwrites.
         rc = timed([&] { return stage(l, true); }, &ms);
         tot += (N[(size_t)l] - N[(size_t)l - 1]) * ms;
-        if (!rc) rc = timed([&] { return stage(l); }, &ms);
-        tot += N[(size_t)l - 1] * ms;
+        if (!rc && modelc && lgd_amr_restrict_fuses(ctx, g, l)) {
+          lgd_amr_parent_step ps{};
+          ps.fill = l >= 2;
+          ps.alpha = 0.5;
+          for (int f = 0; f < nf; ++f) {
+            ps.coarse[f] = amr_prev(sim, f, l - 1)->data;
+            if (ps.fill) {
+              ps.par[f] = amr_cur(sim, f, l - 2)->data;
+              ps.old[f] = amr_prev(sim, f, l - 2)->data;
+            }
+          }
+          rc = timed([&] {
+            return lgd_amr_modelc_step(ctx, g, sim->params, l, dt, amr_cur(sim, 0, l)->data,
+                                       amr_cur(sim, 1, l)->data, amr_prev(sim, 0, l)->data,
+                                       amr_prev(sim, 1, l)->data, sim->seed, salt, sim->wiener != 0,
+                                       nullptr, nullptr, &ps);
+          }, &ms);
+          tot += N[(size_t)l - 1] * ms;
+        } else if (!rc) {
+          rc = timed([&] { return stage(l); }, &ms);
+          tot += N[(size_t)l - 1] * ms;
+          if (!rc && l >= 2) {
+            double *lv[2] = {nullptr, nullptr};
+            const double *par[2] = {nullptr, nullptr}, *old[2] = {nullptr, nullptr}, *fine[2] = {nullptr, nullptr};
+            for (int f = 0; f < nf; ++f) {
+              lv[f] = amr_prev(sim, f, l - 1)->data;
+              par[f] = amr_cur(sim, f, l - 2)->data;
+              old[f] = amr_prev(sim, f, l - 2)->data;
+              fine[f] = amr_cur(sim, f, l)->data;
+            }
+            rc = timed([&] { return lgd_amr_restrict_fill_pp(ctx, g, l - 1, nf, lv, par, old, 0.5, fine); },
+                       &ms);
+            tot += N[(size_t)l - 1] * ms;
+          } else if (!rc) {
+            rc = timed([&] { return restrict_prev(l); }, &ms);
+            tot += N[(size_t)l - 1] * ms;
+          }
+        }
       } else {
         rc = timed([&] { return stage(l); }, &ms);
         tot += N[(size_t)l] * ms;
