@@ -148,8 +148,8 @@ int lgd_amr_default_seg_rows(const lgd_amr &g, int level);
 // launches, the "amr_kernel" knob not asking for the per-cell tiles).
 bool lgd_amr_level_streams(const lgd_ctx *ctx, const lgd_amr &g, int level);
 // Whether the first fill of level + 1 may share the stage launch of `level`
-// (lgd_amr_modelc_step's `cf`): a streamed level with a finer one, batched launches,
-// the "amr_split_fill" knob unset.
+// (lgd_amr_modelc_step's `cf`): a streamed level with a finer one (level 0 on the
+// Cartesian streaming kernel too), batched launches, the "amr_split_fill" knob unset.
 bool lgd_amr_child_fill_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level);
 // Whether the restriction of `level` into its parent (and the parent's next fill) may
 // share the level's stage launch (lgd_amr_modelc_step's `ps`): a streamed level

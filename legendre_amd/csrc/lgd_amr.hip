@@ -1318,8 +1318,8 @@ bool lgd_amr_restrict_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level) {
 }
 
 bool lgd_amr_child_fill_fuses(const lgd_ctx *ctx, const lgd_amr &g, int level) {
-  return level + 1 < g.num_levels() && lgd_amr_level_streams(ctx, g, level) && !ctx->amr_per_block &&
-         !ctx->amr_split_fill;
+  const bool streams = lgd_amr_level_streams(ctx, g, level) || (level == 0 && lgd_amr_level0_streams(ctx, g));
+  return level + 1 < g.num_levels() && streams && !ctx->amr_per_block && !ctx->amr_split_fill;
 }
 
 namespace {
@@ -1481,9 +1481,18 @@ int lgd_amr_modelc_step(lgd_ctx *ctx, const lgd_amr &g, const lgd_modelc_params 
       }
       return LGD_OK;
     }
+    lgd_stream_side_fill side{};
+    if (cf) {
+      const lgd_level_plan &FP = g.plans[1];
+      side.ops = FP.ops;
+      side.corners = FP.corners;
+      side.nops = FP.nops;
+      side.lvl[0] = cf->lvl[0];
+      side.lvl[1] = cf->lvl[1];
+    }
     return lgd_modelc_stream_launch(ctx, g.base.n[0], g.base.n[1], d0, nullptr, nullptr, 0, nb0, phi,
                                     u, phi_n, u_n, dt, noise, salt, wiener0, shape.strips,
-                                    shape.prefetch, pair);
+                                    shape.prefetch, pair, cf ? &side : nullptr);
   }
   int rc;
   if (lgd_amr_level_streams(ctx, g, level)) {

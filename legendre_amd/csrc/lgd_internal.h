@@ -220,16 +220,30 @@ int lgd_modelc_step_impl(lgd_ctx *ctx, const lgd_grid *grid, const lgd_modelc_pa
                          double dt, uint64_t seed, uint64_t salt, int32_t wiener,
                          const double *cs0, const double *sn0);
 
+// The ghost fill of an AMR level at α = 0 from the level a streaming launch steps
+// (its parent): the level's fill ops with their four corner slots each (lgd_amr.h)
+// and its current buffers (fields 0, 1).  Copies read the level's own buffers,
+// prolonged ghosts the corner sum of the launch's inputs (phi, u).
+struct lgd_fill_op;
+struct lgd_corner;
+struct lgd_stream_side_fill {
+  const lgd_fill_op *ops;
+  const lgd_corner *corners;
+  int64_t nops;
+  double *lvl[2];
+};
+
 // Streaming Model C step (lgd_modelc_stream.hip) over blocks
 // [block_begin, block_begin + nblocks); LGD_ERR_UNSUPPORTED when the block is
 // wider than 126 interior cells.  `pair` runs two x-neighbouring blocks per wave
-// (blocks at most 64 wide; see lgd_stream_pairable).
+// (blocks at most 64 wide; see lgd_stream_pairable).  With `side` (one domain, no
+// domain table) the launch's first workgroups write that fill.
 int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_domain &d0,
                              const lgd_domain *domains, const int32_t *block_domain,
                              int64_t block_begin, int64_t nblocks, const double *phi,
                              const double *u, double *phi_n, double *u_n, double dt, bool noise,
                              uint64_t salt, uint64_t stream, int strips, int prefetch,
-                             bool pair = false);
+                             bool pair = false, const lgd_stream_side_fill *side = nullptr);
 
 // Whether a domain of nbx blocks of nx interior columns per row can run as block
 // pairs: nx even, 4 <= nx <= 64, nbx even.

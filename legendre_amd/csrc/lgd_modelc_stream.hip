@@ -36,6 +36,7 @@
 // XCD and share its L2.
 #include <cmath>
 
+#include "lgd_amr.h"
 #include "lgd_internal.h"
 #include "lgd_stream_ops.h"
 
@@ -73,8 +74,10 @@ struct StreamArgs {
   int32_t nx, ny;      // block interior extent
   int32_t strips;      // strips per block
   int32_t rows;        // rows per strip
-  int32_t nwg;         // workgroups in the launch
+  int32_t nwg;         // workgroups of the step in the launch
   Rk4Stage rk;         // kStage >= 1 only
+  lgd_stream_side_fill side;  // kSide only: written by the first side_wg workgroups
+  int32_t side_wg;
 };
 
 __device__ __forceinline__ int64_t wrap_or_reflect(int64_t g, int64_t n, int periodic) {
@@ -108,12 +111,41 @@ struct RowVal {
   double v0, v1, h;
 };
 
-template <bool kNoise, bool kVec, bool kTable, int kDepth, bool kPair, int kStage>
+template <bool kNoise, bool kVec, bool kTable, int kDepth, bool kPair, int kStage, bool kSide = false>
 __global__ __launch_bounds__(64 * kWavesPerWG) void modelc_stream_kernel(StreamArgs a) {
   static_assert(kStage == 0 || (!kNoise && !kPair && !kVec), "RK4 stages: no noise, one block");
+  static_assert(!kSide || (kStage == 0 && !kTable), "a side fill rides on a one-domain step");
+  int w = blockIdx.x;
+  if constexpr (kSide) {
+    // The side fill first (α = 0: the corner sum of this launch's inputs, which it
+    // only reads; copies within the filled level).
+    if (w < a.side_wg) {
+      const int64_t t = (int64_t)w * blockDim.x + threadIdx.x;
+      if (t >= a.side.nops) return;
+      const lgd_fill_op op = a.side.ops[t];
+      if (op.clen == 0) {
+        a.side.lvl[0][op.target] = a.side.lvl[0][op.src];
+        a.side.lvl[1][op.target] = a.side.lvl[1][op.src];
+        return;
+      }
+      lgd_corner k[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) k[c] = a.side.corners[4 * t + c];
+      double vp = 0.0, vu = 0.0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c < op.clen) {
+          vp = vp + a.phi[k[c].src] * k[c].weight;
+          vu = vu + a.u[k[c].src] * k[c].weight;
+        }
+      a.side.lvl[0][op.target] = vp;
+      a.side.lvl[1][op.target] = vu;
+      return;
+    }
+    w -= a.side_wg;
+  }
   // XCD-aware workgroup order: XCD x (= blockIdx.x % 8) takes one contiguous
-  // range of logical workgroups.
-  const int w = blockIdx.x;
+  // range of logical workgroups (the side fill's count is a multiple of 8).
   const int xcd = w % kXcds, idx = w / kXcds;
   const int per = a.nwg / kXcds, rem = a.nwg % kXcds;
   const int lw = xcd * per + min(xcd, rem) + idx;
@@ -481,6 +513,22 @@ void launch_variant(dim3 grid, dim3 block, hipStream_t st, const StreamArgs &a, 
 #undef LGD_STREAM_LAUNCH
 }
 
+// The step with a side fill (one domain, no table).
+template <int kDepth>
+void launch_side(dim3 grid, dim3 block, hipStream_t st, const StreamArgs &a, bool noise, bool vec,
+                 bool pair) {
+  if (pair) {
+    if (noise) modelc_stream_kernel<true, false, false, kDepth, true, 0, true><<<grid, block, 0, st>>>(a);
+    else modelc_stream_kernel<false, false, false, kDepth, true, 0, true><<<grid, block, 0, st>>>(a);
+  } else if (vec) {
+    if (noise) modelc_stream_kernel<true, true, false, kDepth, false, 0, true><<<grid, block, 0, st>>>(a);
+    else modelc_stream_kernel<false, true, false, kDepth, false, 0, true><<<grid, block, 0, st>>>(a);
+  } else {
+    if (noise) modelc_stream_kernel<true, false, false, kDepth, false, 0, true><<<grid, block, 0, st>>>(a);
+    else modelc_stream_kernel<false, false, false, kDepth, false, 0, true><<<grid, block, 0, st>>>(a);
+  }
+}
+
 }  // namespace
 
 bool lgd_stream_pairable(int64_t nx, int64_t nbx) {
@@ -498,9 +546,11 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
                              const lgd_domain *domains, const int32_t *block_domain,
                              int64_t block_begin, int64_t nblocks, const double *phi,
                              const double *u, double *phi_n, double *u_n, double dt, bool noise,
-                             uint64_t salt, uint64_t stream, int strips, int prefetch, bool pair) {
+                             uint64_t salt, uint64_t stream, int strips, int prefetch, bool pair,
+                             const lgd_stream_side_fill *side) {
   if (nx + 2 > 128 || nx < 1 || ny < 1) return LGD_ERR_UNSUPPORTED;
-  if (nblocks <= 0) return LGD_OK;
+  if (side && domains) return lgd::fail(ctx, LGD_ERR_INVALID_ARGUMENT, "a side fill needs one domain");
+  if (nblocks <= 0 && !(side && side->nops > 0)) return LGD_OK;
   if (pair && (!lgd_stream_pairable(nx, 2) || block_begin % 2 != 0 || nblocks % 2 != 0))
     return lgd::fail(ctx, LGD_ERR_INVALID_ARGUMENT, "block pairs need an even block count and width");
   StreamArgs a;
@@ -525,16 +575,24 @@ int lgd_modelc_stream_launch(lgd_ctx *ctx, int64_t nx, int64_t ny, const lgd_dom
   a.strips = (a.ny + a.rows - 1) / a.rows;
   int64_t items = a.nblocks * a.strips;
   int64_t nwg = (items + kWavesPerWG - 1) / kWavesPerWG;
-  if (nwg > INT32_MAX) return LGD_ERR_UNSUPPORTED;
+  const int64_t swg =
+      side && side->nops > 0 ? ((side->nops + 64 * kWavesPerWG - 1) / (64 * kWavesPerWG) + 7) / 8 * 8 : 0;
+  if (nwg + swg > INT32_MAX) return LGD_ERR_UNSUPPORTED;
   a.nwg = (int32_t)nwg;
+  a.side = side ? *side : lgd_stream_side_fill{};
+  a.side_wg = (int32_t)swg;
   const bool vec = ((nx + 2) % 2) == 0;
   const bool table = domains != nullptr;
   const int depth = prefetch == 1 ? 1 : 2;
-  dim3 grid((unsigned)nwg), block(64 * kWavesPerWG);
-  if (depth == 2)
+  dim3 grid((unsigned)(nwg + swg)), block(64 * kWavesPerWG);
+  if (swg > 0) {
+    if (depth == 2) launch_side<2>(grid, block, ctx->stream, a, noise, vec, pair);
+    else launch_side<1>(grid, block, ctx->stream, a, noise, vec, pair);
+  } else if (depth == 2) {
     launch_variant<2>(grid, block, ctx->stream, a, noise, vec, table, pair);
-  else
+  } else {
     launch_variant<1>(grid, block, ctx->stream, a, noise, vec, table, pair);
+  }
   return lgd::launch_check(ctx, "modelc_stream");
 }
 
