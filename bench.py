@@ -878,6 +878,8 @@ def run_amr(args) -> dict:
     ctx = lg.Context(0)
     ctx.set_tuning("modelc_kernel", {"auto": 0, "percell": 1, "stream": 2, "pair": 3}[args.kernel])
     ctx.set_tuning("amr_split_fill", getattr(args, "amr_split_fill", 0))
+    ctx.set_tuning("modelc_strips", getattr(args, "strips", 0) or 0)  # level 0's strips (0: default)
+    ctx.set_tuning("modelc_prefetch", getattr(args, "prefetch", 0) or 0)
     params = lg.ModelCParams(A.eps4, A.lambda_, A.d_thermal)
     pol = lg.RegridPolicy(A.every, A.buffer, A.efficiency, A.min_width, A.threshold, 0)
     grid = lg.CartesianGrid(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h])

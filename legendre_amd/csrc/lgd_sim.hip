@@ -1270,7 +1270,7 @@ int lgd_sim_autotune_amr(lgd_sim *sim, double dt, int32_t reps, lgd_amr_tuning_r
     const lgd_launch_shape d = lgd_amr_level0_default_shape(g, noise0);
     const bool pairable = lgd_stream_pairable(g.base.n[0], g.base.nb[0]);
     best0_ms = 1e300;
-    const int strips[] = {1, 2, 4, 8};
+    const int strips[] = {1, 2, 3, 4, 6, 8};
     for (int kern = 2; kern <= (pairable ? 3 : 2); ++kern)
       for (int st : strips)
         for (int pf = 1; pf <= 2 && !rc; ++pf) {
