@@ -94,6 +94,63 @@ def test_three_level_adaptive_subcycled_model_c_matches_oracle(ctx, orc, noise):
     sim_g.close()
 
 
+# Static three-level hierarchies with a refinement ratio other than 2: (ratios,
+# level-1 boxes, level-2 boxes), on a 48² base in 24² blocks.  [3, 2] has nine
+# level-1 substeps per coarse step and a ratio-2 finest level (its restriction shares
+# its last stage launch); [2, 3] has a ratio-3 finest level (restriction in launches
+# of its own).  Each has a level-2 patch that spans two level-1 patches.
+MIXED_RATIOS = {
+    "3x2": ([3, 2], [((12, 12), (60, 48)), ((60, 12), (96, 42))],
+            [((30, 30), (90, 80)), ((100, 30), (170, 76))]),
+    "2x3": ([2, 3], [((8, 8), (40, 32)), ((40, 8), (56, 24))],
+            [((30, 30), (90, 72)), ((96, 30), (150, 60))]),
+}
+
+
+@pytest.mark.parametrize("noise", [0.0, 0.02])
+@pytest.mark.parametrize("name", list(MIXED_RATIOS))
+def test_mixed_ratio_static_hierarchy_model_c_matches_oracle(ctx, orc, name, noise):
+    """Subcycled Model C on a static three-level hierarchy with ratios [3, 2] and
+    [2, 3], whose levels take their first fill from the parent's stage launch at every
+    ratio: fields after 6 coarse steps bitwise against the oracle without noise,
+    within 1e-10 with it."""
+    say(f"amr mixed ratios {name} noise={noise}")
+    ratios, l1, l2 = MIXED_RATIOS[name]
+    h, n, blk = 0.4, 48, 24
+    levels = [l1, l2]
+    og = orc.AmrGrid.from_patches(orc.grid([n, n], [blk, blk], [0, 0], [h, h]), ratios, levels)
+    gg = lg.AmrGrid.from_patches(ctx, [n, n], [blk, blk], [0, 0], [h, h], ratios, levels)
+    sim_o = orc.AmrSim(og, "modelc", params=orc.params(0.05, 3.19, 2.0, noise), seed=5,
+                       wiener=noise > 0, integ="subcycle")
+    gr = sim_o.grid
+    phi, u = sim_o.field(0), sim_o.field(1)
+    for b in range(gr.num_blocks):
+        vp, vu = gr.view(phi, b), gr.view(u, b)
+        ny, nx = vp.shape[0] - 2, vp.shape[1] - 2
+        for j in range(ny):
+            for i in range(nx):
+                x, y = gr.cell_center(b, i, j)
+                d = math.hypot(x - 6.5, y - 5.0) - 3.0
+                vp[j + 1, i + 1] = -math.tanh(d / math.sqrt(2.0))
+                vu[j + 1, i + 1] = -0.55
+    sim_g = lg.amr_simulation(gg, "modelc", params=lg.ModelCParams(0.05, 3.19, 2.0, noise),
+                              wiener=noise > 0, seed=5, subcycle=True)
+    sim_g.field(0).upload(phi)
+    sim_g.field(1).upload(u)
+    dt = 0.2 * h * h / 2.0
+    sim_o.step(dt, 6)
+    sim_g.step(dt, 6)
+    assert sim_g.time == sim_o.time
+    m = H.amr_interior_mask(sim_g.amr)
+    gp, gu = sim_g.field(0).download()[m], sim_g.field(1).download()[m]
+    if noise == 0.0:
+        assert np.array_equal(gp, sim_o.field(0)[m]) and np.array_equal(gu, sim_o.field(1)[m])
+    else:
+        assert np.max(np.abs(gp - sim_o.field(0)[m])) <= 1e-10
+        assert np.max(np.abs(gu - sim_o.field(1)[m])) <= 1e-10
+    sim_g.close()
+
+
 def lin(c):
     return H._fma(3.0, c[0], 2.0) - c[1]
 
