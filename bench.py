@@ -886,6 +886,8 @@ def run_amr(args) -> dict:
     ic = lg.Simulation.model_c(grid, params)
     ic.initialize_grains(A.grains(), A.delta)
 
+    ctx_per_block = [False]  # set while the per-patch baseline runs
+
     def build():
         gg = lg.AmrGrid.from_patches(ctx, [A.cells] * 2, [A.block] * 2, [0.0, 0.0], [A.h, A.h],
                                      list(A.ratios), [])
@@ -938,7 +940,6 @@ def run_amr(args) -> dict:
         return times, updates, shapes, first
 
     nchunks = max(1, A.coarse_steps // A.every)
-    ctx_per_block = [False]
     sim = build()
     times, updates, shapes, first = run(sim, nchunks, "batched")
     nregrid, phases = sim.regrid_stats()
